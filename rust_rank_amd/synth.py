@@ -1,0 +1,319 @@
+"""Seeded synthetic inputs for the rank engine.
+
+The reference draws its preconditioners and block V from an unseeded thread
+RNG (src/matrices.rs:484-497, src/main.rs:464-473), so two runs never see the
+same inputs.  Here every draw comes from a counter-based SplitMix64 stream
+keyed by (seed, stream id), so the HIP engine and the CPU oracle can be fed
+identical inputs and compared bit for bit.
+
+Workloads (sizes as named):
+  plain_small      2000 x 1800, 8 nnz per base row, 200 dependent rows, b = 8
+  graph_complex    200000 x 180000, 12 distinct Zipf(1.1) columns per row, +-1/+-2, b = 32
+  large_single     5,000,000 x 4,500,000, 20 uniform columns per row, 5000 dependent columns, b = 64
+  batched / served rows log-uniform, cols = 0.9 rows, 6-16 (8-20) nnz per row, +-1/+-2,
+                   1-5 % planted dependent columns
+
+Matrices are CSR with uint64 row pointers, uint32 column indices and uint32
+values in [0, p) (negative entries stored as p - |v|).
+"""
+from __future__ import annotations
+
+import dataclasses
+import numpy as np
+
+P_DEFAULT = 2013265921  # 15 * 2^27 + 1
+
+_GOLD = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def _mix(z: np.ndarray) -> np.ndarray:
+    z = (z ^ (z >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+class Rng:
+    """SplitMix64 stream: output i = mix(key + (i + 1) * golden)."""
+
+    def __init__(self, seed: int, stream: int):
+        with np.errstate(over="ignore"):
+            s = _mix(np.array([seed & 0xFFFFFFFFFFFFFFFF], dtype=np.uint64) + _GOLD)
+            k = _mix(s ^ _mix(np.array([stream & 0xFFFFFFFFFFFFFFFF], dtype=np.uint64) + _GOLD))
+        self.key = k[0]
+        self.ctr = 0
+
+    def u64(self, n: int) -> np.ndarray:
+        with np.errstate(over="ignore"):
+            idx = np.arange(self.ctr + 1, self.ctr + 1 + n, dtype=np.uint64)
+            out = _mix(self.key + idx * _GOLD)
+        self.ctr += n
+        return out
+
+    def integers(self, lo: int, hi: int, n: int) -> np.ndarray:
+        """n integers in [lo, hi) (hi - lo < 2^32), multiply-shift mapping."""
+        span = np.uint64(hi - lo)
+        x = self.u64(n) >> np.uint64(32)
+        return (x * span >> np.uint64(32)).astype(np.int64) + lo
+
+    def uniform(self, n: int) -> np.ndarray:
+        """n doubles in [0, 1)."""
+        return (self.u64(n) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+@dataclasses.dataclass
+class Csr:
+    m: int
+    n: int
+    row_ptr: np.ndarray  # uint64, m + 1
+    col: np.ndarray  # uint32, nnz
+    val: np.ndarray  # uint32, nnz, in [0, p)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.row_ptr[-1])
+
+    def to_dense(self, p: int) -> np.ndarray:
+        d = np.zeros((self.m, self.n), dtype=np.uint64)
+        rows = np.repeat(np.arange(self.m), np.diff(self.row_ptr.astype(np.int64)))
+        np.add.at(d, (rows, self.col.astype(np.int64)), self.val.astype(np.uint64))
+        return (d % np.uint64(p)).astype(np.uint32)
+
+
+def _distinct_rows(cands: np.ndarray, k: np.ndarray | int):
+    """For each row of cands (R x K), the first k distinct entries in draw order.
+    Returns (flat selected values in row order, per-row count found)."""
+    R, K = cands.shape
+    order = np.argsort(cands, axis=1, kind="stable")
+    srt = np.take_along_axis(cands, order, axis=1)
+    dup_sorted = np.zeros_like(srt, dtype=bool)
+    dup_sorted[:, 1:] = srt[:, 1:] == srt[:, :-1]
+    dup = np.empty_like(dup_sorted)
+    np.put_along_axis(dup, order, dup_sorted, axis=1)
+    keep = ~dup
+    rank = np.cumsum(keep, axis=1)
+    kk = np.broadcast_to(np.asarray(k).reshape(-1, 1) if np.ndim(k) else np.full((R, 1), k), (R, 1))
+    sel = keep & (rank <= kk)
+    found = sel.sum(axis=1)
+    return cands[sel], found
+
+
+def _sample_rows(rng: Rng, R: int, k, draw) -> list:
+    """Per row, k (scalar or array) distinct columns from the sampler `draw(count)`."""
+    kk = np.full(R, k, dtype=np.int64) if np.ndim(k) == 0 else np.asarray(k, dtype=np.int64)
+    out_cols = [None] * R
+    todo = np.arange(R)
+    extra = 4
+    while todo.size:
+        K = int(kk[todo].max()) + extra
+        c = draw(todo.size * K).reshape(todo.size, K)
+        vals, found = _distinct_rows(c, kk[todo])
+        ok = found == kk[todo]
+        # split flat values back per row
+        starts = np.concatenate([[0], np.cumsum(found)])
+        for i in np.nonzero(ok)[0]:
+            out_cols[todo[i]] = vals[starts[i]:starts[i + 1]]
+        todo = todo[~ok]
+        extra *= 2
+    return out_cols
+
+
+def _rows_to_csr(m: int, n: int, cols_per_row: list, vals_per_row: list) -> Csr:
+    lens = np.array([len(c) for c in cols_per_row], dtype=np.uint64)
+    rp = np.zeros(m + 1, dtype=np.uint64)
+    np.cumsum(lens, out=rp[1:])
+    col = np.concatenate(cols_per_row).astype(np.uint32) if m else np.zeros(0, np.uint32)
+    val = np.concatenate(vals_per_row).astype(np.uint32) if m else np.zeros(0, np.uint32)
+    return Csr(m, n, rp, col, val)
+
+
+def _fixed_k_csr(m: int, n: int, k: np.ndarray, cols_flat: np.ndarray, vals_flat: np.ndarray) -> Csr:
+    rp = np.zeros(m + 1, dtype=np.uint64)
+    np.cumsum(k.astype(np.uint64), out=rp[1:])
+    # sort columns inside each row
+    rows = np.repeat(np.arange(m, dtype=np.int64), k.astype(np.int64))
+    order = np.lexsort((cols_flat, rows))
+    return Csr(m, n, rp, cols_flat[order].astype(np.uint32), vals_flat[order].astype(np.uint32))
+
+
+def _sample_fixed_k(rng: Rng, m: int, k: np.ndarray, draw) -> np.ndarray:
+    """Flat array of per-row distinct columns (row-major), vectorised."""
+    K = int(k.max()) + 8
+    c = draw(m * K).reshape(m, K)
+    vals, found = _distinct_rows(c, k)
+    bad = np.nonzero(found != k)[0]
+    if bad.size == 0:
+        return vals
+    # rare: redo the short rows one by one
+    starts = np.concatenate([[0], np.cumsum(found)])
+    pieces = []
+    for r in range(m):
+        if found[r] == k[r]:
+            pieces.append(vals[starts[r]:starts[r + 1]])
+        else:
+            pieces.append(_sample_rows(rng, 1, int(k[r]), draw)[0])
+    return np.concatenate(pieces)
+
+
+def small_values(rng: Rng, nnz: int, p: int) -> np.ndarray:
+    """+-1 (70 %) and +-2 (30 %), equal signs, stored mod p."""
+    mag = np.where(rng.uniform(nnz) < 0.7, 1, 2)
+    neg = rng.u64(nnz) & np.uint64(1)
+    v = np.where(neg == 1, p - mag, mag)
+    return v.astype(np.uint32)
+
+
+def plant_dependent_columns(rng: Rng, A: Csr, k: int, p: int, small: bool) -> Csr:
+    """Overwrite k columns with combinations of 2 (small) or 3 other columns, so
+    rank(A) <= n - k.  Small matrices use coefficients +-1 (entries stay in [-4, 4])."""
+    if k <= 0:
+        return A
+    n, m = A.n, A.m
+    perm = np.argsort(rng.u64(n))
+    dep = perm[:k]
+    free = perm[k:]
+    terms = 2 if small else 3
+    src = free[rng.integers(0, free.size, k * terms)].reshape(k, terms)
+    if small:
+        coef = np.where(rng.u64(k * terms) & np.uint64(1), p - 1, 1).astype(np.uint64).reshape(k, terms)
+    else:
+        coef = rng.integers(1, p, k * terms).astype(np.uint64).reshape(k, terms)
+    rows = np.repeat(np.arange(m, dtype=np.int64), np.diff(A.row_ptr.astype(np.int64)))
+    col = A.col.astype(np.int64)
+    val = A.val.astype(np.uint64)
+    is_dep = np.zeros(n, dtype=bool)
+    is_dep[dep] = True
+    keep = ~is_dep[col]
+    rows_k, col_k, val_k = rows[keep], col[keep], val[keep]
+    # for every (dependent column t, term s) collect the entries of column src[t, s]
+    col_of = {}
+    new_r, new_c, new_v = [rows_k], [col_k], [val_k]
+    order = np.argsort(col_k, kind="stable")
+    cs = col_k[order]
+    starts = np.searchsorted(cs, np.arange(n + 1))
+    for t in range(k):
+        for s in range(terms):
+            c = src[t, s]
+            idx = order[starts[c]:starts[c + 1]]
+            new_r.append(rows_k[idx])
+            new_c.append(np.full(idx.size, dep[t], dtype=np.int64))
+            new_v.append(val_k[idx] * coef[t, s] % np.uint64(p))
+    r = np.concatenate(new_r)
+    c = np.concatenate(new_c)
+    v = np.concatenate(new_v)
+    # merge duplicates (same row, same dependent column) mod p, drop exact zeros
+    key = r * n + c
+    o = np.argsort(key, kind="stable")
+    key, v = key[o], v[o]
+    uk, first = np.unique(key, return_index=True)
+    sums = np.add.reduceat(v, first) % np.uint64(p) if v.size else v
+    nz = sums != 0
+    uk, sums = uk[nz], sums[nz]
+    rr = uk // n
+    cc = uk % n
+    rp = np.zeros(m + 1, dtype=np.uint64)
+    np.add.at(rp, rr + 1, 1)
+    rp = np.cumsum(rp).astype(np.uint64)
+    return Csr(m, n, rp, cc.astype(np.uint32), sums.astype(np.uint32))
+
+
+def plain_small(seed: int = 1, p: int = P_DEFAULT, m_base: int = 1800, n: int = 1800,
+                m_dep: int = 200, k: int = 8) -> Csr:
+    """m_base base rows with k distinct uniform columns and values in [1, p-1],
+    then m_dep rows that are combinations c1 r_a + c2 r_b + c3 r_c of 3 distinct
+    base rows (c uniform in [1, p-1]), merged, exact zeros dropped."""
+    rng = Rng(seed, 101)
+    kk = np.full(m_base, k, dtype=np.int64)
+    cols = _sample_fixed_k(rng, m_base, kk, lambda c: rng.integers(0, n, c))
+    vals = rng.integers(1, p, m_base * k)
+    base = _fixed_k_csr(m_base, n, kk, cols, vals)
+    rows_c, rows_v = [], []
+    rp = base.row_ptr.astype(np.int64)
+    for _ in range(m_dep):
+        src = _sample_rows(rng, 1, 3, lambda c: rng.integers(0, m_base, c))[0]
+        coef = rng.integers(1, p, 3)
+        acc = {}
+        for r, cf in zip(src, coef):
+            for e in range(rp[r], rp[r + 1]):
+                c = int(base.col[e])
+                acc[c] = (acc.get(c, 0) + int(cf) * int(base.val[e])) % p
+        cs = sorted(c for c, v in acc.items() if v != 0)
+        rows_c.append(np.array(cs, dtype=np.uint32))
+        rows_v.append(np.array([acc[c] for c in cs], dtype=np.uint32))
+    base_c = [base.col[rp[r]:rp[r + 1]] for r in range(m_base)]
+    base_v = [base.val[rp[r]:rp[r + 1]] for r in range(m_base)]
+    return _rows_to_csr(m_base + m_dep, n, base_c + rows_c, base_v + rows_v)
+
+
+def zipf_sampler(rng: Rng, n: int, s: float):
+    w = np.arange(1, n + 1, dtype=np.float64) ** (-s)
+    cdf = np.cumsum(w)
+    cdf /= cdf[-1]
+    perm = np.argsort(rng.u64(n)).astype(np.int64)
+
+    def draw(count):
+        u = rng.uniform(count)
+        r = np.minimum(np.searchsorted(cdf, u, side="right"), n - 1)
+        return perm[r]
+
+    return draw
+
+
+def graph_complex(seed: int = 2, p: int = P_DEFAULT, m: int = 200000, n: int = 180000,
+                  k: int = 12, s: float = 1.1) -> Csr:
+    """m x n, k distinct columns per row from Zipf(s) over ranks mapped through a
+    seeded column permutation, values +-1 (70 %) / +-2 (30 %)."""
+    rng = Rng(seed, 202)
+    draw = zipf_sampler(rng, n, s)
+    kk = np.full(m, k, dtype=np.int64)
+    cols = _sample_fixed_k(rng, m, kk, draw)
+    vals = small_values(rng, m * k, p)
+    return _fixed_k_csr(m, n, kk, cols, vals)
+
+
+def uniform_sparse(seed: int, stream: int, m: int, n: int, k_lo: int, k_hi: int, p: int,
+                   small: bool) -> Csr:
+    """m x n with k uniform in [k_lo, k_hi] distinct uniform columns per row."""
+    rng = Rng(seed, stream)
+    kk = rng.integers(k_lo, k_hi + 1, m) if k_lo != k_hi else np.full(m, k_lo, dtype=np.int64)
+    kk = np.minimum(kk, n)
+    cols = _sample_fixed_k(rng, m, kk, lambda c: rng.integers(0, n, c))
+    tot = int(kk.sum())
+    vals = small_values(rng, tot, p) if small else rng.integers(1, p, tot).astype(np.uint32)
+    return _fixed_k_csr(m, n, kk, cols, vals)
+
+
+def large_single(seed: int = 3, p: int = P_DEFAULT, m: int = 5_000_000, n: int = 4_500_000,
+                 k: int = 20, deficit: int = 5000) -> Csr:
+    A = uniform_sparse(seed, 303, m, n, k, k, p, small=False)
+    return plant_dependent_columns(Rng(seed, 304), A, deficit, p, small=False)
+
+
+def batch_matrix(seed: int, index: int, p: int = P_DEFAULT, rows_lo: int = 1000, rows_hi: int = 50000,
+                 k_lo: int = 6, k_hi: int = 16) -> Csr:
+    """One matrix of the batched/served workloads: rows log-uniform in
+    [rows_lo, rows_hi], cols = floor(0.9 rows), k_lo..k_hi nnz per row, +-1/+-2,
+    1-5 % of rows planted as dependent columns."""
+    rng = Rng(seed, 10_000 + index)
+    u = rng.uniform(2)
+    rows = int(np.exp(np.log(rows_lo) + u[0] * (np.log(rows_hi) - np.log(rows_lo))))
+    cols = int(0.9 * rows)
+    frac = 0.01 + 0.04 * u[1]
+    A = uniform_sparse(seed, 20_000 + index, rows, cols, k_lo, k_hi, p, small=True)
+    return plant_dependent_columns(Rng(seed, 30_000 + index), A, int(round(frac * rows)), p, small=True)
+
+
+def preconditioners(seed: int, m: int, n: int, b: int, p: int = P_DEFAULT):
+    """D_r uniform in [1, p) (m), D_c uniform in [1, p) (n), V uniform in [0, p)
+    (n x b, row-interleaved), as src/main.rs:464-469 draws them."""
+    rng = Rng(seed, 909)
+    dr = rng.integers(1, p, m).astype(np.uint32)
+    dc = rng.integers(1, p, n).astype(np.uint32)
+    V = rng.integers(0, p, n * b).astype(np.uint32).reshape(n, b)
+    return dr, dc, V
+
+
+def seq_len_plain(n_min: int, b: int) -> int:
+    """Sequence length 2 min(m, n) / b + 8 used by the plain workloads."""
+    return 2 * n_min // b + 8
