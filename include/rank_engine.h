@@ -1,0 +1,162 @@
+/*
+ * rank_engine.h -- C ABI of the MI355X block-Wiedemann rank engine
+ * (librank_engine.so, built from rust_rank_amd/csrc for gfx950).
+ *
+ * This is the drop-in boundary for the reference's Krylov/PM-basis/rank path.
+ * Each entry point names the reference interface it replaces.  Conventions:
+ *   - every call returns an rr_status; no exceptions or aborts cross the ABI;
+ *     rr_last_error() gives a message for the calling thread's last failure;
+ *   - the caller owns all host buffers and passes their sizes; the library
+ *     owns device memory and HIP streams;
+ *   - a context is single-threaded (no concurrent calls on one context);
+ *     different contexts may be used from different threads;
+ *   - all arithmetic is exact mod p; outputs are canonical residues in [0, p).
+ *
+ * Sequence layout: S_t is the symmetric b x b matrix V^T B^t V with
+ * B = D_c A^T D_r A D_c (src/main.rs:520-526, src/vectorstream.rs:46-90).  One
+ * entry is the upper triangle i <= j in row-major pair order
+ * (src/main.rs:113-126), P = b(b+1)/2 words; seq buffers are t-major
+ * [count][P].  Block vectors (V, curv) are n x b row-interleaved.
+ */
+#ifndef RANK_ENGINE_H
+#define RANK_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    RR_OK = 0,
+    RR_EINVAL = -1,   /* bad argument (sizes, p, b, index out of range, ...) */
+    RR_ENOMEM = -2,   /* device or host allocation failed / over budget */
+    RR_EDEVICE = -3,  /* a HIP call failed */
+    RR_ESTATE = -4,   /* call not valid in the context's current state */
+    RR_ENOTFOUND = -5 /* unknown request id */
+} rr_status;
+
+/* Sparse matrix in CSR form: m rows, n columns, values in [0, p). */
+typedef struct {
+    uint64_t m, n;
+    const uint64_t *row_ptr; /* m + 1, non-decreasing, row_ptr[m] = nnz */
+    const uint32_t *col;     /* nnz, each < n */
+    const uint32_t *val;     /* nnz, each < p */
+} rr_csr;
+
+typedef struct rr_krylov rr_krylov;
+
+/* Message for the calling thread's most recent failure ("" if none). */
+const char *rr_last_error(void);
+
+/* Number of visible GPUs (0 when none). */
+int rr_device_count(void);
+
+/*
+ * Replaces VectorStream::new (src/vectorstream.rs:46,131,225) together with
+ * the preconditioning of src/main.rs:520-526: uploads A, builds A^T on the
+ * host, and starts the stream at curv = x_k.
+ *   p            prime, 2 < p < 2^31
+ *   b            block width, 1..64
+ *   row_precond  D_r (m), col_precond D_c (n): entries in [1, p)
+ *   curv         x_k (n x b row-interleaved), entries in [0, p)
+ *   t_have       number of sequence entries the caller already holds: 0 for a
+ *                fresh start (x_0 = V; the first entry produced is S_0), or
+ *                2k+1 when resuming a .wdm checkpoint whose curv is x_k
+ *                (src/main.rs:425-443).  Even t_have > 0 is RR_EINVAL.
+ */
+rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *row_precond,
+                           const uint32_t *col_precond, const uint32_t *curv, uint64_t t_have,
+                           rr_krylov **out);
+
+/*
+ * Replaces the body of main_loop_s_mt2 (src/main.rs:131-182): produces the
+ * next `count` sequence entries S_t, t = t_have .. t_have+count-1, into
+ * seq_out[count][P] (P = b(b+1)/2).
+ */
+rr_status rr_krylov_run(rr_krylov *ctx, uint64_t count, uint32_t *seq_out);
+
+/*
+ * Checkpoint state for a .wdm file (src/main.rs:186-202, src/wdm_files.rs:126):
+ * curv_out (n x b) = x_K with K = floor((t-1)/2), where t is the number of
+ * entries produced so far (t_out).  For odd t this is exactly the reference's
+ * curv after its main loop.
+ */
+rr_status rr_krylov_state(rr_krylov *ctx, uint32_t *curv_out, uint64_t *t_out);
+
+/* Timing of the passes of the most recent rr_krylov_run, from HIP events on
+ * the context's stream (enabled by rr_krylov_set_profiling). */
+typedef struct {
+    uint64_t passes_a, passes_at; /* sparse passes by A and by A^T */
+    double ms_a, ms_at;           /* summed kernel time of those passes */
+    double ms_fixup, ms_reduce;   /* long-row fix-up and sequence reduction kernels */
+    double ms_total;              /* wall time of the run on the stream */
+    uint64_t bytes_a, bytes_at;   /* algorithmic bytes of one pass by A / by A^T */
+} rr_krylov_stats;
+rr_status rr_krylov_set_profiling(rr_krylov *ctx, int on);
+rr_status rr_krylov_stats_get(rr_krylov *ctx, rr_krylov_stats *out);
+
+/* Releases the context and its device memory. */
+void rr_krylov_destroy(rr_krylov *ctx);
+
+/*
+ * Replaces PM_Basis (src/sigma_basis.rs:9-54) with the recursion of
+ * _PM_Basis / M_Basis (:190-219, :348-416), run on the device.  Builds
+ * G = [S(x); I_b] from seq [L][P] and computes the order-d approximant basis
+ * exactly mod p.  Unlike the reference, d is not rounded up to a power of two:
+ * any 1 <= d < L is accepted (for a power of two the result is the
+ * reference's).  basis_out: [2b][2b][d+1] (coefficients innermost);
+ * delta_out: 2b degree shifts.  For p = 2013265921 products use NTTs (d + 1
+ * <= 2^14); any other prime p < 2^31 uses direct products.
+ */
+rr_status rr_pm_basis(const uint32_t *seq, uint32_t b, uint64_t L, uint64_t d, uint32_t p, uint32_t *basis_out,
+                      int64_t *delta_out);
+
+/*
+ * Replaces analyze_pm_basis (src/sigma_basis.rs:71-152) + analyze_min_generators
+ * (src/blockbmtest.rs:574-600), which print "Estimated matrix rank"; the rank is
+ * returned instead: b(K-3) + rank(F_{K-1}) + rank(F_0) on the reversed
+ * generator (K coefficients), evaluated with the reference's unsigned 64-bit
+ * wrap when K < 3.  basis [2b][2b][len].
+ */
+rr_status rr_rank_from_basis(const uint32_t *basis, const int64_t *delta, uint32_t b, uint64_t len, uint32_t p,
+                             int64_t *rank_out);
+
+/*
+ * Replaces ntt / matrix_ntt_parallel (src/ntt.rs:164-211, :263-269) for
+ * p = 2013265921: `count` transforms of 2^logn points in place, natural order,
+ * forward root 31^((p-1)/n), inverse scaled by 1/n.  logn <= 14.
+ */
+rr_status rr_ntt(uint32_t *data, uint32_t logn, uint32_t count, int inverse);
+
+/*
+ * Replaces poly_mat_mul_red_adaptive / poly_mat_mul_fft_red
+ * (src/poly_mat_mul.rs:89-101, :240-256): C [rows][cols][hi-lo] = coefficients
+ * [lo, hi) of A [rows][inner][la] * B [inner][cols][lb], exact mod p
+ * (NTT mod p for p = 2013265921, direct products otherwise).  inner <= 128.
+ */
+rr_status rr_poly_mat_mul(const uint32_t *A, uint64_t la, const uint32_t *B, uint64_t lb, uint32_t rows,
+                          uint32_t inner, uint32_t cols, uint64_t lo, uint64_t hi, uint32_t p, uint32_t *C);
+
+/*
+ * Sequence length and PM-basis order.  reference_policy = 1 reproduces
+ * src/main.rs:497-504 (next power of two of floor(2 min(m,n)/b + 4), plus the
+ * t = 0 entry) and src/main.rs:599-603 (largest power of two <= length);
+ * 0 gives the exact policy: length 2 min(m,n)/b + 8, order length - 1.
+ */
+uint64_t rr_seq_len(uint64_t m, uint64_t n, uint32_t b, int reference_policy);
+uint64_t rr_pm_order(uint64_t seq_len, int reference_policy);
+
+/*
+ * Replaces the rank driver of src/main.rs:397-606 (after loading and drawing):
+ * Krylov sequence of seq_len entries (0: exact policy), PM-basis of the given
+ * order (0: seq_len - 1), rank estimate returned in rank_out.  ms_out (may be
+ * NULL) receives the wall time of the call.
+ */
+rr_status rr_rank(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *row_precond, const uint32_t *col_precond,
+                  const uint32_t *V, uint64_t seq_len, uint64_t order, int64_t *rank_out, double *ms_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RANK_ENGINE_H */

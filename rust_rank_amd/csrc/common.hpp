@@ -1,0 +1,112 @@
+// common.hpp -- shared host/device helpers for the rank engine (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstdarg>
+#include <string>
+#include "../../include/rank_engine.h"
+
+namespace rr {
+
+void set_error(const char *fmt, ...);
+void clear_error();
+
+// ---------------------------------------------------------------------------
+// Exact arithmetic mod p < 2^31.  Integer-valued doubles are exact below 2^53;
+// FP64 FMA is full rate on CDNA4, 32-bit integer multiplies are not.
+
+struct ModP {
+    uint32_t p;
+    double pd;   // (double)p
+    double inv;  // 1.0 / p
+    static ModP make(uint32_t p) { return ModP{p, (double)p, 1.0 / (double)p}; }
+};
+
+// a: integer-valued double with |a| < 2^53  ->  a mod p in [0, p)
+__device__ __forceinline__ double fold(double a, double pd, double inv) {
+    double q = floor(a * inv);
+    double r = fma(-q, pd, a);  // exact: the true result is a small integer
+    r = r < 0.0 ? r + pd : r;
+    r = r >= pd ? r - pd : r;
+    return r;
+}
+
+// a * b mod p for a, b < p < 2^31
+__device__ __forceinline__ uint32_t mulmod(uint32_t a, uint32_t b, uint32_t p, double inv) {
+    double x = (double)a * (double)b;  // relative error <= 2^-53
+    double e = x * inv - 0x1p-19;      // below floor(ab/p) + 1 - 2^-20, above floor(ab/p) - 1
+    uint32_t q = (uint32_t)fmax(e, 0.0);
+    uint32_t r = a * b - q * p;  // true value in [0, 2p), exact mod 2^32
+    return r >= p ? r - p : r;
+}
+
+__device__ __forceinline__ uint32_t addmod(uint32_t a, uint32_t b, uint32_t p) {
+    uint32_t s = a + b;
+    return s >= p ? s - p : s;
+}
+
+__device__ __forceinline__ uint32_t submod(uint32_t a, uint32_t b, uint32_t p) {
+    return a >= b ? a - b : a + (p - b);
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+static inline uint32_t h_mulmod(uint32_t a, uint32_t b, uint32_t p) { return (uint32_t)(((uint64_t)a * b) % p); }
+static inline uint32_t h_powmod(uint32_t a, uint64_t e, uint32_t p) {
+    uint64_t r = 1, x = a % p;
+    while (e) {
+        if (e & 1) r = r * x % p;
+        x = x * x % p;
+        e >>= 1;
+    }
+    return (uint32_t)r;
+}
+static inline uint32_t h_invmod(uint32_t a, uint32_t p) { return h_powmod(a, p - 2, p); }
+static inline bool h_is_prime(uint32_t p) {
+    if (p < 2) return false;
+    for (uint32_t d = 2; (uint64_t)d * d <= p; d++)
+        if (p % d == 0) return false;
+    return true;
+}
+
+// Owned device allocation.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() {
+        if (ptr) (void)hipFree(ptr);
+    }
+    rr_status alloc(size_t n) {
+        bytes = n;
+        if (n == 0) n = 4;
+        hipError_t e = hipMalloc(&ptr, n);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            rr::set_error("hipMalloc(%zu) failed: %s", n, hipGetErrorString(e));
+            return RR_ENOMEM;
+        }
+        return RR_OK;
+    }
+    template <typename T>
+    T *as() const { return (T *)ptr; }
+};
+
+}  // namespace rr
+
+#define RR_HIP(call)                                                                               \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) {                                                                    \
+            rr::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return RR_EDEVICE;                                                                     \
+        }                                                                                          \
+    } while (0)
+
+#define RR_CHECK(cond, code, ...)          \
+    do {                                   \
+        if (!(cond)) {                     \
+            rr::set_error(__VA_ARGS__);    \
+            return code;                   \
+        }                                  \
+    } while (0)

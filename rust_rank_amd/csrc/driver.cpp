@@ -1,0 +1,66 @@
+// driver.cpp -- the rank driver over the C ABI: Krylov sequence on the device,
+// device PM-basis, rank formula.
+//
+// Replaces the reference's rank driver (src/main.rs:397-606): draws are the
+// caller's (src/main.rs:464-473 uses an unseeded RNG, so they are inputs
+// here), the sequence length and PM-basis order follow either the
+// reference's policy (src/main.rs:497-504, 599-603: next power of two of
+// floor(2 min(m,n)/b + 4), plus one entry; largest power of two <= length) or
+// the exact policy used by the workloads (length 2 min(m,n)/b + 8, order
+// length - 1), and the rank is returned instead of printed
+// (src/blockbmtest.rs:598-599).
+#include "common.hpp"
+#include <vector>
+#include <chrono>
+
+extern "C" {
+
+uint64_t rr_seq_len(uint64_t m, uint64_t n, uint32_t b, int reference_policy) {
+    if (b == 0) return 0;
+    uint64_t mn = m < n ? m : n;
+    if (!reference_policy) return 2 * mn / b + 8;
+    // src/main.rs:497-504: floor(2 d / num_v + 4), then next power of two (num_v > 1),
+    // and the loop produces max_nlen + 1 entries (src/main.rs:109-131)
+    uint64_t len = (uint64_t)(2.0 * (double)mn / (double)b + 4.0);
+    if (b > 1) {
+        uint64_t q = 1;
+        while (q < len) q <<= 1;
+        len = q;
+    }
+    return len + 1;
+}
+
+uint64_t rr_pm_order(uint64_t seq_len, int reference_policy) {
+    if (seq_len < 2) return 0;
+    if (!reference_policy) return seq_len - 1;
+    uint64_t d = 1;  // src/main.rs:599-603: largest power of two <= len
+    while (d * 2 <= seq_len) d *= 2;
+    if (d >= seq_len) d /= 2;
+    return d;
+}
+
+rr_status rr_rank(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *row_precond, const uint32_t *col_precond,
+                  const uint32_t *V, uint64_t seq_len, uint64_t order, int64_t *rank_out, double *ms_out) {
+    rr::clear_error();
+    RR_CHECK(A && rank_out, RR_EINVAL, "null argument");
+    auto t0 = std::chrono::steady_clock::now();
+    if (seq_len == 0) seq_len = rr_seq_len(A->m, A->n, b, 0);
+    if (order == 0) order = rr_pm_order(seq_len, 0);
+    const uint64_t P = (uint64_t)b * (b + 1) / 2;
+    std::vector<uint32_t> seq(seq_len * P);
+    rr_krylov *k = nullptr;
+    rr_status s = rr_krylov_create(A, p, b, row_precond, col_precond, V, 0, &k);
+    if (s != RR_OK) return s;
+    s = rr_krylov_run(k, seq_len, seq.data());
+    rr_krylov_destroy(k);
+    if (s != RR_OK) return s;
+    std::vector<uint32_t> basis((size_t)4 * b * b * (order + 1));
+    std::vector<int64_t> delta(2 * b);
+    if ((s = rr_pm_basis(seq.data(), b, seq_len, order, p, basis.data(), delta.data())) != RR_OK) return s;
+    if ((s = rr_rank_from_basis(basis.data(), delta.data(), b, order + 1, p, rank_out)) != RR_OK) return s;
+    if (ms_out)
+        *ms_out = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return RR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,724 @@
+// krylov.hip -- block-Wiedemann Krylov sequence on gfx950.
+//
+// Replaces the reference's Krylov producer and sequence loop
+// (src/vectorstream.rs:46-90 / 225-275, src/main.rs:88-216; the CUDA twin is
+// cuda_test/cuvseq_int.cu:817-955).  The sequence is the reference's, entry by
+// entry: S_t = x_a^T x_b with a + b = t, x_{k+1} = D_c A^T D_r A D_c x_k.
+//
+// MI355X design.  The preconditioners are applied to vectors instead of being
+// baked into A (src/main.rs:524-526 scales A's values), so A keeps its raw
+// entries and graph-complex matrices (entries +-1, +-2) store each nonzero as
+// one int32: a 28-bit column index and a 4-bit signed value.  Each produced
+// entry is one sparse pass plus a b x b projection computed from the rows the
+// pass itself produces, so no vector is read twice:
+//   pass by A   (t = 2k+1):  w = A z_k,    y = D_r w,      S_t = w^T y
+//   pass by A^T (t = 2k+2):  u = A^T y,    z_{k+1} = D_c^2 u, S_t = u^T z_{k+1}
+// with z_k = D_c x_k.  Both S are exactly x_k^T B x_k and x_{k+1}^T x_{k+1}
+// mod p (B = D_c A^T D_r A D_c is symmetric).
+//
+// Layout in HBM: block vectors are row-interleaved with a power-of-two stride
+// B >= b (b columns used, the rest zero), so one row is one contiguous
+// 4B-byte segment and a wavefront of 64 lanes covers 64/B rows.  Lane j of a
+// row owns column j.  Rows longer than LONG_ROW nonzeros (the Zipf head of
+// A^T) are split into pieces processed by many wavefronts and combined by a
+// small fix-up kernel.  Each workgroup writes one partial b x b upper triangle
+// per pass; a reduction kernel folds the partials of a chunk of passes into
+// the sequence.
+#include "common.hpp"
+#include <vector>
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+namespace rr {
+
+constexpr int WG = 256;             // threads per workgroup (4 wavefronts)
+constexpr uint32_t LONG_ROW = 4096; // rows above this many nonzeros are split
+constexpr uint32_t PIECE = 2048;    // nonzeros per piece of a split row
+constexpr int CHUNK = 64;           // passes per sequence reduction
+
+struct PassArgs {
+    const uint32_t *rowptr;  // nrows + 1
+    const uint32_t *col;     // packed (col | value << 28) or plain column
+    const uint32_t *val;     // plain values (general matrices)
+    uint32_t nrows;
+    const uint32_t *in;      // gathered block vector, stride B
+    uint32_t *out;           // produced block vector, stride B
+    const uint32_t *scale;   // per-row factor applied to the produced rows
+    const uint32_t *piece_row, *piece_beg, *piece_end;
+    uint32_t npieces;
+    uint32_t *piece_out;     // [npieces][B] partial row sums mod p
+    const uint32_t *long_rows, *long_pbeg;  // split rows and their piece ranges
+    uint32_t nlong;
+    uint32_t *gram;          // this pass's partials [gridDim.x (+ fixup grid)][P]
+    uint32_t p;
+    double pd, inv;
+    uint32_t b;
+};
+
+// ---------------------------------------------------------------------------
+// one row of the sparse product, lane j: sum_e v_e * in[col_e][j] mod p
+template <int B, bool PACKED>
+__device__ __forceinline__ uint32_t row_dot(const PassArgs &a, uint32_t beg, uint32_t end, int j) {
+    if constexpr (PACKED) {
+        // |v| <= 8, x < 2^31: each term < 2^34, exact in fp64 for 2^19 terms
+        double acc0 = 0.0, acc1 = 0.0;
+        uint32_t e = beg;
+        for (; e + 4 <= end; e += 4) {
+            uint32_t c0 = a.col[e], c1 = a.col[e + 1], c2 = a.col[e + 2], c3 = a.col[e + 3];
+            uint32_t x0 = a.in[(size_t)(c0 & 0x0FFFFFFFu) * B + j];
+            uint32_t x1 = a.in[(size_t)(c1 & 0x0FFFFFFFu) * B + j];
+            uint32_t x2 = a.in[(size_t)(c2 & 0x0FFFFFFFu) * B + j];
+            uint32_t x3 = a.in[(size_t)(c3 & 0x0FFFFFFFu) * B + j];
+            acc0 = fma((double)((int32_t)c0 >> 28), (double)x0, acc0);
+            acc1 = fma((double)((int32_t)c1 >> 28), (double)x1, acc1);
+            acc0 = fma((double)((int32_t)c2 >> 28), (double)x2, acc0);
+            acc1 = fma((double)((int32_t)c3 >> 28), (double)x3, acc1);
+        }
+        for (; e < end; e++) {
+            uint32_t c0 = a.col[e];
+            uint32_t x0 = a.in[(size_t)(c0 & 0x0FFFFFFFu) * B + j];
+            acc0 = fma((double)((int32_t)c0 >> 28), (double)x0, acc0);
+        }
+        return (uint32_t)fold(acc0 + acc1, a.pd, a.inv);
+    } else {
+        // v = vh 2^16 + vl: vh x < 2^46, vl x < 2^47; fold every 64 terms
+        double hi = 0.0, lo = 0.0;
+        uint32_t e = beg;
+        while (e < end) {
+            uint32_t stop = min(end, e + 64u);
+            for (; e + 2 <= stop; e += 2) {
+                uint32_t c0 = a.col[e], c1 = a.col[e + 1];
+                uint32_t v0 = a.val[e], v1 = a.val[e + 1];
+                double x0 = (double)a.in[(size_t)c0 * B + j];
+                double x1 = (double)a.in[(size_t)c1 * B + j];
+                hi = fma((double)(v0 >> 16), x0, hi);
+                lo = fma((double)(v0 & 0xFFFFu), x0, lo);
+                hi = fma((double)(v1 >> 16), x1, hi);
+                lo = fma((double)(v1 & 0xFFFFu), x1, lo);
+            }
+            if (e < stop) {
+                uint32_t c0 = a.col[e], v0 = a.val[e];
+                double x0 = (double)a.in[(size_t)c0 * B + j];
+                hi = fma((double)(v0 >> 16), x0, hi);
+                lo = fma((double)(v0 & 0xFFFFu), x0, lo);
+                e++;
+            }
+            hi = fold(hi, a.pd, a.inv);
+            lo = fold(lo, a.pd, a.inv);
+        }
+        return (uint32_t)fold(fma(hi, 65536.0, lo), a.pd, a.inv);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-lane accumulation of the symmetric projection sum_r o_r s_r^T.
+// Lane j of a row pairs its own s_j with o_{(j+k) mod B} for k = 0..B/2, which
+// covers every unordered pair once (k = B/2 only for j < B/2).  s is split in
+// 16-bit halves so each product fits fp64 exactly; partial sums fold every 64
+// rows.
+template <int B>
+struct Gram {
+    static constexpr int K = B / 2 + 1;
+    double hi[K], lo[K];
+    int rows;
+
+    __device__ void init() {
+#pragma unroll
+        for (int k = 0; k < K; k++) hi[k] = lo[k] = 0.0;
+        rows = 0;
+    }
+    // all 64 lanes must call this together
+    __device__ __forceinline__ void add(uint32_t o, uint32_t s, int lane, double pd, double inv) {
+        const int base = lane & ~(B - 1);
+        const int j = lane & (B - 1);
+        const double sh = (double)(s >> 16), sl = (double)(s & 0xFFFFu);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            uint32_t ok = (k == 0) ? o : (uint32_t)__shfl((int)o, base + ((j + k) & (B - 1)), 64);
+            double od = (double)ok;
+            hi[k] = fma(od, sh, hi[k]);
+            lo[k] = fma(od, sl, lo[k]);
+        }
+        if (++rows == 64) {
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                hi[k] = fold(hi[k], pd, inv);
+                lo[k] = fold(lo[k], pd, inv);
+            }
+            rows = 0;
+        }
+    }
+    // Reduce over the wavefront's rows and the workgroup's wavefronts, then
+    // write the b(b+1)/2 upper-triangle partial to out.
+    __device__ void finish(uint32_t *out, uint32_t b, uint32_t p, double pd, double inv, uint32_t *lds) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = lane & (B - 1);
+        uint32_t v[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = (uint32_t)fold(fma(fold(hi[k], pd, inv), 65536.0, fold(lo[k], pd, inv)), pd, inv);
+        // rows of the same wavefront: lanes j, j + B, j + 2B, ...
+#pragma unroll
+        for (int off = B; off < 64; off <<= 1) {
+#pragma unroll
+            for (int k = 0; k < K; k++) v[k] = addmod(v[k], (uint32_t)__shfl_xor((int)v[k], off, 64), p);
+        }
+        if (lane < B) {
+#pragma unroll
+            for (int k = 0; k < K; k++) lds[(wave * B + j) * K + k] = v[k];
+        }
+        __syncthreads();
+        const int nw = blockDim.x >> 6;
+        for (int t = threadIdx.x; t < B * K; t += blockDim.x) {
+            int jj = t / K, k = t % K;
+            uint32_t s = 0;
+            for (int w = 0; w < nw; w++) s = addmod(s, lds[(w * B + jj) * K + k], p);
+            int i1 = jj, i2 = (jj + k) & (B - 1);
+            if (2 * k == B && jj >= B / 2) continue;  // that pair is covered by jj - B/2
+            if ((uint32_t)i1 >= b || (uint32_t)i2 >= b) continue;
+            uint32_t lo_i = min(i1, i2), hi_i = max(i1, i2);
+            uint32_t ii = lo_i * b - lo_i * (lo_i - 1) / 2 + (hi_i - lo_i);
+            out[ii] = s;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// One sparse pass: pieces of split rows, then every short row (product,
+// scaling, store, projection).
+template <int B, bool PACKED>
+__global__ __launch_bounds__(WG) void k_pass(PassArgs a) {
+    constexpr int G = 64 / B;
+    __shared__ uint32_t lds[(WG / 64) * B * (B / 2 + 1)];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / B, j = lane & (B - 1);
+    const uint32_t gwave = blockIdx.x * (WG / 64) + wave;
+    const uint32_t nwaves = gridDim.x * (WG / 64);
+
+    for (uint32_t pc = gwave * G + slot; pc < a.npieces; pc += nwaves * G)
+        a.piece_out[(size_t)pc * B + j] = row_dot<B, PACKED>(a, a.piece_beg[pc], a.piece_end[pc], j);
+
+    Gram<B> g;
+    g.init();
+    for (uint32_t base = gwave * G; base < a.nrows; base += nwaves * G) {
+        uint32_t r = base + slot;
+        uint32_t o = 0, s = 0;
+        if (r < a.nrows) {
+            uint32_t beg = a.rowptr[r], end = a.rowptr[r + 1];
+            if (end - beg <= LONG_ROW) {
+                o = row_dot<B, PACKED>(a, beg, end, j);
+                s = mulmod(o, a.scale[r], a.p, a.inv);
+                a.out[(size_t)r * B + j] = s;
+            }
+        }
+        g.add(o, s, lane, a.pd, a.inv);
+    }
+    g.finish(a.gram + (size_t)blockIdx.x * (a.b * (a.b + 1) / 2), a.b, a.p, a.pd, a.inv, lds);
+}
+
+// Split rows: sum the pieces, scale, store, project.  Partials go after the
+// main kernel's (gram + (main_grid + blockIdx.x) * P).
+template <int B>
+__global__ __launch_bounds__(WG) void k_fixup(PassArgs a, uint32_t main_grid) {
+    constexpr int G = 64 / B;
+    __shared__ uint32_t lds[(WG / 64) * B * (B / 2 + 1)];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / B, j = lane & (B - 1);
+    const uint32_t gwave = blockIdx.x * (WG / 64) + wave;
+    const uint32_t nwaves = gridDim.x * (WG / 64);
+    Gram<B> g;
+    g.init();
+    for (uint32_t base = gwave * G; base < a.nlong; base += nwaves * G) {
+        uint32_t l = base + slot;
+        uint32_t o = 0, s = 0;
+        if (l < a.nlong) {
+            uint32_t r = a.long_rows[l];
+            for (uint32_t pc = a.long_pbeg[l]; pc < a.long_pbeg[l + 1]; pc++)
+                o = addmod(o, a.piece_out[(size_t)pc * B + j], a.p);
+            s = mulmod(o, a.scale[r], a.p, a.inv);
+            a.out[(size_t)r * B + j] = s;
+        }
+        g.add(o, s, lane, a.pd, a.inv);
+    }
+    g.finish(a.gram + (size_t)(main_grid + blockIdx.x) * (a.b * (a.b + 1) / 2), a.b, a.p, a.pd, a.inv, lds);
+}
+
+// S = x^T x of a block vector (the t = 0 entry), partials per workgroup.
+template <int B>
+__global__ __launch_bounds__(WG) void k_gram(const uint32_t *x, uint32_t nrows, uint32_t *gram, uint32_t b,
+                                             uint32_t p, double pd, double inv) {
+    constexpr int G = 64 / B;
+    __shared__ uint32_t lds[(WG / 64) * B * (B / 2 + 1)];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / B, j = lane & (B - 1);
+    const uint32_t gwave = blockIdx.x * (WG / 64) + wave;
+    const uint32_t nwaves = gridDim.x * (WG / 64);
+    Gram<B> g;
+    g.init();
+    for (uint32_t base = gwave * G; base < nrows; base += nwaves * G) {
+        uint32_t r = base + slot;
+        uint32_t o = r < nrows ? x[(size_t)r * B + j] : 0;
+        g.add(o, o, lane, pd, inv);
+    }
+    g.finish(gram + (size_t)blockIdx.x * (b * (b + 1) / 2), b, p, pd, inv, lds);
+}
+
+// seq[t][ii] = sum_w part[t][w][ii] mod p
+__global__ void k_reduce(const uint32_t *part, uint32_t nparts, uint32_t P, uint32_t count, uint32_t *seq,
+                         uint32_t p) {
+    uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count * P) return;
+    uint32_t t = idx / P, ii = idx % P;
+    const uint32_t *src = part + (size_t)t * nparts * P + ii;
+    uint64_t s = 0;
+    for (uint32_t w = 0; w < nparts; w++) s += src[(size_t)w * P];
+    seq[idx] = (uint32_t)(s % p);
+}
+
+// out[r][j] = in[r][j] * d[r] mod p over rows (block-vector scaling)
+__global__ void k_scale_rows(const uint32_t *in, const uint32_t *d, uint32_t *out, uint32_t nrows, uint32_t B,
+                             uint32_t p, double inv) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)nrows * B) return;
+    out[idx] = mulmod(in[idx], d[idx / B], p, inv);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+struct SpMat {  // one operand (A or A^T) on the device
+    uint32_t nrows = 0;
+    DevBuf rowptr, col, val, piece_row, piece_beg, piece_end, piece_out, long_rows, long_pbeg;
+    uint32_t npieces = 0, nlong = 0;
+};
+
+}  // namespace rr
+
+struct rr_krylov {
+    uint32_t p = 0, b = 0, B = 0;
+    uint64_t m = 0, n = 0, nnz = 0;
+    bool packed = false;
+    rr::ModP mp;
+    rr::SpMat A, At;
+    rr::DevBuf dr, dc, dc2, dcinv, z, y, xtmp, part, seq;
+    uint32_t grid = 0, fix_grid = 0, P = 0;
+    uint64_t t = 0;  // entries produced so far
+    hipStream_t stream = nullptr;
+    int device = 0;
+    bool profiling = false;
+    rr_krylov_stats stats{};
+    ~rr_krylov() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace rr {
+
+static thread_local std::string g_err;
+void set_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+void clear_error() { g_err.clear(); }
+
+template <typename T>
+static rr_status upload(DevBuf &buf, const T *host, size_t count) {
+    rr_status s = buf.alloc(count * sizeof(T));
+    if (s != RR_OK) return s;
+    if (count) RR_HIP(hipMemcpy(buf.ptr, host, count * sizeof(T), hipMemcpyHostToDevice));
+    return RR_OK;
+}
+
+// Builds the device operand from host CSR (row pointers already validated).
+static rr_status build_operand(SpMat &M, uint64_t nrows, const std::vector<uint32_t> &rp,
+                               const std::vector<uint32_t> &colw, const std::vector<uint32_t> &vals, bool packed,
+                               uint32_t B) {
+    M.nrows = (uint32_t)nrows;
+    std::vector<uint32_t> prow, pbeg, pend, lrows, lpbeg;
+    for (uint64_t r = 0; r < nrows; r++) {
+        uint32_t beg = rp[r], end = rp[r + 1];
+        if (end - beg > LONG_ROW) {
+            lrows.push_back((uint32_t)r);
+            lpbeg.push_back((uint32_t)prow.size());
+            for (uint32_t e = beg; e < end; e += PIECE) {
+                prow.push_back((uint32_t)r);
+                pbeg.push_back(e);
+                pend.push_back(std::min(end, e + PIECE));
+            }
+        }
+    }
+    lpbeg.push_back((uint32_t)prow.size());
+    M.npieces = (uint32_t)prow.size();
+    M.nlong = (uint32_t)lrows.size();
+    rr_status s;
+    if ((s = upload(M.rowptr, rp.data(), rp.size())) != RR_OK) return s;
+    if ((s = upload(M.col, colw.data(), colw.size())) != RR_OK) return s;
+    if (!packed && (s = upload(M.val, vals.data(), vals.size())) != RR_OK) return s;
+    if ((s = upload(M.piece_row, prow.data(), prow.size())) != RR_OK) return s;
+    if ((s = upload(M.piece_beg, pbeg.data(), pbeg.size())) != RR_OK) return s;
+    if ((s = upload(M.piece_end, pend.data(), pend.size())) != RR_OK) return s;
+    if ((s = M.piece_out.alloc((size_t)M.npieces * B * 4)) != RR_OK) return s;
+    if ((s = upload(M.long_rows, lrows.data(), lrows.size())) != RR_OK) return s;
+    if ((s = upload(M.long_pbeg, lpbeg.data(), lpbeg.size())) != RR_OK) return s;
+    return RR_OK;
+}
+
+static PassArgs make_args(rr_krylov *c, SpMat &M, const uint32_t *in, uint32_t *out, const uint32_t *scale,
+                          uint32_t *gram) {
+    PassArgs a;
+    a.rowptr = M.rowptr.as<uint32_t>();
+    a.col = M.col.as<uint32_t>();
+    a.val = M.val.as<uint32_t>();
+    a.nrows = M.nrows;
+    a.in = in;
+    a.out = out;
+    a.scale = scale;
+    a.piece_row = M.piece_row.as<uint32_t>();
+    a.piece_beg = M.piece_beg.as<uint32_t>();
+    a.piece_end = M.piece_end.as<uint32_t>();
+    a.npieces = M.npieces;
+    a.piece_out = M.piece_out.as<uint32_t>();
+    a.long_rows = M.long_rows.as<uint32_t>();
+    a.long_pbeg = M.long_pbeg.as<uint32_t>();
+    a.nlong = M.nlong;
+    a.gram = gram;
+    a.p = c->p;
+    a.pd = c->mp.pd;
+    a.inv = c->mp.inv;
+    a.b = c->b;
+    return a;
+}
+
+template <int B>
+static void launch_pass(rr_krylov *c, const PassArgs &a) {
+    if (c->packed)
+        hipLaunchKernelGGL((k_pass<B, true>), dim3(c->grid), dim3(WG), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_pass<B, false>), dim3(c->grid), dim3(WG), 0, c->stream, a);
+}
+
+static void launch_pass_any(rr_krylov *c, const PassArgs &a) {
+    switch (c->B) {
+        case 1: launch_pass<1>(c, a); break;
+        case 2: launch_pass<2>(c, a); break;
+        case 4: launch_pass<4>(c, a); break;
+        case 8: launch_pass<8>(c, a); break;
+        case 16: launch_pass<16>(c, a); break;
+        case 32: launch_pass<32>(c, a); break;
+        default: launch_pass<64>(c, a); break;
+    }
+}
+
+static void launch_fixup_any(rr_krylov *c, const PassArgs &a) {
+    dim3 g(c->fix_grid), t(WG);
+    switch (c->B) {
+        case 1: hipLaunchKernelGGL(k_fixup<1>, g, t, 0, c->stream, a, c->grid); break;
+        case 2: hipLaunchKernelGGL(k_fixup<2>, g, t, 0, c->stream, a, c->grid); break;
+        case 4: hipLaunchKernelGGL(k_fixup<4>, g, t, 0, c->stream, a, c->grid); break;
+        case 8: hipLaunchKernelGGL(k_fixup<8>, g, t, 0, c->stream, a, c->grid); break;
+        case 16: hipLaunchKernelGGL(k_fixup<16>, g, t, 0, c->stream, a, c->grid); break;
+        case 32: hipLaunchKernelGGL(k_fixup<32>, g, t, 0, c->stream, a, c->grid); break;
+        default: hipLaunchKernelGGL(k_fixup<64>, g, t, 0, c->stream, a, c->grid); break;
+    }
+}
+
+static void launch_gram_any(rr_krylov *c, const uint32_t *x, uint32_t nrows, uint32_t *gram) {
+    dim3 g(c->grid), t(WG);
+    uint32_t b = c->b, p = c->p;
+    double pd = c->mp.pd, inv = c->mp.inv;
+    switch (c->B) {
+        case 1: hipLaunchKernelGGL(k_gram<1>, g, t, 0, c->stream, x, nrows, gram, b, p, pd, inv); break;
+        case 2: hipLaunchKernelGGL(k_gram<2>, g, t, 0, c->stream, x, nrows, gram, b, p, pd, inv); break;
+        case 4: hipLaunchKernelGGL(k_gram<4>, g, t, 0, c->stream, x, nrows, gram, b, p, pd, inv); break;
+        case 8: hipLaunchKernelGGL(k_gram<8>, g, t, 0, c->stream, x, nrows, gram, b, p, pd, inv); break;
+        case 16: hipLaunchKernelGGL(k_gram<16>, g, t, 0, c->stream, x, nrows, gram, b, p, pd, inv); break;
+        case 32: hipLaunchKernelGGL(k_gram<32>, g, t, 0, c->stream, x, nrows, gram, b, p, pd, inv); break;
+        default: hipLaunchKernelGGL(k_gram<64>, g, t, 0, c->stream, x, nrows, gram, b, p, pd, inv); break;
+    }
+}
+
+}  // namespace rr
+
+using namespace rr;
+
+extern "C" {
+
+const char *rr_last_error(void) { return rr::g_err.c_str(); }
+
+int rr_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *row_precond,
+                           const uint32_t *col_precond, const uint32_t *curv, uint64_t t_have, rr_krylov **out) {
+    clear_error();
+    RR_CHECK(A && out && row_precond && col_precond && curv, RR_EINVAL, "null argument");
+    *out = nullptr;
+    RR_CHECK(p > 2 && p < (1u << 31) && h_is_prime(p), RR_EINVAL, "p=%u must be a prime in (2, 2^31)", p);
+    RR_CHECK(b >= 1 && b <= 64, RR_EINVAL, "b=%u must be in 1..64", b);
+    RR_CHECK(A->m >= 1 && A->n >= 1 && A->m < (1ull << 31) && A->n < (1ull << 31), RR_EINVAL,
+             "matrix %llux%llu out of range (m, n < 2^31)", (unsigned long long)A->m, (unsigned long long)A->n);
+    RR_CHECK(t_have == 0 || (t_have & 1), RR_EINVAL, "t_have=%llu must be 0 or odd", (unsigned long long)t_have);
+    const uint64_t m = A->m, n = A->n;
+    RR_CHECK(A->row_ptr && A->row_ptr[0] == 0, RR_EINVAL, "row_ptr[0] must be 0");
+    for (uint64_t r = 0; r < m; r++)
+        RR_CHECK(A->row_ptr[r + 1] >= A->row_ptr[r], RR_EINVAL, "row_ptr not monotone at row %llu",
+                 (unsigned long long)r);
+    const uint64_t nnz = A->row_ptr[m];
+    RR_CHECK(nnz < (1ull << 32), RR_EINVAL, "nnz=%llu must be < 2^32", (unsigned long long)nnz);
+    RR_CHECK(nnz == 0 || (A->col && A->val), RR_EINVAL, "null col/val");
+    // entries in [-8, 7] (as signed residues) pack with a 28-bit index
+    bool small = m < (1ull << 28) && n < (1ull << 28);
+    for (uint64_t e = 0; e < nnz; e++) {
+        RR_CHECK(A->col[e] < n, RR_EINVAL, "col[%llu]=%u >= n", (unsigned long long)e, A->col[e]);
+        uint32_t v = A->val[e];
+        RR_CHECK(v < p, RR_EINVAL, "val[%llu]=%u >= p", (unsigned long long)e, v);
+        if (!(v <= 7 || v >= p - 8)) small = false;
+    }
+    for (uint64_t i = 0; i < m; i++)
+        RR_CHECK(row_precond[i] >= 1 && row_precond[i] < p, RR_EINVAL, "row_precond[%llu] not in [1,p)",
+                 (unsigned long long)i);
+    for (uint64_t i = 0; i < n; i++)
+        RR_CHECK(col_precond[i] >= 1 && col_precond[i] < p, RR_EINVAL, "col_precond[%llu] not in [1,p)",
+                 (unsigned long long)i);
+    for (uint64_t i = 0; i < n * b; i++)
+        RR_CHECK(curv[i] < p, RR_EINVAL, "curv[%llu] >= p", (unsigned long long)i);
+
+    std::unique_ptr<rr_krylov> c(new rr_krylov());
+    c->p = p;
+    c->b = b;
+    c->B = 1;
+    while (c->B < b) c->B <<= 1;
+    c->m = m;
+    c->n = n;
+    c->nnz = nnz;
+    c->packed = small;
+    c->mp = ModP::make(p);
+    c->P = b * (b + 1) / 2;
+    c->t = t_have;
+    RR_HIP(hipGetDevice(&c->device));
+    RR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    int cus = 0;
+    RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    c->grid = (uint32_t)std::max(1, cus * 2);
+    c->fix_grid = 8;
+
+    // A and A^T (counting-sort transpose, row order kept: src/matrices.rs:734-778)
+    std::vector<uint32_t> rp(m + 1), colw(nnz), vals(small ? 0 : nnz);
+    std::vector<uint32_t> rpt(n + 1, 0), colt(nnz), valt(small ? 0 : nnz);
+    for (uint64_t r = 0; r <= m; r++) rp[r] = (uint32_t)A->row_ptr[r];
+    for (uint64_t e = 0; e < nnz; e++) rpt[A->col[e] + 1]++;
+    for (uint64_t i = 0; i < n; i++) rpt[i + 1] += rpt[i];
+    {
+        std::vector<uint32_t> next(rpt.begin(), rpt.end() - 1);
+        for (uint64_t r = 0; r < m; r++)
+            for (uint64_t e = rp[r]; e < rp[r + 1]; e++) {
+                uint32_t cidx = A->col[e], v = A->val[e];
+                uint32_t pos = next[cidx]++;
+                if (small) {
+                    int32_t sv = v <= 7 ? (int32_t)v : (int32_t)v - (int32_t)p;
+                    uint32_t code = ((uint32_t)sv & 0xFu) << 28;
+                    colw[e] = cidx | code;
+                    colt[pos] = (uint32_t)r | code;
+                } else {
+                    colw[e] = cidx;
+                    vals[e] = v;
+                    colt[pos] = (uint32_t)r;
+                    valt[pos] = v;
+                }
+            }
+    }
+    rr_status s;
+    if ((s = build_operand(c->A, m, rp, colw, vals, small, c->B)) != RR_OK) return s;
+    if ((s = build_operand(c->At, n, rpt, colt, valt, small, c->B)) != RR_OK) return s;
+
+    // preconditioners: D_r, D_c, D_c^2, D_c^-1
+    std::vector<uint32_t> dc2(n), dcinv(n);
+    for (uint64_t i = 0; i < n; i++) {
+        dc2[i] = h_mulmod(col_precond[i], col_precond[i], p);
+        dcinv[i] = h_invmod(col_precond[i], p);
+    }
+    if ((s = upload(c->dr, row_precond, m)) != RR_OK) return s;
+    if ((s = upload(c->dc, col_precond, n)) != RR_OK) return s;
+    if ((s = upload(c->dc2, dc2.data(), n)) != RR_OK) return s;
+    if ((s = upload(c->dcinv, dcinv.data(), n)) != RR_OK) return s;
+
+    // block vectors, padded to stride B
+    const uint32_t B = c->B;
+    std::vector<uint32_t> xpad((size_t)n * B, 0);
+    for (uint64_t r = 0; r < n; r++) std::memcpy(&xpad[r * B], &curv[r * b], b * 4);
+    if ((s = upload(c->xtmp, xpad.data(), xpad.size())) != RR_OK) return s;
+    if ((s = c->z.alloc((size_t)n * B * 4)) != RR_OK) return s;
+    if ((s = c->y.alloc((size_t)m * B * 4)) != RR_OK) return s;
+    RR_HIP(hipMemsetAsync(c->y.ptr, 0, (size_t)m * B * 4, c->stream));
+    size_t tot = (size_t)n * B;
+    hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
+                       c->xtmp.as<uint32_t>(), c->dc.as<uint32_t>(), c->z.as<uint32_t>(), (uint32_t)n, B, p,
+                       c->mp.inv);
+    RR_HIP(hipGetLastError());
+    uint32_t nparts = c->grid + c->fix_grid;
+    if ((s = c->part.alloc((size_t)CHUNK * nparts * c->P * 4)) != RR_OK) return s;
+    if ((s = c->seq.alloc((size_t)CHUNK * c->P * 4)) != RR_OK) return s;
+    RR_HIP(hipMemsetAsync(c->part.ptr, 0, (size_t)CHUNK * nparts * c->P * 4, c->stream));
+    RR_HIP(hipStreamSynchronize(c->stream));
+    // algorithmic bytes per pass (int32 columns, u32 values, each array once)
+    c->stats.bytes_a = 8 * nnz + 4 * (m + 1) + (uint64_t)4 * b * (n + m);
+    c->stats.bytes_at = 8 * nnz + 4 * (n + 1) + (uint64_t)4 * b * (n + m);
+    *out = c.release();
+    return RR_OK;
+}
+
+rr_status rr_krylov_set_profiling(rr_krylov *c, int on) {
+    RR_CHECK(c, RR_EINVAL, "null context");
+    c->profiling = on != 0;
+    return RR_OK;
+}
+
+rr_status rr_krylov_stats_get(rr_krylov *c, rr_krylov_stats *out) {
+    RR_CHECK(c && out, RR_EINVAL, "null argument");
+    *out = c->stats;
+    return RR_OK;
+}
+
+rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
+    clear_error();
+    RR_CHECK(c && (seq_out || count == 0), RR_EINVAL, "null argument");
+    const uint32_t P = c->P, nparts = c->grid + c->fix_grid;
+    const size_t part_stride = (size_t)nparts * P;
+    uint64_t bytes_a = c->stats.bytes_a, bytes_at = c->stats.bytes_at;
+    c->stats = rr_krylov_stats{};
+    c->stats.bytes_a = bytes_a;
+    c->stats.bytes_at = bytes_at;
+
+    // event pairs per pass when profiling: [start, after pass, after fixup]
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind;  // 0 gram, 1 A pass, 2 A^T pass
+    hipEvent_t e_begin = nullptr, e_end = nullptr;
+    RR_HIP(hipEventCreate(&e_begin));
+    RR_HIP(hipEventCreate(&e_end));
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> red_ev;
+    RR_HIP(hipEventRecord(e_begin, c->stream));
+
+    uint64_t done = 0;
+    while (done < count) {
+        uint32_t chunk = (uint32_t)std::min<uint64_t>(CHUNK, count - done);
+        for (uint32_t q = 0; q < chunk; q++) {
+            uint64_t t = c->t;
+            uint32_t *gram = c->part.as<uint32_t>() + q * part_stride;
+            hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+            if (c->profiling) {
+                RR_HIP(hipEventCreate(&e0));
+                RR_HIP(hipEventCreate(&e1));
+                RR_HIP(hipEventCreate(&e2));
+                RR_HIP(hipEventRecord(e0, c->stream));
+            }
+            int k;
+            if (t == 0) {
+                // S_0 = x_0^T x_0, x_0 = curv; fix-up slots stay zero
+                RR_HIP(hipMemsetAsync(gram, 0, part_stride * 4, c->stream));
+                launch_gram_any(c, c->xtmp.as<uint32_t>(), (uint32_t)c->n, gram);
+                k = 0;
+                if (c->profiling) {
+                    RR_HIP(hipEventRecord(e1, c->stream));
+                    RR_HIP(hipEventRecord(e2, c->stream));
+                }
+            } else {
+                bool byA = (t & 1) == 1;
+                SpMat &M = byA ? c->A : c->At;
+                PassArgs a = byA ? make_args(c, M, c->z.as<uint32_t>(), c->y.as<uint32_t>(), c->dr.as<uint32_t>(), gram)
+                                 : make_args(c, M, c->y.as<uint32_t>(), c->z.as<uint32_t>(), c->dc2.as<uint32_t>(), gram);
+                launch_pass_any(c, a);
+                if (c->profiling) RR_HIP(hipEventRecord(e1, c->stream));
+                if (M.nlong) launch_fixup_any(c, a);
+                else RR_HIP(hipMemsetAsync(gram + (size_t)c->grid * P, 0, (size_t)c->fix_grid * P * 4, c->stream));
+                if (c->profiling) RR_HIP(hipEventRecord(e2, c->stream));
+                k = byA ? 1 : 2;
+            }
+            RR_HIP(hipGetLastError());
+            if (c->profiling) {
+                ev.push_back(e0);
+                ev.push_back(e1);
+                ev.push_back(e2);
+                kind.push_back(k);
+            }
+            if (k == 1) c->stats.passes_a++;
+            if (k == 2) c->stats.passes_at++;
+            c->t++;
+        }
+        hipEvent_t r0 = nullptr, r1 = nullptr;
+        if (c->profiling) {
+            RR_HIP(hipEventCreate(&r0));
+            RR_HIP(hipEventCreate(&r1));
+            RR_HIP(hipEventRecord(r0, c->stream));
+        }
+        uint32_t nthreads = chunk * P;
+        hipLaunchKernelGGL(k_reduce, dim3((nthreads + 255) / 256), dim3(256), 0, c->stream,
+                           c->part.as<uint32_t>(), nparts, P, chunk, c->seq.as<uint32_t>(), c->p);
+        RR_HIP(hipGetLastError());
+        if (c->profiling) {
+            RR_HIP(hipEventRecord(r1, c->stream));
+            red_ev.push_back({r0, r1});
+        }
+        RR_HIP(hipMemcpyAsync(seq_out + done * P, c->seq.ptr, (size_t)chunk * P * 4, hipMemcpyDeviceToHost,
+                              c->stream));
+        // the next chunk reuses the partial buffer and seq buffer: order is kept by the stream
+        RR_HIP(hipStreamSynchronize(c->stream));
+        done += chunk;
+    }
+    RR_HIP(hipEventRecord(e_end, c->stream));
+    RR_HIP(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    RR_HIP(hipEventElapsedTime(&ms, e_begin, e_end));
+    c->stats.ms_total = ms;
+    if (c->profiling) {
+        for (size_t i = 0; i < kind.size(); i++) {
+            float a = 0, f = 0;
+            RR_HIP(hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]));
+            RR_HIP(hipEventElapsedTime(&f, ev[3 * i + 1], ev[3 * i + 2]));
+            if (kind[i] == 1) c->stats.ms_a += a;
+            if (kind[i] == 2) c->stats.ms_at += a;
+            c->stats.ms_fixup += f;
+        }
+        for (auto &pr : red_ev) {
+            float r = 0;
+            RR_HIP(hipEventElapsedTime(&r, pr.first, pr.second));
+            c->stats.ms_reduce += r;
+        }
+        for (auto e : ev) (void)hipEventDestroy(e);
+        for (auto &pr : red_ev) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+    }
+    (void)hipEventDestroy(e_begin);
+    (void)hipEventDestroy(e_end);
+    return RR_OK;
+}
+
+rr_status rr_krylov_state(rr_krylov *c, uint32_t *curv_out, uint64_t *t_out) {
+    clear_error();
+    RR_CHECK(c && curv_out && t_out, RR_EINVAL, "null argument");
+    const uint32_t B = c->B;
+    const uint64_t n = c->n;
+    *t_out = c->t;
+    // z holds D_c x_K for the last even index reached (x_0 before any A^T pass)
+    size_t tot = (size_t)n * B;
+    hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
+                       c->z.as<uint32_t>(), c->dcinv.as<uint32_t>(), c->xtmp.as<uint32_t>(), (uint32_t)n, B, c->p,
+                       c->mp.inv);
+    RR_HIP(hipGetLastError());
+    std::vector<uint32_t> h(tot);
+    RR_HIP(hipMemcpyAsync(h.data(), c->xtmp.ptr, tot * 4, hipMemcpyDeviceToHost, c->stream));
+    RR_HIP(hipStreamSynchronize(c->stream));
+    for (uint64_t r = 0; r < n; r++) std::memcpy(&curv_out[r * c->b], &h[r * B], c->b * 4);
+    return RR_OK;
+}
+
+void rr_krylov_destroy(rr_krylov *c) { delete c; }
+
+}  // extern "C"

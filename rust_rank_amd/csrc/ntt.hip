@@ -1,0 +1,274 @@
+// ntt.hip -- number-theoretic transforms and polynomial-matrix products mod
+// p = 2013265921 (15 * 2^27 + 1, generator 31) on gfx950.
+//
+// Replaces the reference's ntt / matrix_ntt_parallel (src/ntt.rs:164-269) and
+// poly_mat_mul_fft_red (src/poly_mat_mul.rs:113-256); the CUDA twin is
+// cuda_test/cudantt.h:110-415 + cupoly_mat_mul.h:138-305.  The reference lifts
+// its operands to a 61-bit prime and reduces afterwards, which is only exact
+// for small p; here the transforms run directly mod p, so every product is
+// exact.
+//
+// Design: one workgroup transforms whole polynomials in LDS (up to 2^14 words
+// = 64 KB per workgroup; several short polynomials share a workgroup).  The
+// forward transform is decimation-in-frequency (natural order in, bit-reversed
+// out), the inverse is decimation-in-time (bit-reversed in, natural out), so a
+// product never permutes.  Twiddles come from a table w^k, k < N/2, with
+// Shoup companions floor(w^k 2^32 / p), so a twiddle multiply is one mulhi and
+// two mullo.  The 1/N of the inverse is folded into the pointwise product.
+#include "common.hpp"
+#include "ntt.hpp"
+
+namespace rr {
+
+__device__ __forceinline__ uint32_t shoup_mul(uint32_t a, uint32_t w, uint32_t wq, uint32_t p) {
+    uint32_t q = __umulhi(a, wq);
+    uint32_t r = a * w - q * p;  // in [0, 2p)
+    return r >= p ? r - p : r;
+}
+
+constexpr int NTT_THREADS = 256;
+constexpr uint32_t NTT_LDS_WORDS = 16384;
+
+// In-LDS DIF over `npoly` polynomials of length N = 2^logn laid out back to back.
+__device__ void lds_dif(uint32_t *s, uint32_t npoly, uint32_t logn, const uint32_t *tw, const uint32_t *twq,
+                        uint32_t tw_logn, uint32_t p) {
+    const uint32_t N = 1u << logn, half = N >> 1, total = npoly * half;
+    for (uint32_t lh = logn; lh-- > 0;) {  // butterfly span h = 2^lh
+        const uint32_t h = 1u << lh;
+        const uint32_t tstride = 1u << (tw_logn - lh - 1);  // w_{2h}^j = w_Ntab^(j * Ntab / 2h)
+        for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) {
+            uint32_t poly = t >> (logn - 1), bfl = t & (half - 1);
+            uint32_t j = bfl & (h - 1), blk = bfl >> lh;
+            uint32_t i0 = poly * N + blk * 2 * h + j, i1 = i0 + h;
+            uint32_t a = s[i0], b = s[i1];
+            uint32_t k = j * tstride;
+            s[i0] = addmod(a, b, p);
+            s[i1] = shoup_mul(submod(a, b, p), tw[k], twq[k], p);
+        }
+        __syncthreads();
+    }
+}
+
+// In-LDS DIT (inverse twiddles in tw), bit-reversed in, natural out, unscaled.
+__device__ void lds_dit(uint32_t *s, uint32_t npoly, uint32_t logn, const uint32_t *tw, const uint32_t *twq,
+                        uint32_t tw_logn, uint32_t p) {
+    const uint32_t N = 1u << logn, half = N >> 1, total = npoly * half;
+    for (uint32_t lh = 0; lh < logn; lh++) {
+        const uint32_t h = 1u << lh;
+        const uint32_t tstride = 1u << (tw_logn - lh - 1);
+        for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) {
+            uint32_t poly = t >> (logn - 1), bfl = t & (half - 1);
+            uint32_t j = bfl & (h - 1), blk = bfl >> lh;
+            uint32_t i0 = poly * N + blk * 2 * h + j, i1 = i0 + h;
+            uint32_t k = j * tstride;
+            uint32_t a = s[i0], b = shoup_mul(s[i1], tw[k], twq[k], p);
+            s[i0] = addmod(a, b, p);
+            s[i1] = submod(a, b, p);
+        }
+        __syncthreads();
+    }
+}
+
+// Forward transforms: dst[q][0..N) = DIF(src[q][0..len) zero-padded), for the
+// polynomials q of a [rows][cols] matrix with source stride `sstride` (words
+// between consecutive polynomials) and source offset `soff` (first coefficient).
+__global__ __launch_bounds__(NTT_THREADS) void k_ntt_fwd(const uint32_t *src, uint64_t sstride, uint32_t soff,
+                                                         uint32_t len, uint32_t *dst, uint32_t count,
+                                                         uint32_t logn, const uint32_t *tw, const uint32_t *twq,
+                                                         uint32_t tw_logn, uint32_t p) {
+    __shared__ uint32_t s[NTT_LDS_WORDS];
+    const uint32_t N = 1u << logn;
+    const uint32_t per = N >= NTT_LDS_WORDS ? 1 : NTT_LDS_WORDS / N;
+    const uint32_t q0 = blockIdx.x * per;
+    const uint32_t np = min(per, count - q0);
+    for (uint32_t i = threadIdx.x; i < np * N; i += blockDim.x) {
+        uint32_t q = q0 + i / N, k = i % N;
+        s[i] = k < len ? src[(size_t)q * sstride + soff + k] : 0u;
+    }
+    __syncthreads();
+    lds_dif(s, np, logn, tw, twq, tw_logn, p);
+    for (uint32_t i = threadIdx.x; i < np * N; i += blockDim.x) dst[(size_t)q0 * N + i] = s[i];
+}
+
+// Inverse transforms of `count` spectra (already scaled by 1/N), writing
+// coefficients [lo, lo + outlen) of each to dst[q * dstride + k - lo].
+__global__ __launch_bounds__(NTT_THREADS) void k_ntt_inv(const uint32_t *src, uint32_t count, uint32_t logn,
+                                                         uint32_t lo, uint32_t outlen, uint32_t *dst,
+                                                         uint64_t dstride, const uint32_t *tw, const uint32_t *twq,
+                                                         uint32_t tw_logn, uint32_t p) {
+    __shared__ uint32_t s[NTT_LDS_WORDS];
+    const uint32_t N = 1u << logn;
+    const uint32_t per = N >= NTT_LDS_WORDS ? 1 : NTT_LDS_WORDS / N;
+    const uint32_t q0 = blockIdx.x * per;
+    const uint32_t np = min(per, count - q0);
+    for (uint32_t i = threadIdx.x; i < np * N; i += blockDim.x) s[i] = src[(size_t)q0 * N + i];
+    __syncthreads();
+    lds_dit(s, np, logn, tw, twq, tw_logn, p);
+    for (uint32_t i = threadIdx.x; i < np * outlen; i += blockDim.x) {
+        uint32_t q = i / outlen, k = i % outlen;
+        dst[(size_t)(q0 + q) * dstride + k] = s[q * N + lo + k];
+    }
+}
+
+// Pointwise products: C[i][j][l] = scale * sum_r A[i][r][l] B[r][j][l] over
+// spectra of length N.  Thread (l, i) keeps JT entries of row i at point l as
+// exact fp64 sums (A split into 16-bit halves, folded every 64 terms).
+template <int JT>
+__global__ __launch_bounds__(256) void k_pointwise(const uint32_t *A, const uint32_t *B, uint32_t *C, uint32_t rows,
+                                                   uint32_t inner, uint32_t cols, uint32_t N, uint32_t scale,
+                                                   uint32_t p, double pd, double inv) {
+    const uint32_t l = blockIdx.x * 64 + (threadIdx.x & 63);
+    const uint32_t i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const uint32_t j0 = blockIdx.z * JT;
+    if (l >= N || i >= rows) return;
+    double hi[JT], lo[JT];
+#pragma unroll
+    for (int j = 0; j < JT; j++) hi[j] = lo[j] = 0.0;
+    const uint32_t jn = min((uint32_t)JT, cols - j0);
+    for (uint32_t r = 0; r < inner; r++) {
+        uint32_t a = A[((size_t)i * inner + r) * N + l];
+        const double ah = (double)(a >> 16), al = (double)(a & 0xFFFFu);
+        const uint32_t *bp = B + ((size_t)r * cols + j0) * N + l;
+#pragma unroll
+        for (int j = 0; j < JT; j++) {
+            if ((uint32_t)j < jn) {
+                double bd = (double)bp[(size_t)j * N];
+                hi[j] = fma(ah, bd, hi[j]);
+                lo[j] = fma(al, bd, lo[j]);
+            }
+        }
+        if ((r & 63) == 63) {
+#pragma unroll
+            for (int j = 0; j < JT; j++) {
+                hi[j] = fold(hi[j], pd, inv);
+                lo[j] = fold(lo[j], pd, inv);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < JT; j++) {
+        if ((uint32_t)j < jn) {
+            uint32_t v = (uint32_t)fold(fma(fold(hi[j], pd, inv), 65536.0, fold(lo[j], pd, inv)), pd, inv);
+            C[((size_t)i * cols + j0 + j) * N + l] = mulmod(v, scale, p, inv);
+        }
+    }
+}
+
+// Direct (schoolbook) truncated product for short polynomials:
+// C[i][j][k - lo] = sum_r sum_s A[i][r][s] B[r][j][k - s], k in [lo, hi),
+// A [rows][inner][la] (stride sa), B [inner][cols][lb] (stride sb, offset ob).
+__global__ __launch_bounds__(256) void k_pmul_school(const uint32_t *A, uint64_t sa, uint32_t la, const uint32_t *B,
+                                                     uint64_t sb, uint32_t ob, uint32_t lb, uint32_t *C, uint64_t sc,
+                                                     uint32_t rows, uint32_t inner, uint32_t cols, uint32_t lo,
+                                                     uint32_t hi, uint32_t p, double pd, double inv) {
+    const uint32_t outlen = hi - lo;
+    const uint64_t total = (uint64_t)rows * cols * outlen;
+    for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t k = (uint32_t)(idx % outlen) + lo;
+        uint64_t ij = idx / outlen;
+        uint32_t j = (uint32_t)(ij % cols), i = (uint32_t)(ij / cols);
+        uint32_t s_lo = k >= lb ? k - lb + 1 : 0, s_hi = min(k + 1, la);
+        double hi_acc = 0.0, lo_acc = 0.0;
+        uint32_t terms = 0;
+        for (uint32_t r = 0; r < inner; r++) {
+            const uint32_t *ar = A + ((size_t)i * inner + r) * sa;
+            const uint32_t *br = B + ((size_t)r * cols + j) * sb + ob;
+            for (uint32_t s = s_lo; s < s_hi; s++) {
+                uint32_t a = ar[s], b = br[k - s];
+                double bd = (double)b;
+                hi_acc = fma((double)(a >> 16), bd, hi_acc);
+                lo_acc = fma((double)(a & 0xFFFFu), bd, lo_acc);
+                if (++terms == 64) {
+                    hi_acc = fold(hi_acc, pd, inv);
+                    lo_acc = fold(lo_acc, pd, inv);
+                    terms = 0;
+                }
+            }
+        }
+        double v = fma(fold(hi_acc, pd, inv), 65536.0, fold(lo_acc, pd, inv));
+        C[(size_t)(i * cols + j) * sc + (k - lo)] = (uint32_t)fold(v, pd, inv);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+rr_status NttTables::init(uint32_t logn_max, hipStream_t stream) {
+    const uint32_t p = NTT_PRIME;
+    logn = logn_max;
+    uint32_t half = 1u << (logn - 1);
+    if (logn == 0) half = 1;
+    std::vector<uint32_t> f(half), fq(half), iv(half), ivq(half);
+    uint32_t w = h_powmod(NTT_ROOT, (p - 1) >> logn, p), wi = h_invmod(w, p);
+    uint64_t a = 1, b = 1;
+    for (uint32_t k = 0; k < half; k++) {
+        f[k] = (uint32_t)a;
+        iv[k] = (uint32_t)b;
+        fq[k] = (uint32_t)(((uint64_t)f[k] << 32) / p);
+        ivq[k] = (uint32_t)(((uint64_t)iv[k] << 32) / p);
+        a = a * w % p;
+        b = b * wi % p;
+    }
+    rr_status s;
+    if ((s = fwd.alloc(half * 4)) != RR_OK) return s;
+    if ((s = fwdq.alloc(half * 4)) != RR_OK) return s;
+    if ((s = inv.alloc(half * 4)) != RR_OK) return s;
+    if ((s = invq.alloc(half * 4)) != RR_OK) return s;
+    RR_HIP(hipMemcpyAsync(fwd.ptr, f.data(), half * 4, hipMemcpyHostToDevice, stream));
+    RR_HIP(hipMemcpyAsync(fwdq.ptr, fq.data(), half * 4, hipMemcpyHostToDevice, stream));
+    RR_HIP(hipMemcpyAsync(inv.ptr, iv.data(), half * 4, hipMemcpyHostToDevice, stream));
+    RR_HIP(hipMemcpyAsync(invq.ptr, ivq.data(), half * 4, hipMemcpyHostToDevice, stream));
+    RR_HIP(hipStreamSynchronize(stream));
+    return RR_OK;
+}
+
+static inline uint32_t blocks_for(uint32_t count, uint32_t logn) {
+    uint32_t N = 1u << logn;
+    uint32_t per = N >= NTT_LDS_WORDS ? 1 : NTT_LDS_WORDS / N;
+    return (count + per - 1) / per;
+}
+
+rr_status ntt_forward(const NttTables &T, const uint32_t *src, uint64_t sstride, uint32_t soff, uint32_t len,
+                      uint32_t *dst, uint32_t count, uint32_t logn, hipStream_t st) {
+    RR_CHECK(logn >= 1 && logn <= T.logn && logn <= 14, RR_EINVAL, "NTT size 2^%u unsupported", logn);
+    hipLaunchKernelGGL(k_ntt_fwd, dim3(blocks_for(count, logn)), dim3(NTT_THREADS), 0, st, src, sstride, soff,
+                       len, dst, count, logn, T.fwd.as<uint32_t>(), T.fwdq.as<uint32_t>(), T.logn, NTT_PRIME);
+    RR_HIP(hipGetLastError());
+    return RR_OK;
+}
+
+rr_status ntt_inverse(const NttTables &T, const uint32_t *src, uint32_t count, uint32_t logn, uint32_t lo,
+                      uint32_t outlen, uint32_t *dst, uint64_t dstride, hipStream_t st) {
+    RR_CHECK(logn >= 1 && logn <= T.logn && logn <= 14, RR_EINVAL, "NTT size 2^%u unsupported", logn);
+    hipLaunchKernelGGL(k_ntt_inv, dim3(blocks_for(count, logn)), dim3(NTT_THREADS), 0, st, src, count, logn, lo,
+                       outlen, dst, dstride, T.inv.as<uint32_t>(), T.invq.as<uint32_t>(), T.logn, NTT_PRIME);
+    RR_HIP(hipGetLastError());
+    return RR_OK;
+}
+
+rr_status pointwise(const uint32_t *A, const uint32_t *B, uint32_t *C, uint32_t rows, uint32_t inner,
+                    uint32_t cols, uint32_t N, uint32_t scale, hipStream_t st) {
+    const uint32_t p = NTT_PRIME;
+    ModP mp = ModP::make(p);
+    constexpr int JT = 16;
+    dim3 grid((N + 63) / 64, (rows + 3) / 4, (cols + JT - 1) / JT);
+    hipLaunchKernelGGL(k_pointwise<JT>, grid, dim3(256), 0, st, A, B, C, rows, inner, cols, N, scale, p, mp.pd,
+                       mp.inv);
+    RR_HIP(hipGetLastError());
+    return RR_OK;
+}
+
+rr_status pmul_school(const uint32_t *A, uint64_t sa, uint32_t la, const uint32_t *B, uint64_t sb, uint32_t ob,
+                      uint32_t lb, uint32_t *C, uint64_t sc, uint32_t rows, uint32_t inner, uint32_t cols,
+                      uint32_t lo, uint32_t hi, uint32_t p, hipStream_t st) {
+    uint64_t total = (uint64_t)rows * cols * (hi - lo);
+    uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 8192);
+    ModP mp = ModP::make(p);
+    hipLaunchKernelGGL(k_pmul_school, dim3(std::max(blocks, 1u)), dim3(256), 0, st, A, sa, la, B, sb, ob, lb, C, sc,
+                       rows, inner, cols, lo, hi, p, mp.pd, mp.inv);
+    RR_HIP(hipGetLastError());
+    return RR_OK;
+}
+
+}  // namespace rr

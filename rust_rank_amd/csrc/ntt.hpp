@@ -1,0 +1,35 @@
+// ntt.hpp -- device NTT and polynomial-matrix product primitives (mod 2013265921).
+#pragma once
+#include "common.hpp"
+#include <vector>
+
+namespace rr {
+
+constexpr uint32_t NTT_PRIME = 2013265921u;  // 15 * 2^27 + 1 (src/ntt.rs:52)
+constexpr uint32_t NTT_ROOT = 31u;           // generator (src/ntt.rs:53)
+constexpr uint32_t NTT_MAX_LOGN = 14;        // one workgroup, one polynomial in 64 KB of LDS
+
+// Twiddle tables for transforms up to 2^logn points: w^k and w^-k for k < 2^(logn-1)
+// with their Shoup companions.
+struct NttTables {
+    uint32_t logn = 0;
+    DevBuf fwd, fwdq, inv, invq;
+    rr_status init(uint32_t logn_max, hipStream_t stream);
+};
+
+// dst[q][0..2^logn) = forward spectrum (bit-reversed order) of src[q * sstride + soff + k], k < len.
+rr_status ntt_forward(const NttTables &T, const uint32_t *src, uint64_t sstride, uint32_t soff, uint32_t len,
+                      uint32_t *dst, uint32_t count, uint32_t logn, hipStream_t st);
+// dst[q * dstride + k] = coefficient lo + k (k < outlen) of the inverse of spectrum src[q] (no 1/N).
+rr_status ntt_inverse(const NttTables &T, const uint32_t *src, uint32_t count, uint32_t logn, uint32_t lo,
+                      uint32_t outlen, uint32_t *dst, uint64_t dstride, hipStream_t st);
+// C[i][j][l] = scale * sum_r A[i][r][l] B[r][j][l], spectra of length N (mod 2013265921).
+rr_status pointwise(const uint32_t *A, const uint32_t *B, uint32_t *C, uint32_t rows, uint32_t inner,
+                    uint32_t cols, uint32_t N, uint32_t scale, hipStream_t st);
+// Direct truncated product mod any p < 2^31:
+// C[i][j][k - lo] (stride sc) = sum_r sum_s A[i][r][s] B[r][j][ob + k - s], k in [lo, hi).
+rr_status pmul_school(const uint32_t *A, uint64_t sa, uint32_t la, const uint32_t *B, uint64_t sb, uint32_t ob,
+                      uint32_t lb, uint32_t *C, uint64_t sc, uint32_t rows, uint32_t inner, uint32_t cols,
+                      uint32_t lo, uint32_t hi, uint32_t p, hipStream_t st);
+
+}  // namespace rr

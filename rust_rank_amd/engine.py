@@ -1,0 +1,204 @@
+"""ctypes binding of librank_engine.so (include/rank_engine.h).
+
+The library is the product: there is no CPU fallback.  Importing this module
+on a machine without the built library raises, and every call that fails on
+the device raises RankEngineError with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librank_engine.so")
+
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+class RankEngineError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"rank engine status {status}: {msg}")
+        self.status = status
+
+
+class _Csr(C.Structure):
+    _fields_ = [("m", C.c_uint64), ("n", C.c_uint64), ("row_ptr", C.c_void_p), ("col", C.c_void_p),
+                ("val", C.c_void_p)]
+
+
+class KrylovStats(C.Structure):
+    _fields_ = [("passes_a", C.c_uint64), ("passes_at", C.c_uint64), ("ms_a", C.c_double), ("ms_at", C.c_double),
+                ("ms_fixup", C.c_double), ("ms_reduce", C.c_double), ("ms_total", C.c_double),
+                ("bytes_a", C.c_uint64), ("bytes_at", C.c_uint64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Loads librank_engine.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m rust_rank_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    L.rr_last_error.restype = C.c_char_p
+    L.rr_device_count.restype = C.c_int
+    L.rr_krylov_create.argtypes = [C.POINTER(_Csr), C.c_uint32, C.c_uint32, _u32p, _u32p, _u32p, C.c_uint64,
+                                   C.POINTER(C.c_void_p)]
+    L.rr_krylov_run.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
+    L.rr_krylov_state.argtypes = [C.c_void_p, _u32p, C.POINTER(C.c_uint64)]
+    L.rr_krylov_set_profiling.argtypes = [C.c_void_p, C.c_int]
+    L.rr_krylov_stats_get.argtypes = [C.c_void_p, C.POINTER(KrylovStats)]
+    L.rr_krylov_destroy.argtypes = [C.c_void_p]
+    L.rr_krylov_destroy.restype = None
+    L.rr_pm_basis.argtypes = [_u32p, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint32, _u32p, _i64p]
+    L.rr_rank_from_basis.argtypes = [_u32p, _i64p, C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(C.c_int64)]
+    L.rr_ntt.argtypes = [_u32p, C.c_uint32, C.c_uint32, C.c_int]
+    L.rr_poly_mat_mul.argtypes = [_u32p, C.c_uint64, _u32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
+                                  C.c_uint64, C.c_uint64, C.c_uint32, _u32p]
+    L.rr_seq_len.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int]
+    L.rr_seq_len.restype = C.c_uint64
+    L.rr_pm_order.argtypes = [C.c_uint64, C.c_int]
+    L.rr_pm_order.restype = C.c_uint64
+    L.rr_rank.argtypes = [C.POINTER(_Csr), C.c_uint32, C.c_uint32, _u32p, _u32p, _u32p, C.c_uint64, C.c_uint64,
+                          C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    L.rr_rank.restype = C.c_int
+    for name in ("rr_krylov_create", "rr_krylov_run", "rr_krylov_state", "rr_krylov_set_profiling",
+                 "rr_krylov_stats_get", "rr_pm_basis", "rr_rank_from_basis", "rr_ntt", "rr_poly_mat_mul"):
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(status: int):
+    if status != 0:
+        raise RankEngineError(status, lib().rr_last_error().decode())
+
+
+def device_count() -> int:
+    return int(lib().rr_device_count())
+
+
+def _csr_struct(A):
+    rp = np.ascontiguousarray(A.row_ptr, dtype=np.uint64)
+    col = np.ascontiguousarray(A.col, dtype=np.uint32)
+    val = np.ascontiguousarray(A.val, dtype=np.uint32)
+    s = _Csr(A.m, A.n, rp.ctypes.data, col.ctypes.data, val.ctypes.data)
+    return s, (rp, col, val)
+
+
+class Krylov:
+    """Device Krylov stream over B = D_c A^T D_r A D_c (rr_krylov_*)."""
+
+    def __init__(self, A, p: int, b: int, row_precond, col_precond, curv, t_have: int = 0):
+        s, keep = _csr_struct(A)
+        self.b, self.p, self.n = b, p, A.n
+        self.P = b * (b + 1) // 2
+        h = C.c_void_p()
+        _check(lib().rr_krylov_create(C.byref(s), p, b, np.ascontiguousarray(row_precond, dtype=np.uint32),
+                                      np.ascontiguousarray(col_precond, dtype=np.uint32),
+                                      np.ascontiguousarray(curv, dtype=np.uint32), t_have, C.byref(h)))
+        self._h = h
+
+    def run(self, count: int) -> np.ndarray:
+        out = np.zeros((max(count, 1), self.P), np.uint32)
+        _check(lib().rr_krylov_run(self._h, count, out.ctypes.data))
+        return out[:count]
+
+    def state(self):
+        curv = np.zeros((self.n, self.b), np.uint32)
+        t = C.c_uint64()
+        _check(lib().rr_krylov_state(self._h, curv, C.byref(t)))
+        return curv, t.value
+
+    def set_profiling(self, on: bool = True):
+        _check(lib().rr_krylov_set_profiling(self._h, int(on)))
+
+    def stats(self) -> dict:
+        st = KrylovStats()
+        _check(lib().rr_krylov_stats_get(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rr_krylov_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def pm_basis(seq, b: int, d: int, p: int):
+    """Order-d PM-basis of [S(x); I_b] on the device (rr_pm_basis)."""
+    seq = np.ascontiguousarray(seq, dtype=np.uint32)
+    basis = np.zeros((2 * b, 2 * b, d + 1), np.uint32)
+    delta = np.zeros(2 * b, np.int64)
+    _check(lib().rr_pm_basis(seq, b, seq.shape[0], d, p, basis, delta))
+    return basis, delta
+
+
+def rank_from_basis(basis, delta, b: int, p: int) -> int:
+    basis = np.ascontiguousarray(basis, dtype=np.uint32)
+    r = C.c_int64()
+    _check(lib().rr_rank_from_basis(basis, np.ascontiguousarray(delta, dtype=np.int64), b, basis.shape[2], p,
+                                    C.byref(r)))
+    return r.value
+
+
+def ntt(data, inverse: bool = False):
+    """Natural-order NTT mod 2013265921 of each row of `data` (rows of 2^logn points)."""
+    a = np.array(data, dtype=np.uint32, copy=True, order="C")
+    if a.ndim == 1:
+        a = a[None, :]
+    n = a.shape[1]
+    logn = n.bit_length() - 1
+    _check(lib().rr_ntt(a, logn, a.shape[0], int(inverse)))
+    return a
+
+
+def poly_mat_mul(A, B, lo: int, hi: int, p: int):
+    A = np.ascontiguousarray(A, dtype=np.uint32)
+    B = np.ascontiguousarray(B, dtype=np.uint32)
+    rows, inner, la = A.shape
+    inner2, cols, lb = B.shape
+    assert inner == inner2
+    out = np.zeros((rows, cols, hi - lo), np.uint32)
+    _check(lib().rr_poly_mat_mul(A, la, B, lb, rows, inner, cols, lo, hi, p, out))
+    return out
+
+
+def seq_len(m: int, n: int, b: int, reference_policy: bool = False) -> int:
+    return int(lib().rr_seq_len(m, n, b, int(reference_policy)))
+
+
+def pm_order(L: int, reference_policy: bool = False) -> int:
+    return int(lib().rr_pm_order(L, int(reference_policy)))
+
+
+def rank(A, p: int, b: int, row_precond, col_precond, V, seq_len: int = 0, order: int = 0):
+    """Full device pipeline (rr_rank): returns (rank, milliseconds)."""
+    s, keep = _csr_struct(A)
+    r = C.c_int64()
+    ms = C.c_double()
+    _check(lib().rr_rank(C.byref(s), p, b, np.ascontiguousarray(row_precond, dtype=np.uint32),
+                         np.ascontiguousarray(col_precond, dtype=np.uint32), np.ascontiguousarray(V, dtype=np.uint32),
+                         seq_len, order, C.byref(r), C.byref(ms)))
+    return r.value, ms.value

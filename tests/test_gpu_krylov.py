@@ -1,0 +1,95 @@
+"""GPU parity of the Krylov engine (rr_krylov_* through the C ABI) against the
+CPU oracle on identical seeded inputs: every S_t must be bit-identical."""
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from rust_rank_amd import synth
+
+P = 2013265921
+pytestmark = pytest.mark.gpu
+
+
+def _engine():
+    from rust_rank_amd import engine
+    assert engine.device_count() >= 1, "no GPU visible"
+    return engine
+
+
+def _case(A, p, b, seed, L, t_have=0):
+    eng = _engine()
+    dr, dc, V = synth.preconditioners(seed, A.m, A.n, b, p)
+    print(f"case m={A.m} n={A.n} nnz={A.nnz} b={b} p={p} L={L}", flush=True)
+    want, x_want = O.krylov(A, p, b, dr, dc, V, 0, L, want_x=True)
+    with eng.Krylov(A, p, b, dr, dc, V) as K:
+        got = K.run(L)
+        curv, t = K.state()
+    assert t == L
+    assert np.array_equal(got, want), np.argwhere(got != want)[:5]
+    assert np.array_equal(curv, x_want)
+
+
+@pytest.mark.parametrize("b", [1, 2, 3, 4, 8, 16, 32, 64])
+def test_general_values(b):
+    A = synth.uniform_sparse(b, 1, 300, 260, 1, 9, P, small=False)
+    _case(A, P, b, b, 2 * 260 // b + 8)
+
+
+@pytest.mark.parametrize("b", [4, 8, 32])
+def test_small_values_packed(b):
+    A = synth.batch_matrix(3, b, rows_lo=400, rows_hi=600)
+    _case(A, P, b, b + 1, 40)
+
+
+def test_small_prime():
+    A = synth.uniform_sparse(9, 1, 200, 150, 2, 6, 3323, small=False)
+    _case(A, 3323, 8, 9, 45)
+
+
+def test_long_rows_split():
+    """A^T rows far above the split threshold (Zipf head): pieces + fix-up."""
+    A = synth.graph_complex(seed=5, m=40000, n=2000, k=12)
+    assert int(np.bincount(A.col).max()) > 4096
+    _case(A, P, 8, 5, 12)
+
+
+def test_plain_small_full_sequence():
+    A = synth.plain_small()
+    L = synth.seq_len_plain(min(A.m, A.n), 8)
+    _case(A, P, 8, 1, L)
+
+
+def test_resume_matches_continuous():
+    eng = _engine()
+    A = synth.uniform_sparse(2, 1, 500, 400, 3, 8, P, small=False)
+    b = 4
+    dr, dc, V = synth.preconditioners(2, A.m, A.n, b, P)
+    with eng.Krylov(A, P, b, dr, dc, V) as K:
+        first = K.run(21)
+        curv, t = K.state()
+        rest = K.run(20)
+    assert t == 21
+    with eng.Krylov(A, P, b, dr, dc, curv, t_have=21) as K2:
+        rest2 = K2.run(20)
+    assert np.array_equal(rest, rest2)
+    want = O.krylov(A, P, b, dr, dc, V, 0, 41)
+    assert np.array_equal(np.vstack([first, rest]), want)
+
+
+def test_bad_inputs_raise():
+    eng = _engine()
+    A = synth.uniform_sparse(2, 1, 50, 40, 3, 8, P, small=False)
+    dr, dc, V = synth.preconditioners(2, A.m, A.n, 4, P)
+    with pytest.raises(eng.RankEngineError):
+        eng.Krylov(A, P, 65, dr, dc, np.zeros((A.n, 65), np.uint32))
+    with pytest.raises(eng.RankEngineError):
+        eng.Krylov(A, 2013265920, 4, dr, dc, V)  # not prime
+    bad = A.val.copy()
+    bad[0] = P
+    B = synth.Csr(A.m, A.n, A.row_ptr, A.col, bad)
+    with pytest.raises(eng.RankEngineError):
+        eng.Krylov(B, P, 4, dr, dc, V)
+    with pytest.raises(eng.RankEngineError):
+        eng.Krylov(A, P, 4, dr, dc, V, t_have=2)
