@@ -156,6 +156,60 @@ uint64_t rr_pm_order(uint64_t seq_len, int reference_policy);
 rr_status rr_rank(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *row_precond, const uint32_t *col_precond,
                   const uint32_t *V, uint64_t seq_len, uint64_t order, int64_t *rank_out, double *ms_out);
 
+/* Timing of one rr_batch_rank call. */
+typedef struct {
+    double ms_setup, ms_krylov, ms_pm, ms_total;
+    uint64_t ticks, pm_rounds, pm_launches;
+} rr_batch_stats;
+
+/*
+ * Many rank problems on one GPU (no counterpart in the reference, which
+ * solves one matrix per process, src/main.rs:254-639): every matrix i gets
+ * the reference pipeline (Krylov sequence of seq_len[i] entries with block
+ * width b[i] and its own D_r, D_c, V; PM-basis of order order[i]; rank
+ * formula), but all sequences advance together, one entry per tick, in shared
+ * launches, and all PM-bases run as one batched scheduler.  Each result is
+ * bit-identical to solving the matrix alone.  seq_len/order may be NULL or
+ * hold 0 for the exact policy.  V[i] is n_i x b[i] row-interleaved.
+ */
+rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const uint32_t *b,
+                        const uint32_t *const *row_precond, const uint32_t *const *col_precond,
+                        const uint32_t *const *V, const uint64_t *seq_len, const uint64_t *order,
+                        int64_t *rank_out, rr_batch_stats *stats);
+
+/*
+ * Continuous-batching rank server (no counterpart in the reference).  A
+ * worker thread admits submitted requests in arrival order while their
+ * resident footprint fits hbm_budget_bytes, keeps each request's matrix,
+ * vectors and partial sequence on the device, and interleaves the Krylov
+ * entries of all admitted requests (shared launches) with batched PM-basis
+ * rounds of the requests whose sequence is complete.  Each result is
+ * bit-identical to rr_rank on the same inputs; a failing request is reported
+ * with its own status.  Concurrent submits from several threads are allowed.
+ */
+typedef struct rr_server rr_server;
+typedef struct {
+    uint64_t id;
+    int64_t rank;       /* -1 when status != RR_OK */
+    int32_t status;
+    double latency_ms;  /* submit to result */
+    double queue_ms;    /* submit to admission */
+} rr_result;
+typedef struct {
+    uint64_t queued, used_bytes, completed, failed, ticks, pm_rounds, max_active;
+} rr_server_stats;
+
+rr_status rr_server_create(uint32_t p, uint64_t hbm_budget_bytes, rr_server **out);
+/* Copies the inputs; seq_len / order 0 select the exact policy. */
+rr_status rr_server_submit(rr_server *s, const rr_csr *A, uint32_t b, const uint32_t *row_precond,
+                           const uint32_t *col_precond, const uint32_t *V, uint64_t seq_len, uint64_t order,
+                           uint64_t *id_out);
+/* Moves up to cap finished results into out; waits up to wait_ms when none is ready. */
+rr_status rr_server_poll(rr_server *s, rr_result *out, uint32_t cap, uint32_t *n_out, int wait_ms);
+rr_status rr_server_stats_get(rr_server *s, rr_server_stats *out);
+/* Stops the worker (requests still queued or running are dropped) and frees everything. */
+void rr_server_destroy(rr_server *s);
+
 #ifdef __cplusplus
 }
 #endif

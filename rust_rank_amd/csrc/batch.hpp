@@ -1,0 +1,72 @@
+// batch.hpp -- resident rank requests and the tick runner shared by the
+// batched and served modes.
+#pragma once
+#include "common.hpp"
+#include "pmbasis.hpp"
+#include <vector>
+
+namespace rr {
+
+enum : uint32_t { TILE_PACKED = 0, TILE_GENERAL = 1, TILE_GRAM = 2 };
+
+// A row range of one operand of one request, fully self-described so a
+// single launch can mix requests, sizes and pass directions.
+struct Tile {
+    const uint32_t *rowptr, *col, *val;  // operand (val unused when packed)
+    const uint32_t *in;                  // gathered block vector (stride B); the vector itself for TILE_GRAM
+    uint32_t *out;                       // produced rows (stride B)
+    const uint32_t *scale;               // per-row factor
+    uint32_t *part;                      // this tile's b(b+1)/2 projection partial
+    uint32_t row_beg, row_end;
+    uint32_t b, mode;
+};
+
+struct RedJob {  // S_t of one request = sum of its tiles' partials
+    const uint32_t *part;  // ntiles consecutive partials of P words
+    uint32_t ntiles, P;
+    uint32_t *out;
+};
+
+enum : int { STAGE_KRYLOV = 0, STAGE_PM = 1, STAGE_DONE = 2, STAGE_FAILED = 3 };
+
+// resident device footprint of one request (admission control), bytes
+uint64_t request_bytes(uint64_t m, uint64_t n, uint64_t nnz, uint32_t b, uint64_t L, uint64_t d, bool packed);
+
+struct Request {
+    uint32_t p = 0, b = 0, B = 0, P = 0;
+    uint64_t m = 0, n = 0, nnz = 0, L = 0, d = 0, t = 0;
+    bool packed = false;
+    int stage = STAGE_KRYLOV;
+    int64_t rank = -1;
+    uint64_t id = 0, budget_bytes = 0;
+    double t_arrive = 0, t_admit = 0, t_done = 0;
+    DevBuf a_rp, a_col, a_val, t_rp, t_col, t_val, dr_d, dc_d, dc2_d, x0, z, y, seq, G, M, delta, rank_in;
+    std::vector<Tile> tiles_a, tiles_at, tiles_gram;
+    PmJob pm;
+
+    rr_status init(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *dr, const uint32_t *dc,
+                   const uint32_t *V, uint64_t L, uint64_t d, uint32_t tile_rows, hipStream_t st);
+    const std::vector<Tile> &next_tiles() const;
+    rr_status start_pm(hipStream_t st);     // sequence complete: build G, join the PM scheduler
+    rr_status finish_rank(hipStream_t st);  // PM done: rank formula from the extracted coefficients
+    uint64_t device_bytes() const;          // resident footprint estimate (admission control)
+};
+
+struct TickRunner {
+    static constexpr size_t STAGE_BYTES = 4u << 20;
+    static constexpr int NSTAGE = 8;
+    hipStream_t st = nullptr;
+    int cus = 0;
+    char *stage_host = nullptr;
+    hipEvent_t stage_ev[NSTAGE] = {};
+    int stage_next = 0;
+    DevBuf tiles_dev, part;
+    size_t part_cap = 0;
+    uint64_t ticks = 0;
+
+    rr_status init(hipStream_t st, int cus);
+    rr_status tick(std::vector<Request *> &reqs, uint32_t p);
+    ~TickRunner();
+};
+
+}  // namespace rr

@@ -25,6 +25,7 @@
 // per pass; a reduction kernel folds the partials of a chunk of passes into
 // the sequence.
 #include "common.hpp"
+#include "krylov_kernels.hpp"
 #include <vector>
 #include <algorithm>
 #include <cstring>
@@ -32,7 +33,6 @@
 
 namespace rr {
 
-constexpr int WG = 256;             // threads per workgroup (4 wavefronts)
 constexpr uint32_t LONG_ROW = 4096; // rows above this many nonzeros are split
 constexpr uint32_t PIECE = 2048;    // nonzeros per piece of a split row
 constexpr int CHUNK = 64;           // passes per sequence reduction
@@ -57,132 +57,6 @@ struct PassArgs {
 };
 
 // ---------------------------------------------------------------------------
-// one row of the sparse product, lane j: sum_e v_e * in[col_e][j] mod p
-template <int B, bool PACKED>
-__device__ __forceinline__ uint32_t row_dot(const PassArgs &a, uint32_t beg, uint32_t end, int j) {
-    if constexpr (PACKED) {
-        // |v| <= 8, x < 2^31: each term < 2^34, exact in fp64 for 2^19 terms
-        double acc0 = 0.0, acc1 = 0.0;
-        uint32_t e = beg;
-        for (; e + 4 <= end; e += 4) {
-            uint32_t c0 = a.col[e], c1 = a.col[e + 1], c2 = a.col[e + 2], c3 = a.col[e + 3];
-            uint32_t x0 = a.in[(size_t)(c0 & 0x0FFFFFFFu) * B + j];
-            uint32_t x1 = a.in[(size_t)(c1 & 0x0FFFFFFFu) * B + j];
-            uint32_t x2 = a.in[(size_t)(c2 & 0x0FFFFFFFu) * B + j];
-            uint32_t x3 = a.in[(size_t)(c3 & 0x0FFFFFFFu) * B + j];
-            acc0 = fma((double)((int32_t)c0 >> 28), (double)x0, acc0);
-            acc1 = fma((double)((int32_t)c1 >> 28), (double)x1, acc1);
-            acc0 = fma((double)((int32_t)c2 >> 28), (double)x2, acc0);
-            acc1 = fma((double)((int32_t)c3 >> 28), (double)x3, acc1);
-        }
-        for (; e < end; e++) {
-            uint32_t c0 = a.col[e];
-            uint32_t x0 = a.in[(size_t)(c0 & 0x0FFFFFFFu) * B + j];
-            acc0 = fma((double)((int32_t)c0 >> 28), (double)x0, acc0);
-        }
-        return (uint32_t)fold(acc0 + acc1, a.pd, a.inv);
-    } else {
-        // v = vh 2^16 + vl: vh x < 2^46, vl x < 2^47; fold every 64 terms
-        double hi = 0.0, lo = 0.0;
-        uint32_t e = beg;
-        while (e < end) {
-            uint32_t stop = min(end, e + 64u);
-            for (; e + 2 <= stop; e += 2) {
-                uint32_t c0 = a.col[e], c1 = a.col[e + 1];
-                uint32_t v0 = a.val[e], v1 = a.val[e + 1];
-                double x0 = (double)a.in[(size_t)c0 * B + j];
-                double x1 = (double)a.in[(size_t)c1 * B + j];
-                hi = fma((double)(v0 >> 16), x0, hi);
-                lo = fma((double)(v0 & 0xFFFFu), x0, lo);
-                hi = fma((double)(v1 >> 16), x1, hi);
-                lo = fma((double)(v1 & 0xFFFFu), x1, lo);
-            }
-            if (e < stop) {
-                uint32_t c0 = a.col[e], v0 = a.val[e];
-                double x0 = (double)a.in[(size_t)c0 * B + j];
-                hi = fma((double)(v0 >> 16), x0, hi);
-                lo = fma((double)(v0 & 0xFFFFu), x0, lo);
-                e++;
-            }
-            hi = fold(hi, a.pd, a.inv);
-            lo = fold(lo, a.pd, a.inv);
-        }
-        return (uint32_t)fold(fma(hi, 65536.0, lo), a.pd, a.inv);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Per-lane accumulation of the symmetric projection sum_r o_r s_r^T.
-// Lane j of a row pairs its own s_j with o_{(j+k) mod B} for k = 0..B/2, which
-// covers every unordered pair once (k = B/2 only for j < B/2).  s is split in
-// 16-bit halves so each product fits fp64 exactly; partial sums fold every 64
-// rows.
-template <int B>
-struct Gram {
-    static constexpr int K = B / 2 + 1;
-    double hi[K], lo[K];
-    int rows;
-
-    __device__ void init() {
-#pragma unroll
-        for (int k = 0; k < K; k++) hi[k] = lo[k] = 0.0;
-        rows = 0;
-    }
-    // all 64 lanes must call this together
-    __device__ __forceinline__ void add(uint32_t o, uint32_t s, int lane, double pd, double inv) {
-        const int base = lane & ~(B - 1);
-        const int j = lane & (B - 1);
-        const double sh = (double)(s >> 16), sl = (double)(s & 0xFFFFu);
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            uint32_t ok = (k == 0) ? o : (uint32_t)__shfl((int)o, base + ((j + k) & (B - 1)), 64);
-            double od = (double)ok;
-            hi[k] = fma(od, sh, hi[k]);
-            lo[k] = fma(od, sl, lo[k]);
-        }
-        if (++rows == 64) {
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                hi[k] = fold(hi[k], pd, inv);
-                lo[k] = fold(lo[k], pd, inv);
-            }
-            rows = 0;
-        }
-    }
-    // Reduce over the wavefront's rows and the workgroup's wavefronts, then
-    // write the b(b+1)/2 upper-triangle partial to out.
-    __device__ void finish(uint32_t *out, uint32_t b, uint32_t p, double pd, double inv, uint32_t *lds) {
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = lane & (B - 1);
-        uint32_t v[K];
-#pragma unroll
-        for (int k = 0; k < K; k++) v[k] = (uint32_t)fold(fma(fold(hi[k], pd, inv), 65536.0, fold(lo[k], pd, inv)), pd, inv);
-        // rows of the same wavefront: lanes j, j + B, j + 2B, ...
-#pragma unroll
-        for (int off = B; off < 64; off <<= 1) {
-#pragma unroll
-            for (int k = 0; k < K; k++) v[k] = addmod(v[k], (uint32_t)__shfl_xor((int)v[k], off, 64), p);
-        }
-        if (lane < B) {
-#pragma unroll
-            for (int k = 0; k < K; k++) lds[(wave * B + j) * K + k] = v[k];
-        }
-        __syncthreads();
-        const int nw = blockDim.x >> 6;
-        for (int t = threadIdx.x; t < B * K; t += blockDim.x) {
-            int jj = t / K, k = t % K;
-            uint32_t s = 0;
-            for (int w = 0; w < nw; w++) s = addmod(s, lds[(w * B + jj) * K + k], p);
-            int i1 = jj, i2 = (jj + k) & (B - 1);
-            if (2 * k == B && jj >= B / 2) continue;  // that pair is covered by jj - B/2
-            if ((uint32_t)i1 >= b || (uint32_t)i2 >= b) continue;
-            uint32_t lo_i = min(i1, i2), hi_i = max(i1, i2);
-            uint32_t ii = lo_i * b - lo_i * (lo_i - 1) / 2 + (hi_i - lo_i);
-            out[ii] = s;
-        }
-    }
-};
-
-// ---------------------------------------------------------------------------
 // One sparse pass: pieces of split rows, then every short row (product,
 // scaling, store, projection).
 template <int B, bool PACKED>
@@ -195,7 +69,7 @@ __global__ __launch_bounds__(WG) void k_pass(PassArgs a) {
     const uint32_t nwaves = gridDim.x * (WG / 64);
 
     for (uint32_t pc = gwave * G + slot; pc < a.npieces; pc += nwaves * G)
-        a.piece_out[(size_t)pc * B + j] = row_dot<B, PACKED>(a, a.piece_beg[pc], a.piece_end[pc], j);
+        a.piece_out[(size_t)pc * B + j] = row_dot<B, PACKED>(a.col, a.val, a.in, a.piece_beg[pc], a.piece_end[pc], j, a.pd, a.inv);
 
     Gram<B> g;
     g.init();
@@ -205,7 +79,7 @@ __global__ __launch_bounds__(WG) void k_pass(PassArgs a) {
         if (r < a.nrows) {
             uint32_t beg = a.rowptr[r], end = a.rowptr[r + 1];
             if (end - beg <= LONG_ROW) {
-                o = row_dot<B, PACKED>(a, beg, end, j);
+                o = row_dot<B, PACKED>(a.col, a.val, a.in, beg, end, j, a.pd, a.inv);
                 s = mulmod(o, a.scale[r], a.p, a.inv);
                 a.out[(size_t)r * B + j] = s;
             }

@@ -154,43 +154,6 @@ __global__ __launch_bounds__(256) void k_pointwise(const uint32_t *A, const uint
     }
 }
 
-// Direct (schoolbook) truncated product for short polynomials:
-// C[i][j][k - lo] = sum_r sum_s A[i][r][s] B[r][j][k - s], k in [lo, hi),
-// A [rows][inner][la] (stride sa), B [inner][cols][lb] (stride sb, offset ob).
-__global__ __launch_bounds__(256) void k_pmul_school(const uint32_t *A, uint64_t sa, uint32_t la, const uint32_t *B,
-                                                     uint64_t sb, uint32_t ob, uint32_t lb, uint32_t *C, uint64_t sc,
-                                                     uint32_t rows, uint32_t inner, uint32_t cols, uint32_t lo,
-                                                     uint32_t hi, uint32_t p, double pd, double inv) {
-    const uint32_t outlen = hi - lo;
-    const uint64_t total = (uint64_t)rows * cols * outlen;
-    for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t k = (uint32_t)(idx % outlen) + lo;
-        uint64_t ij = idx / outlen;
-        uint32_t j = (uint32_t)(ij % cols), i = (uint32_t)(ij / cols);
-        uint32_t s_lo = k >= lb ? k - lb + 1 : 0, s_hi = min(k + 1, la);
-        double hi_acc = 0.0, lo_acc = 0.0;
-        uint32_t terms = 0;
-        for (uint32_t r = 0; r < inner; r++) {
-            const uint32_t *ar = A + ((size_t)i * inner + r) * sa;
-            const uint32_t *br = B + ((size_t)r * cols + j) * sb + ob;
-            for (uint32_t s = s_lo; s < s_hi; s++) {
-                uint32_t a = ar[s], b = br[k - s];
-                double bd = (double)b;
-                hi_acc = fma((double)(a >> 16), bd, hi_acc);
-                lo_acc = fma((double)(a & 0xFFFFu), bd, lo_acc);
-                if (++terms == 64) {
-                    hi_acc = fold(hi_acc, pd, inv);
-                    lo_acc = fold(lo_acc, pd, inv);
-                    terms = 0;
-                }
-            }
-        }
-        double v = fma(fold(hi_acc, pd, inv), 65536.0, fold(lo_acc, pd, inv));
-        C[(size_t)(i * cols + j) * sc + (k - lo)] = (uint32_t)fold(v, pd, inv);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // host side
 
@@ -255,18 +218,6 @@ rr_status pointwise(const uint32_t *A, const uint32_t *B, uint32_t *C, uint32_t 
     dim3 grid((N + 63) / 64, (rows + 3) / 4, (cols + JT - 1) / JT);
     hipLaunchKernelGGL(k_pointwise<JT>, grid, dim3(256), 0, st, A, B, C, rows, inner, cols, N, scale, p, mp.pd,
                        mp.inv);
-    RR_HIP(hipGetLastError());
-    return RR_OK;
-}
-
-rr_status pmul_school(const uint32_t *A, uint64_t sa, uint32_t la, const uint32_t *B, uint64_t sb, uint32_t ob,
-                      uint32_t lb, uint32_t *C, uint64_t sc, uint32_t rows, uint32_t inner, uint32_t cols,
-                      uint32_t lo, uint32_t hi, uint32_t p, hipStream_t st) {
-    uint64_t total = (uint64_t)rows * cols * (hi - lo);
-    uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 8192);
-    ModP mp = ModP::make(p);
-    hipLaunchKernelGGL(k_pmul_school, dim3(std::max(blocks, 1u)), dim3(256), 0, st, A, sa, la, B, sb, ob, lb, C, sc,
-                       rows, inner, cols, lo, hi, p, mp.pd, mp.inv);
     RR_HIP(hipGetLastError());
     return RR_OK;
 }

@@ -26,10 +26,4 @@ rr_status ntt_inverse(const NttTables &T, const uint32_t *src, uint32_t count, u
 // C[i][j][l] = scale * sum_r A[i][r][l] B[r][j][l], spectra of length N (mod 2013265921).
 rr_status pointwise(const uint32_t *A, const uint32_t *B, uint32_t *C, uint32_t rows, uint32_t inner,
                     uint32_t cols, uint32_t N, uint32_t scale, hipStream_t st);
-// Direct truncated product mod any p < 2^31:
-// C[i][j][k - lo] (stride sc) = sum_r sum_s A[i][r][s] B[r][j][ob + k - s], k in [lo, hi).
-rr_status pmul_school(const uint32_t *A, uint64_t sa, uint32_t la, const uint32_t *B, uint64_t sb, uint32_t ob,
-                      uint32_t lb, uint32_t *C, uint64_t sc, uint32_t rows, uint32_t inner, uint32_t cols,
-                      uint32_t lo, uint32_t hi, uint32_t p, hipStream_t st);
-
 }  // namespace rr

@@ -27,6 +27,7 @@
 // cyclic transform of d+1 points, not 1.5d).
 #include "common.hpp"
 #include "ntt.hpp"
+#include "pmbasis.hpp"
 #include <vector>
 #include <algorithm>
 #include <cstring>
@@ -36,9 +37,8 @@ namespace rr {
 // ---------------------------------------------------------------------------
 // leaf kernel
 // G0(i, j) = G[(i * n + j) * gstride + goff]; M [m][m][2] (stride 2); delta [m] in/out.
-__global__ __launch_bounds__(256) void k_mbasis(const uint32_t *G, uint64_t gstride, uint64_t goff, uint32_t m,
-                                                uint32_t n, int64_t *delta, uint32_t *M, uint32_t p, double pd,
-                                                double inv) {
+__device__ void mbasis_wg(const uint32_t *G, uint64_t gstride, uint64_t goff, uint32_t m, uint32_t n,
+                          int64_t *delta, uint32_t *M, uint32_t p, double pd, double inv) {
     extern __shared__ uint32_t sm[];
     uint32_t *R = sm;                  // [m][n] reduced rows
     uint32_t *U = R + m * n;           // [m][n] coefficients on pivot rows
@@ -152,91 +152,63 @@ __global__ __launch_bounds__(256) void k_mbasis(const uint32_t *G, uint64_t gstr
     }
 }
 
+constexpr int LEAF_PACK = 64;
+struct LeafPack {
+    LeafDesc d[LEAF_PACK];
+};
+
+// one workgroup per leaf
+__global__ __launch_bounds__(256) void k_mbasis_batch(LeafPack pk, uint32_t p, double pd, double inv) {
+    const LeafDesc &D = pk.d[blockIdx.x];
+    mbasis_wg(D.G, D.gs, 0, D.m, D.n, D.delta, D.M, p, pd, inv);
+}
+
+static size_t leaf_lds_bytes(uint32_t m, uint32_t n) {
+    return (size_t)(2 * m * n + 4 * m + n + 6) * 4 + (size_t)m * 8 + 16;
+}
+
+constexpr int MUL_PACK = 48;
+struct MulPack {
+    MulDesc d[MUL_PACK];
+};
+
+// direct products, blockIdx.y = descriptor; thread per output coefficient
+__global__ __launch_bounds__(256) void k_pmul_school_batch(MulPack pk, uint32_t p, double pd, double inv) {
+    const MulDesc &D = pk.d[blockIdx.y];
+    const uint32_t outlen = D.hi - D.lo;
+    const uint64_t total = (uint64_t)D.rows * D.cols * outlen;
+    for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t k = (uint32_t)(idx % outlen) + D.lo;
+        uint64_t ij = idx / outlen;
+        uint32_t j = (uint32_t)(ij % D.cols), i = (uint32_t)(ij / D.cols);
+        uint32_t s_lo = k >= D.lb ? k - D.lb + 1 : 0, s_hi = min(k + 1, D.la);
+        double hacc = 0.0, lacc = 0.0;
+        uint32_t terms = 0;
+        for (uint32_t r = 0; r < D.inner; r++) {
+            const uint32_t *ar = D.A + ((size_t)i * D.inner + r) * D.la;
+            const uint32_t *br = D.B + ((size_t)r * D.cols + j) * D.sb + D.ob;
+            for (uint32_t s = s_lo; s < s_hi; s++) {
+                uint32_t a = ar[s];
+                double bd = (double)br[k - s];
+                hacc = fma((double)(a >> 16), bd, hacc);
+                lacc = fma((double)(a & 0xFFFFu), bd, lacc);
+                if (++terms == 64) {
+                    hacc = fold(hacc, pd, inv);
+                    lacc = fold(lacc, pd, inv);
+                    terms = 0;
+                }
+            }
+        }
+        double v = fma(fold(hacc, pd, inv), 65536.0, fold(lacc, pd, inv));
+        D.C[(size_t)(i * D.cols + j) * outlen + (k - D.lo)] = (uint32_t)fold(v, pd, inv);
+    }
+}
+
 // ---------------------------------------------------------------------------
-struct Pool {  // bump allocator over one device buffer
-    DevBuf buf;
-    size_t top = 0, cap = 0;
-    rr_status init(size_t words) {
-        cap = words;
-        top = 0;
-        return buf.alloc(words * 4);
-    }
-    uint32_t *get(size_t words) {
-        words = (words + 63) & ~size_t(63);
-        if (top + words > cap) return nullptr;
-        uint32_t *p = buf.as<uint32_t>() + top;
-        top += words;
-        return p;
-    }
-};
-
-struct PmCtx {
-    uint32_t p, b, m;  // m = 2b
-    hipStream_t st;
-    NttTables tables;
-    bool use_ntt;
-    Pool pool;
-    DevBuf fa, fb, fc;  // spectra workspace
-    size_t fcap = 0;
-    DevBuf delta;
-    size_t lds_bytes;
-};
-
-// C [rows][cols][hi-lo] (contiguous) = window [lo, hi) of A [rows][inner][la] * B [inner][cols][ob .. ob+lb)
-static rr_status pmat_mul(PmCtx &c, const uint32_t *A, uint32_t la, const uint32_t *B, uint64_t sb, uint32_t ob,
-                          uint32_t lb, uint32_t *C, uint32_t rows, uint32_t inner, uint32_t cols, uint32_t lo,
-                          uint32_t hi) {
-    uint64_t school_cost = (uint64_t)std::min(la, hi) * std::min(lb, hi);
-    if (!c.use_ntt || school_cost <= 256) {
-        return pmul_school(A, la, la, B, sb, ob, lb, C, hi - lo, rows, inner, cols, lo, hi, c.p, c.st);
-    }
-    uint32_t need = std::max<uint32_t>(hi, la + lb - 1 - lo);
-    uint32_t logn = 1;
-    while ((1u << logn) < need) logn++;
-    RR_CHECK(logn <= NTT_MAX_LOGN, RR_EINVAL, "product needs a 2^%u-point transform (max 2^%u)", logn,
-             NTT_MAX_LOGN);
-    const uint32_t N = 1u << logn;
-    size_t na = (size_t)rows * inner * N, nb = (size_t)inner * cols * N, nc = (size_t)rows * cols * N;
-    RR_CHECK(na <= c.fcap && nb <= c.fcap && nc <= c.fcap, RR_ENOMEM, "spectrum workspace too small");
-    rr_status s;
-    if ((s = ntt_forward(c.tables, A, la, 0, la, c.fa.as<uint32_t>(), rows * inner, logn, c.st)) != RR_OK) return s;
-    if ((s = ntt_forward(c.tables, B, sb, ob, lb, c.fb.as<uint32_t>(), inner * cols, logn, c.st)) != RR_OK) return s;
-    uint32_t ninv = h_invmod(N, NTT_PRIME);
-    if ((s = pointwise(c.fa.as<uint32_t>(), c.fb.as<uint32_t>(), c.fc.as<uint32_t>(), rows, inner, cols, N, ninv,
-                       c.st)) != RR_OK)
-        return s;
-    return ntt_inverse(c.tables, c.fc.as<uint32_t>(), rows * cols, logn, lo, hi - lo, C, hi - lo, c.st);
-}
-
-// M [m][m][d+1] = order-d basis of G (m x b polys, stride gs, coefficients 0..d valid).
-static rr_status pm_rec(PmCtx &c, const uint32_t *G, uint64_t gs, uint64_t d, uint32_t *M) {
-    const uint32_t m = c.m, b = c.b;
-    if (d == 1) {
-        hipLaunchKernelGGL(k_mbasis, dim3(1), dim3(256), c.lds_bytes, c.st, G, gs, (uint64_t)0, m, b,
-                           c.delta.as<int64_t>(), M, c.p, (double)c.p, 1.0 / (double)c.p);
-        RR_HIP(hipGetLastError());
-        return RR_OK;
-    }
-    const uint64_t d1 = d - d / 2, d2 = d / 2;
-    size_t mark = c.pool.top;
-    uint32_t *M1 = c.pool.get((size_t)m * m * (d1 + 1));
-    uint32_t *GG = c.pool.get((size_t)m * b * (d2 + 1));
-    uint32_t *M2 = c.pool.get((size_t)m * m * (d2 + 1));
-    RR_CHECK(M1 && GG && M2, RR_ENOMEM, "PM-basis workspace exhausted");
-    rr_status s;
-    if ((s = pm_rec(c, G, gs, d1, M1)) != RR_OK) return s;
-    // GG = coefficients d1..d of M1 * G[0..d]   (src/sigma_basis.rs:205-209)
-    if ((s = pmat_mul(c, M1, (uint32_t)(d1 + 1), G, gs, 0, (uint32_t)(d + 1), GG, m, m, b, (uint32_t)d1,
-                      (uint32_t)(d + 1))) != RR_OK)
-        return s;
-    if ((s = pm_rec(c, GG, d2 + 1, d2, M2)) != RR_OK) return s;
-    // M = M2 * M1   (src/sigma_basis.rs:213)
-    if ((s = pmat_mul(c, M2, (uint32_t)(d2 + 1), M1, d1 + 1, 0, (uint32_t)(d1 + 1), M, m, m, m, 0,
-                      (uint32_t)(d + 1))) != RR_OK)
-        return s;
-    c.pool.top = mark;
-    return RR_OK;
-}
+// The PM-basis of one matrix as an explicit state machine over the
+// recursion of src/sigma_basis.rs:190-219, so that the next device operation
+// of many matrices can be launched together.
 
 static uint64_t rec_words(uint64_t d, uint32_t m, uint32_t b) {
     if (d <= 1) return 0;
@@ -246,36 +218,251 @@ static uint64_t rec_words(uint64_t d, uint32_t m, uint32_t b) {
     return here + std::max(rec_words(d1, m, b), rec_words(d2, m, b));
 }
 
+struct Op {
+    int kind;  // 0 leaf, 1 product
+    LeafDesc leaf;
+    MulDesc mul;
+};
+
+// Advances the job to its next device operation; returns false when finished.
+static bool next_op(PmJob &J, Op &op, rr_status &err) {
+    err = RR_OK;
+    while (!J.stack.empty()) {
+        size_t fi = J.stack.size() - 1;
+        Frame &F = J.stack[fi];
+        if (F.d == 1) {
+            if (F.stage == 0) {
+                F.stage = 1;
+                op.kind = 0;
+                op.leaf = LeafDesc{F.G, F.gs, J.delta, F.M, J.m, J.b};
+                return true;
+            }
+            J.stack.pop_back();
+            continue;
+        }
+        const uint64_t d1 = F.d - F.d / 2, d2 = F.d / 2;
+        switch (F.stage) {
+            case 0: {
+                F.mark = J.pool.top;
+                F.M1 = J.pool.get((size_t)J.m * J.m * (d1 + 1));
+                F.GG = J.pool.get((size_t)J.m * J.b * (d2 + 1));
+                F.M2 = J.pool.get((size_t)J.m * J.m * (d2 + 1));
+                if (!F.M1 || !F.GG || !F.M2) {
+                    set_error("PM-basis workspace exhausted");
+                    err = RR_ENOMEM;
+                    return false;
+                }
+                F.stage = 1;
+                Frame child{F.G, F.gs, d1, F.M1, 0, nullptr, nullptr, nullptr, 0};
+                J.stack.push_back(child);
+                continue;
+            }
+            case 1:  // GG = coefficients d1..d of M1 * G[0..d]   (src/sigma_basis.rs:205-209)
+                F.stage = 2;
+                op.kind = 1;
+                op.mul = MulDesc{F.M1, F.G, F.GG, F.gs, (uint32_t)(d1 + 1), (uint32_t)(F.d + 1), 0,
+                                 (uint32_t)d1, (uint32_t)(F.d + 1), J.m, J.m, J.b};
+                return true;
+            case 2: {
+                F.stage = 3;
+                Frame child{F.GG, d2 + 1, d2, F.M2, 0, nullptr, nullptr, nullptr, 0};
+                J.stack.push_back(child);
+                continue;
+            }
+            case 3:  // M = M2 * M1   (src/sigma_basis.rs:213)
+                F.stage = 4;
+                op.kind = 1;
+                op.mul = MulDesc{F.M2, F.M1, F.M, d1 + 1, (uint32_t)(d2 + 1), (uint32_t)(d1 + 1), 0, 0,
+                                 (uint32_t)(F.d + 1), J.m, J.m, J.m};
+                return true;
+            default:
+                J.pool.top = F.mark;
+                J.stack.pop_back();
+                continue;
+        }
+    }
+    J.done = true;
+    return false;
+}
+
+static uint32_t ntt_logn_for(const MulDesc &D) {
+    uint32_t need = std::max<uint32_t>(D.hi, D.la + D.lb - 1 - D.lo), logn = 1;
+    while ((1u << logn) < need) logn++;
+    return logn;
+}
+
+static bool use_school(const MulDesc &D, uint32_t p) {
+    if (p != NTT_PRIME) return true;
+    uint64_t cost = (uint64_t)std::min(D.la, D.hi) * std::min(D.lb, D.hi);
+    return cost <= 256;
+}
+
+rr_status PmScheduler::ensure_scratch(size_t words) {
+    if (words <= scratch_words) return RR_OK;
+    RR_HIP(hipStreamSynchronize(st));
+    scratch.~DevBuf();
+    new (&scratch) DevBuf();
+    size_t w = std::max(words, scratch_words * 2);
+    rr_status s = scratch.alloc(w * 4);
+    if (s != RR_OK) return s;
+    scratch_words = w;
+    return RR_OK;
+}
+
+rr_status PmScheduler::init(uint32_t p_, hipStream_t st_, uint32_t max_logn, uint32_t max_m, uint32_t max_n) {
+    p = p_;
+    st = st_;
+    rr_status s;
+    if (p == NTT_PRIME && (s = tables.init(std::max<uint32_t>(2, std::min(max_logn, NTT_MAX_LOGN)), st)) != RR_OK)
+        return s;
+    lds = leaf_lds_bytes(max_m, max_n);
+    if (lds > 65536)
+        RR_HIP(hipFuncSetAttribute((const void *)k_mbasis_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+    return RR_OK;
+}
+
+rr_status PmScheduler::add(PmJob *J) {
+    J->m = 2 * J->b;
+    rr_status s;
+    if ((s = J->pool.init(rec_words(J->d, J->m, J->b) + 1024)) != RR_OK) return s;
+    RR_HIP(hipMemsetAsync(J->delta, 0, (size_t)J->m * 8, st));
+    J->stack.clear();
+    J->stack.push_back(Frame{J->G, J->d + 1, J->d, J->M, 0, nullptr, nullptr, nullptr, 0});
+    J->done = false;
+    return RR_OK;
+}
+
+// One round: the next operation of every unfinished job, launched together.
+rr_status PmScheduler::round(std::vector<PmJob *> &jobs, bool *launched) {
+    const ModP mp = ModP::make(p);
+    std::vector<LeafDesc> leaves;
+    std::vector<MulDesc> school;
+    std::vector<std::pair<uint32_t, MulDesc>> ntt;
+    *launched = false;
+    for (auto *J : jobs) {
+        if (J->done) continue;
+        Op op;
+        rr_status e;
+        if (!next_op(*J, op, e)) {
+            if (e != RR_OK) return e;
+            continue;
+        }
+        if (op.kind == 0) leaves.push_back(op.leaf);
+        else if (use_school(op.mul, p)) school.push_back(op.mul);
+        else ntt.push_back({ntt_logn_for(op.mul), op.mul});
+    }
+    if (leaves.empty() && school.empty() && ntt.empty()) return RR_OK;
+    *launched = true;
+    rounds++;
+    for (size_t i = 0; i < leaves.size(); i += LEAF_PACK) {
+        LeafPack pk;
+        uint32_t cnt = (uint32_t)std::min<size_t>(LEAF_PACK, leaves.size() - i);
+        for (uint32_t k = 0; k < cnt; k++) pk.d[k] = leaves[i + k];
+        hipLaunchKernelGGL(k_mbasis_batch, dim3(cnt), dim3(256), lds, st, pk, p, mp.pd, mp.inv);
+        RR_HIP(hipGetLastError());
+        launches++;
+    }
+    for (size_t i = 0; i < school.size(); i += MUL_PACK) {
+        MulPack pk;
+        uint32_t cnt = (uint32_t)std::min<size_t>(MUL_PACK, school.size() - i);
+        uint64_t maxout = 0;
+        for (uint32_t k = 0; k < cnt; k++) {
+            pk.d[k] = school[i + k];
+            const MulDesc &D = pk.d[k];
+            maxout = std::max<uint64_t>(maxout, (uint64_t)D.rows * D.cols * (D.hi - D.lo));
+        }
+        uint32_t bx = (uint32_t)std::min<uint64_t>((maxout + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_pmul_school_batch, dim3(std::max(bx, 1u), cnt), dim3(256), 0, st, pk, p, mp.pd, mp.inv);
+        RR_HIP(hipGetLastError());
+        launches++;
+    }
+    if (!ntt.empty()) return run_ntt_products(ntt);
+    return RR_OK;
+}
+
+rr_status PmScheduler::run(std::vector<PmJob *> &jobs) {
+    uint32_t max_logn = 2, max_m = 0, max_n = 0;
+    for (auto *J : jobs) {
+        uint32_t l = 1;
+        while ((1ull << l) < J->d + 1) l++;
+        max_logn = std::max(max_logn, l);
+        max_m = std::max(max_m, 2 * J->b);
+        max_n = std::max(max_n, J->b);
+    }
+    rr_status s;
+    if ((s = init(p, st, max_logn, max_m, max_n)) != RR_OK) return s;
+    for (auto *J : jobs)
+        if ((s = add(J)) != RR_OK) return s;
+    for (;;) {
+        bool launched;
+        if ((s = round(jobs, &launched)) != RR_OK) return s;
+        if (!launched) break;
+    }
+    return RR_OK;
+}
+
+// Each NTT product: forward transforms of both operands into round scratch,
+// pointwise products (with 1/N), inverse transforms into the output window.
+rr_status PmScheduler::run_ntt_products(std::vector<std::pair<uint32_t, MulDesc>> &ops) {
+        size_t words = 0;
+        for (auto &o : ops) {
+            size_t N = (size_t)1 << o.first;
+            const MulDesc &D = o.second;
+            words += (size_t)(D.rows * D.inner + D.inner * D.cols + D.rows * D.cols) * N + 192;
+        }
+        rr_status s;
+        if ((s = ensure_scratch(words)) != RR_OK) return s;
+        size_t off = 0;
+        for (auto &o : ops) {
+            const uint32_t logn = o.first, N = 1u << logn;
+            const MulDesc &D = o.second;
+            uint32_t *fa = scratch.as<uint32_t>() + off;
+            off += ((size_t)D.rows * D.inner * N + 63) & ~size_t(63);
+            uint32_t *fb = scratch.as<uint32_t>() + off;
+            off += ((size_t)D.inner * D.cols * N + 63) & ~size_t(63);
+            uint32_t *fc = scratch.as<uint32_t>() + off;
+            off += ((size_t)D.rows * D.cols * N + 63) & ~size_t(63);
+            RR_CHECK(logn <= tables.logn, RR_EINVAL, "transform 2^%u above table 2^%u", logn, tables.logn);
+            if ((s = ntt_forward(tables, D.A, D.la, 0, D.la, fa, D.rows * D.inner, logn, st)) != RR_OK) return s;
+            if ((s = ntt_forward(tables, D.B, D.sb, D.ob, D.lb, fb, D.inner * D.cols, logn, st)) != RR_OK) return s;
+            if ((s = pointwise(fa, fb, fc, D.rows, D.inner, D.cols, N, h_invmod(N, NTT_PRIME), st)) != RR_OK) return s;
+            if ((s = ntt_inverse(tables, fc, D.rows * D.cols, logn, D.lo, D.hi - D.lo, D.C, D.hi - D.lo, st)) != RR_OK)
+                return s;
+            launches += 4;
+        }
+        return RR_OK;
+}
+
+// Device PM-basis for several matrices at once.  For each job: G [m][b][d+1]
+// on the device; writes M [m][m][d+1] (device) and delta [m] (device).
+rr_status pm_basis_batch(uint32_t p, std::vector<PmJob *> &jobs, hipStream_t st, uint64_t *rounds,
+                         uint64_t *launches) {
+    PmScheduler S;
+    S.p = p;
+    S.st = st;
+    rr_status s;
+    if ((s = S.run(jobs)) != RR_OK) return s;
+    if (rounds) *rounds = S.rounds;
+    if (launches) *launches = S.launches;
+    return RR_OK;
+}
+
 // Device PM-basis from a device-resident G [m][b][d+1]; writes M [m][m][d+1] (device) and delta (host).
 rr_status pm_basis_device(uint32_t p, uint32_t b, uint64_t d, const uint32_t *G, uint32_t *M, int64_t *delta_host,
                           hipStream_t st) {
-    PmCtx c;
-    c.p = p;
-    c.b = b;
-    c.m = 2 * b;
-    c.st = st;
-    c.use_ntt = (p == NTT_PRIME);
-    const uint32_t m = c.m;
-    c.lds_bytes = (size_t)(2 * m * b + 4 * m + b + 6) * 4 + (size_t)m * 8 + 16;
-    if (c.lds_bytes > 65536)
-        RR_HIP(hipFuncSetAttribute((const void *)k_mbasis, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)c.lds_bytes));
+    PmJob J;
+    J.b = b;
+    J.d = d;
+    J.G = G;
+    J.M = M;
+    DevBuf dl;
     rr_status s;
-    if ((s = c.delta.alloc(m * 8)) != RR_OK) return s;
-    RR_HIP(hipMemsetAsync(c.delta.ptr, 0, m * 8, st));
-    if ((s = c.pool.init(rec_words(d, m, b) + 1024)) != RR_OK) return s;
-    if (c.use_ntt) {
-        uint32_t logn = 1;
-        while ((1ull << logn) < d + 1) logn++;
-        if (logn > NTT_MAX_LOGN) logn = NTT_MAX_LOGN;
-        if ((s = c.tables.init(std::max<uint32_t>(logn, 2), st)) != RR_OK) return s;
-        c.fcap = (size_t)m * m << logn;
-        if ((s = c.fa.alloc(c.fcap * 4)) != RR_OK) return s;
-        if ((s = c.fb.alloc(c.fcap * 4)) != RR_OK) return s;
-        if ((s = c.fc.alloc(c.fcap * 4)) != RR_OK) return s;
-    }
-    if ((s = pm_rec(c, G, d + 1, d, M)) != RR_OK) return s;
-    RR_HIP(hipMemcpyAsync(delta_host, c.delta.ptr, m * 8, hipMemcpyDeviceToHost, st));
+    if ((s = dl.alloc((size_t)2 * b * 8)) != RR_OK) return s;
+    J.delta = dl.as<int64_t>();
+    std::vector<PmJob *> jobs{&J};
+    if ((s = pm_basis_batch(p, jobs, st, nullptr, nullptr)) != RR_OK) return s;
+    RR_HIP(hipMemcpyAsync(delta_host, dl.ptr, (size_t)2 * b * 8, hipMemcpyDeviceToHost, st));
     RR_HIP(hipStreamSynchronize(st));
     return RR_OK;
 }
@@ -480,12 +667,6 @@ rr_status rr_poly_mat_mul(const uint32_t *A, uint64_t la, const uint32_t *B, uin
         hipStream_t s;
         ~S() { (void)hipStreamDestroy(s); }
     } guard{st};
-    PmCtx c;
-    c.p = p;
-    c.b = 0;
-    c.m = 0;
-    c.st = st;
-    c.use_ntt = (p == NTT_PRIME);
     DevBuf dA, dB, dC;
     rr_status s;
     size_t sa = (size_t)rows * inner * la, sbn = (size_t)inner * cols * lb, sc = (size_t)rows * cols * (hi - lo);
@@ -494,19 +675,26 @@ rr_status rr_poly_mat_mul(const uint32_t *A, uint64_t la, const uint32_t *B, uin
     if ((s = dC.alloc(sc * 4)) != RR_OK) return s;
     RR_HIP(hipMemcpyAsync(dA.ptr, A, sa * 4, hipMemcpyHostToDevice, st));
     RR_HIP(hipMemcpyAsync(dB.ptr, B, sbn * 4, hipMemcpyHostToDevice, st));
-    if (c.use_ntt) {
-        uint32_t need = (uint32_t)std::max<uint64_t>(hi, la + lb - 1 - lo), logn = 1;
-        while ((1u << logn) < need) logn++;
+    MulDesc D{dA.as<uint32_t>(), dB.as<uint32_t>(), dC.as<uint32_t>(), lb, (uint32_t)la, (uint32_t)lb, 0,
+              (uint32_t)lo, (uint32_t)hi, rows, inner, cols};
+    PmScheduler S;
+    S.p = p;
+    S.st = st;
+    const ModP mp = ModP::make(p);
+    if (use_school(D, p)) {
+        MulPack pk;
+        pk.d[0] = D;
+        uint64_t total = (uint64_t)rows * cols * (hi - lo);
+        uint32_t bx = (uint32_t)std::min<uint64_t>((total + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_pmul_school_batch, dim3(std::max(bx, 1u), 1), dim3(256), 0, st, pk, p, mp.pd, mp.inv);
+        RR_HIP(hipGetLastError());
+    } else {
+        uint32_t logn = ntt_logn_for(D);
         RR_CHECK(logn <= NTT_MAX_LOGN, RR_EINVAL, "product too long");
-        if ((s = c.tables.init(logn, st)) != RR_OK) return s;
-        c.fcap = (size_t)std::max({rows * inner, inner * cols, rows * cols}) << logn;
-        if ((s = c.fa.alloc(c.fcap * 4)) != RR_OK) return s;
-        if ((s = c.fb.alloc(c.fcap * 4)) != RR_OK) return s;
-        if ((s = c.fc.alloc(c.fcap * 4)) != RR_OK) return s;
+        if ((s = S.tables.init(logn, st)) != RR_OK) return s;
+        std::vector<std::pair<uint32_t, MulDesc>> ops{{logn, D}};
+        if ((s = S.run_ntt_products(ops)) != RR_OK) return s;
     }
-    if ((s = pmat_mul(c, dA.as<uint32_t>(), (uint32_t)la, dB.as<uint32_t>(), lb, 0, (uint32_t)lb, dC.as<uint32_t>(),
-                      rows, inner, cols, (uint32_t)lo, (uint32_t)hi)) != RR_OK)
-        return s;
     RR_HIP(hipMemcpyAsync(C, dC.ptr, sc * 4, hipMemcpyDeviceToHost, st));
     RR_HIP(hipStreamSynchronize(st));
     return RR_OK;

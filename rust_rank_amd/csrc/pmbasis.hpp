@@ -1,0 +1,85 @@
+// pmbasis.hpp -- device PM-basis: jobs, the round scheduler and rank extraction.
+#pragma once
+#include "common.hpp"
+#include "ntt.hpp"
+#include <vector>
+#include <utility>
+
+namespace rr {
+
+struct LeafDesc {
+    const uint32_t *G;  // constant term G(i, j) at G[(i * n + j) * gs]
+    uint64_t gs;
+    int64_t *delta;
+    uint32_t *M;  // [m][m][2]
+    uint32_t m, n;
+};
+// C [rows][cols][hi-lo] = window [lo, hi) of A [rows][inner][la] * B [inner][cols][ob .. ob+lb) (stride sb)
+struct MulDesc {
+    const uint32_t *A, *B;
+    uint32_t *C;
+    uint64_t sb;
+    uint32_t la, lb, ob, lo, hi, rows, inner, cols;
+};
+struct Pool {  // bump allocator over one device buffer
+    DevBuf buf;
+    size_t top = 0, cap = 0;
+    rr_status init(size_t words) {
+        cap = words;
+        top = 0;
+        return buf.alloc(words * 4);
+    }
+    uint32_t *get(size_t words) {
+        words = (words + 63) & ~size_t(63);
+        if (top + words > cap) return nullptr;
+        uint32_t *ptr = buf.as<uint32_t>() + top;
+        top += words;
+        return ptr;
+    }
+};
+
+struct Frame {
+    const uint32_t *G;
+    uint64_t gs, d;
+    uint32_t *M;
+    int stage;
+    uint32_t *M1, *GG, *M2;
+    size_t mark;
+};
+
+struct PmJob {
+    uint32_t b = 0, m = 0;
+    uint64_t d = 0;
+    const uint32_t *G = nullptr;  // [m][b][d+1]
+    uint32_t *M = nullptr;        // [m][m][d+1]
+    int64_t *delta = nullptr;     // device [m]
+    Pool pool;
+    std::vector<Frame> stack;
+    bool done = false;
+};
+
+struct PmScheduler {
+    uint32_t p = 0;
+    hipStream_t st = nullptr;
+    NttTables tables;
+    DevBuf scratch;
+    size_t scratch_words = 0, lds = 0;
+    uint64_t rounds = 0, launches = 0;
+
+    rr_status ensure_scratch(size_t words);
+    rr_status init(uint32_t p, hipStream_t st, uint32_t max_logn, uint32_t max_m, uint32_t max_n);
+    rr_status add(PmJob *J);                                          // job joins; its delta is zeroed
+    rr_status round(std::vector<PmJob *> &jobs, bool *launched);     // next operation of every job
+    rr_status run(std::vector<PmJob *> &jobs);                        // init + add + rounds to completion
+    rr_status run_ntt_products(std::vector<std::pair<uint32_t, MulDesc>> &ops);
+};
+
+rr_status pm_basis_batch(uint32_t p, std::vector<PmJob *> &jobs, hipStream_t st, uint64_t *rounds,
+                         uint64_t *launches);
+rr_status pm_basis_device(uint32_t p, uint32_t b, uint64_t d, const uint32_t *G, uint32_t *M, int64_t *delta_host,
+                          hipStream_t st);
+rr_status build_G_device(const uint32_t *seq_dev, uint32_t b, uint64_t len, uint32_t *G, hipStream_t st);
+int64_t rank_from_basis_host(const uint32_t *basis, const int64_t *delta, uint32_t b, uint64_t len, uint32_t p,
+                             rr_status *st);
+
+}  // namespace rr

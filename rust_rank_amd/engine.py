@@ -39,6 +39,27 @@ class KrylovStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("ms_setup", C.c_double), ("ms_krylov", C.c_double), ("ms_pm", C.c_double), ("ms_total", C.c_double),
+                ("ticks", C.c_uint64), ("pm_rounds", C.c_uint64), ("pm_launches", C.c_uint64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class Result(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("rank", C.c_int64), ("status", C.c_int32), ("latency_ms", C.c_double),
+                ("queue_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("queued", C.c_uint64), ("used_bytes", C.c_uint64), ("completed", C.c_uint64),
+                ("failed", C.c_uint64), ("ticks", C.c_uint64), ("pm_rounds", C.c_uint64), ("max_active", C.c_uint64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -72,6 +93,18 @@ def lib():
     L.rr_rank.argtypes = [C.POINTER(_Csr), C.c_uint32, C.c_uint32, _u32p, _u32p, _u32p, C.c_uint64, C.c_uint64,
                           C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     L.rr_rank.restype = C.c_int
+    L.rr_batch_rank.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, _u32p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_void_p, C.c_void_p, _i64p, C.POINTER(BatchStats)]
+    L.rr_batch_rank.restype = C.c_int
+    L.rr_server_create.argtypes = [C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]
+    L.rr_server_submit.argtypes = [C.c_void_p, C.POINTER(_Csr), C.c_uint32, _u32p, _u32p, _u32p, C.c_uint64,
+                                   C.c_uint64, C.POINTER(C.c_uint64)]
+    L.rr_server_poll.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_int]
+    L.rr_server_stats_get.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
+    L.rr_server_destroy.argtypes = [C.c_void_p]
+    L.rr_server_destroy.restype = None
+    for name in ("rr_server_create", "rr_server_submit", "rr_server_poll", "rr_server_stats_get"):
+        getattr(L, name).restype = C.c_int
     for name in ("rr_krylov_create", "rr_krylov_run", "rr_krylov_state", "rr_krylov_set_profiling",
                  "rr_krylov_stats_get", "rr_pm_basis", "rr_rank_from_basis", "rr_ntt", "rr_poly_mat_mul"):
         getattr(L, name).restype = C.c_int
@@ -202,3 +235,74 @@ def rank(A, p: int, b: int, row_precond, col_precond, V, seq_len: int = 0, order
                          np.ascontiguousarray(col_precond, dtype=np.uint32), np.ascontiguousarray(V, dtype=np.uint32),
                          seq_len, order, C.byref(r), C.byref(ms)))
     return r.value, ms.value
+
+
+def batch_rank(mats, p: int, bs, drs, dcs, Vs, seq_lens=None, orders=None):
+    """Ranks of many matrices in one batched run (rr_batch_rank).  Returns (ranks, stats)."""
+    n = len(mats)
+    keep = []
+    csrs = (_Csr * n)()
+    for i, A in enumerate(mats):
+        s, k = _csr_struct(A)
+        csrs[i] = s
+        keep.append(k)
+
+    def ptrs(arrs):
+        a = [np.ascontiguousarray(x, dtype=np.uint32) for x in arrs]
+        keep.append(a)
+        return (C.c_void_p * n)(*[x.ctypes.data for x in a])
+
+    b = np.ascontiguousarray(bs, dtype=np.uint32)
+    L = np.ascontiguousarray(seq_lens if seq_lens is not None else np.zeros(n), dtype=np.uint64)
+    d = np.ascontiguousarray(orders if orders is not None else np.zeros(n), dtype=np.uint64)
+    ranks = np.zeros(n, np.int64)
+    st = BatchStats()
+    _check(lib().rr_batch_rank(n, C.cast(csrs, C.c_void_p), p, b, ptrs(drs), ptrs(dcs), ptrs(Vs),
+                               L.ctypes.data, d.ctypes.data, ranks, C.byref(st)))
+    return ranks, st.as_dict()
+
+
+class Server:
+    """Continuous-batching rank server (rr_server_*)."""
+
+    def __init__(self, p: int, hbm_budget_bytes: int):
+        h = C.c_void_p()
+        _check(lib().rr_server_create(p, hbm_budget_bytes, C.byref(h)))
+        self._h = h
+
+    def submit(self, A, b: int, row_precond, col_precond, V, seq_len: int = 0, order: int = 0) -> int:
+        s, keep = _csr_struct(A)
+        rid = C.c_uint64()
+        _check(lib().rr_server_submit(self._h, C.byref(s), b, np.ascontiguousarray(row_precond, dtype=np.uint32),
+                                      np.ascontiguousarray(col_precond, dtype=np.uint32),
+                                      np.ascontiguousarray(V, dtype=np.uint32), seq_len, order, C.byref(rid)))
+        return rid.value
+
+    def poll(self, wait_ms: int = 0, cap: int = 256):
+        buf = (Result * cap)()
+        n = C.c_uint32()
+        _check(lib().rr_server_poll(self._h, C.cast(buf, C.c_void_p), cap, C.byref(n), wait_ms))
+        return [{"id": buf[i].id, "rank": buf[i].rank, "status": buf[i].status, "latency_ms": buf[i].latency_ms,
+                 "queue_ms": buf[i].queue_ms} for i in range(n.value)]
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(lib().rr_server_stats_get(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rr_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

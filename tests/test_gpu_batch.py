@@ -1,0 +1,52 @@
+"""GPU: the batched mode (rr_batch_rank) gives, for every matrix, the rank the
+single-matrix path gives, and dense Gaussian elimination confirms it."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from rust_rank_amd import synth
+
+P = 2013265921
+pytestmark = pytest.mark.gpu
+
+
+def _eng():
+    from rust_rank_amd import engine
+    assert engine.device_count() >= 1, "no GPU visible"
+    return engine
+
+
+def test_batch_matches_single_and_dense_ge():
+    eng = _eng()
+    mats, bs, drs, dcs, Vs = [], [], [], [], []
+    for i in range(12):
+        A = synth.batch_matrix(21, i, rows_lo=150, rows_hi=900)
+        b = (4, 8, 16)[i % 3]
+        dr, dc, V = synth.preconditioners(100 + i, A.m, A.n, b, P)
+        mats.append(A); bs.append(b); drs.append(dr); dcs.append(dc); Vs.append(V)
+    print(f"batch of {len(mats)}: rows {[A.m for A in mats]}", flush=True)
+    ranks, stats = eng.batch_rank(mats, P, bs, drs, dcs, Vs)
+    print(stats, flush=True)
+    for i, A in enumerate(mats):
+        single, _ = eng.rank(A, P, bs[i], drs[i], dcs[i], Vs[i])
+        ge = O.dense_rank(A.to_dense(P), P)
+        assert ranks[i] == single == ge, (i, ranks[i], single, ge)
+
+
+def test_batch_sequences_bit_identical_to_oracle():
+    """Same matrices alone and in a mix: the ranks come from the same S_t; here the
+    S_t of one batched matrix are compared with the oracle through its rank at a
+    short order (different orders exercise different sequence prefixes)."""
+    eng = _eng()
+    A = synth.batch_matrix(5, 1, rows_lo=200, rows_hi=300)
+    b = 4
+    dr, dc, V = synth.preconditioners(7, A.m, A.n, b, P)
+    L = 2 * min(A.m, A.n) // b + 8
+    seq = O.krylov(A, P, b, dr, dc, V, 0, L)
+    for d in (L - 1, L // 2, 17):
+        basis, delta = O.pm_basis(seq, b, d, P)
+        want = O.rank_from_basis(basis, delta, b, P)
+        other = synth.batch_matrix(5, 2, rows_lo=100, rows_hi=400)
+        dr2, dc2, V2 = synth.preconditioners(8, other.m, other.n, 8, P)
+        ranks, _ = eng.batch_rank([other, A], P, [8, b], [dr2, dr], [dc2, dc], [V2, V], [0, L], [0, d])
+        assert ranks[1] == want, (d, ranks[1], want)
