@@ -51,6 +51,7 @@ struct PassArgs {
     const uint32_t *long_rows, *long_pbeg;  // split rows and their piece ranges
     uint32_t nlong;
     uint32_t *gram;          // this pass's partials [gridDim.x (+ fixup grid)][P]
+    uint32_t fix_words;      // words of the fix-up partials after the main grid's
     uint32_t p;
     double pd, inv;
     uint32_t b;
@@ -71,6 +72,10 @@ __global__ __launch_bounds__(WG) void k_pass(PassArgs a) {
     for (uint32_t pc = gwave * G + slot; pc < a.npieces; pc += nwaves * G)
         a.piece_out[(size_t)pc * B + j] = row_dot<B, PACKED>(a.col, a.val, a.in, a.piece_beg[pc], a.piece_end[pc], j, a.pd, a.inv);
 
+    if (a.nlong == 0 && blockIdx.x == 0) {  // no fix-up kernel this pass: its partials are zero
+        uint32_t *fx = a.gram + (size_t)gridDim.x * (a.b * (a.b + 1) / 2);
+        for (uint32_t i = threadIdx.x; i < a.fix_words; i += blockDim.x) fx[i] = 0;
+    }
     Gram<B> g;
     g.init();
     for (uint32_t base = gwave * G; base < a.nrows; base += nwaves * G) {
@@ -174,6 +179,7 @@ struct rr_krylov {
     rr::ModP mp;
     rr::SpMat A, At;
     rr::DevBuf dr, dc, dc2, dcinv, z, y, xtmp, part, seq;
+    uint64_t seq_cap = 0;  // entries the device sequence buffer holds
     uint32_t grid = 0, fix_grid = 0, P = 0;
     uint64_t t = 0;  // entries produced so far
     hipStream_t stream = nullptr;
@@ -259,6 +265,7 @@ static PassArgs make_args(rr_krylov *c, SpMat &M, const uint32_t *in, uint32_t *
     a.long_pbeg = M.long_pbeg.as<uint32_t>();
     a.nlong = M.nlong;
     a.gram = gram;
+    a.fix_words = c->fix_grid * c->P;
     a.p = c->p;
     a.pd = c->mp.pd;
     a.inv = c->mp.inv;
@@ -437,7 +444,6 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
     RR_HIP(hipGetLastError());
     uint32_t nparts = c->grid + c->fix_grid;
     if ((s = c->part.alloc((size_t)CHUNK * nparts * c->P * 4)) != RR_OK) return s;
-    if ((s = c->seq.alloc((size_t)CHUNK * c->P * 4)) != RR_OK) return s;
     RR_HIP(hipMemsetAsync(c->part.ptr, 0, (size_t)CHUNK * nparts * c->P * 4, c->stream));
     RR_HIP(hipStreamSynchronize(c->stream));
     // algorithmic bytes per pass (int32 columns, u32 values, each array once)
@@ -478,6 +484,14 @@ rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> red_ev;
     RR_HIP(hipEventRecord(e_begin, c->stream));
 
+    if (count > c->seq_cap) {
+        RR_HIP(hipStreamSynchronize(c->stream));
+        c->seq.~DevBuf();
+        new (&c->seq) DevBuf();
+        rr_status sa = c->seq.alloc((size_t)count * P * 4);
+        if (sa != RR_OK) return sa;
+        c->seq_cap = count;
+    }
     uint64_t done = 0;
     while (done < count) {
         uint32_t chunk = (uint32_t)std::min<uint64_t>(CHUNK, count - done);
@@ -509,7 +523,6 @@ rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
                 launch_pass_any(c, a);
                 if (c->profiling) RR_HIP(hipEventRecord(e1, c->stream));
                 if (M.nlong) launch_fixup_any(c, a);
-                else RR_HIP(hipMemsetAsync(gram + (size_t)c->grid * P, 0, (size_t)c->fix_grid * P * 4, c->stream));
                 if (c->profiling) RR_HIP(hipEventRecord(e2, c->stream));
                 k = byA ? 1 : 2;
             }
@@ -532,19 +545,17 @@ rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
         }
         uint32_t nthreads = chunk * P;
         hipLaunchKernelGGL(k_reduce, dim3((nthreads + 255) / 256), dim3(256), 0, c->stream,
-                           c->part.as<uint32_t>(), nparts, P, chunk, c->seq.as<uint32_t>(), c->p);
+                           c->part.as<uint32_t>(), nparts, P, chunk, c->seq.as<uint32_t>() + done * P, c->p);
         RR_HIP(hipGetLastError());
         if (c->profiling) {
             RR_HIP(hipEventRecord(r1, c->stream));
             red_ev.push_back({r0, r1});
         }
-        RR_HIP(hipMemcpyAsync(seq_out + done * P, c->seq.ptr, (size_t)chunk * P * 4, hipMemcpyDeviceToHost,
-                              c->stream));
-        // the next chunk reuses the partial buffer and seq buffer: order is kept by the stream
-        RR_HIP(hipStreamSynchronize(c->stream));
+        // the next chunk's passes reuse the partial buffer after this reduction (stream order)
         done += chunk;
     }
     RR_HIP(hipEventRecord(e_end, c->stream));
+    RR_HIP(hipMemcpyAsync(seq_out, c->seq.ptr, (size_t)count * P * 4, hipMemcpyDeviceToHost, c->stream));
     RR_HIP(hipStreamSynchronize(c->stream));
     float ms = 0;
     RR_HIP(hipEventElapsedTime(&ms, e_begin, e_end));

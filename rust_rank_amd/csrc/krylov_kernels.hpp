@@ -9,60 +9,57 @@ namespace rr {
 constexpr int WG = 256;  // threads per workgroup (4 wavefronts)
 
 // ---------------------------------------------------------------------------
-// one row of the sparse product, lane j: sum_e v_e * in[col_e][j] mod p
+// One row of the sparse product on the B lanes of a row slot; lane j returns
+// sum_e v_e * in[col_e][j] mod p.  The slot loads up to B column indices (and
+// values) with one coalesced load, hands them round with lane shuffles and
+// issues up to 16 independent gathers before using any, so a row costs about
+// two dependent memory latencies instead of one per nonzero.  Lanes of one
+// slot share the row, so they run the same trip counts.
+//   packed:  |v| <= 8, x < 2^31, each term < 2^34 (exact in fp64), folded per chunk
+//   general: v = vh 2^16 + vl, vh x < 2^46, vl x < 2^47, <= 64 terms per chunk
 template <int B, bool PACKED>
 __device__ __forceinline__ uint32_t row_dot(const uint32_t *__restrict__ col, const uint32_t *__restrict__ val,
                                             const uint32_t *__restrict__ in, uint32_t beg, uint32_t end, int j,
                                             double pd, double inv) {
-    if constexpr (PACKED) {
-        // |v| <= 8, x < 2^31: each term < 2^34, exact in fp64 for 2^19 terms
-        double acc0 = 0.0, acc1 = 0.0;
-        uint32_t e = beg;
-        for (; e + 4 <= end; e += 4) {
-            uint32_t c0 = col[e], c1 = col[e + 1], c2 = col[e + 2], c3 = col[e + 3];
-            uint32_t x0 = in[(size_t)(c0 & 0x0FFFFFFFu) * B + j];
-            uint32_t x1 = in[(size_t)(c1 & 0x0FFFFFFFu) * B + j];
-            uint32_t x2 = in[(size_t)(c2 & 0x0FFFFFFFu) * B + j];
-            uint32_t x3 = in[(size_t)(c3 & 0x0FFFFFFFu) * B + j];
-            acc0 = fma((double)((int32_t)c0 >> 28), (double)x0, acc0);
-            acc1 = fma((double)((int32_t)c1 >> 28), (double)x1, acc1);
-            acc0 = fma((double)((int32_t)c2 >> 28), (double)x2, acc0);
-            acc1 = fma((double)((int32_t)c3 >> 28), (double)x3, acc1);
-        }
-        for (; e < end; e++) {
-            uint32_t c0 = col[e];
-            uint32_t x0 = in[(size_t)(c0 & 0x0FFFFFFFu) * B + j];
-            acc0 = fma((double)((int32_t)c0 >> 28), (double)x0, acc0);
-        }
-        return (uint32_t)fold(acc0 + acc1, pd, inv);
-    } else {
-        // v = vh 2^16 + vl: vh x < 2^46, vl x < 2^47; fold every 64 terms
-        double hi = 0.0, lo = 0.0;
-        uint32_t e = beg;
-        while (e < end) {
-            uint32_t stop = min(end, e + 64u);
-            for (; e + 2 <= stop; e += 2) {
-                uint32_t c0 = col[e], c1 = col[e + 1];
-                uint32_t v0 = val[e], v1 = val[e + 1];
-                double x0 = (double)in[(size_t)c0 * B + j];
-                double x1 = (double)in[(size_t)c1 * B + j];
-                hi = fma((double)(v0 >> 16), x0, hi);
-                lo = fma((double)(v0 & 0xFFFFu), x0, lo);
-                hi = fma((double)(v1 >> 16), x1, hi);
-                lo = fma((double)(v1 & 0xFFFFu), x1, lo);
+    constexpr int GS = B < 16 ? B : 16;
+    const int lb = (int)(threadIdx.x & 63) & ~(B - 1);
+    double a0 = 0.0, a1 = 0.0;
+    for (uint32_t c0 = beg; c0 < end; c0 += B) {
+        const uint32_t n = min((uint32_t)B, end - c0);
+        const uint32_t mc = (uint32_t)j < n ? col[c0 + j] : 0u;
+        uint32_t mv = 0;
+        if constexpr (!PACKED) mv = (uint32_t)j < n ? val[c0 + j] : 0u;
+        for (uint32_t e = 0; e < n; e += GS) {
+            uint32_t c[GS], x[GS], v[GS];
+#pragma unroll
+            for (int k = 0; k < GS; k++) {
+                const int src = lb + (int)((e + k) & (B - 1));
+                c[k] = (uint32_t)__shfl((int)mc, src, 64);
+                if constexpr (!PACKED) v[k] = (uint32_t)__shfl((int)mv, src, 64);
             }
-            if (e < stop) {
-                uint32_t c0 = col[e], v0 = val[e];
-                double x0 = (double)in[(size_t)c0 * B + j];
-                hi = fma((double)(v0 >> 16), x0, hi);
-                lo = fma((double)(v0 & 0xFFFFu), x0, lo);
-                e++;
+#pragma unroll
+            for (int k = 0; k < GS; k++) {
+                const uint32_t idx = PACKED ? (c[k] & 0x0FFFFFFFu) : c[k];
+                x[k] = (e + k < n) ? in[(size_t)idx * B + j] : 0u;
             }
-            hi = fold(hi, pd, inv);
-            lo = fold(lo, pd, inv);
+#pragma unroll
+            for (int k = 0; k < GS; k++) {
+                if constexpr (PACKED) {
+                    const double t = (double)((int32_t)c[k] >> 28);
+                    if (k & 1) a1 = fma(t, (double)x[k], a1);
+                    else a0 = fma(t, (double)x[k], a0);
+                } else {
+                    const double xd = (double)x[k];
+                    a0 = fma((double)(v[k] >> 16), xd, a0);
+                    a1 = fma((double)(v[k] & 0xFFFFu), xd, a1);
+                }
+            }
         }
-        return (uint32_t)fold(fma(hi, 65536.0, lo), pd, inv);
+        a0 = fold(a0, pd, inv);
+        a1 = fold(a1, pd, inv);
     }
+    if constexpr (PACKED) return (uint32_t)fold(a0 + a1, pd, inv);
+    else return (uint32_t)fold(fma(a0, 65536.0, a1), pd, inv);
 }
 
 // ---------------------------------------------------------------------------
