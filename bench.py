@@ -56,6 +56,89 @@ def cpu_baseline(A, p, b, dr, dc, V, iters=256):
                       f"{threads} OpenMP threads; setup cancelled by differencing two runs)"}
 
 
+def batched_and_served(eng, synth, p, nmat, rate_frac, seed=4):
+    """Batched form: nmat matrices of the batched workload (rows log-uniform in
+    [1000, 50000], cols 0.9 rows, 6-16 nnz/row, +-1/+-2, 1-5 % dependent
+    columns), b per matrix from the autotuner, solved in one rr_batch_rank, vs
+    the same matrices solved one at a time by the plain GPU path (rr_rank).
+    Served form: the same matrices as a Poisson stream at rate_frac x the
+    measured batched capacity through rr_server; FIFO plain latencies are
+    computed from the measured plain service times on the same arrival times."""
+    import threading
+    t0 = time.perf_counter()
+    mats = [synth.batch_matrix(seed, i) for i in range(nmat)]
+    bs, pred, probe = eng.autotune_b(mats, p)
+    pre = [synth.preconditioners(5000 + i, A.m, A.n, int(b), p) for i, (A, b) in enumerate(zip(mats, bs))]
+    log(f"batched: {nmat} matrices, rows {min(A.m for A in mats)}..{max(A.m for A in mats)}, "
+        f"b histogram {dict(zip(*np.unique(bs, return_counts=True)))}, setup {time.perf_counter() - t0:.1f}s")
+    # plain: one matrix at a time
+    plain_ms, plain_rank = [], []
+    t1 = time.perf_counter()
+    for (A, b, (dr, dc, V)) in zip(mats, bs, pre):
+        r, ms = eng.rank(A, p, int(b), dr, dc, V)
+        plain_rank.append(r)
+        plain_ms.append(ms)
+    t_plain = time.perf_counter() - t1
+    # batched: one call
+    t2 = time.perf_counter()
+    ranks, st = eng.batch_rank(mats, p, bs, [x[0] for x in pre], [x[1] for x in pre], [x[2] for x in pre])
+    t_batch = time.perf_counter() - t2
+    assert list(ranks) == plain_rank, "batched ranks differ from plain ranks"
+    batched = {"matrices": nmat, "ranks_per_s": round(nmat / t_batch, 3), "plain_ranks_per_s": round(nmat / t_plain, 3),
+               "speedup_vs_plain": round(t_plain / t_batch, 2), "wall_s": round(t_batch, 3),
+               "plain_wall_s": round(t_plain, 3), "ticks": st["ticks"], "pm_rounds": st["pm_rounds"],
+               "ms_krylov": round(st["ms_krylov"], 1), "ms_pm": round(st["ms_pm"], 1),
+               "b_hist": {int(k): int(v) for k, v in zip(*np.unique(bs, return_counts=True))},
+               "autotune_probe": probe, "ranks_equal_plain": True}
+    log(f"batched {t_batch:.2f}s vs plain {t_plain:.2f}s -> {t_plain / t_batch:.1f}x")
+    # served: Poisson arrivals at rate_frac x batched capacity
+    lam = rate_frac * nmat / t_batch
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, nmat))
+    results = {}
+    srv = eng.Server(p, 200 << 30)
+    stop = threading.Event()
+
+    def poller():
+        while not stop.is_set() or len(results) < nmat:
+            for r in srv.poll(wait_ms=50):
+                results[r["id"]] = r
+            if len(results) >= nmat:
+                break
+
+    th = threading.Thread(target=poller)
+    th.start()
+    t3 = time.perf_counter()
+    ids = []
+    for i, (A, b, (dr, dc, V)) in enumerate(zip(mats, bs, pre)):
+        delay = t3 + arrivals[i] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(A, int(b), dr, dc, V))
+    stop.set()
+    th.join(timeout=600)
+    t_served = time.perf_counter() - t3
+    sst = srv.stats()
+    srv.close()
+    lat = np.array([results[i]["latency_ms"] for i in ids])
+    served_ok = all(results[i]["rank"] == plain_rank[k] for k, i in enumerate(ids))
+    # FIFO plain on the same arrivals: one solve at a time in arrival order
+    fin, fifo = 0.0, []
+    for a, ms in zip(arrivals * 1e3, plain_ms):
+        fin = max(fin, a) + ms
+        fifo.append(fin - a)
+    fifo = np.array(fifo)
+    served = {"requests": nmat, "offered_rate_per_s": round(lam, 3), "ranks_per_s": round(nmat / t_served, 3),
+              "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+              "fifo_plain_p50_ms": round(float(np.percentile(fifo, 50)), 2),
+              "fifo_plain_p99_ms": round(float(np.percentile(fifo, 99)), 2),
+              "fifo_plain_ranks_per_s": round(nmat / (fin / 1e3), 3),
+              "max_active": sst["max_active"], "hbm_budget_gb": 200, "ranks_equal_plain": bool(served_ok)}
+    log(f"served: p50 {served['p50_ms']} ms p99 {served['p99_ms']} ms vs FIFO plain p50 {served['fifo_plain_p50_ms']} "
+        f"p99 {served['fifo_plain_p99_ms']}")
+    return batched, served
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,7 +148,16 @@ def main():
     ap.add_argument("--m", type=int, default=200000)
     ap.add_argument("--n", type=int, default=180000)
     ap.add_argument("--b", type=int, default=32)
+    ap.add_argument("--batched", type=int, default=64, help="matrices in the batched/served comparison (0: skip)")
+    ap.add_argument("--rate-frac", type=float, default=0.7, help="served offered load / batched capacity")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
+                    help="only run the plain copy kernel on MB megabytes (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calib:
+        from rust_rank_amd import engine
+        ms, gbs = engine.copy_probe(args.copy_calib << 20, 20)
+        print(json.dumps({"copy_bytes": args.copy_calib << 20, "avg_ms": ms, "read_write_gbs": gbs}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the engine does not shard "
               f"a single matrix across GPUs", file=sys.stderr)
@@ -95,6 +187,8 @@ def main():
     bytes_pass = (st["bytes_a"] * st["passes_a"] + st["bytes_at"] * st["passes_at"]) / max(passes, 1)
     achieved = bytes_pass / (pass_ms * 1e-3) / 1e9
     b_iter = 8 * A.nnz + (4 * b + 2) * (A.m + A.n)
+    log(f"pass by A avg {st['ms_a'] / max(st['passes_a'], 1) * 1e3:.2f} us, by A^T avg "
+        f"{st['ms_at'] / max(st['passes_at'], 1) * 1e3:.2f} us")
     log(f"timed {steps} iterations in {wall:.3f}s -> {iters_s:.1f} iters/s; k_pass avg {pass_ms * 1e3:.2f} us, "
         f"fixup {st['ms_fixup'] / max(passes, 1) * 1e3:.2f} us/pass, reduce {st['ms_reduce']:.2f} ms total, "
         f"stream total {st['ms_total']:.1f} ms")
@@ -108,6 +202,9 @@ def main():
                 "kernel": f"k_pass<{1 << (b - 1).bit_length()}, packed>",
                 "bytes_per_launch": int(bytes_pass), "avg_launch_us": round(pass_ms * 1e3, 3)}
     cpu = None if args.no_cpu else cpu_baseline(A, p, b, dr, dc, V)
+    batched = served = None
+    if args.batched:
+        batched, served = batched_and_served(engine, synth, p, args.batched, args.rate_frac)
     out = {
         "metric": "Krylov iters/s + HBM GB/s; batched ranks/s & p50/p99 vs plain, same GPU",
         "value": round(iters_s, 2), "unit": "iters/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
@@ -115,7 +212,7 @@ def main():
         "vs_baseline": None, "dtype": "u32", "data": "synthetic (seeded SplitMix64)",
         "config": {"workload": f"graph_complex {A.m}x{A.n} 12 nnz/row Zipf(1.1) +-1/+-2 p=2013265921 b={b}",
                    "seq_len": L, "nnz": A.nnz, "hbm_gbs_algorithmic": round(b_iter * iters_s / 1e9, 1)},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "served": served,
     }
     print(json.dumps(out), flush=True)
 

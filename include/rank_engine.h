@@ -178,6 +178,18 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
                         int64_t *rank_out, rr_batch_stats *stats);
 
 /*
+ * Autotuner for the batched and served modes: picks the block width b (and
+ * with it rows per wavefront, 64/b) from {8, 16, 32} for each matrix,
+ * minimising predicted device time from probes measured once per process on
+ * this GPU (Krylov pass time per nonzero per width, PM-basis time at order
+ * 255 per width).  p must be 2013265921.  pred_ms (may be NULL) receives the
+ * prediction.  rr_autotune_probe returns the measured probe constants for
+ * b = 8, 16, 32.
+ */
+rr_status rr_autotune_b(uint32_t count, const rr_csr *mats, uint32_t p, uint32_t *b_out, double *pred_ms);
+rr_status rr_autotune_probe(double *c_ms_per_nnz, double *pm_ms_d255);
+
+/*
  * Continuous-batching rank server (no counterpart in the reference).  A
  * worker thread admits submitted requests in arrival order while their
  * resident footprint fits hbm_budget_bytes, keeps each request's matrix,
@@ -209,6 +221,14 @@ rr_status rr_server_poll(rr_server *s, rr_result *out, uint32_t cap, uint32_t *n
 rr_status rr_server_stats_get(rr_server *s, rr_server_stats *out);
 /* Stops the worker (requests still queued or running are dropped) and frees everything. */
 void rr_server_destroy(rr_server *s);
+
+/*
+ * Measurement helper (no counterpart in the reference): a plain device copy
+ * kernel of `bytes` run `iters` times; average kernel time and read+write
+ * GB/s.  Used to calibrate hardware byte counters and to measure the
+ * achievable HBM bandwidth next to the Krylov kernel's.
+ */
+rr_status rr_copy_probe(uint64_t bytes, uint32_t iters, double *ms_out, double *gbs_out);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,7 @@
 #include <cstring>
 #include <chrono>
 #include <memory>
+#include <cmath>
 
 namespace rr {
 
@@ -33,8 +34,10 @@ __global__ __launch_bounds__(WG) void k_tiles(const Tile *__restrict__ tiles, ui
                                               double inv) {
     constexpr int G = 64 / B;
     __shared__ uint32_t lds[(WG / 64) * B * (B / 2 + 1)];
+    __shared__ WaveLds wl[WG / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = lane / B, j = lane & (B - 1);
+    WaveLds &w = wl[wave];
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile T = tiles[t];
         Gram<B> g;
@@ -47,13 +50,13 @@ __global__ __launch_bounds__(WG) void k_tiles(const Tile *__restrict__ tiles, ui
                     o = s = T.in[(size_t)r * B + j];
                 } else {
                     const uint32_t beg = T.rowptr[r], end = T.rowptr[r + 1];
-                    o = T.mode == TILE_PACKED ? row_dot<B, true>(T.col, T.val, T.in, beg, end, j, pd, inv)
-                                              : row_dot<B, false>(T.col, T.val, T.in, beg, end, j, pd, inv);
+                    o = T.mode == TILE_PACKED ? row_dot<B, true>(T.col, T.val, T.in, beg, end, j, w, pd, inv)
+                                              : row_dot<B, false>(T.col, T.val, T.in, beg, end, j, w, pd, inv);
                     s = mulmod(o, T.scale[r], p, inv);
                     T.out[(size_t)r * B + j] = s;
                 }
             }
-            g.add(o, s, lane, pd, inv);
+            g.add(o, s, lane, w, pd, inv);
         }
         g.finish(T.part, T.b, p, pd, inv, lds);
         __syncthreads();  // lds is reused by the next tile
@@ -296,19 +299,49 @@ rr_status Request::start_pm(hipStream_t st) {
     return RR_OK;
 }
 
-rr_status Request::finish_rank(hipStream_t st) {
-    hipLaunchKernelGGL(k_rank_extract, dim3(1), dim3(256), 0, st, M.as<uint32_t>(), delta.as<int64_t>(), b, d + 1,
-                       rank_in.as<uint32_t>());
+// Rank from a device-resident basis [2b][2b][len] and delta: the two coefficient
+// matrices the formula needs are extracted on the device (2b^2 + 1 words come back).
+rr_status rank_from_device_basis(const uint32_t *M, const int64_t *delta, uint32_t b, uint64_t len, uint32_t p,
+                                 uint32_t *scratch, hipStream_t st, int64_t *rank) {
+    hipLaunchKernelGGL(k_rank_extract, dim3(1), dim3(256), 0, st, M, delta, b, len, scratch);
     RR_HIP(hipGetLastError());
     std::vector<uint32_t> h(2 * b * b + 1);
-    RR_HIP(hipMemcpyAsync(h.data(), rank_in.ptr, h.size() * 4, hipMemcpyDeviceToHost, st));
+    RR_HIP(hipMemcpyAsync(h.data(), scratch, h.size() * 4, hipMemcpyDeviceToHost, st));
     RR_HIP(hipStreamSynchronize(st));
     uint64_t K = h[0];
     std::vector<uint32_t> F0(h.begin() + 1, h.begin() + 1 + b * b), FK(h.begin() + 1 + b * b, h.end());
     uint64_t r = (uint64_t)b * (K - 3) + host_rank_bxb(FK, b, p) + host_rank_bxb(F0, b, p);
-    rank = (int64_t)r;
+    *rank = (int64_t)r;  // the reference's usize arithmetic wraps the same way when K < 3
+    return RR_OK;
+}
+
+rr_status Request::finish_rank(hipStream_t st) {
+    rr_status s = rank_from_device_basis(M.as<uint32_t>(), delta.as<int64_t>(), b, d + 1, p, rank_in.as<uint32_t>(),
+                                         st, &rank);
+    if (s != RR_OK) return s;
     stage = STAGE_DONE;
     return RR_OK;
+}
+
+// Sequence on the device [>= d+1][P] -> rank (G, PM-basis, extraction), all on the device.
+rr_status rank_from_device_seq(const uint32_t *seq_dev, uint32_t b, uint64_t d, uint32_t p, hipStream_t st,
+                               int64_t *rank) {
+    DevBuf G, M, dl, sc;
+    rr_status s;
+    if ((s = G.alloc((size_t)2 * b * b * (d + 1) * 4)) != RR_OK) return s;
+    if ((s = M.alloc((size_t)4 * b * b * (d + 1) * 4)) != RR_OK) return s;
+    if ((s = dl.alloc((size_t)2 * b * 8)) != RR_OK) return s;
+    if ((s = sc.alloc((size_t)(2 * b * b + 1) * 4)) != RR_OK) return s;
+    if ((s = build_G_device(seq_dev, b, d + 1, G.as<uint32_t>(), st)) != RR_OK) return s;
+    PmJob J;
+    J.b = b;
+    J.d = d;
+    J.G = G.as<uint32_t>();
+    J.M = M.as<uint32_t>();
+    J.delta = dl.as<int64_t>();
+    std::vector<PmJob *> jobs{&J};
+    if ((s = pm_basis_batch(p, jobs, st, nullptr, nullptr)) != RR_OK) return s;
+    return rank_from_device_basis(M.as<uint32_t>(), dl.as<int64_t>(), b, d + 1, p, sc.as<uint32_t>(), st, rank);
 }
 
 // ---------------------------------------------------------------------------
@@ -489,6 +522,144 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
         stats->ticks = T.ticks;
         stats->pm_rounds = rounds;
         stats->pm_launches = launches;
+    }
+    return RR_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Autotuner: block width b (and with it rows per wavefront, 64/b) per matrix.
+// Measured once per process on the device:
+//   c_B   Krylov pass time per nonzero for lane width B (tile kernel on a
+//         probe matrix with +-1/+-2 entries, 16 ticks timed with events),
+//   pm_b  PM-basis time per order d = 255 for block width b.
+// Predicted device time of matrix i with width b:
+//   L(b) * nnz_i * c_b + pm_b * (d(b) / 255) * log2(d(b) + 1) / 8,
+//   L(b) = 2 min(m, n) / b + 8, d(b) = L(b) - 1.
+
+namespace rr {
+
+struct TuneProbe {
+    bool done = false;
+    double c[3] = {0, 0, 0}, pm[3] = {0, 0, 0};
+};
+static TuneProbe g_probe;
+static const uint32_t TUNE_B[3] = {8, 16, 32};
+
+static rr_status run_probe(uint32_t p, int cus, hipStream_t st) {
+    const uint64_t m = 24000, n = 21600, k = 11;
+    std::vector<uint64_t> rp(m + 1);
+    std::vector<uint32_t> col(m * k), val(m * k);
+    uint64_t x = 0x9E3779B97F4A7C15ull;
+    auto rnd = [&]() {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        return x;
+    };
+    for (uint64_t r = 0; r < m; r++) {
+        rp[r] = r * k;
+        for (uint64_t e = 0; e < k; e++) {
+            col[r * k + e] = (uint32_t)(rnd() % n);
+            uint32_t mag = (rnd() & 3) == 0 ? 2 : 1;
+            val[r * k + e] = (rnd() & 1) ? p - mag : mag;
+        }
+    }
+    rp[m] = m * k;
+    std::vector<uint32_t> dr(m, 3), dc(n, 5), V(n * 32);
+    for (auto &v : V) v = (uint32_t)(rnd() % p);
+    rr_csr A{m, n, rp.data(), col.data(), val.data()};
+    hipEvent_t e0, e1;
+    RR_HIP(hipEventCreate(&e0));
+    RR_HIP(hipEventCreate(&e1));
+    rr_status s;
+    for (int i = 0; i < 3; i++) {
+        uint32_t b = TUNE_B[i];
+        std::vector<uint32_t> Vb(n * b);
+        for (uint64_t r = 0; r < n; r++) std::memcpy(&Vb[r * b], &V[r * 32], b * 4);
+        Request R;
+        if ((s = R.init(&A, p, b, dr.data(), dc.data(), Vb.data(), 40, 39, 2048, st)) != RR_OK) return s;
+        TickRunner T;
+        if ((s = T.init(st, cus)) != RR_OK) return s;
+        std::vector<Request *> one{&R};
+        for (int w = 0; w < 4; w++)
+            if ((s = T.tick(one, p)) != RR_OK) return s;
+        RR_HIP(hipEventRecord(e0, st));
+        for (int w = 0; w < 16; w++)
+            if ((s = T.tick(one, p)) != RR_OK) return s;
+        RR_HIP(hipEventRecord(e1, st));
+        RR_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        RR_HIP(hipEventElapsedTime(&ms, e0, e1));
+        g_probe.c[i] = (ms / 16.0) / (double)(m * k);
+        // PM-basis probe, order 255, random sequence
+        const uint64_t d = 255, P = b * (b + 1) / 2;
+        std::vector<uint32_t> seq((d + 1) * P);
+        for (auto &v : seq) v = (uint32_t)(rnd() % p);
+        DevBuf dseq, G, M, dl;
+        if ((s = dseq.alloc(seq.size() * 4)) != RR_OK) return s;
+        if ((s = G.alloc((size_t)2 * b * b * (d + 1) * 4)) != RR_OK) return s;
+        if ((s = M.alloc((size_t)4 * b * b * (d + 1) * 4)) != RR_OK) return s;
+        std::vector<int64_t> delta(2 * b);
+        RR_HIP(hipMemcpyAsync(dseq.ptr, seq.data(), seq.size() * 4, hipMemcpyHostToDevice, st));
+        if ((s = build_G_device(dseq.as<uint32_t>(), b, d + 1, G.as<uint32_t>(), st)) != RR_OK) return s;
+        RR_HIP(hipStreamSynchronize(st));
+        auto t0 = std::chrono::steady_clock::now();
+        if ((s = pm_basis_device(p, b, d, G.as<uint32_t>(), M.as<uint32_t>(), delta.data(), st)) != RR_OK) return s;
+        g_probe.pm[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g_probe.done = true;
+    return RR_OK;
+}
+
+}  // namespace rr
+
+extern "C" {
+
+rr_status rr_autotune_b(uint32_t count, const rr_csr *mats, uint32_t p, uint32_t *b_out, double *pred_ms) {
+    clear_error();
+    RR_CHECK(count == 0 || (mats && b_out), RR_EINVAL, "null argument");
+    RR_CHECK(p == NTT_PRIME, RR_EINVAL, "the autotuner's PM-basis probe needs p = 2013265921");
+    if (!g_probe.done) {
+        int dev = 0, cus = 0;
+        RR_HIP(hipGetDevice(&dev));
+        RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        hipStream_t st;
+        RR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        rr_status s = run_probe(p, cus, st);
+        (void)hipStreamDestroy(st);
+        if (s != RR_OK) return s;
+    }
+    for (uint32_t i = 0; i < count; i++) {
+        const rr_csr &A = mats[i];
+        uint64_t nnz = A.row_ptr[A.m], mn = std::min(A.m, A.n);
+        double best = 1e300;
+        uint32_t bb = TUNE_B[0];
+        for (int k = 0; k < 3; k++) {
+            uint32_t b = TUNE_B[k];
+            double L = 2.0 * (double)mn / b + 8, d = L - 1;
+            if (d + 1 > (double)(1u << NTT_MAX_LOGN)) continue;
+            double t = L * (double)nnz * g_probe.c[k] + g_probe.pm[k] * (d / 255.0) * std::log2(d + 1) / 8.0;
+            if (t < best) {
+                best = t;
+                bb = b;
+            }
+        }
+        b_out[i] = bb;
+        if (pred_ms) pred_ms[i] = best;
+    }
+    return RR_OK;
+}
+
+rr_status rr_autotune_probe(double *c_ms_per_nnz, double *pm_ms_d255) {
+    RR_CHECK(c_ms_per_nnz && pm_ms_d255, RR_EINVAL, "null argument");
+    RR_CHECK(g_probe.done, RR_ESTATE, "autotuner has not probed yet");
+    for (int k = 0; k < 3; k++) {
+        c_ms_per_nnz[k] = g_probe.c[k];
+        pm_ms_d255[k] = g_probe.pm[k];
     }
     return RR_OK;
 }

@@ -52,6 +52,11 @@ struct Request {
     uint64_t device_bytes() const;          // resident footprint estimate (admission control)
 };
 
+rr_status rank_from_device_basis(const uint32_t *M, const int64_t *delta, uint32_t b, uint64_t len, uint32_t p,
+                                 uint32_t *scratch, hipStream_t st, int64_t *rank);
+rr_status rank_from_device_seq(const uint32_t *seq_dev, uint32_t b, uint64_t d, uint32_t p, hipStream_t st,
+                               int64_t *rank);
+
 struct TickRunner {
     static constexpr size_t STAGE_BYTES = 4u << 20;
     static constexpr int NSTAGE = 8;

@@ -10,8 +10,14 @@
 // length - 1), and the rank is returned instead of printed
 // (src/blockbmtest.rs:598-599).
 #include "common.hpp"
+#include "batch.hpp"
 #include <vector>
 #include <chrono>
+
+namespace rr {
+const uint32_t *krylov_device_seq(rr_krylov *c);
+hipStream_t krylov_stream(rr_krylov *c);
+}  // namespace rr
 
 extern "C" {
 
@@ -46,18 +52,17 @@ rr_status rr_rank(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *row_p
     auto t0 = std::chrono::steady_clock::now();
     if (seq_len == 0) seq_len = rr_seq_len(A->m, A->n, b, 0);
     if (order == 0) order = rr_pm_order(seq_len, 0);
+    RR_CHECK(order >= 1 && order < seq_len, RR_EINVAL, "order must satisfy 1 <= order < seq_len");
     const uint64_t P = (uint64_t)b * (b + 1) / 2;
     std::vector<uint32_t> seq(seq_len * P);
     rr_krylov *k = nullptr;
     rr_status s = rr_krylov_create(A, p, b, row_precond, col_precond, V, 0, &k);
     if (s != RR_OK) return s;
     s = rr_krylov_run(k, seq_len, seq.data());
+    if (s == RR_OK)  // the sequence is still on the device: G, PM-basis and rank stay there
+        s = rr::rank_from_device_seq(rr::krylov_device_seq(k), b, order, p, rr::krylov_stream(k), rank_out);
     rr_krylov_destroy(k);
     if (s != RR_OK) return s;
-    std::vector<uint32_t> basis((size_t)4 * b * b * (order + 1));
-    std::vector<int64_t> delta(2 * b);
-    if ((s = rr_pm_basis(seq.data(), b, seq_len, order, p, basis.data(), delta.data())) != RR_OK) return s;
-    if ((s = rr_rank_from_basis(basis.data(), delta.data(), b, order + 1, p, rank_out)) != RR_OK) return s;
     if (ms_out)
         *ms_out = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return RR_OK;

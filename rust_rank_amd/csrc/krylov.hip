@@ -33,8 +33,8 @@
 
 namespace rr {
 
-constexpr uint32_t LONG_ROW = 4096; // rows above this many nonzeros are split
-constexpr uint32_t PIECE = 2048;    // nonzeros per piece of a split row
+constexpr uint32_t LONG_ROW = 128;  // rows above this many nonzeros are split
+constexpr uint32_t PIECE = 128;     // nonzeros per piece of a split row
 constexpr int CHUNK = 64;           // passes per sequence reduction
 
 struct PassArgs {
@@ -64,14 +64,19 @@ template <int B, bool PACKED>
 __global__ __launch_bounds__(WG) void k_pass(PassArgs a) {
     constexpr int G = 64 / B;
     __shared__ uint32_t lds[(WG / 64) * B * (B / 2 + 1)];
+    __shared__ WaveLds wl[WG / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = lane / B, j = lane & (B - 1);
     const uint32_t gwave = blockIdx.x * (WG / 64) + wave;
     const uint32_t nwaves = gridDim.x * (WG / 64);
+    WaveLds &w = wl[wave];
 
-    for (uint32_t pc = gwave * G + slot; pc < a.npieces; pc += nwaves * G)
-        a.piece_out[(size_t)pc * B + j] = row_dot<B, PACKED>(a.col, a.val, a.in, a.piece_beg[pc], a.piece_end[pc], j, a.pd, a.inv);
-
+    for (uint32_t base = gwave * G; base < a.npieces; base += nwaves * G) {
+        const uint32_t pc = base + slot;
+        if (pc < a.npieces)
+            a.piece_out[(size_t)pc * B + j] =
+                row_dot<B, PACKED>(a.col, a.val, a.in, a.piece_beg[pc], a.piece_end[pc], j, w, a.pd, a.inv);
+    }
     if (a.nlong == 0 && blockIdx.x == 0) {  // no fix-up kernel this pass: its partials are zero
         uint32_t *fx = a.gram + (size_t)gridDim.x * (a.b * (a.b + 1) / 2);
         for (uint32_t i = threadIdx.x; i < a.fix_words; i += blockDim.x) fx[i] = 0;
@@ -84,39 +89,47 @@ __global__ __launch_bounds__(WG) void k_pass(PassArgs a) {
         if (r < a.nrows) {
             uint32_t beg = a.rowptr[r], end = a.rowptr[r + 1];
             if (end - beg <= LONG_ROW) {
-                o = row_dot<B, PACKED>(a.col, a.val, a.in, beg, end, j, a.pd, a.inv);
+                o = row_dot<B, PACKED>(a.col, a.val, a.in, beg, end, j, w, a.pd, a.inv);
                 s = mulmod(o, a.scale[r], a.p, a.inv);
                 a.out[(size_t)r * B + j] = s;
             }
         }
-        g.add(o, s, lane, a.pd, a.inv);
+        g.add(o, s, lane, w, a.pd, a.inv);
     }
     g.finish(a.gram + (size_t)blockIdx.x * (a.b * (a.b + 1) / 2), a.b, a.p, a.pd, a.inv, lds);
 }
 
-// Split rows: sum the pieces, scale, store, project.  Partials go after the
+// Split rows, one workgroup per row at a time: every row slot of the workgroup
+// sums a strided share of the row's pieces, the shares meet in LDS, and
+// wavefront 0 scales, stores and projects the row.  Partials go after the
 // main kernel's (gram + (main_grid + blockIdx.x) * P).
 template <int B>
 __global__ __launch_bounds__(WG) void k_fixup(PassArgs a, uint32_t main_grid) {
-    constexpr int G = 64 / B;
+    constexpr int G = 64 / B, NS = (WG / 64) * G;
     __shared__ uint32_t lds[(WG / 64) * B * (B / 2 + 1)];
+    __shared__ WaveLds wl[WG / 64];
+    __shared__ uint32_t red[WG];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = lane / B, j = lane & (B - 1);
-    const uint32_t gwave = blockIdx.x * (WG / 64) + wave;
-    const uint32_t nwaves = gridDim.x * (WG / 64);
+    const int gs = wave * G + slot;
     Gram<B> g;
     g.init();
-    for (uint32_t base = gwave * G; base < a.nlong; base += nwaves * G) {
-        uint32_t l = base + slot;
-        uint32_t o = 0, s = 0;
-        if (l < a.nlong) {
-            uint32_t r = a.long_rows[l];
-            for (uint32_t pc = a.long_pbeg[l]; pc < a.long_pbeg[l + 1]; pc++)
-                o = addmod(o, a.piece_out[(size_t)pc * B + j], a.p);
-            s = mulmod(o, a.scale[r], a.p, a.inv);
-            a.out[(size_t)r * B + j] = s;
+    for (uint32_t l = blockIdx.x; l < a.nlong; l += gridDim.x) {
+        const uint32_t r = a.long_rows[l], pb = a.long_pbeg[l], pe = a.long_pbeg[l + 1];
+        uint32_t part = 0;
+        for (uint32_t pc = pb + gs; pc < pe; pc += NS) part = addmod(part, a.piece_out[(size_t)pc * B + j], a.p);
+        red[threadIdx.x] = part;
+        __syncthreads();
+        if (wave == 0) {
+            uint32_t o = 0, s = 0;
+            if (slot == 0) {
+                for (int q = 0; q < NS; q++) o = addmod(o, red[q * B + j], a.p);
+                s = mulmod(o, a.scale[r], a.p, a.inv);
+                a.out[(size_t)r * B + j] = s;
+            }
+            g.add(o, s, lane, wl[0], a.pd, a.inv);
         }
-        g.add(o, s, lane, a.pd, a.inv);
+        __syncthreads();
     }
     g.finish(a.gram + (size_t)(main_grid + blockIdx.x) * (a.b * (a.b + 1) / 2), a.b, a.p, a.pd, a.inv, lds);
 }
@@ -127,6 +140,7 @@ __global__ __launch_bounds__(WG) void k_gram(const uint32_t *x, uint32_t nrows, 
                                              uint32_t p, double pd, double inv) {
     constexpr int G = 64 / B;
     __shared__ uint32_t lds[(WG / 64) * B * (B / 2 + 1)];
+    __shared__ WaveLds wl[WG / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = lane / B, j = lane & (B - 1);
     const uint32_t gwave = blockIdx.x * (WG / 64) + wave;
@@ -136,7 +150,7 @@ __global__ __launch_bounds__(WG) void k_gram(const uint32_t *x, uint32_t nrows, 
     for (uint32_t base = gwave * G; base < nrows; base += nwaves * G) {
         uint32_t r = base + slot;
         uint32_t o = r < nrows ? x[(size_t)r * B + j] : 0;
-        g.add(o, o, lane, pd, inv);
+        g.add(o, o, lane, wl[wave], pd, inv);
     }
     g.finish(gram + (size_t)blockIdx.x * (b * (b + 1) / 2), b, p, pd, inv, lds);
 }
@@ -281,6 +295,26 @@ static void launch_pass(rr_krylov *c, const PassArgs &a) {
         hipLaunchKernelGGL((k_pass<B, false>), dim3(c->grid), dim3(WG), 0, c->stream, a);
 }
 
+template <int B>
+static int pass_blocks_per_cu(bool packed) {
+    int n = 0;
+    hipError_t e = packed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass<B, true>, WG, 0)
+                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass<B, false>, WG, 0);
+    return (e == hipSuccess && n > 0) ? n : 2;
+}
+
+static int pass_blocks_per_cu_any(uint32_t B, bool packed) {
+    switch (B) {
+        case 1: return pass_blocks_per_cu<1>(packed);
+        case 2: return pass_blocks_per_cu<2>(packed);
+        case 4: return pass_blocks_per_cu<4>(packed);
+        case 8: return pass_blocks_per_cu<8>(packed);
+        case 16: return pass_blocks_per_cu<16>(packed);
+        case 32: return pass_blocks_per_cu<32>(packed);
+        default: return pass_blocks_per_cu<64>(packed);
+    }
+}
+
 static void launch_pass_any(rr_krylov *c, const PassArgs &a) {
     switch (c->B) {
         case 1: launch_pass<1>(c, a); break;
@@ -386,8 +420,8 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
     RR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     int cus = 0;
     RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-    c->grid = (uint32_t)std::max(1, cus * 2);
-    c->fix_grid = 8;
+    c->grid = 0;  // set below from the occupancy of the chosen pass kernel
+    c->fix_grid = 64;
 
     // A and A^T (counting-sort transpose, row order kept: src/matrices.rs:734-778)
     std::vector<uint32_t> rp(m + 1), colw(nnz), vals(small ? 0 : nnz);
@@ -414,6 +448,8 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
                 }
             }
     }
+    // one resident wave of workgroups: the pass kernels stride over rows
+    c->grid = (uint32_t)std::max(1, cus * pass_blocks_per_cu_any(c->B, small));
     rr_status s;
     if ((s = build_operand(c->A, m, rp, colw, vals, small, c->B)) != RR_OK) return s;
     if ((s = build_operand(c->At, n, rpt, colt, valt, small, c->B)) != RR_OK) return s;
@@ -605,5 +641,15 @@ rr_status rr_krylov_state(rr_krylov *c, uint32_t *curv_out, uint64_t *t_out) {
 }
 
 void rr_krylov_destroy(rr_krylov *c) { delete c; }
+
+}  // extern "C"
+
+namespace rr {
+// The last run's sequence on the device ([count][P]) and the context's stream.
+const uint32_t *krylov_device_seq(rr_krylov *c) { return c->seq.as<uint32_t>(); }
+hipStream_t krylov_stream(rr_krylov *c) { return c->stream; }
+}  // namespace rr
+
+extern "C" {
 
 }  // extern "C"

@@ -9,33 +9,44 @@ namespace rr {
 constexpr int WG = 256;  // threads per workgroup (4 wavefronts)
 
 // ---------------------------------------------------------------------------
+// Lanes of one wavefront exchange values through a per-wavefront LDS area
+// (ds_bpermute shuffles measure ~10 lane-ops/clk/CU on MI355X, LDS reads 3x
+// more).  LDS operations of one wavefront complete in order; the wave
+// barrier keeps the compiler from moving them across each other.
+struct WaveLds {
+    uint32_t col[64];  // column indices of the current chunk, per row slot
+    uint32_t val[64];  // values of the current chunk (general matrices)
+    uint32_t o[64];    // produced row entries, for the projection
+};
+
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
+
 // One row of the sparse product on the B lanes of a row slot; lane j returns
 // sum_e v_e * in[col_e][j] mod p.  The slot loads up to B column indices (and
-// values) with one coalesced load, hands them round with lane shuffles and
-// issues up to 16 independent gathers before using any, so a row costs about
-// two dependent memory latencies instead of one per nonzero.  Lanes of one
-// slot share the row, so they run the same trip counts.
+// values) with one coalesced load, shares them through LDS and issues up to
+// 16 independent gathers before using any, so a row costs about two dependent
+// memory latencies per B nonzeros.  Lanes of one slot share the row, so they
+// run the same trip counts.
 //   packed:  |v| <= 8, x < 2^31, each term < 2^34 (exact in fp64), folded per chunk
 //   general: v = vh 2^16 + vl, vh x < 2^46, vl x < 2^47, <= 64 terms per chunk
 template <int B, bool PACKED>
 __device__ __forceinline__ uint32_t row_dot(const uint32_t *__restrict__ col, const uint32_t *__restrict__ val,
                                             const uint32_t *__restrict__ in, uint32_t beg, uint32_t end, int j,
-                                            double pd, double inv) {
+                                            WaveLds &w, double pd, double inv) {
     constexpr int GS = B < 16 ? B : 16;
-    const int lb = (int)(threadIdx.x & 63) & ~(B - 1);
+    const int lane = (int)(threadIdx.x & 63), lb = lane & ~(B - 1);
     double a0 = 0.0, a1 = 0.0;
     for (uint32_t c0 = beg; c0 < end; c0 += B) {
         const uint32_t n = min((uint32_t)B, end - c0);
-        const uint32_t mc = (uint32_t)j < n ? col[c0 + j] : 0u;
-        uint32_t mv = 0;
-        if constexpr (!PACKED) mv = (uint32_t)j < n ? val[c0 + j] : 0u;
+        w.col[lane] = (uint32_t)j < n ? col[c0 + j] : 0u;
+        if constexpr (!PACKED) w.val[lane] = (uint32_t)j < n ? val[c0 + j] : 0u;
+        wave_sync();
         for (uint32_t e = 0; e < n; e += GS) {
             uint32_t c[GS], x[GS], v[GS];
 #pragma unroll
             for (int k = 0; k < GS; k++) {
-                const int src = lb + (int)((e + k) & (B - 1));
-                c[k] = (uint32_t)__shfl((int)mc, src, 64);
-                if constexpr (!PACKED) v[k] = (uint32_t)__shfl((int)mv, src, 64);
+                c[k] = w.col[lb + (int)((e + k) & (B - 1))];
+                if constexpr (!PACKED) v[k] = w.val[lb + (int)((e + k) & (B - 1))];
             }
 #pragma unroll
             for (int k = 0; k < GS; k++) {
@@ -55,6 +66,7 @@ __device__ __forceinline__ uint32_t row_dot(const uint32_t *__restrict__ col, co
                 }
             }
         }
+        wave_sync();
         a0 = fold(a0, pd, inv);
         a1 = fold(a1, pd, inv);
     }
@@ -64,10 +76,10 @@ __device__ __forceinline__ uint32_t row_dot(const uint32_t *__restrict__ col, co
 
 // ---------------------------------------------------------------------------
 // Per-lane accumulation of the symmetric projection sum_r o_r s_r^T.
-// Lane j of a row pairs its own s_j with o_{(j+k) mod B} for k = 0..B/2, which
-// covers every unordered pair once (k = B/2 only for j < B/2).  s is split in
-// 16-bit halves so each product fits fp64 exactly; partial sums fold every 64
-// rows.
+// Lane j of a row pairs its own s_j with o_{(j+k) mod B} for k = 0..B/2 (read
+// from LDS), which covers every unordered pair once (k = B/2 only for
+// j < B/2).  s is split in 16-bit halves so each product fits fp64 exactly;
+// partial sums fold every 64 rows.
 template <int B>
 struct Gram {
     static constexpr int K = B / 2 + 1;
@@ -79,18 +91,21 @@ struct Gram {
         for (int k = 0; k < K; k++) hi[k] = lo[k] = 0.0;
         rows = 0;
     }
-    // all 64 lanes must call this together
-    __device__ __forceinline__ void add(uint32_t o, uint32_t s, int lane, double pd, double inv) {
+    // all 64 lanes of the wavefront call this together
+    __device__ __forceinline__ void add(uint32_t o, uint32_t s, int lane, WaveLds &w, double pd, double inv) {
         const int base = lane & ~(B - 1);
         const int j = lane & (B - 1);
         const double sh = (double)(s >> 16), sl = (double)(s & 0xFFFFu);
+        w.o[lane] = o;
+        wave_sync();
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            uint32_t ok = (k == 0) ? o : (uint32_t)__shfl((int)o, base + ((j + k) & (B - 1)), 64);
-            double od = (double)ok;
+            const uint32_t ok = (k == 0) ? o : w.o[base + ((j + k) & (B - 1))];
+            const double od = (double)ok;
             hi[k] = fma(od, sh, hi[k]);
             lo[k] = fma(od, sl, lo[k]);
         }
+        wave_sync();
         if (++rows == 64) {
 #pragma unroll
             for (int k = 0; k < K; k++) {

@@ -105,6 +105,12 @@ def lib():
     L.rr_server_destroy.restype = None
     for name in ("rr_server_create", "rr_server_submit", "rr_server_poll", "rr_server_stats_get"):
         getattr(L, name).restype = C.c_int
+    L.rr_autotune_b.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, _u32p, C.c_void_p]
+    L.rr_autotune_probe.argtypes = [C.c_void_p, C.c_void_p]
+    for name in ("rr_autotune_b", "rr_autotune_probe"):
+        getattr(L, name).restype = C.c_int
+    L.rr_copy_probe.argtypes = [C.c_uint64, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.rr_copy_probe.restype = C.c_int
     for name in ("rr_krylov_create", "rr_krylov_run", "rr_krylov_state", "rr_krylov_set_profiling",
                  "rr_krylov_stats_get", "rr_pm_basis", "rr_rank_from_basis", "rr_ntt", "rr_poly_mat_mul"):
         getattr(L, name).restype = C.c_int
@@ -306,3 +312,28 @@ class Server:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def autotune_b(mats, p: int):
+    """Block width per matrix from the measured-probe model (rr_autotune_b)."""
+    n = len(mats)
+    keep = []
+    csrs = (_Csr * n)()
+    for i, A in enumerate(mats):
+        s, k = _csr_struct(A)
+        csrs[i] = s
+        keep.append(k)
+    out = np.zeros(n, np.uint32)
+    pred = np.zeros(n, np.float64)
+    _check(lib().rr_autotune_b(n, C.cast(csrs, C.c_void_p), p, out, pred.ctypes.data))
+    c = np.zeros(3)
+    pm = np.zeros(3)
+    _check(lib().rr_autotune_probe(c.ctypes.data, pm.ctypes.data))
+    return out, pred, {"c_ms_per_nnz": c.tolist(), "pm_ms_d255": pm.tolist()}
+
+
+def copy_probe(nbytes: int, iters: int = 20):
+    """Plain device copy of nbytes: (avg kernel ms, read+write GB/s)."""
+    ms, gbs = C.c_double(), C.c_double()
+    _check(lib().rr_copy_probe(nbytes, iters, C.byref(ms), C.byref(gbs)))
+    return ms.value, gbs.value
