@@ -33,7 +33,7 @@ template <int B>
 __global__ __launch_bounds__(WG) void k_tiles(const Tile *__restrict__ tiles, uint32_t ntiles, uint32_t p, double pd,
                                               double inv) {
     constexpr int G = 64 / B;
-    __shared__ uint32_t lds[(WG / 64) * B * (B / 2 + 1)];
+    __shared__ uint32_t lds[Gram<B>::LDS_WORDS];
     __shared__ WaveLds wl[WG / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = lane / B, j = lane & (B - 1);

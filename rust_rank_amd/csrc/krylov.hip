@@ -18,95 +18,286 @@
 //
 // Layout in HBM: block vectors are row-interleaved with a power-of-two stride
 // B >= b (b columns used, the rest zero), so one row is one contiguous
-// 4B-byte segment and a wavefront of 64 lanes covers 64/B rows.  Lane j of a
-// row owns column j.  Rows longer than LONG_ROW nonzeros (the Zipf head of
-// A^T) are split into pieces processed by many wavefronts and combined by a
-// small fix-up kernel.  Each workgroup writes one partial b x b upper triangle
-// per pass; a reduction kernel folds the partials of a chunk of passes into
-// the sequence.
+// 4B-byte segment.  A lane holds VW = min(B, 4) consecutive words of a row
+// (one 16-byte gather for B >= 4) and a wavefront works on RPW = 256/B rows
+// side by side (8 for B = 32).  The sparse operands are stored in slices of
+// RPW length-sorted rows, column indices in blocks of SU = 4 per row, and
+// every wavefront owns a contiguous, load-balanced stream of slices.  Rows
+// longer than the balance target (the Zipf head of A^T) are cut into
+// segments whose partial sums a small fix-up kernel combines.  Each
+// workgroup writes one partial b x b upper triangle per pass; a reduction
+// kernel folds the partials of a chunk of passes into the sequence.
 #include "common.hpp"
 #include "krylov_kernels.hpp"
 #include <vector>
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <queue>
 
 namespace rr {
 
-constexpr uint32_t LONG_ROW = 128;  // rows above this many nonzeros are split
-constexpr uint32_t PIECE = 128;     // nonzeros per piece of a split row
-constexpr int CHUNK = 64;           // passes per sequence reduction
+constexpr int CHUNK = 64;             // passes per sequence reduction
+constexpr int SU = 4;                 // nonzero steps per column block
+constexpr uint32_t SEG_FLAG = 0x80000000u;
+constexpr uint32_t NO_ROW = 0xFFFFFFFFu;
 
-struct PassArgs {
-    const uint32_t *rowptr;  // nrows + 1
-    const uint32_t *col;     // packed (col | value << 28) or plain column
-    const uint32_t *val;     // plain values (general matrices)
-    uint32_t nrows;
-    const uint32_t *in;      // gathered block vector, stride B
-    uint32_t *out;           // produced block vector, stride B
-    const uint32_t *scale;   // per-row factor applied to the produced rows
-    const uint32_t *piece_row, *piece_beg, *piece_end;
-    uint32_t npieces;
-    uint32_t *piece_out;     // [npieces][B] partial row sums mod p
-    const uint32_t *long_rows, *long_pbeg;  // split rows and their piece ranges
+// Lane layout of a produced row: VW consecutive words per lane (one 4-, 8- or
+// 16-byte access), LPR lanes per row, RPW rows per wavefront.
+template <int B>
+struct Lay {
+    static constexpr int VW = B < 4 ? B : 4;
+    static constexpr int LPR = B / VW;
+    static constexpr int RPW = 64 / LPR;
+};
+
+template <int VW>
+__device__ __forceinline__ void vload(const uint32_t *src, uint32_t (&x)[VW]) {
+    if constexpr (VW == 4) {
+        const uint4 t = *(const uint4 *)src;
+        x[0] = t.x, x[1] = t.y, x[2] = t.z, x[3] = t.w;
+    } else if constexpr (VW == 2) {
+        const uint2 t = *(const uint2 *)src;
+        x[0] = t.x, x[1] = t.y;
+    } else {
+        x[0] = *src;
+    }
+}
+
+// Gathers go through a buffer resource over the input block vector: a 32-bit
+// byte offset per lane, and an offset past the end (padding entries) returns
+// zeros without touching memory, so the gathers need no branches.
+constexpr uint32_t OOB = 0x80000000u;  // block vectors stay below 2^31 bytes
+
+template <int VW>
+__device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t rs, uint32_t off, uint32_t (&x)[VW]) {
+    if constexpr (VW == 4) {
+        typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+        const v4u t = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+        x[0] = t[0], x[1] = t[1], x[2] = t[2], x[3] = t[3];
+    } else if constexpr (VW == 2) {
+        typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+        const v2u t = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
+        x[0] = t[0], x[1] = t[1];
+    } else {
+        x[0] = __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
+    }
+}
+
+template <int VW>
+__device__ __forceinline__ void vstore(uint32_t *dst, const uint32_t (&x)[VW]) {
+    if constexpr (VW == 4) *(uint4 *)dst = make_uint4(x[0], x[1], x[2], x[3]);
+    else if constexpr (VW == 2) *(uint2 *)dst = make_uint2(x[0], x[1]);
+    else *dst = x[0];
+}
+
+// One operand (A or A^T) in sliced form.  A slice is RPW units processed side
+// by side by the RPW row groups of one wavefront, SU nonzeros of each unit per
+// column block.  Row slices hold RPW rows (length-sorted, so padding is
+// small); segment slices hold RPW consecutive chunks of one long row, whose
+// sum is one partial of that row.  Each wavefront owns a contiguous stream of
+// slices (balanced by longest-processing-time assignment) and of their column
+// blocks: block k of a slice is RPW x SU words, group g's SU words contiguous.
+struct SpmmArgs {
+    const uint32_t *meta;      // per slice: end block in its wavefront's stream | SEG_FLAG
+    const uint2 *unit;         // per slice x RPW: {row, scale[row]} (NO_ROW: none); segment slices: [0].x = segment
+    const uint32_t *wave_sl;   // per wavefront: first slice (nw + 1)
+    const uint32_t *wave_blk;  // per wavefront: first column block (nw + 1); streams end with 2 zero blocks
+    const uint32_t *col;       // packed (col | value << 28) or plain column, 0 value = padding
+    const uint32_t *val;       // plain values (general matrices), 0 = padding
+    const uint32_t *in;        // gathered block vector, stride B
+    uint32_t in_bytes;         // its size (< 2^31)
+    uint32_t *out;             // produced block vector, stride B
+    const uint32_t *scale;     // per-row factor (split rows, in the fix-up)
+    uint32_t *seg_out;         // [segments][B] partial sums of split rows
+    const uint32_t *long_rows, *long_sbeg;  // split rows and their segment ranges
     uint32_t nlong;
-    uint32_t *gram;          // this pass's partials [gridDim.x (+ fixup grid)][P]
-    uint32_t fix_words;      // words of the fix-up partials after the main grid's
+    uint32_t *gram;            // this pass's partials [gridDim.x (+ fix-up grid)][P]
+    uint32_t fix_words;        // words of the fix-up partials after the main grid's
     uint32_t p;
     double pd, inv;
     uint32_t b;
 };
 
-// ---------------------------------------------------------------------------
-// One sparse pass: pieces of split rows, then every short row (product,
-// scaling, store, projection).
+// One sparse pass: every slice of this wavefront's stream, then the
+// workgroup's projection partial.  Column blocks are loaded two blocks ahead
+// and gathered one block ahead of the block being accumulated, across slice
+// boundaries, so a wavefront keeps SU row gathers per row group plus a column
+// load in flight while it computes.
 template <int B, bool PACKED>
-__global__ __launch_bounds__(WG) void k_pass(PassArgs a) {
-    constexpr int G = 64 / B;
-    __shared__ uint32_t lds[(WG / 64) * B * (B / 2 + 1)];
-    __shared__ WaveLds wl[WG / 64];
+__global__ __launch_bounds__(WG) void k_spmm(SpmmArgs a) {
+    using L = Lay<B>;
+    constexpr int VW = L::VW, LPR = L::LPR, RPW = L::RPW, BW = RPW * SU;
+    __shared__ uint32_t o2s[WG / 64][RPW * 2 * B];  // produced rows, each twice
+    __shared__ uint32_t ss[WG / 64][RPW * B];       // their scaled rows
+    __shared__ uint32_t fin[GramRing<B>::LDS_WORDS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int slot = lane / B, j = lane & (B - 1);
-    const uint32_t gwave = blockIdx.x * (WG / 64) + wave;
-    const uint32_t nwaves = gridDim.x * (WG / 64);
-    WaveLds &w = wl[wave];
+    const int g = lane / LPR, q = lane % LPR, ci = lane % B;
+    const uint32_t gw = __builtin_amdgcn_readfirstlane(blockIdx.x * (WG / 64) + wave);
+    uint32_t sl = a.wave_sl[gw];
+    const uint32_t s_end = a.wave_sl[gw + 1];
+    const size_t wblk = a.wave_blk[gw];
+    const uint32_t nblk = a.wave_blk[gw + 1] - a.wave_blk[gw] - 2;  // without the two zero blocks
+    const uint32_t *cb = a.col + wblk * BW + g * SU;
+    const uint32_t *vb = PACKED ? nullptr : a.val + wblk * BW + g * SU;
+    uint32_t *o2 = o2s[wave], *sv = ss[wave];
 
-    for (uint32_t base = gwave * G; base < a.npieces; base += nwaves * G) {
-        const uint32_t pc = base + slot;
-        if (pc < a.npieces)
-            a.piece_out[(size_t)pc * B + j] =
-                row_dot<B, PACKED>(a.col, a.val, a.in, a.piece_beg[pc], a.piece_end[pc], j, w, a.pd, a.inv);
-    }
     if (a.nlong == 0 && blockIdx.x == 0) {  // no fix-up kernel this pass: its partials are zero
         uint32_t *fx = a.gram + (size_t)gridDim.x * (a.b * (a.b + 1) / 2);
         for (uint32_t i = threadIdx.x; i < a.fix_words; i += blockDim.x) fx[i] = 0;
     }
-    Gram<B> g;
-    g.init();
-    for (uint32_t base = gwave * G; base < a.nrows; base += nwaves * G) {
-        uint32_t r = base + slot;
-        uint32_t o = 0, s = 0;
-        if (r < a.nrows) {
-            uint32_t beg = a.rowptr[r], end = a.rowptr[r + 1];
-            if (end - beg <= LONG_ROW) {
-                o = row_dot<B, PACKED>(a.col, a.val, a.in, beg, end, j, w, a.pd, a.inv);
-                s = mulmod(o, a.scale[r], a.p, a.inv);
-                a.out[(size_t)r * B + j] = s;
+    GramRing<B> gr;
+    gr.init();
+
+    // accumulators of the current slice: packed |v| <= 8 -> int64 sums;
+    // general v = vh 2^16 + vl -> two exact fp64 sums, folded every 64 steps
+    int64_t ai[PACKED ? VW : 1];
+    double ah[PACKED ? 1 : VW], al[PACKED ? 1 : VW];
+    auto zero_acc = [&]() {
+#pragma unroll
+        for (int c = 0; c < VW; c++) {
+            if constexpr (PACKED) ai[c] = 0;
+            else ah[c] = al[c] = 0.0;
+        }
+    };
+    zero_acc();
+
+    // slice descriptors are loaded one slice ahead, so their loads are older
+    // than the gathers in flight when they are used
+    auto meta_at = [&](uint32_t i) { return i < s_end ? a.meta[i] : 0u; };
+    auto unit_at = [&](uint32_t i) { return i < s_end ? a.unit[(size_t)i * RPW + g] : make_uint2(NO_ROW, 0u); };
+    uint32_t meta = __builtin_amdgcn_readfirstlane(meta_at(sl)), meta_next = meta_at(sl + 1);
+    uint2 un = unit_at(sl), un_next = unit_at(sl + 1);
+
+    // Produces the current slice's rows (or segment partial) and its share of
+    // the projection, then moves to the next slice.
+    auto finish = [&]() {
+        uint32_t o[VW], s[VW];
+#pragma unroll
+        for (int c = 0; c < VW; c++) {
+            if constexpr (PACKED) o[c] = (uint32_t)fold((double)ai[c], a.pd, a.inv);
+            else o[c] = (uint32_t)fold(fma(fold(ah[c], a.pd, a.inv), 65536.0, fold(al[c], a.pd, a.inv)), a.pd, a.inv);
+        }
+        if (meta & SEG_FLAG) {
+#pragma unroll
+            for (int c = 0; c < VW; c++) sv[g * B + q * VW + c] = o[c];
+            wave_sync();
+            if (g == 0) {
+                const uint32_t seg = un.x;
+#pragma unroll
+                for (int c = 0; c < VW; c++) {
+                    uint32_t t = 0;
+#pragma unroll 4
+                    for (int r = 0; r < RPW; r++) t = addmod(t, sv[r * B + q * VW + c], a.p);
+                    s[c] = t;
+                }
+                vstore<VW>(a.seg_out + (size_t)seg * B + q * VW, s);
+            }
+            wave_sync();
+        } else {
+            const bool ok = un.x != NO_ROW;
+#pragma unroll
+            for (int c = 0; c < VW; c++) {
+                o[c] = ok ? o[c] : 0u;
+                s[c] = mulmod(o[c], un.y, a.p, a.inv);
+            }
+            if (ok) vstore<VW>(a.out + (size_t)un.x * B + q * VW, s);
+#pragma unroll
+            for (int c = 0; c < VW; c++) {
+                o2[g * 2 * B + q * VW + c] = o[c];
+                o2[g * 2 * B + B + q * VW + c] = o[c];
+                sv[g * B + q * VW + c] = s[c];
+            }
+            wave_sync();
+#pragma unroll 1
+            for (int r = 0; r < RPW; r++) gr.add(o2 + r * 2 * B, sv[r * B + ci], a.p, a.pd, a.inv);
+            wave_sync();
+        }
+        zero_acc();
+        sl++;
+        meta = __builtin_amdgcn_readfirstlane(meta_next);
+        meta_next = meta_at(sl + 1);
+        un = un_next;
+        un_next = unit_at(sl + 1);
+    };
+
+    // slices with no column blocks at the head of the stream (empty rows)
+    while (sl < s_end && (meta & ~SEG_FLAG) == 0) finish();
+
+    // block pipeline: cA/cB column words, xA/xB gathered rows, vA/vB values
+    uint32_t cA[SU], cB[SU], xA[SU][VW], xB[SU][VW];
+    int32_t vA[SU], vB[SU];
+    auto load_cols = [&](uint32_t (&c)[SU], uint32_t k) {
+        vload<SU>(cb + (size_t)k * BW, c);
+    };
+    const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc((void *)a.in, 0, (int)a.in_bytes, 0x00020000);
+    auto gather = [&](const uint32_t (&c)[SU], uint32_t (&x)[SU][VW], int32_t (&v)[SU], uint32_t k) {
+        uint32_t vw[SU];
+        if constexpr (!PACKED) vload<SU>(vb + (size_t)k * BW, vw);
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+            const uint32_t idx = PACKED ? (c[u] & 0x0FFFFFFFu) : c[u];
+            v[u] = PACKED ? ((int32_t)c[u] >> 28) : (int32_t)vw[u];
+            bload<VW>(rin, v[u] != 0 ? idx * (B * 4) + q * (VW * 4) : OOB, x[u]);
+        }
+    };
+    auto consume = [&](const uint32_t (&x)[SU][VW], const int32_t (&v)[SU], uint32_t k) {
+#pragma unroll
+        for (int u = 0; u < SU; u++)
+#pragma unroll
+            for (int c = 0; c < VW; c++) {
+                if constexpr (PACKED) {
+                    ai[c] += (int64_t)v[u] * (int64_t)(int32_t)x[u][c];
+                } else {
+                    const double xd = (double)x[u][c];
+                    ah[c] = fma((double)((uint32_t)v[u] >> 16), xd, ah[c]);
+                    al[c] = fma((double)((uint32_t)v[u] & 0xFFFFu), xd, al[c]);
+                }
+            }
+        if constexpr (!PACKED) {
+            if ((k & 15) == 15) {  // <= 64 steps between folds
+#pragma unroll
+                for (int c = 0; c < VW; c++) {
+                    ah[c] = fold(ah[c], a.pd, a.inv);
+                    al[c] = fold(al[c], a.pd, a.inv);
+                }
             }
         }
-        g.add(o, s, lane, w, a.pd, a.inv);
+    };
+
+    // Issue order keeps every wait short of the newest loads: the column
+    // block two ahead is requested before the gathers of the next block, and
+    // a block's gathers complete while the block before it is accumulated.
+    // Every stream ends with two zero blocks, so the look-ahead needs no
+    // bounds tests (zero entries gather nothing).  Slices hold an even number
+    // of blocks, so they end only after the second block of a pair.
+    if (nblk > 0) {
+        load_cols(cA, 0);
+        load_cols(cB, 1);
+        gather(cA, xA, vA, 0);
     }
-    g.finish(a.gram + (size_t)blockIdx.x * (a.b * (a.b + 1) / 2), a.b, a.p, a.pd, a.inv, lds);
+    for (uint32_t k = 0; k < nblk; k += 2) {
+        load_cols(cA, k + 2);
+        gather(cB, xB, vB, k + 1);
+        consume(xA, vA, k);
+        load_cols(cB, k + 3);
+        gather(cA, xA, vA, k + 2);
+        consume(xB, vB, k + 1);
+        // slices ending after this pair (and empty ones right behind them)
+        while (sl < s_end && (meta & ~SEG_FLAG) <= k + 2) finish();
+    }
+    __syncthreads();
+    gr.finish(a.gram + (size_t)blockIdx.x * (a.b * (a.b + 1) / 2), a.b, a.p, a.pd, a.inv, fin);
 }
 
 // Split rows, one workgroup per row at a time: every row slot of the workgroup
-// sums a strided share of the row's pieces, the shares meet in LDS, and
-// wavefront 0 scales, stores and projects the row.  Partials go after the
+// sums a strided share of the row's segment partials, the shares meet in LDS,
+// and wavefront 0 scales, stores and projects the row.  Partials go after the
 // main kernel's (gram + (main_grid + blockIdx.x) * P).
 template <int B>
-__global__ __launch_bounds__(WG) void k_fixup(PassArgs a, uint32_t main_grid) {
+__global__ __launch_bounds__(WG) void k_fixup(SpmmArgs a, uint32_t main_grid) {
     constexpr int G = 64 / B, NS = (WG / 64) * G;
-    __shared__ uint32_t lds[(WG / 64) * B * (B / 2 + 1)];
+    __shared__ uint32_t lds[Gram<B>::LDS_WORDS];
     __shared__ WaveLds wl[WG / 64];
     __shared__ uint32_t red[WG];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -115,10 +306,17 @@ __global__ __launch_bounds__(WG) void k_fixup(PassArgs a, uint32_t main_grid) {
     Gram<B> g;
     g.init();
     for (uint32_t l = blockIdx.x; l < a.nlong; l += gridDim.x) {
-        const uint32_t r = a.long_rows[l], pb = a.long_pbeg[l], pe = a.long_pbeg[l + 1];
-        uint32_t part = 0;
-        for (uint32_t pc = pb + gs; pc < pe; pc += NS) part = addmod(part, a.piece_out[(size_t)pc * B + j], a.p);
-        red[threadIdx.x] = part;
+        const uint32_t r = a.long_rows[l], pb = a.long_sbeg[l], pe = a.long_sbeg[l + 1];
+        uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // < 2^31 per term
+        uint32_t pc = pb + gs;
+        for (; pc + 3 * NS < pe; pc += 4 * NS) {
+            s0 += a.seg_out[(size_t)pc * B + j];
+            s1 += a.seg_out[(size_t)(pc + NS) * B + j];
+            s2 += a.seg_out[(size_t)(pc + 2 * NS) * B + j];
+            s3 += a.seg_out[(size_t)(pc + 3 * NS) * B + j];
+        }
+        for (; pc < pe; pc += NS) s0 += a.seg_out[(size_t)pc * B + j];
+        red[threadIdx.x] = (uint32_t)((s0 + s1 + s2 + s3) % a.p);
         __syncthreads();
         if (wave == 0) {
             uint32_t o = 0, s = 0;
@@ -139,7 +337,7 @@ template <int B>
 __global__ __launch_bounds__(WG) void k_gram(const uint32_t *x, uint32_t nrows, uint32_t *gram, uint32_t b,
                                              uint32_t p, double pd, double inv) {
     constexpr int G = 64 / B;
-    __shared__ uint32_t lds[(WG / 64) * B * (B / 2 + 1)];
+    __shared__ uint32_t lds[Gram<B>::LDS_WORDS];
     __shared__ WaveLds wl[WG / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = lane / B, j = lane & (B - 1);
@@ -178,10 +376,10 @@ __global__ void k_scale_rows(const uint32_t *in, const uint32_t *d, uint32_t *ou
 // ---------------------------------------------------------------------------
 // host side
 
-struct SpMat {  // one operand (A or A^T) on the device
-    uint32_t nrows = 0;
-    DevBuf rowptr, col, val, piece_row, piece_beg, piece_end, piece_out, long_rows, long_pbeg;
-    uint32_t npieces = 0, nlong = 0;
+struct SliceMat {  // one operand (A or A^T) on the device, in sliced form
+    uint32_t nrows = 0, nslices = 0, nseg = 0, nlong = 0;
+    uint64_t blocks = 0;
+    DevBuf meta, unit, wave_sl, wave_blk, col, val, seg_out, long_rows, long_sbeg;
 };
 
 }  // namespace rr
@@ -191,7 +389,7 @@ struct rr_krylov {
     uint64_t m = 0, n = 0, nnz = 0;
     bool packed = false;
     rr::ModP mp;
-    rr::SpMat A, At;
+    rr::SliceMat A, At;
     rr::DevBuf dr, dc, dc2, dcinv, z, y, xtmp, part, seq;
     uint64_t seq_cap = 0;  // entries the device sequence buffer holds
     uint32_t grid = 0, fix_grid = 0, P = 0;
@@ -226,57 +424,157 @@ static rr_status upload(DevBuf &buf, const T *host, size_t count) {
     return RR_OK;
 }
 
-// Builds the device operand from host CSR (row pointers already validated).
-static rr_status build_operand(SpMat &M, uint64_t nrows, const std::vector<uint32_t> &rp,
-                               const std::vector<uint32_t> &colw, const std::vector<uint32_t> &vals, bool packed,
-                               uint32_t B) {
+// Lays out one operand for k_spmm over nw wavefronts (rows validated, values
+// packed or plain).  Rows up to long_row nonzeros become units of row slices,
+// sorted by length so the RPW rows of a slice need about the same number of
+// blocks (rounded up to an even count); longer rows become segment slices.  Slices are dealt to wavefronts
+// longest first, each to the least loaded one, and every wavefront's slices
+// and column blocks are stored contiguously in that order.
+static rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> &rp,
+                              const std::vector<uint32_t> &colw, const std::vector<uint32_t> &vals, bool packed,
+                              uint32_t B, uint32_t nw, const uint32_t *scale) {
+    const uint32_t VW = B < 4 ? B : 4, LPR = B / VW, RPW = 64 / LPR, BW = RPW * SU;
+    const uint64_t nnz = rp[nrows];
+    constexpr uint32_t FIN = 8;  // cost of producing a slice, in steps
+    auto up = [](uint64_t x, uint64_t m) { return (x + m - 1) / m * m; };
+    const uint64_t per_wave = (nnz / RPW + (nrows / RPW + 1) * FIN) / std::max<uint32_t>(nw, 1) + 1;
+    const uint32_t long_row = (uint32_t)up(std::max<uint64_t>(64, std::min<uint64_t>(per_wave, 4096)), SU);
+    const uint32_t chunk = (uint32_t)up(std::max<uint32_t>(16, long_row / 2), SU);
     M.nrows = (uint32_t)nrows;
-    std::vector<uint32_t> prow, pbeg, pend, lrows, lpbeg;
+
+    std::vector<uint32_t> shortr, lrows, lsbeg;
     for (uint64_t r = 0; r < nrows; r++) {
-        uint32_t beg = rp[r], end = rp[r + 1];
-        if (end - beg > LONG_ROW) {
-            lrows.push_back((uint32_t)r);
-            lpbeg.push_back((uint32_t)prow.size());
-            for (uint32_t e = beg; e < end; e += PIECE) {
-                prow.push_back((uint32_t)r);
-                pbeg.push_back(e);
-                pend.push_back(std::min(end, e + PIECE));
-            }
+        if (rp[r + 1] - rp[r] > long_row) lrows.push_back((uint32_t)r);
+        else shortr.push_back((uint32_t)r);
+    }
+    std::stable_sort(shortr.begin(), shortr.end(),
+                     [&](uint32_t x, uint32_t y) { return rp[x + 1] - rp[x] > rp[y + 1] - rp[y]; });
+    struct Sl {
+        uint32_t nblk;
+        uint32_t seg;    // segment id, or NO_ROW for a row slice
+        uint32_t first;  // row slices: index into shortr; segments: first nonzero
+        uint32_t end;    // segments: end of the segment's nonzeros
+    };
+    std::vector<Sl> sls;
+    for (size_t i = 0; i < shortr.size(); i += RPW) {
+        const uint32_t len = rp[shortr[i] + 1] - rp[shortr[i]];
+        sls.push_back(Sl{(uint32_t)up((len + SU - 1) / SU, 2), NO_ROW, (uint32_t)i, 0});
+    }
+    const uint64_t seg_nnz = (uint64_t)RPW * chunk;
+    uint32_t nseg = 0;
+    for (uint32_t r : lrows) {
+        lsbeg.push_back(nseg);
+        for (uint64_t e = rp[r]; e < rp[r + 1]; e += seg_nnz) {
+            const uint64_t se = std::min<uint64_t>(rp[r + 1], e + seg_nnz);
+            const uint64_t glen = std::min<uint64_t>(chunk, se - e);
+            sls.push_back(Sl{(uint32_t)up((glen + SU - 1) / SU, 2), nseg++, (uint32_t)e, (uint32_t)se});
         }
     }
-    lpbeg.push_back((uint32_t)prow.size());
-    M.npieces = (uint32_t)prow.size();
+    lsbeg.push_back(nseg);
+
+    // longest processing time first
+    std::vector<uint32_t> order(sls.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = (uint32_t)i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return sls[x].nblk > sls[y].nblk; });
+    typedef std::pair<uint64_t, uint32_t> Load;
+    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+    for (uint32_t w = 0; w < nw; w++) heap.push(Load(0, w));
+    std::vector<std::vector<uint32_t>> mine(nw);
+    for (uint32_t i : order) {
+        Load l = heap.top();
+        heap.pop();
+        mine[l.second].push_back(i);
+        heap.push(Load(l.first + (uint64_t)sls[i].nblk * SU + FIN, l.second));
+    }
+
+    std::vector<uint32_t> meta, wsl(nw + 1), wblk(nw + 1);
+    std::vector<uint2> unit;
+    meta.reserve(sls.size());
+    unit.reserve(sls.size() * RPW);
+    uint64_t blocks = 0;
+    for (uint32_t w = 0; w < nw; w++) {
+        wsl[w] = (uint32_t)meta.size();
+        wblk[w] = (uint32_t)blocks;
+        uint32_t end = 0;
+        for (uint32_t i : mine[w]) {
+            const Sl &S = sls[i];
+            end += S.nblk;
+            meta.push_back(end | (S.seg != NO_ROW ? SEG_FLAG : 0u));
+            for (uint32_t g = 0; g < RPW; g++) {
+                if (S.seg != NO_ROW) {
+                    unit.push_back(make_uint2(g == 0 ? S.seg : NO_ROW, 0u));
+                } else {
+                    const size_t k = (size_t)S.first + g;
+                    if (k < shortr.size()) unit.push_back(make_uint2(shortr[k], scale[shortr[k]]));
+                    else unit.push_back(make_uint2(NO_ROW, 0u));
+                }
+            }
+        }
+        blocks += end + 2;  // two zero blocks end every stream
+        RR_CHECK(blocks < (1ull << 32) / BW, RR_EINVAL, "matrix too large for the sliced layout");
+    }
+    wsl[nw] = (uint32_t)meta.size();
+    wblk[nw] = (uint32_t)blocks;
+
+    std::vector<uint32_t> cw((size_t)blocks * BW, 0u), vw(packed ? 0 : (size_t)blocks * BW, 0u);
+    size_t blk = 0;
+    for (uint32_t w = 0; w < nw; w++) {
+        for (uint32_t i : mine[w]) {
+            const Sl &S = sls[i];
+            for (uint32_t g = 0; g < RPW; g++) {
+                uint64_t beg, len;
+                if (S.seg != NO_ROW) {
+                    beg = (uint64_t)S.first + (uint64_t)g * chunk;
+                    len = beg < S.end ? std::min<uint64_t>(chunk, S.end - beg) : 0;
+                } else {
+                    const size_t k = (size_t)S.first + g;
+                    if (k >= shortr.size()) continue;
+                    beg = rp[shortr[k]];
+                    len = rp[shortr[k] + 1] - beg;
+                }
+                for (uint64_t e = 0; e < len; e++) {
+                    const size_t wpos = (blk + e / SU) * BW + g * SU + e % SU;
+                    cw[wpos] = colw[beg + e];
+                    if (!packed) vw[wpos] = vals[beg + e];
+                }
+            }
+            blk += S.nblk;
+        }
+        blk += 2;
+    }
+    M.nslices = (uint32_t)sls.size();
+    M.nseg = nseg;
     M.nlong = (uint32_t)lrows.size();
+    M.blocks = blocks;
     rr_status s;
-    if ((s = upload(M.rowptr, rp.data(), rp.size())) != RR_OK) return s;
-    if ((s = upload(M.col, colw.data(), colw.size())) != RR_OK) return s;
-    if (!packed && (s = upload(M.val, vals.data(), vals.size())) != RR_OK) return s;
-    if ((s = upload(M.piece_row, prow.data(), prow.size())) != RR_OK) return s;
-    if ((s = upload(M.piece_beg, pbeg.data(), pbeg.size())) != RR_OK) return s;
-    if ((s = upload(M.piece_end, pend.data(), pend.size())) != RR_OK) return s;
-    if ((s = M.piece_out.alloc((size_t)M.npieces * B * 4)) != RR_OK) return s;
+    if ((s = upload(M.meta, meta.data(), meta.size())) != RR_OK) return s;
+    if ((s = upload(M.unit, unit.data(), unit.size())) != RR_OK) return s;
+    if ((s = upload(M.wave_sl, wsl.data(), wsl.size())) != RR_OK) return s;
+    if ((s = upload(M.wave_blk, wblk.data(), wblk.size())) != RR_OK) return s;
+    if ((s = upload(M.col, cw.data(), cw.size())) != RR_OK) return s;
+    if (!packed && (s = upload(M.val, vw.data(), vw.size())) != RR_OK) return s;
+    if ((s = M.seg_out.alloc((size_t)nseg * B * 4)) != RR_OK) return s;
     if ((s = upload(M.long_rows, lrows.data(), lrows.size())) != RR_OK) return s;
-    if ((s = upload(M.long_pbeg, lpbeg.data(), lpbeg.size())) != RR_OK) return s;
+    if ((s = upload(M.long_sbeg, lsbeg.data(), lsbeg.size())) != RR_OK) return s;
     return RR_OK;
 }
 
-static PassArgs make_args(rr_krylov *c, SpMat &M, const uint32_t *in, uint32_t *out, const uint32_t *scale,
+static SpmmArgs make_args(rr_krylov *c, SliceMat &M, const uint32_t *in, uint32_t *out, const uint32_t *scale,
                           uint32_t *gram) {
-    PassArgs a;
-    a.rowptr = M.rowptr.as<uint32_t>();
+    SpmmArgs a;
+    a.meta = M.meta.as<uint32_t>();
+    a.unit = M.unit.as<uint2>();
+    a.wave_sl = M.wave_sl.as<uint32_t>();
+    a.wave_blk = M.wave_blk.as<uint32_t>();
     a.col = M.col.as<uint32_t>();
     a.val = M.val.as<uint32_t>();
-    a.nrows = M.nrows;
     a.in = in;
+    a.in_bytes = (uint32_t)((in == c->z.as<uint32_t>() ? c->n : c->m) * c->B * 4);
     a.out = out;
     a.scale = scale;
-    a.piece_row = M.piece_row.as<uint32_t>();
-    a.piece_beg = M.piece_beg.as<uint32_t>();
-    a.piece_end = M.piece_end.as<uint32_t>();
-    a.npieces = M.npieces;
-    a.piece_out = M.piece_out.as<uint32_t>();
+    a.seg_out = M.seg_out.as<uint32_t>();
     a.long_rows = M.long_rows.as<uint32_t>();
-    a.long_pbeg = M.long_pbeg.as<uint32_t>();
+    a.long_sbeg = M.long_sbeg.as<uint32_t>();
     a.nlong = M.nlong;
     a.gram = gram;
     a.fix_words = c->fix_grid * c->P;
@@ -288,18 +586,18 @@ static PassArgs make_args(rr_krylov *c, SpMat &M, const uint32_t *in, uint32_t *
 }
 
 template <int B>
-static void launch_pass(rr_krylov *c, const PassArgs &a) {
+static void launch_pass(rr_krylov *c, const SpmmArgs &a) {
     if (c->packed)
-        hipLaunchKernelGGL((k_pass<B, true>), dim3(c->grid), dim3(WG), 0, c->stream, a);
+        hipLaunchKernelGGL((k_spmm<B, true>), dim3(c->grid), dim3(WG), 0, c->stream, a);
     else
-        hipLaunchKernelGGL((k_pass<B, false>), dim3(c->grid), dim3(WG), 0, c->stream, a);
+        hipLaunchKernelGGL((k_spmm<B, false>), dim3(c->grid), dim3(WG), 0, c->stream, a);
 }
 
 template <int B>
 static int pass_blocks_per_cu(bool packed) {
     int n = 0;
-    hipError_t e = packed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass<B, true>, WG, 0)
-                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass<B, false>, WG, 0);
+    hipError_t e = packed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_spmm<B, true>, WG, 0)
+                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_spmm<B, false>, WG, 0);
     return (e == hipSuccess && n > 0) ? n : 2;
 }
 
@@ -315,7 +613,7 @@ static int pass_blocks_per_cu_any(uint32_t B, bool packed) {
     }
 }
 
-static void launch_pass_any(rr_krylov *c, const PassArgs &a) {
+static void launch_pass_any(rr_krylov *c, const SpmmArgs &a) {
     switch (c->B) {
         case 1: launch_pass<1>(c, a); break;
         case 2: launch_pass<2>(c, a); break;
@@ -327,7 +625,7 @@ static void launch_pass_any(rr_krylov *c, const PassArgs &a) {
     }
 }
 
-static void launch_fixup_any(rr_krylov *c, const PassArgs &a) {
+static void launch_fixup_any(rr_krylov *c, const SpmmArgs &a) {
     dim3 g(c->fix_grid), t(WG);
     switch (c->B) {
         case 1: hipLaunchKernelGGL(k_fixup<1>, g, t, 0, c->stream, a, c->grid); break;
@@ -378,6 +676,13 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
     RR_CHECK(b >= 1 && b <= 64, RR_EINVAL, "b=%u must be in 1..64", b);
     RR_CHECK(A->m >= 1 && A->n >= 1 && A->m < (1ull << 31) && A->n < (1ull << 31), RR_EINVAL,
              "matrix %llux%llu out of range (m, n < 2^31)", (unsigned long long)A->m, (unsigned long long)A->n);
+    {
+        uint64_t Bp = 1;
+        while (Bp < b) Bp <<= 1;
+        RR_CHECK(std::max(A->m, A->n) * Bp * 4 < (1ull << 31), RR_EINVAL,
+                 "block vectors of %llu rows x %llu words exceed 2^31 bytes",
+                 (unsigned long long)std::max(A->m, A->n), (unsigned long long)Bp);
+    }
     RR_CHECK(t_have == 0 || (t_have & 1), RR_EINVAL, "t_have=%llu must be 0 or odd", (unsigned long long)t_have);
     const uint64_t m = A->m, n = A->n;
     RR_CHECK(A->row_ptr && A->row_ptr[0] == 0, RR_EINVAL, "row_ptr[0] must be 0");
@@ -448,18 +753,18 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
                 }
             }
     }
-    // one resident wave of workgroups: the pass kernels stride over rows
-    c->grid = (uint32_t)std::max(1, cus * pass_blocks_per_cu_any(c->B, small));
-    rr_status s;
-    if ((s = build_operand(c->A, m, rp, colw, vals, small, c->B)) != RR_OK) return s;
-    if ((s = build_operand(c->At, n, rpt, colt, valt, small, c->B)) != RR_OK) return s;
-
     // preconditioners: D_r, D_c, D_c^2, D_c^-1
     std::vector<uint32_t> dc2(n), dcinv(n);
     for (uint64_t i = 0; i < n; i++) {
         dc2[i] = h_mulmod(col_precond[i], col_precond[i], p);
         dcinv[i] = h_invmod(col_precond[i], p);
     }
+    // one resident wave of workgroups; every wavefront owns a fixed stream of slices
+    c->grid = (uint32_t)std::max(1, cus * pass_blocks_per_cu_any(c->B, small));
+    rr_status s;
+    if ((s = build_slices(c->A, m, rp, colw, vals, small, c->B, c->grid * (WG / 64), row_precond)) != RR_OK) return s;
+    if ((s = build_slices(c->At, n, rpt, colt, valt, small, c->B, c->grid * (WG / 64), dc2.data())) != RR_OK)
+        return s;
     if ((s = upload(c->dr, row_precond, m)) != RR_OK) return s;
     if ((s = upload(c->dc, col_precond, n)) != RR_OK) return s;
     if ((s = upload(c->dc2, dc2.data(), n)) != RR_OK) return s;
@@ -553,8 +858,8 @@ rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
                 }
             } else {
                 bool byA = (t & 1) == 1;
-                SpMat &M = byA ? c->A : c->At;
-                PassArgs a = byA ? make_args(c, M, c->z.as<uint32_t>(), c->y.as<uint32_t>(), c->dr.as<uint32_t>(), gram)
+                SliceMat &M = byA ? c->A : c->At;
+                SpmmArgs a = byA ? make_args(c, M, c->z.as<uint32_t>(), c->y.as<uint32_t>(), c->dr.as<uint32_t>(), gram)
                                  : make_args(c, M, c->y.as<uint32_t>(), c->z.as<uint32_t>(), c->dc2.as<uint32_t>(), gram);
                 launch_pass_any(c, a);
                 if (c->profiling) RR_HIP(hipEventRecord(e1, c->stream));
@@ -649,7 +954,3 @@ namespace rr {
 const uint32_t *krylov_device_seq(rr_krylov *c) { return c->seq.as<uint32_t>(); }
 hipStream_t krylov_stream(rr_krylov *c) { return c->stream; }
 }  // namespace rr
-
-extern "C" {
-
-}  // extern "C"

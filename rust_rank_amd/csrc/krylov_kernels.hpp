@@ -74,6 +74,49 @@ __device__ __forceinline__ uint32_t row_dot(const uint32_t *__restrict__ col, co
     else return (uint32_t)fold(fma(a0, 65536.0, a1), pd, inv);
 }
 
+// One row whose column indices (and values) are already staged in LDS at
+// tc[0 .. n): the B lanes gather and accumulate as row_dot does.
+template <int B, bool PACKED>
+__device__ __forceinline__ uint32_t row_dot_staged(const uint32_t *tc, const uint32_t *tv,
+                                                   const uint32_t *__restrict__ in, uint32_t n, int j, double pd,
+                                                   double inv) {
+    constexpr int GS = B < 16 ? B : 16;
+    double a0 = 0.0, a1 = 0.0;
+    for (uint32_t c0 = 0; c0 < n; c0 += 64) {  // <= 64 terms between folds
+        const uint32_t cn = min(64u, n - c0);
+        for (uint32_t e = 0; e < cn; e += GS) {
+            uint32_t c[GS], x[GS], v[GS];
+#pragma unroll
+            for (int k = 0; k < GS; k++) {
+                const uint32_t q = min(c0 + e + k, n - 1);
+                c[k] = tc[q];
+                if constexpr (!PACKED) v[k] = tv[q];
+            }
+#pragma unroll
+            for (int k = 0; k < GS; k++) {
+                const uint32_t idx = PACKED ? (c[k] & 0x0FFFFFFFu) : c[k];
+                x[k] = (e + k < cn) ? in[(size_t)idx * B + j] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < GS; k++) {
+                if constexpr (PACKED) {
+                    const double t = (double)((int32_t)c[k] >> 28);
+                    if (k & 1) a1 = fma(t, (double)x[k], a1);
+                    else a0 = fma(t, (double)x[k], a0);
+                } else {
+                    const double xd = (double)x[k];
+                    a0 = fma((double)(v[k] >> 16), xd, a0);
+                    a1 = fma((double)(v[k] & 0xFFFFu), xd, a1);
+                }
+            }
+        }
+        a0 = fold(a0, pd, inv);
+        a1 = fold(a1, pd, inv);
+    }
+    if constexpr (PACKED) return (uint32_t)fold(a0 + a1, pd, inv);
+    else return (uint32_t)fold(fma(a0, 65536.0, a1), pd, inv);
+}
+
 // ---------------------------------------------------------------------------
 // Per-lane accumulation of the symmetric projection sum_r o_r s_r^T.
 // Lane j of a row pairs its own s_j with o_{(j+k) mod B} for k = 0..B/2 (read
@@ -83,6 +126,7 @@ __device__ __forceinline__ uint32_t row_dot(const uint32_t *__restrict__ col, co
 template <int B>
 struct Gram {
     static constexpr int K = B / 2 + 1;
+    static constexpr int LDS_WORDS = (WG / 64) * B * K;
     double hi[K], lo[K];
     int rows;
 
@@ -145,6 +189,166 @@ struct Gram {
             uint32_t ii = lo_i * b - lo_i * (lo_i - 1) / 2 + (hi_i - lo_i);
             out[ii] = s;
         }
+    }
+};
+
+// x mod p for x < 2^63 (x = h 2^32 + l): fold h, multiply by 2^32 mod p, add l.
+__device__ __forceinline__ uint32_t fold_u64(uint64_t x, uint32_t p, double pd, double inv) {
+    const uint32_t h = (uint32_t)fold((double)(uint32_t)(x >> 32), pd, inv);
+    const uint32_t c32 = (uint32_t)fold(4294967296.0, pd, inv);
+    return (uint32_t)fold((double)mulmod(h, c32, p, inv) + (double)(uint32_t)x, pd, inv);
+}
+
+// ---------------------------------------------------------------------------
+// The symmetric projection with the wavefront's 64 lanes sharing ONE copy of
+// the b x b upper triangle.  Lane l owns column i = l % B and the ring offsets
+// k in [h KS, (h + 1) KS), h = l / B, of the pairs (i, (i + k) mod B),
+// k = 0 .. B/2 (k = B/2 only for i < B/2): every unordered pair exactly once.
+// Rows are added one at a time from LDS: the producing lanes store each row's
+// o twice in a row (2B words, so the ring needs no wrap-around) and s once.
+// S[i][j] = sum_r s_r[i] o_r[j]; s is split in 16-bit halves, so each product
+// is below 2^47 and 2^16 rows fit an unsigned 64-bit accumulator.
+template <int B>
+struct GramRing {
+    static constexpr int H = 64 / B;            // lanes per column
+    static constexpr int K = B / 2 + 1;         // ring offsets
+    static constexpr int KS = (K + H - 1) / H;  // ring offsets per lane
+    static constexpr int P = B * (B + 1) / 2;
+    static constexpr int LDS_WORDS = (WG / 64) * P;
+    uint64_t hi[KS], lo[KS];
+    uint32_t rows;
+
+    __device__ void init() {
+#pragma unroll
+        for (int t = 0; t < KS; t++) hi[t] = lo[t] = 0;
+        rows = 0;
+    }
+    __device__ static __forceinline__ bool owns(int i, int k) { return k < K && !(2 * k == B && i >= B / 2); }
+    // All 64 lanes call this together; o2: the row's o values twice, s_i =
+    // s[i].  Accumulators of offsets a lane does not own collect products of
+    // other pairs and are never written out, so the reads need no masking
+    // (for B >= 8 they stay inside the doubled row).
+    __device__ __forceinline__ void add(const uint32_t *o2, uint32_t s_i, uint32_t p, double pd, double inv) {
+        const int lane = threadIdx.x & 63, i = lane % B, h = lane / B;
+        const uint32_t sh = s_i >> 16, sl = s_i & 0xFFFFu;
+        const uint32_t *src = o2 + i + h * KS;
+#pragma unroll
+        for (int t = 0; t < KS; t++) {
+            uint32_t o;
+            if constexpr (H * KS > B) o = o2[i + min(h * KS + t, K - 1)];
+            else o = src[t];
+            hi[t] += (uint64_t)o * sh;
+            lo[t] += (uint64_t)o * sl;
+        }
+        if (++rows == 65536u) {
+#pragma unroll
+            for (int t = 0; t < KS; t++) {
+                hi[t] = fold_u64(hi[t], p, pd, inv);
+                lo[t] = fold_u64(lo[t], p, pd, inv);
+            }
+            rows = 0;
+        }
+    }
+    // Sums the workgroup's wavefronts and writes the b(b+1)/2 upper-triangle
+    // partial to out (row-major pair index).
+    __device__ void finish(uint32_t *out, uint32_t b, uint32_t p, double pd, double inv, uint32_t *lds) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, i = lane % B, h = lane / B;
+        const uint32_t Pb = b * (b + 1) / 2;
+#pragma unroll
+        for (int t = 0; t < KS; t++) {
+            const int k = h * KS + t;
+            const int j = (i + k) % B;
+            if (owns(i, k) && (uint32_t)i < b && (uint32_t)j < b) {
+                const uint32_t v = addmod(mulmod(fold_u64(hi[t], p, pd, inv), 65536u, p, inv),
+                                          fold_u64(lo[t], p, pd, inv), p);
+                const uint32_t lo_i = (uint32_t)min(i, j), hi_i = (uint32_t)max(i, j);
+                lds[wave * Pb + lo_i * b - lo_i * (lo_i - 1) / 2 + (hi_i - lo_i)] = v;
+            }
+        }
+        __syncthreads();
+        const int nw = blockDim.x >> 6;
+        for (uint32_t t = threadIdx.x; t < Pb; t += blockDim.x) {
+            uint32_t sum = 0;
+            for (int w = 0; w < nw; w++) sum = addmod(sum, lds[w * Pb + t], p);
+            out[t] = sum;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Exact projection on the matrix cores, for lane width 32 (b = 17..32).
+// Each value x in [0, p) is replaced by its signed representative
+// x' in (-p/2, p/2) and written in balanced base 256, x' = sum_a d_a 256^a
+// with d_a in [-128, 127]; the four digits are the bytes of
+// (x' + 0x80808080) ^ 0x80808080.  Then
+//   sum_r o_r s_r^T = sum_{a,b} 256^(a+b) sum_r d_a(o_r) d_b(s_r)^T,
+// and every inner sum is an int8 matrix product accumulated in int32 by
+// v_mfma_i32_32x32x16_i8 (A 32 x 16: lane l holds A[l % 32][8 (l / 32) + byte];
+// B 16 x 32: lane l holds B[8 (l / 32) + byte][l % 32]; C 32 x 32: lane l,
+// register v holds C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32]).  In the pass,
+// lane l owns column l % 32 of a row, so eight successive rows of a lane are
+// exactly its operand bytes: no data moves between lanes.  Seven accumulators
+// hold the weights 256^0 .. 256^6; |digit products| <= 2^14, so 2^16 rows fit
+// int32 before the flush into residues mod p that happens every 16384 rows.
+typedef int mfma_v16i __attribute__((ext_vector_type(16)));
+
+struct GramMfma32 {
+    mfma_v16i acc[7];
+    uint64_t qa[4], qb[4];  // operand bytes being assembled: row t of the block in byte t
+    uint32_t res[16];       // flushed residues mod p of sum_c 256^c acc[c]
+    int nbuf, blocks;
+    uint32_t p;
+    double inv;
+
+    __device__ void init_p(uint32_t p_, double inv_) {
+#pragma unroll
+        for (int c = 0; c < 7; c++) acc[c] = mfma_v16i{0};
+#pragma unroll
+        for (int v = 0; v < 16; v++) res[v] = 0;
+#pragma unroll
+        for (int a = 0; a < 4; a++) qa[a] = qb[a] = 0;
+        nbuf = 0;
+        blocks = 0;
+        p = p_;
+        inv = inv_;
+    }
+    __device__ static __forceinline__ uint32_t digits(uint32_t x, uint32_t p) {
+        int32_t xs = x > (p >> 1) ? (int32_t)x - (int32_t)p : (int32_t)x;
+        return ((uint32_t)xs + 0x80808080u) ^ 0x80808080u;
+    }
+    __device__ void flush() {
+        // res += sum_c 256^c acc[c] mod p, then acc = 0
+        const double pd = (double)p;
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            uint32_t t = res[v], wgt = 1;
+#pragma unroll
+            for (int c = 0; c < 7; c++) {
+                const uint32_t a = (uint32_t)fold((double)acc[c][v], pd, inv);  // |acc| < 2^31
+                t = addmod(t, mulmod(a, wgt, p, inv), p);
+                wgt = mulmod(wgt, 256u, p, inv);
+            }
+            res[v] = t;
+        }
+#pragma unroll
+        for (int c = 0; c < 7; c++) acc[c] = mfma_v16i{0};
+        blocks = 0;
+    }
+    __device__ __forceinline__ void add(uint32_t o, uint32_t s) {
+        const uint32_t wo = digits(o, p), ws = digits(s, p);
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            qa[a] = (qa[a] >> 8) | ((uint64_t)((wo >> (8 * a)) & 0xFFu) << 56);
+            qb[a] = (qb[a] >> 8) | ((uint64_t)((ws >> (8 * a)) & 0xFFu) << 56);
+        }
+        if (++nbuf < 8) return;
+        nbuf = 0;
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                acc[a + b] = __builtin_amdgcn_mfma_i32_32x32x16_i8((long)qa[a], (long)qb[b], acc[a + b], 0, 0, 0);
+        if (++blocks == 1024) flush();
     }
 };
 
