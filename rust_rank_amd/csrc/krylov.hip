@@ -24,7 +24,8 @@
 // RPW length-sorted rows, column indices in blocks of SU = 4 per row, and
 // every wavefront owns a contiguous, load-balanced stream of slices.  Rows
 // longer than the balance target (the Zipf head of A^T) are cut into
-// segments whose partial sums a small fix-up kernel combines.  Each
+// segments whose partial sums are added atomically into one 64-bit
+// accumulator row per split row; a small fix-up kernel finishes those rows.  Each
 // workgroup writes one partial b x b upper triangle per pass; a reduction
 // kernel folds the partials of a chunk of passes into the sequence.
 #include "common.hpp"
@@ -41,6 +42,8 @@ constexpr int CHUNK = 64;             // passes per sequence reduction
 constexpr int SU = 4;                 // nonzero steps per column block
 constexpr uint32_t SEG_FLAG = 0x80000000u;
 constexpr uint32_t NO_ROW = 0xFFFFFFFFu;
+constexpr uint32_t MAX_COPIES = 32;  // accumulator rows per split row
+constexpr uint32_t NXCD = 8;         // XCDs of an MI355X, each with its own L2
 
 // Lane layout of a produced row: VW consecutive words per lane (one 4-, 8- or
 // 16-byte access), LPR lanes per row, RPW rows per wavefront.
@@ -100,7 +103,7 @@ __device__ __forceinline__ void vstore(uint32_t *dst, const uint32_t (&x)[VW]) {
 // blocks: block k of a slice is RPW x SU words, group g's SU words contiguous.
 struct SpmmArgs {
     const uint32_t *meta;      // per slice: end block in its wavefront's stream | SEG_FLAG
-    const uint2 *unit;         // per slice x RPW: {row, scale[row]} (NO_ROW: none); segment slices: [0].x = segment
+    const uint2 *unit;         // per slice x RPW: {row, scale[row]} (NO_ROW: none); segment slices: [0].x = accumulator row
     const uint32_t *wave_sl;   // per wavefront: first slice (nw + 1)
     const uint32_t *wave_blk;  // per wavefront: first column block (nw + 1); streams end with 2 zero blocks
     const uint32_t *col;       // packed (col | value << 28) or plain column, 0 value = padding
@@ -109,9 +112,10 @@ struct SpmmArgs {
     uint32_t in_bytes;         // its size (< 2^31)
     uint32_t *out;             // produced block vector, stride B
     const uint32_t *scale;     // per-row factor (split rows, in the fix-up)
-    uint32_t *seg_out;         // [segments][B] partial sums of split rows
-    const uint32_t *long_rows, *long_sbeg;  // split rows and their segment ranges
-    uint32_t nlong;
+    unsigned long long *lacc;  // [accumulator rows][B] sums of segment partials (zero between passes)
+    uint32_t lacc_bytes;       // its size (< 2^31)
+    const uint4 *linfo;        // per split row: {row, first accumulator row, accumulator rows, segments}
+    uint32_t nlong;            // split rows
     uint32_t *gram;            // this pass's partials [gridDim.x (+ fix-up grid)][P]
     uint32_t fix_words;        // words of the fix-up partials after the main grid's
     uint32_t p;
@@ -125,7 +129,7 @@ struct SpmmArgs {
 // boundaries, so a wavefront keeps SU row gathers per row group plus a column
 // load in flight while it computes.
 template <int B, bool PACKED>
-__global__ __launch_bounds__(WG) void k_spmm(SpmmArgs a) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_spmm(SpmmArgs a) {
     using L = Lay<B>;
     constexpr int VW = L::VW, LPR = L::LPR, RPW = L::RPW, BW = RPW * SU;
     __shared__ uint32_t o2s[WG / 64][RPW * 2 * B];  // produced rows, each twice
@@ -169,8 +173,9 @@ __global__ __launch_bounds__(WG) void k_spmm(SpmmArgs a) {
     uint32_t meta = __builtin_amdgcn_readfirstlane(meta_at(sl)), meta_next = meta_at(sl + 1);
     uint2 un = unit_at(sl), un_next = unit_at(sl + 1);
 
-    // Produces the current slice's rows (or segment partial) and its share of
-    // the projection, then moves to the next slice.
+    // Produces the current slice's rows (or segment partial), leaves the rows
+    // in LDS for the projection, and moves to the next slice.
+    bool rows_ready = false;
     auto finish = [&]() {
         uint32_t o[VW], s[VW];
 #pragma unroll
@@ -182,16 +187,15 @@ __global__ __launch_bounds__(WG) void k_spmm(SpmmArgs a) {
 #pragma unroll
             for (int c = 0; c < VW; c++) sv[g * B + q * VW + c] = o[c];
             wave_sync();
-            if (g == 0) {
-                const uint32_t seg = un.x;
+            if (g == 0) {  // one partial of the split row, added into its accumulator
+                unsigned long long *dst = a.lacc + (size_t)un.x * B + q * VW;
 #pragma unroll
                 for (int c = 0; c < VW; c++) {
                     uint32_t t = 0;
 #pragma unroll 4
                     for (int r = 0; r < RPW; r++) t = addmod(t, sv[r * B + q * VW + c], a.p);
-                    s[c] = t;
+                    atomicAdd(dst + c, (unsigned long long)t);
                 }
-                vstore<VW>(a.seg_out + (size_t)seg * B + q * VW, s);
             }
             wave_sync();
         } else {
@@ -208,10 +212,7 @@ __global__ __launch_bounds__(WG) void k_spmm(SpmmArgs a) {
                 o2[g * 2 * B + B + q * VW + c] = o[c];
                 sv[g * B + q * VW + c] = s[c];
             }
-            wave_sync();
-#pragma unroll 1
-            for (int r = 0; r < RPW; r++) gr.add(o2 + r * 2 * B, sv[r * B + ci], a.p, a.pd, a.inv);
-            wave_sync();
+            rows_ready = true;
         }
         zero_acc();
         sl++;
@@ -221,33 +222,43 @@ __global__ __launch_bounds__(WG) void k_spmm(SpmmArgs a) {
         un_next = unit_at(sl + 1);
     };
 
-    // slices with no column blocks at the head of the stream (empty rows)
+    // slices with no column blocks at the head of the stream (empty rows,
+    // nothing to project)
     while (sl < s_end && (meta & ~SEG_FLAG) == 0) finish();
 
     // block pipeline: cA/cB column words, xA/xB gathered rows, vA/vB values
+    // (packed matrices: four 4-bit values in vA[0] / vB[0])
+    constexpr int NV = PACKED ? 1 : SU;
     uint32_t cA[SU], cB[SU], xA[SU][VW], xB[SU][VW];
-    int32_t vA[SU], vB[SU];
+    int32_t vA[NV], vB[NV];
     auto load_cols = [&](uint32_t (&c)[SU], uint32_t k) {
         vload<SU>(cb + (size_t)k * BW, c);
     };
     const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc((void *)a.in, 0, (int)a.in_bytes, 0x00020000);
-    auto gather = [&](const uint32_t (&c)[SU], uint32_t (&x)[SU][VW], int32_t (&v)[SU], uint32_t k) {
+    auto gather = [&](const uint32_t (&c)[SU], uint32_t (&x)[SU][VW], int32_t (&v)[NV], uint32_t k) {
         uint32_t vw[SU];
         if constexpr (!PACKED) vload<SU>(vb + (size_t)k * BW, vw);
+        else v[0] = 0;
 #pragma unroll
         for (int u = 0; u < SU; u++) {
-            const uint32_t idx = PACKED ? (c[u] & 0x0FFFFFFFu) : c[u];
-            v[u] = PACKED ? ((int32_t)c[u] >> 28) : (int32_t)vw[u];
-            bload<VW>(rin, v[u] != 0 ? idx * (B * 4) + q * (VW * 4) : OOB, x[u]);
+            if constexpr (PACKED) {
+                const uint32_t code = c[u] >> 28;
+                v[0] |= (int32_t)(code << (4 * u));
+                bload<VW>(rin, code ? (c[u] & 0x0FFFFFFFu) * (B * 4) + q * (VW * 4) : OOB, x[u]);
+            } else {
+                v[u] = (int32_t)vw[u];
+                bload<VW>(rin, vw[u] ? c[u] * (B * 4) + q * (VW * 4) : OOB, x[u]);
+            }
         }
     };
-    auto consume = [&](const uint32_t (&x)[SU][VW], const int32_t (&v)[SU], uint32_t k) {
+    auto consume = [&](const uint32_t (&x)[SU][VW], const int32_t (&v)[NV], uint32_t k) {
 #pragma unroll
         for (int u = 0; u < SU; u++)
 #pragma unroll
             for (int c = 0; c < VW; c++) {
                 if constexpr (PACKED) {
-                    ai[c] += (int64_t)v[u] * (int64_t)(int32_t)x[u][c];
+                    const int32_t vu = (int32_t)((uint32_t)v[0] << (28 - 4 * u)) >> 28;  // signed 4-bit value
+                    ai[c] += (int64_t)vu * (int64_t)(int32_t)x[u][c];
                 } else {
                     const double xd = (double)x[u][c];
                     ah[c] = fma((double)((uint32_t)v[u] >> 16), xd, ah[c]);
@@ -284,50 +295,63 @@ __global__ __launch_bounds__(WG) void k_spmm(SpmmArgs a) {
         gather(cA, xA, vA, k + 2);
         consume(xB, vB, k + 1);
         // slices ending after this pair (and empty ones right behind them)
-        while (sl < s_end && (meta & ~SEG_FLAG) <= k + 2) finish();
+        while (sl < s_end && (meta & ~SEG_FLAG) <= k + 2) {
+            rows_ready = false;
+            finish();
+            if (rows_ready) {  // the slice's share of the projection
+                wave_sync();
+#pragma unroll 1
+                for (int r = 0; r < RPW; r++) gr.add(o2 + r * 2 * B, sv[r * B + ci], a.p, a.pd, a.inv);
+                wave_sync();
+            }
+        }
     }
     __syncthreads();
     gr.finish(a.gram + (size_t)blockIdx.x * (a.b * (a.b + 1) / 2), a.b, a.p, a.pd, a.inv, fin);
 }
 
-// Split rows, one workgroup per row at a time: every row slot of the workgroup
-// sums a strided share of the row's segment partials, the shares meet in LDS,
-// and wavefront 0 scales, stores and projects the row.  Partials go after the
-// main kernel's (gram + (main_grid + blockIdx.x) * P).
+// Split rows: the segment slices of the pass have added their partials into
+// the row's accumulator rows in lacc (spread over up to MAX_COPIES rows so
+// that few atomics meet at one address; fewer than 2^32 terms below p each).
+// Every row slot of B lanes takes one split row: sums and clears its
+// accumulator rows (all loads in flight at once), scales, stores and
+// projects the row.  Partials go after the main kernel's
+// (gram + (main_grid + blockIdx.x) * P).
 template <int B>
 __global__ __launch_bounds__(WG) void k_fixup(SpmmArgs a, uint32_t main_grid) {
     constexpr int G = 64 / B, NS = (WG / 64) * G;
     __shared__ uint32_t lds[Gram<B>::LDS_WORDS];
     __shared__ WaveLds wl[WG / 64];
-    __shared__ uint32_t red[WG];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = lane / B, j = lane & (B - 1);
-    const int gs = wave * G + slot;
+    const __amdgpu_buffer_rsrc_t racc = __builtin_amdgcn_make_buffer_rsrc((void *)a.lacc, 0, (int)a.lacc_bytes, 0x00020000);
     Gram<B> g;
     g.init();
-    for (uint32_t l = blockIdx.x; l < a.nlong; l += gridDim.x) {
-        const uint32_t r = a.long_rows[l], pb = a.long_sbeg[l], pe = a.long_sbeg[l + 1];
-        uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // < 2^31 per term
-        uint32_t pc = pb + gs;
-        for (; pc + 3 * NS < pe; pc += 4 * NS) {
-            s0 += a.seg_out[(size_t)pc * B + j];
-            s1 += a.seg_out[(size_t)(pc + NS) * B + j];
-            s2 += a.seg_out[(size_t)(pc + 2 * NS) * B + j];
-            s3 += a.seg_out[(size_t)(pc + 3 * NS) * B + j];
-        }
-        for (; pc < pe; pc += NS) s0 += a.seg_out[(size_t)pc * B + j];
-        red[threadIdx.x] = (uint32_t)((s0 + s1 + s2 + s3) % a.p);
-        __syncthreads();
-        if (wave == 0) {
-            uint32_t o = 0, s = 0;
-            if (slot == 0) {
-                for (int q = 0; q < NS; q++) o = addmod(o, red[q * B + j], a.p);
-                s = mulmod(o, a.scale[r], a.p, a.inv);
-                a.out[(size_t)r * B + j] = s;
+    for (uint32_t r0 = blockIdx.x * NS; r0 < a.nlong; r0 += gridDim.x * NS) {
+        const uint32_t l = r0 + wave * G + slot;
+        uint32_t o = 0, s = 0;
+        if (l < a.nlong) {
+            const uint4 li = a.linfo[l];
+            const uint32_t sc = a.scale[li.x];
+            // buffer accesses past the end read zeros and write nothing, so
+            // all MAX_COPIES loads and stores issue without branches
+            unsigned long long t = 0;  // < 2^63
+            const uint32_t base = (li.y * B + j) * 8u;
+            typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+#pragma unroll
+            for (uint32_t q = 0; q < MAX_COPIES; q++) {
+                const v2u w = __builtin_amdgcn_raw_buffer_load_b64(racc, q < li.z ? base + q * (B * 8u) : OOB, 0, 0);
+                t += ((unsigned long long)w[1] << 32) | w[0];
             }
-            g.add(o, s, lane, wl[0], a.pd, a.inv);
+            const v2u zero = {0u, 0u};
+#pragma unroll
+            for (uint32_t q = 0; q < MAX_COPIES; q++)
+                __builtin_amdgcn_raw_buffer_store_b64(zero, racc, q < li.z ? base + q * (B * 8u) : OOB, 0, 0);
+            o = fold_u64(t, a.p, a.pd, a.inv);
+            s = mulmod(o, sc, a.p, a.inv);
+            a.out[(size_t)li.x * B + j] = s;
         }
-        __syncthreads();
+        g.add(o, s, lane, wl[wave], a.pd, a.inv);
     }
     g.finish(a.gram + (size_t)(main_grid + blockIdx.x) * (a.b * (a.b + 1) / 2), a.b, a.p, a.pd, a.inv, lds);
 }
@@ -353,16 +377,34 @@ __global__ __launch_bounds__(WG) void k_gram(const uint32_t *x, uint32_t nrows, 
     g.finish(gram + (size_t)blockIdx.x * (b * (b + 1) / 2), b, p, pd, inv, lds);
 }
 
-// seq[t][ii] = sum_w part[t][w][ii] mod p
-__global__ void k_reduce(const uint32_t *part, uint32_t nparts, uint32_t P, uint32_t count, uint32_t *seq,
-                         uint32_t p) {
-    uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= count * P) return;
-    uint32_t t = idx / P, ii = idx % P;
+// seq[t][ii] = sum_w part[t][w][ii] mod p.  Workgroup (ii block of 64, t):
+// lane -> ii (coalesced rows of the partial matrix), the 16 wavefronts take
+// every 16th partial with eight loads in flight each, then meet in LDS.
+constexpr int RED_WG = 1024;
+__global__ __launch_bounds__(RED_WG) void k_reduce(const uint32_t *part, uint32_t nparts, uint32_t P,
+                                                   uint32_t *seq, uint32_t p) {
+    __shared__ uint64_t acc[RED_WG / 64][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int NWV = RED_WG / 64;
+    const uint32_t ii = blockIdx.x * 64 + lane, t = blockIdx.y;
     const uint32_t *src = part + (size_t)t * nparts * P + ii;
-    uint64_t s = 0;
-    for (uint32_t w = 0; w < nparts; w++) s += src[(size_t)w * P];
-    seq[idx] = (uint32_t)(s % p);
+    uint64_t s[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // each term < 2^31
+    if (ii < P) {
+        uint32_t w = wave;
+        for (; w + 7 * NWV < nparts; w += 8 * NWV) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) s[u] += src[(size_t)(w + u * NWV) * P];
+        }
+        for (; w < nparts; w += NWV) s[0] += src[(size_t)w * P];
+    }
+    acc[wave][lane] = s[0] + s[1] + s[2] + s[3] + s[4] + s[5] + s[6] + s[7];
+    __syncthreads();
+    if (wave == 0 && ii < P) {
+        uint64_t tot = 0;
+#pragma unroll
+        for (int q = 0; q < NWV; q++) tot += acc[q][lane];
+        seq[(size_t)t * P + ii] = (uint32_t)(tot % p);
+    }
 }
 
 // out[r][j] = in[r][j] * d[r] mod p over rows (block-vector scaling)
@@ -379,7 +421,7 @@ __global__ void k_scale_rows(const uint32_t *in, const uint32_t *d, uint32_t *ou
 struct SliceMat {  // one operand (A or A^T) on the device, in sliced form
     uint32_t nrows = 0, nslices = 0, nseg = 0, nlong = 0;
     uint64_t blocks = 0;
-    DevBuf meta, unit, wave_sl, wave_blk, col, val, seg_out, long_rows, long_sbeg;
+    DevBuf meta, unit, wave_sl, wave_blk, col, val, lacc, linfo;
 };
 
 }  // namespace rr
@@ -425,14 +467,14 @@ static rr_status upload(DevBuf &buf, const T *host, size_t count) {
 }
 
 // Lays out one operand for k_spmm over nw wavefronts (rows validated, values
-// packed or plain).  Rows up to long_row nonzeros become units of row slices,
+// packed or plain; in_rows: rows of the gathered vector).  Rows up to long_row nonzeros become units of row slices,
 // sorted by length so the RPW rows of a slice need about the same number of
 // blocks (rounded up to an even count); longer rows become segment slices.  Slices are dealt to wavefronts
 // longest first, each to the least loaded one, and every wavefront's slices
 // and column blocks are stored contiguously in that order.
 static rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> &rp,
                               const std::vector<uint32_t> &colw, const std::vector<uint32_t> &vals, bool packed,
-                              uint32_t B, uint32_t nw, const uint32_t *scale) {
+                              uint32_t B, uint32_t nw, const uint32_t *scale, uint64_t in_rows) {
     const uint32_t VW = B < 4 ? B : 4, LPR = B / VW, RPW = 64 / LPR, BW = RPW * SU;
     const uint64_t nnz = rp[nrows];
     constexpr uint32_t FIN = 8;  // cost of producing a slice, in steps
@@ -442,7 +484,8 @@ static rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uin
     const uint32_t chunk = (uint32_t)up(std::max<uint32_t>(16, long_row / 2), SU);
     M.nrows = (uint32_t)nrows;
 
-    std::vector<uint32_t> shortr, lrows, lsbeg;
+    std::vector<uint32_t> shortr, lrows;
+    std::vector<uint4> linfo;
     for (uint64_t r = 0; r < nrows; r++) {
         if (rp[r + 1] - rp[r] > long_row) lrows.push_back((uint32_t)r);
         else shortr.push_back((uint32_t)r);
@@ -451,7 +494,7 @@ static rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uin
                      [&](uint32_t x, uint32_t y) { return rp[x + 1] - rp[x] > rp[y + 1] - rp[y]; });
     struct Sl {
         uint32_t nblk;
-        uint32_t seg;    // segment id, or NO_ROW for a row slice
+        uint32_t seg;    // accumulator row of a segment, or NO_ROW for a row slice
         uint32_t first;  // row slices: index into shortr; segments: first nonzero
         uint32_t end;    // segments: end of the segment's nonzeros
     };
@@ -462,25 +505,54 @@ static rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uin
     }
     const uint64_t seg_nnz = (uint64_t)RPW * chunk;
     uint32_t nseg = 0;
-    for (uint32_t r : lrows) {
-        lsbeg.push_back(nseg);
-        for (uint64_t e = rp[r]; e < rp[r + 1]; e += seg_nnz) {
+    uint32_t nacc = 0;
+    for (size_t l = 0; l < lrows.size(); l++) {
+        const uint32_t r = lrows[l];
+        const uint64_t ns = (rp[r + 1] - rp[r] + seg_nnz - 1) / seg_nnz;
+        const uint32_t ncopy = (uint32_t)std::min<uint64_t>(MAX_COPIES, ns);
+        linfo.push_back(make_uint4(r, nacc, ncopy, (uint32_t)ns));
+        uint32_t si = 0;
+        for (uint64_t e = rp[r]; e < rp[r + 1]; e += seg_nnz, si++) {
             const uint64_t se = std::min<uint64_t>(rp[r + 1], e + seg_nnz);
             const uint64_t glen = std::min<uint64_t>(chunk, se - e);
-            sls.push_back(Sl{(uint32_t)up((glen + SU - 1) / SU, 2), nseg++, (uint32_t)e, (uint32_t)se});
+            sls.push_back(Sl{(uint32_t)up((glen + SU - 1) / SU, 2), nacc + si % ncopy, (uint32_t)e, (uint32_t)se});
+            nseg++;
         }
+        nacc += ncopy;
     }
-    lsbeg.push_back(nseg);
 
     // longest processing time first
     std::vector<uint32_t> order(sls.size());
     for (size_t i = 0; i < order.size(); i++) order[i] = (uint32_t)i;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return sls[x].nblk > sls[y].nblk; });
     typedef std::pair<uint64_t, uint32_t> Load;
-    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
-    for (uint32_t w = 0; w < nw; w++) heap.push(Load(0, w));
     std::vector<std::vector<uint32_t>> mine(nw);
+    std::vector<uint64_t> load(nw, 0);
+    // Segment slices first, each on the XCD that owns the eighth of the
+    // gathered vector its nonzeros fall in (workgroup i runs on XCD i mod 8),
+    // so each XCD's L2 serves one eighth of the vector to its segments.
+    const bool xcd = nw % (NXCD * (WG / 64)) == 0;
+    if (xcd) {
+        std::vector<std::priority_queue<Load, std::vector<Load>, std::greater<Load>>> xh(NXCD);
+        for (uint32_t w = 0; w < nw; w++) xh[(w / (WG / 64)) % NXCD].push(Load(0, w));
+        const uint32_t mask = packed ? 0x0FFFFFFFu : 0xFFFFFFFFu;
+        for (uint32_t i : order) {
+            const Sl &S = sls[i];
+            if (S.seg == NO_ROW) continue;
+            const uint64_t mid = ((uint64_t)(colw[S.first] & mask) + (colw[S.end - 1] & mask)) / 2;
+            const uint32_t x = (uint32_t)std::min<uint64_t>(NXCD - 1, mid * NXCD / std::max<uint64_t>(in_rows, 1));
+            Load l = xh[x].top();
+            xh[x].pop();
+            mine[l.second].push_back(i);
+            l.first += (uint64_t)S.nblk * SU + FIN;
+            load[l.second] = l.first;
+            xh[x].push(l);
+        }
+    }
+    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+    for (uint32_t w = 0; w < nw; w++) heap.push(Load(load[w], w));
     for (uint32_t i : order) {
+        if (xcd && sls[i].seg != NO_ROW) continue;  // placed above
         Load l = heap.top();
         heap.pop();
         mine[l.second].push_back(i);
@@ -553,9 +625,10 @@ static rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uin
     if ((s = upload(M.wave_blk, wblk.data(), wblk.size())) != RR_OK) return s;
     if ((s = upload(M.col, cw.data(), cw.size())) != RR_OK) return s;
     if (!packed && (s = upload(M.val, vw.data(), vw.size())) != RR_OK) return s;
-    if ((s = M.seg_out.alloc((size_t)nseg * B * 4)) != RR_OK) return s;
-    if ((s = upload(M.long_rows, lrows.data(), lrows.size())) != RR_OK) return s;
-    if ((s = upload(M.long_sbeg, lsbeg.data(), lsbeg.size())) != RR_OK) return s;
+    RR_CHECK((uint64_t)nacc * B * 8 < OOB, RR_EINVAL, "too many split rows");
+    if ((s = M.lacc.alloc((size_t)nacc * B * 8)) != RR_OK) return s;
+    if (nacc) RR_HIP(hipMemset(M.lacc.ptr, 0, (size_t)nacc * B * 8));
+    if ((s = upload(M.linfo, linfo.data(), linfo.size())) != RR_OK) return s;
     return RR_OK;
 }
 
@@ -572,9 +645,9 @@ static SpmmArgs make_args(rr_krylov *c, SliceMat &M, const uint32_t *in, uint32_
     a.in_bytes = (uint32_t)((in == c->z.as<uint32_t>() ? c->n : c->m) * c->B * 4);
     a.out = out;
     a.scale = scale;
-    a.seg_out = M.seg_out.as<uint32_t>();
-    a.long_rows = M.long_rows.as<uint32_t>();
-    a.long_sbeg = M.long_sbeg.as<uint32_t>();
+    a.lacc = M.lacc.as<unsigned long long>();
+    a.lacc_bytes = (uint32_t)M.lacc.bytes;
+    a.linfo = M.linfo.as<uint4>();
     a.nlong = M.nlong;
     a.gram = gram;
     a.fix_words = c->fix_grid * c->P;
@@ -726,7 +799,6 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
     int cus = 0;
     RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
     c->grid = 0;  // set below from the occupancy of the chosen pass kernel
-    c->fix_grid = 64;
 
     // A and A^T (counting-sort transpose, row order kept: src/matrices.rs:734-778)
     std::vector<uint32_t> rp(m + 1), colw(nnz), vals(small ? 0 : nnz);
@@ -759,11 +831,13 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
         dc2[i] = h_mulmod(col_precond[i], col_precond[i], p);
         dcinv[i] = h_invmod(col_precond[i], p);
     }
+    c->fix_grid = 256;
     // one resident wave of workgroups; every wavefront owns a fixed stream of slices
     c->grid = (uint32_t)std::max(1, cus * pass_blocks_per_cu_any(c->B, small));
     rr_status s;
-    if ((s = build_slices(c->A, m, rp, colw, vals, small, c->B, c->grid * (WG / 64), row_precond)) != RR_OK) return s;
-    if ((s = build_slices(c->At, n, rpt, colt, valt, small, c->B, c->grid * (WG / 64), dc2.data())) != RR_OK)
+    if ((s = build_slices(c->A, m, rp, colw, vals, small, c->B, c->grid * (WG / 64), row_precond, n)) != RR_OK)
+        return s;
+    if ((s = build_slices(c->At, n, rpt, colt, valt, small, c->B, c->grid * (WG / 64), dc2.data(), m)) != RR_OK)
         return s;
     if ((s = upload(c->dr, row_precond, m)) != RR_OK) return s;
     if ((s = upload(c->dc, col_precond, n)) != RR_OK) return s;
@@ -884,9 +958,8 @@ rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
             RR_HIP(hipEventCreate(&r1));
             RR_HIP(hipEventRecord(r0, c->stream));
         }
-        uint32_t nthreads = chunk * P;
-        hipLaunchKernelGGL(k_reduce, dim3((nthreads + 255) / 256), dim3(256), 0, c->stream,
-                           c->part.as<uint32_t>(), nparts, P, chunk, c->seq.as<uint32_t>() + done * P, c->p);
+        hipLaunchKernelGGL(k_reduce, dim3((P + 63) / 64, chunk), dim3(RED_WG), 0, c->stream, c->part.as<uint32_t>(),
+                           nparts, P, c->seq.as<uint32_t>() + done * P, c->p);
         RR_HIP(hipGetLastError());
         if (c->profiling) {
             RR_HIP(hipEventRecord(r1, c->stream));
