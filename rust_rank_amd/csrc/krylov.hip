@@ -21,7 +21,7 @@
 // 4B-byte segment.  A lane holds VW = min(B, 4) consecutive words of a row
 // (one 16-byte gather for B >= 4) and a wavefront works on RPW = 256/B rows
 // side by side (8 for B = 32).  The sparse operands are stored in slices of
-// RPW length-sorted rows, column indices in blocks of SU = 4 per row, and
+// RPW length-sorted rows, column indices in blocks of SU = 3 or 4 per row, and
 // every wavefront owns a contiguous, load-balanced stream of slices.  Rows
 // longer than the balance target (the Zipf head of A^T) are cut into
 // segments whose partial sums are added atomically into one 64-bit
@@ -39,7 +39,6 @@
 namespace rr {
 
 constexpr int CHUNK = 64;             // passes per sequence reduction
-constexpr int SU = 4;                 // nonzero steps per column block
 constexpr uint32_t SEG_FLAG = 0x80000000u;
 constexpr uint32_t NO_ROW = 0xFFFFFFFFu;
 constexpr uint32_t MAX_COPIES = 32;  // accumulator rows per split row
@@ -59,6 +58,8 @@ __device__ __forceinline__ void vload(const uint32_t *src, uint32_t (&x)[VW]) {
     if constexpr (VW == 4) {
         const uint4 t = *(const uint4 *)src;
         x[0] = t.x, x[1] = t.y, x[2] = t.z, x[3] = t.w;
+    } else if constexpr (VW == 3) {
+        x[0] = src[0], x[1] = src[1], x[2] = src[2];
     } else if constexpr (VW == 2) {
         const uint2 t = *(const uint2 *)src;
         x[0] = t.x, x[1] = t.y;
@@ -128,7 +129,7 @@ struct SpmmArgs {
 // and gathered one block ahead of the block being accumulated, across slice
 // boundaries, so a wavefront keeps SU row gathers per row group plus a column
 // load in flight while it computes.
-template <int B, bool PACKED>
+template <int B, bool PACKED, int SU>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_spmm(SpmmArgs a) {
     using L = Lay<B>;
     constexpr int VW = L::VW, LPR = L::LPR, RPW = L::RPW, BW = RPW * SU;
@@ -266,7 +267,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
                 }
             }
         if constexpr (!PACKED) {
-            if ((k & 15) == 15) {  // <= 64 steps between folds
+            if ((k & 15) == 15) {  // <= 16 SU <= 64 steps between folds
 #pragma unroll
                 for (int c = 0; c < VW; c++) {
                     ah[c] = fold(ah[c], a.pd, a.inv);
@@ -427,7 +428,7 @@ struct SliceMat {  // one operand (A or A^T) on the device, in sliced form
 }  // namespace rr
 
 struct rr_krylov {
-    uint32_t p = 0, b = 0, B = 0;
+    uint32_t p = 0, b = 0, B = 0, su = 4;
     uint64_t m = 0, n = 0, nnz = 0;
     bool packed = false;
     rr::ModP mp;
@@ -474,7 +475,7 @@ static rr_status upload(DevBuf &buf, const T *host, size_t count) {
 // and column blocks are stored contiguously in that order.
 static rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> &rp,
                               const std::vector<uint32_t> &colw, const std::vector<uint32_t> &vals, bool packed,
-                              uint32_t B, uint32_t nw, const uint32_t *scale, uint64_t in_rows) {
+                              uint32_t B, uint32_t SU, uint32_t nw, const uint32_t *scale, uint64_t in_rows) {
     const uint32_t VW = B < 4 ? B : 4, LPR = B / VW, RPW = 64 / LPR, BW = RPW * SU;
     const uint64_t nnz = rp[nrows];
     constexpr uint32_t FIN = 8;  // cost of producing a slice, in steps
@@ -660,29 +661,33 @@ static SpmmArgs make_args(rr_krylov *c, SliceMat &M, const uint32_t *in, uint32_
 
 template <int B>
 static void launch_pass(rr_krylov *c, const SpmmArgs &a) {
-    if (c->packed)
-        hipLaunchKernelGGL((k_spmm<B, true>), dim3(c->grid), dim3(WG), 0, c->stream, a);
-    else
-        hipLaunchKernelGGL((k_spmm<B, false>), dim3(c->grid), dim3(WG), 0, c->stream, a);
+    dim3 g(c->grid), t(WG);
+    if (c->packed && c->su == 3) hipLaunchKernelGGL((k_spmm<B, true, 3>), g, t, 0, c->stream, a);
+    else if (c->packed) hipLaunchKernelGGL((k_spmm<B, true, 4>), g, t, 0, c->stream, a);
+    else if (c->su == 3) hipLaunchKernelGGL((k_spmm<B, false, 3>), g, t, 0, c->stream, a);
+    else hipLaunchKernelGGL((k_spmm<B, false, 4>), g, t, 0, c->stream, a);
 }
 
 template <int B>
-static int pass_blocks_per_cu(bool packed) {
+static int pass_blocks_per_cu(bool packed, uint32_t su) {
     int n = 0;
-    hipError_t e = packed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_spmm<B, true>, WG, 0)
-                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_spmm<B, false>, WG, 0);
+    hipError_t e;
+    if (packed && su == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_spmm<B, true, 3>, WG, 0);
+    else if (packed) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_spmm<B, true, 4>, WG, 0);
+    else if (su == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_spmm<B, false, 3>, WG, 0);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_spmm<B, false, 4>, WG, 0);
     return (e == hipSuccess && n > 0) ? n : 2;
 }
 
-static int pass_blocks_per_cu_any(uint32_t B, bool packed) {
+static int pass_blocks_per_cu_any(uint32_t B, bool packed, uint32_t su) {
     switch (B) {
-        case 1: return pass_blocks_per_cu<1>(packed);
-        case 2: return pass_blocks_per_cu<2>(packed);
-        case 4: return pass_blocks_per_cu<4>(packed);
-        case 8: return pass_blocks_per_cu<8>(packed);
-        case 16: return pass_blocks_per_cu<16>(packed);
-        case 32: return pass_blocks_per_cu<32>(packed);
-        default: return pass_blocks_per_cu<64>(packed);
+        case 1: return pass_blocks_per_cu<1>(packed, su);
+        case 2: return pass_blocks_per_cu<2>(packed, su);
+        case 4: return pass_blocks_per_cu<4>(packed, su);
+        case 8: return pass_blocks_per_cu<8>(packed, su);
+        case 16: return pass_blocks_per_cu<16>(packed, su);
+        case 32: return pass_blocks_per_cu<32>(packed, su);
+        default: return pass_blocks_per_cu<64>(packed, su);
     }
 }
 
@@ -832,12 +837,25 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
         dcinv[i] = h_invmod(col_precond[i], p);
     }
     c->fix_grid = 256;
+    // column block size: the one that pads the rows of both operands least
+    {
+        uint64_t pad[2] = {0, 0};
+        for (int k = 0; k < 2; k++) {
+            const uint32_t su = 3 + k;
+            auto cost = [&](uint32_t len) { return (uint64_t)((len + 2 * su - 1) / (2 * su)) * 2 * su; };
+            for (uint64_t r = 0; r < m; r++) pad[k] += cost(std::min<uint32_t>(rp[r + 1] - rp[r], 256));
+            for (uint64_t r = 0; r < n; r++) pad[k] += cost(std::min<uint32_t>(rpt[r + 1] - rpt[r], 256));
+        }
+        c->su = pad[0] < pad[1] ? 3 : 4;
+    }
     // one resident wave of workgroups; every wavefront owns a fixed stream of slices
-    c->grid = (uint32_t)std::max(1, cus * pass_blocks_per_cu_any(c->B, small));
+    c->grid = (uint32_t)std::max(1, cus * pass_blocks_per_cu_any(c->B, small, c->su));
     rr_status s;
-    if ((s = build_slices(c->A, m, rp, colw, vals, small, c->B, c->grid * (WG / 64), row_precond, n)) != RR_OK)
+    if ((s = build_slices(c->A, m, rp, colw, vals, small, c->B, c->su, c->grid * (WG / 64), row_precond, n)) !=
+        RR_OK)
         return s;
-    if ((s = build_slices(c->At, n, rpt, colt, valt, small, c->B, c->grid * (WG / 64), dc2.data(), m)) != RR_OK)
+    if ((s = build_slices(c->At, n, rpt, colt, valt, small, c->B, c->su, c->grid * (WG / 64), dc2.data(), m)) !=
+        RR_OK)
         return s;
     if ((s = upload(c->dr, row_precond, m)) != RR_OK) return s;
     if ((s = upload(c->dc, col_precond, n)) != RR_OK) return s;
