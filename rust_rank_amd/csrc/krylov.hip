@@ -42,6 +42,7 @@ constexpr int CHUNK = 64;             // passes per sequence reduction
 constexpr uint32_t SEG_FLAG = 0x80000000u;
 constexpr uint32_t NO_ROW = 0xFFFFFFFFu;
 constexpr uint32_t MAX_COPIES = 32;  // accumulator rows per split row
+constexpr uint32_t MAX_PEND = 16;    // segment slices per wavefront stream
 constexpr uint32_t NXCD = 8;         // XCDs of an MI355X, each with its own L2
 
 // Lane layout of a produced row: VW consecutive words per lane (one 4-, 8- or
@@ -66,6 +67,17 @@ __device__ __forceinline__ void vload(const uint32_t *src, uint32_t (&x)[VW]) {
     } else {
         x[0] = *src;
     }
+}
+
+// Orders this wavefront's later memory operations after the completion of its
+// earlier vector memory operations (a workgroup-scope fence and a wait for
+// the vector memory counter; no cache maintenance).  With agent-scope atomics,
+// which bypass the per-XCD L2s, this is all that cross-workgroup hand-offs
+// through atomics need on gfx950.
+__device__ __forceinline__ void order_after_vmem() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));  // vmcnt(0)
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
 // Gathers go through a buffer resource over the input block vector: a 32-bit
@@ -104,7 +116,7 @@ __device__ __forceinline__ void vstore(uint32_t *dst, const uint32_t (&x)[VW]) {
 // blocks: block k of a slice is RPW x SU words, group g's SU words contiguous.
 struct SpmmArgs {
     const uint32_t *meta;      // per slice: end block in its wavefront's stream | SEG_FLAG
-    const uint2 *unit;         // per slice x RPW: {row, scale[row]} (NO_ROW: none); segment slices: [0].x = accumulator row
+    const uint2 *unit;         // per slice x RPW: {row, scale[row]} (NO_ROW: none); segment slices: {accumulator row, split row}
     const uint32_t *wave_sl;   // per wavefront: first slice (nw + 1)
     const uint32_t *wave_blk;  // per wavefront: first column block (nw + 1); streams end with 2 zero blocks
     const uint32_t *col;       // packed (col | value << 28) or plain column, 0 value = padding
@@ -114,11 +126,9 @@ struct SpmmArgs {
     uint32_t *out;             // produced block vector, stride B
     const uint32_t *scale;     // per-row factor (split rows, in the fix-up)
     unsigned long long *lacc;  // [accumulator rows][B] sums of segment partials (zero between passes)
-    uint32_t lacc_bytes;       // its size (< 2^31)
     const uint4 *linfo;        // per split row: {row, first accumulator row, accumulator rows, segments}
-    uint32_t nlong;            // split rows
-    uint32_t *gram;            // this pass's partials [gridDim.x (+ fix-up grid)][P]
-    uint32_t fix_words;        // words of the fix-up partials after the main grid's
+    uint32_t *lcnt;            // per split row: segments added in this pass (zero between passes)
+    uint32_t *gram;            // this pass's partials [gridDim.x][P]
     uint32_t p;
     double pd, inv;
     uint32_t b;
@@ -136,6 +146,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     __shared__ uint32_t o2s[WG / 64][RPW * 2 * B];  // produced rows, each twice
     __shared__ uint32_t ss[WG / 64][RPW * B];       // their scaled rows
     __shared__ uint32_t fin[GramRing<B>::LDS_WORDS];
+    __shared__ uint32_t pend[WG / 64][MAX_PEND];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane / LPR, q = lane % LPR, ci = lane % B;
     const uint32_t gw = __builtin_amdgcn_readfirstlane(blockIdx.x * (WG / 64) + wave);
@@ -147,12 +158,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     const uint32_t *vb = PACKED ? nullptr : a.val + wblk * BW + g * SU;
     uint32_t *o2 = o2s[wave], *sv = ss[wave];
 
-    if (a.nlong == 0 && blockIdx.x == 0) {  // no fix-up kernel this pass: its partials are zero
-        uint32_t *fx = a.gram + (size_t)gridDim.x * (a.b * (a.b + 1) / 2);
-        for (uint32_t i = threadIdx.x; i < a.fix_words; i += blockDim.x) fx[i] = 0;
-    }
     GramRing<B> gr;
     gr.init();
+    uint32_t npend = 0;  // split rows whose last segment this wavefront added
 
     // accumulators of the current slice: packed |v| <= 8 -> int64 sums;
     // general v = vh 2^16 + vl -> two exact fp64 sums, folded every 64 steps
@@ -199,6 +207,19 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
                 }
             }
             wave_sync();
+            // Count the segment once its adds are complete.  Agent-scope
+            // atomics are performed past the non-coherent per-XCD L2s, so
+            // completion order is all that is needed (no cache flush): the
+            // wavefront that adds the last segment sees every sum.
+            order_after_vmem();
+            const uint32_t l = __builtin_amdgcn_readfirstlane(un.y);
+            uint32_t last = 0;
+            if (lane == 0) last = atomicAdd(a.lcnt + l, 1u) + 1 == a.linfo[l].w;
+            last = __builtin_amdgcn_readfirstlane(last);
+            if (last) {
+                if (lane == 0) pend[wave][npend] = l;
+                npend++;
+            }
         } else {
             const bool ok = un.x != NO_ROW;
 #pragma unroll
@@ -307,54 +328,42 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
             }
         }
     }
+    // Split rows completed by this wavefront: take the sums out of their
+    // accumulator rows (exchanging them for zeros, again past the L2s),
+    // scale, store and project.  Nothing in this pass reads these rows.
+    wave_sync();
+    for (uint32_t k = 0; k < npend; k++) {
+        const uint32_t l = pend[wave][k];
+        order_after_vmem();
+        const uint4 li = a.linfo[l];
+        constexpr int H = 64 / B;
+        const int hh = lane / B;
+        unsigned long long t = 0;  // < 2^63
+        unsigned long long *src = a.lacc + (size_t)li.y * B + ci;
+#pragma unroll 4
+        for (uint32_t cpy = 0; cpy < (MAX_COPIES + H - 1) / H; cpy++) {
+            const uint32_t cc = cpy * H + hh;
+            if (cc < li.z) t += atomicExch(src + (size_t)cc * B, 0ull);
+        }
+        sv[hh * B + ci] = fold_u64(t, a.p, a.pd, a.inv);
+        wave_sync();
+        uint32_t oj = 0;
+        for (int h2 = 0; h2 < H; h2++) oj = addmod(oj, sv[h2 * B + ci], a.p);
+        const uint32_t sj = mulmod(oj, a.scale[li.x], a.p, a.inv);
+        wave_sync();
+        if (hh == 0) {
+            a.out[(size_t)li.x * B + ci] = sj;
+            o2[ci] = oj;
+            o2[B + ci] = oj;
+            sv[ci] = sj;
+        }
+        if (lane == 0) atomicExch(a.lcnt + l, 0u);
+        wave_sync();
+        gr.add(o2, sv[ci], a.p, a.pd, a.inv);
+        wave_sync();
+    }
     __syncthreads();
     gr.finish(a.gram + (size_t)blockIdx.x * (a.b * (a.b + 1) / 2), a.b, a.p, a.pd, a.inv, fin);
-}
-
-// Split rows: the segment slices of the pass have added their partials into
-// the row's accumulator rows in lacc (spread over up to MAX_COPIES rows so
-// that few atomics meet at one address; fewer than 2^32 terms below p each).
-// Every row slot of B lanes takes one split row: sums and clears its
-// accumulator rows (all loads in flight at once), scales, stores and
-// projects the row.  Partials go after the main kernel's
-// (gram + (main_grid + blockIdx.x) * P).
-template <int B>
-__global__ __launch_bounds__(WG) void k_fixup(SpmmArgs a, uint32_t main_grid) {
-    constexpr int G = 64 / B, NS = (WG / 64) * G;
-    __shared__ uint32_t lds[Gram<B>::LDS_WORDS];
-    __shared__ WaveLds wl[WG / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int slot = lane / B, j = lane & (B - 1);
-    const __amdgpu_buffer_rsrc_t racc = __builtin_amdgcn_make_buffer_rsrc((void *)a.lacc, 0, (int)a.lacc_bytes, 0x00020000);
-    Gram<B> g;
-    g.init();
-    for (uint32_t r0 = blockIdx.x * NS; r0 < a.nlong; r0 += gridDim.x * NS) {
-        const uint32_t l = r0 + wave * G + slot;
-        uint32_t o = 0, s = 0;
-        if (l < a.nlong) {
-            const uint4 li = a.linfo[l];
-            const uint32_t sc = a.scale[li.x];
-            // buffer accesses past the end read zeros and write nothing, so
-            // all MAX_COPIES loads and stores issue without branches
-            unsigned long long t = 0;  // < 2^63
-            const uint32_t base = (li.y * B + j) * 8u;
-            typedef uint32_t v2u __attribute__((ext_vector_type(2)));
-#pragma unroll
-            for (uint32_t q = 0; q < MAX_COPIES; q++) {
-                const v2u w = __builtin_amdgcn_raw_buffer_load_b64(racc, q < li.z ? base + q * (B * 8u) : OOB, 0, 0);
-                t += ((unsigned long long)w[1] << 32) | w[0];
-            }
-            const v2u zero = {0u, 0u};
-#pragma unroll
-            for (uint32_t q = 0; q < MAX_COPIES; q++)
-                __builtin_amdgcn_raw_buffer_store_b64(zero, racc, q < li.z ? base + q * (B * 8u) : OOB, 0, 0);
-            o = fold_u64(t, a.p, a.pd, a.inv);
-            s = mulmod(o, sc, a.p, a.inv);
-            a.out[(size_t)li.x * B + j] = s;
-        }
-        g.add(o, s, lane, wl[wave], a.pd, a.inv);
-    }
-    g.finish(a.gram + (size_t)(main_grid + blockIdx.x) * (a.b * (a.b + 1) / 2), a.b, a.p, a.pd, a.inv, lds);
 }
 
 // S = x^T x of a block vector (the t = 0 entry), partials per workgroup.
@@ -422,7 +431,7 @@ __global__ void k_scale_rows(const uint32_t *in, const uint32_t *d, uint32_t *ou
 struct SliceMat {  // one operand (A or A^T) on the device, in sliced form
     uint32_t nrows = 0, nslices = 0, nseg = 0, nlong = 0;
     uint64_t blocks = 0;
-    DevBuf meta, unit, wave_sl, wave_blk, col, val, lacc, linfo;
+    DevBuf meta, unit, wave_sl, wave_blk, col, val, lacc, linfo, lcnt;
 };
 
 }  // namespace rr
@@ -435,7 +444,7 @@ struct rr_krylov {
     rr::SliceMat A, At;
     rr::DevBuf dr, dc, dc2, dcinv, z, y, xtmp, part, seq;
     uint64_t seq_cap = 0;  // entries the device sequence buffer holds
-    uint32_t grid = 0, fix_grid = 0, P = 0;
+    uint32_t grid = 0, P = 0;
     uint64_t t = 0;  // entries produced so far
     hipStream_t stream = nullptr;
     int device = 0;
@@ -496,13 +505,14 @@ static rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uin
     struct Sl {
         uint32_t nblk;
         uint32_t seg;    // accumulator row of a segment, or NO_ROW for a row slice
+        uint32_t lrow;   // segments: index of the split row
         uint32_t first;  // row slices: index into shortr; segments: first nonzero
         uint32_t end;    // segments: end of the segment's nonzeros
     };
     std::vector<Sl> sls;
     for (size_t i = 0; i < shortr.size(); i += RPW) {
         const uint32_t len = rp[shortr[i] + 1] - rp[shortr[i]];
-        sls.push_back(Sl{(uint32_t)up((len + SU - 1) / SU, 2), NO_ROW, (uint32_t)i, 0});
+        sls.push_back(Sl{(uint32_t)up((len + SU - 1) / SU, 2), NO_ROW, 0, (uint32_t)i, 0});
     }
     const uint64_t seg_nnz = (uint64_t)RPW * chunk;
     uint32_t nseg = 0;
@@ -516,7 +526,8 @@ static rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uin
         for (uint64_t e = rp[r]; e < rp[r + 1]; e += seg_nnz, si++) {
             const uint64_t se = std::min<uint64_t>(rp[r + 1], e + seg_nnz);
             const uint64_t glen = std::min<uint64_t>(chunk, se - e);
-            sls.push_back(Sl{(uint32_t)up((glen + SU - 1) / SU, 2), nacc + si % ncopy, (uint32_t)e, (uint32_t)se});
+            sls.push_back(Sl{(uint32_t)up((glen + SU - 1) / SU, 2), nacc + si % ncopy, (uint32_t)l, (uint32_t)e,
+                             (uint32_t)se});
             nseg++;
         }
         nacc += ncopy;
@@ -532,34 +543,49 @@ static rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uin
     // Segment slices first, each on the XCD that owns the eighth of the
     // gathered vector its nonzeros fall in (workgroup i runs on XCD i mod 8),
     // so each XCD's L2 serves one eighth of the vector to its segments.
+    // At most MAX_PEND segment slices per wavefront (a wavefront holding that
+    // many is set aside for the remaining segments of its group).
     const bool xcd = nw % (NXCD * (WG / 64)) == 0;
-    if (xcd) {
-        std::vector<std::priority_queue<Load, std::vector<Load>, std::greater<Load>>> xh(NXCD);
-        for (uint32_t w = 0; w < nw; w++) xh[(w / (WG / 64)) % NXCD].push(Load(0, w));
-        const uint32_t mask = packed ? 0x0FFFFFFFu : 0xFFFFFFFFu;
-        for (uint32_t i : order) {
-            const Sl &S = sls[i];
-            if (S.seg == NO_ROW) continue;
-            const uint64_t mid = ((uint64_t)(colw[S.first] & mask) + (colw[S.end - 1] & mask)) / 2;
-            const uint32_t x = (uint32_t)std::min<uint64_t>(NXCD - 1, mid * NXCD / std::max<uint64_t>(in_rows, 1));
-            Load l = xh[x].top();
-            xh[x].pop();
-            mine[l.second].push_back(i);
-            l.first += (uint64_t)S.nblk * SU + FIN;
-            load[l.second] = l.first;
-            xh[x].push(l);
-        }
+    const uint32_t ngroups = xcd ? NXCD : 1;
+    std::vector<std::priority_queue<Load, std::vector<Load>, std::greater<Load>>> xh(ngroups);
+    for (uint32_t w = 0; w < nw; w++) xh[xcd ? (w / (WG / 64)) % NXCD : 0].push(Load(0, w));
+    std::vector<uint32_t> nsegw(nw, 0);
+    const uint32_t mask = packed ? 0x0FFFFFFFu : 0xFFFFFFFFu;
+    for (uint32_t i : order) {
+        const Sl &S = sls[i];
+        if (S.seg == NO_ROW) continue;
+        const uint64_t mid = ((uint64_t)(colw[S.first] & mask) + (colw[S.end - 1] & mask)) / 2;
+        const uint32_t x =
+            xcd ? (uint32_t)std::min<uint64_t>(NXCD - 1, mid * NXCD / std::max<uint64_t>(in_rows, 1)) : 0;
+        RR_CHECK(!xh[x].empty(), RR_EINVAL, "too many split-row segments for %u wavefronts", nw);
+        Load l = xh[x].top();
+        xh[x].pop();
+        mine[l.second].push_back(i);
+        l.first += (uint64_t)S.nblk * SU + FIN;
+        load[l.second] = l.first;
+        if (++nsegw[l.second] < MAX_PEND) xh[x].push(l);
     }
     std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
     for (uint32_t w = 0; w < nw; w++) heap.push(Load(load[w], w));
     for (uint32_t i : order) {
-        if (xcd && sls[i].seg != NO_ROW) continue;  // placed above
+        if (sls[i].seg != NO_ROW) continue;  // placed above
         Load l = heap.top();
         heap.pop();
         mine[l.second].push_back(i);
         heap.push(Load(l.first + (uint64_t)sls[i].nblk * SU + FIN, l.second));
     }
 
+    // a wavefront's segment slices in order of their gathered rows, so the
+    // wavefronts of an XCD sweep its eighth of the vector together
+    if (xcd) {
+        const uint32_t mask = packed ? 0x0FFFFFFFu : 0xFFFFFFFFu;
+        for (uint32_t w = 0; w < nw; w++) {
+            auto &v = mine[w];
+            auto seg_end = std::stable_partition(v.begin(), v.end(), [&](uint32_t i) { return sls[i].seg != NO_ROW; });
+            std::stable_sort(v.begin(), seg_end,
+                             [&](uint32_t x, uint32_t y) { return (colw[sls[x].first] & mask) < (colw[sls[y].first] & mask); });
+        }
+    }
     std::vector<uint32_t> meta, wsl(nw + 1), wblk(nw + 1);
     std::vector<uint2> unit;
     meta.reserve(sls.size());
@@ -575,7 +601,7 @@ static rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uin
             meta.push_back(end | (S.seg != NO_ROW ? SEG_FLAG : 0u));
             for (uint32_t g = 0; g < RPW; g++) {
                 if (S.seg != NO_ROW) {
-                    unit.push_back(make_uint2(g == 0 ? S.seg : NO_ROW, 0u));
+                    unit.push_back(make_uint2(S.seg, S.lrow));  // the same in every group
                 } else {
                     const size_t k = (size_t)S.first + g;
                     if (k < shortr.size()) unit.push_back(make_uint2(shortr[k], scale[shortr[k]]));
@@ -630,6 +656,8 @@ static rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uin
     if ((s = M.lacc.alloc((size_t)nacc * B * 8)) != RR_OK) return s;
     if (nacc) RR_HIP(hipMemset(M.lacc.ptr, 0, (size_t)nacc * B * 8));
     if ((s = upload(M.linfo, linfo.data(), linfo.size())) != RR_OK) return s;
+    if ((s = M.lcnt.alloc(lrows.size() * 4)) != RR_OK) return s;
+    if (!lrows.empty()) RR_HIP(hipMemset(M.lcnt.ptr, 0, lrows.size() * 4));
     return RR_OK;
 }
 
@@ -647,11 +675,9 @@ static SpmmArgs make_args(rr_krylov *c, SliceMat &M, const uint32_t *in, uint32_
     a.out = out;
     a.scale = scale;
     a.lacc = M.lacc.as<unsigned long long>();
-    a.lacc_bytes = (uint32_t)M.lacc.bytes;
     a.linfo = M.linfo.as<uint4>();
-    a.nlong = M.nlong;
+    a.lcnt = M.lcnt.as<uint32_t>();
     a.gram = gram;
-    a.fix_words = c->fix_grid * c->P;
     a.p = c->p;
     a.pd = c->mp.pd;
     a.inv = c->mp.inv;
@@ -700,19 +726,6 @@ static void launch_pass_any(rr_krylov *c, const SpmmArgs &a) {
         case 16: launch_pass<16>(c, a); break;
         case 32: launch_pass<32>(c, a); break;
         default: launch_pass<64>(c, a); break;
-    }
-}
-
-static void launch_fixup_any(rr_krylov *c, const SpmmArgs &a) {
-    dim3 g(c->fix_grid), t(WG);
-    switch (c->B) {
-        case 1: hipLaunchKernelGGL(k_fixup<1>, g, t, 0, c->stream, a, c->grid); break;
-        case 2: hipLaunchKernelGGL(k_fixup<2>, g, t, 0, c->stream, a, c->grid); break;
-        case 4: hipLaunchKernelGGL(k_fixup<4>, g, t, 0, c->stream, a, c->grid); break;
-        case 8: hipLaunchKernelGGL(k_fixup<8>, g, t, 0, c->stream, a, c->grid); break;
-        case 16: hipLaunchKernelGGL(k_fixup<16>, g, t, 0, c->stream, a, c->grid); break;
-        case 32: hipLaunchKernelGGL(k_fixup<32>, g, t, 0, c->stream, a, c->grid); break;
-        default: hipLaunchKernelGGL(k_fixup<64>, g, t, 0, c->stream, a, c->grid); break;
     }
 }
 
@@ -836,7 +849,6 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
         dc2[i] = h_mulmod(col_precond[i], col_precond[i], p);
         dcinv[i] = h_invmod(col_precond[i], p);
     }
-    c->fix_grid = 256;
     // column block size: the one that pads the rows of both operands least
     {
         uint64_t pad[2] = {0, 0};
@@ -875,7 +887,7 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
                        c->xtmp.as<uint32_t>(), c->dc.as<uint32_t>(), c->z.as<uint32_t>(), (uint32_t)n, B, p,
                        c->mp.inv);
     RR_HIP(hipGetLastError());
-    uint32_t nparts = c->grid + c->fix_grid;
+    uint32_t nparts = c->grid;
     if ((s = c->part.alloc((size_t)CHUNK * nparts * c->P * 4)) != RR_OK) return s;
     RR_HIP(hipMemsetAsync(c->part.ptr, 0, (size_t)CHUNK * nparts * c->P * 4, c->stream));
     RR_HIP(hipStreamSynchronize(c->stream));
@@ -901,14 +913,14 @@ rr_status rr_krylov_stats_get(rr_krylov *c, rr_krylov_stats *out) {
 rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
     clear_error();
     RR_CHECK(c && (seq_out || count == 0), RR_EINVAL, "null argument");
-    const uint32_t P = c->P, nparts = c->grid + c->fix_grid;
+    const uint32_t P = c->P, nparts = c->grid;
     const size_t part_stride = (size_t)nparts * P;
     uint64_t bytes_a = c->stats.bytes_a, bytes_at = c->stats.bytes_at;
     c->stats = rr_krylov_stats{};
     c->stats.bytes_a = bytes_a;
     c->stats.bytes_at = bytes_at;
 
-    // event pairs per pass when profiling: [start, after pass, after fixup]
+    // event triples per pass when profiling: [start, after pass, after pass]
     std::vector<hipEvent_t> ev;
     std::vector<int> kind;  // 0 gram, 1 A pass, 2 A^T pass
     hipEvent_t e_begin = nullptr, e_end = nullptr;
@@ -955,7 +967,6 @@ rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
                                  : make_args(c, M, c->y.as<uint32_t>(), c->z.as<uint32_t>(), c->dc2.as<uint32_t>(), gram);
                 launch_pass_any(c, a);
                 if (c->profiling) RR_HIP(hipEventRecord(e1, c->stream));
-                if (M.nlong) launch_fixup_any(c, a);
                 if (c->profiling) RR_HIP(hipEventRecord(e2, c->stream));
                 k = byA ? 1 : 2;
             }

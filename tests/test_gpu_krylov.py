@@ -55,6 +55,20 @@ def test_long_rows_split():
     _case(A, P, 8, 5, 12)
 
 
+def test_graph_complex_full_size_prefix():
+    """The benchmark workload itself (200000 x 180000, b = 32): 48 entries,
+    the A^T passes with hundreds of split-row segments finished in-pass."""
+    A = synth.graph_complex()
+    _case(A, P, 32, 2, 48)
+
+
+def test_split_rows_many_passes():
+    """Every A^T row split (40000 x 2000, ~240 nonzeros per column), 96
+    entries at b = 16: the segment hand-off through atomics, repeatedly."""
+    A = synth.graph_complex(seed=6, m=40000, n=2000, k=12)
+    _case(A, P, 16, 6, 96)
+
+
 def test_plain_small_full_sequence():
     A = synth.plain_small()
     L = synth.seq_len_plain(min(A.m, A.n), 8)
