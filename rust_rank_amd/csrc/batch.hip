@@ -29,34 +29,23 @@ namespace rr {
 // ---------------------------------------------------------------------------
 // kernels
 
+// t = 0 projections x^T x of row ranges of several requests' vectors.
 template <int B>
-__global__ __launch_bounds__(WG) void k_tiles(const Tile *__restrict__ tiles, uint32_t ntiles, uint32_t p, double pd,
-                                              double inv) {
+__global__ __launch_bounds__(WG) void k_gram_tiles(const GramTile *__restrict__ tiles, uint32_t ntiles, uint32_t p,
+                                                   double pd, double inv) {
     constexpr int G = 64 / B;
     __shared__ uint32_t lds[Gram<B>::LDS_WORDS];
     __shared__ WaveLds wl[WG / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = lane / B, j = lane & (B - 1);
-    WaveLds &w = wl[wave];
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const Tile T = tiles[t];
+        const GramTile T = tiles[t];
         Gram<B> g;
         g.init();
         for (uint32_t base = T.row_beg + wave * G; base < T.row_end; base += (WG / 64) * G) {
             const uint32_t r = base + slot;
-            uint32_t o = 0, s = 0;
-            if (r < T.row_end) {
-                if (T.mode == TILE_GRAM) {
-                    o = s = T.in[(size_t)r * B + j];
-                } else {
-                    const uint32_t beg = T.rowptr[r], end = T.rowptr[r + 1];
-                    o = T.mode == TILE_PACKED ? row_dot<B, true>(T.col, T.val, T.in, beg, end, j, w, pd, inv)
-                                              : row_dot<B, false>(T.col, T.val, T.in, beg, end, j, w, pd, inv);
-                    s = mulmod(o, T.scale[r], p, inv);
-                    T.out[(size_t)r * B + j] = s;
-                }
-            }
-            g.add(o, s, lane, w, pd, inv);
+            const uint32_t o = r < T.row_end ? T.in[(size_t)r * B + j] : 0u;
+            g.add(o, o, lane, wl[wave], pd, inv);
         }
         g.finish(T.part, T.b, p, pd, inv, lds);
         __syncthreads();  // lds is reused by the next tile
@@ -153,7 +142,7 @@ uint64_t request_bytes(uint64_t m, uint64_t n, uint64_t nnz, uint32_t b, uint64_
     uint64_t B = 1;
     while (B < b) B <<= 1;
     uint64_t P = (uint64_t)b * (b + 1) / 2;
-    uint64_t words = 2 * nnz + (m + 1) + (n + 1) + (packed ? 0 : 2 * nnz) + m + 2 * n  // operands + scales
+    uint64_t words = 3 * nnz + 2 * (m + n) + (packed ? 0 : 3 * nnz) + m + 2 * n  // sliced operands + scales
                      + B * (2 * n + m)                                                   // z, x0, y
                      + L * P                                                             // sequence
                      + (uint64_t)2 * b * b * (d + 1) + (uint64_t)4 * b * b * (d + 1)     // G, M
@@ -184,48 +173,23 @@ rr_status Request::init(const rr_csr *A, uint32_t p_, uint32_t b_, const uint32_
     for (uint64_t r = 0; r < m; r++) RR_CHECK(A->row_ptr[r + 1] >= A->row_ptr[r], RR_EINVAL, "row_ptr not monotone");
     nnz = A->row_ptr[m];
     RR_CHECK(nnz < (1ull << 32), RR_EINVAL, "nnz must be < 2^32");
-    packed = m < (1ull << 28) && n < (1ull << 28);
-    for (uint64_t e = 0; e < nnz; e++) {
+    RR_CHECK(std::max(m, n) * B * 4 < (1ull << 31), RR_EINVAL, "block vectors exceed 2^31 bytes");
+    RR_CHECK(nnz == 0 || (A->col && A->val), RR_EINVAL, "null col/val");
+    for (uint64_t e = 0; e < nnz; e++)
         RR_CHECK(A->col[e] < n && A->val[e] < p, RR_EINVAL, "entry %llu out of range", (unsigned long long)e);
-        uint32_t v = A->val[e];
-        if (!(v <= 7 || v >= p - 8)) packed = false;
-    }
     for (uint64_t i = 0; i < m; i++) RR_CHECK(dr[i] >= 1 && dr[i] < p, RR_EINVAL, "row_precond out of range");
     for (uint64_t i = 0; i < n; i++) RR_CHECK(dcv[i] >= 1 && dcv[i] < p, RR_EINVAL, "col_precond out of range");
     for (uint64_t i = 0; i < n * b; i++) RR_CHECK(V[i] < p, RR_EINVAL, "V out of range");
 
-    // A and A^T, raw values (src/matrices.rs:734-778 transpose order)
-    std::vector<uint32_t> rp(m + 1), cw(nnz), vw(packed ? 0 : nnz), rpt(n + 1, 0), ct(nnz), vt(packed ? 0 : nnz);
-    for (uint64_t r = 0; r <= m; r++) rp[r] = (uint32_t)A->row_ptr[r];
-    for (uint64_t e = 0; e < nnz; e++) rpt[A->col[e] + 1]++;
-    for (uint64_t i = 0; i < n; i++) rpt[i + 1] += rpt[i];
-    std::vector<uint32_t> next(rpt.begin(), rpt.end() - 1);
-    for (uint64_t r = 0; r < m; r++)
-        for (uint64_t e = rp[r]; e < rp[r + 1]; e++) {
-            uint32_t c = A->col[e], v = A->val[e], pos = next[c]++;
-            if (packed) {
-                int32_t sv = v <= 7 ? (int32_t)v : (int32_t)v - (int32_t)p;
-                uint32_t code = ((uint32_t)sv & 0xFu) << 28;
-                cw[e] = c | code;
-                ct[pos] = (uint32_t)r | code;
-            } else {
-                cw[e] = c;
-                vw[e] = v;
-                ct[pos] = (uint32_t)r;
-                vt[pos] = v;
-            }
-        }
+    std::vector<uint32_t> rp, cw, vw, rpt, ct, vt;
+    packed = csr_packs(A, p);
+    build_operands(A, p, packed, rp, cw, vw, rpt, ct, vt);
+    su = choose_su(rp, rpt);
     std::vector<uint32_t> dc2(n);
     for (uint64_t i = 0; i < n; i++) dc2[i] = h_mulmod(dcv[i], dcv[i], p);
     std::vector<uint32_t> xpad((size_t)n * B, 0);
     for (uint64_t r = 0; r < n; r++) std::memcpy(&xpad[r * B], &V[r * b], b * 4);
     rr_status s;
-    if ((s = upload_vec(a_rp, rp, st)) != RR_OK) return s;
-    if ((s = upload_vec(a_col, cw, st)) != RR_OK) return s;
-    if ((s = upload_vec(a_val, vw, st)) != RR_OK) return s;
-    if ((s = upload_vec(t_rp, rpt, st)) != RR_OK) return s;
-    if ((s = upload_vec(t_col, ct, st)) != RR_OK) return s;
-    if ((s = upload_vec(t_val, vt, st)) != RR_OK) return s;
     if ((s = upload_vec(dr_d, std::vector<uint32_t>(dr, dr + m), st)) != RR_OK) return s;
     if ((s = upload_vec(dc_d, std::vector<uint32_t>(dcv, dcv + n), st)) != RR_OK) return s;
     if ((s = upload_vec(dc2_d, dc2, st)) != RR_OK) return s;
@@ -240,40 +204,37 @@ rr_status Request::init(const rr_csr *A, uint32_t p_, uint32_t b_, const uint32_
                        dc_d.as<uint32_t>(), z.as<uint32_t>(), (uint32_t)n, B, p, mp.inv);
     RR_HIP(hipGetLastError());
 
-    // tiles of each pass kind
-    auto make = [&](std::vector<Tile> &out, uint64_t rows, const uint32_t *rowptr, const uint32_t *col,
-                    const uint32_t *val, const uint32_t *in, uint32_t *o, const uint32_t *scale, uint32_t mode) {
-        out.clear();
-        for (uint64_t r0 = 0; r0 < rows; r0 += tile_rows) {
-            Tile T;
-            T.rowptr = rowptr;
-            T.col = col;
-            T.val = val;
-            T.in = in;
-            T.out = o;
-            T.scale = scale;
-            T.part = nullptr;
-            T.row_beg = (uint32_t)r0;
-            T.row_end = (uint32_t)std::min<uint64_t>(rows, r0 + tile_rows);
-            T.b = b;
-            T.mode = mode;
-            out.push_back(T);
-        }
-    };
-    uint32_t mode = packed ? TILE_PACKED : TILE_GENERAL;
-    make(tiles_a, m, a_rp.as<uint32_t>(), a_col.as<uint32_t>(), a_val.as<uint32_t>(), z.as<uint32_t>(),
-         y.as<uint32_t>(), dr_d.as<uint32_t>(), mode);
-    make(tiles_at, n, t_rp.as<uint32_t>(), t_col.as<uint32_t>(), t_val.as<uint32_t>(), y.as<uint32_t>(),
-         z.as<uint32_t>(), dc2_d.as<uint32_t>(), mode);
-    make(tiles_gram, n, nullptr, nullptr, nullptr, x0.as<uint32_t>(), nullptr, nullptr, TILE_GRAM);
+    // Sliced operands for nwg workgroups (a multiple of the XCD count, so the
+    // job keeps its XCD placement wherever it starts in a tick's launch).
+    const uint32_t RPW = 64 / (B / (B < 4 ? B : 4));
+    const uint64_t steps = std::max(nnz, std::max(m, n)) / RPW + 1;
+    const uint64_t waves = (steps + BATCH_STEPS_PER_WAVE - 1) / BATCH_STEPS_PER_WAVE;
+    int dev = 0, cus = 0;
+    RR_HIP(hipGetDevice(&dev));
+    RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint64_t max_wg = (uint64_t)cus * spmm_blocks_per_cu(B, packed, su);
+    nwg = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(NXCD, (waves / (WG / 64) + NXCD - 1) / NXCD * NXCD),
+                                       std::max<uint64_t>(NXCD, max_wg / NXCD * NXCD));
+    if ((s = build_slices(sa, m, rp, cw, vw, packed, B, su, nwg * (WG / 64), dr, n)) != RR_OK) return s;
+    if ((s = build_slices(sat, n, rpt, ct, vt, packed, B, su, nwg * (WG / 64), dc2.data(), m)) != RR_OK) return s;
+    args[0] = spmm_args(sa, z.as<uint32_t>(), n, B, y.as<uint32_t>(), dr_d.as<uint32_t>(), p, b);
+    args[1] = spmm_args(sat, y.as<uint32_t>(), m, B, z.as<uint32_t>(), dc2_d.as<uint32_t>(), p, b);
+
+    // t = 0 projection tiles
+    tiles_gram.clear();
+    for (uint64_t r0 = 0; r0 < n; r0 += tile_rows) {
+        GramTile T;
+        T.in = x0.as<uint32_t>();
+        T.part = nullptr;
+        T.row_beg = (uint32_t)r0;
+        T.row_end = (uint32_t)std::min<uint64_t>(n, r0 + tile_rows);
+        T.b = b;
+        T.pad = 0;
+        tiles_gram.push_back(T);
+    }
     t = 0;
     stage = STAGE_KRYLOV;
     return RR_OK;
-}
-
-const std::vector<Tile> &Request::next_tiles() const {
-    if (t == 0) return tiles_gram;
-    return (t & 1) ? tiles_a : tiles_at;
 }
 
 rr_status Request::start_pm(hipStream_t st) {
@@ -289,13 +250,12 @@ rr_status Request::start_pm(hipStream_t st) {
     pm.M = M.as<uint32_t>();
     pm.delta = delta.as<int64_t>();
     stage = STAGE_PM;
-    // vectors and operands are no longer needed
-    a_rp.~DevBuf(); new (&a_rp) DevBuf();
-    a_col.~DevBuf(); new (&a_col) DevBuf();
-    a_val.~DevBuf(); new (&a_val) DevBuf();
-    t_rp.~DevBuf(); new (&t_rp) DevBuf();
-    t_col.~DevBuf(); new (&t_col) DevBuf();
-    t_val.~DevBuf(); new (&t_val) DevBuf();
+    // the operands are no longer needed (the stream orders this after their last pass)
+    RR_HIP(hipStreamSynchronize(st));
+    sa.~SliceMat();
+    new (&sa) SliceMat();
+    sat.~SliceMat();
+    new (&sat) SliceMat();
     return RR_OK;
 }
 
@@ -352,7 +312,7 @@ rr_status TickRunner::init(hipStream_t st_, int cus_) {
     RR_HIP(hipHostMalloc((void **)&stage_host, STAGE_BYTES * NSTAGE, hipHostMallocDefault));
     for (int i = 0; i < NSTAGE; i++) RR_HIP(hipEventCreateWithFlags(&stage_ev[i], hipEventDisableTiming));
     rr_status s;
-    if ((s = tiles_dev.alloc(STAGE_BYTES)) != RR_OK) return s;
+    if ((s = desc_dev.alloc(STAGE_BYTES)) != RR_OK) return s;
     return RR_OK;
 }
 
@@ -366,22 +326,29 @@ TickRunner::~TickRunner() {
 }
 
 template <int B>
-static void launch_tiles(const Tile *tiles, uint32_t n, uint32_t grid, hipStream_t st, const ModP &mp) {
-    hipLaunchKernelGGL(k_tiles<B>, dim3(grid), dim3(WG), 0, st, tiles, n, mp.p, mp.pd, mp.inv);
+static void launch_gram_tiles(const GramTile *tiles, uint32_t n, uint32_t grid, hipStream_t st, const ModP &mp) {
+    hipLaunchKernelGGL(k_gram_tiles<B>, dim3(grid), dim3(WG), 0, st, tiles, n, mp.p, mp.pd, mp.inv);
 }
 
-// One tick: the next entry of every request in `reqs` (all in the Krylov stage).
+// One tick: the next entry of every request in `reqs` (all in the Krylov
+// stage).  Requests at t = 0 contribute projection tiles; the others one
+// pass job each, grouped by (lane width, value kind, block size) into one
+// k_spmm launch per group.  Every request's partials are reduced alone.
 rr_status TickRunner::tick(std::vector<Request *> &reqs, uint32_t p) {
     if (reqs.empty()) return RR_OK;
     ModP mp = ModP::make(p);
-    // group tiles by lane width; assign partial slots
-    std::vector<Tile> by_b[7];
-    std::vector<RedJob> jobs;
+    struct Group {
+        uint32_t B = 0, su = 0;
+        bool packed = false;
+        std::vector<SpmmArgs> jobs;
+        std::vector<uint32_t> wg;  // prefix of workgroups, jobs + 1 entries
+    };
+    std::vector<Group> groups;
+    std::vector<GramTile> gram_by_b[7];
+    std::vector<RedJob> red;
     size_t part_words = 0;
-    for (auto *R : reqs) {
-        const auto &src = R->next_tiles();
-        part_words += src.size() * R->P;
-    }
+    for (auto *R : reqs)
+        part_words += (size_t)(R->t == 0 ? R->tiles_gram.size() : R->nwg) * R->P;
     rr_status s;
     if (part_words > part_cap) {
         RR_HIP(hipStreamSynchronize(st));
@@ -390,63 +357,102 @@ rr_status TickRunner::tick(std::vector<Request *> &reqs, uint32_t p) {
         part_cap = std::max(part_words, part_cap * 2);
         if ((s = part.alloc(part_cap * 4)) != RR_OK) return s;
     }
+    uint32_t *pbase = part.as<uint32_t>();
     size_t off = 0;
     for (auto *R : reqs) {
-        const auto &src = R->next_tiles();
-        int lb = 0;
-        while ((1u << lb) < R->B) lb++;
         RedJob J;
-        J.part = part.as<uint32_t>() + off;
-        J.ntiles = (uint32_t)src.size();
+        J.part = pbase + off;
         J.P = R->P;
         J.out = R->seq.as<uint32_t>() + R->t * R->P;
-        for (const Tile &T0 : src) {
-            Tile T = T0;
-            T.part = part.as<uint32_t>() + off;
-            off += R->P;
-            by_b[lb].push_back(T);
+        if (R->t == 0) {
+            int lb = 0;
+            while ((1u << lb) < R->B) lb++;
+            J.ntiles = (uint32_t)R->tiles_gram.size();
+            for (const GramTile &T0 : R->tiles_gram) {
+                GramTile T = T0;
+                T.part = pbase + off;
+                off += R->P;
+                gram_by_b[lb].push_back(T);
+            }
+        } else {
+            Group *G = nullptr;
+            for (auto &g : groups)
+                if (g.B == R->B && g.su == R->su && g.packed == R->packed) G = &g;
+            if (!G) {
+                groups.emplace_back();
+                G = &groups.back();
+                G->B = R->B;
+                G->su = R->su;
+                G->packed = R->packed;
+                G->wg.push_back(0);
+            }
+            SpmmArgs a = R->args[(R->t & 1) ? 0 : 1];
+            a.gram_off = off;
+            G->jobs.push_back(a);
+            G->wg.push_back(G->wg.back() + R->nwg);
+            J.ntiles = R->nwg;
+            off += (size_t)R->nwg * R->P;
         }
-        jobs.push_back(J);
+        red.push_back(J);
     }
-    // stage descriptors: [tiles by width][jobs] in one pinned block, one copy
-    size_t ntiles_total = 0;
-    for (auto &v : by_b) ntiles_total += v.size();
-    size_t bytes = ntiles_total * sizeof(Tile) + jobs.size() * sizeof(RedJob);
+    // stage descriptors in one pinned block, one copy:
+    // [gram tiles by width][per group: jobs, workgroup prefix][reduction jobs]
+    auto align = [](size_t x) { return (x + 15) & ~(size_t)15; };
+    size_t bytes = 0;
+    size_t gram_off[7], job_off[64], wg_off[64];
+    for (int lb = 0; lb < 7; lb++) {
+        gram_off[lb] = bytes;
+        bytes = align(bytes + gram_by_b[lb].size() * sizeof(GramTile));
+    }
+    RR_CHECK(groups.size() <= 64, RR_EINVAL, "too many pass kinds in one tick");
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        job_off[gi] = bytes;
+        bytes = align(bytes + groups[gi].jobs.size() * sizeof(SpmmArgs));
+        wg_off[gi] = bytes;
+        bytes = align(bytes + groups[gi].wg.size() * sizeof(uint32_t));
+    }
+    const size_t red_off = bytes;
+    bytes += red.size() * sizeof(RedJob);
     RR_CHECK(bytes <= STAGE_BYTES, RR_ENOMEM, "tick descriptors exceed staging (%zu bytes)", bytes);
     int si = stage_next;
     stage_next = (stage_next + 1) % NSTAGE;
     RR_HIP(hipEventSynchronize(stage_ev[si]));  // the copy that last used this slot is done
     char *h = stage_host + (size_t)si * STAGE_BYTES;
-    size_t pos = 0;
-    size_t tile_off[7];
-    for (int lb = 0; lb < 7; lb++) {
-        tile_off[lb] = pos;
-        if (!by_b[lb].empty()) std::memcpy(h + pos, by_b[lb].data(), by_b[lb].size() * sizeof(Tile));
-        pos += by_b[lb].size() * sizeof(Tile);
+    for (int lb = 0; lb < 7; lb++)
+        if (!gram_by_b[lb].empty())
+            std::memcpy(h + gram_off[lb], gram_by_b[lb].data(), gram_by_b[lb].size() * sizeof(GramTile));
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        std::memcpy(h + job_off[gi], groups[gi].jobs.data(), groups[gi].jobs.size() * sizeof(SpmmArgs));
+        std::memcpy(h + wg_off[gi], groups[gi].wg.data(), groups[gi].wg.size() * sizeof(uint32_t));
     }
-    size_t job_off = pos;
-    std::memcpy(h + pos, jobs.data(), jobs.size() * sizeof(RedJob));
-    RR_HIP(hipMemcpyAsync(tiles_dev.ptr, h, bytes, hipMemcpyHostToDevice, st));
+    std::memcpy(h + red_off, red.data(), red.size() * sizeof(RedJob));
+    RR_HIP(hipMemcpyAsync(desc_dev.ptr, h, bytes, hipMemcpyHostToDevice, st));
     RR_HIP(hipEventRecord(stage_ev[si], st));
-    const char *dbase = tiles_dev.as<char>();
+    const char *dbase = desc_dev.as<char>();
     for (int lb = 0; lb < 7; lb++) {
-        uint32_t n = (uint32_t)by_b[lb].size();
+        uint32_t n = (uint32_t)gram_by_b[lb].size();
         if (!n) continue;
-        const Tile *tl = (const Tile *)(dbase + tile_off[lb]);
+        const GramTile *tl = (const GramTile *)(dbase + gram_off[lb]);
         uint32_t grid = std::min<uint32_t>(n, (uint32_t)cus * 4);
         switch (lb) {
-            case 0: launch_tiles<1>(tl, n, grid, st, mp); break;
-            case 1: launch_tiles<2>(tl, n, grid, st, mp); break;
-            case 2: launch_tiles<4>(tl, n, grid, st, mp); break;
-            case 3: launch_tiles<8>(tl, n, grid, st, mp); break;
-            case 4: launch_tiles<16>(tl, n, grid, st, mp); break;
-            case 5: launch_tiles<32>(tl, n, grid, st, mp); break;
-            default: launch_tiles<64>(tl, n, grid, st, mp); break;
+            case 0: launch_gram_tiles<1>(tl, n, grid, st, mp); break;
+            case 1: launch_gram_tiles<2>(tl, n, grid, st, mp); break;
+            case 2: launch_gram_tiles<4>(tl, n, grid, st, mp); break;
+            case 3: launch_gram_tiles<8>(tl, n, grid, st, mp); break;
+            case 4: launch_gram_tiles<16>(tl, n, grid, st, mp); break;
+            case 5: launch_gram_tiles<32>(tl, n, grid, st, mp); break;
+            default: launch_gram_tiles<64>(tl, n, grid, st, mp); break;
         }
         RR_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_reduce_jobs, dim3((uint32_t)jobs.size()), dim3(256), 0, st, (const RedJob *)(dbase + job_off),
-                       p);
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        const Group &G = groups[gi];
+        launch_spmm(G.B, G.packed, G.su, (const SpmmArgs *)(dbase + job_off[gi]),
+                    (const uint32_t *)(dbase + wg_off[gi]), (uint32_t)G.jobs.size(), G.wg.back(), pbase, st);
+        RR_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_reduce_jobs, dim3((uint32_t)red.size()), dim3(256), 0, st,
+                       (const RedJob *)(dbase + red_off), p);
     RR_HIP(hipGetLastError());
     for (auto *R : reqs) R->t++;
     ticks++;

@@ -3,22 +3,18 @@
 #pragma once
 #include "common.hpp"
 #include "pmbasis.hpp"
+#include "spmm.hpp"
 #include <vector>
 
 namespace rr {
 
-enum : uint32_t { TILE_PACKED = 0, TILE_GENERAL = 1, TILE_GRAM = 2 };
-
-// A row range of one operand of one request, fully self-described so a
-// single launch can mix requests, sizes and pass directions.
-struct Tile {
-    const uint32_t *rowptr, *col, *val;  // operand (val unused when packed)
-    const uint32_t *in;                  // gathered block vector (stride B); the vector itself for TILE_GRAM
-    uint32_t *out;                       // produced rows (stride B)
-    const uint32_t *scale;               // per-row factor
-    uint32_t *part;                      // this tile's b(b+1)/2 projection partial
+// Rows [row_beg, row_end) of a block vector whose t = 0 projection x^T x is
+// wanted; self-described so a single launch can mix requests.
+struct GramTile {
+    const uint32_t *in;  // the block vector (stride B)
+    uint32_t *part;      // this tile's b(b+1)/2 projection partial
     uint32_t row_beg, row_end;
-    uint32_t b, mode;
+    uint32_t b, pad;
 };
 
 struct RedJob {  // S_t of one request = sum of its tiles' partials
@@ -32,21 +28,27 @@ enum : int { STAGE_KRYLOV = 0, STAGE_PM = 1, STAGE_DONE = 2, STAGE_FAILED = 3 };
 // resident device footprint of one request (admission control), bytes
 uint64_t request_bytes(uint64_t m, uint64_t n, uint64_t nnz, uint32_t b, uint64_t L, uint64_t d, bool packed);
 
+// Steps (nonzeros per row group) a wavefront of a batched request is given
+// when its operands are sliced: small requests get few wavefronts, large ones
+// up to a full device.
+constexpr uint32_t BATCH_STEPS_PER_WAVE = 96;
+
 struct Request {
-    uint32_t p = 0, b = 0, B = 0, P = 0;
+    uint32_t p = 0, b = 0, B = 0, P = 0, su = 4, nwg = 0;
     uint64_t m = 0, n = 0, nnz = 0, L = 0, d = 0, t = 0;
     bool packed = false;
     int stage = STAGE_KRYLOV;
     int64_t rank = -1;
     uint64_t id = 0, budget_bytes = 0;
     double t_arrive = 0, t_admit = 0, t_done = 0;
-    DevBuf a_rp, a_col, a_val, t_rp, t_col, t_val, dr_d, dc_d, dc2_d, x0, z, y, seq, G, M, delta, rank_in;
-    std::vector<Tile> tiles_a, tiles_at, tiles_gram;
+    SliceMat sa, sat;    // A and A^T in sliced form, nwg workgroups each
+    SpmmArgs args[2];    // pass by A, by A^T (gram_off set per tick)
+    DevBuf dr_d, dc_d, dc2_d, x0, z, y, seq, G, M, delta, rank_in;
+    std::vector<GramTile> tiles_gram;
     PmJob pm;
 
     rr_status init(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *dr, const uint32_t *dc,
                    const uint32_t *V, uint64_t L, uint64_t d, uint32_t tile_rows, hipStream_t st);
-    const std::vector<Tile> &next_tiles() const;
     rr_status start_pm(hipStream_t st);     // sequence complete: build G, join the PM scheduler
     rr_status finish_rank(hipStream_t st);  // PM done: rank formula from the extracted coefficients
     uint64_t device_bytes() const;          // resident footprint estimate (admission control)
@@ -65,7 +67,7 @@ struct TickRunner {
     char *stage_host = nullptr;
     hipEvent_t stage_ev[NSTAGE] = {};
     int stage_next = 0;
-    DevBuf tiles_dev, part;
+    DevBuf desc_dev, part;
     size_t part_cap = 0;
     uint64_t ticks = 0;
 

@@ -30,6 +30,7 @@
 // kernel folds the partials of a chunk of passes into the sequence.
 #include "common.hpp"
 #include "krylov_kernels.hpp"
+#include "spmm.hpp"
 #include <vector>
 #include <algorithm>
 #include <cstring>
@@ -39,11 +40,6 @@
 namespace rr {
 
 constexpr int CHUNK = 64;             // passes per sequence reduction
-constexpr uint32_t SEG_FLAG = 0x80000000u;
-constexpr uint32_t NO_ROW = 0xFFFFFFFFu;
-constexpr uint32_t MAX_COPIES = 32;  // accumulator rows per split row
-constexpr uint32_t MAX_PEND = 16;    // segment slices per wavefront stream
-constexpr uint32_t NXCD = 8;         // XCDs of an MI355X, each with its own L2
 
 // Lane layout of a produced row: VW consecutive words per lane (one 4-, 8- or
 // 16-byte access), LPR lanes per row, RPW rows per wavefront.
@@ -107,40 +103,23 @@ __device__ __forceinline__ void vstore(uint32_t *dst, const uint32_t (&x)[VW]) {
     else *dst = x[0];
 }
 
-// One operand (A or A^T) in sliced form.  A slice is RPW units processed side
-// by side by the RPW row groups of one wavefront, SU nonzeros of each unit per
-// column block.  Row slices hold RPW rows (length-sorted, so padding is
-// small); segment slices hold RPW consecutive chunks of one long row, whose
-// sum is one partial of that row.  Each wavefront owns a contiguous stream of
-// slices (balanced by longest-processing-time assignment) and of their column
-// blocks: block k of a slice is RPW x SU words, group g's SU words contiguous.
-struct SpmmArgs {
-    const uint32_t *meta;      // per slice: end block in its wavefront's stream | SEG_FLAG
-    const uint2 *unit;         // per slice x RPW: {row, scale[row]} (NO_ROW: none); segment slices: {accumulator row, split row}
-    const uint32_t *wave_sl;   // per wavefront: first slice (nw + 1)
-    const uint32_t *wave_blk;  // per wavefront: first column block (nw + 1); streams end with 2 zero blocks
-    const uint32_t *col;       // packed (col | value << 28) or plain column, 0 value = padding
-    const uint32_t *val;       // plain values (general matrices), 0 = padding
-    const uint32_t *in;        // gathered block vector, stride B
-    uint32_t in_bytes;         // its size (< 2^31)
-    uint32_t *out;             // produced block vector, stride B
-    const uint32_t *scale;     // per-row factor (split rows, in the fix-up)
-    unsigned long long *lacc;  // [accumulator rows][B] sums of segment partials (zero between passes)
-    const uint4 *linfo;        // per split row: {row, first accumulator row, accumulator rows, segments}
-    uint32_t *lcnt;            // per split row: segments added in this pass (zero between passes)
-    uint32_t *gram;            // this pass's partials [gridDim.x][P]
-    uint32_t p;
-    double pd, inv;
-    uint32_t b;
-};
-
 // One sparse pass: every slice of this wavefront's stream, then the
 // workgroup's projection partial.  Column blocks are loaded two blocks ahead
 // and gathered one block ahead of the block being accumulated, across slice
 // boundaries, so a wavefront keeps SU row gathers per row group plus a column
 // load in flight while it computes.
 template <int B, bool PACKED, int SU>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_spmm(SpmmArgs a) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_spmm(
+    const SpmmArgs *__restrict__ jobs, const uint32_t *__restrict__ wg_begin, uint32_t njobs, uint32_t *gram_base) {
+    // this workgroup's job: the last j with wg_begin[j] <= blockIdx.x
+    uint32_t jlo = 0, jhi = njobs;
+    while (jhi - jlo > 1) {
+        const uint32_t mid = (jlo + jhi) / 2;
+        if (wg_begin[mid] <= blockIdx.x) jlo = mid;
+        else jhi = mid;
+    }
+    const SpmmArgs a = jobs[jlo];
+    const uint32_t lwg = blockIdx.x - wg_begin[jlo];  // workgroup within the job
     using L = Lay<B>;
     constexpr int VW = L::VW, LPR = L::LPR, RPW = L::RPW, BW = RPW * SU;
     __shared__ uint32_t o2s[WG / 64][RPW * 2 * B];  // produced rows, each twice
@@ -149,7 +128,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     __shared__ uint32_t pend[WG / 64][MAX_PEND];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane / LPR, q = lane % LPR, ci = lane % B;
-    const uint32_t gw = __builtin_amdgcn_readfirstlane(blockIdx.x * (WG / 64) + wave);
+    const uint32_t gw = __builtin_amdgcn_readfirstlane(lwg * (WG / 64) + wave);
     uint32_t sl = a.wave_sl[gw];
     const uint32_t s_end = a.wave_sl[gw + 1];
     const size_t wblk = a.wave_blk[gw];
@@ -363,7 +342,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
         wave_sync();
     }
     __syncthreads();
-    gr.finish(a.gram + (size_t)blockIdx.x * (a.b * (a.b + 1) / 2), a.b, a.p, a.pd, a.inv, fin);
+    gr.finish(gram_base + a.gram_off + (size_t)lwg * (a.b * (a.b + 1) / 2), a.b, a.p, a.pd, a.inv, fin);
 }
 
 // S = x^T x of a block vector (the t = 0 entry), partials per workgroup.
@@ -428,12 +407,6 @@ __global__ void k_scale_rows(const uint32_t *in, const uint32_t *d, uint32_t *ou
 // ---------------------------------------------------------------------------
 // host side
 
-struct SliceMat {  // one operand (A or A^T) on the device, in sliced form
-    uint32_t nrows = 0, nslices = 0, nseg = 0, nlong = 0;
-    uint64_t blocks = 0;
-    DevBuf meta, unit, wave_sl, wave_blk, col, val, lacc, linfo, lcnt;
-};
-
 }  // namespace rr
 
 struct rr_krylov {
@@ -443,6 +416,7 @@ struct rr_krylov {
     rr::ModP mp;
     rr::SliceMat A, At;
     rr::DevBuf dr, dc, dc2, dcinv, z, y, xtmp, part, seq;
+    rr::DevBuf jobs;  // [2] pass descriptors (by A, by A^T) and the one-job workgroup table {0, grid}
     uint64_t seq_cap = 0;  // entries the device sequence buffer holds
     uint32_t grid = 0, P = 0;
     uint64_t t = 0;  // entries produced so far
@@ -482,7 +456,7 @@ static rr_status upload(DevBuf &buf, const T *host, size_t count) {
 // blocks (rounded up to an even count); longer rows become segment slices.  Slices are dealt to wavefronts
 // longest first, each to the least loaded one, and every wavefront's slices
 // and column blocks are stored contiguously in that order.
-static rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> &rp,
+rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> &rp,
                               const std::vector<uint32_t> &colw, const std::vector<uint32_t> &vals, bool packed,
                               uint32_t B, uint32_t SU, uint32_t nw, const uint32_t *scale, uint64_t in_rows) {
     const uint32_t VW = B < 4 ? B : 4, LPR = B / VW, RPW = 64 / LPR, BW = RPW * SU;
@@ -641,6 +615,7 @@ static rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uin
         }
         blk += 2;
     }
+    M.nw = nw;
     M.nslices = (uint32_t)sls.size();
     M.nseg = nseg;
     M.nlong = (uint32_t)lrows.size();
@@ -661,8 +636,8 @@ static rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uin
     return RR_OK;
 }
 
-static SpmmArgs make_args(rr_krylov *c, SliceMat &M, const uint32_t *in, uint32_t *out, const uint32_t *scale,
-                          uint32_t *gram) {
+SpmmArgs spmm_args(const SliceMat &M, const uint32_t *in, uint64_t in_rows, uint32_t B, uint32_t *out,
+                   const uint32_t *scale, uint32_t p, uint32_t b) {
     SpmmArgs a;
     a.meta = M.meta.as<uint32_t>();
     a.unit = M.unit.as<uint2>();
@@ -671,31 +646,46 @@ static SpmmArgs make_args(rr_krylov *c, SliceMat &M, const uint32_t *in, uint32_
     a.col = M.col.as<uint32_t>();
     a.val = M.val.as<uint32_t>();
     a.in = in;
-    a.in_bytes = (uint32_t)((in == c->z.as<uint32_t>() ? c->n : c->m) * c->B * 4);
     a.out = out;
     a.scale = scale;
     a.lacc = M.lacc.as<unsigned long long>();
     a.linfo = M.linfo.as<uint4>();
     a.lcnt = M.lcnt.as<uint32_t>();
-    a.gram = gram;
-    a.p = c->p;
-    a.pd = c->mp.pd;
-    a.inv = c->mp.inv;
-    a.b = c->b;
+    a.gram_off = 0;
+    a.in_bytes = (uint32_t)(in_rows * B * 4);
+    const ModP mp = ModP::make(p);
+    a.p = p;
+    a.pd = mp.pd;
+    a.inv = mp.inv;
+    a.b = b;
     return a;
 }
 
 template <int B>
-static void launch_pass(rr_krylov *c, const SpmmArgs &a) {
-    dim3 g(c->grid), t(WG);
-    if (c->packed && c->su == 3) hipLaunchKernelGGL((k_spmm<B, true, 3>), g, t, 0, c->stream, a);
-    else if (c->packed) hipLaunchKernelGGL((k_spmm<B, true, 4>), g, t, 0, c->stream, a);
-    else if (c->su == 3) hipLaunchKernelGGL((k_spmm<B, false, 3>), g, t, 0, c->stream, a);
-    else hipLaunchKernelGGL((k_spmm<B, false, 4>), g, t, 0, c->stream, a);
+static void launch_b(bool packed, uint32_t su, const SpmmArgs *jobs, const uint32_t *wg_begin, uint32_t njobs,
+                     uint32_t total_wg, uint32_t *gram_base, hipStream_t st) {
+    dim3 g(total_wg), t(WG);
+    if (packed && su == 3) hipLaunchKernelGGL((k_spmm<B, true, 3>), g, t, 0, st, jobs, wg_begin, njobs, gram_base);
+    else if (packed) hipLaunchKernelGGL((k_spmm<B, true, 4>), g, t, 0, st, jobs, wg_begin, njobs, gram_base);
+    else if (su == 3) hipLaunchKernelGGL((k_spmm<B, false, 3>), g, t, 0, st, jobs, wg_begin, njobs, gram_base);
+    else hipLaunchKernelGGL((k_spmm<B, false, 4>), g, t, 0, st, jobs, wg_begin, njobs, gram_base);
+}
+
+void launch_spmm(uint32_t B, bool packed, uint32_t su, const SpmmArgs *jobs, const uint32_t *wg_begin,
+                 uint32_t njobs, uint32_t total_wg, uint32_t *gram_base, hipStream_t st) {
+    switch (B) {
+        case 1: launch_b<1>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st); break;
+        case 2: launch_b<2>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st); break;
+        case 4: launch_b<4>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st); break;
+        case 8: launch_b<8>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st); break;
+        case 16: launch_b<16>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st); break;
+        case 32: launch_b<32>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st); break;
+        default: launch_b<64>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st); break;
+    }
 }
 
 template <int B>
-static int pass_blocks_per_cu(bool packed, uint32_t su) {
+static int blocks_b(bool packed, uint32_t su) {
     int n = 0;
     hipError_t e;
     if (packed && su == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_spmm<B, true, 3>, WG, 0);
@@ -705,28 +695,69 @@ static int pass_blocks_per_cu(bool packed, uint32_t su) {
     return (e == hipSuccess && n > 0) ? n : 2;
 }
 
-static int pass_blocks_per_cu_any(uint32_t B, bool packed, uint32_t su) {
+int spmm_blocks_per_cu(uint32_t B, bool packed, uint32_t su) {
     switch (B) {
-        case 1: return pass_blocks_per_cu<1>(packed, su);
-        case 2: return pass_blocks_per_cu<2>(packed, su);
-        case 4: return pass_blocks_per_cu<4>(packed, su);
-        case 8: return pass_blocks_per_cu<8>(packed, su);
-        case 16: return pass_blocks_per_cu<16>(packed, su);
-        case 32: return pass_blocks_per_cu<32>(packed, su);
-        default: return pass_blocks_per_cu<64>(packed, su);
+        case 1: return blocks_b<1>(packed, su);
+        case 2: return blocks_b<2>(packed, su);
+        case 4: return blocks_b<4>(packed, su);
+        case 8: return blocks_b<8>(packed, su);
+        case 16: return blocks_b<16>(packed, su);
+        case 32: return blocks_b<32>(packed, su);
+        default: return blocks_b<64>(packed, su);
     }
 }
 
-static void launch_pass_any(rr_krylov *c, const SpmmArgs &a) {
-    switch (c->B) {
-        case 1: launch_pass<1>(c, a); break;
-        case 2: launch_pass<2>(c, a); break;
-        case 4: launch_pass<4>(c, a); break;
-        case 8: launch_pass<8>(c, a); break;
-        case 16: launch_pass<16>(c, a); break;
-        case 32: launch_pass<32>(c, a); break;
-        default: launch_pass<64>(c, a); break;
+bool csr_packs(const rr_csr *A, uint32_t p) {
+    if (A->m >= (1ull << 28) || A->n >= (1ull << 28)) return false;
+    const uint64_t nnz = A->row_ptr[A->m];
+    for (uint64_t e = 0; e < nnz; e++) {
+        const uint32_t v = A->val[e];
+        if (!(v <= 7 || v >= p - 8)) return false;
     }
+    return true;
+}
+
+void build_operands(const rr_csr *A, uint32_t p, bool packed, std::vector<uint32_t> &rp, std::vector<uint32_t> &colw,
+                    std::vector<uint32_t> &vals, std::vector<uint32_t> &rpt, std::vector<uint32_t> &colt,
+                    std::vector<uint32_t> &valt) {
+    const uint64_t m = A->m, n = A->n, nnz = A->row_ptr[m];
+    rp.assign(m + 1, 0);
+    colw.assign(nnz, 0);
+    vals.assign(packed ? 0 : nnz, 0);
+    rpt.assign(n + 1, 0);
+    colt.assign(nnz, 0);
+    valt.assign(packed ? 0 : nnz, 0);
+    for (uint64_t r = 0; r <= m; r++) rp[r] = (uint32_t)A->row_ptr[r];
+    for (uint64_t e = 0; e < nnz; e++) rpt[A->col[e] + 1]++;
+    for (uint64_t i = 0; i < n; i++) rpt[i + 1] += rpt[i];
+    std::vector<uint32_t> next(rpt.begin(), rpt.end() - 1);
+    for (uint64_t r = 0; r < m; r++)
+        for (uint64_t e = rp[r]; e < rp[r + 1]; e++) {
+            const uint32_t cidx = A->col[e], v = A->val[e];
+            const uint32_t pos = next[cidx]++;
+            if (packed) {
+                const int32_t sv = v <= 7 ? (int32_t)v : (int32_t)v - (int32_t)p;
+                const uint32_t code = ((uint32_t)sv & 0xFu) << 28;
+                colw[e] = cidx | code;
+                colt[pos] = (uint32_t)r | code;
+            } else {
+                colw[e] = cidx;
+                vals[e] = v;
+                colt[pos] = (uint32_t)r;
+                valt[pos] = v;
+            }
+        }
+}
+
+uint32_t choose_su(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &rpt) {
+    uint64_t pad[2] = {0, 0};
+    for (int k = 0; k < 2; k++) {
+        const uint32_t su = 3 + k;
+        auto cost = [&](uint32_t len) { return (uint64_t)((len + 2 * su - 1) / (2 * su)) * 2 * su; };
+        for (size_t r = 0; r + 1 < rp.size(); r++) pad[k] += cost(std::min<uint32_t>(rp[r + 1] - rp[r], 256));
+        for (size_t r = 0; r + 1 < rpt.size(); r++) pad[k] += cost(std::min<uint32_t>(rpt[r + 1] - rpt[r], 256));
+    }
+    return pad[0] < pad[1] ? 3 : 4;
 }
 
 static void launch_gram_any(rr_krylov *c, const uint32_t *x, uint32_t nrows, uint32_t *gram) {
@@ -783,14 +814,11 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
     const uint64_t nnz = A->row_ptr[m];
     RR_CHECK(nnz < (1ull << 32), RR_EINVAL, "nnz=%llu must be < 2^32", (unsigned long long)nnz);
     RR_CHECK(nnz == 0 || (A->col && A->val), RR_EINVAL, "null col/val");
-    // entries in [-8, 7] (as signed residues) pack with a 28-bit index
-    bool small = m < (1ull << 28) && n < (1ull << 28);
     for (uint64_t e = 0; e < nnz; e++) {
         RR_CHECK(A->col[e] < n, RR_EINVAL, "col[%llu]=%u >= n", (unsigned long long)e, A->col[e]);
-        uint32_t v = A->val[e];
-        RR_CHECK(v < p, RR_EINVAL, "val[%llu]=%u >= p", (unsigned long long)e, v);
-        if (!(v <= 7 || v >= p - 8)) small = false;
+        RR_CHECK(A->val[e] < p, RR_EINVAL, "val[%llu]=%u >= p", (unsigned long long)e, A->val[e]);
     }
+    const bool small = csr_packs(A, p);
     for (uint64_t i = 0; i < m; i++)
         RR_CHECK(row_precond[i] >= 1 && row_precond[i] < p, RR_EINVAL, "row_precond[%llu] not in [1,p)",
                  (unsigned long long)i);
@@ -818,50 +846,17 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
     RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
     c->grid = 0;  // set below from the occupancy of the chosen pass kernel
 
-    // A and A^T (counting-sort transpose, row order kept: src/matrices.rs:734-778)
-    std::vector<uint32_t> rp(m + 1), colw(nnz), vals(small ? 0 : nnz);
-    std::vector<uint32_t> rpt(n + 1, 0), colt(nnz), valt(small ? 0 : nnz);
-    for (uint64_t r = 0; r <= m; r++) rp[r] = (uint32_t)A->row_ptr[r];
-    for (uint64_t e = 0; e < nnz; e++) rpt[A->col[e] + 1]++;
-    for (uint64_t i = 0; i < n; i++) rpt[i + 1] += rpt[i];
-    {
-        std::vector<uint32_t> next(rpt.begin(), rpt.end() - 1);
-        for (uint64_t r = 0; r < m; r++)
-            for (uint64_t e = rp[r]; e < rp[r + 1]; e++) {
-                uint32_t cidx = A->col[e], v = A->val[e];
-                uint32_t pos = next[cidx]++;
-                if (small) {
-                    int32_t sv = v <= 7 ? (int32_t)v : (int32_t)v - (int32_t)p;
-                    uint32_t code = ((uint32_t)sv & 0xFu) << 28;
-                    colw[e] = cidx | code;
-                    colt[pos] = (uint32_t)r | code;
-                } else {
-                    colw[e] = cidx;
-                    vals[e] = v;
-                    colt[pos] = (uint32_t)r;
-                    valt[pos] = v;
-                }
-            }
-    }
+    std::vector<uint32_t> rp, colw, vals, rpt, colt, valt;
+    build_operands(A, p, small, rp, colw, vals, rpt, colt, valt);
     // preconditioners: D_r, D_c, D_c^2, D_c^-1
     std::vector<uint32_t> dc2(n), dcinv(n);
     for (uint64_t i = 0; i < n; i++) {
         dc2[i] = h_mulmod(col_precond[i], col_precond[i], p);
         dcinv[i] = h_invmod(col_precond[i], p);
     }
-    // column block size: the one that pads the rows of both operands least
-    {
-        uint64_t pad[2] = {0, 0};
-        for (int k = 0; k < 2; k++) {
-            const uint32_t su = 3 + k;
-            auto cost = [&](uint32_t len) { return (uint64_t)((len + 2 * su - 1) / (2 * su)) * 2 * su; };
-            for (uint64_t r = 0; r < m; r++) pad[k] += cost(std::min<uint32_t>(rp[r + 1] - rp[r], 256));
-            for (uint64_t r = 0; r < n; r++) pad[k] += cost(std::min<uint32_t>(rpt[r + 1] - rpt[r], 256));
-        }
-        c->su = pad[0] < pad[1] ? 3 : 4;
-    }
+    c->su = choose_su(rp, rpt);
     // one resident wave of workgroups; every wavefront owns a fixed stream of slices
-    c->grid = (uint32_t)std::max(1, cus * pass_blocks_per_cu_any(c->B, small, c->su));
+    c->grid = (uint32_t)std::max(1, cus * spmm_blocks_per_cu(c->B, small, c->su));
     rr_status s;
     if ((s = build_slices(c->A, m, rp, colw, vals, small, c->B, c->su, c->grid * (WG / 64), row_precond, n)) !=
         RR_OK)
@@ -887,6 +882,16 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
                        c->xtmp.as<uint32_t>(), c->dc.as<uint32_t>(), c->z.as<uint32_t>(), (uint32_t)n, B, p,
                        c->mp.inv);
     RR_HIP(hipGetLastError());
+    // pass descriptors and the one-job workgroup table
+    {
+        SpmmArgs args[2] = {spmm_args(c->A, c->z.as<uint32_t>(), n, B, c->y.as<uint32_t>(), c->dr.as<uint32_t>(), p, b),
+                            spmm_args(c->At, c->y.as<uint32_t>(), m, B, c->z.as<uint32_t>(), c->dc2.as<uint32_t>(), p,
+                                      b)};
+        uint32_t wgs[2] = {0, c->grid};
+        if ((s = c->jobs.alloc(sizeof args + sizeof wgs)) != RR_OK) return s;
+        RR_HIP(hipMemcpy(c->jobs.ptr, args, sizeof args, hipMemcpyHostToDevice));
+        RR_HIP(hipMemcpy(c->jobs.as<char>() + sizeof args, wgs, sizeof wgs, hipMemcpyHostToDevice));
+    }
     uint32_t nparts = c->grid;
     if ((s = c->part.alloc((size_t)CHUNK * nparts * c->P * 4)) != RR_OK) return s;
     RR_HIP(hipMemsetAsync(c->part.ptr, 0, (size_t)CHUNK * nparts * c->P * 4, c->stream));
@@ -962,10 +967,9 @@ rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
                 }
             } else {
                 bool byA = (t & 1) == 1;
-                SliceMat &M = byA ? c->A : c->At;
-                SpmmArgs a = byA ? make_args(c, M, c->z.as<uint32_t>(), c->y.as<uint32_t>(), c->dr.as<uint32_t>(), gram)
-                                 : make_args(c, M, c->y.as<uint32_t>(), c->z.as<uint32_t>(), c->dc2.as<uint32_t>(), gram);
-                launch_pass_any(c, a);
+                const SpmmArgs *job = c->jobs.as<SpmmArgs>() + (byA ? 0 : 1);
+                const uint32_t *wgs = (const uint32_t *)(c->jobs.as<char>() + 2 * sizeof(SpmmArgs));
+                launch_spmm(c->B, c->packed, c->su, job, wgs, 1, c->grid, gram, c->stream);
                 if (c->profiling) RR_HIP(hipEventRecord(e1, c->stream));
                 if (c->profiling) RR_HIP(hipEventRecord(e2, c->stream));
                 k = byA ? 1 : 2;

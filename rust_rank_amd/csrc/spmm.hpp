@@ -1,0 +1,83 @@
+// spmm.hpp -- the sliced sparse pass (k_spmm) shared by the single-matrix
+// Krylov context and the batched / served tick runner.
+//
+// One launch runs any number of jobs of the same lane width, value kind and
+// column block size: job j owns workgroups [wg_begin[j], wg_begin[j + 1]) and
+// describes one operand (A or A^T of one matrix) in sliced form, the vector
+// it gathers, the vector it produces and where its projection partials go.
+#pragma once
+#include "common.hpp"
+#include <vector>
+
+namespace rr {
+
+constexpr uint32_t SEG_FLAG = 0x80000000u;
+constexpr uint32_t NO_ROW = 0xFFFFFFFFu;
+constexpr uint32_t MAX_COPIES = 32;  // accumulator rows per split row
+constexpr uint32_t MAX_PEND = 16;    // segment slices per wavefront stream
+constexpr uint32_t NXCD = 8;         // XCDs of an MI355X, each with its own L2
+
+// One operand in sliced form.  A slice is RPW units processed side by side by
+// the RPW row groups of one wavefront, SU nonzeros of each unit per column
+// block.  Row slices hold RPW rows (length-sorted, so padding is small);
+// segment slices hold RPW consecutive chunks of one long row, whose sum is
+// one partial of that row.  Each wavefront owns a contiguous stream of slices
+// (balanced by longest-processing-time assignment) and of their column
+// blocks: block k of a slice is RPW x SU words, group g's SU words contiguous.
+struct SpmmArgs {
+    const uint32_t *meta;      // per slice: end block in its wavefront's stream | SEG_FLAG
+    const uint2 *unit;         // per slice x RPW: {row, scale[row]} (NO_ROW: none); segment slices: {accumulator row, split row}
+    const uint32_t *wave_sl;   // per wavefront: first slice (nw + 1)
+    const uint32_t *wave_blk;  // per wavefront: first column block (nw + 1); streams end with 2 zero blocks
+    const uint32_t *col;       // packed (col | value << 28) or plain column, 0 value = padding
+    const uint32_t *val;       // plain values (general matrices), 0 = padding
+    const uint32_t *in;        // gathered block vector, stride B
+    uint32_t *out;             // produced block vector, stride B
+    const uint32_t *scale;     // per-row factor (split rows)
+    unsigned long long *lacc;  // [accumulator rows][B] sums of segment partials (zero between passes)
+    const uint4 *linfo;        // per split row: {row, first accumulator row, accumulator rows, segments}
+    uint32_t *lcnt;            // per split row: segments added in this pass (zero between passes)
+    uint64_t gram_off;         // words from the launch's partial base to this job's first partial
+    uint32_t in_bytes;         // size of in (< 2^31)
+    uint32_t p;
+    double pd, inv;
+    uint32_t b;
+};
+
+struct SliceMat {  // one operand (A or A^T) on the device, in sliced form
+    uint32_t nrows = 0, nslices = 0, nseg = 0, nlong = 0, nw = 0;
+    uint64_t blocks = 0;
+    DevBuf meta, unit, wave_sl, wave_blk, col, val, lacc, linfo, lcnt;
+};
+
+// A and A^T of a validated CSR matrix with values < p: row pointers and
+// column words (packed: column | 4-bit signed value << 28; plain: column, with
+// values separately).  The transpose keeps row order within each column
+// (src/matrices.rs:734-778).
+void build_operands(const rr_csr *A, uint32_t p, bool packed, std::vector<uint32_t> &rp, std::vector<uint32_t> &colw,
+                    std::vector<uint32_t> &vals, std::vector<uint32_t> &rpt, std::vector<uint32_t> &colt,
+                    std::vector<uint32_t> &valt);
+
+// True when every value is a signed residue in [-8, 7] and m, n < 2^28.
+bool csr_packs(const rr_csr *A, uint32_t p);
+
+// Column block size (3 or 4) that pads the rows of both operands least.
+uint32_t choose_su(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &rpt);
+
+// Lays out one operand for nw wavefronts (nw a multiple of 4); in_rows: rows
+// of the vector it gathers.
+rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> &rp,
+                       const std::vector<uint32_t> &colw, const std::vector<uint32_t> &vals, bool packed, uint32_t B,
+                       uint32_t SU, uint32_t nw, const uint32_t *scale, uint64_t in_rows);
+
+SpmmArgs spmm_args(const SliceMat &M, const uint32_t *in, uint64_t in_rows, uint32_t B, uint32_t *out,
+                   const uint32_t *scale, uint32_t p, uint32_t b);
+
+// Resident workgroups per CU of the pass kernel for (B, packed, SU).
+int spmm_blocks_per_cu(uint32_t B, bool packed, uint32_t su);
+
+// One launch over njobs jobs (device arrays); total_wg = wg_begin[njobs].
+void launch_spmm(uint32_t B, bool packed, uint32_t su, const SpmmArgs *jobs, const uint32_t *wg_begin,
+                 uint32_t njobs, uint32_t total_wg, uint32_t *gram_base, hipStream_t st);
+
+}  // namespace rr
