@@ -17,6 +17,7 @@
 // two mullo.  The 1/N of the inverse is folded into the pointwise product.
 #include "common.hpp"
 #include "ntt.hpp"
+#include <algorithm>
 
 namespace rr {
 
@@ -71,42 +72,43 @@ __device__ void lds_dit(uint32_t *s, uint32_t npoly, uint32_t logn, const uint32
 
 // Forward transforms: dst[q][0..N) = DIF(src[q][0..len) zero-padded), for the
 // polynomials q of a [rows][cols] matrix with source stride `sstride` (words
-// between consecutive polynomials) and source offset `soff` (first coefficient).
-__global__ __launch_bounds__(NTT_THREADS) void k_ntt_fwd(const uint32_t *src, uint64_t sstride, uint32_t soff,
-                                                         uint32_t len, uint32_t *dst, uint32_t count,
-                                                         uint32_t logn, const uint32_t *tw, const uint32_t *twq,
-                                                         uint32_t tw_logn, uint32_t p) {
+// between consecutive polynomials) and source offset `soff` (first
+// coefficient).  blockIdx.y selects the descriptor of the pack.
+__global__ __launch_bounds__(NTT_THREADS) void k_ntt_fwd(NttFwdPack pk, uint32_t logn, const uint32_t *tw,
+                                                         const uint32_t *twq, uint32_t tw_logn, uint32_t p) {
     __shared__ uint32_t s[NTT_LDS_WORDS];
+    const NttFwdDesc &D = pk.d[blockIdx.y];
     const uint32_t N = 1u << logn;
     const uint32_t per = N >= NTT_LDS_WORDS ? 1 : NTT_LDS_WORDS / N;
     const uint32_t q0 = blockIdx.x * per;
-    const uint32_t np = min(per, count - q0);
+    if (q0 >= D.count) return;
+    const uint32_t np = min(per, D.count - q0);
     for (uint32_t i = threadIdx.x; i < np * N; i += blockDim.x) {
         uint32_t q = q0 + i / N, k = i % N;
-        s[i] = k < len ? src[(size_t)q * sstride + soff + k] : 0u;
+        s[i] = k < D.len ? D.src[(size_t)q * D.sstride + D.soff + k] : 0u;
     }
     __syncthreads();
     lds_dif(s, np, logn, tw, twq, tw_logn, p);
-    for (uint32_t i = threadIdx.x; i < np * N; i += blockDim.x) dst[(size_t)q0 * N + i] = s[i];
+    for (uint32_t i = threadIdx.x; i < np * N; i += blockDim.x) D.dst[(size_t)q0 * N + i] = s[i];
 }
 
 // Inverse transforms of `count` spectra (already scaled by 1/N), writing
 // coefficients [lo, lo + outlen) of each to dst[q * dstride + k - lo].
-__global__ __launch_bounds__(NTT_THREADS) void k_ntt_inv(const uint32_t *src, uint32_t count, uint32_t logn,
-                                                         uint32_t lo, uint32_t outlen, uint32_t *dst,
-                                                         uint64_t dstride, const uint32_t *tw, const uint32_t *twq,
-                                                         uint32_t tw_logn, uint32_t p) {
+__global__ __launch_bounds__(NTT_THREADS) void k_ntt_inv(NttInvPack pk, uint32_t logn, const uint32_t *tw,
+                                                         const uint32_t *twq, uint32_t tw_logn, uint32_t p) {
     __shared__ uint32_t s[NTT_LDS_WORDS];
+    const NttInvDesc &D = pk.d[blockIdx.y];
     const uint32_t N = 1u << logn;
     const uint32_t per = N >= NTT_LDS_WORDS ? 1 : NTT_LDS_WORDS / N;
     const uint32_t q0 = blockIdx.x * per;
-    const uint32_t np = min(per, count - q0);
-    for (uint32_t i = threadIdx.x; i < np * N; i += blockDim.x) s[i] = src[(size_t)q0 * N + i];
+    if (q0 >= D.count) return;
+    const uint32_t np = min(per, D.count - q0);
+    for (uint32_t i = threadIdx.x; i < np * N; i += blockDim.x) s[i] = D.src[(size_t)q0 * N + i];
     __syncthreads();
     lds_dit(s, np, logn, tw, twq, tw_logn, p);
-    for (uint32_t i = threadIdx.x; i < np * outlen; i += blockDim.x) {
-        uint32_t q = i / outlen, k = i % outlen;
-        dst[(size_t)(q0 + q) * dstride + k] = s[q * N + lo + k];
+    for (uint32_t i = threadIdx.x; i < np * D.outlen; i += blockDim.x) {
+        uint32_t q = i / D.outlen, k = i % D.outlen;
+        D.dst[(size_t)(q0 + q) * D.dstride + k] = s[q * N + D.lo + k];
     }
 }
 
@@ -114,17 +116,21 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_inv(const uint32_t *src, ui
 // spectra of length N.  Thread (l, i) keeps JT entries of row i at point l as
 // exact fp64 sums (A split into 16-bit halves, folded every 64 terms).
 template <int JT>
-__global__ __launch_bounds__(256) void k_pointwise(const uint32_t *A, const uint32_t *B, uint32_t *C, uint32_t rows,
-                                                   uint32_t inner, uint32_t cols, uint32_t N, uint32_t scale,
-                                                   uint32_t p, double pd, double inv) {
+__global__ __launch_bounds__(256) void k_pointwise(PwPack pk, uint32_t ztiles, uint32_t N, uint32_t p, double pd,
+                                                   double inv) {
+    const PwDesc &Dd = pk.d[blockIdx.z / ztiles];
+    const uint32_t *A = Dd.A, *B = Dd.B;
+    uint32_t *C = Dd.C;
+    const uint32_t rows = Dd.rows, inner = Dd.inner, cols = Dd.cols, scale = Dd.scale;
     const uint32_t l = blockIdx.x * 64 + (threadIdx.x & 63);
     const uint32_t i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const uint32_t j0 = blockIdx.z * JT;
-    if (l >= N || i >= rows) return;
+    const uint32_t j0 = (blockIdx.z % ztiles) * JT;
+    if (l >= N || i >= rows || j0 >= cols) return;
     double hi[JT], lo[JT];
 #pragma unroll
     for (int j = 0; j < JT; j++) hi[j] = lo[j] = 0.0;
     const uint32_t jn = min((uint32_t)JT, cols - j0);
+#pragma unroll 4
     for (uint32_t r = 0; r < inner; r++) {
         uint32_t a = A[((size_t)i * inner + r) * N + l];
         const double ah = (double)(a >> 16), al = (double)(a & 0xFFFFu);
@@ -192,34 +198,66 @@ static inline uint32_t blocks_for(uint32_t count, uint32_t logn) {
     return (count + per - 1) / per;
 }
 
-rr_status ntt_forward(const NttTables &T, const uint32_t *src, uint64_t sstride, uint32_t soff, uint32_t len,
-                      uint32_t *dst, uint32_t count, uint32_t logn, hipStream_t st) {
+rr_status ntt_forward_pack(const NttTables &T, const NttFwdPack &pk, uint32_t ndesc, uint32_t logn,
+                           hipStream_t st) {
     RR_CHECK(logn >= 1 && logn <= T.logn && logn <= 14, RR_EINVAL, "NTT size 2^%u unsupported", logn);
-    hipLaunchKernelGGL(k_ntt_fwd, dim3(blocks_for(count, logn)), dim3(NTT_THREADS), 0, st, src, sstride, soff,
-                       len, dst, count, logn, T.fwd.as<uint32_t>(), T.fwdq.as<uint32_t>(), T.logn, NTT_PRIME);
+    RR_CHECK(ndesc >= 1 && ndesc <= NTT_PACK, RR_EINVAL, "bad pack size");
+    uint32_t bx = 0;
+    for (uint32_t i = 0; i < ndesc; i++) bx = std::max(bx, blocks_for(pk.d[i].count, logn));
+    hipLaunchKernelGGL(k_ntt_fwd, dim3(std::max(bx, 1u), ndesc), dim3(NTT_THREADS), 0, st, pk, logn,
+                       T.fwd.as<uint32_t>(), T.fwdq.as<uint32_t>(), T.logn, NTT_PRIME);
     RR_HIP(hipGetLastError());
     return RR_OK;
+}
+
+rr_status ntt_inverse_pack(const NttTables &T, const NttInvPack &pk, uint32_t ndesc, uint32_t logn,
+                           hipStream_t st) {
+    RR_CHECK(logn >= 1 && logn <= T.logn && logn <= 14, RR_EINVAL, "NTT size 2^%u unsupported", logn);
+    RR_CHECK(ndesc >= 1 && ndesc <= NTT_PACK, RR_EINVAL, "bad pack size");
+    uint32_t bx = 0;
+    for (uint32_t i = 0; i < ndesc; i++) bx = std::max(bx, blocks_for(pk.d[i].count, logn));
+    hipLaunchKernelGGL(k_ntt_inv, dim3(std::max(bx, 1u), ndesc), dim3(NTT_THREADS), 0, st, pk, logn,
+                       T.inv.as<uint32_t>(), T.invq.as<uint32_t>(), T.logn, NTT_PRIME);
+    RR_HIP(hipGetLastError());
+    return RR_OK;
+}
+
+rr_status pointwise_pack(const PwPack &pk, uint32_t ndesc, uint32_t N, hipStream_t st) {
+    RR_CHECK(ndesc >= 1 && ndesc <= NTT_PACK, RR_EINVAL, "bad pack size");
+    const uint32_t p = NTT_PRIME;
+    ModP mp = ModP::make(p);
+    constexpr int JT = 16;
+    uint32_t rows = 0, cols = 0;
+    for (uint32_t i = 0; i < ndesc; i++) {
+        rows = std::max(rows, pk.d[i].rows);
+        cols = std::max(cols, pk.d[i].cols);
+    }
+    const uint32_t zt = (cols + JT - 1) / JT;
+    dim3 grid((N + 63) / 64, (rows + 3) / 4, zt * ndesc);
+    hipLaunchKernelGGL(k_pointwise<JT>, grid, dim3(256), 0, st, pk, zt, N, p, mp.pd, mp.inv);
+    RR_HIP(hipGetLastError());
+    return RR_OK;
+}
+
+rr_status ntt_forward(const NttTables &T, const uint32_t *src, uint64_t sstride, uint32_t soff, uint32_t len,
+                      uint32_t *dst, uint32_t count, uint32_t logn, hipStream_t st) {
+    NttFwdPack pk;
+    pk.d[0] = NttFwdDesc{src, sstride, soff, len, dst, count};
+    return ntt_forward_pack(T, pk, 1, logn, st);
 }
 
 rr_status ntt_inverse(const NttTables &T, const uint32_t *src, uint32_t count, uint32_t logn, uint32_t lo,
                       uint32_t outlen, uint32_t *dst, uint64_t dstride, hipStream_t st) {
-    RR_CHECK(logn >= 1 && logn <= T.logn && logn <= 14, RR_EINVAL, "NTT size 2^%u unsupported", logn);
-    hipLaunchKernelGGL(k_ntt_inv, dim3(blocks_for(count, logn)), dim3(NTT_THREADS), 0, st, src, count, logn, lo,
-                       outlen, dst, dstride, T.inv.as<uint32_t>(), T.invq.as<uint32_t>(), T.logn, NTT_PRIME);
-    RR_HIP(hipGetLastError());
-    return RR_OK;
+    NttInvPack pk;
+    pk.d[0] = NttInvDesc{src, count, lo, outlen, dst, dstride};
+    return ntt_inverse_pack(T, pk, 1, logn, st);
 }
 
 rr_status pointwise(const uint32_t *A, const uint32_t *B, uint32_t *C, uint32_t rows, uint32_t inner,
                     uint32_t cols, uint32_t N, uint32_t scale, hipStream_t st) {
-    const uint32_t p = NTT_PRIME;
-    ModP mp = ModP::make(p);
-    constexpr int JT = 16;
-    dim3 grid((N + 63) / 64, (rows + 3) / 4, (cols + JT - 1) / JT);
-    hipLaunchKernelGGL(k_pointwise<JT>, grid, dim3(256), 0, st, A, B, C, rows, inner, cols, N, scale, p, mp.pd,
-                       mp.inv);
-    RR_HIP(hipGetLastError());
-    return RR_OK;
+    PwPack pk;
+    pk.d[0] = PwDesc{A, B, C, rows, inner, cols, scale};
+    return pointwise_pack(pk, 1, N, st);
 }
 
 }  // namespace rr

@@ -17,6 +17,40 @@ struct NttTables {
     rr_status init(uint32_t logn_max, hipStream_t stream);
 };
 
+// Descriptor packs: one launch runs up to NTT_PACK independent transforms or
+// pointwise products of the same size (blockIdx selects the descriptor).
+constexpr uint32_t NTT_PACK = 32;
+struct NttFwdDesc {  // dst[q][0..N) = spectrum of src[q * sstride + soff + k], k < len, q < count
+    const uint32_t *src;
+    uint64_t sstride;
+    uint32_t soff, len;
+    uint32_t *dst;
+    uint32_t count;
+};
+struct NttInvDesc {  // dst[q * dstride + k] = coefficient lo + k of the inverse of src[q], k < outlen
+    const uint32_t *src;
+    uint32_t count, lo, outlen;
+    uint32_t *dst;
+    uint64_t dstride;
+};
+struct PwDesc {  // C[i][j][l] = scale * sum_r A[i][r][l] B[r][j][l]
+    const uint32_t *A, *B;
+    uint32_t *C;
+    uint32_t rows, inner, cols, scale;
+};
+struct NttFwdPack {
+    NttFwdDesc d[NTT_PACK];
+};
+struct NttInvPack {
+    NttInvDesc d[NTT_PACK];
+};
+struct PwPack {
+    PwDesc d[NTT_PACK];
+};
+rr_status ntt_forward_pack(const NttTables &T, const NttFwdPack &pk, uint32_t ndesc, uint32_t logn, hipStream_t st);
+rr_status ntt_inverse_pack(const NttTables &T, const NttInvPack &pk, uint32_t ndesc, uint32_t logn, hipStream_t st);
+rr_status pointwise_pack(const PwPack &pk, uint32_t ndesc, uint32_t N, hipStream_t st);
+
 // dst[q][0..2^logn) = forward spectrum (bit-reversed order) of src[q * sstride + soff + k], k < len.
 rr_status ntt_forward(const NttTables &T, const uint32_t *src, uint64_t sstride, uint32_t soff, uint32_t len,
                       uint32_t *dst, uint32_t count, uint32_t logn, hipStream_t st);

@@ -35,136 +35,289 @@
 namespace rr {
 
 // ---------------------------------------------------------------------------
-// leaf kernel
-// G0(i, j) = G[(i * n + j) * gstride + goff]; M [m][m][2] (stride 2); delta [m] in/out.
-__device__ void mbasis_wg(const uint32_t *G, uint64_t gstride, uint64_t goff, uint32_t m, uint32_t n,
-                          int64_t *delta, uint32_t *M, uint32_t p, double pd, double inv) {
-    extern __shared__ uint32_t sm[];
-    uint32_t *R = sm;                  // [m][n] reduced rows
-    uint32_t *U = R + m * n;           // [m][n] coefficients on pivot rows
-    uint32_t *sig = U + m * n;         // [m]
-    uint32_t *fcol = sig + m;          // [m] snapshot of the pivot column
-    uint32_t *perm = fcol + m;         // [m]
-    uint32_t *colord = perm + m;       // [m] column at each position
-    uint32_t *pivrow = colord + m;     // [n]
-    uint32_t *ctl = pivrow + n;        // [4]: pivot position, pivot column, rank
-    int64_t *dl = (int64_t *)(ctl + 4 + ((m + n) & 1));  // [m] (8-byte aligned)
-    const uint32_t tid = threadIdx.x, nt = blockDim.x;
-
-    for (uint32_t i = tid; i < m; i += nt) dl[i] = delta[i];
-    __syncthreads();
-    // stable descending sort of delta: rank_i = #{j: dl_j > dl_i} + #{j < i: dl_j == dl_i}
-    for (uint32_t i = tid; i < m; i += nt) {
-        int64_t di = dl[i];
-        uint32_t r = 0;
-        for (uint32_t j = 0; j < m; j++) r += (dl[j] > di) || (dl[j] == di && j < i);
-        perm[r] = i;
-    }
-    for (uint32_t i = tid; i < m; i += nt) {
-        colord[i] = i;
-        sig[i] = 1;
-    }
-    __syncthreads();
-    for (uint32_t idx = tid; idx < m * n; idx += nt) {
-        uint32_t i = idx / n, j = idx % n;
-        R[idx] = G[((size_t)perm[i] * n + j) * gstride + goff] % p;
-        U[idx] = 0;
-    }
-    if (tid == 0) ctl[2] = 0;
-    __syncthreads();
-
-    for (uint32_t k = 0; k < m; k++) {
-        uint32_t rank = ctl[2];
-        if (rank >= n) break;  // every later row is zero: rank <= n
-        if (tid == 0) {
-            uint32_t pos = m;
-            for (uint32_t q = rank; q < m; q++) {
-                uint32_t c = colord[q];
-                if (c < n && R[k * n + c] != 0) { pos = q; break; }
-            }
-            ctl[0] = pos;
-            if (pos < m) {
-                uint32_t c = colord[pos];
-                colord[pos] = colord[rank];
-                colord[rank] = c;
-                ctl[1] = c;
-                pivrow[rank] = k;
-            }
-        }
-        __syncthreads();
-        const uint32_t pos = ctl[0], c = ctl[1];
-        __syncthreads();  // ctl is rewritten by thread 0 in the next iteration
-        if (pos >= m) continue;  // zero row (uniform branch)
-        for (uint32_t i = k + 1 + tid; i < m; i += nt) fcol[i] = R[i * n + c];
-        __syncthreads();
-        const uint32_t pk = R[k * n + c], sk = sig[k];
-        const uint32_t W = n + rank + 1;  // R columns, U columns < rank, the new U column
-        for (uint32_t idx = tid; idx < (m - k - 1) * W; idx += nt) {
-            uint32_t i = k + 1 + idx / W, col = idx % W;
-            uint32_t f = fcol[i];
-            if (f == 0) continue;
-            if (col < n) {
-                R[i * n + col] = submod(mulmod(pk, R[i * n + col], p, inv), mulmod(f, R[k * n + col], p, inv), p);
-            } else if (col < n + rank) {
-                uint32_t q = col - n;
-                U[i * n + q] = submod(mulmod(pk, U[i * n + q], p, inv), mulmod(f, U[k * n + q], p, inv), p);
-            } else {
-                U[i * n + rank] = submod(0u, mulmod(f, sk, p, inv), p);
-                sig[i] = mulmod(pk, sig[i], p, inv);
-            }
-        }
-        if (tid == 0) ctl[2] = rank + 1;
-        __syncthreads();
-    }
-    const uint32_t rank = ctl[2];
-    // sigma_i^-1 in parallel (Fermat), then Linv row i = coefficients / sigma_i
-    for (uint32_t i = tid; i < m; i += nt) {
-        uint32_t s = sig[i], r = 1, e = p - 2;
-        while (e) {
-            if (e & 1) r = mulmod(r, s, p, inv);
-            s = mulmod(s, s, p, inv);
-            e >>= 1;
-        }
-        sig[i] = r;
-    }
-    __syncthreads();
-    // row i is nonzero iff it was a pivot row
-    for (uint32_t idx = tid; idx < m * m; idx += nt) {
-        uint32_t i = idx / m, j = idx % m;
-        M[(size_t)idx * 2] = 0;
-        M[(size_t)idx * 2 + 1] = 0;
-        (void)j;
-    }
-    __syncthreads();
-    for (uint32_t i = tid; i < m; i += nt) {
-        bool nz = false;
-        for (uint32_t q = 0; q < rank; q++) nz |= pivrow[q] == i;
-        uint32_t slot = nz ? 1 : 0;
-        // T = Linv pi: T[i][perm[j]] = Linv[i][j]
-        M[((size_t)i * m + perm[i]) * 2 + slot] = 1;
-        for (uint32_t q = 0; q < rank; q++) {
-            uint32_t k = pivrow[q];
-            if (k >= i) continue;
-            uint32_t v = mulmod(U[i * n + q], sig[i], p, inv);
-            M[((size_t)i * m + perm[k]) * 2 + slot] = v;
-        }
-        delta[i] = dl[perm[i]] - (nz ? 1 : 0);
-    }
-}
-
+// Leaf: leaf_order(m) or fewer M-Basis steps in one workgroup.  Step s takes the
+// constant term of the current residual P = M_s G (coefficient s), sorts its
+// rows by delta (stable, descending, as Rust's sort_by), eliminates LSP-style
+// (pivot = first nonzero position >= rank in the swapped column order,
+// src/modular_linalg.rs:188-239) fraction-free, so no pivot needs an inverse
+// inside the loop: every row carries its multiplier sigma_i and its
+// coefficients on earlier pivot rows; one parallel inversion gives Linv.
+// The step basis is T = D1 Linv pi + x Dx Linv pi (src/sigma_basis.rs:384-396),
+// and both P and M advance by it: row i of T X is row perm(i) of X plus the
+// Linv coefficients times the pivot rows, shifted by x when row i was a
+// pivot row.  This is the reference's chain of order-1 leaves, kept in one
+// launch.
+// Steps per leaf: the in-leaf basis update costs m^2 rank per coefficient,
+// so wider blocks take shorter leaves.
+__host__ __device__ constexpr uint32_t leaf_order(uint32_t m) { return m <= 16 ? 16 : m <= 32 ? 8 : m <= 64 ? 4 : 1; }
 constexpr int LEAF_PACK = 64;
 struct LeafPack {
     LeafDesc d[LEAF_PACK];
 };
 
-// one workgroup per leaf
-__global__ __launch_bounds__(256) void k_mbasis_batch(LeafPack pk, uint32_t p, double pd, double inv) {
-    const LeafDesc &D = pk.d[blockIdx.x];
-    mbasis_wg(D.G, D.gs, 0, D.m, D.n, D.delta, D.M, p, pd, inv);
+// LDS of a leaf: the residual P [m][n][K] and the basis M [m][m][K+1]
+// stay resident (K > 1), with one coefficient slice X for the in-place
+// update, the elimination arrays R, U and the control words.
+struct LeafLds {
+    uint32_t *R, *U, *sig, *fcol, *perm, *colord, *pivrow, *isx;
+    uint32_t *ctl;  // two slots of {pivot position, pivot column, rank at the row, -}
+    uint32_t *prow;  // per pivot q: the row of X it reads, perm(pivrow q)
+    uint32_t *P, *M;
+    int64_t *dl, *dln;
+};
+
+__device__ LeafLds leaf_lds(uint32_t m, uint32_t n, uint32_t K) {
+    extern __shared__ uint32_t sm[];
+    LeafLds L;
+    L.R = sm;
+    L.U = L.R + m * n;
+    L.sig = L.U + m * n;
+    L.fcol = L.sig + m;
+    L.perm = L.fcol + m;
+    L.colord = L.perm + m;
+    L.isx = L.colord + m;
+    L.pivrow = L.isx + m;
+    L.prow = L.pivrow + n;
+    L.ctl = L.prow + n;
+    L.dl = (int64_t *)(L.ctl + 8 + ((5 * m + 2 * n) & 1));  // 8-byte aligned
+    L.dln = L.dl + m;
+    L.P = (uint32_t *)(L.dln + m);
+    L.M = L.P + (K > 1 ? m * n * K : 0);
+    return L;
 }
 
-static size_t leaf_lds_bytes(uint32_t m, uint32_t n) {
-    return (size_t)(2 * m * n + 4 * m + n + 6) * 4 + (size_t)m * 8 + 16;
+static size_t leaf_lds_words(uint32_t m, uint32_t n, uint32_t K) {
+    size_t w = (size_t)2 * m * n + 5 * m + 2 * n + 10 + 4 * m + 2;
+    if (K > 1) w += (size_t)m * n * K + (size_t)m * m * (K + 1);
+    return w;
+}
+
+// In-place X <- T X on coefficients: X [m][cols][stride] valid on [klo, khi],
+// rewritten on [olo, ohi].  Row i of T X is row perm(i) of X plus the Linv
+// coefficients U'[i][q] times the pivot rows L.prow[q], shifted up one
+// coefficient when row i was a pivot row.  Every output is computed into
+// registers before any is stored (inputs and outputs share X); outputs no
+// input reaches are zeroed.  Lanes run over coefficients first, so a
+// wavefront reads consecutive words of X.
+constexpr uint32_t LEAF_THREADS = 1024;
+constexpr uint32_t ADV_MAX = 16;  // outputs per thread: m * cols * (khi - klo + 1) <= ADV_MAX * LEAF_THREADS
+__device__ void leaf_advance(const LeafLds &L, uint32_t *X, uint32_t cols, uint32_t stride, uint32_t klo,
+                             uint32_t khi, uint32_t olo, uint32_t ohi, uint32_t rank, uint32_t m, uint32_t n,
+                             uint32_t p, double pd, double inv) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nsl = khi - klo + 1, tot = m * cols * nsl;
+    uint32_t res[ADV_MAX];
+#pragma unroll
+    for (uint32_t e = 0; e < ADV_MAX; e++) {
+        const uint32_t idx = tid + e * LEAF_THREADS;
+        res[e] = 0;
+        if (idx < tot) {
+            const uint32_t rc = idx / nsl, kk = klo + (idx - rc * nsl);
+            const uint32_t i = rc / cols, col = rc - i * cols;
+            const uint32_t *xc = X + col * stride + kk;
+            double hacc = 0.0, lacc = (double)xc[L.perm[i] * cols * stride];
+            const uint32_t *ui = L.U + i * n;
+            for (uint32_t q0 = 0; q0 < rank; q0 += 32) {
+                const uint32_t qe = min(rank, q0 + 32);
+                for (uint32_t q = q0; q < qe; q++) {
+                    const uint32_t u = ui[q];
+                    const double xd = (double)xc[L.prow[q] * cols * stride];
+                    hacc = fma((double)(u >> 16), xd, hacc);
+                    lacc = fma((double)(u & 0xFFFFu), xd, lacc);
+                }
+                hacc = fold(hacc, pd, inv);
+                lacc = fold(lacc, pd, inv);
+            }
+            res[e] = (uint32_t)fold(fma(hacc, 65536.0, lacc), pd, inv);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t e = 0; e < ADV_MAX; e++) {
+        const uint32_t idx = tid + e * LEAF_THREADS;
+        if (idx < tot) {
+            const uint32_t rc = idx / nsl, kk = klo + (idx - rc * nsl);
+            const uint32_t ko = kk + L.isx[rc / cols];
+            if (ko >= olo && ko <= ohi) X[rc * stride + ko] = res[e];
+        }
+    }
+    for (uint32_t idx = tid; idx < m * cols; idx += LEAF_THREADS) {
+        const uint32_t sh = L.isx[idx / cols];
+        for (uint32_t ko = olo; ko <= ohi; ko++)
+            if (ko < sh || ko - sh < klo || ko - sh > khi) X[idx * stride + ko] = 0u;
+    }
+    __syncthreads();
+}
+
+// One M-Basis step on the constant term already in L.R (rows in the delta
+// order L.perm): fraction-free LSP, then U' = Linv coefficients, isx, the
+// new delta in L.dln.  Returns the rank.
+__device__ uint32_t leaf_eliminate(const LeafLds &L, uint32_t m, uint32_t n, uint32_t p, double inv) {
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    uint32_t k_end = m;  // the rank after the loop sits in control slot k_end & 1
+    for (uint32_t k = 0; k < m; k++) {
+        const uint32_t rank = L.ctl[2 + (k & 1) * 4];
+        if (rank >= n) {  // every later row is zero: rank <= n
+            k_end = k;
+            break;
+        }
+        // pivot: the first position q >= rank whose column is nonzero in row k,
+        // found by a ballot of wavefront 0 (m <= 128: at most two words)
+        if (tid < 64) {
+            uint32_t pos = m;
+            for (uint32_t q0 = 0; q0 < m && pos == m; q0 += 64) {
+                const uint32_t q = q0 + tid;
+                bool nz = false;
+                if (q >= rank && q < m) {
+                    const uint32_t cq = L.colord[q];
+                    nz = cq < n && L.R[k * n + cq] != 0;
+                }
+                const uint64_t bal = __ballot(nz);
+                if (bal) pos = q0 + (uint32_t)__ffsll((unsigned long long)bal) - 1;
+            }
+            if (tid == 0) {
+                L.ctl[(k & 1) * 4] = pos;
+                L.ctl[2 + ((k + 1) & 1) * 4] = pos < m ? rank + 1 : rank;
+                if (pos < m) {
+                    const uint32_t c = L.colord[pos];
+                    L.colord[pos] = L.colord[rank];
+                    L.colord[rank] = c;
+                    L.ctl[1 + (k & 1) * 4] = c;
+                    L.pivrow[rank] = k;
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t pos = L.ctl[(k & 1) * 4], c = L.ctl[1 + (k & 1) * 4];
+        if (pos >= m) continue;  // zero row (uniform branch); the next row reads the other control slot
+        // Rows below lose their column-c entry: that entry (the factor f) is
+        // read, never rewritten, and column c is never searched again.
+        // Lanes run over the W columns of a row, wavefronts over rows.
+        const uint32_t pkv = L.R[k * n + c], sk = L.sig[k];
+        const uint32_t W = n + rank + 1;  // R columns, U columns < rank, the new U column
+        for (uint32_t i = k + 1 + tid / 64; i < m; i += nt / 64) {
+            const uint32_t f = L.R[i * n + c];
+            if (f == 0) continue;
+            for (uint32_t col = tid % 64; col < W; col += 64) {
+                if (col < n) {
+                    if (col != c)
+                        L.R[i * n + col] =
+                            submod(mulmod(pkv, L.R[i * n + col], p, inv), mulmod(f, L.R[k * n + col], p, inv), p);
+                } else if (col < n + rank) {
+                    const uint32_t q = col - n;
+                    L.U[i * n + q] = submod(mulmod(pkv, L.U[i * n + q], p, inv), mulmod(f, L.U[k * n + q], p, inv), p);
+                } else {
+                    L.U[i * n + rank] = submod(0u, mulmod(f, sk, p, inv), p);
+                    L.sig[i] = mulmod(pkv, L.sig[i], p, inv);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const uint32_t rank = L.ctl[2 + (k_end & 1) * 4];
+    // Linv row i = e_i + (U row i) / sigma_i over the earlier pivot rows
+    for (uint32_t i = tid; i < m; i += nt) {
+        uint32_t sv = L.sig[i], r = 1, e = p - 2;
+        while (e) {
+            if (e & 1) r = mulmod(r, sv, p, inv);
+            sv = mulmod(sv, sv, p, inv);
+            e >>= 1;
+        }
+        L.fcol[i] = r;
+    }
+    for (uint32_t q = tid; q < rank; q += nt) {
+        L.isx[L.pivrow[q]] = 1;
+        L.prow[q] = L.perm[L.pivrow[q]];
+    }
+    __syncthreads();
+    for (uint32_t idx = tid; idx < m * rank; idx += nt) {
+        const uint32_t i = idx / rank, q = idx % rank;
+        const uint32_t u = L.U[i * n + q];
+        L.U[i * n + q] = L.pivrow[q] < i ? mulmod(u, L.fcol[i], p, inv) : 0u;
+    }
+    for (uint32_t i = tid; i < m; i += nt) L.dln[i] = L.dl[L.perm[i]] - (int64_t)L.isx[i];
+    __syncthreads();
+    return rank;
+}
+
+// Sorts rows by delta (stable, descending: rank_i = #{j: dl_j > dl_i} +
+// #{j < i: dl_j == dl_i}) and resets the elimination state.
+__device__ void leaf_sort(const LeafLds &L, uint32_t m) {
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    for (uint32_t i = tid; i < m; i += nt) {
+        const int64_t di = L.dl[i];
+        uint32_t r = 0;
+        for (uint32_t j = 0; j < m; j++) r += (L.dl[j] > di) || (L.dl[j] == di && j < i);
+        L.perm[r] = i;
+    }
+    for (uint32_t i = tid; i < m; i += nt) {
+        L.colord[i] = i;
+        L.sig[i] = 1;
+        L.isx[i] = 0;
+    }
+    if (tid == 0) L.ctl[2] = 0;
+    __syncthreads();
+}
+
+// D.G: residual window G(i, j, k) = G[(i * n + j) * gs + k], k < d; D.M: [m][m][d + 1].
+__global__ __launch_bounds__(LEAF_THREADS) void k_mbasis_iter(LeafPack pk, uint32_t p, double pd, double inv) {
+    const LeafDesc D = pk.d[blockIdx.x];
+    const uint32_t m = D.m, n = D.n, K = D.d;
+    const LeafLds L = leaf_lds(m, n, K);
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    for (uint32_t i = tid; i < m; i += nt) L.dl[i] = D.delta[i];
+    __syncthreads();
+    if (K == 1) {  // a single step: M = D1 Linv pi + x Dx Linv pi, written directly
+        leaf_sort(L, m);
+        for (uint32_t idx = tid; idx < m * n; idx += nt) {
+            const uint32_t i = idx / n, j = idx % n;
+            L.R[idx] = D.G[((size_t)L.perm[i] * n + j) * D.gs] % p;
+            L.U[idx] = 0;
+        }
+        __syncthreads();
+        const uint32_t rank = leaf_eliminate(L, m, n, p, inv);
+        for (uint32_t idx = tid; idx < m * m; idx += nt) {
+            D.M[(size_t)idx * 2] = 0;
+            D.M[(size_t)idx * 2 + 1] = 0;
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < m; i += nt) {
+            const uint32_t slot = L.isx[i];
+            D.M[((size_t)i * m + L.perm[i]) * 2 + slot] = 1;
+            for (uint32_t q = 0; q < rank; q++)
+                if (L.pivrow[q] < i) D.M[((size_t)i * m + L.perm[L.pivrow[q]]) * 2 + slot] = L.U[i * n + q];
+            D.delta[i] = L.dln[i];
+        }
+        return;
+    }
+    const uint32_t MS = K + 1;
+    for (uint32_t idx = tid; idx < m * n * K; idx += nt) {
+        const uint32_t ij = idx / K, k = idx - ij * K;
+        L.P[idx] = D.G[(size_t)ij * D.gs + k] % p;
+    }
+    for (uint32_t idx = tid; idx < m * m * MS; idx += nt) {
+        const uint32_t ij = idx / MS, k = idx - ij * MS;
+        L.M[idx] = (k == 0 && ij / m == ij % m) ? 1u : 0u;
+    }
+    __syncthreads();
+    for (uint32_t s = 0; s < K; s++) {
+        leaf_sort(L, m);
+        for (uint32_t idx = tid; idx < m * n; idx += nt) {
+            const uint32_t i = idx / n, j = idx - i * n;
+            L.R[idx] = L.P[(L.perm[i] * n + j) * K + s];
+            L.U[idx] = 0;
+        }
+        __syncthreads();
+        const uint32_t rank = leaf_eliminate(L, m, n, p, inv);
+        // P <- T P on the coefficients later steps read (s .. K-1 in, s+1 .. K-1 out)
+        if (s + 1 < K) leaf_advance(L, L.P, n, K, s, K - 1, s + 1, K - 1, rank, m, n, p, pd, inv);
+        // M <- T M (degree s -> s + 1)
+        leaf_advance(L, L.M, m, MS, 0, s, 0, s + 1, rank, m, n, p, pd, inv);
+        for (uint32_t i = tid; i < m; i += nt) L.dl[i] = L.dln[i];
+        __syncthreads();
+    }
+    for (uint32_t idx = tid; idx < m * m * MS; idx += nt) D.M[idx] = L.M[idx];
+    for (uint32_t i = tid; i < m; i += nt) D.delta[i] = L.dl[i];
 }
 
 constexpr int MUL_PACK = 48;
@@ -230,11 +383,11 @@ static bool next_op(PmJob &J, Op &op, rr_status &err) {
     while (!J.stack.empty()) {
         size_t fi = J.stack.size() - 1;
         Frame &F = J.stack[fi];
-        if (F.d == 1) {
+        if (F.d <= leaf_order(J.m)) {
             if (F.stage == 0) {
                 F.stage = 1;
                 op.kind = 0;
-                op.leaf = LeafDesc{F.G, F.gs, J.delta, F.M, J.m, J.b};
+                op.leaf = LeafDesc{F.G, F.gs, J.delta, F.M, J.m, J.b, (uint32_t)F.d};
                 return true;
             }
             J.stack.pop_back();
@@ -315,10 +468,17 @@ rr_status PmScheduler::init(uint32_t p_, hipStream_t st_, uint32_t max_logn, uin
     rr_status s;
     if (p == NTT_PRIME && (s = tables.init(std::max<uint32_t>(2, std::min(max_logn, NTT_MAX_LOGN)), st)) != RR_OK)
         return s;
-    lds = leaf_lds_bytes(max_m, max_n);
-    if (lds > 65536)
-        RR_HIP(hipFuncSetAttribute((const void *)k_mbasis_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
+    // any mix of block widths: the largest leaf (b = 32: m = 64, K = 4) needs 150 KB
+    lds = 0;
+    for (uint32_t mm = 2; mm <= 128; mm *= 2) {
+        const uint32_t K = leaf_order(mm);
+        lds = std::max(lds, leaf_lds_words(mm, mm / 2, K) * 4);
+        RR_CHECK(K == 1 || (size_t)mm * mm * K <= (size_t)ADV_MAX * LEAF_THREADS, RR_EINVAL, "leaf too large");
+    }
+    (void)max_m;
+    (void)max_n;
+    RR_CHECK(lds <= 160 * 1024, RR_EINVAL, "leaf needs %zu bytes of LDS", lds);
+    RR_HIP(hipFuncSetAttribute((const void *)k_mbasis_iter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     return RR_OK;
 }
 
@@ -359,7 +519,9 @@ rr_status PmScheduler::round(std::vector<PmJob *> &jobs, bool *launched) {
         LeafPack pk;
         uint32_t cnt = (uint32_t)std::min<size_t>(LEAF_PACK, leaves.size() - i);
         for (uint32_t k = 0; k < cnt; k++) pk.d[k] = leaves[i + k];
-        hipLaunchKernelGGL(k_mbasis_batch, dim3(cnt), dim3(256), lds, st, pk, p, mp.pd, mp.inv);
+        size_t lw = 0;
+        for (uint32_t k = 0; k < cnt; k++) lw = std::max(lw, leaf_lds_words(pk.d[k].m, pk.d[k].n, pk.d[k].d));
+        hipLaunchKernelGGL(k_mbasis_iter, dim3(cnt), dim3(LEAF_THREADS), lw * 4, st, pk, p, mp.pd, mp.inv);
         RR_HIP(hipGetLastError());
         launches++;
     }
@@ -405,33 +567,50 @@ rr_status PmScheduler::run(std::vector<PmJob *> &jobs) {
 // Each NTT product: forward transforms of both operands into round scratch,
 // pointwise products (with 1/N), inverse transforms into the output window.
 rr_status PmScheduler::run_ntt_products(std::vector<std::pair<uint32_t, MulDesc>> &ops) {
-        size_t words = 0;
-        for (auto &o : ops) {
-            size_t N = (size_t)1 << o.first;
-            const MulDesc &D = o.second;
-            words += (size_t)(D.rows * D.inner + D.inner * D.cols + D.rows * D.cols) * N + 192;
-        }
-        rr_status s;
-        if ((s = ensure_scratch(words)) != RR_OK) return s;
-        size_t off = 0;
-        for (auto &o : ops) {
-            const uint32_t logn = o.first, N = 1u << logn;
-            const MulDesc &D = o.second;
+    size_t words = 0;
+    for (auto &o : ops) {
+        size_t N = (size_t)1 << o.first;
+        const MulDesc &D = o.second;
+        words += (size_t)(D.rows * D.inner + D.inner * D.cols + D.rows * D.cols) * N + 192;
+    }
+    rr_status s;
+    if ((s = ensure_scratch(words)) != RR_OK) return s;
+    // same-size products share their launches (forward, pointwise, inverse)
+    std::stable_sort(ops.begin(), ops.end(),
+                     [](const std::pair<uint32_t, MulDesc> &x, const std::pair<uint32_t, MulDesc> &y) {
+                         return x.first < y.first;
+                     });
+    size_t off = 0;
+    for (size_t g0 = 0; g0 < ops.size();) {
+        const uint32_t logn = ops[g0].first, N = 1u << logn;
+        RR_CHECK(logn <= tables.logn, RR_EINVAL, "transform 2^%u above table 2^%u", logn, tables.logn);
+        size_t g1 = g0;
+        while (g1 < ops.size() && ops[g1].first == logn && g1 - g0 < NTT_PACK / 2) g1++;
+        NttFwdPack fp;
+        PwPack pp;
+        NttInvPack ip;
+        uint32_t nf = 0, np = 0;
+        for (size_t k = g0; k < g1; k++) {
+            const MulDesc &D = ops[k].second;
             uint32_t *fa = scratch.as<uint32_t>() + off;
             off += ((size_t)D.rows * D.inner * N + 63) & ~size_t(63);
             uint32_t *fb = scratch.as<uint32_t>() + off;
             off += ((size_t)D.inner * D.cols * N + 63) & ~size_t(63);
             uint32_t *fc = scratch.as<uint32_t>() + off;
             off += ((size_t)D.rows * D.cols * N + 63) & ~size_t(63);
-            RR_CHECK(logn <= tables.logn, RR_EINVAL, "transform 2^%u above table 2^%u", logn, tables.logn);
-            if ((s = ntt_forward(tables, D.A, D.la, 0, D.la, fa, D.rows * D.inner, logn, st)) != RR_OK) return s;
-            if ((s = ntt_forward(tables, D.B, D.sb, D.ob, D.lb, fb, D.inner * D.cols, logn, st)) != RR_OK) return s;
-            if ((s = pointwise(fa, fb, fc, D.rows, D.inner, D.cols, N, h_invmod(N, NTT_PRIME), st)) != RR_OK) return s;
-            if ((s = ntt_inverse(tables, fc, D.rows * D.cols, logn, D.lo, D.hi - D.lo, D.C, D.hi - D.lo, st)) != RR_OK)
-                return s;
-            launches += 4;
+            fp.d[nf++] = NttFwdDesc{D.A, D.la, 0, D.la, fa, D.rows * D.inner};
+            fp.d[nf++] = NttFwdDesc{D.B, D.sb, D.ob, D.lb, fb, D.inner * D.cols};
+            pp.d[np] = PwDesc{fa, fb, fc, D.rows, D.inner, D.cols, h_invmod(N, NTT_PRIME)};
+            ip.d[np] = NttInvDesc{fc, D.rows * D.cols, D.lo, D.hi - D.lo, D.C, D.hi - D.lo};
+            np++;
         }
-        return RR_OK;
+        if ((s = ntt_forward_pack(tables, fp, nf, logn, st)) != RR_OK) return s;
+        if ((s = pointwise_pack(pp, np, N, st)) != RR_OK) return s;
+        if ((s = ntt_inverse_pack(tables, ip, np, logn, st)) != RR_OK) return s;
+        launches += 3;
+        g0 = g1;
+    }
+    return RR_OK;
 }
 
 // Device PM-basis for several matrices at once.  For each job: G [m][b][d+1]

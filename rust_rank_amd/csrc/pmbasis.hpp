@@ -8,11 +8,11 @@
 namespace rr {
 
 struct LeafDesc {
-    const uint32_t *G;  // constant term G(i, j) at G[(i * n + j) * gs]
+    const uint32_t *G;  // residual G(i, j, k) at G[(i * n + j) * gs + k], k < d
     uint64_t gs;
     int64_t *delta;
-    uint32_t *M;  // [m][m][2]
-    uint32_t m, n;
+    uint32_t *M;  // [m][m][d + 1]
+    uint32_t m, n, d;
 };
 // C [rows][cols][hi-lo] = window [lo, hi) of A [rows][inner][la] * B [inner][cols][ob .. ob+lb) (stride sb)
 struct MulDesc {
