@@ -28,47 +28,69 @@ __device__ __forceinline__ uint32_t shoup_mul(uint32_t a, uint32_t w, uint32_t w
 }
 
 constexpr int NTT_THREADS = 256;
-constexpr uint32_t NTT_LDS_WORDS = 16384;
+constexpr uint32_t NTT_MAX_WORDS = 16384;  // largest transform, one workgroup's LDS
+constexpr uint32_t NTT_WG_WORDS = 2048;    // shorter transforms share a workgroup up to this many words
+constexpr uint32_t NTT_U = 4;              // butterflies per thread in flight
+
+__host__ __device__ constexpr uint32_t ntt_per_wg(uint32_t N) { return N >= NTT_WG_WORDS ? 1 : NTT_WG_WORDS / N; }
+
+// One radix-2 stage over `total` butterflies of span h = 2^lh in LDS.  Every
+// thread issues NTT_U butterflies' LDS and twiddle loads before it computes;
+// butterflies past `total` repeat the thread's first one and are not stored.
+template <bool DIF>
+__device__ __forceinline__ void lds_stage(uint32_t *s, uint32_t total, uint32_t logn, uint32_t lh, const uint32_t *tw,
+                                          const uint32_t *twq, uint32_t tstride, uint32_t p) {
+    const uint32_t h = 1u << lh, N = 1u << logn, half = N >> 1;
+    for (uint32_t t0 = threadIdx.x; t0 < total; t0 += NTT_THREADS * NTT_U) {
+        uint32_t i0[NTT_U], x0[NTT_U], x1[NTT_U], w[NTT_U], wq[NTT_U];
+#pragma unroll
+        for (uint32_t u = 0; u < NTT_U; u++) {
+            uint32_t t = t0 + u * NTT_THREADS;
+            t = t < total ? t : t0;
+            const uint32_t poly = t >> (logn - 1), bfl = t & (half - 1);
+            const uint32_t j = bfl & (h - 1), blk = bfl >> lh;
+            i0[u] = poly * N + blk * 2 * h + j;
+            x0[u] = s[i0[u]];
+            x1[u] = s[i0[u] + h];
+            w[u] = tw[j * tstride];
+            wq[u] = twq[j * tstride];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < NTT_U; u++) {
+            uint32_t y0, y1;
+            if constexpr (DIF) {
+                y0 = addmod(x0[u], x1[u], p);
+                y1 = shoup_mul(submod(x0[u], x1[u], p), w[u], wq[u], p);
+            } else {
+                const uint32_t bt = shoup_mul(x1[u], w[u], wq[u], p);
+                y0 = addmod(x0[u], bt, p);
+                y1 = submod(x0[u], bt, p);
+            }
+            if (t0 + u * NTT_THREADS < total) {
+                s[i0[u]] = y0;
+                s[i0[u] + h] = y1;
+            }
+        }
+    }
+    __syncthreads();
+}
 
 // In-LDS DIF over `npoly` polynomials of length N = 2^logn laid out back to back.
 __device__ void lds_dif(uint32_t *s, uint32_t npoly, uint32_t logn, const uint32_t *tw, const uint32_t *twq,
                         uint32_t tw_logn, uint32_t p) {
-    const uint32_t N = 1u << logn, half = N >> 1, total = npoly * half;
-    for (uint32_t lh = logn; lh-- > 0;) {  // butterfly span h = 2^lh
-        const uint32_t h = 1u << lh;
-        const uint32_t tstride = 1u << (tw_logn - lh - 1);  // w_{2h}^j = w_Ntab^(j * Ntab / 2h)
-        for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) {
-            uint32_t poly = t >> (logn - 1), bfl = t & (half - 1);
-            uint32_t j = bfl & (h - 1), blk = bfl >> lh;
-            uint32_t i0 = poly * N + blk * 2 * h + j, i1 = i0 + h;
-            uint32_t a = s[i0], b = s[i1];
-            uint32_t k = j * tstride;
-            s[i0] = addmod(a, b, p);
-            s[i1] = shoup_mul(submod(a, b, p), tw[k], twq[k], p);
-        }
-        __syncthreads();
-    }
+    const uint32_t total = npoly << (logn - 1);
+    for (uint32_t lh = logn; lh-- > 0;)  // butterfly span h = 2^lh; w_{2h}^j = w_Ntab^(j * Ntab / 2h)
+        lds_stage<true>(s, total, logn, lh, tw, twq, 1u << (tw_logn - lh - 1), p);
 }
 
 // In-LDS DIT (inverse twiddles in tw), bit-reversed in, natural out, unscaled.
 __device__ void lds_dit(uint32_t *s, uint32_t npoly, uint32_t logn, const uint32_t *tw, const uint32_t *twq,
                         uint32_t tw_logn, uint32_t p) {
-    const uint32_t N = 1u << logn, half = N >> 1, total = npoly * half;
-    for (uint32_t lh = 0; lh < logn; lh++) {
-        const uint32_t h = 1u << lh;
-        const uint32_t tstride = 1u << (tw_logn - lh - 1);
-        for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) {
-            uint32_t poly = t >> (logn - 1), bfl = t & (half - 1);
-            uint32_t j = bfl & (h - 1), blk = bfl >> lh;
-            uint32_t i0 = poly * N + blk * 2 * h + j, i1 = i0 + h;
-            uint32_t k = j * tstride;
-            uint32_t a = s[i0], b = shoup_mul(s[i1], tw[k], twq[k], p);
-            s[i0] = addmod(a, b, p);
-            s[i1] = submod(a, b, p);
-        }
-        __syncthreads();
-    }
+    const uint32_t total = npoly << (logn - 1);
+    for (uint32_t lh = 0; lh < logn; lh++) lds_stage<false>(s, total, logn, lh, tw, twq, 1u << (tw_logn - lh - 1), p);
 }
+
+constexpr uint32_t NTT_LD = 8;  // global loads per thread in flight when staging
 
 // Forward transforms: dst[q][0..N) = DIF(src[q][0..len) zero-padded), for the
 // polynomials q of a [rows][cols] matrix with source stride `sstride` (words
@@ -76,39 +98,75 @@ __device__ void lds_dit(uint32_t *s, uint32_t npoly, uint32_t logn, const uint32
 // coefficient).  blockIdx.y selects the descriptor of the pack.
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_fwd(NttFwdPack pk, uint32_t logn, const uint32_t *tw,
                                                          const uint32_t *twq, uint32_t tw_logn, uint32_t p) {
-    __shared__ uint32_t s[NTT_LDS_WORDS];
+    extern __shared__ uint32_t s[];
     const NttFwdDesc &D = pk.d[blockIdx.y];
-    const uint32_t N = 1u << logn;
-    const uint32_t per = N >= NTT_LDS_WORDS ? 1 : NTT_LDS_WORDS / N;
+    const uint32_t N = 1u << logn, per = ntt_per_wg(N);
     const uint32_t q0 = blockIdx.x * per;
     if (q0 >= D.count) return;
-    const uint32_t np = min(per, D.count - q0);
-    for (uint32_t i = threadIdx.x; i < np * N; i += blockDim.x) {
-        uint32_t q = q0 + i / N, k = i % N;
-        s[i] = k < D.len ? D.src[(size_t)q * D.sstride + D.soff + k] : 0u;
+    const uint32_t np = min(per, D.count - q0), words = np * N;
+    const uint32_t *src = D.src + (size_t)q0 * D.sstride + D.soff;
+    // lanes past the data or the source length load the first word and store zero
+    for (uint32_t i0 = threadIdx.x; i0 < words; i0 += NTT_THREADS * NTT_LD) {
+        uint32_t v[NTT_LD];
+#pragma unroll
+        for (uint32_t e = 0; e < NTT_LD; e++) {
+            const uint32_t i = i0 + e * NTT_THREADS, q = i >> logn, k = i & (N - 1);
+            const bool ok = i < words && k < D.len;
+            v[e] = D.len ? src[ok ? (size_t)q * D.sstride + k : 0] : 0u;
+        }
+#pragma unroll
+        for (uint32_t e = 0; e < NTT_LD; e++) {
+            const uint32_t i = i0 + e * NTT_THREADS;
+            if (i < words) s[i] = (i & (N - 1)) < D.len ? v[e] : 0u;
+        }
     }
     __syncthreads();
     lds_dif(s, np, logn, tw, twq, tw_logn, p);
-    for (uint32_t i = threadIdx.x; i < np * N; i += blockDim.x) D.dst[(size_t)q0 * N + i] = s[i];
+    uint32_t *dst = D.dst + (size_t)q0 * N;
+    for (uint32_t i0 = threadIdx.x; i0 < words; i0 += NTT_THREADS * NTT_LD) {
+        uint32_t v[NTT_LD];
+#pragma unroll
+        for (uint32_t e = 0; e < NTT_LD; e++) v[e] = s[min(i0 + e * NTT_THREADS, words - 1)];
+#pragma unroll
+        for (uint32_t e = 0; e < NTT_LD; e++)
+            if (i0 + e * NTT_THREADS < words) dst[i0 + e * NTT_THREADS] = v[e];
+    }
 }
 
 // Inverse transforms of `count` spectra (already scaled by 1/N), writing
 // coefficients [lo, lo + outlen) of each to dst[q * dstride + k - lo].
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_inv(NttInvPack pk, uint32_t logn, const uint32_t *tw,
                                                          const uint32_t *twq, uint32_t tw_logn, uint32_t p) {
-    __shared__ uint32_t s[NTT_LDS_WORDS];
+    extern __shared__ uint32_t s[];
     const NttInvDesc &D = pk.d[blockIdx.y];
-    const uint32_t N = 1u << logn;
-    const uint32_t per = N >= NTT_LDS_WORDS ? 1 : NTT_LDS_WORDS / N;
+    const uint32_t N = 1u << logn, per = ntt_per_wg(N);
     const uint32_t q0 = blockIdx.x * per;
     if (q0 >= D.count) return;
-    const uint32_t np = min(per, D.count - q0);
-    for (uint32_t i = threadIdx.x; i < np * N; i += blockDim.x) s[i] = D.src[(size_t)q0 * N + i];
+    const uint32_t np = min(per, D.count - q0), words = np * N;
+    const uint32_t *src = D.src + (size_t)q0 * N;
+    for (uint32_t i0 = threadIdx.x; i0 < words; i0 += NTT_THREADS * NTT_LD) {
+        uint32_t v[NTT_LD];
+#pragma unroll
+        for (uint32_t e = 0; e < NTT_LD; e++) v[e] = src[min(i0 + e * NTT_THREADS, words - 1)];
+#pragma unroll
+        for (uint32_t e = 0; e < NTT_LD; e++)
+            if (i0 + e * NTT_THREADS < words) s[i0 + e * NTT_THREADS] = v[e];
+    }
     __syncthreads();
     lds_dit(s, np, logn, tw, twq, tw_logn, p);
-    for (uint32_t i = threadIdx.x; i < np * D.outlen; i += blockDim.x) {
-        uint32_t q = i / D.outlen, k = i % D.outlen;
-        D.dst[(size_t)(q0 + q) * D.dstride + k] = s[q * N + D.lo + k];
+    const uint32_t outs = np * D.outlen;
+    for (uint32_t i0 = threadIdx.x; i0 < outs; i0 += NTT_THREADS * NTT_LD) {
+        uint32_t v[NTT_LD];
+#pragma unroll
+        for (uint32_t e = 0; e < NTT_LD; e++) {
+            const uint32_t i = min(i0 + e * NTT_THREADS, outs - 1), q = i / D.outlen, k = i - q * D.outlen;
+            v[e] = s[q * N + D.lo + k];
+        }
+#pragma unroll
+        for (uint32_t e = 0; e < NTT_LD; e++) {
+            const uint32_t i = i0 + e * NTT_THREADS, q = i / D.outlen, k = i - q * D.outlen;
+            if (i < outs) D.dst[(size_t)(q0 + q) * D.dstride + k] = v[e];
+        }
     }
 }
 
@@ -129,32 +187,41 @@ __global__ __launch_bounds__(256) void k_pointwise(PwPack pk, uint32_t ztiles, u
     double hi[JT], lo[JT];
 #pragma unroll
     for (int j = 0; j < JT; j++) hi[j] = lo[j] = 0.0;
+    // columns past the matrix re-read its last column (every load is
+    // unconditional, so a whole row step is in flight at once) and are dropped
     const uint32_t jn = min((uint32_t)JT, cols - j0);
-#pragma unroll 4
-    for (uint32_t r = 0; r < inner; r++) {
-        uint32_t a = A[((size_t)i * inner + r) * N + l];
-        const double ah = (double)(a >> 16), al = (double)(a & 0xFFFFu);
-        const uint32_t *bp = B + ((size_t)r * cols + j0) * N + l;
+    uint32_t boff[JT];
 #pragma unroll
-        for (int j = 0; j < JT; j++) {
-            if ((uint32_t)j < jn) {
-                double bd = (double)bp[(size_t)j * N];
+    for (int j = 0; j < JT; j++) boff[j] = (j0 + min((uint32_t)j, jn - 1)) * N + l;
+    const uint32_t *ap = A + (size_t)i * inner * N + l;
+    const size_t bstep = (size_t)cols * N;
+    for (uint32_t r0 = 0; r0 < inner; r0 += 64) {
+        const uint32_t re = min(inner, r0 + 64);
+#pragma unroll 2
+        for (uint32_t r = r0; r < re; r++) {
+            const uint32_t a = ap[(size_t)r * N];
+            const uint32_t *bp = B + r * bstep;
+            uint32_t bv[JT];
+#pragma unroll
+            for (int j = 0; j < JT; j++) bv[j] = bp[boff[j]];
+            const double ah = (double)(a >> 16), al = (double)(a & 0xFFFFu);
+#pragma unroll
+            for (int j = 0; j < JT; j++) {
+                const double bd = (double)bv[j];
                 hi[j] = fma(ah, bd, hi[j]);
                 lo[j] = fma(al, bd, lo[j]);
             }
         }
-        if ((r & 63) == 63) {
 #pragma unroll
-            for (int j = 0; j < JT; j++) {
-                hi[j] = fold(hi[j], pd, inv);
-                lo[j] = fold(lo[j], pd, inv);
-            }
+        for (int j = 0; j < JT; j++) {
+            hi[j] = fold(hi[j], pd, inv);
+            lo[j] = fold(lo[j], pd, inv);
         }
     }
 #pragma unroll
     for (int j = 0; j < JT; j++) {
         if ((uint32_t)j < jn) {
-            uint32_t v = (uint32_t)fold(fma(fold(hi[j], pd, inv), 65536.0, fold(lo[j], pd, inv)), pd, inv);
+            uint32_t v = (uint32_t)fold(fma(hi[j], 65536.0, lo[j]), pd, inv);
             C[((size_t)i * cols + j0 + j) * N + l] = mulmod(v, scale, p, inv);
         }
     }
@@ -193,10 +260,11 @@ rr_status NttTables::init(uint32_t logn_max, hipStream_t stream) {
 }
 
 static inline uint32_t blocks_for(uint32_t count, uint32_t logn) {
-    uint32_t N = 1u << logn;
-    uint32_t per = N >= NTT_LDS_WORDS ? 1 : NTT_LDS_WORDS / N;
+    const uint32_t per = ntt_per_wg(1u << logn);
     return (count + per - 1) / per;
 }
+
+static inline size_t lds_for(uint32_t logn) { return (size_t)std::max(1u << logn, NTT_WG_WORDS) * 4; }
 
 rr_status ntt_forward_pack(const NttTables &T, const NttFwdPack &pk, uint32_t ndesc, uint32_t logn,
                            hipStream_t st) {
@@ -204,7 +272,7 @@ rr_status ntt_forward_pack(const NttTables &T, const NttFwdPack &pk, uint32_t nd
     RR_CHECK(ndesc >= 1 && ndesc <= NTT_PACK, RR_EINVAL, "bad pack size");
     uint32_t bx = 0;
     for (uint32_t i = 0; i < ndesc; i++) bx = std::max(bx, blocks_for(pk.d[i].count, logn));
-    hipLaunchKernelGGL(k_ntt_fwd, dim3(std::max(bx, 1u), ndesc), dim3(NTT_THREADS), 0, st, pk, logn,
+    hipLaunchKernelGGL(k_ntt_fwd, dim3(std::max(bx, 1u), ndesc), dim3(NTT_THREADS), lds_for(logn), st, pk, logn,
                        T.fwd.as<uint32_t>(), T.fwdq.as<uint32_t>(), T.logn, NTT_PRIME);
     RR_HIP(hipGetLastError());
     return RR_OK;
@@ -216,7 +284,7 @@ rr_status ntt_inverse_pack(const NttTables &T, const NttInvPack &pk, uint32_t nd
     RR_CHECK(ndesc >= 1 && ndesc <= NTT_PACK, RR_EINVAL, "bad pack size");
     uint32_t bx = 0;
     for (uint32_t i = 0; i < ndesc; i++) bx = std::max(bx, blocks_for(pk.d[i].count, logn));
-    hipLaunchKernelGGL(k_ntt_inv, dim3(std::max(bx, 1u), ndesc), dim3(NTT_THREADS), 0, st, pk, logn,
+    hipLaunchKernelGGL(k_ntt_inv, dim3(std::max(bx, 1u), ndesc), dim3(NTT_THREADS), lds_for(logn), st, pk, logn,
                        T.inv.as<uint32_t>(), T.invq.as<uint32_t>(), T.logn, NTT_PRIME);
     RR_HIP(hipGetLastError());
     return RR_OK;

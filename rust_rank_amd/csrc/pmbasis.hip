@@ -59,27 +59,28 @@ struct LeafPack {
 // stay resident (K > 1), with one coefficient slice X for the in-place
 // update, the elimination arrays R, U and the control words.
 struct LeafLds {
-    uint32_t *R, *U, *sig, *fcol, *perm, *colord, *pivrow, *isx;
-    uint32_t *ctl;  // two slots of {pivot position, pivot column, rank at the row, -}
+    uint32_t *R, *U, *sig, *fcol, *perm, *pivrow, *isx;
     uint32_t *prow;  // per pivot q: the row of X it reads, perm(pivrow q)
+    uint32_t *ctl;   // four 16-byte slots of {pivot position, pivot column, rank before the row, -}
     uint32_t *P, *M;
     int64_t *dl, *dln;
 };
 
+__host__ __device__ constexpr uint32_t leaf_ctl_off(uint32_t m, uint32_t n) { return (2 * m * n + 4 * m + 2 * n + 3) & ~3u; }
+
 __device__ LeafLds leaf_lds(uint32_t m, uint32_t n, uint32_t K) {
-    extern __shared__ uint32_t sm[];
+    extern __shared__ __align__(16) uint32_t sm[];
     LeafLds L;
     L.R = sm;
     L.U = L.R + m * n;
     L.sig = L.U + m * n;
     L.fcol = L.sig + m;
     L.perm = L.fcol + m;
-    L.colord = L.perm + m;
-    L.isx = L.colord + m;
+    L.isx = L.perm + m;
     L.pivrow = L.isx + m;
     L.prow = L.pivrow + n;
-    L.ctl = L.prow + n;
-    L.dl = (int64_t *)(L.ctl + 8 + ((5 * m + 2 * n) & 1));  // 8-byte aligned
+    L.ctl = sm + leaf_ctl_off(m, n);
+    L.dl = (int64_t *)(L.ctl + 16);
     L.dln = L.dl + m;
     L.P = (uint32_t *)(L.dln + m);
     L.M = L.P + (K > 1 ? m * n * K : 0);
@@ -87,9 +88,55 @@ __device__ LeafLds leaf_lds(uint32_t m, uint32_t n, uint32_t K) {
 }
 
 static size_t leaf_lds_words(uint32_t m, uint32_t n, uint32_t K) {
-    size_t w = (size_t)2 * m * n + 5 * m + 2 * n + 10 + 4 * m + 2;
+    size_t w = (size_t)leaf_ctl_off(m, n) + 16 + 4 * m;
     if (K > 1) w += (size_t)m * n * K + (size_t)m * m * (K + 1);
     return w;
+}
+
+constexpr uint32_t LEAF_THREADS = 1024;
+constexpr uint32_t ADV_MAX = 16;  // outputs per thread: m * cols * (khi - klo + 1) <= ADV_MAX * LEAF_THREADS
+
+// E outputs per thread (idx = tid + e * LEAF_THREADS), accumulated together so
+// that each pivot row's loads are shared by E independent sums; indices past
+// tot recompute the last output, which is then not stored.
+template <uint32_t E>
+__device__ __forceinline__ void leaf_advance_sums(const LeafLds &L, const uint32_t *X, uint32_t e0, uint32_t tot,
+                                                  uint32_t nsl, uint32_t cols, uint32_t stride, uint32_t klo,
+                                                  uint32_t rank, uint32_t n, double pd, double inv, uint32_t *res) {
+    const uint32_t tid = threadIdx.x, rstride = cols * stride;
+    uint32_t xo[E], uo[E];
+    double hacc[E], lacc[E];
+#pragma unroll
+    for (uint32_t e = 0; e < E; e++) {
+        uint32_t idx = tid + (e0 + e) * LEAF_THREADS;
+        idx = idx < tot ? idx : tot - 1;
+        const uint32_t rc = idx / nsl, kk = klo + (idx - rc * nsl);
+        const uint32_t i = rc / cols, col = rc - i * cols;
+        xo[e] = col * stride + kk;
+        uo[e] = i * n;
+        hacc[e] = 0.0;
+        lacc[e] = (double)X[L.perm[i] * rstride + xo[e]];
+    }
+    for (uint32_t q0 = 0; q0 < rank; q0 += 32) {
+        const uint32_t qe = min(rank, q0 + 32);
+        for (uint32_t q = q0; q < qe; q++) {
+            const uint32_t *xr = X + L.prow[q] * rstride;
+#pragma unroll
+            for (uint32_t e = 0; e < E; e++) {
+                const uint32_t u = L.U[uo[e] + q];
+                const double xd = (double)xr[xo[e]];
+                hacc[e] = fma((double)(u >> 16), xd, hacc[e]);
+                lacc[e] = fma((double)(u & 0xFFFFu), xd, lacc[e]);
+            }
+        }
+#pragma unroll
+        for (uint32_t e = 0; e < E; e++) {
+            hacc[e] = fold(hacc[e], pd, inv);
+            lacc[e] = fold(lacc[e], pd, inv);
+        }
+    }
+#pragma unroll
+    for (uint32_t e = 0; e < E; e++) res[e0 + e] = (uint32_t)fold(fma(hacc[e], 65536.0, lacc[e]), pd, inv);
 }
 
 // In-place X <- T X on coefficients: X [m][cols][stride] valid on [klo, khi],
@@ -99,37 +146,21 @@ static size_t leaf_lds_words(uint32_t m, uint32_t n, uint32_t K) {
 // registers before any is stored (inputs and outputs share X); outputs no
 // input reaches are zeroed.  Lanes run over coefficients first, so a
 // wavefront reads consecutive words of X.
-constexpr uint32_t LEAF_THREADS = 1024;
-constexpr uint32_t ADV_MAX = 16;  // outputs per thread: m * cols * (khi - klo + 1) <= ADV_MAX * LEAF_THREADS
 __device__ void leaf_advance(const LeafLds &L, uint32_t *X, uint32_t cols, uint32_t stride, uint32_t klo,
                              uint32_t khi, uint32_t olo, uint32_t ohi, uint32_t rank, uint32_t m, uint32_t n,
                              uint32_t p, double pd, double inv) {
     const uint32_t tid = threadIdx.x;
     const uint32_t nsl = khi - klo + 1, tot = m * cols * nsl;
     uint32_t res[ADV_MAX];
-#pragma unroll
-    for (uint32_t e = 0; e < ADV_MAX; e++) {
-        const uint32_t idx = tid + e * LEAF_THREADS;
-        res[e] = 0;
-        if (idx < tot) {
-            const uint32_t rc = idx / nsl, kk = klo + (idx - rc * nsl);
-            const uint32_t i = rc / cols, col = rc - i * cols;
-            const uint32_t *xc = X + col * stride + kk;
-            double hacc = 0.0, lacc = (double)xc[L.perm[i] * cols * stride];
-            const uint32_t *ui = L.U + i * n;
-            for (uint32_t q0 = 0; q0 < rank; q0 += 32) {
-                const uint32_t qe = min(rank, q0 + 32);
-                for (uint32_t q = q0; q < qe; q++) {
-                    const uint32_t u = ui[q];
-                    const double xd = (double)xc[L.prow[q] * cols * stride];
-                    hacc = fma((double)(u >> 16), xd, hacc);
-                    lacc = fma((double)(u & 0xFFFFu), xd, lacc);
-                }
-                hacc = fold(hacc, pd, inv);
-                lacc = fold(lacc, pd, inv);
-            }
-            res[e] = (uint32_t)fold(fma(hacc, 65536.0, lacc), pd, inv);
-        }
+    if (tot <= LEAF_THREADS) {
+        leaf_advance_sums<1>(L, X, 0, tot, nsl, cols, stride, klo, rank, n, pd, inv, res);
+    } else if (tot <= 2 * LEAF_THREADS) {
+        leaf_advance_sums<2>(L, X, 0, tot, nsl, cols, stride, klo, rank, n, pd, inv, res);
+    } else if (tot <= 4 * LEAF_THREADS) {
+        leaf_advance_sums<4>(L, X, 0, tot, nsl, cols, stride, klo, rank, n, pd, inv, res);
+    } else {
+        leaf_advance_sums<8>(L, X, 0, tot, nsl, cols, stride, klo, rank, n, pd, inv, res);
+        if (tot > 8 * LEAF_THREADS) leaf_advance_sums<8>(L, X, 8, tot, nsl, cols, stride, klo, rank, n, pd, inv, res);
     }
     __syncthreads();
 #pragma unroll
@@ -152,69 +183,92 @@ __device__ void leaf_advance(const LeafLds &L, uint32_t *X, uint32_t cols, uint3
 // One M-Basis step on the constant term already in L.R (rows in the delta
 // order L.perm): fraction-free LSP, then U' = Linv coefficients, isx, the
 // new delta in L.dln.  Returns the rank.
-__device__ uint32_t leaf_eliminate(const LeafLds &L, uint32_t m, uint32_t n, uint32_t p, double inv) {
-    const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    uint32_t k_end = m;  // the rank after the loop sits in control slot k_end & 1
-    for (uint32_t k = 0; k < m; k++) {
-        const uint32_t rank = L.ctl[2 + (k & 1) * 4];
-        if (rank >= n) {  // every later row is zero: rank <= n
-            k_end = k;
-            break;
-        }
-        // pivot: the first position q >= rank whose column is nonzero in row k,
-        // found by a ballot of wavefront 0 (m <= 128: at most two words)
-        if (tid < 64) {
-            uint32_t pos = m;
-            for (uint32_t q0 = 0; q0 < m && pos == m; q0 += 64) {
-                const uint32_t q = q0 + tid;
-                bool nz = false;
-                if (q >= rank && q < m) {
-                    const uint32_t cq = L.colord[q];
-                    nz = cq < n && L.R[k * n + cq] != 0;
-                }
-                const uint64_t bal = __ballot(nz);
-                if (bal) pos = q0 + (uint32_t)__ffsll((unsigned long long)bal) - 1;
-            }
-            if (tid == 0) {
-                L.ctl[(k & 1) * 4] = pos;
-                L.ctl[2 + ((k + 1) & 1) * 4] = pos < m ? rank + 1 : rank;
-                if (pos < m) {
-                    const uint32_t c = L.colord[pos];
-                    L.colord[pos] = L.colord[rank];
-                    L.colord[rank] = c;
-                    L.ctl[1 + (k & 1) * 4] = c;
-                    L.pivrow[rank] = k;
-                }
-            }
-        }
-        __syncthreads();
-        const uint32_t pos = L.ctl[(k & 1) * 4], c = L.ctl[1 + (k & 1) * 4];
-        if (pos >= m) continue;  // zero row (uniform branch); the next row reads the other control slot
-        // Rows below lose their column-c entry: that entry (the factor f) is
-        // read, never rewritten, and column c is never searched again.
-        // Lanes run over the W columns of a row, wavefronts over rows.
-        const uint32_t pkv = L.R[k * n + c], sk = L.sig[k];
-        const uint32_t W = n + rank + 1;  // R columns, U columns < rank, the new U column
-        for (uint32_t i = k + 1 + tid / 64; i < m; i += nt / 64) {
-            const uint32_t f = L.R[i * n + c];
-            if (f == 0) continue;
-            for (uint32_t col = tid % 64; col < W; col += 64) {
-                if (col < n) {
-                    if (col != c)
-                        L.R[i * n + col] =
-                            submod(mulmod(pkv, L.R[i * n + col], p, inv), mulmod(f, L.R[k * n + col], p, inv), p);
-                } else if (col < n + rank) {
-                    const uint32_t q = col - n;
-                    L.U[i * n + q] = submod(mulmod(pkv, L.U[i * n + q], p, inv), mulmod(f, L.U[k * n + q], p, inv), p);
-                } else {
-                    L.U[i * n + rank] = submod(0u, mulmod(f, sk, p, inv), p);
-                    L.sig[i] = mulmod(pkv, L.sig[i], p, inv);
-                }
-            }
-        }
-        __syncthreads();
+//
+// Wavefront 0 owns the search: the column order (lane q holds the column at
+// position q, q < 128 over two registers) and the rank live in its
+// registers, and it searches row r + 1 as soon as it has applied pivot r to
+// that row, so each row costs one barrier.  Control slot r & 3 holds {pivot
+// position of row r (m: none), its column, rank before row r}.
+struct LeafSearch {
+    uint32_t cord0, cord1, rank;
+};
+
+__device__ __forceinline__ void leaf_search(const LeafLds &L, LeafSearch &S, uint32_t r, uint32_t m, uint32_t n) {
+    const uint32_t lane = threadIdx.x;
+    // the first position q >= rank whose column is nonzero in row r
+    const uint32_t *row = L.R + r * n;
+    const bool nz0 = lane >= S.rank && lane < m && S.cord0 < n && row[min(S.cord0, n - 1)] != 0;
+    const uint64_t b0 = __ballot(nz0);
+    uint32_t pos = m;
+    if (b0) {
+        pos = (uint32_t)__ffsll((unsigned long long)b0) - 1;
+    } else if (m > 64) {
+        const bool nz1 = 64 + lane >= S.rank && 64 + lane < m && S.cord1 < n && row[min(S.cord1, n - 1)] != 0;
+        const uint64_t b1 = __ballot(nz1);
+        if (b1) pos = 64 + (uint32_t)__ffsll((unsigned long long)b1) - 1;
     }
-    const uint32_t rank = L.ctl[2 + (k_end & 1) * 4];
+    uint32_t c = 0;
+    if (pos < m) {  // swap positions pos and rank of the column order
+        c = __builtin_amdgcn_readlane(pos < 64 ? S.cord0 : S.cord1, pos & 63);
+        const uint32_t cr = __builtin_amdgcn_readlane(S.rank < 64 ? S.cord0 : S.cord1, S.rank & 63);
+        if (lane == (pos & 63)) {
+            if (pos < 64) S.cord0 = cr;
+            else S.cord1 = cr;
+        }
+        if (lane == (S.rank & 63)) {
+            if (S.rank < 64) S.cord0 = c;
+            else S.cord1 = c;
+        }
+    }
+    if (lane == 0) {
+        *(uint4 *)(L.ctl + 4 * (r & 3)) = make_uint4(pos, c, S.rank, 0u);
+        if (pos < m) L.pivrow[S.rank] = r;
+    }
+    if (pos < m) S.rank++;
+}
+
+__device__ uint32_t leaf_eliminate(const LeafLds &L, uint32_t m, uint32_t n, uint32_t p, double inv) {
+    const uint32_t tid = threadIdx.x, nt = blockDim.x, wave = tid / 64, lane = tid % 64;
+    LeafSearch S{lane, 64 + lane, 0u};
+    if (wave == 0) leaf_search(L, S, 0, m, n);
+    __syncthreads();
+    uint32_t rank = 0;
+    uint32_t k = 0;
+    for (; k < m; k++) {
+        const uint4 cs = *(const uint4 *)(L.ctl + 4 * (k & 3));
+        const uint32_t pos = cs.x, c = cs.y;
+        rank = cs.z;
+        if (rank >= n) break;  // every later row is zero: rank <= n
+        if (pos < m) {
+            // Row i <- pkv row i - f row k over the R columns, the U columns < rank
+            // and the new U column rank (row i holds 0 there, row k's entry is
+            // its multiplier sk); f = R[i][c] is uniform over the wavefront.
+            // Lanes run over the W columns of a row, wavefronts over rows.
+            const uint32_t pkv = L.R[k * n + c], sk = L.sig[k];
+            const uint32_t W = n + rank + 1;
+            for (uint32_t i = k + 1 + wave; i < m; i += nt / 64) {
+                const uint32_t f = L.R[i * n + c];
+                if (f != 0) {
+                    for (uint32_t col = lane; col < W; col += 64) {
+                        uint32_t *xi = col < n ? L.R + i * n + col : L.U + i * n + (col - n);
+                        const uint32_t *yk = col < n ? L.R + k * n + col : L.U + k * n + (col - n);
+                        const uint32_t y = col == n + rank ? sk : *yk;
+                        *xi = submod(mulmod(pkv, *xi, p, inv), mulmod(f, y, p, inv), p);
+                    }
+                }
+                if (i == k + 1) {  // wavefront 0: row k + 1 is final, search it now
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                    __builtin_amdgcn_wave_barrier();
+                    leaf_search(L, S, k + 1, m, n);
+                }
+                if (f != 0 && lane == 0) L.sig[i] = mulmod(pkv, L.sig[i], p, inv);
+            }
+        } else if (wave == 0 && k + 1 < m) {
+            leaf_search(L, S, k + 1, m, n);  // row k + 1 is unchanged
+        }
+        __syncthreads();
+        if (k + 1 == m) rank += pos < m;  // the rank after the last row
+    }
     // Linv row i = e_i + (U row i) / sigma_i over the earlier pivot rows
     for (uint32_t i = tid; i < m; i += nt) {
         uint32_t sv = L.sig[i], r = 1, e = p - 2;
@@ -251,11 +305,9 @@ __device__ void leaf_sort(const LeafLds &L, uint32_t m) {
         L.perm[r] = i;
     }
     for (uint32_t i = tid; i < m; i += nt) {
-        L.colord[i] = i;
         L.sig[i] = 1;
         L.isx[i] = 0;
     }
-    if (tid == 0) L.ctl[2] = 0;
     __syncthreads();
 }
 
@@ -356,6 +408,126 @@ __global__ __launch_bounds__(256) void k_pmul_school_batch(MulPack pk, uint32_t 
         double v = fma(fold(hacc, pd, inv), 65536.0, fold(lacc, pd, inv));
         D.C[(size_t)(i * D.cols + j) * outlen + (k - D.lo)] = (uint32_t)fold(v, pd, inv);
     }
+}
+
+// Short products with operands staged in LDS: block (x, i, desc) computes row
+// i, columns [x jb, x jb + jb) and all output coefficients.  Only the A
+// coefficients below hi and the B coefficients that meet [lo, hi) are
+// staged.  Each output's sum over the inner index is split over up to
+// 256 / outputs threads (residues mod p, added up through LDS), so a block
+// keeps all its lanes busy however few outputs it has.
+constexpr uint32_t SCH_LDS_WORDS = 12288;
+constexpr uint32_t SCH_JB = 4;
+struct SchoolTile {
+    uint32_t la, t0, lbw, jb;  // staged A length, first staged B coefficient, staged B length, columns per block
+};
+__host__ __device__ inline SchoolTile school_tile(const MulDesc &D) {
+    SchoolTile T;
+    T.la = D.la < D.hi ? D.la : D.hi;
+    T.t0 = D.lo + 1 > T.la ? D.lo + 1 - T.la : 0;
+    const uint32_t t1 = D.hi < D.lb ? D.hi : D.lb;
+    T.lbw = t1 > T.t0 ? t1 - T.t0 : 0;
+    const uint32_t outlen = D.hi - D.lo;
+    const uint32_t fixed = D.inner * T.la + 256 + SCH_JB * outlen;  // A, partial sums
+    const uint32_t per_j = D.inner * (T.lbw > 0 ? T.lbw : 1);
+    uint32_t jb = fixed < SCH_LDS_WORDS ? (SCH_LDS_WORDS - fixed) / per_j : 0;
+    jb = jb < SCH_JB ? jb : SCH_JB;
+    T.jb = T.la <= 64 ? (jb < D.cols ? jb : D.cols) : 0;  // 0: does not fit
+    return T;
+}
+
+__global__ __launch_bounds__(256) void k_pmul_school_lds(MulPack pk, uint32_t p, double pd, double inv) {
+    __shared__ uint32_t sm[SCH_LDS_WORDS];
+    const MulDesc &D = pk.d[blockIdx.z];
+    const uint32_t i = blockIdx.y;
+    if (i >= D.rows) return;
+    const SchoolTile T = school_tile(D);
+    const uint32_t j0 = blockIdx.x * T.jb;
+    if (j0 >= D.cols) return;
+    const uint32_t jn = min(T.jb, D.cols - j0), outlen = D.hi - D.lo, inner = D.inner, la = T.la, lbw = T.lbw;
+    const uint32_t nout = jn * outlen, rs = max(1u, min(inner, 256u / nout));  // inner-index groups
+    uint32_t *As = sm, *Bs = sm + inner * la, *part = Bs + inner * jn * lbw;
+    for (uint32_t idx = threadIdx.x; idx < inner * la; idx += 256) {
+        const uint32_t r = idx / la, sx = idx - r * la;
+        As[idx] = D.A[((size_t)i * inner + r) * D.la + sx];
+    }
+    for (uint32_t idx = threadIdx.x; idx < inner * jn * lbw; idx += 256) {
+        const uint32_t rj = idx / lbw, t = idx - rj * lbw, r = rj / jn, jj = rj - r * jn;
+        Bs[idx] = D.B[((size_t)r * D.cols + j0 + jj) * D.sb + D.ob + T.t0 + t];
+    }
+    __syncthreads();
+    for (uint32_t w = threadIdx.x; w < nout * rs; w += 256) {
+        const uint32_t g = w / nout, o = w - g * nout, jj = o / outlen, k = D.lo + (o - jj * outlen);
+        const uint32_t s_lo = k >= D.lb ? k - D.lb + 1 : 0, s_hi = min(k + 1, la);
+        double hacc = 0.0, lacc = 0.0;
+        uint32_t terms = 0;
+        if (s_lo < s_hi) {
+            const uint32_t *br = Bs + jj * lbw + (k - T.t0);  // B coefficient k - s of (r, jj) at br[r jn lbw - s]
+            for (uint32_t r = g; r < inner; r += rs) {
+                if (terms + la > 64) {
+                    hacc = fold(hacc, pd, inv);
+                    lacc = fold(lacc, pd, inv);
+                    terms = 0;
+                }
+                terms += s_hi - s_lo;
+                const uint32_t *ar = As + r * la, *brr = br + r * jn * lbw;
+                for (uint32_t sx = s_lo; sx < s_hi; sx++) {
+                    const uint32_t a = ar[sx];
+                    const double bd = (double)brr[-(int)sx];
+                    hacc = fma((double)(a >> 16), bd, hacc);
+                    lacc = fma((double)(a & 0xFFFFu), bd, lacc);
+                }
+            }
+        }
+        const uint32_t v = (uint32_t)fold(fma(fold(hacc, pd, inv), 65536.0, fold(lacc, pd, inv)), pd, inv);
+        if (rs == 1) D.C[(size_t)(i * D.cols + j0 + jj) * outlen + (k - D.lo)] = v;
+        else part[w] = v;
+    }
+    if (rs == 1) return;
+    __syncthreads();
+    for (uint32_t o = threadIdx.x; o < nout; o += 256) {
+        uint32_t v = part[o];
+        for (uint32_t g = 1; g < rs; g++) v = addmod(v, part[g * nout + o], p);
+        const uint32_t jj = o / outlen, k = o - jj * outlen;
+        D.C[(size_t)(i * D.cols + j0 + jj) * outlen + k] = v;
+    }
+}
+
+// Launches the short products: LDS-tiled where the operands fit, otherwise
+// the thread-per-output kernel.
+static rr_status launch_school(const std::vector<MulDesc> &ops, uint32_t p, hipStream_t st, uint64_t *launches) {
+    const ModP mp = ModP::make(p);
+    std::vector<MulDesc> tiled, direct;
+    for (const MulDesc &D : ops) (school_tile(D).jb ? tiled : direct).push_back(D);
+    for (size_t g = 0; g < tiled.size(); g += MUL_PACK) {
+        MulPack pk;
+        const uint32_t cnt = (uint32_t)std::min<size_t>(MUL_PACK, tiled.size() - g);
+        uint32_t bx = 1, by = 1;
+        for (uint32_t k = 0; k < cnt; k++) {
+            pk.d[k] = tiled[g + k];
+            const SchoolTile T = school_tile(pk.d[k]);
+            bx = std::max(bx, (pk.d[k].cols + T.jb - 1) / T.jb);
+            by = std::max(by, pk.d[k].rows);
+        }
+        hipLaunchKernelGGL(k_pmul_school_lds, dim3(bx, by, cnt), dim3(256), 0, st, pk, p, mp.pd, mp.inv);
+        RR_HIP(hipGetLastError());
+        if (launches) (*launches)++;
+    }
+    for (size_t g = 0; g < direct.size(); g += MUL_PACK) {
+        MulPack pk;
+        const uint32_t cnt = (uint32_t)std::min<size_t>(MUL_PACK, direct.size() - g);
+        uint64_t maxout = 0;
+        for (uint32_t k = 0; k < cnt; k++) {
+            pk.d[k] = direct[g + k];
+            const MulDesc &D = pk.d[k];
+            maxout = std::max<uint64_t>(maxout, (uint64_t)D.rows * D.cols * (D.hi - D.lo));
+        }
+        const uint32_t bx = (uint32_t)std::min<uint64_t>((maxout + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_pmul_school_batch, dim3(std::max(bx, 1u), cnt), dim3(256), 0, st, pk, p, mp.pd, mp.inv);
+        RR_HIP(hipGetLastError());
+        if (launches) (*launches)++;
+    }
+    return RR_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -490,6 +662,7 @@ rr_status PmScheduler::add(PmJob *J) {
     J->stack.clear();
     J->stack.push_back(Frame{J->G, J->d + 1, J->d, J->M, 0, nullptr, nullptr, nullptr, 0});
     J->done = false;
+    J->has_op = false;
     return RR_OK;
 }
 
@@ -500,17 +673,32 @@ rr_status PmScheduler::round(std::vector<PmJob *> &jobs, bool *launched) {
     std::vector<MulDesc> school;
     std::vector<std::pair<uint32_t, MulDesc>> ntt;
     *launched = false;
+    // Products go first and leaves wait for a round in which no job has a
+    // product pending, so that the leaves of many jobs share one launch (a
+    // job reaches a leaf after at most its recursion depth of products).
+    bool any_mul = false;
     for (auto *J : jobs) {
         if (J->done) continue;
-        Op op;
-        rr_status e;
-        if (!next_op(*J, op, e)) {
-            if (e != RR_OK) return e;
-            continue;
+        if (!J->has_op) {
+            Op op;
+            rr_status e;
+            if (!next_op(*J, op, e)) {
+                if (e != RR_OK) return e;
+                continue;
+            }
+            J->has_op = true;
+            J->op_kind = op.kind;
+            J->op_leaf = op.leaf;
+            J->op_mul = op.mul;
         }
-        if (op.kind == 0) leaves.push_back(op.leaf);
-        else if (use_school(op.mul, p)) school.push_back(op.mul);
-        else ntt.push_back({ntt_logn_for(op.mul), op.mul});
+        any_mul |= J->op_kind == 1;
+    }
+    for (auto *J : jobs) {
+        if (J->done || !J->has_op || J->op_kind != (any_mul ? 1 : 0)) continue;
+        J->has_op = false;
+        if (J->op_kind == 0) leaves.push_back(J->op_leaf);
+        else if (use_school(J->op_mul, p)) school.push_back(J->op_mul);
+        else ntt.push_back({ntt_logn_for(J->op_mul), J->op_mul});
     }
     if (leaves.empty() && school.empty() && ntt.empty()) return RR_OK;
     *launched = true;
@@ -525,19 +713,9 @@ rr_status PmScheduler::round(std::vector<PmJob *> &jobs, bool *launched) {
         RR_HIP(hipGetLastError());
         launches++;
     }
-    for (size_t i = 0; i < school.size(); i += MUL_PACK) {
-        MulPack pk;
-        uint32_t cnt = (uint32_t)std::min<size_t>(MUL_PACK, school.size() - i);
-        uint64_t maxout = 0;
-        for (uint32_t k = 0; k < cnt; k++) {
-            pk.d[k] = school[i + k];
-            const MulDesc &D = pk.d[k];
-            maxout = std::max<uint64_t>(maxout, (uint64_t)D.rows * D.cols * (D.hi - D.lo));
-        }
-        uint32_t bx = (uint32_t)std::min<uint64_t>((maxout + 255) / 256, 4096);
-        hipLaunchKernelGGL(k_pmul_school_batch, dim3(std::max(bx, 1u), cnt), dim3(256), 0, st, pk, p, mp.pd, mp.inv);
-        RR_HIP(hipGetLastError());
-        launches++;
+    if (!school.empty()) {
+        rr_status e = launch_school(school, p, st, &launches);
+        if (e != RR_OK) return e;
     }
     if (!ntt.empty()) return run_ntt_products(ntt);
     return RR_OK;
@@ -859,14 +1037,8 @@ rr_status rr_poly_mat_mul(const uint32_t *A, uint64_t la, const uint32_t *B, uin
     PmScheduler S;
     S.p = p;
     S.st = st;
-    const ModP mp = ModP::make(p);
     if (use_school(D, p)) {
-        MulPack pk;
-        pk.d[0] = D;
-        uint64_t total = (uint64_t)rows * cols * (hi - lo);
-        uint32_t bx = (uint32_t)std::min<uint64_t>((total + 255) / 256, 4096);
-        hipLaunchKernelGGL(k_pmul_school_batch, dim3(std::max(bx, 1u), 1), dim3(256), 0, st, pk, p, mp.pd, mp.inv);
-        RR_HIP(hipGetLastError());
+        if ((s = launch_school(std::vector<MulDesc>{D}, p, st, nullptr)) != RR_OK) return s;
     } else {
         uint32_t logn = ntt_logn_for(D);
         RR_CHECK(logn <= NTT_MAX_LOGN, RR_EINVAL, "product too long");

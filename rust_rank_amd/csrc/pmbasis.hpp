@@ -56,6 +56,11 @@ struct PmJob {
     Pool pool;
     std::vector<Frame> stack;
     bool done = false;
+    // the next device operation, taken from the recursion but not yet launched
+    bool has_op = false;
+    int op_kind = 0;  // 0 leaf, 1 product
+    LeafDesc op_leaf{};
+    MulDesc op_mul{};
 };
 
 struct PmScheduler {
@@ -69,7 +74,7 @@ struct PmScheduler {
     rr_status ensure_scratch(size_t words);
     rr_status init(uint32_t p, hipStream_t st, uint32_t max_logn, uint32_t max_m, uint32_t max_n);
     rr_status add(PmJob *J);                                          // job joins; its delta is zeroed
-    rr_status round(std::vector<PmJob *> &jobs, bool *launched);     // next operation of every job
+    rr_status round(std::vector<PmJob *> &jobs, bool *launched);     // next products, else next leaves
     rr_status run(std::vector<PmJob *> &jobs);                        // init + add + rounds to completion
     rr_status run_ntt_products(std::vector<std::pair<uint32_t, MulDesc>> &ops);
 };
