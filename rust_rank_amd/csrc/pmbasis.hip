@@ -56,17 +56,16 @@ struct LeafPack {
 };
 
 // LDS of a leaf: the residual P [m][n][K] and the basis M [m][m][K+1]
-// stay resident (K > 1), with one coefficient slice X for the in-place
-// update, the elimination arrays R, U and the control words.
+// stay resident (K > 1), beside the elimination arrays R, U, the row
+// multipliers and permutations and the row degrees (delta).
 struct LeafLds {
     uint32_t *R, *U, *sig, *fcol, *perm, *pivrow, *isx;
     uint32_t *prow;  // per pivot q: the row of X it reads, perm(pivrow q)
-    uint32_t *ctl;   // four 16-byte slots of {pivot position, pivot column, rank before the row, -}
     uint32_t *P, *M;
     int64_t *dl, *dln;
 };
 
-__host__ __device__ constexpr uint32_t leaf_ctl_off(uint32_t m, uint32_t n) { return (2 * m * n + 4 * m + 2 * n + 3) & ~3u; }
+__host__ __device__ constexpr uint32_t leaf_dl_off(uint32_t m, uint32_t n) { return (2 * m * n + 4 * m + 2 * n + 1) & ~1u; }
 
 __device__ LeafLds leaf_lds(uint32_t m, uint32_t n, uint32_t K) {
     extern __shared__ __align__(16) uint32_t sm[];
@@ -79,8 +78,7 @@ __device__ LeafLds leaf_lds(uint32_t m, uint32_t n, uint32_t K) {
     L.isx = L.perm + m;
     L.pivrow = L.isx + m;
     L.prow = L.pivrow + n;
-    L.ctl = sm + leaf_ctl_off(m, n);
-    L.dl = (int64_t *)(L.ctl + 16);
+    L.dl = (int64_t *)(sm + leaf_dl_off(m, n));  // 8-byte aligned
     L.dln = L.dl + m;
     L.P = (uint32_t *)(L.dln + m);
     L.M = L.P + (K > 1 ? m * n * K : 0);
@@ -88,7 +86,7 @@ __device__ LeafLds leaf_lds(uint32_t m, uint32_t n, uint32_t K) {
 }
 
 static size_t leaf_lds_words(uint32_t m, uint32_t n, uint32_t K) {
-    size_t w = (size_t)leaf_ctl_off(m, n) + 16 + 4 * m;
+    size_t w = (size_t)leaf_dl_off(m, n) + 4 * m;
     if (K > 1) w += (size_t)m * n * K + (size_t)m * m * (K + 1);
     return w;
 }
@@ -184,22 +182,23 @@ __device__ void leaf_advance(const LeafLds &L, uint32_t *X, uint32_t cols, uint3
 // order L.perm): fraction-free LSP, then U' = Linv coefficients, isx, the
 // new delta in L.dln.  Returns the rank.
 //
-// Wavefront 0 owns the search: the column order (lane q holds the column at
-// position q, q < 128 over two registers) and the rank live in its
-// registers, and it searches row r + 1 as soon as it has applied pivot r to
-// that row, so each row costs one barrier.  Control slot r & 3 holds {pivot
-// position of row r (m: none), its column, rank before row r}.
+// Every wavefront keeps its own copy of the search state in registers (the
+// column order: lane q holds the column at position q, q < 128 over two
+// registers; the rank) and searches row k itself right after the barrier
+// that made row k final, so a row costs one barrier and no control words.
 struct LeafSearch {
     uint32_t cord0, cord1, rank;
 };
 
-__device__ __forceinline__ void leaf_search(const LeafLds &L, LeafSearch &S, uint32_t r, uint32_t m, uint32_t n) {
-    const uint32_t lane = threadIdx.x;
-    // the first position q >= rank whose column is nonzero in row r
+// Pivot of row r: the first position q >= rank whose column is nonzero in
+// row r (m when none); swaps positions pos and rank, returns the column.
+__device__ __forceinline__ uint32_t leaf_search(const LeafLds &L, LeafSearch &S, uint32_t r, uint32_t m, uint32_t n,
+                                                uint32_t &pos) {
+    const uint32_t lane = threadIdx.x % 64;
     const uint32_t *row = L.R + r * n;
     const bool nz0 = lane >= S.rank && lane < m && S.cord0 < n && row[min(S.cord0, n - 1)] != 0;
     const uint64_t b0 = __ballot(nz0);
-    uint32_t pos = m;
+    pos = m;
     if (b0) {
         pos = (uint32_t)__ffsll((unsigned long long)b0) - 1;
     } else if (m > 64) {
@@ -208,7 +207,7 @@ __device__ __forceinline__ void leaf_search(const LeafLds &L, LeafSearch &S, uin
         if (b1) pos = 64 + (uint32_t)__ffsll((unsigned long long)b1) - 1;
     }
     uint32_t c = 0;
-    if (pos < m) {  // swap positions pos and rank of the column order
+    if (pos < m) {
         c = __builtin_amdgcn_readlane(pos < 64 ? S.cord0 : S.cord1, pos & 63);
         const uint32_t cr = __builtin_amdgcn_readlane(S.rank < 64 ? S.cord0 : S.cord1, S.rank & 63);
         if (lane == (pos & 63)) {
@@ -220,55 +219,49 @@ __device__ __forceinline__ void leaf_search(const LeafLds &L, LeafSearch &S, uin
             else S.cord1 = c;
         }
     }
-    if (lane == 0) {
-        *(uint4 *)(L.ctl + 4 * (r & 3)) = make_uint4(pos, c, S.rank, 0u);
-        if (pos < m) L.pivrow[S.rank] = r;
-    }
-    if (pos < m) S.rank++;
+    return c;
 }
 
 __device__ uint32_t leaf_eliminate(const LeafLds &L, uint32_t m, uint32_t n, uint32_t p, double inv) {
-    const uint32_t tid = threadIdx.x, nt = blockDim.x, wave = tid / 64, lane = tid % 64;
+    const uint32_t tid = threadIdx.x, nt = blockDim.x, lane = tid % 64;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid / 64), nw = nt / 64;
     LeafSearch S{lane, 64 + lane, 0u};
-    if (wave == 0) leaf_search(L, S, 0, m, n);
-    __syncthreads();
-    uint32_t rank = 0;
-    uint32_t k = 0;
-    for (; k < m; k++) {
-        const uint4 cs = *(const uint4 *)(L.ctl + 4 * (k & 3));
-        const uint32_t pos = cs.x, c = cs.y;
-        rank = cs.z;
-        if (rank >= n) break;  // every later row is zero: rank <= n
+    for (uint32_t k = 0; k < m && S.rank < n; k++) {  // rank n: every later row is zero
+        uint32_t pos;
+        const uint32_t c = leaf_search(L, S, k, m, n, pos);
         if (pos < m) {
-            // Row i <- pkv row i - f row k over the R columns, the U columns < rank
-            // and the new U column rank (row i holds 0 there, row k's entry is
-            // its multiplier sk); f = R[i][c] is uniform over the wavefront.
-            // Lanes run over the W columns of a row, wavefronts over rows.
-            const uint32_t pkv = L.R[k * n + c], sk = L.sig[k];
-            const uint32_t W = n + rank + 1;
-            for (uint32_t i = k + 1 + wave; i < m; i += nt / 64) {
-                const uint32_t f = L.R[i * n + c];
-                if (f != 0) {
-                    for (uint32_t col = lane; col < W; col += 64) {
-                        uint32_t *xi = col < n ? L.R + i * n + col : L.U + i * n + (col - n);
-                        const uint32_t *yk = col < n ? L.R + k * n + col : L.U + k * n + (col - n);
-                        const uint32_t y = col == n + rank ? sk : *yk;
-                        *xi = submod(mulmod(pkv, *xi, p, inv), mulmod(f, y, p, inv), p);
-                    }
+            const uint32_t rank = S.rank;  // pivot index of row k
+            if (tid == 0) L.pivrow[rank] = k;
+            S.rank++;
+            // Row i <- pkv row i - f row k over the columns [R | U (< rank) |
+            // new U column rank | sigma]: row i holds 0 in the new U column,
+            // row k's entries there are its multiplier sk and 0, so sigma_i
+            // becomes pkv sigma_i.  A row with f = 0 is only scaled, with its
+            // multiplier: its Linv row U / sigma and its zero pattern stay.
+            // Lanes run over the columns, each wavefront over two rows at a
+            // time, all loads issued before the arithmetic.
+            const uint32_t W = n + rank + 2;
+            const uint32_t *Rk = L.R + k * n, *Uk = L.U + k * n;
+            for (uint32_t i0 = k + 1 + wave; i0 < m; i0 += 2 * nw) {
+                const uint32_t i1 = i0 + nw < m ? i0 + nw : i0;
+                const uint32_t pkv = Rk[c], sk = L.sig[k], f0 = L.R[i0 * n + c], f1 = L.R[i1 * n + c];
+                for (uint32_t col = lane; col < W; col += 64) {
+                    const uint32_t cu = col - n;
+                    uint32_t *x0 = col < n ? L.R + i0 * n + col : col <= n + rank ? L.U + i0 * n + cu : L.sig + i0;
+                    uint32_t *x1 = col < n ? L.R + i1 * n + col : col <= n + rank ? L.U + i1 * n + cu : L.sig + i1;
+                    const uint32_t *ya = col < n ? Rk + col : Uk + min(cu, n - 1);
+                    const uint32_t yv = *ya, a0 = *x0, a1 = *x1;
+                    const uint32_t y = col < n + rank ? yv : col == n + rank ? sk : 0u;
+                    const uint32_t r0 = submod(mulmod(pkv, a0, p, inv), mulmod(f0, y, p, inv), p);
+                    const uint32_t r1 = submod(mulmod(pkv, a1, p, inv), mulmod(f1, y, p, inv), p);
+                    *x0 = r0;
+                    if (i1 != i0) *x1 = r1;
                 }
-                if (i == k + 1) {  // wavefront 0: row k + 1 is final, search it now
-                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-                    __builtin_amdgcn_wave_barrier();
-                    leaf_search(L, S, k + 1, m, n);
-                }
-                if (f != 0 && lane == 0) L.sig[i] = mulmod(pkv, L.sig[i], p, inv);
             }
-        } else if (wave == 0 && k + 1 < m) {
-            leaf_search(L, S, k + 1, m, n);  // row k + 1 is unchanged
         }
         __syncthreads();
-        if (k + 1 == m) rank += pos < m;  // the rank after the last row
     }
+    const uint32_t rank = S.rank;
     // Linv row i = e_i + (U row i) / sigma_i over the earlier pivot rows
     for (uint32_t i = tid; i < m; i += nt) {
         uint32_t sv = L.sig[i], r = 1, e = p - 2;
