@@ -23,6 +23,9 @@
 #include <chrono>
 #include <memory>
 #include <cmath>
+#include <thread>
+#include <atomic>
+#include <string>
 
 namespace rr {
 
@@ -52,13 +55,35 @@ __global__ __launch_bounds__(WG) void k_gram_tiles(const GramTile *__restrict__ 
     }
 }
 
-// out[ii] = sum_k part[k * P + ii] mod p, one workgroup per job
-__global__ void k_reduce_jobs(const RedJob *__restrict__ jobs, uint32_t p) {
+// out[ii] = sum_k part[k * P + ii] mod p.  Workgroup (job, y) sums columns
+// [64 y, 64 y + 64) of its job: lanes over the columns, its 16 wavefronts over
+// partials k = w mod 16 with four loads in flight per lane, the sums
+// combined in LDS.
+constexpr uint32_t RJ_WAVES = 16;
+__global__ __launch_bounds__(RJ_WAVES * 64) void k_reduce_jobs(const RedJob *__restrict__ jobs, uint32_t p) {
+    __shared__ uint64_t red[RJ_WAVES][64];
     const RedJob J = jobs[blockIdx.x];
-    for (uint32_t ii = threadIdx.x; ii < J.P; ii += blockDim.x) {
-        uint64_t s = 0;
-        for (uint32_t k = 0; k < J.ntiles; k++) s += J.part[(size_t)k * J.P + ii];
-        J.out[ii] = (uint32_t)(s % p);
+    const uint32_t c0 = blockIdx.y * 64;
+    if (c0 >= J.P) return;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t ii = c0 + lane, ic = min(ii, J.P - 1);
+    const uint32_t *src = J.part + ic;
+    uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // partials < 2^31, fewer than 2^32 of them
+    uint32_t k = w;
+    for (; k + 3 * RJ_WAVES < J.ntiles; k += 4 * RJ_WAVES) {
+        s0 += src[(size_t)k * J.P];
+        s1 += src[(size_t)(k + RJ_WAVES) * J.P];
+        s2 += src[(size_t)(k + 2 * RJ_WAVES) * J.P];
+        s3 += src[(size_t)(k + 3 * RJ_WAVES) * J.P];
+    }
+    for (; k < J.ntiles; k += RJ_WAVES) s0 += src[(size_t)k * J.P];
+    red[w][lane] = s0 + s1 + s2 + s3;
+    __syncthreads();
+    if (w == 0 && ii < J.P) {
+        uint64_t t = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < RJ_WAVES; q++) t += red[q][lane];
+        J.out[ii] = (uint32_t)(t % p);
     }
 }
 
@@ -451,7 +476,9 @@ rr_status TickRunner::tick(std::vector<Request *> &reqs, uint32_t p) {
                     (const uint32_t *)(dbase + wg_off[gi]), (uint32_t)G.jobs.size(), G.wg.back(), pbase, st);
         RR_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_reduce_jobs, dim3((uint32_t)red.size()), dim3(256), 0, st,
+    uint32_t maxP = 0;
+    for (const RedJob &J : red) maxP = std::max(maxP, J.P);
+    hipLaunchKernelGGL(k_reduce_jobs, dim3((uint32_t)red.size(), (maxP + 63) / 64), dim3(RJ_WAVES * 64), 0, st,
                        (const RedJob *)(dbase + red_off), p);
     RR_HIP(hipGetLastError());
     for (auto *R : reqs) R->t++;
@@ -482,13 +509,41 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
     } guard{st};
     std::vector<std::unique_ptr<Request>> reqs(count);
     rr_status s;
-    for (uint32_t i = 0; i < count; i++) {
-        reqs[i].reset(new Request());
-        uint64_t L = seq_len && seq_len[i] ? seq_len[i] : 2 * std::min(mats[i].m, mats[i].n) / b[i] + 8;
-        uint64_t d = order && order[i] ? order[i] : L - 1;
-        if ((s = reqs[i]->init(&mats[i], p, b[i], dr[i], dc[i], V[i], L, d, 2048, st)) != RR_OK) {
-            set_error("matrix %u: %s", i, rr_last_error());
-            return s;
+    // Host set-up (validation, transpose, slicing, uploads) of the matrices
+    // runs on several threads; each request's copies and kernels keep their
+    // order in the one stream.
+    {
+        std::vector<rr_status> st_i(count, RR_OK);
+        std::vector<std::string> msg(count);
+        std::atomic<uint32_t> next{0};
+        auto work = [&]() {
+            if (hipSetDevice(dev) != hipSuccess) return;
+            for (uint32_t i; (i = next++) < count;) {
+                try {
+                    reqs[i].reset(new Request());
+                    uint64_t L = seq_len && seq_len[i] ? seq_len[i]
+                                 : b[i] ? 2 * std::min(mats[i].m, mats[i].n) / b[i] + 8 : 0;
+                    uint64_t d = order && order[i] ? order[i] : L - 1;
+                    st_i[i] = reqs[i]->init(&mats[i], p, b[i], dr[i], dc[i], V[i], L, d, 2048, st);
+                    if (st_i[i] != RR_OK) msg[i] = rr_last_error();
+                } catch (const std::bad_alloc &) {
+                    st_i[i] = RR_ENOMEM;
+                    msg[i] = "host allocation failed";
+                }
+            }
+        };
+        const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
+        const uint32_t nth = std::min<uint32_t>(count, std::min<uint32_t>(hw, 16));
+        std::vector<std::thread> pool;
+        for (uint32_t t = 1; t < nth; t++) pool.emplace_back(work);
+        work();
+        for (auto &th : pool) th.join();
+        for (uint32_t i = 0; i < count; i++) {
+            RR_CHECK(reqs[i], RR_EDEVICE, "set-up thread could not select device %d", dev);
+            if (st_i[i] != RR_OK) {
+                set_error("matrix %u: %s", i, msg[i].c_str());
+                return st_i[i];
+            }
         }
     }
     RR_HIP(hipStreamSynchronize(st));

@@ -31,7 +31,7 @@ uint64_t request_bytes(uint64_t m, uint64_t n, uint64_t nnz, uint32_t b, uint64_
 // Steps (nonzeros per row group) a wavefront of a batched request is given
 // when its operands are sliced: small requests get few wavefronts, large ones
 // up to a full device.
-constexpr uint32_t BATCH_STEPS_PER_WAVE = 96;
+constexpr uint32_t BATCH_STEPS_PER_WAVE = 32;
 
 struct Request {
     uint32_t p = 0, b = 0, B = 0, P = 0, su = 4, nwg = 0;
