@@ -96,14 +96,16 @@ constexpr uint32_t ADV_MAX = 16;  // outputs per thread: m * cols * (khi - klo +
 
 // E outputs per thread (idx = tid + e * LEAF_THREADS), accumulated together so
 // that each pivot row's loads are shared by E independent sums; indices past
-// tot recompute the last output, which is then not stored.
+// tot recompute the last output, which is then not stored.  A coefficient u
+// enters as u_hi 2^16 + u_lo with 64-bit integer multiply-adds; with at most
+// 64 pivots both sums stay below 2^53, so they convert to fp64 exactly.
 template <uint32_t E>
 __device__ __forceinline__ void leaf_advance_sums(const LeafLds &L, const uint32_t *X, uint32_t e0, uint32_t tot,
                                                   uint32_t nsl, uint32_t cols, uint32_t stride, uint32_t klo,
                                                   uint32_t rank, uint32_t n, double pd, double inv, uint32_t *res) {
     const uint32_t tid = threadIdx.x, rstride = cols * stride;
     uint32_t xo[E], uo[E];
-    double hacc[E], lacc[E];
+    uint64_t hacc[E], lacc[E];
 #pragma unroll
     for (uint32_t e = 0; e < E; e++) {
         uint32_t idx = tid + (e0 + e) * LEAF_THREADS;
@@ -112,29 +114,25 @@ __device__ __forceinline__ void leaf_advance_sums(const LeafLds &L, const uint32
         const uint32_t i = rc / cols, col = rc - i * cols;
         xo[e] = col * stride + kk;
         uo[e] = i * n;
-        hacc[e] = 0.0;
-        lacc[e] = (double)X[L.perm[i] * rstride + xo[e]];
+        hacc[e] = 0;
+        lacc[e] = X[L.perm[i] * rstride + xo[e]];
     }
-    for (uint32_t q0 = 0; q0 < rank; q0 += 32) {
-        const uint32_t qe = min(rank, q0 + 32);
-        for (uint32_t q = q0; q < qe; q++) {
-            const uint32_t *xr = X + L.prow[q] * rstride;
-#pragma unroll
-            for (uint32_t e = 0; e < E; e++) {
-                const uint32_t u = L.U[uo[e] + q];
-                const double xd = (double)xr[xo[e]];
-                hacc[e] = fma((double)(u >> 16), xd, hacc[e]);
-                lacc[e] = fma((double)(u & 0xFFFFu), xd, lacc[e]);
-            }
-        }
+    // lane q holds pivot row q's offset in X (rank <= n <= 64)
+    const uint32_t prow_l = (tid & 63) < rank ? L.prow[tid & 63] * rstride : 0u;
+#pragma unroll 2
+    for (uint32_t q = 0; q < rank; q++) {
+        const uint32_t *xr = X + __builtin_amdgcn_readlane(prow_l, q);
 #pragma unroll
         for (uint32_t e = 0; e < E; e++) {
-            hacc[e] = fold(hacc[e], pd, inv);
-            lacc[e] = fold(lacc[e], pd, inv);
+            const uint32_t u = L.U[uo[e] + q];
+            const uint32_t x = xr[xo[e]];
+            hacc[e] += (uint64_t)(u >> 16) * x;
+            lacc[e] += (uint64_t)(u & 0xFFFFu) * x;
         }
     }
 #pragma unroll
-    for (uint32_t e = 0; e < E; e++) res[e0 + e] = (uint32_t)fold(fma(hacc[e], 65536.0, lacc[e]), pd, inv);
+    for (uint32_t e = 0; e < E; e++)
+        res[e0 + e] = (uint32_t)fold(fma(fold((double)hacc[e], pd, inv), 65536.0, fold((double)lacc[e], pd, inv)), pd, inv);
 }
 
 // In-place X <- T X on coefficients: X [m][cols][stride] valid on [klo, khi],
@@ -222,8 +220,23 @@ __device__ __forceinline__ uint32_t leaf_search(const LeafLds &L, LeafSearch &S,
     return c;
 }
 
+// Montgomery reduction with R = 2^32: x < p 2^32 -> x R^-1 mod p; pneg = -p^-1 mod 2^32.
+__device__ __forceinline__ uint32_t redc(uint64_t x, uint32_t p, uint32_t pneg) {
+    const uint32_t mq = (uint32_t)x * pneg;
+    const uint32_t r = (uint32_t)((x + (uint64_t)mq * p) >> 32);  // < 2p
+    return r >= p ? r - p : r;
+}
+
+__device__ __forceinline__ uint32_t neg_inv_2_32(uint32_t p) {  // p odd
+    uint32_t x = p;  // correct to 3 bits; each Newton step doubles them
+#pragma unroll
+    for (int i = 0; i < 4; i++) x *= 2u - p * x;
+    return 0u - x;
+}
+
 __device__ uint32_t leaf_eliminate(const LeafLds &L, uint32_t m, uint32_t n, uint32_t p, double inv) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x, lane = tid % 64;
+    const uint32_t pneg = neg_inv_2_32(p);
     const uint32_t wave = __builtin_amdgcn_readfirstlane(tid / 64), nw = nt / 64;
     LeafSearch S{lane, 64 + lane, 0u};
     for (uint32_t k = 0; k < m && S.rank < n; k++) {  // rank n: every later row is zero
@@ -233,18 +246,20 @@ __device__ uint32_t leaf_eliminate(const LeafLds &L, uint32_t m, uint32_t n, uin
             const uint32_t rank = S.rank;  // pivot index of row k
             if (tid == 0) L.pivrow[rank] = k;
             S.rank++;
-            // Row i <- pkv row i - f row k over the columns [R | U (< rank) |
-            // new U column rank | sigma]: row i holds 0 in the new U column,
-            // row k's entries there are its multiplier sk and 0, so sigma_i
-            // becomes pkv sigma_i.  A row with f = 0 is only scaled, with its
-            // multiplier: its Linv row U / sigma and its zero pattern stay.
+            // Row i <- (pkv row i - f row k) R^-1 over the columns [R | U (<
+            // rank) | new U column rank | sigma]: row i holds 0 in the new U
+            // column, row k's entries there are its multiplier sk and 0, so
+            // sigma_i becomes pkv sigma_i R^-1.  Scaling a whole row with its
+            // multiplier (the Montgomery factor R^-1 = 2^-32, and pkv when
+            // f = 0) leaves its Linv row U / sigma and its zero pattern as
+            // they are.  One reduction per entry: pkv x + (p - f) y < 2 p^2.
             // Lanes run over the columns, each wavefront over two rows at a
             // time, all loads issued before the arithmetic.
             const uint32_t W = n + rank + 2;
             const uint32_t *Rk = L.R + k * n, *Uk = L.U + k * n;
             for (uint32_t i0 = k + 1 + wave; i0 < m; i0 += 2 * nw) {
                 const uint32_t i1 = i0 + nw < m ? i0 + nw : i0;
-                const uint32_t pkv = Rk[c], sk = L.sig[k], f0 = L.R[i0 * n + c], f1 = L.R[i1 * n + c];
+                const uint32_t pkv = Rk[c], sk = L.sig[k], nf0 = p - L.R[i0 * n + c], nf1 = p - L.R[i1 * n + c];
                 for (uint32_t col = lane; col < W; col += 64) {
                     const uint32_t cu = col - n;
                     uint32_t *x0 = col < n ? L.R + i0 * n + col : col <= n + rank ? L.U + i0 * n + cu : L.sig + i0;
@@ -252,8 +267,8 @@ __device__ uint32_t leaf_eliminate(const LeafLds &L, uint32_t m, uint32_t n, uin
                     const uint32_t *ya = col < n ? Rk + col : Uk + min(cu, n - 1);
                     const uint32_t yv = *ya, a0 = *x0, a1 = *x1;
                     const uint32_t y = col < n + rank ? yv : col == n + rank ? sk : 0u;
-                    const uint32_t r0 = submod(mulmod(pkv, a0, p, inv), mulmod(f0, y, p, inv), p);
-                    const uint32_t r1 = submod(mulmod(pkv, a1, p, inv), mulmod(f1, y, p, inv), p);
+                    const uint32_t r0 = redc((uint64_t)pkv * a0 + (uint64_t)nf0 * y, p, pneg);
+                    const uint32_t r1 = redc((uint64_t)pkv * a1 + (uint64_t)nf1 * y, p, pneg);
                     *x0 = r0;
                     if (i1 != i0) *x1 = r1;
                 }
