@@ -173,10 +173,14 @@ def main():
     log(f"workload {A.m}x{A.n} nnz={A.nnz} b={b} L={L}; generated in {time.perf_counter() - t0:.1f}s")
 
     K = engine.Krylov(A, p, b, dr, dc, V)
-    K.run(max(args.warmup, 1))
+    seq = [K.run(max(args.warmup, 1))]
     t1 = time.perf_counter()
-    K.run(steps)  # timed: no per-pass events
+    seq.append(K.run(steps))  # timed: no per-pass events
     t2 = time.perf_counter()
+    have = sum(len(x) for x in seq)
+    if have < L:  # the rest of the sequence for the PM-basis, untimed
+        seq.append(K.run(L - have))
+    seq = np.concatenate(seq)[:L]
     # kernel breakdown on a further stretch of the same sequence, with HIP
     # events around every launch on the engine's stream
     K.set_profiling(True)
@@ -204,6 +208,16 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": f"k_spmm<{1 << (b - 1).bit_length()}, packed>",
                 "bytes_per_launch": int(bytes_pass), "avg_launch_us": round(pass_ms * 1e3, 3)}
+    # the rest of the rank pipeline on this sequence: device PM-basis of order
+    # L - 1, then the rank formula
+    t3 = time.perf_counter()
+    basis, delta = engine.pm_basis(seq, b, L - 1, p)
+    t4 = time.perf_counter()
+    rank = engine.rank_from_basis(basis, delta, b, p)
+    del basis
+    pm = {"b": b, "order": L - 1, "ms": round((t4 - t3) * 1e3, 1), "rank": int(rank),
+          "rank_pipeline_s": round(L / iters_s + (t4 - t3), 3)}
+    log(f"PM-basis of order {L - 1}: {pm['ms']} ms; rank {rank}")
     cpu = None if args.no_cpu else cpu_baseline(A, p, b, dr, dc, V)
     batched = served = None
     if args.batched:
@@ -215,7 +229,7 @@ def main():
         "vs_baseline": None, "dtype": "u32", "data": "synthetic (seeded SplitMix64)",
         "config": {"workload": f"graph_complex {A.m}x{A.n} 12 nnz/row Zipf(1.1) +-1/+-2 p=2013265921 b={b}",
                    "seq_len": L, "nnz": A.nnz, "hbm_gbs_algorithmic": round(b_iter * iters_s / 1e9, 1)},
-        "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "served": served,
+        "roofline": roofline, "cpu_baseline": cpu, "pm_basis": pm, "batched": batched, "served": served,
     }
     print(json.dumps(out), flush=True)
 
