@@ -47,7 +47,7 @@ __global__ __launch_bounds__(WG) void k_gram_tiles(const GramTile *__restrict__ 
         g.init();
         for (uint32_t base = T.row_beg + wave * G; base < T.row_end; base += (WG / 64) * G) {
             const uint32_t r = base + slot;
-            const uint32_t o = r < T.row_end ? T.in[(size_t)r * B + j] : 0u;
+            const uint32_t o = r < T.row_end ? as_global(T.in)[(size_t)r * B + j] : 0u;
             g.add(o, o, lane, wl[wave], pd, inv);
         }
         g.finish(T.part, T.b, p, pd, inv, lds);
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(RJ_WAVES * 64) void k_reduce_jobs(const RedJob *__r
     if (c0 >= J.P) return;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t ii = c0 + lane, ic = min(ii, J.P - 1);
-    const uint32_t *src = J.part + ic;
+    const GLOBAL_AS uint32_t *src = as_global(J.part) + ic;
     uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // partials < 2^31, fewer than 2^32 of them
     uint32_t k = w;
     for (; k + 3 * RJ_WAVES < J.ntiles; k += 4 * RJ_WAVES) {
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(RJ_WAVES * 64) void k_reduce_jobs(const RedJob *__r
         uint64_t t = 0;
 #pragma unroll
         for (uint32_t q = 0; q < RJ_WAVES; q++) t += red[q][lane];
-        J.out[ii] = (uint32_t)(t % p);
+        as_global(J.out)[ii] = (uint32_t)(t % p);
     }
 }
 

@@ -12,6 +12,21 @@ namespace rr {
 void set_error(const char *fmt, ...);
 void clear_error();
 
+// Loads and stores through a pointer marked global: the compiler emits
+// global_* (in-order VMEM) instead of flat_* instructions, whose completion
+// it cannot order against other memory operations and so waits out in full.
+#define GLOBAL_AS __attribute__((address_space(1)))
+typedef uint32_t gv2u __attribute__((ext_vector_type(2)));
+typedef uint32_t gv4u __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ const GLOBAL_AS T *as_global(const T *p) {
+    return (const GLOBAL_AS T *)p;
+}
+template <typename T>
+__device__ __forceinline__ GLOBAL_AS T *as_global(T *p) {
+    return (GLOBAL_AS T *)p;
+}
+
 // ---------------------------------------------------------------------------
 // Exact arithmetic mod p < 2^31.  Integer-valued doubles are exact below 2^53;
 // FP64 FMA is full rate on CDNA4, 32-bit integer multiplies are not.

@@ -52,16 +52,17 @@ struct Lay {
 
 template <int VW>
 __device__ __forceinline__ void vload(const uint32_t *src, uint32_t (&x)[VW]) {
+    const GLOBAL_AS uint32_t *g = as_global(src);
     if constexpr (VW == 4) {
-        const uint4 t = *(const uint4 *)src;
-        x[0] = t.x, x[1] = t.y, x[2] = t.z, x[3] = t.w;
+        const gv4u t = *(const GLOBAL_AS gv4u *)g;
+        x[0] = t[0], x[1] = t[1], x[2] = t[2], x[3] = t[3];
     } else if constexpr (VW == 3) {
-        x[0] = src[0], x[1] = src[1], x[2] = src[2];
+        x[0] = g[0], x[1] = g[1], x[2] = g[2];
     } else if constexpr (VW == 2) {
-        const uint2 t = *(const uint2 *)src;
-        x[0] = t.x, x[1] = t.y;
+        const gv2u t = *(const GLOBAL_AS gv2u *)g;
+        x[0] = t[0], x[1] = t[1];
     } else {
-        x[0] = *src;
+        x[0] = *g;
     }
 }
 
@@ -98,9 +99,10 @@ __device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t rs, uint32_t off, u
 
 template <int VW>
 __device__ __forceinline__ void vstore(uint32_t *dst, const uint32_t (&x)[VW]) {
-    if constexpr (VW == 4) *(uint4 *)dst = make_uint4(x[0], x[1], x[2], x[3]);
-    else if constexpr (VW == 2) *(uint2 *)dst = make_uint2(x[0], x[1]);
-    else *dst = x[0];
+    GLOBAL_AS uint32_t *g = as_global(dst);
+    if constexpr (VW == 4) *(GLOBAL_AS gv4u *)g = gv4u{x[0], x[1], x[2], x[3]};
+    else if constexpr (VW == 2) *(GLOBAL_AS gv2u *)g = gv2u{x[0], x[1]};
+    else *g = x[0];
 }
 
 // One sparse pass: every slice of this wavefront's stream, then the
@@ -129,10 +131,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane / LPR, q = lane % LPR, ci = lane % B;
     const uint32_t gw = __builtin_amdgcn_readfirstlane(lwg * (WG / 64) + wave);
-    uint32_t sl = a.wave_sl[gw];
-    const uint32_t s_end = a.wave_sl[gw + 1];
-    const size_t wblk = a.wave_blk[gw];
-    const uint32_t nblk = a.wave_blk[gw + 1] - a.wave_blk[gw] - 2;  // without the two zero blocks
+    uint32_t sl = as_global(a.wave_sl)[gw];
+    const uint32_t s_end = as_global(a.wave_sl)[gw + 1];
+    const size_t wblk = as_global(a.wave_blk)[gw];
+    const uint32_t nblk = as_global(a.wave_blk)[gw + 1] - (uint32_t)wblk - 2;  // without the two zero blocks
     const uint32_t *cb = a.col + wblk * BW + g * SU;
     const uint32_t *vb = PACKED ? nullptr : a.val + wblk * BW + g * SU;
     uint32_t *o2 = o2s[wave], *sv = ss[wave];
@@ -156,8 +158,12 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
 
     // slice descriptors are loaded one slice ahead, so their loads are older
     // than the gathers in flight when they are used
-    auto meta_at = [&](uint32_t i) { return i < s_end ? a.meta[i] : 0u; };
-    auto unit_at = [&](uint32_t i) { return i < s_end ? a.unit[(size_t)i * RPW + g] : make_uint2(NO_ROW, 0u); };
+    auto meta_at = [&](uint32_t i) { return i < s_end ? as_global(a.meta)[i] : 0u; };
+    auto unit_at = [&](uint32_t i) {
+        if (i >= s_end) return make_uint2(NO_ROW, 0u);
+        const gv2u u = *(const GLOBAL_AS gv2u *)(as_global(a.unit) + (size_t)i * RPW + g);
+        return make_uint2(u[0], u[1]);
+    };
     uint32_t meta = __builtin_amdgcn_readfirstlane(meta_at(sl)), meta_next = meta_at(sl + 1);
     uint2 un = unit_at(sl), un_next = unit_at(sl + 1);
 
@@ -193,7 +199,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
             order_after_vmem();
             const uint32_t l = __builtin_amdgcn_readfirstlane(un.y);
             uint32_t last = 0;
-            if (lane == 0) last = atomicAdd(a.lcnt + l, 1u) + 1 == a.linfo[l].w;
+            if (lane == 0) last = atomicAdd(a.lcnt + l, 1u) + 1 == as_global(a.linfo)[l].w;
             last = __builtin_amdgcn_readfirstlane(last);
             if (last) {
                 if (lane == 0) pend[wave][npend] = l;
@@ -314,7 +320,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     for (uint32_t k = 0; k < npend; k++) {
         const uint32_t l = pend[wave][k];
         order_after_vmem();
-        const uint4 li = a.linfo[l];
+        const gv4u lv = *(const GLOBAL_AS gv4u *)(as_global(a.linfo) + l);
+        const uint4 li = make_uint4(lv[0], lv[1], lv[2], lv[3]);
         constexpr int H = 64 / B;
         const int hh = lane / B;
         unsigned long long t = 0;  // < 2^63
@@ -328,10 +335,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
         wave_sync();
         uint32_t oj = 0;
         for (int h2 = 0; h2 < H; h2++) oj = addmod(oj, sv[h2 * B + ci], a.p);
-        const uint32_t sj = mulmod(oj, a.scale[li.x], a.p, a.inv);
+        const uint32_t sj = mulmod(oj, as_global(a.scale)[li.x], a.p, a.inv);
         wave_sync();
         if (hh == 0) {
-            a.out[(size_t)li.x * B + ci] = sj;
+            as_global(a.out)[(size_t)li.x * B + ci] = sj;
             o2[ci] = oj;
             o2[B + ci] = oj;
             sv[ci] = sj;
