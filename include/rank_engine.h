@@ -125,7 +125,7 @@ rr_status rr_rank_from_basis(const uint32_t *basis, const int64_t *delta, uint32
 /*
  * Replaces ntt / matrix_ntt_parallel (src/ntt.rs:164-211, :263-269) for
  * p = 2013265921: `count` transforms of 2^logn points in place, natural order,
- * forward root 31^((p-1)/n), inverse scaled by 1/n.  logn <= 14.
+ * forward root 31^((p-1)/n), inverse scaled by 1/n.  logn <= 20 (above 14: four steps through device memory).
  */
 rr_status rr_ntt(uint32_t *data, uint32_t logn, uint32_t count, int inverse);
 

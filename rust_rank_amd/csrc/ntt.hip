@@ -39,7 +39,7 @@ __host__ __device__ constexpr uint32_t ntt_per_wg(uint32_t N) { return N >= NTT_
 // butterflies past `total` repeat the thread's first one and are not stored.
 template <bool DIF>
 __device__ __forceinline__ void lds_stage(uint32_t *s, uint32_t total, uint32_t logn, uint32_t lh, const uint32_t *tw,
-                                          const uint32_t *twq, uint32_t tstride, uint32_t p) {
+                                          const uint32_t *twq, uint32_t tstride, uint32_t p, uint32_t ps) {
     const uint32_t h = 1u << lh, N = 1u << logn, half = N >> 1;
     for (uint32_t t0 = threadIdx.x; t0 < total; t0 += NTT_THREADS * NTT_U) {
         uint32_t i0[NTT_U], x0[NTT_U], x1[NTT_U], w[NTT_U], wq[NTT_U];
@@ -49,7 +49,7 @@ __device__ __forceinline__ void lds_stage(uint32_t *s, uint32_t total, uint32_t 
             t = t < total ? t : t0;
             const uint32_t poly = t >> (logn - 1), bfl = t & (half - 1);
             const uint32_t j = bfl & (h - 1), blk = bfl >> lh;
-            i0[u] = poly * N + blk * 2 * h + j;
+            i0[u] = poly * ps + blk * 2 * h + j;
             x0[u] = s[i0[u]];
             x1[u] = s[i0[u] + h];
             w[u] = tw[j * tstride];
@@ -75,19 +75,21 @@ __device__ __forceinline__ void lds_stage(uint32_t *s, uint32_t total, uint32_t 
     __syncthreads();
 }
 
-// In-LDS DIF over `npoly` polynomials of length N = 2^logn laid out back to back.
+// In-LDS DIF over `npoly` polynomials of length N = 2^logn, ps words apart
+// (N, or N + 1 where a transposed fill must avoid bank conflicts).
 __device__ void lds_dif(uint32_t *s, uint32_t npoly, uint32_t logn, const uint32_t *tw, const uint32_t *twq,
-                        uint32_t tw_logn, uint32_t p) {
+                        uint32_t tw_logn, uint32_t p, uint32_t ps) {
     const uint32_t total = npoly << (logn - 1);
     for (uint32_t lh = logn; lh-- > 0;)  // butterfly span h = 2^lh; w_{2h}^j = w_Ntab^(j * Ntab / 2h)
-        lds_stage<true>(s, total, logn, lh, tw, twq, 1u << (tw_logn - lh - 1), p);
+        lds_stage<true>(s, total, logn, lh, tw, twq, 1u << (tw_logn - lh - 1), p, ps);
 }
 
 // In-LDS DIT (inverse twiddles in tw), bit-reversed in, natural out, unscaled.
 __device__ void lds_dit(uint32_t *s, uint32_t npoly, uint32_t logn, const uint32_t *tw, const uint32_t *twq,
-                        uint32_t tw_logn, uint32_t p) {
+                        uint32_t tw_logn, uint32_t p, uint32_t ps) {
     const uint32_t total = npoly << (logn - 1);
-    for (uint32_t lh = 0; lh < logn; lh++) lds_stage<false>(s, total, logn, lh, tw, twq, 1u << (tw_logn - lh - 1), p);
+    for (uint32_t lh = 0; lh < logn; lh++)
+        lds_stage<false>(s, total, logn, lh, tw, twq, 1u << (tw_logn - lh - 1), p, ps);
 }
 
 constexpr uint32_t NTT_LD = 8;  // global loads per thread in flight when staging
@@ -121,7 +123,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_fwd(NttFwdPack pk, uint32_t
         }
     }
     __syncthreads();
-    lds_dif(s, np, logn, tw, twq, tw_logn, p);
+    lds_dif(s, np, logn, tw, twq, tw_logn, p, N);
     uint32_t *dst = D.dst + (size_t)q0 * N;
     for (uint32_t i0 = threadIdx.x; i0 < words; i0 += NTT_THREADS * NTT_LD) {
         uint32_t v[NTT_LD];
@@ -153,7 +155,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_inv(NttInvPack pk, uint32_t
             if (i0 + e * NTT_THREADS < words) s[i0 + e * NTT_THREADS] = v[e];
     }
     __syncthreads();
-    lds_dit(s, np, logn, tw, twq, tw_logn, p);
+    lds_dit(s, np, logn, tw, twq, tw_logn, p, N);
     const uint32_t outs = np * D.outlen;
     for (uint32_t i0 = threadIdx.x; i0 < outs; i0 += NTT_THREADS * NTT_LD) {
         uint32_t v[NTT_LD];
@@ -167,6 +169,92 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_inv(NttInvPack pk, uint32_t
             const uint32_t i = i0 + e * NTT_THREADS, q = i / D.outlen, k = i - q * D.outlen;
             if (i < outs) D.dst[(size_t)(q0 + q) * D.dstride + k] = v[e];
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Transforms above 2^14 points (up to 2^20): four steps over N = N1 N2, with
+// element n = n1 + N1 n2, N2 = 2^ceil(logn/2) and N1 = 2^floor(logn/2).
+// Forward: N2-point DIF along n2 for each n1 (column kernel, a tile of TC
+// columns transposed into LDS), the twiddle w_N^(n1 k2), then N1-point DIF
+// along n1 for each k2 (rows, contiguous, the ordinary kernel in place).
+// Spectrum entry N1 j + j1 holds X[brev(j) + N2 brev(j1)]; the inverse takes
+// the same order back (rows, then the conjugate twiddle, then columns), so
+// a product never permutes.
+__host__ __device__ inline uint32_t big_l2(uint32_t logn) { return (logn + 1) / 2; }
+__host__ __device__ inline uint32_t big_ltc(uint32_t logn) {  // log2 of the columns per workgroup
+    const uint32_t l1 = logn - big_l2(logn), byl = 13 - big_l2(logn);  // (N2 + 1) TC words, about 32 KB
+    return l1 < byl ? l1 : byl;
+}
+__host__ __device__ inline uint32_t big_tc(uint32_t logn) { return 1u << big_ltc(logn); }
+
+// w_N^e * x for e < N, from the table w_T^k, k < T/2 (T = 2^tw_logn >= N);
+// w_T^(k + T/2) = -w_T^k.
+__device__ __forceinline__ uint32_t tw_mul(uint32_t x, uint32_t e, uint32_t logn, const uint32_t *tw,
+                                           uint32_t tw_logn, uint32_t p, double inv) {
+    const uint32_t k = e << (tw_logn - logn), half = 1u << (tw_logn - 1);
+    const uint32_t w = tw[k & (half - 1)];
+    const uint32_t r = mulmod(x, w, p, inv);
+    return (k & half) && r ? p - r : r;
+}
+
+__global__ __launch_bounds__(NTT_THREADS) void k_ntt_cols_fwd(NttFwdPack pk, uint32_t logn, const uint32_t *tw,
+                                                              const uint32_t *twq, uint32_t tw_logn, uint32_t p,
+                                                              double inv) {
+    extern __shared__ uint32_t s[];
+    const NttFwdDesc &D = pk.d[blockIdx.y];
+    const uint32_t l2 = big_l2(logn), l1 = logn - l2, N1 = 1u << l1, N2 = 1u << l2;
+    const uint32_t ltc = big_ltc(logn), TC = 1u << ltc;
+    const uint32_t tiles = N1 / TC, q = blockIdx.x / tiles, c0 = (blockIdx.x - q * tiles) * TC;
+    if (q >= D.count) return;
+    const uint32_t ps = N2 + 1, words = TC * N2;
+    const uint32_t *src = D.src + (size_t)q * D.sstride + D.soff;
+    for (uint32_t i0 = threadIdx.x; i0 < words; i0 += NTT_THREADS * NTT_LD) {
+        uint32_t v[NTT_LD];
+#pragma unroll
+        for (uint32_t e = 0; e < NTT_LD; e++) {
+            const uint32_t i = i0 + e * NTT_THREADS, c = i & (TC - 1), n = c0 + c + N1 * (i >> ltc);
+            const bool ok = i < words && n < D.len;
+            v[e] = D.len ? src[ok ? n : 0] : 0u;
+        }
+#pragma unroll
+        for (uint32_t e = 0; e < NTT_LD; e++) {
+            const uint32_t i = i0 + e * NTT_THREADS, c = i & (TC - 1), n2 = i >> ltc;
+            if (i < words) s[c * ps + n2] = (c0 + c + N1 * n2) < D.len ? v[e] : 0u;
+        }
+    }
+    __syncthreads();
+    lds_dif(s, TC, l2, tw, twq, tw_logn, p, ps);
+    uint32_t *dst = D.dst + ((size_t)q << logn);
+    for (uint32_t i = threadIdx.x; i < words; i += NTT_THREADS) {
+        const uint32_t c = i & (TC - 1), j = i >> ltc, n1 = c0 + c;
+        const uint32_t k2 = __brev(j) >> (32 - l2);
+        dst[n1 + N1 * j] = tw_mul(s[c * ps + j], n1 * k2, logn, tw, tw_logn, p, inv);
+    }
+}
+
+__global__ __launch_bounds__(NTT_THREADS) void k_ntt_cols_inv(NttInvPack pk, uint32_t logn, const uint32_t *tw,
+                                                              const uint32_t *twq, uint32_t tw_logn, uint32_t p,
+                                                              double inv) {
+    extern __shared__ uint32_t s[];
+    const NttInvDesc &D = pk.d[blockIdx.y];
+    const uint32_t l2 = big_l2(logn), l1 = logn - l2, N1 = 1u << l1, N2 = 1u << l2;
+    const uint32_t ltc = big_ltc(logn), TC = 1u << ltc;
+    const uint32_t tiles = N1 / TC, q = blockIdx.x / tiles, c0 = (blockIdx.x - q * tiles) * TC;
+    if (q >= D.count) return;
+    const uint32_t ps = N2 + 1, words = TC * N2;
+    const uint32_t *src = D.src + ((size_t)q << logn);
+    for (uint32_t i = threadIdx.x; i < words; i += NTT_THREADS) {
+        const uint32_t c = i & (TC - 1), j = i >> ltc, n1 = c0 + c;
+        const uint32_t k2 = __brev(j) >> (32 - l2);
+        s[c * ps + j] = tw_mul(src[n1 + N1 * j], n1 * k2, logn, tw, tw_logn, p, inv);  // tw holds w^-k here
+    }
+    __syncthreads();
+    lds_dit(s, TC, l2, tw, twq, tw_logn, p, ps);
+    uint32_t *dst = D.dst + (size_t)q * D.dstride;
+    for (uint32_t i = threadIdx.x; i < words; i += NTT_THREADS) {
+        const uint32_t c = i & (TC - 1), n2 = i >> ltc, n = c0 + c + N1 * n2;
+        if (n >= D.lo && n - D.lo < D.outlen) dst[n - D.lo] = s[c * ps + n2];
     }
 }
 
@@ -266,10 +354,50 @@ static inline uint32_t blocks_for(uint32_t count, uint32_t logn) {
 
 static inline size_t lds_for(uint32_t logn) { return (size_t)std::max(1u << logn, NTT_WG_WORDS) * 4; }
 
+static rr_status ntt_forward_big(const NttTables &T, const NttFwdPack &pk, uint32_t ndesc, uint32_t logn,
+                                 hipStream_t st) {
+    const uint32_t l2 = big_l2(logn), l1 = logn - l2, N1 = 1u << l1, N2 = 1u << l2, TC = big_tc(logn);
+    uint32_t qmax = 0;
+    NttFwdPack rows;
+    for (uint32_t i = 0; i < ndesc; i++) {
+        qmax = std::max(qmax, pk.d[i].count);
+        RR_CHECK((uint64_t)pk.d[i].count * N2 < (1ull << 32), RR_EINVAL, "too many transforms");
+        rows.d[i] = NttFwdDesc{pk.d[i].dst, N1, 0, N1, pk.d[i].dst, pk.d[i].count * N2};
+    }
+    const ModP mp = ModP::make(NTT_PRIME);
+    const size_t lds = (size_t)(N2 + 1) * TC * 4;
+    hipLaunchKernelGGL(k_ntt_cols_fwd, dim3(qmax * (N1 / TC), ndesc), dim3(NTT_THREADS), lds, st, pk, logn,
+                       T.fwd.as<uint32_t>(), T.fwdq.as<uint32_t>(), T.logn, NTT_PRIME, mp.inv);
+    RR_HIP(hipGetLastError());
+    return ntt_forward_pack(T, rows, ndesc, l1, st);
+}
+
+static rr_status ntt_inverse_big(const NttTables &T, const NttInvPack &pk, uint32_t ndesc, uint32_t logn,
+                                 hipStream_t st) {
+    const uint32_t l2 = big_l2(logn), l1 = logn - l2, N1 = 1u << l1, N2 = 1u << l2, TC = big_tc(logn);
+    uint32_t qmax = 0;
+    NttInvPack rows;
+    for (uint32_t i = 0; i < ndesc; i++) {
+        qmax = std::max(qmax, pk.d[i].count);
+        RR_CHECK((uint64_t)pk.d[i].count * N2 < (1ull << 32), RR_EINVAL, "too many transforms");
+        // rows in place: the spectra are the round's own scratch
+        rows.d[i] = NttInvDesc{pk.d[i].src, pk.d[i].count * N2, 0, N1, const_cast<uint32_t *>(pk.d[i].src), N1};
+    }
+    rr_status s;
+    if ((s = ntt_inverse_pack(T, rows, ndesc, l1, st)) != RR_OK) return s;
+    const ModP mp = ModP::make(NTT_PRIME);
+    const size_t lds = (size_t)(N2 + 1) * TC * 4;
+    hipLaunchKernelGGL(k_ntt_cols_inv, dim3(qmax * (N1 / TC), ndesc), dim3(NTT_THREADS), lds, st, pk, logn,
+                       T.inv.as<uint32_t>(), T.invq.as<uint32_t>(), T.logn, NTT_PRIME, mp.inv);
+    RR_HIP(hipGetLastError());
+    return RR_OK;
+}
+
 rr_status ntt_forward_pack(const NttTables &T, const NttFwdPack &pk, uint32_t ndesc, uint32_t logn,
                            hipStream_t st) {
-    RR_CHECK(logn >= 1 && logn <= T.logn && logn <= 14, RR_EINVAL, "NTT size 2^%u unsupported", logn);
+    RR_CHECK(logn >= 1 && logn <= T.logn && logn <= NTT_MAX_LOGN, RR_EINVAL, "NTT size 2^%u unsupported", logn);
     RR_CHECK(ndesc >= 1 && ndesc <= NTT_PACK, RR_EINVAL, "bad pack size");
+    if (logn > NTT_LDS_LOGN) return ntt_forward_big(T, pk, ndesc, logn, st);
     uint32_t bx = 0;
     for (uint32_t i = 0; i < ndesc; i++) bx = std::max(bx, blocks_for(pk.d[i].count, logn));
     hipLaunchKernelGGL(k_ntt_fwd, dim3(std::max(bx, 1u), ndesc), dim3(NTT_THREADS), lds_for(logn), st, pk, logn,
@@ -280,8 +408,9 @@ rr_status ntt_forward_pack(const NttTables &T, const NttFwdPack &pk, uint32_t nd
 
 rr_status ntt_inverse_pack(const NttTables &T, const NttInvPack &pk, uint32_t ndesc, uint32_t logn,
                            hipStream_t st) {
-    RR_CHECK(logn >= 1 && logn <= T.logn && logn <= 14, RR_EINVAL, "NTT size 2^%u unsupported", logn);
+    RR_CHECK(logn >= 1 && logn <= T.logn && logn <= NTT_MAX_LOGN, RR_EINVAL, "NTT size 2^%u unsupported", logn);
     RR_CHECK(ndesc >= 1 && ndesc <= NTT_PACK, RR_EINVAL, "bad pack size");
+    if (logn > NTT_LDS_LOGN) return ntt_inverse_big(T, pk, ndesc, logn, st);
     uint32_t bx = 0;
     for (uint32_t i = 0; i < ndesc; i++) bx = std::max(bx, blocks_for(pk.d[i].count, logn));
     hipLaunchKernelGGL(k_ntt_inv, dim3(std::max(bx, 1u), ndesc), dim3(NTT_THREADS), lds_for(logn), st, pk, logn,

@@ -7,7 +7,8 @@ namespace rr {
 
 constexpr uint32_t NTT_PRIME = 2013265921u;  // 15 * 2^27 + 1 (src/ntt.rs:52)
 constexpr uint32_t NTT_ROOT = 31u;           // generator (src/ntt.rs:53)
-constexpr uint32_t NTT_MAX_LOGN = 14;        // one workgroup, one polynomial in 64 KB of LDS
+constexpr uint32_t NTT_LDS_LOGN = 14;        // one workgroup, one polynomial in 64 KB of LDS
+constexpr uint32_t NTT_MAX_LOGN = 20;        // larger transforms: four steps through global memory
 
 // Twiddle tables for transforms up to 2^logn points: w^k and w^-k for k < 2^(logn-1)
 // with their Shoup companions.

@@ -20,7 +20,8 @@ def _eng():
     return engine
 
 
-@pytest.mark.parametrize("logn,count", [(1, 3), (4, 5), (6, 100), (10, 2), (10, 5), (11, 3), (14, 1), (12, 7)])
+@pytest.mark.parametrize("logn,count", [(1, 3), (4, 5), (6, 100), (10, 2), (10, 5), (11, 3), (14, 1), (12, 7),
+                                        (15, 3), (17, 1), (18, 2)])
 def test_ntt_matches_oracle(logn, count):
     eng = _eng()
     print(f"ntt logn={logn} count={count}", flush=True)
@@ -37,6 +38,8 @@ def test_ntt_matches_oracle(logn, count):
     (P, (8, 8, 4, 40, 70), 20, 70),       # NTT, middle window
     (P, (16, 16, 16, 129, 200), 0, 328),  # NTT, full product
     (P, (5, 7, 21, 40, 70), 0, 109),      # NTT, column tiles with a ragged last tile
+    (P, (2, 2, 3, 9000, 9000), 0, 17999),   # four-step NTT (2^15 points), full product
+    (P, (2, 3, 2, 9000, 20000), 9000, 20000),  # four-step NTT, middle window
     (97, (4, 4, 2, 30, 50), 10, 60),      # direct products, small prime
 ])
 def test_poly_mat_mul_matches_oracle(p, shape, lo, hi):
@@ -66,7 +69,8 @@ def test_pm_basis_matches_reference_golden():
         i += 1
 
 
-@pytest.mark.parametrize("b,d", [(1, 7), (2, 37), (4, 100), (8, 457), (16, 300), (32, 90), (64, 20)])
+@pytest.mark.parametrize("b,d", [(1, 7), (2, 37), (4, 100), (8, 457), (16, 300), (32, 90), (64, 20), (1, 20000),
+                                 (2, 17000)])
 def test_pm_basis_matches_oracle_big_p(b, d):
     eng = _eng()
     print(f"pm_basis b={b} d={d}", flush=True)
