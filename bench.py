@@ -147,7 +147,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--m", type=int, default=200000)
     ap.add_argument("--n", type=int, default=180000)
-    ap.add_argument("--b", type=int, default=32)
+    ap.add_argument("--b", type=int, default=0, help="block width (default: 32, or 64 for --workload large)")
+    ap.add_argument("--workload", choices=["graph_complex", "large"], default="graph_complex",
+                    help="large: the 5000000 x 4500000, 20 nnz/row single matrix (Krylov only, bounded steps)")
     ap.add_argument("--batched", type=int, default=64, help="matrices in the batched/served comparison (0: skip)")
     ap.add_argument("--rate-frac", type=float, default=0.7, help="served offered load / batched capacity")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
@@ -164,12 +166,18 @@ def main():
         sys.exit(2)
 
     from rust_rank_amd import engine, synth
-    p, b = synth.P_DEFAULT, args.b
+    large = args.workload == "large"
+    p, b = synth.P_DEFAULT, args.b or (64 if large else 32)
     t0 = time.perf_counter()
-    A = synth.graph_complex(m=args.m, n=args.n)
+    if large:  # Krylov rate on a bounded stretch: the whole sequence is about 140k iterations
+        A = synth.large_single()
+        desc = f"large_single {A.m}x{A.n} 20 nnz/row uniform, values uniform in [1,p), deficit 5000, p=2013265921 b={b}"
+    else:
+        A = synth.graph_complex(m=args.m, n=args.n)
+        desc = f"graph_complex {A.m}x{A.n} 12 nnz/row Zipf(1.1) +-1/+-2 p=2013265921 b={b}"
     dr, dc, V = synth.preconditioners(2, A.m, A.n, b, p)
     L = synth.seq_len_plain(min(A.m, A.n), b)
-    steps = args.steps or L
+    steps = args.steps or (256 if large else L)
     log(f"workload {A.m}x{A.n} nnz={A.nnz} b={b} L={L}; generated in {time.perf_counter() - t0:.1f}s")
 
     K = engine.Krylov(A, p, b, dr, dc, V)
@@ -178,7 +186,7 @@ def main():
     seq.append(K.run(steps))  # timed: no per-pass events
     t2 = time.perf_counter()
     have = sum(len(x) for x in seq)
-    if have < L:  # the rest of the sequence for the PM-basis, untimed
+    if have < L and not large:  # the rest of the sequence for the PM-basis, untimed
         seq.append(K.run(L - have))
     seq = np.concatenate(seq)[:L]
     # kernel breakdown on a further stretch of the same sequence, with HIP
@@ -197,37 +205,39 @@ def main():
     log(f"pass by A avg {st['ms_a'] / max(st['passes_a'], 1) * 1e3:.2f} us, by A^T avg "
         f"{st['ms_at'] / max(st['passes_at'], 1) * 1e3:.2f} us")
     log(f"timed {steps} iterations in {wall:.3f}s -> {iters_s:.1f} iters/s; profiled stretch: k_spmm avg "
-        f"{pass_ms * 1e3:.2f} us, fixup {st['ms_fixup'] / max(st['passes_at'], 1) * 1e3:.2f} us per A^T pass, "
+        f"{pass_ms * 1e3:.2f} us, "
         f"reduce {st['ms_reduce']:.2f} ms, stream total {st['ms_total']:.1f} ms")
 
     traffic = None
-    if os.path.exists(PROFILE_TRAFFIC):
+    if os.path.exists(PROFILE_TRAFFIC) and not large:
         with open(PROFILE_TRAFFIC) as f:
             traffic = json.load(f).get("bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": f"k_spmm<{1 << (b - 1).bit_length()}, packed>",
+                "kernel": f"k_spmm<{1 << (b - 1).bit_length()}, {'general' if large else 'packed'}>",
                 "bytes_per_launch": int(bytes_pass), "avg_launch_us": round(pass_ms * 1e3, 3)}
     # the rest of the rank pipeline on this sequence: device PM-basis of order
     # L - 1, then the rank formula
-    t3 = time.perf_counter()
-    basis, delta = engine.pm_basis(seq, b, L - 1, p)
-    t4 = time.perf_counter()
-    rank = engine.rank_from_basis(basis, delta, b, p)
-    del basis
-    pm = {"b": b, "order": L - 1, "ms": round((t4 - t3) * 1e3, 1), "rank": int(rank),
-          "rank_pipeline_s": round(L / iters_s + (t4 - t3), 3)}
-    log(f"PM-basis of order {L - 1}: {pm['ms']} ms; rank {rank}")
-    cpu = None if args.no_cpu else cpu_baseline(A, p, b, dr, dc, V)
+    pm = None
+    if not large:
+        t3 = time.perf_counter()
+        basis, delta = engine.pm_basis(seq, b, L - 1, p)
+        t4 = time.perf_counter()
+        rank = engine.rank_from_basis(basis, delta, b, p)
+        del basis
+        pm = {"b": b, "order": L - 1, "ms": round((t4 - t3) * 1e3, 1), "rank": int(rank),
+              "rank_pipeline_s": round(L / iters_s + (t4 - t3), 3)}
+        log(f"PM-basis of order {L - 1}: {pm['ms']} ms; rank {rank}")
+    cpu = None if (args.no_cpu or large) else cpu_baseline(A, p, b, dr, dc, V)
     batched = served = None
-    if args.batched:
+    if args.batched and not large:
         batched, served = batched_and_served(engine, synth, p, args.batched, args.rate_frac)
     out = {
         "metric": "Krylov iters/s + HBM GB/s; batched ranks/s & p50/p99 vs plain, same GPU",
         "value": round(iters_s, 2), "unit": "iters/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
         "ms_per_step": round(wall * 1e3 / steps, 5), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32", "data": "synthetic (seeded SplitMix64)",
-        "config": {"workload": f"graph_complex {A.m}x{A.n} 12 nnz/row Zipf(1.1) +-1/+-2 p=2013265921 b={b}",
+        "config": {"workload": desc,
                    "seq_len": L, "nnz": A.nnz, "hbm_gbs_algorithmic": round(b_iter * iters_s / 1e9, 1)},
         "roofline": roofline, "cpu_baseline": cpu, "pm_basis": pm, "batched": batched, "served": served,
     }

@@ -89,7 +89,7 @@ rr_status rr_krylov_state(rr_krylov *ctx, uint32_t *curv_out, uint64_t *t_out);
 typedef struct {
     uint64_t passes_a, passes_at; /* sparse passes by A and by A^T */
     double ms_a, ms_at;           /* summed kernel time of those passes */
-    double ms_fixup, ms_reduce;   /* long-row fix-up and sequence reduction kernels */
+    double ms_reduce;             /* sequence reduction kernels */
     double ms_total;              /* wall time of the run on the stream */
     uint64_t bytes_a, bytes_at;   /* algorithmic bytes of one pass by A / by A^T */
 } rr_krylov_stats;

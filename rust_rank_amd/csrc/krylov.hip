@@ -24,8 +24,8 @@
 // RPW length-sorted rows, column indices in blocks of SU = 3 or 4 per row, and
 // every wavefront owns a contiguous, load-balanced stream of slices.  Rows
 // longer than the balance target (the Zipf head of A^T) are cut into
-// segments whose partial sums are added atomically into one 64-bit
-// accumulator row per split row; a small fix-up kernel finishes those rows.  Each
+// segments whose partial sums are added atomically into 64-bit accumulator
+// rows; the wavefront that adds a row's last segment finishes it.  Each
 // workgroup writes one partial b x b upper triangle per pass; a reduction
 // kernel folds the partials of a chunk of passes into the sequence.
 #include "common.hpp"
@@ -955,37 +955,31 @@ rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
         for (uint32_t q = 0; q < chunk; q++) {
             uint64_t t = c->t;
             uint32_t *gram = c->part.as<uint32_t>() + q * part_stride;
-            hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
             if (c->profiling) {
                 RR_HIP(hipEventCreate(&e0));
                 RR_HIP(hipEventCreate(&e1));
-                RR_HIP(hipEventCreate(&e2));
                 RR_HIP(hipEventRecord(e0, c->stream));
             }
             int k;
             if (t == 0) {
-                // S_0 = x_0^T x_0, x_0 = curv; fix-up slots stay zero
+                // S_0 = x_0^T x_0, x_0 = curv
                 RR_HIP(hipMemsetAsync(gram, 0, part_stride * 4, c->stream));
                 launch_gram_any(c, c->xtmp.as<uint32_t>(), (uint32_t)c->n, gram);
                 k = 0;
-                if (c->profiling) {
-                    RR_HIP(hipEventRecord(e1, c->stream));
-                    RR_HIP(hipEventRecord(e2, c->stream));
-                }
+                if (c->profiling) RR_HIP(hipEventRecord(e1, c->stream));
             } else {
                 bool byA = (t & 1) == 1;
                 const SpmmArgs *job = c->jobs.as<SpmmArgs>() + (byA ? 0 : 1);
                 const uint32_t *wgs = (const uint32_t *)(c->jobs.as<char>() + 2 * sizeof(SpmmArgs));
                 launch_spmm(c->B, c->packed, c->su, job, wgs, 1, c->grid, gram, c->stream);
                 if (c->profiling) RR_HIP(hipEventRecord(e1, c->stream));
-                if (c->profiling) RR_HIP(hipEventRecord(e2, c->stream));
                 k = byA ? 1 : 2;
             }
             RR_HIP(hipGetLastError());
             if (c->profiling) {
                 ev.push_back(e0);
                 ev.push_back(e1);
-                ev.push_back(e2);
                 kind.push_back(k);
             }
             if (k == 1) c->stats.passes_a++;
@@ -1016,12 +1010,10 @@ rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
     c->stats.ms_total = ms;
     if (c->profiling) {
         for (size_t i = 0; i < kind.size(); i++) {
-            float a = 0, f = 0;
-            RR_HIP(hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]));
-            RR_HIP(hipEventElapsedTime(&f, ev[3 * i + 1], ev[3 * i + 2]));
+            float a = 0;
+            RR_HIP(hipEventElapsedTime(&a, ev[2 * i], ev[2 * i + 1]));
             if (kind[i] == 1) c->stats.ms_a += a;
             if (kind[i] == 2) c->stats.ms_at += a;
-            c->stats.ms_fixup += f;
         }
         for (auto &pr : red_ev) {
             float r = 0;

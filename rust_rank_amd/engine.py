@@ -32,7 +32,7 @@ class _Csr(C.Structure):
 
 class KrylovStats(C.Structure):
     _fields_ = [("passes_a", C.c_uint64), ("passes_at", C.c_uint64), ("ms_a", C.c_double), ("ms_at", C.c_double),
-                ("ms_fixup", C.c_double), ("ms_reduce", C.c_double), ("ms_total", C.c_double),
+                ("ms_reduce", C.c_double), ("ms_total", C.c_double),
                 ("bytes_a", C.c_uint64), ("bytes_at", C.c_uint64)]
 
     def as_dict(self):

@@ -49,7 +49,7 @@ def test_small_prime():
 
 
 def test_long_rows_split():
-    """A^T rows far above the split threshold (Zipf head): pieces + fix-up."""
+    """A^T rows far above the split threshold (Zipf head): segments finished in the pass."""
     A = synth.graph_complex(seed=5, m=40000, n=2000, k=12)
     assert int(np.bincount(A.col).max()) > 4096
     _case(A, P, 8, 5, 12)
