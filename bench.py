@@ -145,6 +145,7 @@ def main():
     ap.add_argument("--steps", type=int, default=0, help="timed iterations (default: full sequence)")
     ap.add_argument("--warmup", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-pm", action="store_true", help="skip the PM-basis and rank of the sequence")
     ap.add_argument("--m", type=int, default=200000)
     ap.add_argument("--n", type=int, default=180000)
     ap.add_argument("--b", type=int, default=0, help="block width (default: 32, or 64 for --workload large)")
@@ -167,6 +168,7 @@ def main():
 
     from rust_rank_amd import engine, synth
     large = args.workload == "large"
+    with_pm = not (large or args.no_pm)
     p, b = synth.P_DEFAULT, args.b or (64 if large else 32)
     t0 = time.perf_counter()
     if large:  # Krylov rate on a bounded stretch: the whole sequence is about 140k iterations
@@ -186,7 +188,7 @@ def main():
     seq.append(K.run(steps))  # timed: no per-pass events
     t2 = time.perf_counter()
     have = sum(len(x) for x in seq)
-    if have < L and not large:  # the rest of the sequence for the PM-basis, untimed
+    if have < L and with_pm:  # the rest of the sequence for the PM-basis, untimed
         seq.append(K.run(L - have))
     seq = np.concatenate(seq)[:L]
     # kernel breakdown on a further stretch of the same sequence, with HIP
@@ -219,7 +221,7 @@ def main():
     # the rest of the rank pipeline on this sequence: device PM-basis of order
     # L - 1, then the rank formula
     pm = None
-    if not large:
+    if with_pm:
         t3 = time.perf_counter()
         basis, delta = engine.pm_basis(seq, b, L - 1, p)
         t4 = time.perf_counter()

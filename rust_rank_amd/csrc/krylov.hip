@@ -31,6 +31,7 @@
 #include "common.hpp"
 #include "krylov_kernels.hpp"
 #include "spmm.hpp"
+#include <hip/hip_ext.h>
 #include <vector>
 #include <algorithm>
 #include <cstring>
@@ -670,24 +671,30 @@ SpmmArgs spmm_args(const SliceMat &M, const uint32_t *in, uint64_t in_rows, uint
 
 template <int B>
 static void launch_b(bool packed, uint32_t su, const SpmmArgs *jobs, const uint32_t *wg_begin, uint32_t njobs,
-                     uint32_t total_wg, uint32_t *gram_base, hipStream_t st) {
+                     uint32_t total_wg, uint32_t *gram_base, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     dim3 g(total_wg), t(WG);
-    if (packed && su == 3) hipLaunchKernelGGL((k_spmm<B, true, 3>), g, t, 0, st, jobs, wg_begin, njobs, gram_base);
-    else if (packed) hipLaunchKernelGGL((k_spmm<B, true, 4>), g, t, 0, st, jobs, wg_begin, njobs, gram_base);
-    else if (su == 3) hipLaunchKernelGGL((k_spmm<B, false, 3>), g, t, 0, st, jobs, wg_begin, njobs, gram_base);
-    else hipLaunchKernelGGL((k_spmm<B, false, 4>), g, t, 0, st, jobs, wg_begin, njobs, gram_base);
+    // with events, the dispatch itself stamps its start and end (no marker between launches)
+    if (packed && su == 3)
+        hipExtLaunchKernelGGL((k_spmm<B, true, 3>), g, t, 0, st, ev0, ev1, 0, jobs, wg_begin, njobs, gram_base);
+    else if (packed)
+        hipExtLaunchKernelGGL((k_spmm<B, true, 4>), g, t, 0, st, ev0, ev1, 0, jobs, wg_begin, njobs, gram_base);
+    else if (su == 3)
+        hipExtLaunchKernelGGL((k_spmm<B, false, 3>), g, t, 0, st, ev0, ev1, 0, jobs, wg_begin, njobs, gram_base);
+    else
+        hipExtLaunchKernelGGL((k_spmm<B, false, 4>), g, t, 0, st, ev0, ev1, 0, jobs, wg_begin, njobs, gram_base);
 }
 
 void launch_spmm(uint32_t B, bool packed, uint32_t su, const SpmmArgs *jobs, const uint32_t *wg_begin,
-                 uint32_t njobs, uint32_t total_wg, uint32_t *gram_base, hipStream_t st) {
+                 uint32_t njobs, uint32_t total_wg, uint32_t *gram_base, hipStream_t st, hipEvent_t ev0,
+                 hipEvent_t ev1) {
     switch (B) {
-        case 1: launch_b<1>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st); break;
-        case 2: launch_b<2>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st); break;
-        case 4: launch_b<4>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st); break;
-        case 8: launch_b<8>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st); break;
-        case 16: launch_b<16>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st); break;
-        case 32: launch_b<32>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st); break;
-        default: launch_b<64>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st); break;
+        case 1: launch_b<1>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
+        case 2: launch_b<2>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
+        case 4: launch_b<4>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
+        case 8: launch_b<8>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
+        case 16: launch_b<16>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
+        case 32: launch_b<32>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
+        default: launch_b<64>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
     }
 }
 
@@ -959,11 +966,11 @@ rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
             if (c->profiling) {
                 RR_HIP(hipEventCreate(&e0));
                 RR_HIP(hipEventCreate(&e1));
-                RR_HIP(hipEventRecord(e0, c->stream));
             }
             int k;
             if (t == 0) {
                 // S_0 = x_0^T x_0, x_0 = curv
+                if (c->profiling) RR_HIP(hipEventRecord(e0, c->stream));
                 RR_HIP(hipMemsetAsync(gram, 0, part_stride * 4, c->stream));
                 launch_gram_any(c, c->xtmp.as<uint32_t>(), (uint32_t)c->n, gram);
                 k = 0;
@@ -972,8 +979,7 @@ rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
                 bool byA = (t & 1) == 1;
                 const SpmmArgs *job = c->jobs.as<SpmmArgs>() + (byA ? 0 : 1);
                 const uint32_t *wgs = (const uint32_t *)(c->jobs.as<char>() + 2 * sizeof(SpmmArgs));
-                launch_spmm(c->B, c->packed, c->su, job, wgs, 1, c->grid, gram, c->stream);
-                if (c->profiling) RR_HIP(hipEventRecord(e1, c->stream));
+                launch_spmm(c->B, c->packed, c->su, job, wgs, 1, c->grid, gram, c->stream, e0, e1);
                 k = byA ? 1 : 2;
             }
             RR_HIP(hipGetLastError());

@@ -77,7 +77,9 @@ SpmmArgs spmm_args(const SliceMat &M, const uint32_t *in, uint64_t in_rows, uint
 int spmm_blocks_per_cu(uint32_t B, bool packed, uint32_t su);
 
 // One launch over njobs jobs (device arrays); total_wg = wg_begin[njobs].
+// ev0 / ev1 (may be null): stamped with the kernel's own start and end.
 void launch_spmm(uint32_t B, bool packed, uint32_t su, const SpmmArgs *jobs, const uint32_t *wg_begin,
-                 uint32_t njobs, uint32_t total_wg, uint32_t *gram_base, hipStream_t st);
+                 uint32_t njobs, uint32_t total_wg, uint32_t *gram_base, hipStream_t st, hipEvent_t ev0 = nullptr,
+                 hipEvent_t ev1 = nullptr);
 
 }  // namespace rr
