@@ -596,8 +596,10 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
 //         probe matrix with +-1/+-2 entries, 16 ticks timed with events),
 //   pm_b  PM-basis time per order d = 255 for block width b.
 // Predicted device time of matrix i with width b:
-//   L(b) * nnz_i * c_b + pm_b * (d(b) / 255) * log2(d(b) + 1) / 8,
+//   L(b) * nnz_i * c_b + pm_b * d(b) / 255,
 //   L(b) = 2 min(m, n) / b + 8, d(b) = L(b) - 1.
+// The PM-basis term is linear in d: its time is dominated by the chain of
+// leaves (measured: d = 255, 2047, 4095 at b = 16 take 10.5, 85, 168 ms).
 
 namespace rr {
 
@@ -703,7 +705,7 @@ rr_status rr_autotune_b(uint32_t count, const rr_csr *mats, uint32_t p, uint32_t
             uint32_t b = TUNE_B[k];
             double L = 2.0 * (double)mn / b + 8, d = L - 1;
             if (d + 1 > (double)(1u << NTT_MAX_LOGN)) continue;
-            double t = L * (double)nnz * g_probe.c[k] + g_probe.pm[k] * (d / 255.0) * std::log2(d + 1) / 8.0;
+            double t = L * (double)nnz * g_probe.c[k] + g_probe.pm[k] * (d / 255.0);
             if (t < best) {
                 best = t;
                 bb = b;

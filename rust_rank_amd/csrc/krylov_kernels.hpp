@@ -1,6 +1,6 @@
-// krylov_kernels.hpp -- device building blocks of the fused sparse pass:
-// one row of the mod-p sparse product on B lanes, and the per-lane
-// accumulation of the symmetric b x b projection.
+// krylov_kernels.hpp -- device building blocks of the fused sparse pass and
+// the t = 0 projections: exact reductions mod p and the accumulation of the
+// symmetric b x b projection.
 #pragma once
 #include "common.hpp"
 
@@ -14,108 +14,10 @@ constexpr int WG = 256;  // threads per workgroup (4 wavefronts)
 // more).  LDS operations of one wavefront complete in order; the wave
 // barrier keeps the compiler from moving them across each other.
 struct WaveLds {
-    uint32_t col[64];  // column indices of the current chunk, per row slot
-    uint32_t val[64];  // values of the current chunk (general matrices)
-    uint32_t o[64];    // produced row entries, for the projection
+    uint32_t o[64];  // the rows' entries, for the projection
 };
 
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
-
-// One row of the sparse product on the B lanes of a row slot; lane j returns
-// sum_e v_e * in[col_e][j] mod p.  The slot loads up to B column indices (and
-// values) with one coalesced load, shares them through LDS and issues up to
-// 16 independent gathers before using any, so a row costs about two dependent
-// memory latencies per B nonzeros.  Lanes of one slot share the row, so they
-// run the same trip counts.
-//   packed:  |v| <= 8, x < 2^31, each term < 2^34 (exact in fp64), folded per chunk
-//   general: v = vh 2^16 + vl, vh x < 2^46, vl x < 2^47, <= 64 terms per chunk
-template <int B, bool PACKED>
-__device__ __forceinline__ uint32_t row_dot(const uint32_t *__restrict__ col, const uint32_t *__restrict__ val,
-                                            const uint32_t *__restrict__ in, uint32_t beg, uint32_t end, int j,
-                                            WaveLds &w, double pd, double inv) {
-    constexpr int GS = B < 16 ? B : 16;
-    const int lane = (int)(threadIdx.x & 63), lb = lane & ~(B - 1);
-    double a0 = 0.0, a1 = 0.0;
-    for (uint32_t c0 = beg; c0 < end; c0 += B) {
-        const uint32_t n = min((uint32_t)B, end - c0);
-        w.col[lane] = (uint32_t)j < n ? col[c0 + j] : 0u;
-        if constexpr (!PACKED) w.val[lane] = (uint32_t)j < n ? val[c0 + j] : 0u;
-        wave_sync();
-        for (uint32_t e = 0; e < n; e += GS) {
-            uint32_t c[GS], x[GS], v[GS];
-#pragma unroll
-            for (int k = 0; k < GS; k++) {
-                c[k] = w.col[lb + (int)((e + k) & (B - 1))];
-                if constexpr (!PACKED) v[k] = w.val[lb + (int)((e + k) & (B - 1))];
-            }
-#pragma unroll
-            for (int k = 0; k < GS; k++) {
-                const uint32_t idx = PACKED ? (c[k] & 0x0FFFFFFFu) : c[k];
-                x[k] = (e + k < n) ? in[(size_t)idx * B + j] : 0u;
-            }
-#pragma unroll
-            for (int k = 0; k < GS; k++) {
-                if constexpr (PACKED) {
-                    const double t = (double)((int32_t)c[k] >> 28);
-                    if (k & 1) a1 = fma(t, (double)x[k], a1);
-                    else a0 = fma(t, (double)x[k], a0);
-                } else {
-                    const double xd = (double)x[k];
-                    a0 = fma((double)(v[k] >> 16), xd, a0);
-                    a1 = fma((double)(v[k] & 0xFFFFu), xd, a1);
-                }
-            }
-        }
-        wave_sync();
-        a0 = fold(a0, pd, inv);
-        a1 = fold(a1, pd, inv);
-    }
-    if constexpr (PACKED) return (uint32_t)fold(a0 + a1, pd, inv);
-    else return (uint32_t)fold(fma(a0, 65536.0, a1), pd, inv);
-}
-
-// One row whose column indices (and values) are already staged in LDS at
-// tc[0 .. n): the B lanes gather and accumulate as row_dot does.
-template <int B, bool PACKED>
-__device__ __forceinline__ uint32_t row_dot_staged(const uint32_t *tc, const uint32_t *tv,
-                                                   const uint32_t *__restrict__ in, uint32_t n, int j, double pd,
-                                                   double inv) {
-    constexpr int GS = B < 16 ? B : 16;
-    double a0 = 0.0, a1 = 0.0;
-    for (uint32_t c0 = 0; c0 < n; c0 += 64) {  // <= 64 terms between folds
-        const uint32_t cn = min(64u, n - c0);
-        for (uint32_t e = 0; e < cn; e += GS) {
-            uint32_t c[GS], x[GS], v[GS];
-#pragma unroll
-            for (int k = 0; k < GS; k++) {
-                const uint32_t q = min(c0 + e + k, n - 1);
-                c[k] = tc[q];
-                if constexpr (!PACKED) v[k] = tv[q];
-            }
-#pragma unroll
-            for (int k = 0; k < GS; k++) {
-                const uint32_t idx = PACKED ? (c[k] & 0x0FFFFFFFu) : c[k];
-                x[k] = (e + k < cn) ? in[(size_t)idx * B + j] : 0u;
-            }
-#pragma unroll
-            for (int k = 0; k < GS; k++) {
-                if constexpr (PACKED) {
-                    const double t = (double)((int32_t)c[k] >> 28);
-                    if (k & 1) a1 = fma(t, (double)x[k], a1);
-                    else a0 = fma(t, (double)x[k], a0);
-                } else {
-                    const double xd = (double)x[k];
-                    a0 = fma((double)(v[k] >> 16), xd, a0);
-                    a1 = fma((double)(v[k] & 0xFFFFu), xd, a1);
-                }
-            }
-        }
-        a0 = fold(a0, pd, inv);
-        a1 = fold(a1, pd, inv);
-    }
-    if constexpr (PACKED) return (uint32_t)fold(a0 + a1, pd, inv);
-    else return (uint32_t)fold(fma(a0, 65536.0, a1), pd, inv);
-}
 
 // ---------------------------------------------------------------------------
 // Per-lane accumulation of the symmetric projection sum_r o_r s_r^T.
@@ -272,83 +174,6 @@ struct GramRing {
             for (int w = 0; w < nw; w++) sum = addmod(sum, lds[w * Pb + t], p);
             out[t] = sum;
         }
-    }
-};
-
-// ---------------------------------------------------------------------------
-// Exact projection on the matrix cores, for lane width 32 (b = 17..32).
-// Each value x in [0, p) is replaced by its signed representative
-// x' in (-p/2, p/2) and written in balanced base 256, x' = sum_a d_a 256^a
-// with d_a in [-128, 127]; the four digits are the bytes of
-// (x' + 0x80808080) ^ 0x80808080.  Then
-//   sum_r o_r s_r^T = sum_{a,b} 256^(a+b) sum_r d_a(o_r) d_b(s_r)^T,
-// and every inner sum is an int8 matrix product accumulated in int32 by
-// v_mfma_i32_32x32x16_i8 (A 32 x 16: lane l holds A[l % 32][8 (l / 32) + byte];
-// B 16 x 32: lane l holds B[8 (l / 32) + byte][l % 32]; C 32 x 32: lane l,
-// register v holds C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32]).  In the pass,
-// lane l owns column l % 32 of a row, so eight successive rows of a lane are
-// exactly its operand bytes: no data moves between lanes.  Seven accumulators
-// hold the weights 256^0 .. 256^6; |digit products| <= 2^14, so 2^16 rows fit
-// int32 before the flush into residues mod p that happens every 16384 rows.
-typedef int mfma_v16i __attribute__((ext_vector_type(16)));
-
-struct GramMfma32 {
-    mfma_v16i acc[7];
-    uint64_t qa[4], qb[4];  // operand bytes being assembled: row t of the block in byte t
-    uint32_t res[16];       // flushed residues mod p of sum_c 256^c acc[c]
-    int nbuf, blocks;
-    uint32_t p;
-    double inv;
-
-    __device__ void init_p(uint32_t p_, double inv_) {
-#pragma unroll
-        for (int c = 0; c < 7; c++) acc[c] = mfma_v16i{0};
-#pragma unroll
-        for (int v = 0; v < 16; v++) res[v] = 0;
-#pragma unroll
-        for (int a = 0; a < 4; a++) qa[a] = qb[a] = 0;
-        nbuf = 0;
-        blocks = 0;
-        p = p_;
-        inv = inv_;
-    }
-    __device__ static __forceinline__ uint32_t digits(uint32_t x, uint32_t p) {
-        int32_t xs = x > (p >> 1) ? (int32_t)x - (int32_t)p : (int32_t)x;
-        return ((uint32_t)xs + 0x80808080u) ^ 0x80808080u;
-    }
-    __device__ void flush() {
-        // res += sum_c 256^c acc[c] mod p, then acc = 0
-        const double pd = (double)p;
-#pragma unroll
-        for (int v = 0; v < 16; v++) {
-            uint32_t t = res[v], wgt = 1;
-#pragma unroll
-            for (int c = 0; c < 7; c++) {
-                const uint32_t a = (uint32_t)fold((double)acc[c][v], pd, inv);  // |acc| < 2^31
-                t = addmod(t, mulmod(a, wgt, p, inv), p);
-                wgt = mulmod(wgt, 256u, p, inv);
-            }
-            res[v] = t;
-        }
-#pragma unroll
-        for (int c = 0; c < 7; c++) acc[c] = mfma_v16i{0};
-        blocks = 0;
-    }
-    __device__ __forceinline__ void add(uint32_t o, uint32_t s) {
-        const uint32_t wo = digits(o, p), ws = digits(s, p);
-#pragma unroll
-        for (int a = 0; a < 4; a++) {
-            qa[a] = (qa[a] >> 8) | ((uint64_t)((wo >> (8 * a)) & 0xFFu) << 56);
-            qb[a] = (qb[a] >> 8) | ((uint64_t)((ws >> (8 * a)) & 0xFFu) << 56);
-        }
-        if (++nbuf < 8) return;
-        nbuf = 0;
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int b = 0; b < 4; b++)
-                acc[a + b] = __builtin_amdgcn_mfma_i32_32x32x16_i8((long)qa[a], (long)qb[b], acc[a + b], 0, 0, 0);
-        if (++blocks == 1024) flush();
     }
 };
 

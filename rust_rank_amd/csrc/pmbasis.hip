@@ -8,10 +8,12 @@
 // basis is done.
 //
 // The order d is arbitrary (the reference needs a power of two,
-// src/sigma_basis.rs:22-29): it splits as d1 = d - d/2, d2 = d/2, which is the
-// reference's split for powers of two.  The basis is the product of the
-// order-1 leaves M_{d-1} ... M_0, each computed from the constant term of the
-// exact residual, so any split gives the same bits.
+// src/sigma_basis.rs:22-29): a node of order d > K (the job's leaf order) gives
+// its left part about half of its ceil(d / K) leaves, all of full order K, so
+// every leaf but the last has order K; for powers of two this is the
+// reference's split d1 = d2 = d/2.  The basis is the product of the order-1
+// steps M_{d-1} ... M_0, each computed from the constant term of the exact
+// residual, so any split gives the same bits.
 //
 // Leaf (M_Basis): one workgroup.  Rows of the constant term are sorted by
 // delta (stable, descending), then eliminated LSP-style (pivot = first nonzero
@@ -35,7 +37,7 @@
 namespace rr {
 
 // ---------------------------------------------------------------------------
-// Leaf: leaf_order(m) or fewer M-Basis steps in one workgroup.  Step s takes the
+// Leaf: up to the job's leaf order of M-Basis steps in one workgroup.  Step s takes the
 // constant term of the current residual P = M_s G (coefficient s), sorts its
 // rows by delta (stable, descending, as Rust's sort_by), eliminates LSP-style
 // (pivot = first nonzero position >= rank in the swapped column order,
@@ -50,6 +52,11 @@ namespace rr {
 // Steps per leaf: the in-leaf basis update costs m^2 rank per coefficient,
 // so wider blocks take shorter leaves.
 __host__ __device__ constexpr uint32_t leaf_order(uint32_t m) { return m <= 16 ? 16 : m <= 32 ? 8 : m <= 64 ? 4 : 1; }
+// A PM-basis running alone is a chain of leaf launches, and shorter leaves
+// (less basis update per coefficient) finish it sooner; in batches and in
+// the server the leaves of many jobs share rounds, and longer leaves mean
+// fewer rounds.  The basis is the same either way.
+__host__ __device__ constexpr uint32_t leaf_order_alone(uint32_t m) { return m <= 64 ? leaf_order(m) / 2 : 1; }
 constexpr int LEAF_PACK = 64;
 struct LeafPack {
     LeafDesc d[LEAF_PACK];
@@ -696,12 +703,21 @@ static rr_status launch_school(const std::vector<MulDesc> &ops, uint32_t p, hipS
 // recursion of src/sigma_basis.rs:190-219, so that the next device operation
 // of many matrices can be launched together.
 
-static uint64_t rec_words(uint64_t d, uint32_t m, uint32_t b) {
-    if (d <= 1) return 0;
-    uint64_t d1 = d - d / 2, d2 = d / 2;
+// Order of the left part of a node of order d > K: K times half (rounded
+// up) of its ceil(d / K) leaves; 1 <= d1 < d.
+static inline uint64_t split_left(uint64_t d, uint64_t K) {
+    const uint64_t leaves = (d + K - 1) / K;
+    return K * ((leaves + 1) / 2);
+}
+
+// Workspace words along the deepest path of the recursion below an order-d
+// node with leaves of order K.
+static uint64_t rec_words(uint64_t d, uint64_t K, uint32_t m, uint32_t b) {
+    if (d <= K) return 0;
+    const uint64_t d1 = split_left(d, K), d2 = d - d1;
     uint64_t here = ((uint64_t)m * m * (d1 + 1) + 63) + ((uint64_t)m * b * (d2 + 1) + 63) +
                     ((uint64_t)m * m * (d2 + 1) + 63);
-    return here + std::max(rec_words(d1, m, b), rec_words(d2, m, b));
+    return here + std::max(rec_words(d1, K, m, b), rec_words(d2, K, m, b));
 }
 
 struct Op {
@@ -716,7 +732,7 @@ static bool next_op(PmJob &J, Op &op, rr_status &err) {
     while (!J.stack.empty()) {
         size_t fi = J.stack.size() - 1;
         Frame &F = J.stack[fi];
-        if (F.d <= leaf_order(J.m)) {
+        if (F.d <= J.K) {
             if (F.stage == 0) {
                 F.stage = 1;
                 op.kind = 0;
@@ -726,7 +742,7 @@ static bool next_op(PmJob &J, Op &op, rr_status &err) {
             J.stack.pop_back();
             continue;
         }
-        const uint64_t d1 = F.d - F.d / 2, d2 = F.d / 2;
+        const uint64_t d1 = split_left(F.d, J.K), d2 = F.d - d1;
         switch (F.stage) {
             case 0: {
                 F.mark = J.pool.top;
@@ -818,8 +834,10 @@ rr_status PmScheduler::init(uint32_t p_, hipStream_t st_, uint32_t max_logn, uin
 
 rr_status PmScheduler::add(PmJob *J) {
     J->m = 2 * J->b;
+    if (!J->K) J->K = leaf_order(J->m);
+    RR_CHECK(J->K >= 1 && J->K <= leaf_order(J->m), RR_EINVAL, "leaf order %u above %u", J->K, leaf_order(J->m));
     rr_status s;
-    if ((s = J->pool.init(rec_words(J->d, J->m, J->b) + 1024)) != RR_OK) return s;
+    if ((s = J->pool.init(rec_words(J->d, J->K, J->m, J->b) + 1024)) != RR_OK) return s;
     RR_HIP(hipMemsetAsync(J->delta, 0, (size_t)J->m * 8, st));
     J->stack.clear();
     J->stack.push_back(Frame{J->G, J->d + 1, J->d, J->M, 0, nullptr, nullptr, nullptr, 0});
@@ -902,6 +920,7 @@ rr_status PmScheduler::run(std::vector<PmJob *> &jobs) {
     }
     rr_status s;
     if ((s = init(p, st, max_logn, max_m, max_n)) != RR_OK) return s;
+    if (jobs.size() == 1 && !jobs[0]->K) jobs[0]->K = leaf_order_alone(2 * jobs[0]->b);
     for (auto *J : jobs)
         if ((s = add(J)) != RR_OK) return s;
     for (;;) {

@@ -49,6 +49,7 @@ struct Frame {
 
 struct PmJob {
     uint32_t b = 0, m = 0;
+    uint32_t K = 0;  // leaf order (0: chosen when the job joins)
     uint64_t d = 0;
     const uint32_t *G = nullptr;  // [m][b][d+1]
     uint32_t *M = nullptr;        // [m][m][d+1]
