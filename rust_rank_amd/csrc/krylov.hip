@@ -37,6 +37,7 @@
 #include <cstring>
 #include <memory>
 #include <queue>
+#include <type_traits>
 
 namespace rr {
 
@@ -127,7 +128,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     constexpr int VW = L::VW, LPR = L::LPR, RPW = L::RPW, BW = RPW * SU;
     __shared__ uint32_t o2s[WG / 64][RPW * 2 * B];  // produced rows, each twice
     __shared__ uint32_t ss[WG / 64][RPW * B];       // their scaled rows
-    __shared__ uint32_t fin[GramRing<B>::LDS_WORDS];
+    // the matrix-core projection where the registers allow it (no spills)
+    constexpr bool MMA = B == 32 && PACKED && SU == 3;
+    using GramT = std::conditional_t<MMA, GramMma32, GramRing<B>>;
+    __shared__ uint32_t fin[GramT::LDS_WORDS];
     __shared__ uint32_t pend[WG / 64][MAX_PEND];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane / LPR, q = lane % LPR, ci = lane % B;
@@ -140,7 +144,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     const uint32_t *vb = PACKED ? nullptr : a.val + wblk * BW + g * SU;
     uint32_t *o2 = o2s[wave], *sv = ss[wave];
 
-    GramRing<B> gr;
+    GramT gr;
     gr.init();
     uint32_t npend = 0;  // split rows whose last segment this wavefront added
 
@@ -308,8 +312,12 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
             finish();
             if (rows_ready) {  // the slice's share of the projection
                 wave_sync();
+                if constexpr (MMA) {
+                    gr.add_slice(o2, sv, a.pd, a.inv);
+                } else {
 #pragma unroll 1
-                for (int r = 0; r < RPW; r++) gr.add(o2 + r * 2 * B, sv[r * B + ci], a.p, a.pd, a.inv);
+                    for (int r = 0; r < RPW; r++) gr.add(o2 + r * 2 * B, sv[r * B + ci], a.p, a.pd, a.inv);
+                }
                 wave_sync();
             }
         }
@@ -346,7 +354,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
         }
         if (lane == 0) atomicExch(a.lcnt + l, 0u);
         wave_sync();
-        gr.add(o2, sv[ci], a.p, a.pd, a.inv);
+        if constexpr (MMA) gr.add_row(o2, sv, a.pd, a.inv);
+        else gr.add(o2, sv[ci], a.p, a.pd, a.inv);
         wave_sync();
     }
     __syncthreads();

@@ -177,4 +177,86 @@ struct GramRing {
     }
 };
 
+// ---------------------------------------------------------------------------
+// The projection for lane width 32 on the matrix cores: S = sum_r o_r s_r^T
+// as fp64 products v_mfma_f64_16x16x4f64 (lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16]; register v of C holds C[4 v + l / 16][l % 16]) over
+// four rows at a time, for the three upper 16 x 16 tiles (0,0), (0,1), (1,1)
+// of the 32 x 32 triangle.  s is split into 16-bit halves, so every product
+// is an integer below 2^47 and 64 rows of them stay below 2^53: the sums are
+// exact whatever the order, and are folded mod p every 64 rows.
+typedef double gram_v4d __attribute__((ext_vector_type(4)));
+
+struct GramMma32 {
+    static constexpr int P = 32 * 33 / 2;
+    static constexpr int LDS_WORDS = (WG / 64) * P;
+    gram_v4d acc[6];  // tile (0,0) hi, lo; (0,1) hi, lo; (1,1) hi, lo
+    uint32_t rows;
+
+    __device__ void init() {
+#pragma unroll
+        for (int t = 0; t < 6; t++) acc[t] = gram_v4d{0.0, 0.0, 0.0, 0.0};
+        rows = 0;
+    }
+    // Rows k < nrows (<= 4) of o (stride ostride words) and s (stride 32 words).
+    __device__ __forceinline__ void step(const uint32_t *o, uint32_t ostride, const uint32_t *s, uint32_t nrows,
+                                         double pd, double inv) {
+        const uint32_t lane = threadIdx.x & 63, k = lane >> 4, i = lane & 15;
+        const bool ok = k < nrows;
+        const uint32_t kk = ok ? k : 0;
+        const uint32_t o0 = o[kk * ostride + i], o1 = o[kk * ostride + 16 + i];
+        const uint32_t s0 = s[kk * 32 + i], s1 = s[kk * 32 + 16 + i];
+        const double a0 = ok ? (double)o0 : 0.0, a1 = ok ? (double)o1 : 0.0;
+        const double b0h = (double)(s0 >> 16), b0l = (double)(s0 & 0xFFFFu);
+        const double b1h = (double)(s1 >> 16), b1l = (double)(s1 & 0xFFFFu);
+        acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0h, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0l, acc[1], 0, 0, 0);
+        acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1h, acc[2], 0, 0, 0);
+        acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1l, acc[3], 0, 0, 0);
+        acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1h, acc[4], 0, 0, 0);
+        acc[5] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1l, acc[5], 0, 0, 0);
+        rows += 4;
+        if (rows == 64) {
+#pragma unroll
+            for (int t = 0; t < 6; t++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) acc[t][v] = fold(acc[t][v], pd, inv);
+            rows = 0;
+        }
+    }
+    // The eight rows of a slice: o rows doubled (stride 64 words), s rows (stride 32).
+    __device__ __forceinline__ void add_slice(const uint32_t *o2, const uint32_t *s, double pd, double inv) {
+        step(o2, 64, s, 4, pd, inv);
+        step(o2 + 4 * 64, 64, s + 4 * 32, 4, pd, inv);
+    }
+    __device__ __forceinline__ void add_row(const uint32_t *o, const uint32_t *s, double pd, double inv) {
+        step(o, 64, s, 1, pd, inv);
+    }
+    // Sums the workgroup's wavefronts and writes the b(b+1)/2 upper-triangle
+    // partial to out (row-major pair index), as GramRing::finish.
+    __device__ void finish(uint32_t *out, uint32_t b, uint32_t p, double pd, double inv, uint32_t *lds) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const uint32_t Pb = b * (b + 1) / 2;
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const uint32_t ti = t == 2 ? 1 : 0, tj = t == 0 ? 0 : 1;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const uint32_t i = 16 * ti + 4 * v + (lane >> 4), j = 16 * tj + (lane & 15);
+                if (i <= j && j < b) {
+                    const double h = fold(acc[2 * t][v], pd, inv), l = fold(acc[2 * t + 1][v], pd, inv);
+                    lds[wave * Pb + i * b - i * (i - 1) / 2 + (j - i)] = (uint32_t)fold(fma(h, 65536.0, l), pd, inv);
+                }
+            }
+        }
+        __syncthreads();
+        const int nw = blockDim.x >> 6;
+        for (uint32_t t = threadIdx.x; t < Pb; t += blockDim.x) {
+            uint32_t sum = 0;
+            for (int w = 0; w < nw; w++) sum = addmod(sum, lds[w * Pb + t], p);
+            out[t] = sum;
+        }
+    }
+};
+
 }  // namespace rr
