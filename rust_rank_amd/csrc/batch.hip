@@ -355,25 +355,30 @@ static void launch_gram_tiles(const GramTile *tiles, uint32_t n, uint32_t grid, 
     hipLaunchKernelGGL(k_gram_tiles<B>, dim3(grid), dim3(WG), 0, st, tiles, n, mp.p, mp.pd, mp.inv);
 }
 
-// One tick: the next entry of every request in `reqs` (all in the Krylov
-// stage).  Requests at t = 0 contribute projection tiles; the others one
-// pass job each, grouped by (lane width, value kind, block size) into one
-// k_spmm launch per group.  Every request's partials are reduced alone.
-rr_status TickRunner::tick(std::vector<Request *> &reqs, uint32_t p) {
-    if (reqs.empty()) return RR_OK;
+// nticks ticks: the next nticks entries of every request in `reqs` (all in
+// the Krylov stage).  Requests at t = 0 contribute projection tiles (then
+// nticks must be 1); the others one pass job per tick, grouped by (lane
+// width, value kind, block size) into one k_spmm launch per group and tick.
+// The descriptors of all the ticks go down in one copy, and one
+// k_reduce_jobs launch folds every request's partials of every tick (a
+// request's partials are always reduced alone).
+rr_status TickRunner::tick(std::vector<Request *> &reqs, uint32_t p, uint32_t nticks) {
+    if (reqs.empty() || nticks == 0) return RR_OK;
     ModP mp = ModP::make(p);
+    for (auto *R : reqs)
+        RR_CHECK(R->t + nticks <= R->L && (nticks == 1 || R->t > 0), RR_EINVAL, "tick chunk past a sequence");
     struct Group {
         uint32_t B = 0, su = 0;
         bool packed = false;
         std::vector<SpmmArgs> jobs;
         std::vector<uint32_t> wg;  // prefix of workgroups, jobs + 1 entries
     };
-    std::vector<Group> groups;
+    std::vector<std::vector<Group>> groups(nticks);
     std::vector<GramTile> gram_by_b[7];
     std::vector<RedJob> red;
     size_t part_words = 0;
     for (auto *R : reqs)
-        part_words += (size_t)(R->t == 0 ? R->tiles_gram.size() : R->nwg) * R->P;
+        part_words += (size_t)(R->t == 0 ? R->tiles_gram.size() : (size_t)R->nwg * nticks) * R->P;
     rr_status s;
     if (part_words > part_cap) {
         RR_HIP(hipStreamSynchronize(st));
@@ -384,58 +389,62 @@ rr_status TickRunner::tick(std::vector<Request *> &reqs, uint32_t p) {
     }
     uint32_t *pbase = part.as<uint32_t>();
     size_t off = 0;
-    for (auto *R : reqs) {
-        RedJob J;
-        J.part = pbase + off;
-        J.P = R->P;
-        J.out = R->seq.as<uint32_t>() + R->t * R->P;
-        if (R->t == 0) {
-            int lb = 0;
-            while ((1u << lb) < R->B) lb++;
-            J.ntiles = (uint32_t)R->tiles_gram.size();
-            for (const GramTile &T0 : R->tiles_gram) {
-                GramTile T = T0;
-                T.part = pbase + off;
-                off += R->P;
-                gram_by_b[lb].push_back(T);
+    for (uint32_t c = 0; c < nticks; c++) {
+        for (auto *R : reqs) {
+            const uint64_t t = R->t + c;
+            RedJob J;
+            J.part = pbase + off;
+            J.P = R->P;
+            J.out = R->seq.as<uint32_t>() + t * R->P;
+            if (t == 0) {
+                int lb = 0;
+                while ((1u << lb) < R->B) lb++;
+                J.ntiles = (uint32_t)R->tiles_gram.size();
+                for (const GramTile &T0 : R->tiles_gram) {
+                    GramTile T = T0;
+                    T.part = pbase + off;
+                    off += R->P;
+                    gram_by_b[lb].push_back(T);
+                }
+            } else {
+                Group *G = nullptr;
+                for (auto &g : groups[c])
+                    if (g.B == R->B && g.su == R->su && g.packed == R->packed) G = &g;
+                if (!G) {
+                    groups[c].emplace_back();
+                    G = &groups[c].back();
+                    G->B = R->B;
+                    G->su = R->su;
+                    G->packed = R->packed;
+                    G->wg.push_back(0);
+                }
+                SpmmArgs a = R->args[(t & 1) ? 0 : 1];
+                a.gram_off = off;
+                G->jobs.push_back(a);
+                G->wg.push_back(G->wg.back() + R->nwg);
+                J.ntiles = R->nwg;
+                off += (size_t)R->nwg * R->P;
             }
-        } else {
-            Group *G = nullptr;
-            for (auto &g : groups)
-                if (g.B == R->B && g.su == R->su && g.packed == R->packed) G = &g;
-            if (!G) {
-                groups.emplace_back();
-                G = &groups.back();
-                G->B = R->B;
-                G->su = R->su;
-                G->packed = R->packed;
-                G->wg.push_back(0);
-            }
-            SpmmArgs a = R->args[(R->t & 1) ? 0 : 1];
-            a.gram_off = off;
-            G->jobs.push_back(a);
-            G->wg.push_back(G->wg.back() + R->nwg);
-            J.ntiles = R->nwg;
-            off += (size_t)R->nwg * R->P;
+            red.push_back(J);
         }
-        red.push_back(J);
     }
     // stage descriptors in one pinned block, one copy:
-    // [gram tiles by width][per group: jobs, workgroup prefix][reduction jobs]
+    // [gram tiles by width][per tick, per group: jobs, workgroup prefix][reduction jobs]
     auto align = [](size_t x) { return (x + 15) & ~(size_t)15; };
     size_t bytes = 0;
-    size_t gram_off[7], job_off[64], wg_off[64];
+    size_t gram_off[7];
+    std::vector<std::vector<size_t>> job_off(nticks), wg_off(nticks);
     for (int lb = 0; lb < 7; lb++) {
         gram_off[lb] = bytes;
         bytes = align(bytes + gram_by_b[lb].size() * sizeof(GramTile));
     }
-    RR_CHECK(groups.size() <= 64, RR_EINVAL, "too many pass kinds in one tick");
-    for (size_t gi = 0; gi < groups.size(); gi++) {
-        job_off[gi] = bytes;
-        bytes = align(bytes + groups[gi].jobs.size() * sizeof(SpmmArgs));
-        wg_off[gi] = bytes;
-        bytes = align(bytes + groups[gi].wg.size() * sizeof(uint32_t));
-    }
+    for (uint32_t c = 0; c < nticks; c++)
+        for (size_t gi = 0; gi < groups[c].size(); gi++) {
+            job_off[c].push_back(bytes);
+            bytes = align(bytes + groups[c][gi].jobs.size() * sizeof(SpmmArgs));
+            wg_off[c].push_back(bytes);
+            bytes = align(bytes + groups[c][gi].wg.size() * sizeof(uint32_t));
+        }
     const size_t red_off = bytes;
     bytes += red.size() * sizeof(RedJob);
     RR_CHECK(bytes <= STAGE_BYTES, RR_ENOMEM, "tick descriptors exceed staging (%zu bytes)", bytes);
@@ -446,10 +455,12 @@ rr_status TickRunner::tick(std::vector<Request *> &reqs, uint32_t p) {
     for (int lb = 0; lb < 7; lb++)
         if (!gram_by_b[lb].empty())
             std::memcpy(h + gram_off[lb], gram_by_b[lb].data(), gram_by_b[lb].size() * sizeof(GramTile));
-    for (size_t gi = 0; gi < groups.size(); gi++) {
-        std::memcpy(h + job_off[gi], groups[gi].jobs.data(), groups[gi].jobs.size() * sizeof(SpmmArgs));
-        std::memcpy(h + wg_off[gi], groups[gi].wg.data(), groups[gi].wg.size() * sizeof(uint32_t));
-    }
+    for (uint32_t c = 0; c < nticks; c++)
+        for (size_t gi = 0; gi < groups[c].size(); gi++) {
+            const Group &G = groups[c][gi];
+            std::memcpy(h + job_off[c][gi], G.jobs.data(), G.jobs.size() * sizeof(SpmmArgs));
+            std::memcpy(h + wg_off[c][gi], G.wg.data(), G.wg.size() * sizeof(uint32_t));
+        }
     std::memcpy(h + red_off, red.data(), red.size() * sizeof(RedJob));
     RR_HIP(hipMemcpyAsync(desc_dev.ptr, h, bytes, hipMemcpyHostToDevice, st));
     RR_HIP(hipEventRecord(stage_ev[si], st));
@@ -470,19 +481,21 @@ rr_status TickRunner::tick(std::vector<Request *> &reqs, uint32_t p) {
         }
         RR_HIP(hipGetLastError());
     }
-    for (size_t gi = 0; gi < groups.size(); gi++) {
-        const Group &G = groups[gi];
-        launch_spmm(G.B, G.packed, G.su, (const SpmmArgs *)(dbase + job_off[gi]),
-                    (const uint32_t *)(dbase + wg_off[gi]), (uint32_t)G.jobs.size(), G.wg.back(), pbase, st);
-        RR_HIP(hipGetLastError());
-    }
+    for (uint32_t c = 0; c < nticks; c++)
+        for (size_t gi = 0; gi < groups[c].size(); gi++) {
+            const Group &G = groups[c][gi];
+            launch_spmm(G.B, G.packed, G.su, (const SpmmArgs *)(dbase + job_off[c][gi]),
+                        (const uint32_t *)(dbase + wg_off[c][gi]), (uint32_t)G.jobs.size(), G.wg.back(), pbase, st);
+            RR_HIP(hipGetLastError());
+        }
     uint32_t maxP = 0;
     for (const RedJob &J : red) maxP = std::max(maxP, J.P);
+    RR_CHECK(red.size() < 65536 * 32768ull, RR_EINVAL, "too many reductions");
     hipLaunchKernelGGL(k_reduce_jobs, dim3((uint32_t)red.size(), (maxP + 63) / 64), dim3(RJ_WAVES * 64), 0, st,
                        (const RedJob *)(dbase + red_off), p);
     RR_HIP(hipGetLastError());
-    for (auto *R : reqs) R->t++;
-    ticks++;
+    for (auto *R : reqs) R->t += nticks;
+    ticks += nticks;
     return RR_OK;
 }
 
@@ -554,10 +567,21 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
     std::vector<Request *> active;
     for (;;) {
         active.clear();
+        bool first = false;
+        uint64_t left = ~0ull;
+        size_t words = 0;  // partial words of one tick
         for (auto &r : reqs)
-            if (r->stage == STAGE_KRYLOV && r->t < r->L) active.push_back(r.get());
+            if (r->stage == STAGE_KRYLOV && r->t < r->L) {
+                active.push_back(r.get());
+                first |= r->t == 0;
+                left = std::min<uint64_t>(left, r->L - r->t);
+                words += (size_t)r->nwg * r->P;
+            }
         if (active.empty()) break;
-        if ((s = T.tick(active, p)) != RR_OK) return s;
+        // up to BATCH_TICKS ticks per chunk, none past a sequence's end, partials within 64 M words
+        uint64_t n = first ? 1 : std::min<uint64_t>(BATCH_TICKS, left);
+        n = std::max<uint64_t>(1, std::min<uint64_t>(n, (64ull << 20) / std::max<size_t>(words, 1)));
+        if ((s = T.tick(active, p, (uint32_t)n)) != RR_OK) return s;
     }
     RR_HIP(hipStreamSynchronize(st));
     auto t2 = std::chrono::steady_clock::now();

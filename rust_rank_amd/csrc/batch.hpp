@@ -32,6 +32,9 @@ uint64_t request_bytes(uint64_t m, uint64_t n, uint64_t nnz, uint32_t b, uint64_
 // when its operands are sliced: small requests get few wavefronts, large ones
 // up to a full device.
 constexpr uint32_t BATCH_STEPS_PER_WAVE = 32;
+// Ticks per chunk in rr_batch_rank: their passes are launched back to back
+// and their partials reduced by one launch.
+constexpr uint32_t BATCH_TICKS = 16;
 
 struct Request {
     uint32_t p = 0, b = 0, B = 0, P = 0, su = 4, nwg = 0;
@@ -72,7 +75,7 @@ struct TickRunner {
     uint64_t ticks = 0;
 
     rr_status init(hipStream_t st, int cus);
-    rr_status tick(std::vector<Request *> &reqs, uint32_t p);
+    rr_status tick(std::vector<Request *> &reqs, uint32_t p, uint32_t nticks = 1);
     ~TickRunner();
 };
 
