@@ -35,6 +35,8 @@ constexpr uint32_t BATCH_STEPS_PER_WAVE = 32;
 // Ticks per chunk in rr_batch_rank: their passes are launched back to back
 // and their partials reduced by one launch.
 constexpr uint32_t BATCH_TICKS = 16;
+// Ticks per loop of the server (between two PM-basis rounds).
+constexpr uint32_t SERVER_TICKS = 4;
 
 struct Request {
     uint32_t p = 0, b = 0, B = 0, P = 0, su = 4, nwg = 0;

@@ -119,11 +119,20 @@ void rr_server::run() {
             active.push_back(std::move(R));
         }
         max_active = std::max<uint64_t>(max_active, active.size());
-        // one Krylov entry for every request still producing its sequence
+        // the next Krylov entries (up to SERVER_TICKS, none past a sequence's
+        // end; one while a request is at its first) of every request still
+        // producing its sequence
         kry.clear();
+        bool first = false;
+        uint64_t left = ~0ull;
         for (auto &R : active)
-            if (R->stage == STAGE_KRYLOV && R->t < R->L) kry.push_back(R.get());
-        if (!kry.empty() && ticks.tick(kry, p) != RR_OK)
+            if (R->stage == STAGE_KRYLOV && R->t < R->L) {
+                kry.push_back(R.get());
+                first |= R->t == 0;
+                left = std::min<uint64_t>(left, R->L - R->t);
+            }
+        const uint32_t nt = first ? 1u : (uint32_t)std::min<uint64_t>(SERVER_TICKS, left);
+        if (!kry.empty() && ticks.tick(kry, p, nt) != RR_OK)
             for (auto *R : kry) R->stage = STAGE_FAILED;
         // sequences completed: build G on the Krylov stream, hand over to the PM stream
         bool handed = false;
