@@ -156,7 +156,9 @@ uint64_t rr_pm_order(uint64_t seq_len, int reference_policy);
 rr_status rr_rank(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *row_precond, const uint32_t *col_precond,
                   const uint32_t *V, uint64_t seq_len, uint64_t order, int64_t *rank_out, double *ms_out);
 
-/* Timing of one rr_batch_rank call. */
+/* Timing of one rr_batch_rank call: ms_krylov runs until every sequence is
+ * complete (PM-basis rounds of the matrices already done overlap it), ms_pm is
+ * the PM-basis work left after that. */
 typedef struct {
     double ms_setup, ms_krylov, ms_pm, ms_total;
     uint64_t ticks, pm_rounds, pm_launches;

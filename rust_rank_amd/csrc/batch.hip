@@ -262,7 +262,7 @@ rr_status Request::init(const rr_csr *A, uint32_t p_, uint32_t b_, const uint32_
     return RR_OK;
 }
 
-rr_status Request::start_pm(hipStream_t st) {
+rr_status Request::start_pm(hipStream_t st, bool release) {
     rr_status s;
     if ((s = G.alloc((size_t)2 * b * b * (d + 1) * 4)) != RR_OK) return s;
     if ((s = M.alloc((size_t)4 * b * b * (d + 1) * 4)) != RR_OK) return s;
@@ -275,6 +275,7 @@ rr_status Request::start_pm(hipStream_t st) {
     pm.M = M.as<uint32_t>();
     pm.delta = delta.as<int64_t>();
     stage = STAGE_PM;
+    if (!release) return RR_OK;
     // the operands are no longer needed (the stream orders this after their last pass)
     RR_HIP(hipStreamSynchronize(st));
     sa.~SliceMat();
@@ -499,6 +500,39 @@ rr_status TickRunner::tick(std::vector<Request *> &reqs, uint32_t p, uint32_t nt
     return RR_OK;
 }
 
+// Events marking the last `cap` items issued to one stream, oldest first.
+struct EventRing {
+    std::vector<hipEvent_t> ev;
+    size_t head = 0, used = 0;
+    rr_status init(size_t cap) {
+        ev.assign(cap, nullptr);
+        for (auto &e : ev) RR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return RR_OK;
+    }
+    ~EventRing() {
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    bool room() {
+        retire();
+        return used < ev.size();
+    }
+    rr_status push(hipStream_t st) {
+        RR_HIP(hipEventRecord(ev[(head + used) % ev.size()], st));
+        used++;
+        return RR_OK;
+    }
+    bool retire() {  // drops the items the device has finished; true if any
+        bool any = false;
+        while (used && hipEventQuery(ev[head]) == hipSuccess) {
+            head = (head + 1) % ev.size();
+            used--;
+            any = true;
+        }
+        return any;
+    }
+};
+
 }  // namespace rr
 
 using namespace rr;
@@ -561,11 +595,40 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
     }
     RR_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::steady_clock::now();
-    // Krylov: ticks until every sequence is complete
+    // Krylov chunks on st and PM-basis rounds on st_pm, issued from this one
+    // thread and run side by side on the device: a matrix whose sequence is
+    // complete joins the PM scheduler while the longer sequences still tick.
+    // Each stream is kept a few items ahead of the device (events), so
+    // neither waits for the host.
+    hipStream_t st_pm;
+    RR_HIP(hipStreamCreateWithFlags(&st_pm, hipStreamNonBlocking));
+    SG guard_pm{st_pm};
     TickRunner T;
     if ((s = T.init(st, cus)) != RR_OK) return s;
+    PmScheduler S;
+    {
+        uint32_t max_logn = 2;
+        for (auto &r : reqs) {
+            uint32_t l = 1;
+            while ((1ull << l) < r->d + 1) l++;
+            max_logn = std::max(max_logn, l);
+        }
+        if ((s = S.init(p, st_pm, max_logn, 128, 64)) != RR_OK) return s;
+    }
+    EventRing kq, pq;
+    if ((s = kq.init(2)) != RR_OK || (s = pq.init(BATCH_PM_AHEAD)) != RR_OK) return s;
+    hipEvent_t handoff;
+    RR_HIP(hipEventCreateWithFlags(&handoff, hipEventDisableTiming));
+    struct EG {
+        hipEvent_t e;
+        ~EG() { (void)hipEventDestroy(e); }
+    } guard_ev{handoff};
     std::vector<Request *> active;
+    std::vector<PmJob *> jobs;
+    bool krylov_done = false, pm_idle = false;
+    auto t2 = t1;
     for (;;) {
+        bool issued = false;
         active.clear();
         bool first = false;
         uint64_t left = ~0ull;
@@ -577,24 +640,59 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
                 left = std::min<uint64_t>(left, r->L - r->t);
                 words += (size_t)r->nwg * r->P;
             }
-        if (active.empty()) break;
-        // up to BATCH_TICKS ticks per chunk, none past a sequence's end, partials within 64 M words
-        uint64_t n = first ? 1 : std::min<uint64_t>(BATCH_TICKS, left);
-        n = std::max<uint64_t>(1, std::min<uint64_t>(n, (64ull << 20) / std::max<size_t>(words, 1)));
-        if ((s = T.tick(active, p, (uint32_t)n)) != RR_OK) return s;
+        if (!active.empty() && kq.room()) {
+            // up to BATCH_TICKS ticks per chunk, none past a sequence's end, partials within 64 M words
+            uint64_t n = first ? 1 : std::min<uint64_t>(BATCH_TICKS, left);
+            n = std::max<uint64_t>(1, std::min<uint64_t>(n, (64ull << 20) / std::max<size_t>(words, 1)));
+            if ((s = T.tick(active, p, (uint32_t)n)) != RR_OK) return s;
+            if ((s = kq.push(st)) != RR_OK) return s;
+            issued = true;
+        }
+        // complete sequences: G is built on st, the PM stream waits for it
+        const size_t njobs = jobs.size();
+        for (auto &r : reqs)
+            if (r->stage == STAGE_KRYLOV && r->t >= r->L) {
+                if ((s = r->start_pm(st, false)) != RR_OK) return s;
+                jobs.push_back(&r->pm);
+            }
+        const bool handed = jobs.size() > njobs;
+        if (handed) {
+            RR_HIP(hipEventRecord(handoff, st));
+            RR_HIP(hipStreamWaitEvent(st_pm, handoff, 0));
+            for (size_t j = njobs; j < jobs.size(); j++)
+                if ((s = S.add(jobs[j])) != RR_OK) return s;
+            pm_idle = false;
+            issued = true;
+        }
+        if (!pm_idle && pq.room()) {
+            bool launched = false;
+            if ((s = S.round(jobs, &launched)) != RR_OK) return s;
+            if (launched) {
+                if ((s = pq.push(st_pm)) != RR_OK) return s;
+                issued = true;
+            } else {
+                pm_idle = true;  // every joined job is done
+            }
+        }
+        if (!krylov_done && active.empty() && !handed) {
+            RR_HIP(hipStreamSynchronize(st));
+            krylov_done = true;
+            t2 = std::chrono::steady_clock::now();
+        }
+        if (krylov_done && pm_idle) break;
+        if (!issued) {
+            // both streams are far enough ahead: wait for either to retire an item
+            for (;;) {
+                bool k = kq.retire(), q = pq.retire();
+                if (k || q) break;
+                std::this_thread::yield();
+            }
+        }
     }
-    RR_HIP(hipStreamSynchronize(st));
-    auto t2 = std::chrono::steady_clock::now();
-    // PM-basis of every request, batched rounds
-    std::vector<PmJob *> jobs;
-    for (auto &r : reqs) {
-        if ((s = r->start_pm(st)) != RR_OK) return s;
-        jobs.push_back(&r->pm);
-    }
-    uint64_t rounds = 0, launches = 0;
-    if ((s = pm_basis_batch(p, jobs, st, &rounds, &launches)) != RR_OK) return s;
+    RR_HIP(hipStreamSynchronize(st_pm));
+    uint64_t rounds = S.rounds, launches = S.launches;
     for (uint32_t i = 0; i < count; i++) {
-        if ((s = reqs[i]->finish_rank(st)) != RR_OK) return s;
+        if ((s = reqs[i]->finish_rank(st_pm)) != RR_OK) return s;
         rank_out[i] = reqs[i]->rank;
     }
     auto t3 = std::chrono::steady_clock::now();

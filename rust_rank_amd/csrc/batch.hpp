@@ -35,6 +35,8 @@ constexpr uint32_t BATCH_STEPS_PER_WAVE = 32;
 // Ticks per chunk in rr_batch_rank: their passes are launched back to back
 // and their partials reduced by one launch.
 constexpr uint32_t BATCH_TICKS = 16;
+// PM-basis rounds rr_batch_rank keeps queued ahead of the device.
+constexpr uint32_t BATCH_PM_AHEAD = 8;
 // Ticks per loop of the server (between two PM-basis rounds).
 constexpr uint32_t SERVER_TICKS = 4;
 
@@ -54,7 +56,9 @@ struct Request {
 
     rr_status init(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *dr, const uint32_t *dc,
                    const uint32_t *V, uint64_t L, uint64_t d, uint32_t tile_rows, hipStream_t st);
-    rr_status start_pm(hipStream_t st);     // sequence complete: build G, join the PM scheduler
+    // sequence complete: build G for the PM scheduler; with release, wait for
+    // the stream and free the sliced operands
+    rr_status start_pm(hipStream_t st, bool release = true);
     rr_status finish_rank(hipStream_t st);  // PM done: rank formula from the extracted coefficients
     uint64_t device_bytes() const;          // resident footprint estimate (admission control)
 };

@@ -49,14 +49,6 @@ namespace rr {
 // Linv coefficients times the pivot rows, shifted by x when row i was a
 // pivot row.  This is the reference's chain of order-1 leaves, kept in one
 // launch.
-// Steps per leaf: the in-leaf basis update costs m^2 rank per coefficient,
-// so wider blocks take shorter leaves.
-__host__ __device__ constexpr uint32_t leaf_order(uint32_t m) { return m <= 16 ? 16 : m <= 32 ? 8 : m <= 64 ? 4 : 1; }
-// A PM-basis running alone is a chain of leaf launches, and shorter leaves
-// (less basis update per coefficient) finish it sooner; in batches and in
-// the server the leaves of many jobs share rounds, and longer leaves mean
-// fewer rounds.  The basis is the same either way.
-__host__ __device__ constexpr uint32_t leaf_order_alone(uint32_t m) { return m <= 64 ? leaf_order(m) / 2 : 1; }
 constexpr int LEAF_PACK = 64;
 struct LeafPack {
     LeafDesc d[LEAF_PACK];
