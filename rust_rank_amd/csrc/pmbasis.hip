@@ -785,10 +785,17 @@ static uint32_t ntt_logn_for(const MulDesc &D) {
     return logn;
 }
 
-static bool use_school(const MulDesc &D, uint32_t p) {
+// Direct (schoolbook) product or NTT product.  A direct product costs about
+// la lb multiply-adds per output entry against 2 max(la, lb) for the NTT's
+// pointwise part, but it is one launch instead of three: a round with few
+// products (a PM-basis running alone, latency-bound) takes the direct form
+// up to la lb = 256, a round with many (batched, throughput-bound) only up
+// to 16.
+constexpr size_t SCHOOL_MANY = 8;
+static bool use_school(const MulDesc &D, uint32_t p, size_t round_products = 1) {
     if (p != NTT_PRIME) return true;
     uint64_t cost = (uint64_t)std::min(D.la, D.hi) * std::min(D.lb, D.hi);
-    return cost <= 256;
+    return cost <= (round_products >= SCHOOL_MANY ? 16u : 256u);
 }
 
 rr_status PmScheduler::ensure_scratch(size_t words) {
@@ -865,11 +872,14 @@ rr_status PmScheduler::round(std::vector<PmJob *> &jobs, bool *launched) {
         }
         any_mul |= J->op_kind == 1;
     }
+    size_t nmul = 0;
+    if (any_mul)
+        for (auto *J : jobs) nmul += !J->done && J->has_op && J->op_kind == 1;
     for (auto *J : jobs) {
         if (J->done || !J->has_op || J->op_kind != (any_mul ? 1 : 0)) continue;
         J->has_op = false;
         if (J->op_kind == 0) leaves.push_back(J->op_leaf);
-        else if (use_school(J->op_mul, p)) school.push_back(J->op_mul);
+        else if (use_school(J->op_mul, p, nmul)) school.push_back(J->op_mul);
         else ntt.push_back({ntt_logn_for(J->op_mul), J->op_mul});
     }
     if (leaves.empty() && school.empty() && ntt.empty()) return RR_OK;

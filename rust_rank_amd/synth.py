@@ -133,7 +133,7 @@ def _fixed_k_csr(m: int, n: int, k: np.ndarray, cols_flat: np.ndarray, vals_flat
     np.cumsum(k.astype(np.uint64), out=rp[1:])
     # sort columns inside each row
     rows = np.repeat(np.arange(m, dtype=np.int64), k.astype(np.int64))
-    order = np.lexsort((cols_flat, rows))
+    order = np.argsort(rows * n + cols_flat.astype(np.int64), kind="stable")  # by row, then column
     return Csr(m, n, rp, cols_flat[order].astype(np.uint32), vals_flat[order].astype(np.uint32))
 
 
@@ -187,21 +187,17 @@ def plant_dependent_columns(rng: Rng, A: Csr, k: int, p: int, small: bool) -> Cs
     keep = ~is_dep[col]
     rows_k, col_k, val_k = rows[keep], col[keep], val[keep]
     # for every (dependent column t, term s) collect the entries of column src[t, s]
-    col_of = {}
-    new_r, new_c, new_v = [rows_k], [col_k], [val_k]
+    # (t, s) in row-major order, each term's entries in the column's row order
     order = np.argsort(col_k, kind="stable")
-    cs = col_k[order]
-    starts = np.searchsorted(cs, np.arange(n + 1))
-    for t in range(k):
-        for s in range(terms):
-            c = src[t, s]
-            idx = order[starts[c]:starts[c + 1]]
-            new_r.append(rows_k[idx])
-            new_c.append(np.full(idx.size, dep[t], dtype=np.int64))
-            new_v.append(val_k[idx] * coef[t, s] % np.uint64(p))
-    r = np.concatenate(new_r)
-    c = np.concatenate(new_c)
-    v = np.concatenate(new_v)
+    starts = np.searchsorted(col_k[order], np.arange(n + 1))
+    cflat = src.reshape(-1)
+    cnt = starts[cflat + 1] - starts[cflat]
+    first = np.repeat(starts[cflat] - np.concatenate([[0], np.cumsum(cnt)[:-1]]), cnt)
+    idx = order[first + np.arange(int(cnt.sum()))]
+    pair = np.repeat(np.arange(cflat.size), cnt)
+    r = np.concatenate([rows_k, rows_k[idx]])
+    c = np.concatenate([col_k, dep[pair // terms].astype(np.int64)])
+    v = np.concatenate([val_k, val_k[idx] * coef.reshape(-1)[pair] % np.uint64(p)])
     # merge duplicates (same row, same dependent column) mod p, drop exact zeros
     key = r * n + c
     o = np.argsort(key, kind="stable")
@@ -213,8 +209,7 @@ def plant_dependent_columns(rng: Rng, A: Csr, k: int, p: int, small: bool) -> Cs
     rr = uk // n
     cc = uk % n
     rp = np.zeros(m + 1, dtype=np.uint64)
-    np.add.at(rp, rr + 1, 1)
-    rp = np.cumsum(rp).astype(np.uint64)
+    np.cumsum(np.bincount(rr.astype(np.int64), minlength=m), out=rp[1:])
     return Csr(m, n, rp, cc.astype(np.uint32), sums.astype(np.uint32))
 
 
