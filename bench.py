@@ -66,7 +66,7 @@ def batched_and_served(eng, synth, p, nmat, rate_frac, seed=4):
     computed from the measured plain service times on the same arrival times."""
     import threading
     t0 = time.perf_counter()
-    mats = [synth.batch_matrix(seed, i) for i in range(nmat)]
+    mats = synth.batch_matrices(seed, nmat)
     bs, pred, probe = eng.autotune_b(mats, p)
     pre = [synth.preconditioners(5000 + i, A.m, A.n, int(b), p) for i, (A, b) in enumerate(zip(mats, bs))]
     log(f"batched: {nmat} matrices, rows {min(A.m for A in mats)}..{max(A.m for A in mats)}, "
@@ -151,7 +151,7 @@ def main():
     ap.add_argument("--b", type=int, default=0, help="block width (default: 32, or 64 for --workload large)")
     ap.add_argument("--workload", choices=["graph_complex", "large"], default="graph_complex",
                     help="large: the 5000000 x 4500000, 20 nnz/row single matrix (Krylov only, bounded steps)")
-    ap.add_argument("--batched", type=int, default=64, help="matrices in the batched/served comparison (0: skip)")
+    ap.add_argument("--batched", type=int, default=512, help="matrices in the batched/served comparison (0: skip)")
     ap.add_argument("--rate-frac", type=float, default=0.7, help="served offered load / batched capacity")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
                     help="only run the plain copy kernel on MB megabytes (counter calibration) and exit")

@@ -712,16 +712,23 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// Autotuner: block width b (and with it rows per wavefront, 64/b) per matrix.
-// Measured once per process on the device:
-//   c_B   Krylov pass time per nonzero for lane width B (tile kernel on a
-//         probe matrix with +-1/+-2 entries, 16 ticks timed with events),
-//   pm_b  PM-basis time per order d = 255 for block width b.
+// Autotuner: block width b (and with it rows per wavefront, 64/b) per matrix,
+// for the batched and served modes.  There many matrices share the device,
+// so what a width costs is the device time its work takes among the others
+// (throughput), not its latency alone.  Measured once per process on the
+// device, with TUNE_COPIES copies of each probe running together:
+//   c_b   Krylov time per nonzero and tick for width b: TUNE_COPIES copies of
+//         a probe matrix with +-1/+-2 entries ticking together, 16 ticks
+//         timed with events, divided by the copies' nonzeros;
+//   pm_b  PM-basis time per job at order 255 for width b: TUNE_COPIES jobs
+//         of random sequences in one batched scheduler run, divided by the
+//         jobs.
 // Predicted device time of matrix i with width b:
 //   L(b) * nnz_i * c_b + pm_b * d(b) / 255,
 //   L(b) = 2 min(m, n) / b + 8, d(b) = L(b) - 1.
-// The PM-basis term is linear in d: its time is dominated by the chain of
-// leaves (measured: d = 255, 2047, 4095 at b = 16 take 10.5, 85, 168 ms).
+// The PM-basis term is linear in d: its leaves, the bulk of its time, are
+// one per K coefficients.
+constexpr uint32_t TUNE_COPIES = 8;
 
 namespace rr {
 
@@ -763,36 +770,54 @@ static rr_status run_probe(uint32_t p, int cus, hipStream_t st) {
         uint32_t b = TUNE_B[i];
         std::vector<uint32_t> Vb(n * b);
         for (uint64_t r = 0; r < n; r++) std::memcpy(&Vb[r * b], &V[r * 32], b * 4);
-        Request R;
-        if ((s = R.init(&A, p, b, dr.data(), dc.data(), Vb.data(), 40, 39, 2048, st)) != RR_OK) return s;
+        std::vector<std::unique_ptr<Request>> R(TUNE_COPIES);
+        std::vector<Request *> all;
+        for (auto &r : R) {
+            r.reset(new Request());
+            if ((s = r->init(&A, p, b, dr.data(), dc.data(), Vb.data(), 40, 39, 2048, st)) != RR_OK) return s;
+            all.push_back(r.get());
+        }
         TickRunner T;
         if ((s = T.init(st, cus)) != RR_OK) return s;
-        std::vector<Request *> one{&R};
-        for (int w = 0; w < 4; w++)
-            if ((s = T.tick(one, p)) != RR_OK) return s;
+        if ((s = T.tick(all, p)) != RR_OK) return s;  // t = 0 alone
+        if ((s = T.tick(all, p, 3)) != RR_OK) return s;
         RR_HIP(hipEventRecord(e0, st));
-        for (int w = 0; w < 16; w++)
-            if ((s = T.tick(one, p)) != RR_OK) return s;
+        if ((s = T.tick(all, p, 16)) != RR_OK) return s;
         RR_HIP(hipEventRecord(e1, st));
         RR_HIP(hipEventSynchronize(e1));
         float ms = 0;
         RR_HIP(hipEventElapsedTime(&ms, e0, e1));
-        g_probe.c[i] = (ms / 16.0) / (double)(m * k);
-        // PM-basis probe, order 255, random sequence
+        g_probe.c[i] = (ms / 16.0) / ((double)TUNE_COPIES * (double)(m * k));
+        // PM-basis probe: TUNE_COPIES jobs of order 255, random sequences
         const uint64_t d = 255, P = b * (b + 1) / 2;
-        std::vector<uint32_t> seq((d + 1) * P);
+        std::vector<uint32_t> seq((d + 1) * P * TUNE_COPIES);
         for (auto &v : seq) v = (uint32_t)(rnd() % p);
-        DevBuf dseq, G, M, dl;
+        DevBuf dseq;
+        std::vector<DevBuf> G(TUNE_COPIES), M(TUNE_COPIES), dl(TUNE_COPIES);
+        std::vector<PmJob> J(TUNE_COPIES);
+        std::vector<PmJob *> jobs;
         if ((s = dseq.alloc(seq.size() * 4)) != RR_OK) return s;
-        if ((s = G.alloc((size_t)2 * b * b * (d + 1) * 4)) != RR_OK) return s;
-        if ((s = M.alloc((size_t)4 * b * b * (d + 1) * 4)) != RR_OK) return s;
-        std::vector<int64_t> delta(2 * b);
         RR_HIP(hipMemcpyAsync(dseq.ptr, seq.data(), seq.size() * 4, hipMemcpyHostToDevice, st));
-        if ((s = build_G_device(dseq.as<uint32_t>(), b, d + 1, G.as<uint32_t>(), st)) != RR_OK) return s;
+        for (uint32_t c = 0; c < TUNE_COPIES; c++) {
+            if ((s = G[c].alloc((size_t)2 * b * b * (d + 1) * 4)) != RR_OK) return s;
+            if ((s = M[c].alloc((size_t)4 * b * b * (d + 1) * 4)) != RR_OK) return s;
+            if ((s = dl[c].alloc((size_t)2 * b * 8)) != RR_OK) return s;
+            if ((s = build_G_device(dseq.as<uint32_t>() + (size_t)c * (d + 1) * P, b, d + 1, G[c].as<uint32_t>(),
+                                    st)) != RR_OK)
+                return s;
+            J[c].b = b;
+            J[c].d = d;
+            J[c].G = G[c].as<uint32_t>();
+            J[c].M = M[c].as<uint32_t>();
+            J[c].delta = dl[c].as<int64_t>();
+            jobs.push_back(&J[c]);
+        }
         RR_HIP(hipStreamSynchronize(st));
         auto t0 = std::chrono::steady_clock::now();
-        if ((s = pm_basis_device(p, b, d, G.as<uint32_t>(), M.as<uint32_t>(), delta.data(), st)) != RR_OK) return s;
-        g_probe.pm[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if ((s = pm_basis_batch(p, jobs, st, nullptr, nullptr)) != RR_OK) return s;
+        RR_HIP(hipStreamSynchronize(st));
+        g_probe.pm[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
+                        TUNE_COPIES;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -259,60 +259,76 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_cols_inv(NttInvPack pk, uin
 }
 
 // Pointwise products: C[i][j][l] = scale * sum_r A[i][r][l] B[r][j][l] over
-// spectra of length N.  Thread (l, i) keeps JT entries of row i at point l as
-// exact fp64 sums (A split into 16-bit halves, folded every 64 terms).
-template <int JT>
-__global__ __launch_bounds__(256) void k_pointwise(PwPack pk, uint32_t ztiles, uint32_t N, uint32_t p, double pd,
-                                                   double inv) {
-    const PwDesc &Dd = pk.d[blockIdx.z / ztiles];
-    const uint32_t *A = Dd.A, *B = Dd.B;
-    uint32_t *C = Dd.C;
-    const uint32_t rows = Dd.rows, inner = Dd.inner, cols = Dd.cols, scale = Dd.scale;
-    const uint32_t l = blockIdx.x * 64 + (threadIdx.x & 63);
-    const uint32_t i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const uint32_t j0 = (blockIdx.z % ztiles) * JT;
-    if (l >= N || i >= rows || j0 >= cols) return;
-    double hi[JT], lo[JT];
+// spectra of length N (a power of two).  A thread owns a TI x TJ tile of C at
+// one point l, so each step over r loads TI + TJ words for TI TJ products;
+// threads run over (point, row tile, column tile, descriptor), points
+// fastest, so short spectra still fill every lane and a wavefront's loads are
+// runs of consecutive words.  Sums are exact in fp64: A is split into 16-bit
+// halves (products < 2^47), folded every 64 terms.  Rows and columns past a
+// descriptor's matrix re-read its last row / column and are not stored.
+template <int TI, int TJ>
+__global__ __launch_bounds__(256) void k_pointwise(PwPack pk, uint32_t ndesc, uint32_t logn, uint32_t rt,
+                                                   uint32_t ct, uint32_t p, double pd, double inv) {
+    const uint32_t N = 1u << logn;
+    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint32_t l = (uint32_t)(t & (N - 1));
+    uint64_t q = t >> logn;
+    const uint32_t it = (uint32_t)(q % rt);
+    q /= rt;
+    const uint32_t jt = (uint32_t)(q % ct);
+    q /= ct;
+    if (q >= ndesc) return;
+    const PwDesc &D = pk.d[q];
+    const uint32_t rows = D.rows, inner = D.inner, cols = D.cols, i0 = it * TI, j0 = jt * TJ;
+    if (i0 >= rows || j0 >= cols) return;
+    const GLOBAL_AS uint32_t *ap[TI];
+    const GLOBAL_AS uint32_t *bp[TJ];
 #pragma unroll
-    for (int j = 0; j < JT; j++) hi[j] = lo[j] = 0.0;
-    // columns past the matrix re-read its last column (every load is
-    // unconditional, so a whole row step is in flight at once) and are dropped
-    const uint32_t jn = min((uint32_t)JT, cols - j0);
-    uint32_t boff[JT];
+    for (int a = 0; a < TI; a++) ap[a] = as_global(D.A) + (size_t)min(i0 + a, rows - 1) * inner * N + l;
 #pragma unroll
-    for (int j = 0; j < JT; j++) boff[j] = (j0 + min((uint32_t)j, jn - 1)) * N + l;
-    const uint32_t *ap = A + (size_t)i * inner * N + l;
+    for (int c = 0; c < TJ; c++) bp[c] = as_global(D.B) + (size_t)min(j0 + c, cols - 1) * N + l;
     const size_t bstep = (size_t)cols * N;
+    double hi[TI][TJ], lo[TI][TJ];
+#pragma unroll
+    for (int a = 0; a < TI; a++)
+#pragma unroll
+        for (int c = 0; c < TJ; c++) hi[a][c] = lo[a][c] = 0.0;
     for (uint32_t r0 = 0; r0 < inner; r0 += 64) {
         const uint32_t re = min(inner, r0 + 64);
-#pragma unroll 2
         for (uint32_t r = r0; r < re; r++) {
-            const uint32_t a = ap[(size_t)r * N];
-            const uint32_t *bp = B + r * bstep;
-            uint32_t bv[JT];
+            uint32_t av[TI], bv[TJ];
 #pragma unroll
-            for (int j = 0; j < JT; j++) bv[j] = bp[boff[j]];
-            const double ah = (double)(a >> 16), al = (double)(a & 0xFFFFu);
+            for (int a = 0; a < TI; a++) av[a] = ap[a][(size_t)r * N];
 #pragma unroll
-            for (int j = 0; j < JT; j++) {
-                const double bd = (double)bv[j];
-                hi[j] = fma(ah, bd, hi[j]);
-                lo[j] = fma(al, bd, lo[j]);
+            for (int c = 0; c < TJ; c++) bv[c] = bp[c][r * bstep];
+#pragma unroll
+            for (int a = 0; a < TI; a++) {
+                const double ah = (double)(av[a] >> 16), al = (double)(av[a] & 0xFFFFu);
+#pragma unroll
+                for (int c = 0; c < TJ; c++) {
+                    const double bd = (double)bv[c];
+                    hi[a][c] = fma(ah, bd, hi[a][c]);
+                    lo[a][c] = fma(al, bd, lo[a][c]);
+                }
             }
         }
 #pragma unroll
-        for (int j = 0; j < JT; j++) {
-            hi[j] = fold(hi[j], pd, inv);
-            lo[j] = fold(lo[j], pd, inv);
-        }
-    }
+        for (int a = 0; a < TI; a++)
 #pragma unroll
-    for (int j = 0; j < JT; j++) {
-        if ((uint32_t)j < jn) {
-            uint32_t v = (uint32_t)fold(fma(hi[j], 65536.0, lo[j]), pd, inv);
-            C[((size_t)i * cols + j0 + j) * N + l] = mulmod(v, scale, p, inv);
-        }
+            for (int c = 0; c < TJ; c++) {
+                hi[a][c] = fold(hi[a][c], pd, inv);
+                lo[a][c] = fold(lo[a][c], pd, inv);
+            }
     }
+    GLOBAL_AS uint32_t *C = as_global(D.C);
+#pragma unroll
+    for (int a = 0; a < TI; a++)
+#pragma unroll
+        for (int c = 0; c < TJ; c++)
+            if (i0 + a < rows && j0 + c < cols) {
+                const uint32_t v = (uint32_t)fold(fma(hi[a][c], 65536.0, lo[a][c]), pd, inv);
+                C[((size_t)(i0 + a) * cols + j0 + c) * N + l] = mulmod(v, D.scale, p, inv);
+            }
 }
 
 // ---------------------------------------------------------------------------
@@ -421,17 +437,21 @@ rr_status ntt_inverse_pack(const NttTables &T, const NttInvPack &pk, uint32_t nd
 
 rr_status pointwise_pack(const PwPack &pk, uint32_t ndesc, uint32_t N, hipStream_t st) {
     RR_CHECK(ndesc >= 1 && ndesc <= NTT_PACK, RR_EINVAL, "bad pack size");
+    RR_CHECK(N >= 1 && (N & (N - 1)) == 0, RR_EINVAL, "transform length %u is not a power of two", N);
     const uint32_t p = NTT_PRIME;
     ModP mp = ModP::make(p);
-    constexpr int JT = 16;
-    uint32_t rows = 0, cols = 0;
+    constexpr int TI = 4, TJ = 4;
+    uint32_t rows = 0, cols = 0, logn = 0;
     for (uint32_t i = 0; i < ndesc; i++) {
         rows = std::max(rows, pk.d[i].rows);
         cols = std::max(cols, pk.d[i].cols);
     }
-    const uint32_t zt = (cols + JT - 1) / JT;
-    dim3 grid((N + 63) / 64, (rows + 3) / 4, zt * ndesc);
-    hipLaunchKernelGGL(k_pointwise<JT>, grid, dim3(256), 0, st, pk, zt, N, p, mp.pd, mp.inv);
+    while ((1u << logn) < N) logn++;
+    const uint32_t rt = (rows + TI - 1) / TI, ct = (cols + TJ - 1) / TJ;
+    const uint64_t threads = ((uint64_t)ndesc * rt * ct) << logn;
+    RR_CHECK((threads + 255) / 256 < (1ull << 31), RR_EINVAL, "pointwise product too large");
+    hipLaunchKernelGGL((k_pointwise<TI, TJ>), dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, st, pk, ndesc,
+                       logn, rt, ct, p, mp.pd, mp.inv);
     RR_HIP(hipGetLastError());
     return RR_OK;
 }
