@@ -53,6 +53,16 @@ const char *rr_last_error(void);
 int rr_device_count(void);
 
 /*
+ * Device memory (no counterpart in the reference): the library keeps the
+ * device buffers its calls free in a per-device cache and hands them out
+ * again, so repeated and batched calls do not pay hipMalloc / hipFree (a
+ * block is reused only after the device has been synchronised).  This waits
+ * for the current device and returns its cached blocks to the system; the
+ * cache is also released by itself when an allocation fails.
+ */
+rr_status rr_release_device_memory(void);
+
+/*
  * Replaces VectorStream::new (src/vectorstream.rs:46,131,225) together with
  * the preconditioning of src/main.rs:520-526: uploads A, builds A^T on the
  * host, and starts the stream at curv = x_k.
@@ -156,9 +166,10 @@ uint64_t rr_pm_order(uint64_t seq_len, int reference_policy);
 rr_status rr_rank(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *row_precond, const uint32_t *col_precond,
                   const uint32_t *V, uint64_t seq_len, uint64_t order, int64_t *rank_out, double *ms_out);
 
-/* Timing of one rr_batch_rank call: ms_krylov runs until every sequence is
- * complete (PM-basis rounds of the matrices already done overlap it), ms_pm is
- * the PM-basis work left after that. */
+/* Timing of one rr_batch_rank call: ms_setup until every matrix is set up
+ * on the host (the device already works on those set up), ms_krylov from
+ * then until every sequence is complete (PM-basis rounds of the matrices
+ * already done overlap it), ms_pm the PM-basis work left after that. */
 typedef struct {
     double ms_setup, ms_krylov, ms_pm, ms_total;
     uint64_t ticks, pm_rounds, pm_launches;

@@ -556,45 +556,88 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
     } guard{st};
     std::vector<std::unique_ptr<Request>> reqs(count);
     rr_status s;
-    // Host set-up (validation, transpose, slicing, uploads) of the matrices
-    // runs on several threads; each request's copies and kernels keep their
-    // order in the one stream.
-    {
-        std::vector<rr_status> st_i(count, RR_OK);
-        std::vector<std::string> msg(count);
+    auto seq_len_of = [&](uint32_t i) -> uint64_t {
+        return seq_len && seq_len[i] ? seq_len[i] : b[i] ? 2 * std::min(mats[i].m, mats[i].n) / b[i] + 8 : 0;
+    };
+    // Host set-up (validation, transpose, slicing, uploads) runs on worker
+    // threads, each with its own stream, while this thread already ticks the
+    // matrices that are ready: the largest work (sequence length x nonzeros)
+    // is set up first, since its sequence is the longest chain.  A request
+    // joins the tick loop once its stream's event says its uploads are done.
+    struct Setup {
+        std::vector<rr_status> st;
+        std::vector<std::string> msg;
+        std::vector<hipEvent_t> ev;
+        std::unique_ptr<std::atomic<int>[]> ready;  // 0 pending, 1 set up, 2 failed
+        std::vector<uint32_t> order;
         std::atomic<uint32_t> next{0};
-        auto work = [&]() {
-            if (hipSetDevice(dev) != hipSuccess) return;
-            for (uint32_t i; (i = next++) < count;) {
-                try {
-                    reqs[i].reset(new Request());
-                    uint64_t L = seq_len && seq_len[i] ? seq_len[i]
-                                 : b[i] ? 2 * std::min(mats[i].m, mats[i].n) / b[i] + 8 : 0;
-                    uint64_t d = order && order[i] ? order[i] : L - 1;
-                    st_i[i] = reqs[i]->init(&mats[i], p, b[i], dr[i], dc[i], V[i], L, d, 2048, st);
-                    if (st_i[i] != RR_OK) msg[i] = rr_last_error();
-                } catch (const std::bad_alloc &) {
-                    st_i[i] = RR_ENOMEM;
-                    msg[i] = "host allocation failed";
-                }
-            }
-        };
-        const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
-        const uint32_t nth = std::min<uint32_t>(count, std::min<uint32_t>(hw, 16));
+        std::atomic<bool> stop{false};
+        std::vector<hipStream_t> ws;
         std::vector<std::thread> pool;
-        for (uint32_t t = 1; t < nth; t++) pool.emplace_back(work);
-        work();
-        for (auto &th : pool) th.join();
-        for (uint32_t i = 0; i < count; i++) {
-            RR_CHECK(reqs[i], RR_EDEVICE, "set-up thread could not select device %d", dev);
-            if (st_i[i] != RR_OK) {
-                set_error("matrix %u: %s", i, msg[i].c_str());
-                return st_i[i];
+        ~Setup() {
+            stop = true;
+            for (auto &t : pool) t.join();
+            for (auto w : ws) {
+                (void)hipStreamSynchronize(w);
+                (void)hipStreamDestroy(w);
             }
+            for (auto e : ev)
+                if (e) (void)hipEventDestroy(e);
         }
+    } su;
+    su.st.assign(count, RR_OK);
+    su.msg.resize(count);
+    su.ev.assign(count, nullptr);
+    su.ready.reset(new std::atomic<int>[count]);
+    su.order.resize(count);
+    for (uint32_t i = 0; i < count; i++) {
+        su.ready[i] = 0;
+        su.order[i] = i;
+        RR_HIP(hipEventCreateWithFlags(&su.ev[i], hipEventDisableTiming));
     }
-    RR_HIP(hipStreamSynchronize(st));
-    auto t1 = std::chrono::steady_clock::now();
+    std::vector<double> work(count);
+    for (uint32_t i = 0; i < count; i++) work[i] = (double)seq_len_of(i) * (double)mats[i].row_ptr[mats[i].m];
+    std::stable_sort(su.order.begin(), su.order.end(), [&](uint32_t x, uint32_t y) { return work[x] > work[y]; });
+    {
+        const uint32_t hw = std::max(2u, std::thread::hardware_concurrency());
+        const uint32_t nth = std::max(1u, std::min<uint32_t>(count, std::min<uint32_t>(hw, 16) - 1));
+        for (uint32_t w = 0; w < nth; w++) {
+            hipStream_t ws;
+            RR_HIP(hipStreamCreateWithFlags(&ws, hipStreamNonBlocking));
+            su.ws.push_back(ws);
+        }
+        for (uint32_t w = 0; w < nth; w++)
+            su.pool.emplace_back([&, w]() {
+                const bool dev_ok = hipSetDevice(dev) == hipSuccess;
+                for (uint32_t k; !su.stop && (k = su.next++) < count;) {
+                    const uint32_t i = su.order[k];
+                    rr_status e = RR_EDEVICE;
+                    try {
+                        if (dev_ok) {
+                            reqs[i].reset(new Request());
+                            const uint64_t L = seq_len_of(i);
+                            const uint64_t d = order && order[i] ? order[i] : L - 1;
+                            e = reqs[i]->init(&mats[i], p, b[i], dr[i], dc[i], V[i], L, d, 2048, su.ws[w]);
+                            if (e == RR_OK && hipEventRecord(su.ev[i], su.ws[w]) != hipSuccess) {
+                                e = RR_EDEVICE;
+                                set_error("event record failed");
+                            }
+                            if (e != RR_OK) su.msg[i] = rr_last_error();
+                        } else {
+                            su.msg[i] = "set-up thread could not select the device";
+                        }
+                    } catch (const std::bad_alloc &) {
+                        e = RR_ENOMEM;
+                        su.msg[i] = "host allocation failed";
+                    }
+                    su.st[i] = e;
+                    su.ready[i].store(e == RR_OK ? 1 : 2, std::memory_order_release);
+                }
+            });
+    }
+    std::vector<char> joined(count, 0);
+    uint32_t njoined = 0;
+    auto t1 = t0;
     // Krylov chunks on st and PM-basis rounds on st_pm, issued from this one
     // thread and run side by side on the device: a matrix whose sequence is
     // complete joins the PM scheduler while the longer sequences still tick.
@@ -608,9 +651,10 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
     PmScheduler S;
     {
         uint32_t max_logn = 2;
-        for (auto &r : reqs) {
+        for (uint32_t i = 0; i < count; i++) {
+            const uint64_t d = order && order[i] ? order[i] : seq_len_of(i) - 1;
             uint32_t l = 1;
-            while ((1ull << l) < r->d + 1) l++;
+            while (l < 63 && (1ull << l) < d + 1) l++;
             max_logn = std::max(max_logn, l);
         }
         if ((s = S.init(p, st_pm, max_logn, 128, 64)) != RR_OK) return s;
@@ -629,29 +673,54 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
     auto t2 = t1;
     for (;;) {
         bool issued = false;
+        // requests whose set-up is done join the tick loop
+        if (njoined < count) {
+            for (uint32_t i = 0; i < count; i++) {
+                if (joined[i]) continue;
+                const int rd = su.ready[i].load(std::memory_order_acquire);
+                if (rd == 2) {
+                    set_error("matrix %u: %s", i, su.msg[i].c_str());
+                    return su.st[i];
+                }
+                if (rd == 1) {
+                    RR_HIP(hipStreamWaitEvent(st, su.ev[i], 0));
+                    joined[i] = 1;
+                    njoined++;
+                }
+            }
+            if (njoined == count) t1 = std::chrono::steady_clock::now();
+        }
+        std::vector<Request *> fresh;
         active.clear();
-        bool first = false;
         uint64_t left = ~0ull;
         size_t words = 0;  // partial words of one tick
-        for (auto &r : reqs)
-            if (r->stage == STAGE_KRYLOV && r->t < r->L) {
-                active.push_back(r.get());
-                first |= r->t == 0;
-                left = std::min<uint64_t>(left, r->L - r->t);
-                words += (size_t)r->nwg * r->P;
+        for (uint32_t i = 0; i < count; i++) {
+            Request *r = reqs[i].get();
+            if (!joined[i] || r->stage != STAGE_KRYLOV || r->t >= r->L) continue;
+            if (r->t == 0) {
+                fresh.push_back(r);
+                continue;
             }
-        if (!active.empty() && kq.room()) {
-            // up to BATCH_TICKS ticks per chunk, none past a sequence's end, partials within 64 M words
-            uint64_t n = first ? 1 : std::min<uint64_t>(BATCH_TICKS, left);
-            n = std::max<uint64_t>(1, std::min<uint64_t>(n, (64ull << 20) / std::max<size_t>(words, 1)));
-            if ((s = T.tick(active, p, (uint32_t)n)) != RR_OK) return s;
+            active.push_back(r);
+            left = std::min<uint64_t>(left, r->L - r->t);
+            words += (size_t)r->nwg * r->P;
+        }
+        if ((!fresh.empty() || !active.empty()) && kq.room()) {
+            // requests at t = 0 take their projection tick alone; the others up to
+            // BATCH_TICKS ticks per chunk, none past a sequence's end, partials within 64 M words
+            if (!fresh.empty() && (s = T.tick(fresh, p, 1)) != RR_OK) return s;
+            if (!active.empty()) {
+                uint64_t n = std::min<uint64_t>(BATCH_TICKS, left);
+                n = std::max<uint64_t>(1, std::min<uint64_t>(n, (64ull << 20) / std::max<size_t>(words, 1)));
+                if ((s = T.tick(active, p, (uint32_t)n)) != RR_OK) return s;
+            }
             if ((s = kq.push(st)) != RR_OK) return s;
             issued = true;
         }
         // complete sequences: G is built on st, the PM stream waits for it
         const size_t njobs = jobs.size();
-        for (auto &r : reqs)
-            if (r->stage == STAGE_KRYLOV && r->t >= r->L) {
+        for (uint32_t i = 0; i < count; i++)
+            if (Request *r = joined[i] ? reqs[i].get() : nullptr; r && r->stage == STAGE_KRYLOV && r->t >= r->L) {
                 if ((s = r->start_pm(st, false)) != RR_OK) return s;
                 jobs.push_back(&r->pm);
             }
@@ -674,20 +743,13 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
                 pm_idle = true;  // every joined job is done
             }
         }
-        if (!krylov_done && active.empty() && !handed) {
+        if (!krylov_done && njoined == count && active.empty() && fresh.empty() && !handed) {
             RR_HIP(hipStreamSynchronize(st));
             krylov_done = true;
             t2 = std::chrono::steady_clock::now();
         }
         if (krylov_done && pm_idle) break;
-        if (!issued) {
-            // both streams are far enough ahead: wait for either to retire an item
-            for (;;) {
-                bool k = kq.retire(), q = pq.retire();
-                if (k || q) break;
-                std::this_thread::yield();
-            }
-        }
+        if (!issued && !kq.retire() && !pq.retire()) std::this_thread::yield();  // streams ahead, or set-up pending
     }
     RR_HIP(hipStreamSynchronize(st_pm));
     uint64_t rounds = S.rounds, launches = S.launches;

@@ -85,19 +85,37 @@ static inline bool h_is_prime(uint32_t p) {
     return true;
 }
 
+// Device memory through a per-device cache of freed blocks (size classes
+// within 12.5 % of the request): hipMalloc / hipFree cost tens of
+// microseconds each and hipFree waits for the device, which a batch of
+// hundreds of matrices with dozens of buffers each pays thousands of times.
+// A freed block may still be in use by kernels in flight, so the cache hands
+// it out again only after a device synchronisation (one for all the blocks
+// freed before it), the wait hipFree would have done.  When hipMalloc fails
+// the cached blocks are released and the allocation is retried.
+void *dev_alloc(size_t bytes, size_t *cls, int *device, hipError_t *err);
+void dev_free(void *ptr, size_t cls, int device);
+void dev_cache_release(int device);  // waits for the device, frees its cached blocks
+
 // Owned device allocation.
 struct DevBuf {
     void *ptr = nullptr;
-    size_t bytes = 0;
+    size_t bytes = 0, cls = 0;
+    int device = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes), cls(o.cls), device(o.device) { o.ptr = nullptr; }
     ~DevBuf() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr) dev_free(ptr, cls, device);
     }
     rr_status alloc(size_t n) {
+        if (ptr) dev_free(ptr, cls, device);
+        ptr = nullptr;
         bytes = n;
-        if (n == 0) n = 4;
-        hipError_t e = hipMalloc(&ptr, n);
-        if (e != hipSuccess) {
-            ptr = nullptr;
+        hipError_t e = hipSuccess;
+        ptr = dev_alloc(n == 0 ? 4 : n, &cls, &device, &e);
+        if (!ptr) {
             rr::set_error("hipMalloc(%zu) failed: %s", n, hipGetErrorString(e));
             return RR_ENOMEM;
         }

@@ -1,9 +1,98 @@
-// util.hip -- measurement helpers: a plain device copy kernel of known size,
-// used to calibrate hardware byte counters (FETCH_SIZE / WRITE_SIZE) and to
-// measure achievable HBM bandwidth on this GPU.
+// util.hip -- the device memory cache behind DevBuf, and measurement
+// helpers: a plain device copy kernel of known size, used to calibrate
+// hardware byte counters (FETCH_SIZE / WRITE_SIZE) and to measure achievable
+// HBM bandwidth on this GPU.
 #include "common.hpp"
+#include <deque>
+#include <map>
+#include <mutex>
+#include <vector>
 
 namespace rr {
+
+namespace {
+struct FreeBlock {
+    void *ptr;
+    uint64_t gen;  // free generation: safe to reuse once the device was synchronised after it
+};
+struct DevCache {
+    std::map<size_t, std::deque<FreeBlock>> free;  // per size class, oldest first
+    uint64_t gen_freed = 0, gen_synced = 0;
+};
+std::mutex g_cache_mu;
+std::map<int, DevCache> g_cache;
+
+size_t size_class(size_t n) {
+    if (n <= 256) return 256;
+    int k = 63 - __builtin_clzll((unsigned long long)(n - 1));  // n in (2^k, 2^(k+1)]
+    if (k < 20) return (size_t)1 << (k + 1);
+    const size_t step = (size_t)1 << (k - 3);
+    return (n + step - 1) / step * step;
+}
+}  // namespace
+
+void *dev_alloc(size_t bytes, size_t *cls, int *device, hipError_t *err) {
+    int dev = 0;
+    if ((*err = hipGetDevice(&dev)) != hipSuccess) return nullptr;
+    const size_t c = size_class(bytes);
+    *cls = c;
+    *device = dev;
+    {
+        std::unique_lock<std::mutex> lk(g_cache_mu);
+        DevCache &D = g_cache[dev];
+        for (;;) {
+            auto it = D.free.find(c);
+            if (it == D.free.end() || it->second.empty()) break;
+            FreeBlock fb = it->second.front();
+            if (fb.gen > D.gen_synced) {  // possibly still in use: wait for the device once
+                const uint64_t g = D.gen_freed;
+                lk.unlock();
+                *err = hipDeviceSynchronize();
+                lk.lock();
+                if (*err != hipSuccess) return nullptr;
+                D.gen_synced = std::max(D.gen_synced, g);
+                continue;
+            }
+            it->second.pop_front();
+            return fb.ptr;
+        }
+    }
+    void *p = nullptr;
+    *err = hipMalloc(&p, c);
+    if (*err == hipErrorOutOfMemory || *err == hipErrorMemoryAllocation) {
+        (void)hipGetLastError();
+        dev_cache_release(dev);
+        *err = hipMalloc(&p, c);
+    }
+    if (*err != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return p;
+}
+
+void dev_free(void *ptr, size_t cls, int device) {
+    std::lock_guard<std::mutex> g(g_cache_mu);
+    DevCache &D = g_cache[device];
+    D.free[cls].push_back(FreeBlock{ptr, ++D.gen_freed});
+}
+
+void dev_cache_release(int device) {
+    std::vector<void *> blocks;
+    {
+        std::lock_guard<std::mutex> g(g_cache_mu);
+        DevCache &D = g_cache[device];
+        for (auto &kv : D.free)
+            for (const FreeBlock &fb : kv.second) blocks.push_back(fb.ptr);
+        D.free.clear();
+    }
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(device);
+    (void)hipDeviceSynchronize();
+    for (void *p : blocks) (void)hipFree(p);
+    (void)hipSetDevice(cur);
+}
 
 __global__ __launch_bounds__(256) void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -15,6 +104,14 @@ __global__ __launch_bounds__(256) void k_copy(const uint4 *__restrict__ src, uin
 using namespace rr;
 
 extern "C" {
+
+rr_status rr_release_device_memory(void) {
+    clear_error();
+    int dev = 0;
+    RR_HIP(hipGetDevice(&dev));
+    dev_cache_release(dev);
+    return RR_OK;
+}
 
 // Copies `bytes` (multiple of 16) device-to-device with k_copy, `iters` times;
 // ms_out receives the average kernel time and gbs_out read+write GB/s.

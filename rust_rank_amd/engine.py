@@ -73,6 +73,7 @@ def lib():
     L = C.CDLL(LIB_PATH)
     L.rr_last_error.restype = C.c_char_p
     L.rr_device_count.restype = C.c_int
+    L.rr_release_device_memory.argtypes = []
     L.rr_krylov_create.argtypes = [C.POINTER(_Csr), C.c_uint32, C.c_uint32, _u32p, _u32p, _u32p, C.c_uint64,
                                    C.POINTER(C.c_void_p)]
     L.rr_krylov_run.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
@@ -125,6 +126,11 @@ def _check(status: int):
 
 def device_count() -> int:
     return int(lib().rr_device_count())
+
+
+def release_device_memory():
+    """Returns the library's cached device buffers to the system."""
+    _check(lib().rr_release_device_memory())
 
 
 def _csr_struct(A):

@@ -19,6 +19,8 @@ values in [0, p) (negative entries stored as p - |v|).
 from __future__ import annotations
 
 import dataclasses
+import os
+
 import numpy as np
 
 P_DEFAULT = 2013265921  # 15 * 2^27 + 1
@@ -297,6 +299,22 @@ def batch_matrix(seed: int, index: int, p: int = P_DEFAULT, rows_lo: int = 1000,
     frac = 0.01 + 0.04 * u[1]
     A = uniform_sparse(seed, 20_000 + index, rows, cols, k_lo, k_hi, p, small=True)
     return plant_dependent_columns(Rng(seed, 30_000 + index), A, int(round(frac * rows)), p, small=True)
+
+
+def _batch_matrix_args(args):
+    return batch_matrix(*args)
+
+
+def batch_matrices(seed: int, count: int, workers: int = 0) -> list:
+    """batch_matrix(seed, i) for i < count, generated by up to `workers`
+    processes (0: one per CPU, at most 16); the same matrices as one by one."""
+    workers = workers or min(16, os.cpu_count() or 1)
+    if workers <= 1 or count < 16:
+        return [batch_matrix(seed, i) for i in range(count)]
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_batch_matrix_args, [(seed, i) for i in range(count)], chunksize=8))
 
 
 def preconditioners(seed: int, m: int, n: int, b: int, p: int = P_DEFAULT):

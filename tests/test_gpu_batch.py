@@ -50,3 +50,67 @@ def test_batch_sequences_bit_identical_to_oracle():
         dr2, dc2, V2 = synth.preconditioners(8, other.m, other.n, 8, P)
         ranks, _ = eng.batch_rank([other, A], P, [8, b], [dr2, dr], [dc2, dc], [V2, V], [0, L], [0, d])
         assert ranks[1] == want, (d, ranks[1], want)
+
+
+def test_batch_many_matrices_match_single():
+    """48 matrices: set-up on worker threads overlaps the device work, so
+    requests join the tick loop at different times; every rank still equals
+    the single-matrix path's."""
+    eng = _eng()
+    mats, bs, pre = [], [], []
+    for i in range(48):
+        A = synth.batch_matrix(33, i, rows_lo=100, rows_hi=3000)
+        b = (8, 16, 4, 32)[i % 4]
+        mats.append(A)
+        bs.append(b)
+        pre.append(synth.preconditioners(300 + i, A.m, A.n, b, P))
+    print(f"batch of {len(mats)}: rows {min(A.m for A in mats)}..{max(A.m for A in mats)}", flush=True)
+    ranks, stats = eng.batch_rank(mats, P, bs, [x[0] for x in pre], [x[1] for x in pre], [x[2] for x in pre])
+    print(stats, flush=True)
+    for i, A in enumerate(mats):
+        single, _ = eng.rank(A, P, bs[i], *pre[i])
+        assert ranks[i] == single, (i, ranks[i], single)
+
+
+def test_batch_rejects_invalid_matrix_and_recovers():
+    """A value out of range in one matrix of a batch: the call fails with an
+    error naming that matrix (its set-up threads joined), and the next batch
+    on the same device runs normally."""
+    eng = _eng()
+    mats, pre = [], []
+    for i in range(20):
+        A = synth.batch_matrix(44, i, rows_lo=100, rows_hi=800)
+        mats.append(A)
+        pre.append(synth.preconditioners(400 + i, A.m, A.n, 8, P))
+    bad = mats[7]
+    val = bad.val.copy()
+    val[3] = P  # not a residue
+    mats[7] = synth.Csr(bad.m, bad.n, bad.row_ptr, bad.col, val)
+    with pytest.raises(RuntimeError, match="matrix 7"):
+        eng.batch_rank(mats, P, [8] * 20, [x[0] for x in pre], [x[1] for x in pre], [x[2] for x in pre])
+    mats[7] = bad
+    ranks, _ = eng.batch_rank(mats[:6], P, [8] * 6, [x[0] for x in pre[:6]], [x[1] for x in pre[:6]],
+                              [x[2] for x in pre[:6]])
+    for i in range(6):
+        single, _ = eng.rank(mats[i], P, 8, *pre[i])
+        assert ranks[i] == single, (i, ranks[i], single)
+
+
+def test_cached_device_memory_reuse_and_release():
+    """Buffers freed by one call are reused by the next (the cache waits for
+    the device before handing them out) and can be returned to the system;
+    results stay equal across reuse and release."""
+    eng = _eng()
+    mats, pre = [], []
+    for i in range(10):
+        A = synth.batch_matrix(55, i, rows_lo=200, rows_hi=2000)
+        mats.append(A)
+        pre.append(synth.preconditioners(500 + i, A.m, A.n, 16, P))
+    args = (P, [16] * 10, [x[0] for x in pre], [x[1] for x in pre], [x[2] for x in pre])
+    first, _ = eng.batch_rank(mats, *args)
+    again, _ = eng.batch_rank(mats, *args)  # served from the cache
+    eng.release_device_memory()
+    fresh, _ = eng.batch_rank(mats, *args)
+    assert list(first) == list(again) == list(fresh)
+    singles = [eng.rank(A, P, 16, *pre[i])[0] for i, A in enumerate(mats)]
+    assert list(first) == singles
