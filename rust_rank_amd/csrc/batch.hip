@@ -782,14 +782,13 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
 //   c_b   Krylov time per nonzero and tick for width b: TUNE_COPIES copies of
 //         a probe matrix with +-1/+-2 entries ticking together, 16 ticks
 //         timed with events, divided by the copies' nonzeros;
-//   pm_b  PM-basis time per job at order 255 for width b: TUNE_COPIES jobs
-//         of random sequences in one batched scheduler run, divided by the
-//         jobs.
+//   pm_b(d) PM-basis time per job at orders 255 and 1023 for width b:
+//         TUNE_COPIES jobs of random sequences in one batched scheduler run,
+//         divided by the jobs, fitted as alpha_b d + beta_b d log2 d (leaves
+//         are linear in d, the products of the d&c add d log d).
 // Predicted device time of matrix i with width b:
-//   L(b) * nnz_i * c_b + pm_b * d(b) / 255,
+//   L(b) * nnz_i * c_b + pm_b(d(b)),
 //   L(b) = 2 min(m, n) / b + 8, d(b) = L(b) - 1.
-// The PM-basis term is linear in d: its leaves, the bulk of its time, are
-// one per K coefficients.
 constexpr uint32_t TUNE_COPIES = 8;
 
 namespace rr {
@@ -797,7 +796,9 @@ namespace rr {
 struct TuneProbe {
     bool done = false;
     double c[3] = {0, 0, 0}, pm[3] = {0, 0, 0};
+    double pm_at[3][2] = {}, alpha[3] = {0, 0, 0}, beta[3] = {0, 0, 0};
 };
+static const uint64_t TUNE_D[2] = {255, 1023};
 static TuneProbe g_probe;
 static const uint32_t TUNE_B[3] = {8, 16, 32};
 
@@ -850,36 +851,45 @@ static rr_status run_probe(uint32_t p, int cus, hipStream_t st) {
         float ms = 0;
         RR_HIP(hipEventElapsedTime(&ms, e0, e1));
         g_probe.c[i] = (ms / 16.0) / ((double)TUNE_COPIES * (double)(m * k));
-        // PM-basis probe: TUNE_COPIES jobs of order 255, random sequences
-        const uint64_t d = 255, P = b * (b + 1) / 2;
-        std::vector<uint32_t> seq((d + 1) * P * TUNE_COPIES);
-        for (auto &v : seq) v = (uint32_t)(rnd() % p);
-        DevBuf dseq;
-        std::vector<DevBuf> G(TUNE_COPIES), M(TUNE_COPIES), dl(TUNE_COPIES);
-        std::vector<PmJob> J(TUNE_COPIES);
-        std::vector<PmJob *> jobs;
-        if ((s = dseq.alloc(seq.size() * 4)) != RR_OK) return s;
-        RR_HIP(hipMemcpyAsync(dseq.ptr, seq.data(), seq.size() * 4, hipMemcpyHostToDevice, st));
-        for (uint32_t c = 0; c < TUNE_COPIES; c++) {
-            if ((s = G[c].alloc((size_t)2 * b * b * (d + 1) * 4)) != RR_OK) return s;
-            if ((s = M[c].alloc((size_t)4 * b * b * (d + 1) * 4)) != RR_OK) return s;
-            if ((s = dl[c].alloc((size_t)2 * b * 8)) != RR_OK) return s;
-            if ((s = build_G_device(dseq.as<uint32_t>() + (size_t)c * (d + 1) * P, b, d + 1, G[c].as<uint32_t>(),
-                                    st)) != RR_OK)
-                return s;
-            J[c].b = b;
-            J[c].d = d;
-            J[c].G = G[c].as<uint32_t>();
-            J[c].M = M[c].as<uint32_t>();
-            J[c].delta = dl[c].as<int64_t>();
-            jobs.push_back(&J[c]);
+        // PM-basis probe: TUNE_COPIES jobs of random sequences at each order of TUNE_D
+        const uint64_t P = b * (b + 1) / 2;
+        for (int o = 0; o < 2; o++) {
+            const uint64_t d = TUNE_D[o];
+            std::vector<uint32_t> seq((d + 1) * P * TUNE_COPIES);
+            for (auto &v : seq) v = (uint32_t)(rnd() % p);
+            DevBuf dseq;
+            std::vector<DevBuf> G(TUNE_COPIES), M(TUNE_COPIES), dl(TUNE_COPIES);
+            std::vector<PmJob> J(TUNE_COPIES);
+            std::vector<PmJob *> jobs;
+            if ((s = dseq.alloc(seq.size() * 4)) != RR_OK) return s;
+            RR_HIP(hipMemcpyAsync(dseq.ptr, seq.data(), seq.size() * 4, hipMemcpyHostToDevice, st));
+            for (uint32_t c = 0; c < TUNE_COPIES; c++) {
+                if ((s = G[c].alloc((size_t)2 * b * b * (d + 1) * 4)) != RR_OK) return s;
+                if ((s = M[c].alloc((size_t)4 * b * b * (d + 1) * 4)) != RR_OK) return s;
+                if ((s = dl[c].alloc((size_t)2 * b * 8)) != RR_OK) return s;
+                if ((s = build_G_device(dseq.as<uint32_t>() + (size_t)c * (d + 1) * P, b, d + 1,
+                                        G[c].as<uint32_t>(), st)) != RR_OK)
+                    return s;
+                J[c].b = b;
+                J[c].d = d;
+                J[c].G = G[c].as<uint32_t>();
+                J[c].M = M[c].as<uint32_t>();
+                J[c].delta = dl[c].as<int64_t>();
+                jobs.push_back(&J[c]);
+            }
+            RR_HIP(hipStreamSynchronize(st));
+            auto t0 = std::chrono::steady_clock::now();
+            if ((s = pm_basis_batch(p, jobs, st, nullptr, nullptr)) != RR_OK) return s;
+            RR_HIP(hipStreamSynchronize(st));
+            g_probe.pm_at[i][o] =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / TUNE_COPIES;
         }
-        RR_HIP(hipStreamSynchronize(st));
-        auto t0 = std::chrono::steady_clock::now();
-        if ((s = pm_basis_batch(p, jobs, st, nullptr, nullptr)) != RR_OK) return s;
-        RR_HIP(hipStreamSynchronize(st));
-        g_probe.pm[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
-                        TUNE_COPIES;
+        // pm(d) = alpha d + beta d log2 d through the two probes
+        const double l0 = std::log2((double)TUNE_D[0]), l1 = std::log2((double)TUNE_D[1]);
+        const double r0 = g_probe.pm_at[i][0] / TUNE_D[0], r1 = g_probe.pm_at[i][1] / TUNE_D[1];
+        g_probe.beta[i] = std::max(0.0, (r1 - r0) / (l1 - l0));
+        g_probe.alpha[i] = std::max(0.0, r0 - g_probe.beta[i] * l0);
+        g_probe.pm[i] = g_probe.pm_at[i][0];
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -914,7 +924,7 @@ rr_status rr_autotune_b(uint32_t count, const rr_csr *mats, uint32_t p, uint32_t
             uint32_t b = TUNE_B[k];
             double L = 2.0 * (double)mn / b + 8, d = L - 1;
             if (d + 1 > (double)(1u << NTT_MAX_LOGN)) continue;
-            double t = L * (double)nnz * g_probe.c[k] + g_probe.pm[k] * (d / 255.0);
+            double t = L * (double)nnz * g_probe.c[k] + d * (g_probe.alpha[k] + g_probe.beta[k] * std::log2(d));
             if (t < best) {
                 best = t;
                 bb = b;

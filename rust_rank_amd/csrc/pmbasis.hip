@@ -60,12 +60,15 @@ struct LeafPack {
 struct LeafLds {
     uint32_t *R, *U, *sig, *fcol, *perm, *pivrow, *isx;
     uint32_t *prow;  // per pivot q: the row of X it reads, perm(pivrow q)
-    uint32_t *ctl;   // a block's pivot count and its pivots {row, column, index, entry}
+    uint32_t *pbuf;  // two slots of LEAF_PBUF words: a pivot row and its pivot fields
     uint32_t *P, *M;
     int64_t *dl, *dln;
 };
 
-__host__ __device__ constexpr uint32_t leaf_dl_off(uint32_t m, uint32_t n) { return (2 * m * n + 4 * m + 2 * n + 33 + 1) & ~1u; }
+constexpr uint32_t LEAF_PBUF = 64 * 2 + 8;  // a row of up to 128 columns, then {position, column, entry, sigma}
+__host__ __device__ constexpr uint32_t leaf_dl_off(uint32_t m, uint32_t n) {
+    return (2 * m * n + 4 * m + 2 * n + 2 * LEAF_PBUF + 1) & ~1u;
+}
 
 __device__ LeafLds leaf_lds(uint32_t m, uint32_t n, uint32_t K) {
     extern __shared__ __align__(16) uint32_t sm[];
@@ -78,7 +81,7 @@ __device__ LeafLds leaf_lds(uint32_t m, uint32_t n, uint32_t K) {
     L.isx = L.perm + m;
     L.pivrow = L.isx + m;
     L.prow = L.pivrow + n;
-    L.ctl = L.prow + n;  // 1 + 4 LEAF_BLK = 33 words
+    L.pbuf = L.prow + n;
     L.dl = (int64_t *)(sm + leaf_dl_off(m, n));  // 8-byte aligned
     L.dln = L.dl + m;
     L.P = (uint32_t *)(L.dln + m);
@@ -181,75 +184,24 @@ __device__ __forceinline__ void leaf_advance(const LeafLds &L, uint32_t *X, uint
 // order L.perm): fraction-free LSP, then U' = Linv coefficients, isx, the
 // new delta in L.dln.  Returns the rank.
 //
-// A row is the 2n + 1 columns [R | U | sigma] (U: its coefficients on the
-// pivot rows, sigma: its multiplier), NREG registers per lane, column
-// j = 64 r + lane in register r.  One update of row i by the pivot of row k
+// A row is [R | U] (n + n columns) and its multiplier sigma: U holds its
+// coefficients on the pivot rows.  One update of row i by the pivot of row k
 // (column c, pivot index q):
 //   x <- (pkv x + (p - f) y) 2^-32,  pkv = R[k][c], f = R[i][c],
 // y = row k's column, except U column q, where row i holds 0 and y is
-// sigma_k, and sigma, where y = 0.  Scaling a whole row with its multiplier
-// (the Montgomery factor, and pkv when f = 0) leaves its Linv row U / sigma
-// and its zero pattern as they are; pkv x + (p - f) y < 2 p^2.
+// sigma_k; sigma_i <- pkv sigma_i 2^-32.  Scaling a whole row with its
+// multiplier (the Montgomery factor, and pkv when f = 0) leaves its Linv row
+// U / sigma and its zero pattern as they are; pkv x + (p - f) y < 2 p^2.
 //
-// Rows of at most 64 columns go in blocks of LEAF_BLK.  Wavefront 0 holds a block in registers,
-// finds its pivots one after the other with the reference's rule (the first
-// position >= rank, in a column order it keeps in registers: lane q holds
-// the column at position q, q < 128 over two registers, whose entry in the
-// row is nonzero), applies each pivot to the block's later rows, and stores
-// the block and its pivot list.  Then every wavefront applies the block's
-// pivots, in order, to its share of the rows below.  Two barriers a block.
-constexpr uint32_t LEAF_BLK = 4;
-
-struct LeafSearch {
-    uint32_t cord0, cord1, rank;
-};
-
-// Pivot of a row, given its entries v0, v1 in the columns at positions
-// lane and 64 + lane: the first position q >= rank whose column is nonzero
-// (m when none); swaps positions pos and rank and returns the column.
-__device__ __forceinline__ uint32_t leaf_pick(LeafSearch &S, uint32_t v0, uint32_t v1, uint32_t m, uint32_t n,
-                                              uint32_t lane, uint32_t &pos) {
-    const uint64_t b0 = __ballot(lane >= S.rank && lane < m && S.cord0 < n && v0 != 0);
-    pos = m;
-    if (b0) {
-        pos = (uint32_t)__ffsll((unsigned long long)b0) - 1;
-    } else if (m > 64) {
-        const uint64_t b1 = __ballot(64 + lane >= S.rank && 64 + lane < m && S.cord1 < n && v1 != 0);
-        if (b1) pos = 64 + (uint32_t)__ffsll((unsigned long long)b1) - 1;
-    }
-    uint32_t c = 0;
-    if (pos < m) {
-        c = __builtin_amdgcn_readlane(pos < 64 ? S.cord0 : S.cord1, pos & 63);
-        const uint32_t cr = __builtin_amdgcn_readlane(S.rank < 64 ? S.cord0 : S.cord1, S.rank & 63);
-        if (lane == (pos & 63)) {
-            if (pos < 64) S.cord0 = cr;
-            else S.cord1 = cr;
-        }
-        if (lane == (S.rank & 63)) {
-            if (S.rank < 64) S.cord0 = c;
-            else S.cord1 = c;
-        }
-    }
-    return c;
-}
-
-// ... of a row held in registers (x0: lane l holds column l)
-__device__ __forceinline__ uint32_t leaf_search(LeafSearch &S, uint32_t x0, uint32_t m, uint32_t n, uint32_t lane,
-                                                uint32_t &pos) {
-    const uint32_t v0 = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(min(S.cord0, 63u) * 4), (int)x0);
-    const uint32_t v1 = m > 64 ? (uint32_t)__builtin_amdgcn_ds_bpermute((int)(min(S.cord1, 63u) * 4), (int)x0) : 0u;
-    return leaf_pick(S, v0, v1, m, n, lane, pos);
-}
-
-// ... of row r of L.R
-__device__ __forceinline__ uint32_t leaf_search_lds(const LeafLds &L, LeafSearch &S, uint32_t r, uint32_t m,
-                                                    uint32_t n, uint32_t lane, uint32_t &pos) {
-    const uint32_t *row = L.R + r * n;
-    const uint32_t v0 = row[min(S.cord0, n - 1)];
-    const uint32_t v1 = m > 64 ? row[min(S.cord1, n - 1)] : 0u;
-    return leaf_pick(S, v0, v1, m, n, lane, pos);
-}
-
+// The rows stay in registers for the whole elimination: row i belongs to
+// wavefront i mod W (W wavefronts), slot i / W; lanes run over the columns
+// (column j = 64 g + lane in register g, NREG = 1 for n <= 32, 2 for
+// n <= 64), sigma is a uniform register.  For each row k in order its owner
+// searches it (the reference's rule: the first position >= rank, in a column
+// order every wavefront keeps in one register, whose column is nonzero),
+// then writes the pivot row and its pivot to one of two LDS slots; after one
+// barrier every wavefront applies the pivot to its rows below k.  Only the
+// pivot row crosses LDS, once; one barrier a row.
 // Montgomery reduction with R = 2^32: x < p 2^32 -> x R^-1 mod p; pneg = -p^-1 mod 2^32.
 __device__ __forceinline__ uint32_t redc(uint64_t x, uint32_t p, uint32_t pneg) {
     const uint32_t mq = (uint32_t)x * pneg;
@@ -264,169 +216,116 @@ __device__ __forceinline__ uint32_t neg_inv_2_32(uint32_t p) {  // p odd
     return 0u - x;
 }
 
-template <int NREG>
-struct LeafRow {
-    uint32_t x[NREG];
-};
-
-template <int NREG>
-__device__ __forceinline__ LeafRow<NREG> leaf_row_load(const LeafLds &L, uint32_t i, uint32_t n, uint32_t lane) {
-    LeafRow<NREG> r;
-#pragma unroll
-    for (int g = 0; g < NREG; g++) {
-        const uint32_t j = 64 * g + lane;
-        const uint32_t vr = L.R[i * n + min(j, n - 1)];
-        const uint32_t vu = L.U[i * n + min(j - n, n - 1)];  // j < n wraps and is not used
-        const uint32_t vs = L.sig[i];
-        r.x[g] = j < n ? vr : j < 2 * n ? vu : j == 2 * n ? vs : 0u;
-    }
-    return r;
-}
-
-template <int NREG>
-__device__ __forceinline__ void leaf_row_store(const LeafLds &L, uint32_t i, uint32_t n, uint32_t lane,
-                                               const LeafRow<NREG> &r) {
-#pragma unroll
-    for (int g = 0; g < NREG; g++) {
-        const uint32_t j = 64 * g + lane;
-        if (j < n) L.R[i * n + j] = r.x[g];
-        else if (j < 2 * n) L.U[i * n + j - n] = r.x[g];
-        else if (j == 2 * n) L.sig[i] = r.x[g];
-    }
-}
-
-// Row r by the pivot row pr (pivot column c, index q, pkv = pr's entry in column c).
-template <int NREG>
-__device__ __forceinline__ void leaf_apply(LeafRow<NREG> &r, const LeafRow<NREG> &pr, uint32_t c, uint32_t q,
-                                           uint32_t pkv, uint32_t n, uint32_t lane, uint32_t p, uint32_t pneg) {
-    const uint32_t nf = p - (uint32_t)__builtin_amdgcn_readlane((int)r.x[0], (int)c);
-    uint32_t sreg = pr.x[0];  // the register holding sigma_k (column 2n)
-#pragma unroll
-    for (int g = 1; g < NREG; g++) sreg = (uint32_t)g == (2 * n) / 64 ? pr.x[g] : sreg;
-    const uint32_t sk = (uint32_t)__builtin_amdgcn_readlane((int)sreg, (int)((2 * n) & 63));
-#pragma unroll
-    for (int g = 0; g < NREG; g++) {
-        const uint32_t j = 64 * g + lane;
-        const uint32_t y = j == n + q ? sk : j == 2 * n ? 0u : pr.x[g];
-        r.x[g] = redc((uint64_t)pkv * r.x[g] + (uint64_t)nf * y, p, pneg);
-    }
-}
-
-template <int NREG>
+template <int NREG, int S>
 __device__ __forceinline__ uint32_t leaf_lsp(const LeafLds &L, uint32_t m, uint32_t n, uint32_t p) {
-    const uint32_t tid = threadIdx.x, lane = tid % 64;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid / 64), nw = blockDim.x / 64;
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = blockDim.x >> 6;
     const uint32_t pneg = neg_inv_2_32(p);
-    LeafSearch S{lane, 64 + lane, 0u};
+    uint32_t x[S][NREG], sg[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const uint32_t i = s * nw + wave, ic = i < m ? i : m - 1;
+#pragma unroll
+        for (int g = 0; g < NREG; g++) {
+            const uint32_t j = 64 * g + lane;
+            const uint32_t vr = L.R[ic * n + min(j, n - 1)], vu = L.U[ic * n + min(j - n, n - 1)];
+            x[s][g] = j < n ? vr : j < 2 * n ? vu : 0u;
+        }
+        sg[s] = L.sig[ic];
+    }
+    uint32_t cord = lane;  // lane q: the column at position q of the column order
     uint32_t rank = 0;
-    for (uint32_t k0 = 0; k0 < m && rank < n; k0 += LEAF_BLK) {  // rank n: every later row is zero
-        if (wave == 0) {
-            LeafRow<NREG> rows[LEAF_BLK];
+    for (uint32_t k = 0; k < m && rank < n; k++) {  // rank n: every later row is zero
+        uint32_t *B = L.pbuf + (k & 1) * LEAF_PBUF;
+        if (wave == k % nw) {
+            const uint32_t ks = k / nw;
+            uint32_t r0 = x[0][0], rs = sg[0];
 #pragma unroll
-            for (uint32_t j = 0; j < LEAF_BLK; j++) rows[j] = leaf_row_load<NREG>(L, min(k0 + j, m - 1), n, lane);
-            uint32_t npiv = 0;
+            for (int s = 1; s < S; s++) {
+                r0 = (uint32_t)s == ks ? x[s][0] : r0;
+                rs = (uint32_t)s == ks ? sg[s] : rs;
+            }
+            // lane q: row k's entry in the column at position q
+            const uint32_t v = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(min(cord, 63u) * 4), (int)r0);
+            const uint64_t bal = __ballot(lane >= rank && lane < n && v != 0);
+            const uint32_t pos = bal ? (uint32_t)__ffsll((unsigned long long)bal) - 1 : n;
+            if (pos < n) {
 #pragma unroll
-            for (uint32_t j = 0; j < LEAF_BLK; j++) {
-                if (k0 + j < m && S.rank < n) {
-                    uint32_t pos;
-                    const uint32_t c = leaf_search(S, rows[j].x[0], m, n, lane, pos);
-                    if (pos < m) {
-                        const uint32_t q = S.rank++;
-                        const uint32_t pkv = (uint32_t)__builtin_amdgcn_readlane((int)rows[j].x[0], (int)c);
-                        if (lane == 0) {
-                            L.pivrow[q] = k0 + j;
-                            L.ctl[1 + 4 * npiv] = k0 + j;
-                            L.ctl[2 + 4 * npiv] = c;
-                            L.ctl[3 + 4 * npiv] = q;
-                            L.ctl[4 + 4 * npiv] = pkv;
-                        }
-                        npiv++;
+                for (int g = 0; g < NREG; g++) {
+                    uint32_t rg = x[0][g];
 #pragma unroll
-                        for (uint32_t jj = j + 1; jj < LEAF_BLK; jj++)
-                            leaf_apply<NREG>(rows[jj], rows[j], c, q, pkv, n, lane, p, pneg);
-                    }
+                    for (int s = 1; s < S; s++) rg = (uint32_t)s == ks ? x[s][g] : rg;
+                    B[64 * g + lane] = rg;
                 }
             }
-#pragma unroll
-            for (uint32_t j = 0; j < LEAF_BLK; j++)
-                if (k0 + j < m) leaf_row_store<NREG>(L, k0 + j, n, lane, rows[j]);
-            if (lane == 0) L.ctl[0] = npiv;
-        }
-        __syncthreads();
-        const uint32_t npiv = __builtin_amdgcn_readfirstlane(L.ctl[0]);
-        rank += npiv;
-        if (npiv && k0 + LEAF_BLK < m) {
-            LeafRow<NREG> pr[LEAF_BLK];
-            uint32_t pc[LEAF_BLK], pq[LEAF_BLK], pk[LEAF_BLK];
-#pragma unroll
-            for (uint32_t j = 0; j < LEAF_BLK; j++) {
-                const uint32_t jc = j < npiv ? j : 0;
-                pr[j] = leaf_row_load<NREG>(L, L.ctl[1 + 4 * jc], n, lane);
-                pc[j] = __builtin_amdgcn_readfirstlane(L.ctl[2 + 4 * jc]);
-                pq[j] = __builtin_amdgcn_readfirstlane(L.ctl[3 + 4 * jc]);
-                pk[j] = __builtin_amdgcn_readfirstlane(L.ctl[4 + 4 * jc]);
-            }
-            for (uint32_t i = k0 + LEAF_BLK + wave; i < m; i += nw) {
-                LeafRow<NREG> r = leaf_row_load<NREG>(L, i, n, lane);
-#pragma unroll
-                for (uint32_t j = 0; j < LEAF_BLK; j++)
-                    if (j < npiv) leaf_apply<NREG>(r, pr[j], pc[j], pq[j], pk[j], n, lane, p, pneg);
-                leaf_row_store<NREG>(L, i, n, lane, r);
+            if (lane == 0) {
+                B[128] = pos;
+                if (pos < n) {
+                    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)cord, (int)pos);
+                    B[129] = c;
+                    B[130] = (uint32_t)__builtin_amdgcn_readlane((int)r0, (int)c);
+                    B[131] = rs;
+                    L.pivrow[rank] = k;
+                }
             }
         }
         __syncthreads();
+        const uint32_t pos = __builtin_amdgcn_readfirstlane(B[128]);
+        if (pos >= n) continue;  // row k is zero past the pivots: no pivot
+        const uint32_t c = __builtin_amdgcn_readfirstlane(B[129]);
+        const uint32_t pkv = __builtin_amdgcn_readfirstlane(B[130]);
+        const uint32_t sk = __builtin_amdgcn_readfirstlane(B[131]);
+        // swap positions pos and rank of the column order
+        const uint32_t cr = (uint32_t)__builtin_amdgcn_readlane((int)cord, (int)rank);
+        if (lane == pos) cord = cr;
+        if (lane == rank) cord = c;
+        const uint32_t q = rank++;
+        uint32_t y[NREG];
+#pragma unroll
+        for (int g = 0; g < NREG; g++) {
+            const uint32_t j = 64 * g + lane;
+            y[g] = j == n + q ? sk : B[j];
+        }
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const uint32_t i = s * nw + wave;
+            if (i > k && i < m) {
+                const uint32_t nf = p - (uint32_t)__builtin_amdgcn_readlane((int)x[s][0], (int)c);
+#pragma unroll
+                for (int g = 0; g < NREG; g++) x[s][g] = redc((uint64_t)pkv * x[s][g] + (uint64_t)nf * y[g], p, pneg);
+                sg[s] = redc((uint64_t)pkv * sg[s], p, pneg);
+            }
+        }
     }
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const uint32_t i = s * nw + wave;
+        if (i < m) {
+#pragma unroll
+            for (int g = 0; g < NREG; g++) {
+                const uint32_t j = 64 * g + lane;
+                if (j < n) L.R[i * n + j] = x[s][g];
+                else if (j < 2 * n) L.U[i * n + j - n] = x[s][g];
+            }
+            if (lane == 0) L.sig[i] = sg[s];
+        }
+    }
+    __syncthreads();
     return rank;
 }
 
-// Row by row, for rows wider than a wavefront: every wavefront keeps its
-// own copy of the column order and searches row k itself right after the
-// barrier that made row k final (one barrier a row); lanes run over the
-// columns of a row, each wavefront over two rows at a time, all loads issued
-// before the arithmetic.  Same update as above.
-__device__ __forceinline__ uint32_t leaf_lsp_rows(const LeafLds &L, uint32_t m, uint32_t n, uint32_t p) {
-    const uint32_t tid = threadIdx.x, lane = tid % 64;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid / 64), nw = blockDim.x / 64;
-    const uint32_t pneg = neg_inv_2_32(p);
-    LeafSearch S{lane, 64 + lane, 0u};
-    for (uint32_t k = 0; k < m && S.rank < n; k++) {  // rank n: every later row is zero
-        uint32_t pos;
-        const uint32_t c = leaf_search_lds(L, S, k, m, n, lane, pos);
-        if (pos < m) {
-            const uint32_t rank = S.rank++;  // pivot index of row k
-            if (tid == 0) L.pivrow[rank] = k;
-            const uint32_t W = n + rank + 2;  // R, U (< rank), U column rank, sigma
-            const uint32_t *Rk = L.R + k * n, *Uk = L.U + k * n;
-            for (uint32_t i0 = k + 1 + wave; i0 < m; i0 += 2 * nw) {
-                const uint32_t i1 = i0 + nw < m ? i0 + nw : i0;
-                const uint32_t pkv = Rk[c], sk = L.sig[k], nf0 = p - L.R[i0 * n + c], nf1 = p - L.R[i1 * n + c];
-                for (uint32_t col = lane; col < W; col += 64) {
-                    const uint32_t cu = col - n;
-                    uint32_t *x0 = col < n ? L.R + i0 * n + col : col <= n + rank ? L.U + i0 * n + cu : L.sig + i0;
-                    uint32_t *x1 = col < n ? L.R + i1 * n + col : col <= n + rank ? L.U + i1 * n + cu : L.sig + i1;
-                    const uint32_t *ya = col < n ? Rk + col : Uk + min(cu, n - 1);
-                    const uint32_t yv = *ya, a0 = *x0, a1 = *x1;
-                    const uint32_t y = col < n + rank ? yv : col == n + rank ? sk : 0u;
-                    const uint32_t r0 = redc((uint64_t)pkv * a0 + (uint64_t)nf0 * y, p, pneg);
-                    const uint32_t r1 = redc((uint64_t)pkv * a1 + (uint64_t)nf1 * y, p, pneg);
-                    *x0 = r0;
-                    if (i1 != i0) *x1 = r1;
-                }
-            }
-        }
-        __syncthreads();
-    }
-    return S.rank;
+// Rows per wavefront and registers per row for m rows of n columns (m <= 128, n <= 64).
+__device__ __forceinline__ uint32_t leaf_lsp_any(const LeafLds &L, uint32_t m, uint32_t n, uint32_t p) {
+    const uint32_t nw = blockDim.x >> 6;
+    if (n > 32) return leaf_lsp<2, 128 / 16>(L, m, n, p);
+    if (m <= nw) return leaf_lsp<1, 1>(L, m, n, p);
+    if (m <= 2 * nw) return leaf_lsp<1, 2>(L, m, n, p);
+    return leaf_lsp<1, 4>(L, m, n, p);
 }
 
-// Elimination kind: 1, blocks in registers (rows [R | U | sigma] of at most
-// 64 columns, n <= 31); 2, row by row.
-__host__ __device__ constexpr int leaf_kind(uint32_t n) { return 2 * n + 1 <= 64 ? 1 : 2; }
-
-template <int KIND>
 __device__ __forceinline__ uint32_t leaf_eliminate(const LeafLds &L, uint32_t m, uint32_t n, uint32_t p, double inv) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    const uint32_t rank = KIND == 1 ? leaf_lsp<1>(L, m, n, p) : leaf_lsp_rows(L, m, n, p);
+    const uint32_t rank = leaf_lsp_any(L, m, n, p);
     // Linv row i = e_i + (U row i) / sigma_i over the earlier pivot rows
     for (uint32_t i = tid; i < m; i += nt) {
         uint32_t sv = L.sig[i], r = 1, e = p - 2;
@@ -470,8 +369,6 @@ __device__ __forceinline__ void leaf_sort(const LeafLds &L, uint32_t m) {
 }
 
 // D.G: residual window G(i, j, k) = G[(i * n + j) * gs + k], k < d; D.M: [m][m][d + 1].
-// Leaves of elimination kind KIND (leaf_kind(n)).
-template <int KIND>
 __global__ __launch_bounds__(LEAF_THREADS) void k_mbasis_iter(LeafPack pk, uint32_t p, double pd, double inv) {
     const LeafDesc D = pk.d[blockIdx.x];
     const uint32_t m = D.m, n = D.n, K = D.d;
@@ -487,7 +384,7 @@ __global__ __launch_bounds__(LEAF_THREADS) void k_mbasis_iter(LeafPack pk, uint3
             L.U[idx] = 0;
         }
         __syncthreads();
-        const uint32_t rank = leaf_eliminate<KIND>(L, m, n, p, inv);
+        const uint32_t rank = leaf_eliminate(L, m, n, p, inv);
         for (uint32_t idx = tid; idx < m * m; idx += nt) {
             D.M[(size_t)idx * 2] = 0;
             D.M[(size_t)idx * 2 + 1] = 0;
@@ -520,7 +417,7 @@ __global__ __launch_bounds__(LEAF_THREADS) void k_mbasis_iter(LeafPack pk, uint3
             L.U[idx] = 0;
         }
         __syncthreads();
-        const uint32_t rank = leaf_eliminate<KIND>(L, m, n, p, inv);
+        const uint32_t rank = leaf_eliminate(L, m, n, p, inv);
         // P <- T P on the coefficients later steps read (s .. K-1 in, s+1 .. K-1 out)
         if (s + 1 < K) leaf_advance(L, L.P, n, K, s, K - 1, s + 1, K - 1, rank, m, n, p, pd, inv);
         // M <- T M (degree s -> s + 1)
@@ -826,8 +723,7 @@ rr_status PmScheduler::init(uint32_t p_, hipStream_t st_, uint32_t max_logn, uin
     (void)max_m;
     (void)max_n;
     RR_CHECK(lds <= 160 * 1024, RR_EINVAL, "leaf needs %zu bytes of LDS", lds);
-    RR_HIP(hipFuncSetAttribute((const void *)k_mbasis_iter<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    RR_HIP(hipFuncSetAttribute((const void *)k_mbasis_iter<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    RR_HIP(hipFuncSetAttribute((const void *)k_mbasis_iter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     return RR_OK;
 }
 
@@ -885,21 +781,17 @@ rr_status PmScheduler::round(std::vector<PmJob *> &jobs, bool *launched) {
     if (leaves.empty() && school.empty() && ntt.empty()) return RR_OK;
     *launched = true;
     rounds++;
-    // one launch per elimination kind (and per LEAF_PACK leaves)
-    std::stable_sort(leaves.begin(), leaves.end(),
-                     [](const LeafDesc &x, const LeafDesc &y) { return leaf_kind(x.n) < leaf_kind(y.n); });
+    // one launch per LEAF_PACK leaves
     for (size_t i = 0; i < leaves.size();) {
         LeafPack pk;
-        const int kind = leaf_kind(leaves[i].n);
         uint32_t cnt = 0;
         size_t lw = 0;
-        while (i < leaves.size() && cnt < LEAF_PACK && leaf_kind(leaves[i].n) == kind) {
+        while (i < leaves.size() && cnt < LEAF_PACK) {
             pk.d[cnt++] = leaves[i];
             lw = std::max(lw, leaf_lds_words(leaves[i].m, leaves[i].n, leaves[i].d));
             i++;
         }
-        if (kind == 1) hipLaunchKernelGGL(k_mbasis_iter<1>, dim3(cnt), dim3(LEAF_THREADS), lw * 4, st, pk, p, mp.pd, mp.inv);
-        else hipLaunchKernelGGL(k_mbasis_iter<2>, dim3(cnt), dim3(LEAF_THREADS), lw * 4, st, pk, p, mp.pd, mp.inv);
+        hipLaunchKernelGGL(k_mbasis_iter, dim3(cnt), dim3(LEAF_THREADS), lw * 4, st, pk, p, mp.pd, mp.inv);
         RR_HIP(hipGetLastError());
         launches++;
     }
