@@ -258,12 +258,15 @@ __device__ __forceinline__ uint32_t leaf_lsp(const LeafLds &L, uint32_t m, uint3
                     B[64 * g + lane] = rg;
                 }
             }
+            // cross-lane reads stay outside the lane-0 branch: inside it the
+            // compiler may replace their operand by its value in lane 0
+            const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)cord, (int)(pos < n ? pos : 0));
+            const uint32_t pkv = (uint32_t)__builtin_amdgcn_readlane((int)r0, (int)c);
             if (lane == 0) {
                 B[128] = pos;
                 if (pos < n) {
-                    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)cord, (int)pos);
                     B[129] = c;
-                    B[130] = (uint32_t)__builtin_amdgcn_readlane((int)r0, (int)c);
+                    B[130] = pkv;
                     B[131] = rs;
                     L.pivrow[rank] = k;
                 }
