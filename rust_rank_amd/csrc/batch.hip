@@ -778,8 +778,8 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
 // for the batched and served modes.  There many matrices share the device,
 // so what a width costs is the device time its work takes among the others
 // (throughput), not its latency alone.  Measured once per process on the
-// device, with TUNE_COPIES copies of each probe running together:
-//   c_b   Krylov time per nonzero and tick for width b: TUNE_COPIES copies of
+// device, with copies of each probe running together:
+//   c_b   Krylov time per nonzero and tick for width b: TUNE_KCOPIES copies of
 //         a probe matrix with +-1/+-2 entries ticking together, 16 ticks
 //         timed with events, divided by the copies' nonzeros;
 //   pm_b(d) PM-basis time per job at orders 255 and 1023 for width b:
@@ -789,7 +789,7 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
 // Predicted device time of matrix i with width b:
 //   L(b) * nnz_i * c_b + pm_b(d(b)),
 //   L(b) = 2 min(m, n) / b + 8, d(b) = L(b) - 1.
-constexpr uint32_t TUNE_COPIES = 8;
+constexpr uint32_t TUNE_KCOPIES = 32, TUNE_COPIES = 8;
 
 namespace rr {
 
@@ -833,7 +833,7 @@ static rr_status run_probe(uint32_t p, int cus, hipStream_t st) {
         uint32_t b = TUNE_B[i];
         std::vector<uint32_t> Vb(n * b);
         for (uint64_t r = 0; r < n; r++) std::memcpy(&Vb[r * b], &V[r * 32], b * 4);
-        std::vector<std::unique_ptr<Request>> R(TUNE_COPIES);
+        std::vector<std::unique_ptr<Request>> R(TUNE_KCOPIES);
         std::vector<Request *> all;
         for (auto &r : R) {
             r.reset(new Request());
@@ -850,7 +850,7 @@ static rr_status run_probe(uint32_t p, int cus, hipStream_t st) {
         RR_HIP(hipEventSynchronize(e1));
         float ms = 0;
         RR_HIP(hipEventElapsedTime(&ms, e0, e1));
-        g_probe.c[i] = (ms / 16.0) / ((double)TUNE_COPIES * (double)(m * k));
+        g_probe.c[i] = (ms / 16.0) / ((double)TUNE_KCOPIES * (double)(m * k));
         // PM-basis probe: TUNE_COPIES jobs of random sequences at each order of TUNE_D
         const uint64_t P = b * (b + 1) / 2;
         for (int o = 0; o < 2; o++) {
