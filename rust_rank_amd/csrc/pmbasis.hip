@@ -96,47 +96,13 @@ static size_t leaf_lds_words(uint32_t m, uint32_t n, uint32_t K) {
 }
 
 constexpr uint32_t LEAF_THREADS = 1024;
-constexpr uint32_t ADV_MAX = 16;  // outputs per thread: m * cols * (khi - klo + 1) <= ADV_MAX * LEAF_THREADS
-
-// E outputs per thread (idx = tid + e * LEAF_THREADS), accumulated together so
-// that each pivot row's loads are shared by E independent sums; indices past
-// tot recompute the last output, which is then not stored.  A coefficient u
-// enters as u_hi 2^16 + u_lo with 64-bit integer multiply-adds; with at most
-// 64 pivots both sums stay below 2^53, so they convert to fp64 exactly.
-template <uint32_t E>
-__device__ __forceinline__ void leaf_advance_sums(const LeafLds &L, const uint32_t *X, uint32_t e0, uint32_t tot,
-                                                  uint32_t nsl, uint32_t cols, uint32_t stride, uint32_t klo,
-                                                  uint32_t rank, uint32_t n, double pd, double inv, uint32_t *res) {
-    const uint32_t tid = threadIdx.x, rstride = cols * stride;
-    uint32_t xo[E], uo[E];
-    uint64_t hacc[E], lacc[E];
-#pragma unroll
-    for (uint32_t e = 0; e < E; e++) {
-        uint32_t idx = tid + (e0 + e) * LEAF_THREADS;
-        idx = idx < tot ? idx : tot - 1;
-        const uint32_t rc = idx / nsl, kk = klo + (idx - rc * nsl);
-        const uint32_t i = rc / cols, col = rc - i * cols;
-        xo[e] = col * stride + kk;
-        uo[e] = i * n;
-        hacc[e] = 0;
-        lacc[e] = X[L.perm[i] * rstride + xo[e]];
-    }
-    // lane q holds pivot row q's offset in X (rank <= n <= 64)
-    const uint32_t prow_l = (tid & 63) < rank ? L.prow[tid & 63] * rstride : 0u;
-#pragma unroll 2
-    for (uint32_t q = 0; q < rank; q++) {
-        const uint32_t *xr = X + __builtin_amdgcn_readlane(prow_l, q);
-#pragma unroll
-        for (uint32_t e = 0; e < E; e++) {
-            const uint32_t u = L.U[uo[e] + q];
-            const uint32_t x = xr[xo[e]];
-            hacc[e] += (uint64_t)(u >> 16) * x;
-            lacc[e] += (uint64_t)(u & 0xFFFFu) * x;
-        }
-    }
-#pragma unroll
-    for (uint32_t e = 0; e < E; e++)
-        res[e0 + e] = (uint32_t)fold(fma(fold((double)hacc[e], pd, inv), 65536.0, fold((double)lacc[e], pd, inv)), pd, inv);
+// Advance tiles: a wavefront takes ADV_ROWS rows x 64 consecutive coefficient
+// slots of X, so each pivot row's word is loaded once for ADV_ROWS rows and
+// each U coefficient is one broadcast read; a wavefront holds at most
+// ADV_ITEMS such tiles.
+constexpr uint32_t ADV_ROWS = 4, ADV_ITEMS = 5;
+__host__ __device__ constexpr uint32_t adv_items(uint32_t m, uint32_t cols, uint32_t nsl) {
+    return ((cols * nsl + 63) / 64) * ((m + ADV_ROWS - 1) / ADV_ROWS);
 }
 
 // In-place X <- T X on coefficients: X [m][cols][stride] valid on [klo, khi],
@@ -144,35 +110,69 @@ __device__ __forceinline__ void leaf_advance_sums(const LeafLds &L, const uint32
 // coefficients U'[i][q] times the pivot rows L.prow[q], shifted up one
 // coefficient when row i was a pivot row.  Every output is computed into
 // registers before any is stored (inputs and outputs share X); outputs no
-// input reaches are zeroed.  Lanes run over coefficients first, so a
-// wavefront reads consecutive words of X.
+// input reaches are zeroed.  Tile (row group, slot block) of ADV_ROWS rows and
+// 64 slots (slot = col * nsl + coefficient - klo, lanes consecutive, so a
+// wavefront reads runs of consecutive words of X): tile t belongs to
+// wavefront t mod W.  A coefficient u enters as u_hi 2^16 + u_lo with 64-bit
+// integer multiply-adds; with at most 64 pivots both sums stay below 2^53,
+// so they convert to fp64 exactly.
 __device__ __forceinline__ void leaf_advance(const LeafLds &L, uint32_t *X, uint32_t cols, uint32_t stride, uint32_t klo,
-                             uint32_t khi, uint32_t olo, uint32_t ohi, uint32_t rank, uint32_t m, uint32_t n,
-                             uint32_t p, double pd, double inv) {
-    const uint32_t tid = threadIdx.x;
-    const uint32_t nsl = khi - klo + 1, tot = m * cols * nsl;
-    uint32_t res[ADV_MAX];
-    if (tot <= LEAF_THREADS) {
-        leaf_advance_sums<1>(L, X, 0, tot, nsl, cols, stride, klo, rank, n, pd, inv, res);
-    } else if (tot <= 2 * LEAF_THREADS) {
-        leaf_advance_sums<2>(L, X, 0, tot, nsl, cols, stride, klo, rank, n, pd, inv, res);
-    } else if (tot <= 4 * LEAF_THREADS) {
-        leaf_advance_sums<4>(L, X, 0, tot, nsl, cols, stride, klo, rank, n, pd, inv, res);
-    } else {
-        leaf_advance_sums<8>(L, X, 0, tot, nsl, cols, stride, klo, rank, n, pd, inv, res);
-        if (tot > 8 * LEAF_THREADS) leaf_advance_sums<8>(L, X, 8, tot, nsl, cols, stride, klo, rank, n, pd, inv, res);
+                                             uint32_t khi, uint32_t olo, uint32_t ohi, uint32_t rank, uint32_t m,
+                                             uint32_t n, uint32_t p, double pd, double inv) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = blockDim.x >> 6;
+    const uint32_t nsl = khi - klo + 1, ncc = cols * nsl, ccb = (ncc + 63) / 64;
+    const uint32_t items = ccb * ((m + ADV_ROWS - 1) / ADV_ROWS), rstride = cols * stride;
+    // lane q holds pivot row q's offset in X (rank <= n <= 64)
+    const uint32_t prow_l = lane < rank ? L.prow[lane] * rstride : 0u;
+    uint32_t res[ADV_ITEMS][ADV_ROWS];
+#pragma unroll
+    for (uint32_t it = 0; it < ADV_ITEMS; it++) {
+        const uint32_t item = wave + it * nw;
+        if (item >= items) break;
+        const uint32_t i0 = (item / ccb) * ADV_ROWS, cc = (item % ccb) * 64 + lane;
+        const uint32_t ccc = min(cc, ncc - 1), col = ccc / nsl, xo = col * stride + klo + (ccc - col * nsl);
+        uint32_t uo[ADV_ROWS];
+        uint64_t hacc[ADV_ROWS], lacc[ADV_ROWS];
+#pragma unroll
+        for (uint32_t r = 0; r < ADV_ROWS; r++) {
+            const uint32_t i = min(i0 + r, m - 1);
+            uo[r] = i * n;
+            hacc[r] = 0;
+            lacc[r] = X[L.perm[i] * rstride + xo];
+        }
+#pragma unroll 2
+        for (uint32_t q = 0; q < rank; q++) {
+            const uint32_t x = X[__builtin_amdgcn_readlane(prow_l, q) + xo];
+#pragma unroll
+            for (uint32_t r = 0; r < ADV_ROWS; r++) {
+                const uint32_t u = L.U[uo[r] + q];
+                hacc[r] += (uint64_t)(u >> 16) * x;
+                lacc[r] += (uint64_t)(u & 0xFFFFu) * x;
+            }
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < ADV_ROWS; r++)
+            res[it][r] =
+                (uint32_t)fold(fma(fold((double)hacc[r], pd, inv), 65536.0, fold((double)lacc[r], pd, inv)), pd, inv);
     }
     __syncthreads();
 #pragma unroll
-    for (uint32_t e = 0; e < ADV_MAX; e++) {
-        const uint32_t idx = tid + e * LEAF_THREADS;
-        if (idx < tot) {
-            const uint32_t rc = idx / nsl, kk = klo + (idx - rc * nsl);
-            const uint32_t ko = kk + L.isx[rc / cols];
-            if (ko >= olo && ko <= ohi) X[rc * stride + ko] = res[e];
+    for (uint32_t it = 0; it < ADV_ITEMS; it++) {
+        const uint32_t item = wave + it * nw;
+        if (item >= items) break;
+        const uint32_t i0 = (item / ccb) * ADV_ROWS, cc = (item % ccb) * 64 + lane;
+        if (cc >= ncc) continue;
+        const uint32_t col = cc / nsl, kk = klo + (cc - col * nsl);
+#pragma unroll
+        for (uint32_t r = 0; r < ADV_ROWS; r++) {
+            const uint32_t i = i0 + r;
+            if (i >= m) break;
+            const uint32_t ko = kk + L.isx[i];
+            if (ko >= olo && ko <= ohi) X[i * rstride + col * stride + ko] = res[it][r];
         }
     }
-    for (uint32_t idx = tid; idx < m * cols; idx += LEAF_THREADS) {
+    for (uint32_t idx = tid; idx < m * cols; idx += blockDim.x) {
         const uint32_t sh = L.isx[idx / cols];
         for (uint32_t ko = olo; ko <= ohi; ko++)
             if (ko < sh || ko - sh < klo || ko - sh > khi) X[idx * stride + ko] = 0u;
@@ -718,10 +718,12 @@ rr_status PmScheduler::init(uint32_t p_, hipStream_t st_, uint32_t max_logn, uin
         return s;
     // any mix of block widths: the largest leaf (b = 32: m = 64, K = 4) needs 150 KB
     lds = 0;
-    for (uint32_t mm = 2; mm <= 128; mm *= 2) {
+    for (uint32_t mm = 2; mm <= 128; mm += 2) {
         const uint32_t K = leaf_order(mm);
         lds = std::max(lds, leaf_lds_words(mm, mm / 2, K) * 4);
-        RR_CHECK(K == 1 || (size_t)mm * mm * K <= (size_t)ADV_MAX * LEAF_THREADS, RR_EINVAL, "leaf too large");
+        RR_CHECK(K == 1 || (adv_items(mm, mm, K) <= ADV_ITEMS * (LEAF_THREADS / 64) &&
+                            adv_items(mm, mm / 2, K) <= ADV_ITEMS * (LEAF_THREADS / 64)),
+                 RR_EINVAL, "leaf of %u rows too large for the advance tiles", mm);
     }
     (void)max_m;
     (void)max_n;
