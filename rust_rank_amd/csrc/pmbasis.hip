@@ -355,16 +355,19 @@ __device__ __forceinline__ uint32_t leaf_eliminate(const LeafLds &L, uint32_t m,
 }
 
 // Sorts rows by delta (stable, descending: rank_i = #{j: dl_j > dl_i} +
-// #{j < i: dl_j == dl_i}) and resets the elimination state.
+// #{j < i: dl_j == dl_i}) and resets the elimination state.  A wavefront
+// ranks one row at a time: lane j compares row j with it, one ballot counts.
 __device__ __forceinline__ void leaf_sort(const LeafLds &L, uint32_t m) {
-    const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    for (uint32_t i = tid; i < m; i += nt) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = blockDim.x >> 6;
+    const int64_t d0 = L.dl[min(lane, m - 1)], d1 = L.dl[min(64 + lane, m - 1)];
+    for (uint32_t i = wave; i < m; i += nw) {
         const int64_t di = L.dl[i];
-        uint32_t r = 0;
-        for (uint32_t j = 0; j < m; j++) r += (L.dl[j] > di) || (L.dl[j] == di && j < i);
-        L.perm[r] = i;
+        uint32_t r = (uint32_t)__popcll(__ballot(lane < m && (d0 > di || (d0 == di && lane < i))));
+        if (m > 64) r += (uint32_t)__popcll(__ballot(64 + lane < m && (d1 > di || (d1 == di && 64 + lane < i))));
+        if (lane == 0) L.perm[r] = i;
     }
-    for (uint32_t i = tid; i < m; i += nt) {
+    for (uint32_t i = tid; i < m; i += blockDim.x) {
         L.sig[i] = 1;
         L.isx[i] = 0;
     }
