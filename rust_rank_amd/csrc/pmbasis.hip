@@ -273,22 +273,22 @@ __device__ __forceinline__ uint32_t leaf_lsp(const LeafLds &L, uint32_t m, uint3
             }
         }
         __syncthreads();
+        // all reads of the slot issued together (the row is stale when row k has no pivot; unused then)
         const uint32_t pos = __builtin_amdgcn_readfirstlane(B[128]);
-        if (pos >= n) continue;  // row k is zero past the pivots: no pivot
         const uint32_t c = __builtin_amdgcn_readfirstlane(B[129]);
         const uint32_t pkv = __builtin_amdgcn_readfirstlane(B[130]);
         const uint32_t sk = __builtin_amdgcn_readfirstlane(B[131]);
+        uint32_t y[NREG];
+#pragma unroll
+        for (int g = 0; g < NREG; g++) y[g] = B[64 * g + lane];
+        if (pos >= n) continue;  // row k is zero past the pivots: no pivot
         // swap positions pos and rank of the column order
         const uint32_t cr = (uint32_t)__builtin_amdgcn_readlane((int)cord, (int)rank);
         if (lane == pos) cord = cr;
         if (lane == rank) cord = c;
         const uint32_t q = rank++;
-        uint32_t y[NREG];
 #pragma unroll
-        for (int g = 0; g < NREG; g++) {
-            const uint32_t j = 64 * g + lane;
-            y[g] = j == n + q ? sk : B[j];
-        }
+        for (int g = 0; g < NREG; g++) y[g] = 64 * g + lane == n + q ? sk : y[g];
 #pragma unroll
         for (int s = 0; s < S; s++) {
             const uint32_t i = s * nw + wave;
