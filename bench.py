@@ -56,7 +56,7 @@ def cpu_baseline(A, p, b, dr, dc, V, iters=256):
                       f"{threads} OpenMP threads; setup cancelled by differencing two runs)"}
 
 
-def batched_and_served(eng, synth, p, nmat, rate_frac, seed=4):
+def batched_and_served(eng, synth, p, nmat, rate_frac, seed=4, plain=True):
     """Batched form: nmat matrices of the batched workload (rows log-uniform in
     [1000, 50000], cols 0.9 rows, 6-16 nnz/row, +-1/+-2, 1-5 % dependent
     columns), b per matrix from the autotuner, solved in one rr_batch_rank, vs
@@ -71,6 +71,13 @@ def batched_and_served(eng, synth, p, nmat, rate_frac, seed=4):
     pre = [synth.preconditioners(5000 + i, A.m, A.n, int(b), p) for i, (A, b) in enumerate(zip(mats, bs))]
     log(f"batched: {nmat} matrices, rows {min(A.m for A in mats)}..{max(A.m for A in mats)}, "
         f"b histogram {dict(zip(*np.unique(bs, return_counts=True)))}, setup {time.perf_counter() - t0:.1f}s")
+    if not plain:  # the batched call alone (for kernel traces)
+        t2 = time.perf_counter()
+        ranks, st = eng.batch_rank(mats, p, bs, [x[0] for x in pre], [x[1] for x in pre], [x[2] for x in pre])
+        t_batch = time.perf_counter() - t2
+        log(f"batched {t_batch:.2f}s (plain solves skipped)")
+        return {"matrices": nmat, "ranks_per_s": round(nmat / t_batch, 3), "wall_s": round(t_batch, 3),
+                "ms_krylov": round(st["ms_krylov"], 1), "ms_pm": round(st["ms_pm"], 1)}, None
     # plain: one matrix at a time
     plain_ms, plain_rank = [], []
     t1 = time.perf_counter()
@@ -153,6 +160,8 @@ def main():
                     help="large: the 5000000 x 4500000, 20 nnz/row single matrix (Krylov only, bounded steps)")
     ap.add_argument("--batched", type=int, default=512, help="matrices in the batched/served comparison (0: skip)")
     ap.add_argument("--rate-frac", type=float, default=0.7, help="served offered load / batched capacity")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batched call only: skip the plain solves and the served form (for kernel traces)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
                     help="only run the plain copy kernel on MB megabytes (counter calibration) and exit")
     args = ap.parse_args()
@@ -233,7 +242,7 @@ def main():
     cpu = None if (args.no_cpu or large) else cpu_baseline(A, p, b, dr, dc, V)
     batched = served = None
     if args.batched and not large:
-        batched, served = batched_and_served(engine, synth, p, args.batched, args.rate_frac)
+        batched, served = batched_and_served(engine, synth, p, args.batched, args.rate_frac, plain=not args.no_plain)
     out = {
         "metric": "Krylov iters/s + HBM GB/s; batched ranks/s & p50/p99 vs plain, same GPU",
         "value": round(iters_s, 2), "unit": "iters/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
