@@ -61,13 +61,14 @@ struct LeafLds {
     uint32_t *R, *U, *sig, *fcol, *perm, *pivrow, *isx;
     uint32_t *prow;  // per pivot q: the row of X it reads, perm(pivrow q)
     uint32_t *pbuf;  // two slots of LEAF_PBUF words: a pivot row and its pivot fields
+    uint32_t *cs;    // per row of P and M (K > 1): the factor c it is kept scaled by
     uint32_t *P, *M;
     int64_t *dl, *dln;
 };
 
 constexpr uint32_t LEAF_PBUF = 64 * 2 + 8;  // a row of up to 128 columns, then {position, column, entry, sigma}
 __host__ __device__ constexpr uint32_t leaf_dl_off(uint32_t m, uint32_t n) {
-    return (2 * m * n + 4 * m + 2 * n + 2 * LEAF_PBUF + 1) & ~1u;
+    return (2 * m * n + 5 * m + 2 * n + 2 * LEAF_PBUF + 1) & ~1u;
 }
 
 __device__ LeafLds leaf_lds(uint32_t m, uint32_t n, uint32_t K) {
@@ -82,6 +83,7 @@ __device__ LeafLds leaf_lds(uint32_t m, uint32_t n, uint32_t K) {
     L.pivrow = L.isx + m;
     L.prow = L.pivrow + n;
     L.pbuf = L.prow + n;
+    L.cs = L.pbuf + 2 * LEAF_PBUF;
     L.dl = (int64_t *)(sm + leaf_dl_off(m, n));  // 8-byte aligned
     L.dln = L.dl + m;
     L.P = (uint32_t *)(L.dln + m);
@@ -139,7 +141,7 @@ __device__ __forceinline__ void leaf_advance(const LeafLds &L, uint32_t *X, uint
             const uint32_t i = min(i0 + r, m - 1);
             uo[r] = i * n;
             hacc[r] = 0;
-            lacc[r] = X[L.perm[i] * rstride + xo];
+            lacc[r] = mulmod(X[L.perm[i] * rstride + xo], L.sig[i], p, inv);
         }
 #pragma unroll 2
         for (uint32_t q = 0; q < rank; q++) {
@@ -326,29 +328,40 @@ __device__ __forceinline__ uint32_t leaf_lsp_any(const LeafLds &L, uint32_t m, u
     return leaf_lsp<1, 4>(L, m, n, p);
 }
 
-__device__ __forceinline__ uint32_t leaf_eliminate(const LeafLds &L, uint32_t m, uint32_t n, uint32_t p, double inv) {
+// The elimination and what the advance needs from it.  normalize: Linv =
+// U / sigma (one inversion per row; a single-step leaf writes its basis from
+// it).  Otherwise rows stay scaled: the step basis of the scaled rows is row
+// i = sigma_i e_perm(i) + U_i (the fraction-free result itself), which is
+// sigma_i c_perm(i) times the true step applied to the true rows when row r
+// is kept as c_r times the true one; the new factors go to fcol.
+__device__ __forceinline__ uint32_t leaf_eliminate(const LeafLds &L, uint32_t m, uint32_t n, uint32_t p, double inv,
+                                                   bool normalize) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     const uint32_t rank = leaf_lsp_any(L, m, n, p);
-    // Linv row i = e_i + (U row i) / sigma_i over the earlier pivot rows
     for (uint32_t i = tid; i < m; i += nt) {
-        uint32_t sv = L.sig[i], r = 1, e = p - 2;
-        while (e) {
-            if (e & 1) r = mulmod(r, sv, p, inv);
-            sv = mulmod(sv, sv, p, inv);
-            e >>= 1;
+        if (normalize) {  // Linv row i = e_i + (U row i) / sigma_i over the earlier pivot rows
+            uint32_t sv = L.sig[i], r = 1, e = p - 2;
+            while (e) {
+                if (e & 1) r = mulmod(r, sv, p, inv);
+                sv = mulmod(sv, sv, p, inv);
+                e >>= 1;
+            }
+            L.fcol[i] = r;
+        } else {
+            L.fcol[i] = mulmod(L.sig[i], L.cs[L.perm[i]], p, inv);
         }
-        L.fcol[i] = r;
     }
     for (uint32_t q = tid; q < rank; q += nt) {
         L.isx[L.pivrow[q]] = 1;
         L.prow[q] = L.perm[L.pivrow[q]];
     }
     __syncthreads();
-    for (uint32_t idx = tid; idx < m * rank; idx += nt) {
-        const uint32_t i = idx / rank, q = idx % rank;
-        const uint32_t u = L.U[i * n + q];
-        L.U[i * n + q] = L.pivrow[q] < i ? mulmod(u, L.fcol[i], p, inv) : 0u;
-    }
+    // U[i][q] is nonzero only for pivots of earlier rows (pivrow q < i): only they update row i
+    if (normalize)
+        for (uint32_t idx = tid; idx < m * rank; idx += nt) {
+            const uint32_t i = idx / rank, q = idx % rank;
+            L.U[i * n + q] = mulmod(L.U[i * n + q], L.fcol[i], p, inv);
+        }
     for (uint32_t i = tid; i < m; i += nt) L.dln[i] = L.dl[L.perm[i]] - (int64_t)L.isx[i];
     __syncthreads();
     return rank;
@@ -390,7 +403,7 @@ __global__ __launch_bounds__(LEAF_THREADS) void k_mbasis_iter(LeafPack pk, uint3
             L.U[idx] = 0;
         }
         __syncthreads();
-        const uint32_t rank = leaf_eliminate(L, m, n, p, inv);
+        const uint32_t rank = leaf_eliminate(L, m, n, p, inv, true);
         for (uint32_t idx = tid; idx < m * m; idx += nt) {
             D.M[(size_t)idx * 2] = 0;
             D.M[(size_t)idx * 2 + 1] = 0;
@@ -414,6 +427,7 @@ __global__ __launch_bounds__(LEAF_THREADS) void k_mbasis_iter(LeafPack pk, uint3
         const uint32_t ij = idx / MS, k = idx - ij * MS;
         L.M[idx] = (k == 0 && ij / m == ij % m) ? 1u : 0u;
     }
+    for (uint32_t i = tid; i < m; i += nt) L.cs[i] = 1;
     __syncthreads();
     for (uint32_t s = 0; s < K; s++) {
         leaf_sort(L, m);
@@ -423,15 +437,29 @@ __global__ __launch_bounds__(LEAF_THREADS) void k_mbasis_iter(LeafPack pk, uint3
             L.U[idx] = 0;
         }
         __syncthreads();
-        const uint32_t rank = leaf_eliminate(L, m, n, p, inv);
+        const uint32_t rank = leaf_eliminate(L, m, n, p, inv, false);
         // P <- T P on the coefficients later steps read (s .. K-1 in, s+1 .. K-1 out)
         if (s + 1 < K) leaf_advance(L, L.P, n, K, s, K - 1, s + 1, K - 1, rank, m, n, p, pd, inv);
         // M <- T M (degree s -> s + 1)
         leaf_advance(L, L.M, m, MS, 0, s, 0, s + 1, rank, m, n, p, pd, inv);
-        for (uint32_t i = tid; i < m; i += nt) L.dl[i] = L.dln[i];
+        for (uint32_t i = tid; i < m; i += nt) {
+            L.dl[i] = L.dln[i];
+            L.cs[i] = L.fcol[i];
+        }
         __syncthreads();
     }
-    for (uint32_t idx = tid; idx < m * m * MS; idx += nt) D.M[idx] = L.M[idx];
+    // the true basis: row i of M divided by its factor
+    for (uint32_t i = tid; i < m; i += nt) {
+        uint32_t sv = L.cs[i], r = 1, e = p - 2;
+        while (e) {
+            if (e & 1) r = mulmod(r, sv, p, inv);
+            sv = mulmod(sv, sv, p, inv);
+            e >>= 1;
+        }
+        L.fcol[i] = r;
+    }
+    __syncthreads();
+    for (uint32_t idx = tid; idx < m * m * MS; idx += nt) D.M[idx] = mulmod(L.M[idx], L.fcol[idx / (m * MS)], p, inv);
     for (uint32_t i = tid; i < m; i += nt) D.delta[i] = L.dl[i];
 }
 
