@@ -7,14 +7,17 @@
 
 namespace rr {
 
-// Steps per leaf: the in-leaf basis update costs m^2 rank per coefficient,
-// so wider blocks take shorter leaves.
-__host__ __device__ constexpr uint32_t leaf_order(uint32_t m) { return m <= 16 ? 16 : m <= 32 ? 8 : m <= 64 ? 4 : 1; }
+// Steps per leaf in batches and in the server: the in-leaf basis update
+// costs m^2 rank per coefficient, and P and M of a leaf must fit in LDS, so
+// wider blocks take shorter leaves (m = 64: 4, the most that fits).  With
+// many jobs sharing rounds, long leaves mean fewer rounds and fewer products
+// (m = 32: 16 steps made 512 batched matrices 10% faster than 8).
+__host__ __device__ constexpr uint32_t leaf_order(uint32_t m) { return m <= 32 ? 16 : m <= 64 ? 4 : 1; }
 // A PM-basis running alone is a chain of leaf launches, and shorter leaves
 // (less basis update per coefficient) finish it sooner; in batches and in
 // the server the leaves of many jobs share rounds, and longer leaves mean
 // fewer rounds.  The basis is the same either way.
-__host__ __device__ constexpr uint32_t leaf_order_alone(uint32_t m) { return m <= 64 ? leaf_order(m) / 2 : 1; }
+__host__ __device__ constexpr uint32_t leaf_order_alone(uint32_t m) { return m <= 16 ? 8 : m <= 32 ? 4 : m <= 64 ? 2 : 1; }
 
 struct LeafDesc {
     const uint32_t *G;  // residual G(i, j, k) at G[(i * n + j) * gs + k], k < d
