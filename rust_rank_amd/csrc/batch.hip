@@ -779,15 +779,15 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
 // so what a width costs is the device time its work takes among the others
 // (throughput), not its latency alone.  Measured once per process on the
 // device, with copies of each probe running together:
-//   c_b   Krylov time per nonzero and tick for width b: TUNE_KCOPIES copies of
-//         a probe matrix with +-1/+-2 entries ticking together, 16 ticks
-//         timed with events, divided by the copies' nonzeros;
+//   c_b, f_b  Krylov time per tick and matrix, c_b per nonzero plus f_b:
+//         TUNE_KCOPIES copies of a probe matrix with +-1/+-2 entries ticking
+//         together, 16 ticks timed with events, at two sizes (TUNE_M rows);
 //   pm_b(d) PM-basis time per job at orders 255 and 1023 for width b:
 //         TUNE_COPIES jobs of random sequences in one batched scheduler run,
 //         divided by the jobs, fitted as alpha_b d + beta_b d log2 d (leaves
 //         are linear in d, the products of the d&c add d log d).
 // Predicted device time of matrix i with width b:
-//   L(b) * nnz_i * c_b + pm_b(d(b)),
+//   L(b) * (nnz_i * c_b + f_b) + pm_b(d(b)),
 //   L(b) = 2 min(m, n) / b + 8, d(b) = L(b) - 1.
 constexpr uint32_t TUNE_KCOPIES = 32, TUNE_COPIES = 8;
 
@@ -796,16 +796,14 @@ namespace rr {
 struct TuneProbe {
     bool done = false;
     double c[3] = {0, 0, 0}, pm[3] = {0, 0, 0};
-    double pm_at[3][2] = {}, alpha[3] = {0, 0, 0}, beta[3] = {0, 0, 0};
+    double pm_at[3][2] = {}, alpha[3] = {0, 0, 0}, beta[3] = {0, 0, 0}, f[3] = {0, 0, 0};
 };
 static const uint64_t TUNE_D[2] = {255, 1023};
+static const uint64_t TUNE_M[2] = {24000, 3000};
 static TuneProbe g_probe;
 static const uint32_t TUNE_B[3] = {8, 16, 32};
 
 static rr_status run_probe(uint32_t p, int cus, hipStream_t st) {
-    const uint64_t m = 24000, n = 21600, k = 11;
-    std::vector<uint64_t> rp(m + 1);
-    std::vector<uint32_t> col(m * k), val(m * k);
     uint64_t x = 0x9E3779B97F4A7C15ull;
     auto rnd = [&]() {
         x ^= x << 13;
@@ -813,31 +811,33 @@ static rr_status run_probe(uint32_t p, int cus, hipStream_t st) {
         x ^= x << 17;
         return x;
     };
-    for (uint64_t r = 0; r < m; r++) {
-        rp[r] = r * k;
-        for (uint64_t e = 0; e < k; e++) {
-            col[r * k + e] = (uint32_t)(rnd() % n);
-            uint32_t mag = (rnd() & 3) == 0 ? 2 : 1;
-            val[r * k + e] = (rnd() & 1) ? p - mag : mag;
-        }
-    }
-    rp[m] = m * k;
-    std::vector<uint32_t> dr(m, 3), dc(n, 5), V(n * 32);
-    for (auto &v : V) v = (uint32_t)(rnd() % p);
-    rr_csr A{m, n, rp.data(), col.data(), val.data()};
     hipEvent_t e0, e1;
     RR_HIP(hipEventCreate(&e0));
     RR_HIP(hipEventCreate(&e1));
-    rr_status s;
-    for (int i = 0; i < 3; i++) {
-        uint32_t b = TUNE_B[i];
-        std::vector<uint32_t> Vb(n * b);
-        for (uint64_t r = 0; r < n; r++) std::memcpy(&Vb[r * b], &V[r * 32], b * 4);
+    // ms per tick and copy: TUNE_KCOPIES copies of an m x 0.9m matrix with 11
+    // +-1/+-2 entries per row ticking together
+    auto krylov_probe = [&](uint64_t m, uint32_t b, double *ms_tick) -> rr_status {
+        const uint64_t n = m * 9 / 10, k = 11;
+        std::vector<uint64_t> rp(m + 1);
+        std::vector<uint32_t> col(m * k), val(m * k);
+        for (uint64_t r = 0; r < m; r++) {
+            rp[r] = r * k;
+            for (uint64_t e = 0; e < k; e++) {
+                col[r * k + e] = (uint32_t)(rnd() % n);
+                uint32_t mag = (rnd() & 3) == 0 ? 2 : 1;
+                val[r * k + e] = (rnd() & 1) ? p - mag : mag;
+            }
+        }
+        rp[m] = m * k;
+        std::vector<uint32_t> dr(m, 3), dc(n, 5), V(n * b);
+        for (auto &v : V) v = (uint32_t)(rnd() % p);
+        rr_csr A{m, n, rp.data(), col.data(), val.data()};
+        rr_status s;
         std::vector<std::unique_ptr<Request>> R(TUNE_KCOPIES);
         std::vector<Request *> all;
         for (auto &r : R) {
             r.reset(new Request());
-            if ((s = r->init(&A, p, b, dr.data(), dc.data(), Vb.data(), 40, 39, 2048, st)) != RR_OK) return s;
+            if ((s = r->init(&A, p, b, dr.data(), dc.data(), V.data(), 40, 39, 2048, st)) != RR_OK) return s;
             all.push_back(r.get());
         }
         TickRunner T;
@@ -850,7 +850,19 @@ static rr_status run_probe(uint32_t p, int cus, hipStream_t st) {
         RR_HIP(hipEventSynchronize(e1));
         float ms = 0;
         RR_HIP(hipEventElapsedTime(&ms, e0, e1));
-        g_probe.c[i] = (ms / 16.0) / ((double)TUNE_KCOPIES * (double)(m * k));
+        *ms_tick = ms / 16.0 / TUNE_KCOPIES;
+        return RR_OK;
+    };
+    rr_status s;
+    for (int i = 0; i < 3; i++) {
+        uint32_t b = TUNE_B[i];
+        // cost per tick = c nnz + f: c from the difference of two sizes, f the rest
+        double t_big = 0, t_small = 0;
+        if ((s = krylov_probe(TUNE_M[0], b, &t_big)) != RR_OK) return s;
+        if ((s = krylov_probe(TUNE_M[1], b, &t_small)) != RR_OK) return s;
+        const double nz_big = 11.0 * TUNE_M[0], nz_small = 11.0 * TUNE_M[1];
+        g_probe.c[i] = std::max(0.0, (t_big - t_small) / (nz_big - nz_small));
+        g_probe.f[i] = std::max(0.0, t_small - g_probe.c[i] * nz_small);
         // PM-basis probe: TUNE_COPIES jobs of random sequences at each order of TUNE_D
         const uint64_t P = b * (b + 1) / 2;
         for (int o = 0; o < 2; o++) {
@@ -924,7 +936,8 @@ rr_status rr_autotune_b(uint32_t count, const rr_csr *mats, uint32_t p, uint32_t
             uint32_t b = TUNE_B[k];
             double L = 2.0 * (double)mn / b + 8, d = L - 1;
             if (d + 1 > (double)(1u << NTT_MAX_LOGN)) continue;
-            double t = L * (double)nnz * g_probe.c[k] + d * (g_probe.alpha[k] + g_probe.beta[k] * std::log2(d));
+            double t = L * ((double)nnz * g_probe.c[k] + g_probe.f[k]) +
+                       d * (g_probe.alpha[k] + g_probe.beta[k] * std::log2(d));
             if (t < best) {
                 best = t;
                 bb = b;
