@@ -192,13 +192,15 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
 
 /*
  * Autotuner for the batched and served modes: picks the block width b (and
- * with it rows per wavefront, 64/b) from {8, 16, 32} for each matrix,
- * minimising predicted device time from probes measured once per process on
- * this GPU, each with 8 copies running together as in a batch (Krylov time
- * per nonzero and tick per width, PM-basis time per job at order 255 per
- * width).  p must be 2013265921.  pred_ms (may be NULL) receives the
- * prediction (device time the matrix takes in a batch).  rr_autotune_probe
- * returns the measured probe constants for b = 8, 16, 32.
+ * with it rows per wavefront, 64/b) from {8, 16, 32} for each matrix from
+ * the device time predicted by probes measured once per process on this GPU
+ * with copies running together as in a batch (Krylov time per tick: per
+ * nonzero and per matrix, from 32 copies at two sizes; PM-basis time per
+ * job at orders 255 and 1023, from 8 jobs).  b = 16 unless another width is
+ * predicted at least 25% faster.  p must be 2013265921.  pred_ms (may be
+ * NULL) receives the prediction for the chosen width.  rr_autotune_probe
+ * returns the Krylov time per nonzero and the PM-basis time at order 255
+ * for b = 8, 16, 32.
  */
 rr_status rr_autotune_b(uint32_t count, const rr_csr *mats, uint32_t p, uint32_t *b_out, double *pred_ms);
 rr_status rr_autotune_probe(double *c_ms_per_nnz, double *pm_ms_d255);

@@ -800,6 +800,7 @@ struct TuneProbe {
 };
 static const uint64_t TUNE_D[2] = {255, 1023};
 static const uint64_t TUNE_M[2] = {24000, 3000};
+constexpr double TUNE_MARGIN = 0.75;
 static TuneProbe g_probe;
 static const uint32_t TUNE_B[3] = {8, 16, 32};
 
@@ -927,24 +928,25 @@ rr_status rr_autotune_b(uint32_t count, const rr_csr *mats, uint32_t p, uint32_t
         (void)hipStreamDestroy(st);
         if (s != RR_OK) return s;
     }
+    // b = 16 unless another width is predicted TUNE_MARGIN faster: the probes
+    // vary by 10-20 % between runs, and a width chosen on noise costs the
+    // whole batch (b = 8 doubles a sequence's ticks)
     for (uint32_t i = 0; i < count; i++) {
         const rr_csr &A = mats[i];
         uint64_t nnz = A.row_ptr[A.m], mn = std::min(A.m, A.n);
-        double best = 1e300;
-        uint32_t bb = TUNE_B[0];
+        double t[3];
         for (int k = 0; k < 3; k++) {
-            uint32_t b = TUNE_B[k];
-            double L = 2.0 * (double)mn / b + 8, d = L - 1;
-            if (d + 1 > (double)(1u << NTT_MAX_LOGN)) continue;
-            double t = L * ((double)nnz * g_probe.c[k] + g_probe.f[k]) +
-                       d * (g_probe.alpha[k] + g_probe.beta[k] * std::log2(d));
-            if (t < best) {
-                best = t;
-                bb = b;
-            }
+            const double L = 2.0 * (double)mn / TUNE_B[k] + 8, d = L - 1;
+            t[k] = d + 1 > (double)(1u << NTT_MAX_LOGN)
+                       ? 1e300
+                       : L * ((double)nnz * g_probe.c[k] + g_probe.f[k]) +
+                             d * (g_probe.alpha[k] + g_probe.beta[k] * std::log2(d));
         }
-        b_out[i] = bb;
-        if (pred_ms) pred_ms[i] = best;
+        int kb = 1;
+        for (int k = 0; k < 3; k++)
+            if (k != 1 && t[k] < TUNE_MARGIN * t[1] && t[k] < t[kb]) kb = k;
+        b_out[i] = TUNE_B[kb];
+        if (pred_ms) pred_ms[i] = t[kb];
     }
     return RR_OK;
 }
