@@ -20,7 +20,7 @@ struct NttTables {
 
 // Descriptor packs: one launch runs up to NTT_PACK independent transforms or
 // pointwise products of the same size (blockIdx selects the descriptor).
-constexpr uint32_t NTT_PACK = 32;
+constexpr uint32_t NTT_PACK = 96;
 struct NttFwdDesc {  // dst[q][0..N) = spectrum of src[q * sstride + soff + k], k < len, q < count
     const uint32_t *src;
     uint64_t sstride;

