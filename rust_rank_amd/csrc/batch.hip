@@ -171,7 +171,8 @@ uint64_t request_bytes(uint64_t m, uint64_t n, uint64_t nnz, uint32_t b, uint64_
                      + B * (2 * n + m)                                                   // z, x0, y
                      + L * P                                                             // sequence
                      + (uint64_t)2 * b * b * (d + 1) + (uint64_t)4 * b * b * (d + 1)     // G, M
-                     + (uint64_t)2 * (4ull * b * b * (d / 2 + 2) + 2ull * b * b * (d / 2 + 2)) + 2 * b * b + 64;
+                     + (uint64_t)2 * (4ull * b * b * (d / 2 + 2) + 2ull * b * b * (d / 2 + 2)) + 2 * b * b + 64
+                     + (uint64_t)2 * 4ull * b * b * 2 * (d + 1);  // kept spectra along the recursion
     return words * 4;
 }
 

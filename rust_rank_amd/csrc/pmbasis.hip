@@ -638,16 +638,31 @@ static inline uint64_t split_left(uint64_t d, uint64_t K) {
 
 // Workspace words along the deepest path of the recursion below an order-d
 // node with leaves of order K.
+// Transform size of both products of an order-d node (the middle product
+// M1 G and M2 M1 both need d + 1 coefficients), and whether the node keeps
+// M1's spectrum for the second: only where the first product is an NTT one
+// in any round (la lb above the direct-product limits).
+constexpr size_t PM_MAX_DEPTH = 64;  // recursion frames per job (orders below 2^20 need about 20)
+
+static uint32_t node_logn(uint64_t d) {
+    uint32_t l = 1;
+    while ((1ull << l) < d + 1) l++;
+    return l;
+}
+static bool node_keeps(uint64_t d, uint64_t d1) { return (d1 + 1) * (d + 1) > 256 && node_logn(d) <= NTT_MAX_LOGN; }
+
 static uint64_t rec_words(uint64_t d, uint64_t K, uint32_t m, uint32_t b) {
     if (d <= K) return 0;
     const uint64_t d1 = split_left(d, K), d2 = d - d1;
     uint64_t here = ((uint64_t)m * m * (d1 + 1) + 63) + ((uint64_t)m * b * (d2 + 1) + 63) +
                     ((uint64_t)m * m * (d2 + 1) + 63);
+    if (node_keeps(d, d1)) here += ((uint64_t)m * m << node_logn(d)) + 63;
     return here + std::max(rec_words(d1, K, m, b), rec_words(d2, K, m, b));
 }
 
 struct Op {
     int kind;  // 0 leaf, 1 product
+    Frame *keep = nullptr, *given = nullptr;  // spectrum reuse (products)
     LeafDesc leaf;
     MulDesc mul;
 };
@@ -675,12 +690,19 @@ static bool next_op(PmJob &J, Op &op, rr_status &err) {
                 F.M1 = J.pool.get((size_t)J.m * J.m * (d1 + 1));
                 F.GG = J.pool.get((size_t)J.m * J.b * (d2 + 1));
                 F.M2 = J.pool.get((size_t)J.m * J.m * (d2 + 1));
-                if (!F.M1 || !F.GG || !F.M2) {
+                F.SP = node_keeps(F.d, d1) ? J.pool.get((size_t)J.m * J.m << node_logn(F.d)) : nullptr;
+                F.sp_logn = -1;
+                if (!F.M1 || !F.GG || !F.M2 || (node_keeps(F.d, d1) && !F.SP)) {
                     set_error("PM-basis workspace exhausted");
                     err = RR_ENOMEM;
                     return false;
                 }
                 F.stage = 1;
+                if (J.stack.size() >= J.stack.capacity()) {
+                    set_error("PM-basis recursion deeper than %zu levels", J.stack.capacity());
+                    err = RR_EINVAL;
+                    return false;
+                }
                 Frame child{F.G, F.gs, d1, F.M1, 0, nullptr, nullptr, nullptr, 0};
                 J.stack.push_back(child);
                 continue;
@@ -690,9 +712,16 @@ static bool next_op(PmJob &J, Op &op, rr_status &err) {
                 op.kind = 1;
                 op.mul = MulDesc{F.M1, F.G, F.GG, F.gs, (uint32_t)(d1 + 1), (uint32_t)(F.d + 1), 0,
                                  (uint32_t)d1, (uint32_t)(F.d + 1), J.m, J.m, J.b};
+                op.keep = F.SP ? &F : nullptr;
+                op.given = nullptr;
                 return true;
             case 2: {
                 F.stage = 3;
+                if (J.stack.size() >= J.stack.capacity()) {
+                    set_error("PM-basis recursion deeper than %zu levels", J.stack.capacity());
+                    err = RR_EINVAL;
+                    return false;
+                }
                 Frame child{F.GG, d2 + 1, d2, F.M2, 0, nullptr, nullptr, nullptr, 0};
                 J.stack.push_back(child);
                 continue;
@@ -702,6 +731,8 @@ static bool next_op(PmJob &J, Op &op, rr_status &err) {
                 op.kind = 1;
                 op.mul = MulDesc{F.M2, F.M1, F.M, d1 + 1, (uint32_t)(d2 + 1), (uint32_t)(d1 + 1), 0, 0,
                                  (uint32_t)(F.d + 1), J.m, J.m, J.m};
+                op.keep = nullptr;
+                op.given = F.sp_logn >= 0 ? &F : nullptr;
                 return true;
             default:
                 J.pool.top = F.mark;
@@ -774,6 +805,7 @@ rr_status PmScheduler::add(PmJob *J) {
     if ((s = J->pool.init(rec_words(J->d, J->K, J->m, J->b) + 1024)) != RR_OK) return s;
     RR_HIP(hipMemsetAsync(J->delta, 0, (size_t)J->m * 8, st));
     J->stack.clear();
+    J->stack.reserve(PM_MAX_DEPTH);  // frames hold pointers to each other's spectra: no reallocation
     J->stack.push_back(Frame{J->G, J->d + 1, J->d, J->M, 0, nullptr, nullptr, nullptr, 0});
     J->done = false;
     J->has_op = false;
@@ -785,7 +817,7 @@ rr_status PmScheduler::round(std::vector<PmJob *> &jobs, bool *launched) {
     const ModP mp = ModP::make(p);
     std::vector<LeafDesc> leaves;
     std::vector<MulDesc> school;
-    std::vector<std::pair<uint32_t, MulDesc>> ntt;
+    std::vector<NttProduct> ntt;
     *launched = false;
     // Products go first and leaves wait for a round in which no job has a
     // product pending, so that the leaves of many jobs share one launch (a
@@ -804,6 +836,8 @@ rr_status PmScheduler::round(std::vector<PmJob *> &jobs, bool *launched) {
             J->op_kind = op.kind;
             J->op_leaf = op.leaf;
             J->op_mul = op.mul;
+            J->op_keep = op.keep;
+            J->op_given = op.given;
         }
         any_mul |= J->op_kind == 1;
     }
@@ -813,9 +847,23 @@ rr_status PmScheduler::round(std::vector<PmJob *> &jobs, bool *launched) {
     for (auto *J : jobs) {
         if (J->done || !J->has_op || J->op_kind != (any_mul ? 1 : 0)) continue;
         J->has_op = false;
-        if (J->op_kind == 0) leaves.push_back(J->op_leaf);
-        else if (use_school(J->op_mul, p, nmul)) school.push_back(J->op_mul);
-        else ntt.push_back({ntt_logn_for(J->op_mul), J->op_mul});
+        if (J->op_kind == 0) {
+            leaves.push_back(J->op_leaf);
+        } else if (use_school(J->op_mul, p, nmul)) {
+            school.push_back(J->op_mul);
+            if (J->op_keep) J->op_keep->sp_logn = -1;  // nothing kept: the later product transforms M1 itself
+        } else {
+            NttProduct o{ntt_logn_for(J->op_mul), J->op_mul, nullptr, nullptr};
+            if (J->op_keep && o.logn == node_logn(J->op_keep->d)) {
+                o.keep = J->op_keep->SP;
+                J->op_keep->sp_logn = (int)o.logn;
+            } else if (J->op_keep) {
+                J->op_keep->sp_logn = -1;
+            }
+            if (J->op_given && J->op_given->sp_logn == (int)o.logn) o.given = J->op_given->SP;
+            ntt.push_back(o);
+        }
+        J->op_keep = J->op_given = nullptr;
     }
     if (leaves.empty() && school.empty() && ntt.empty()) return RR_OK;
     *launched = true;
@@ -866,40 +914,47 @@ rr_status PmScheduler::run(std::vector<PmJob *> &jobs) {
 
 // Each NTT product: forward transforms of both operands into round scratch,
 // pointwise products (with 1/N), inverse transforms into the output window.
-rr_status PmScheduler::run_ntt_products(std::vector<std::pair<uint32_t, MulDesc>> &ops) {
+rr_status PmScheduler::run_ntt_products(std::vector<NttProduct> &ops) {
     size_t words = 0;
     for (auto &o : ops) {
-        size_t N = (size_t)1 << o.first;
-        const MulDesc &D = o.second;
-        words += (size_t)(D.rows * D.inner + D.inner * D.cols + D.rows * D.cols) * N + 192;
+        size_t N = (size_t)1 << o.logn;
+        const MulDesc &D = o.D;
+        words += (size_t)((o.keep ? 0 : D.rows * D.inner) + (o.given ? 0 : D.inner * D.cols) + D.rows * D.cols) * N +
+                 192;
     }
     rr_status s;
     if ((s = ensure_scratch(words)) != RR_OK) return s;
     // same-size products share their launches (forward, pointwise, inverse)
     std::stable_sort(ops.begin(), ops.end(),
-                     [](const std::pair<uint32_t, MulDesc> &x, const std::pair<uint32_t, MulDesc> &y) {
-                         return x.first < y.first;
-                     });
+                     [](const NttProduct &x, const NttProduct &y) { return x.logn < y.logn; });
     size_t off = 0;
     for (size_t g0 = 0; g0 < ops.size();) {
-        const uint32_t logn = ops[g0].first, N = 1u << logn;
+        const uint32_t logn = ops[g0].logn, N = 1u << logn;
         RR_CHECK(logn <= tables.logn, RR_EINVAL, "transform 2^%u above table 2^%u", logn, tables.logn);
         size_t g1 = g0;
-        while (g1 < ops.size() && ops[g1].first == logn && g1 - g0 < NTT_PACK / 2) g1++;
+        while (g1 < ops.size() && ops[g1].logn == logn && g1 - g0 < NTT_PACK / 2) g1++;
         NttFwdPack fp;
         PwPack pp;
         NttInvPack ip;
         uint32_t nf = 0, np = 0;
         for (size_t k = g0; k < g1; k++) {
-            const MulDesc &D = ops[k].second;
-            uint32_t *fa = scratch.as<uint32_t>() + off;
-            off += ((size_t)D.rows * D.inner * N + 63) & ~size_t(63);
-            uint32_t *fb = scratch.as<uint32_t>() + off;
-            off += ((size_t)D.inner * D.cols * N + 63) & ~size_t(63);
+            const MulDesc &D = ops[k].D;
+            uint32_t *fa = ops[k].keep;
+            if (!fa) {
+                fa = scratch.as<uint32_t>() + off;
+                off += ((size_t)D.rows * D.inner * N + 63) & ~size_t(63);
+            }
+            const uint32_t *fb = ops[k].given;
+            uint32_t *fbw = nullptr;
+            if (!fb) {
+                fbw = scratch.as<uint32_t>() + off;
+                off += ((size_t)D.inner * D.cols * N + 63) & ~size_t(63);
+                fb = fbw;
+            }
             uint32_t *fc = scratch.as<uint32_t>() + off;
             off += ((size_t)D.rows * D.cols * N + 63) & ~size_t(63);
             fp.d[nf++] = NttFwdDesc{D.A, D.la, 0, D.la, fa, D.rows * D.inner};
-            fp.d[nf++] = NttFwdDesc{D.B, D.sb, D.ob, D.lb, fb, D.inner * D.cols};
+            if (fbw) fp.d[nf++] = NttFwdDesc{D.B, D.sb, D.ob, D.lb, fbw, D.inner * D.cols};
             pp.d[np] = PwDesc{fa, fb, fc, D.rows, D.inner, D.cols, h_invmod(N, NTT_PRIME)};
             ip.d[np] = NttInvDesc{fc, D.rows * D.cols, D.lo, D.hi - D.lo, D.C, D.hi - D.lo};
             np++;
@@ -1165,7 +1220,7 @@ rr_status rr_poly_mat_mul(const uint32_t *A, uint64_t la, const uint32_t *B, uin
         uint32_t logn = ntt_logn_for(D);
         RR_CHECK(logn <= NTT_MAX_LOGN, RR_EINVAL, "product too long");
         if ((s = S.tables.init(logn, st)) != RR_OK) return s;
-        std::vector<std::pair<uint32_t, MulDesc>> ops{{logn, D}};
+        std::vector<NttProduct> ops{{logn, D, nullptr, nullptr}};
         if ((s = S.run_ntt_products(ops)) != RR_OK) return s;
     }
     RR_HIP(hipMemcpyAsync(C, dC.ptr, sc * 4, hipMemcpyDeviceToHost, st));

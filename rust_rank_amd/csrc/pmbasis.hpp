@@ -50,6 +50,15 @@ struct Pool {  // bump allocator over one device buffer
     }
 };
 
+// An NTT product of a round: transform size, operands, and where its A
+// spectrum is kept (keep) or its B spectrum taken from (given); both may be null.
+struct NttProduct {
+    uint32_t logn;
+    MulDesc D;
+    uint32_t *keep;
+    const uint32_t *given;
+};
+
 struct Frame {
     const uint32_t *G;
     uint64_t gs, d;
@@ -57,6 +66,10 @@ struct Frame {
     int stage;
     uint32_t *M1, *GG, *M2;
     size_t mark;
+    // M1's spectrum, kept from the product M1 G for the product M2 M1 (both
+    // transform at the same size): SP [m][m][2^sp_logn], valid when sp_logn >= 0
+    uint32_t *SP = nullptr;
+    int sp_logn = -1;
 };
 
 struct PmJob {
@@ -74,6 +87,8 @@ struct PmJob {
     int op_kind = 0;  // 0 leaf, 1 product
     LeafDesc op_leaf{};
     MulDesc op_mul{};
+    Frame *op_keep = nullptr;   // product keeps its A spectrum in this frame's SP
+    Frame *op_given = nullptr;  // product takes its B spectrum from this frame's SP
 };
 
 struct PmScheduler {
@@ -89,7 +104,7 @@ struct PmScheduler {
     rr_status add(PmJob *J);                                          // job joins; its delta is zeroed
     rr_status round(std::vector<PmJob *> &jobs, bool *launched);     // next products, else next leaves
     rr_status run(std::vector<PmJob *> &jobs);                        // init + add + rounds to completion
-    rr_status run_ntt_products(std::vector<std::pair<uint32_t, MulDesc>> &ops);
+    rr_status run_ntt_products(std::vector<NttProduct> &ops);
 };
 
 rr_status pm_basis_batch(uint32_t p, std::vector<PmJob *> &jobs, hipStream_t st, uint64_t *rounds,
