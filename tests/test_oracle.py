@@ -294,3 +294,15 @@ def test_plain_small_rank():
     basis, delta = O.pm_basis(seq, b, L - 1, P)
     ge = O.dense_rank(A.to_dense(P), P)
     assert O.rank_from_basis(basis, delta, b, P) == ge
+
+
+def test_batch_matrices_parallel_equals_serial():
+    """The batched workload generated by worker processes is the same
+    matrices as generated one by one (the bench and the GPU tests rely on it)."""
+    par = synth.batch_matrices(4, 20, workers=2)
+    ser = [synth.batch_matrix(4, i) for i in range(20)]
+    for A, B in zip(par, ser):
+        assert (A.m, A.n) == (B.m, B.n)
+        assert np.array_equal(A.row_ptr, B.row_ptr)
+        assert np.array_equal(A.col, B.col)
+        assert np.array_equal(A.val, B.val)
