@@ -497,28 +497,64 @@ rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> 
         uint32_t nblk;
         uint32_t seg;    // accumulator row of a segment, or NO_ROW for a row slice
         uint32_t lrow;   // segments: index of the split row
-        uint32_t first;  // row slices: index into shortr; segments: first nonzero
-        uint32_t end;    // segments: end of the segment's nonzeros
+        uint32_t first;  // row slices: index into shortr; segments: first nonzero in lc
+        uint32_t end;    // segments: end of the segment's nonzeros in lc
+        uint32_t chunk;  // segments: nonzeros per row group
     };
     std::vector<Sl> sls;
     for (size_t i = 0; i < shortr.size(); i += RPW) {
         const uint32_t len = rp[shortr[i] + 1] - rp[shortr[i]];
-        sls.push_back(Sl{(uint32_t)up((len + SU - 1) / SU, 2), NO_ROW, 0, (uint32_t)i, 0});
+        sls.push_back(Sl{(uint32_t)up((len + SU - 1) / SU, 2), NO_ROW, 0, (uint32_t)i, 0, 0});
     }
+    // Long rows' nonzeros in order of their gathered rows (segment sums are
+    // exact, so the order does not change the result), cut into segments of
+    // at most seg_nnz nonzeros.  With XCD placement a segment stays inside one
+    // eighth of the gathered vector unless that eighth holds fewer than
+    // min_piece of the row's nonzeros, in which case it runs on into the next.
+    const bool xcd = nw % (NXCD * (WG / 64)) == 0;
+    const uint32_t mask = packed ? 0x0FFFFFFFu : 0xFFFFFFFFu;
     const uint64_t seg_nnz = (uint64_t)RPW * chunk;
+    const uint64_t min_piece = seg_nnz / 4;
+    const uint64_t in_n = std::max<uint64_t>(in_rows, 1);
+    std::vector<uint32_t> lc, lv, idx;
     uint32_t nseg = 0;
     uint32_t nacc = 0;
     for (size_t l = 0; l < lrows.size(); l++) {
         const uint32_t r = lrows[l];
-        const uint64_t ns = (rp[r + 1] - rp[r] + seg_nnz - 1) / seg_nnz;
+        const uint64_t base = lc.size(), end = base + (rp[r + 1] - rp[r]);
+        idx.resize(end - base);
+        for (size_t i = 0; i < idx.size(); i++) idx[i] = rp[r] + (uint32_t)i;
+        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) { return (colw[x] & mask) < (colw[y] & mask); });
+        for (uint32_t e : idx) {
+            lc.push_back(colw[e]);
+            if (!packed) lv.push_back(vals[e]);
+        }
+        std::vector<std::pair<uint64_t, uint64_t>> cuts;  // segments [first, end) in lc
+        for (uint64_t e = base; e < end;) {
+            uint64_t pe = end;
+            if (xcd) {
+                uint64_t x = std::min<uint64_t>(NXCD - 1, (uint64_t)(lc[e] & mask) * NXCD / in_n);
+                do {
+                    const uint64_t bound = (x + 1) * in_n;  // first column c of the next eighth: c * NXCD >= bound
+                    pe = x + 1 < NXCD ? (uint64_t)(std::lower_bound(lc.begin() + e, lc.begin() + end, bound,
+                                                                    [&](uint32_t c, uint64_t b) { return (uint64_t)(c & mask) * NXCD < b; }) -
+                                                   lc.begin())
+                                      : end;
+                    x++;
+                } while (pe < end && pe - e < min_piece);
+                if (end - pe < min_piece) pe = end;
+            }
+            const uint64_t ns = (pe - e + seg_nnz - 1) / seg_nnz, ls = (pe - e + ns - 1) / ns;
+            for (uint64_t s = e; s < pe; s += ls) cuts.push_back({s, std::min(pe, s + ls)});
+            e = pe;
+        }
+        const uint64_t ns = cuts.size();
         const uint32_t ncopy = (uint32_t)std::min<uint64_t>(MAX_COPIES, ns);
         linfo.push_back(make_uint4(r, nacc, ncopy, (uint32_t)ns));
-        uint32_t si = 0;
-        for (uint64_t e = rp[r]; e < rp[r + 1]; e += seg_nnz, si++) {
-            const uint64_t se = std::min<uint64_t>(rp[r + 1], e + seg_nnz);
-            const uint64_t glen = std::min<uint64_t>(chunk, se - e);
-            sls.push_back(Sl{(uint32_t)up((glen + SU - 1) / SU, 2), nacc + si % ncopy, (uint32_t)l, (uint32_t)e,
-                             (uint32_t)se});
+        for (uint64_t si = 0; si < ns; si++) {
+            const uint64_t len = cuts[si].second - cuts[si].first, c = (len + RPW - 1) / RPW;
+            sls.push_back(Sl{(uint32_t)up((c + SU - 1) / SU, 2), nacc + (uint32_t)(si % ncopy), (uint32_t)l,
+                             (uint32_t)cuts[si].first, (uint32_t)cuts[si].second, (uint32_t)c});
             nseg++;
         }
         nacc += ncopy;
@@ -536,18 +572,15 @@ rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> 
     // so each XCD's L2 serves one eighth of the vector to its segments.
     // At most MAX_PEND segment slices per wavefront (a wavefront holding that
     // many is set aside for the remaining segments of its group).
-    const bool xcd = nw % (NXCD * (WG / 64)) == 0;
     const uint32_t ngroups = xcd ? NXCD : 1;
     std::vector<std::priority_queue<Load, std::vector<Load>, std::greater<Load>>> xh(ngroups);
     for (uint32_t w = 0; w < nw; w++) xh[xcd ? (w / (WG / 64)) % NXCD : 0].push(Load(0, w));
     std::vector<uint32_t> nsegw(nw, 0);
-    const uint32_t mask = packed ? 0x0FFFFFFFu : 0xFFFFFFFFu;
     for (uint32_t i : order) {
         const Sl &S = sls[i];
         if (S.seg == NO_ROW) continue;
-        const uint64_t mid = ((uint64_t)(colw[S.first] & mask) + (colw[S.end - 1] & mask)) / 2;
-        const uint32_t x =
-            xcd ? (uint32_t)std::min<uint64_t>(NXCD - 1, mid * NXCD / std::max<uint64_t>(in_rows, 1)) : 0;
+        const uint64_t mid = ((uint64_t)(lc[S.first] & mask) + (lc[S.end - 1] & mask)) / 2;
+        const uint32_t x = xcd ? (uint32_t)std::min<uint64_t>(NXCD - 1, mid * NXCD / in_n) : 0;
         RR_CHECK(!xh[x].empty(), RR_EINVAL, "too many split-row segments for %u wavefronts", nw);
         Load l = xh[x].top();
         xh[x].pop();
@@ -569,12 +602,11 @@ rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> 
     // a wavefront's segment slices in order of their gathered rows, so the
     // wavefronts of an XCD sweep its eighth of the vector together
     if (xcd) {
-        const uint32_t mask = packed ? 0x0FFFFFFFu : 0xFFFFFFFFu;
         for (uint32_t w = 0; w < nw; w++) {
             auto &v = mine[w];
             auto seg_end = std::stable_partition(v.begin(), v.end(), [&](uint32_t i) { return sls[i].seg != NO_ROW; });
             std::stable_sort(v.begin(), seg_end,
-                             [&](uint32_t x, uint32_t y) { return (colw[sls[x].first] & mask) < (colw[sls[y].first] & mask); });
+                             [&](uint32_t x, uint32_t y) { return (lc[sls[x].first] & mask) < (lc[sls[y].first] & mask); });
         }
     }
     std::vector<uint32_t> meta, wsl(nw + 1), wblk(nw + 1);
@@ -613,9 +645,12 @@ rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> 
             const Sl &S = sls[i];
             for (uint32_t g = 0; g < RPW; g++) {
                 uint64_t beg, len;
+                const uint32_t *sc = colw.data(), *sv = vals.data();
                 if (S.seg != NO_ROW) {
-                    beg = (uint64_t)S.first + (uint64_t)g * chunk;
-                    len = beg < S.end ? std::min<uint64_t>(chunk, S.end - beg) : 0;
+                    beg = (uint64_t)S.first + (uint64_t)g * S.chunk;
+                    len = beg < S.end ? std::min<uint64_t>(S.chunk, S.end - beg) : 0;
+                    sc = lc.data();
+                    sv = lv.data();
                 } else {
                     const size_t k = (size_t)S.first + g;
                     if (k >= shortr.size()) continue;
@@ -624,8 +659,8 @@ rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> 
                 }
                 for (uint64_t e = 0; e < len; e++) {
                     const size_t wpos = (blk + e / SU) * BW + g * SU + e % SU;
-                    cw[wpos] = colw[beg + e];
-                    if (!packed) vw[wpos] = vals[beg + e];
+                    cw[wpos] = sc[beg + e];
+                    if (!packed) vw[wpos] = sv[beg + e];
                 }
             }
             blk += S.nblk;
