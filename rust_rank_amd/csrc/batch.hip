@@ -263,18 +263,36 @@ rr_status Request::init(const rr_csr *A, uint32_t p_, uint32_t b_, const uint32_
     return RR_OK;
 }
 
-rr_status Request::start_pm(hipStream_t st, bool release) {
+rr_status Request::alloc_pm() {
     rr_status s;
     if ((s = G.alloc((size_t)2 * b * b * (d + 1) * 4)) != RR_OK) return s;
     if ((s = M.alloc((size_t)4 * b * b * (d + 1) * 4)) != RR_OK) return s;
     if ((s = delta.alloc((size_t)2 * b * 8)) != RR_OK) return s;
     if ((s = rank_in.alloc((size_t)(2 * b * b + 1) * 4)) != RR_OK) return s;
-    if ((s = build_G_device(seq.as<uint32_t>(), b, d + 1, G.as<uint32_t>(), st)) != RR_OK) return s;
     pm.b = b;
     pm.d = d;
     pm.G = G.as<uint32_t>();
     pm.M = M.as<uint32_t>();
     pm.delta = delta.as<int64_t>();
+    pm.avail = 0;
+    return RR_OK;
+}
+
+rr_status Request::grow_G(hipStream_t st) {
+    const uint64_t k1 = std::min<uint64_t>(t, d + 1);
+    if (pm.avail == ~0ull || k1 <= pm.avail) return RR_OK;
+    rr_status s = build_G_range(seq.as<uint32_t>(), b, d + 1, pm.avail, k1, G.as<uint32_t>(), st);
+    if (s != RR_OK) return s;
+    pm.avail = k1 == d + 1 ? ~0ull : k1;
+    return RR_OK;
+}
+
+rr_status Request::start_pm(hipStream_t st, bool release) {
+    rr_status s;
+    RR_CHECK(t > d, RR_EINVAL, "sequence of %llu entries too short for order %llu", (unsigned long long)t,
+             (unsigned long long)d);
+    if (!G.ptr && (s = alloc_pm()) != RR_OK) return s;
+    if ((s = grow_G(st)) != RR_OK) return s;
     stage = STAGE_PM;
     if (!release) return RR_OK;
     // the operands are no longer needed (the stream orders this after their last pass)
@@ -687,6 +705,11 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
                     RR_HIP(hipStreamWaitEvent(st, su.ev[i], 0));
                     joined[i] = 1;
                     njoined++;
+                    Request *r = reqs[i].get();
+                    if (r->d >= BATCH_STREAM_D) {  // its PM-basis starts on the leading coefficients
+                        if ((s = r->alloc_pm()) != RR_OK || (s = S.add(&r->pm)) != RR_OK) return s;
+                        jobs.push_back(&r->pm);
+                    }
                 }
             }
             if (njoined == count) t1 = std::chrono::steady_clock::now();
@@ -718,15 +741,26 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
             if ((s = kq.push(st)) != RR_OK) return s;
             issued = true;
         }
-        // complete sequences: G is built on st, the PM stream waits for it
+        // complete sequences: G is built on st, the PM stream waits for it;
+        // PM-basis jobs already running get the coefficients their next
+        // operation waits for as soon as the sequence has them
         const size_t njobs = jobs.size();
-        for (uint32_t i = 0; i < count; i++)
-            if (Request *r = joined[i] ? reqs[i].get() : nullptr; r && r->stage == STAGE_KRYLOV && r->t >= r->L) {
+        bool grown = false;
+        for (uint32_t i = 0; i < count; i++) {
+            Request *r = joined[i] ? reqs[i].get() : nullptr;
+            if (!r || r->stage != STAGE_KRYLOV) continue;
+            const bool running = r->G.ptr != nullptr;
+            if (r->t >= r->L) {
                 if ((s = r->start_pm(st, false)) != RR_OK) return s;
-                jobs.push_back(&r->pm);
+                if (running) grown = true;
+                else jobs.push_back(&r->pm);
+            } else if (running && r->pm.has_op && r->pm.op_need > r->pm.avail && r->t >= r->pm.op_need) {
+                if ((s = r->grow_G(st)) != RR_OK) return s;
+                grown = true;
             }
+        }
         const bool handed = jobs.size() > njobs;
-        if (handed) {
+        if (handed || grown) {
             RR_HIP(hipEventRecord(handoff, st));
             RR_HIP(hipStreamWaitEvent(st_pm, handoff, 0));
             for (size_t j = njobs; j < jobs.size(); j++)
@@ -741,10 +775,10 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
                 if ((s = pq.push(st_pm)) != RR_OK) return s;
                 issued = true;
             } else {
-                pm_idle = true;  // every joined job is done
+                pm_idle = true;  // every joined job is done or waits for its sequence
             }
         }
-        if (!krylov_done && njoined == count && active.empty() && fresh.empty() && !handed) {
+        if (!krylov_done && njoined == count && active.empty() && fresh.empty() && !handed && !grown) {
             RR_HIP(hipStreamSynchronize(st));
             krylov_done = true;
             t2 = std::chrono::steady_clock::now();

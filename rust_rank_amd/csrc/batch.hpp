@@ -39,6 +39,10 @@ constexpr uint32_t BATCH_TICKS = 16;
 constexpr uint32_t BATCH_PM_AHEAD = 8;
 // Ticks per loop of the server (between two PM-basis rounds).
 constexpr uint32_t SERVER_TICKS = 4;
+// rr_batch_rank starts the PM-basis of a request of at least this order while
+// its sequence is still being produced: the leftmost chain of the recursion
+// reads only leading coefficients of G.
+constexpr uint64_t BATCH_STREAM_D = 1024;
 
 struct Request {
     uint32_t p = 0, b = 0, B = 0, P = 0, su = 4, nwg = 0;
@@ -56,6 +60,10 @@ struct Request {
 
     rr_status init(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *dr, const uint32_t *dc,
                    const uint32_t *V, uint64_t L, uint64_t d, uint32_t tile_rows, hipStream_t st);
+    // PM-basis buffers; the job then reads the coefficients of G built so far
+    rr_status alloc_pm();
+    // G's coefficients for the sequence entries produced so far (t), on st
+    rr_status grow_G(hipStream_t st);
     // sequence complete: build G for the PM scheduler; with release, wait for
     // the stream and free the sliced operands
     rr_status start_pm(hipStream_t st, bool release = true);

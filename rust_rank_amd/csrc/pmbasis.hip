@@ -662,6 +662,7 @@ static uint64_t rec_words(uint64_t d, uint64_t K, uint32_t m, uint32_t b) {
 
 struct Op {
     int kind;  // 0 leaf, 1 product
+    uint64_t need = 0;  // leading coefficients of the job's G it reads
     Frame *keep = nullptr, *given = nullptr;  // spectrum reuse (products)
     LeafDesc leaf;
     MulDesc mul;
@@ -678,6 +679,7 @@ static bool next_op(PmJob &J, Op &op, rr_status &err) {
                 F.stage = 1;
                 op.kind = 0;
                 op.leaf = LeafDesc{F.G, F.gs, J.delta, F.M, J.m, J.b, (uint32_t)F.d};
+                op.need = F.G == J.G ? F.d + 1 : 0;
                 return true;
             }
             J.stack.pop_back();
@@ -714,6 +716,7 @@ static bool next_op(PmJob &J, Op &op, rr_status &err) {
                                  (uint32_t)d1, (uint32_t)(F.d + 1), J.m, J.m, J.b};
                 op.keep = F.SP ? &F : nullptr;
                 op.given = nullptr;
+                op.need = F.G == J.G ? F.d + 1 : 0;
                 return true;
             case 2: {
                 F.stage = 3;
@@ -733,6 +736,7 @@ static bool next_op(PmJob &J, Op &op, rr_status &err) {
                                  (uint32_t)(F.d + 1), J.m, J.m, J.m};
                 op.keep = nullptr;
                 op.given = F.sp_logn >= 0 ? &F : nullptr;
+                op.need = 0;
                 return true;
             default:
                 J.pool.top = F.mark;
@@ -813,6 +817,8 @@ rr_status PmScheduler::add(PmJob *J) {
 }
 
 // One round: the next operation of every unfinished job, launched together.
+// A job whose next operation reads coefficients of G beyond those written so
+// far (its sequence still being produced) waits.
 rr_status PmScheduler::round(std::vector<PmJob *> &jobs, bool *launched) {
     const ModP mp = ModP::make(p);
     std::vector<LeafDesc> leaves;
@@ -823,6 +829,7 @@ rr_status PmScheduler::round(std::vector<PmJob *> &jobs, bool *launched) {
     // product pending, so that the leaves of many jobs share one launch (a
     // job reaches a leaf after at most its recursion depth of products).
     bool any_mul = false;
+    waiting = 0;
     for (auto *J : jobs) {
         if (J->done) continue;
         if (!J->has_op) {
@@ -838,14 +845,20 @@ rr_status PmScheduler::round(std::vector<PmJob *> &jobs, bool *launched) {
             J->op_mul = op.mul;
             J->op_keep = op.keep;
             J->op_given = op.given;
+            J->op_need = op.need;
+        }
+        if (J->op_need > J->avail) {
+            waiting++;
+            continue;
         }
         any_mul |= J->op_kind == 1;
     }
+    auto ready = [](const PmJob *J) { return !J->done && J->has_op && J->op_need <= J->avail; };
     size_t nmul = 0;
     if (any_mul)
-        for (auto *J : jobs) nmul += !J->done && J->has_op && J->op_kind == 1;
+        for (auto *J : jobs) nmul += ready(J) && J->op_kind == 1;
     for (auto *J : jobs) {
-        if (J->done || !J->has_op || J->op_kind != (any_mul ? 1 : 0)) continue;
+        if (!ready(J) || J->op_kind != (any_mul ? 1 : 0)) continue;
         J->has_op = false;
         if (J->op_kind == 0) {
             leaves.push_back(J->op_leaf);
@@ -1057,13 +1070,14 @@ int64_t rank_from_basis_host(const uint32_t *basis, const int64_t *delta, uint32
     return (int64_t)r;
 }
 
-// G [2b][b][d+1] from the t-major upper-triangle sequence [L][P] (src/sigma_basis.rs:33-46)
-__global__ void k_build_G(const uint32_t *seq, uint32_t b, uint64_t len, uint32_t *G) {
+// G [2b][b][len] from the t-major upper-triangle sequence [L][P]
+// (src/sigma_basis.rs:33-46): coefficients k0 <= k < k1 (coefficient k is S_k)
+__global__ void k_build_G(const uint32_t *seq, uint32_t b, uint64_t len, uint64_t k0, uint64_t k1, uint32_t *G) {
     const uint32_t P = b * (b + 1) / 2;
-    const uint64_t total = (uint64_t)2 * b * b * len;
+    const uint64_t w = k1 - k0, total = (uint64_t)2 * b * b * w;
     for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
          idx += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t k = idx % len, ij = idx / len;
+        const uint64_t k = k0 + idx % w, ij = idx / w;
         uint32_t j = (uint32_t)(ij % b), i = (uint32_t)(ij / b);
         uint32_t v;
         if (i < b) {
@@ -1073,16 +1087,24 @@ __global__ void k_build_G(const uint32_t *seq, uint32_t b, uint64_t len, uint32_
         } else {
             v = (k == 0 && i - b == j) ? 1u : 0u;
         }
-        G[idx] = v;
+        G[ij * len + k] = v;
     }
 }
 
-rr_status build_G_device(const uint32_t *seq_dev, uint32_t b, uint64_t len, uint32_t *G, hipStream_t st) {
-    uint64_t total = (uint64_t)2 * b * b * len;
-    uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 16384);
-    hipLaunchKernelGGL(k_build_G, dim3(std::max(1u, blocks)), dim3(256), 0, st, seq_dev, b, len, G);
+rr_status build_G_range(const uint32_t *seq_dev, uint32_t b, uint64_t len, uint64_t k0, uint64_t k1, uint32_t *G,
+                        hipStream_t st) {
+    RR_CHECK(k0 <= k1 && k1 <= len, RR_EINVAL, "coefficients [%llu, %llu) outside [0, %llu)", (unsigned long long)k0,
+             (unsigned long long)k1, (unsigned long long)len);
+    if (k0 == k1) return RR_OK;
+    const uint64_t total = (uint64_t)2 * b * b * (k1 - k0);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_build_G, dim3(std::max(1u, blocks)), dim3(256), 0, st, seq_dev, b, len, k0, k1, G);
     RR_HIP(hipGetLastError());
     return RR_OK;
+}
+
+rr_status build_G_device(const uint32_t *seq_dev, uint32_t b, uint64_t len, uint32_t *G, hipStream_t st) {
+    return build_G_range(seq_dev, b, len, 0, len, G, st);
 }
 
 }  // namespace rr

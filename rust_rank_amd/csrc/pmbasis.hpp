@@ -77,6 +77,7 @@ struct PmJob {
     uint32_t K = 0;  // leaf order (0: chosen when the job joins)
     uint64_t d = 0;
     const uint32_t *G = nullptr;  // [m][b][d+1]
+    uint64_t avail = ~0ull;       // leading coefficients of G written so far (the rest still to come)
     uint32_t *M = nullptr;        // [m][m][d+1]
     int64_t *delta = nullptr;     // device [m]
     Pool pool;
@@ -89,6 +90,7 @@ struct PmJob {
     MulDesc op_mul{};
     Frame *op_keep = nullptr;   // product keeps its A spectrum in this frame's SP
     Frame *op_given = nullptr;  // product takes its B spectrum from this frame's SP
+    uint64_t op_need = 0;       // leading coefficients of G the next operation reads
 };
 
 struct PmScheduler {
@@ -98,6 +100,7 @@ struct PmScheduler {
     DevBuf scratch;
     size_t scratch_words = 0, lds = 0;
     uint64_t rounds = 0, launches = 0;
+    uint32_t waiting = 0;  // after a round: jobs whose next operation waits for coefficients of G
 
     rr_status ensure_scratch(size_t words);
     rr_status init(uint32_t p, hipStream_t st, uint32_t max_logn, uint32_t max_m, uint32_t max_n);
@@ -112,6 +115,9 @@ rr_status pm_basis_batch(uint32_t p, std::vector<PmJob *> &jobs, hipStream_t st,
 rr_status pm_basis_device(uint32_t p, uint32_t b, uint64_t d, const uint32_t *G, uint32_t *M, int64_t *delta_host,
                           hipStream_t st);
 rr_status build_G_device(const uint32_t *seq_dev, uint32_t b, uint64_t len, uint32_t *G, hipStream_t st);
+// coefficients [k0, k1) of the same G (len = its coefficients per entry)
+rr_status build_G_range(const uint32_t *seq_dev, uint32_t b, uint64_t len, uint64_t k0, uint64_t k1, uint32_t *G,
+                        hipStream_t st);
 int64_t rank_from_basis_host(const uint32_t *basis, const int64_t *delta, uint32_t b, uint64_t len, uint32_t p,
                              rr_status *st);
 
