@@ -52,6 +52,27 @@ def test_batch_sequences_bit_identical_to_oracle():
         assert ranks[1] == want, (d, ranks[1], want)
 
 
+def test_batch_streamed_pm_basis_matches_oracle():
+    """Orders of at least 1024 start their PM-basis while the sequence is still
+    being produced (the leftmost chain of the recursion reads only leading
+    coefficients); the rank at each such order equals the oracle's."""
+    eng = _eng()
+    A = synth.batch_matrix(6, 3, rows_lo=2600, rows_hi=2700)
+    b = 4
+    dr, dc, V = synth.preconditioners(9, A.m, A.n, b, P)
+    L = 2 * min(A.m, A.n) // b + 8
+    assert L - 1 >= 1024
+    seq = O.krylov(A, P, b, dr, dc, V, 0, L)
+    other = synth.batch_matrix(6, 4, rows_lo=100, rows_hi=400)
+    dr2, dc2, V2 = synth.preconditioners(10, other.m, other.n, 8, P)
+    for d in (L - 1, 1024 + 37):
+        basis, delta = O.pm_basis(seq, b, d, P)
+        want = O.rank_from_basis(basis, delta, b, P)
+        ranks, _ = eng.batch_rank([other, A], P, [8, b], [dr2, dr], [dc2, dc], [V2, V], [0, L], [0, d])
+        print(f"m={A.m} n={A.n} L={L} d={d}: rank {ranks[1]} oracle {want}", flush=True)
+        assert ranks[1] == want, (d, ranks[1], want)
+
+
 def test_batch_many_matrices_match_single():
     """48 matrices: set-up on worker threads overlaps the device work, so
     requests join the tick loop at different times; every rank still equals
