@@ -706,7 +706,7 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
                     joined[i] = 1;
                     njoined++;
                     Request *r = reqs[i].get();
-                    if (r->d >= BATCH_STREAM_D) {  // its PM-basis starts on the leading coefficients
+                    if (r->d >= PM_STREAM_D) {  // its PM-basis starts on the leading coefficients
                         if ((s = r->alloc_pm()) != RR_OK || (s = S.add(&r->pm)) != RR_OK) return s;
                         jobs.push_back(&r->pm);
                     }

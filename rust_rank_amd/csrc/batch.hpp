@@ -39,10 +39,10 @@ constexpr uint32_t BATCH_TICKS = 16;
 constexpr uint32_t BATCH_PM_AHEAD = 8;
 // Ticks per loop of the server (between two PM-basis rounds).
 constexpr uint32_t SERVER_TICKS = 4;
-// rr_batch_rank starts the PM-basis of a request of at least this order while
-// its sequence is still being produced: the leftmost chain of the recursion
-// reads only leading coefficients of G.
-constexpr uint64_t BATCH_STREAM_D = 1024;
+// The batched and served modes start the PM-basis of a request of at least
+// this order while its sequence is still being produced: the leftmost chain
+// of the recursion reads only leading coefficients of G.
+constexpr uint64_t PM_STREAM_D = 1024;
 
 struct Request {
     uint32_t p = 0, b = 0, B = 0, P = 0, su = 4, nwg = 0;

@@ -112,6 +112,8 @@ void rr_server::run() {
             R->t_admit = now_ms();
             R->budget_bytes = request_bytes(q->m, q->n, q->rp[q->m], q->b, q->L, q->d, false);
             rr_status s = R->init(&A, p, q->b, q->dr.data(), q->dc.data(), q->V.data(), q->L, q->d, 2048, st_k);
+            // a long sequence's PM-basis starts on its leading coefficients
+            if (s == RR_OK && R->d >= PM_STREAM_D && (s = R->alloc_pm()) == RR_OK) s = pm.add(&R->pm);
             if (s != RR_OK) {
                 publish(R.get(), s, R->budget_bytes);
                 continue;
@@ -134,16 +136,27 @@ void rr_server::run() {
         const uint32_t nt = first ? 1u : (uint32_t)std::min<uint64_t>(SERVER_TICKS, left);
         if (!kry.empty() && ticks.tick(kry, p, nt) != RR_OK)
             for (auto *R : kry) R->stage = STAGE_FAILED;
-        // sequences completed: build G on the Krylov stream, hand over to the PM stream
+        // sequences completed: build G on the Krylov stream, hand over to the
+        // PM stream; PM-basis jobs already running get the coefficients their
+        // next operation waits for as soon as the sequence has them
         bool handed = false;
-        for (auto &R : active)
-            if (R->stage == STAGE_KRYLOV && R->t >= R->L) {
-                if (R->start_pm(st_k) != RR_OK || pm.add(&R->pm) != RR_OK) {
+        for (auto &R : active) {
+            if (R->stage != STAGE_KRYLOV) continue;
+            const bool running = R->G.ptr != nullptr;
+            if (R->t >= R->L) {
+                if (R->start_pm(st_k) != RR_OK || (!running && pm.add(&R->pm) != RR_OK)) {
+                    R->stage = STAGE_FAILED;
+                    continue;
+                }
+                handed = true;
+            } else if (running && R->pm.has_op && R->pm.op_need > R->pm.avail && R->t >= R->pm.op_need) {
+                if (R->grow_G(st_k) != RR_OK) {
                     R->stage = STAGE_FAILED;
                     continue;
                 }
                 handed = true;
             }
+        }
         if (handed) {
             hipEvent_t ev;
             if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
@@ -152,16 +165,20 @@ void rr_server::run() {
                 (void)hipEventDestroy(ev);
             }
         }
-        // one PM-basis round for every request whose sequence is complete
+        // one PM-basis round for every request whose sequence is complete or
+        // whose PM-basis runs on its leading coefficients
+        auto in_pm = [](const Request *R) {
+            return !R->pm.done && (R->stage == STAGE_PM || (R->stage == STAGE_KRYLOV && R->G.ptr));
+        };
         jobs.clear();
         for (auto &R : active)
-            if (R->stage == STAGE_PM && !R->pm.done) jobs.push_back(&R->pm);
+            if (in_pm(R.get())) jobs.push_back(&R->pm);
         if (!jobs.empty()) {
             bool launched = false;
             if (pm.round(jobs, &launched) != RR_OK)
                 for (auto &R : active)
-                    if (R->stage == STAGE_PM && !R->pm.done) R->stage = STAGE_FAILED;
-            pm_rounds++;
+                    if (in_pm(R.get())) R->stage = STAGE_FAILED;
+            pm_rounds += launched;
         }
         // completion
         for (size_t i = 0; i < active.size();) {
