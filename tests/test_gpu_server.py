@@ -86,3 +86,20 @@ def test_server_small_budget_serialises_admission():
     assert st["max_active"] == 1
     for rid, (A, b, dr, dc, V) in zip(ids, reqs):
         assert got[rid]["rank"] == eng.rank(A, P, b, dr, dc, V)[0]
+
+
+def test_server_streamed_pm_basis_equals_single():
+    """A request of order >= 1024 starts its PM-basis at admission on the
+    leading coefficients of G, among short requests; its rank equals the
+    single-matrix path's."""
+    eng = _eng()
+    A = synth.batch_matrix(6, 3, rows_lo=2600, rows_hi=2700)
+    dr, dc, V = synth.preconditioners(9, A.m, A.n, 4, P)
+    reqs = [(A, 4, dr, dc, V)] + _requests(4, 37)
+    alone = [eng.rank(M, P, b, r, c, v)[0] for (M, b, r, c, v) in reqs]
+    with eng.Server(P, 64 << 30) as srv:
+        ids = [srv.submit(M, b, r, c, v) for (M, b, r, c, v) in reqs]
+        got = _collect(srv, ids)
+    print(f"m={A.m} n={A.n} b=4: rank {got[ids[0]]['rank']} alone {alone[0]}", flush=True)
+    for i, rid in enumerate(ids):
+        assert got[rid]["status"] == 0 and got[rid]["rank"] == alone[i], (i, got[rid], alone[i])
