@@ -38,7 +38,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(A, p, b, dr, dc, V, iters=256):
+def cpu_baseline(A, p, b, dr, dc, V, iters=1024):
     """Oracle (restated reference CPU path, OpenMP) on a prefix of the same
     workload: iterations/s from the difference of two runs (setup cancels)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
