@@ -552,6 +552,85 @@ struct EventRing {
     }
 };
 
+// Entries per rr_krylov_run call and PM-basis rounds kept queued by rank_streamed.
+constexpr uint64_t STREAM_CHUNK = 128;
+constexpr size_t STREAM_ROUNDS_AHEAD = 96;
+
+rr_status rank_streamed(rr_krylov *k, uint32_t b, uint64_t L, uint64_t d, uint32_t p, uint32_t *seq_out,
+                        int64_t *rank) {
+    RR_CHECK(d >= 1 && d < L, RR_EINVAL, "order must satisfy 1 <= order < seq_len");
+    const uint64_t P = (uint64_t)b * (b + 1) / 2;
+    const hipStream_t sk = krylov_stream(k);
+    DevBuf G, M, dl, sc;
+    rr_status s;
+    if ((s = G.alloc((size_t)2 * b * b * (d + 1) * 4)) != RR_OK) return s;
+    if ((s = M.alloc((size_t)4 * b * b * (d + 1) * 4)) != RR_OK) return s;
+    if ((s = dl.alloc((size_t)2 * b * 8)) != RR_OK) return s;
+    if ((s = sc.alloc((size_t)(2 * b * b + 1) * 4)) != RR_OK) return s;
+    hipStream_t sp = nullptr;
+    hipEvent_t ev = nullptr;
+    struct Guard {
+        hipStream_t &sp;
+        hipEvent_t &ev;
+        ~Guard() {
+            if (sp) {
+                (void)hipStreamSynchronize(sp);
+                (void)hipStreamDestroy(sp);
+            }
+            if (ev) (void)hipEventDestroy(ev);
+        }
+    } guard{sp, ev};
+    RR_HIP(hipStreamCreateWithFlags(&sp, hipStreamNonBlocking));
+    RR_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    uint32_t logn = 2;
+    while ((1ull << logn) < d + 1) logn++;
+    PmScheduler S;
+    if ((s = S.init(p, sp, logn, 2 * b, b)) != RR_OK) return s;
+    PmJob J;
+    J.b = b;
+    J.d = d;
+    J.G = G.as<uint32_t>();
+    J.M = M.as<uint32_t>();
+    J.delta = dl.as<int64_t>();
+    if ((s = S.add(&J)) != RR_OK) return s;
+    J.avail = 0;
+    std::vector<PmJob *> jobs{&J};
+    EventRing ring;
+    if ((s = ring.init(STREAM_ROUNDS_AHEAD)) != RR_OK) return s;
+    auto rounds = [&]() -> rr_status {  // the rounds the coefficients allow, within the ring
+        while (!J.done && ring.room()) {
+            bool launched = false;
+            rr_status e = S.round(jobs, &launched);
+            if (e != RR_OK) return e;
+            if (!launched) break;
+            if ((e = ring.push(sp)) != RR_OK) return e;
+        }
+        return RR_OK;
+    };
+    for (uint64_t t = 0; t < L;) {
+        const uint64_t n = std::min<uint64_t>(STREAM_CHUNK, L - t);
+        if ((s = rr_krylov_run(k, n, seq_out + t * P)) != RR_OK) return s;
+        const uint64_t k1 = std::min<uint64_t>(t + n, d + 1);
+        if (k1 > t) {  // coefficients [t, k1) from this call's entries, then to the PM stream
+            if ((s = build_G_range(krylov_device_seq(k), b, d + 1, t, k1, G.as<uint32_t>(), sk, t)) != RR_OK) return s;
+            RR_HIP(hipEventRecord(ev, sk));
+            RR_HIP(hipStreamWaitEvent(sp, ev, 0));
+            J.avail = k1 == d + 1 ? ~0ull : k1;
+        }
+        t += n;
+        if ((s = rounds()) != RR_OK) return s;
+    }
+    while (!J.done) {
+        if ((s = rounds()) != RR_OK) return s;
+        if (!J.done && !ring.room()) std::this_thread::yield();
+        if (!J.done && ring.room() && S.waiting) {
+            set_error("PM-basis waits for coefficients past the sequence");
+            return RR_EINVAL;
+        }
+    }
+    return rank_from_device_basis(M.as<uint32_t>(), dl.as<int64_t>(), b, d + 1, p, sc.as<uint32_t>(), sp, rank);
+}
+
 }  // namespace rr
 
 using namespace rr;

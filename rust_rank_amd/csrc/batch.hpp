@@ -76,6 +76,15 @@ rr_status rank_from_device_basis(const uint32_t *M, const int64_t *delta, uint32
 rr_status rank_from_device_seq(const uint32_t *seq_dev, uint32_t b, uint64_t d, uint32_t p, hipStream_t st,
                                int64_t *rank);
 
+const uint32_t *krylov_device_seq(rr_krylov *c);  // the last rr_krylov_run's entries, from index 0
+hipStream_t krylov_stream(rr_krylov *c);
+
+// The L-entry sequence of context k (copied to seq_out) and the rank from its
+// PM-basis of order d: the sequence is produced in chunks, and the PM-basis
+// runs on a second stream on the coefficients of G produced so far.
+rr_status rank_streamed(rr_krylov *k, uint32_t b, uint64_t L, uint64_t d, uint32_t p, uint32_t *seq_out,
+                        int64_t *rank);
+
 struct TickRunner {
     static constexpr size_t STAGE_BYTES = 4u << 20;
     static constexpr int NSTAGE = 8;

@@ -14,10 +14,6 @@
 #include <vector>
 #include <chrono>
 
-namespace rr {
-const uint32_t *krylov_device_seq(rr_krylov *c);
-hipStream_t krylov_stream(rr_krylov *c);
-}  // namespace rr
 
 extern "C" {
 
@@ -58,9 +54,13 @@ rr_status rr_rank(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *row_p
     rr_krylov *k = nullptr;
     rr_status s = rr_krylov_create(A, p, b, row_precond, col_precond, V, 0, &k);
     if (s != RR_OK) return s;
-    s = rr_krylov_run(k, seq_len, seq.data());
-    if (s == RR_OK)  // the sequence is still on the device: G, PM-basis and rank stay there
-        s = rr::rank_from_device_seq(rr::krylov_device_seq(k), b, order, p, rr::krylov_stream(k), rank_out);
+    if (order >= rr::PM_STREAM_D) {  // the PM-basis runs on the leading coefficients while the sequence grows
+        s = rr::rank_streamed(k, b, seq_len, order, p, seq.data(), rank_out);
+    } else {
+        s = rr_krylov_run(k, seq_len, seq.data());
+        if (s == RR_OK)  // the sequence is still on the device: G, PM-basis and rank stay there
+            s = rr::rank_from_device_seq(rr::krylov_device_seq(k), b, order, p, rr::krylov_stream(k), rank_out);
+    }
     rr_krylov_destroy(k);
     if (s != RR_OK) return s;
     if (ms_out)

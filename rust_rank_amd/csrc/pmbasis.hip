@@ -1071,8 +1071,10 @@ int64_t rank_from_basis_host(const uint32_t *basis, const int64_t *delta, uint32
 }
 
 // G [2b][b][len] from the t-major upper-triangle sequence [L][P]
-// (src/sigma_basis.rs:33-46): coefficients k0 <= k < k1 (coefficient k is S_k)
-__global__ void k_build_G(const uint32_t *seq, uint32_t b, uint64_t len, uint64_t k0, uint64_t k1, uint32_t *G) {
+// (src/sigma_basis.rs:33-46): coefficients k0 <= k < k1 (coefficient k is
+// S_k, which seq holds at entry k - s0)
+__global__ void k_build_G(const uint32_t *seq, uint32_t b, uint64_t len, uint64_t k0, uint64_t k1, uint64_t s0,
+                          uint32_t *G) {
     const uint32_t P = b * (b + 1) / 2;
     const uint64_t w = k1 - k0, total = (uint64_t)2 * b * b * w;
     for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
@@ -1083,7 +1085,7 @@ __global__ void k_build_G(const uint32_t *seq, uint32_t b, uint64_t len, uint64_
         if (i < b) {
             uint32_t lo = min(i, j), hi = max(i, j);
             uint32_t ii = lo * b - lo * (lo - 1) / 2 + (hi - lo);
-            v = seq[k * P + ii];
+            v = seq[(k - s0) * P + ii];
         } else {
             v = (k == 0 && i - b == j) ? 1u : 0u;
         }
@@ -1092,13 +1094,13 @@ __global__ void k_build_G(const uint32_t *seq, uint32_t b, uint64_t len, uint64_
 }
 
 rr_status build_G_range(const uint32_t *seq_dev, uint32_t b, uint64_t len, uint64_t k0, uint64_t k1, uint32_t *G,
-                        hipStream_t st) {
-    RR_CHECK(k0 <= k1 && k1 <= len, RR_EINVAL, "coefficients [%llu, %llu) outside [0, %llu)", (unsigned long long)k0,
-             (unsigned long long)k1, (unsigned long long)len);
+                        hipStream_t st, uint64_t s0) {
+    RR_CHECK(s0 <= k0 && k0 <= k1 && k1 <= len, RR_EINVAL, "coefficients [%llu, %llu) outside [%llu, %llu)",
+             (unsigned long long)k0, (unsigned long long)k1, (unsigned long long)s0, (unsigned long long)len);
     if (k0 == k1) return RR_OK;
     const uint64_t total = (uint64_t)2 * b * b * (k1 - k0);
     const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 16384);
-    hipLaunchKernelGGL(k_build_G, dim3(std::max(1u, blocks)), dim3(256), 0, st, seq_dev, b, len, k0, k1, G);
+    hipLaunchKernelGGL(k_build_G, dim3(std::max(1u, blocks)), dim3(256), 0, st, seq_dev, b, len, k0, k1, s0, G);
     RR_HIP(hipGetLastError());
     return RR_OK;
 }

@@ -115,9 +115,10 @@ rr_status pm_basis_batch(uint32_t p, std::vector<PmJob *> &jobs, hipStream_t st,
 rr_status pm_basis_device(uint32_t p, uint32_t b, uint64_t d, const uint32_t *G, uint32_t *M, int64_t *delta_host,
                           hipStream_t st);
 rr_status build_G_device(const uint32_t *seq_dev, uint32_t b, uint64_t len, uint32_t *G, hipStream_t st);
-// coefficients [k0, k1) of the same G (len = its coefficients per entry)
+// coefficients [k0, k1) of the same G (len = its coefficients per entry);
+// seq_dev holds the sequence from entry s0 <= k0 on
 rr_status build_G_range(const uint32_t *seq_dev, uint32_t b, uint64_t len, uint64_t k0, uint64_t k1, uint32_t *G,
-                        hipStream_t st);
+                        hipStream_t st, uint64_t s0 = 0);
 int64_t rank_from_basis_host(const uint32_t *basis, const int64_t *delta, uint32_t b, uint64_t len, uint32_t p,
                              rr_status *st);
 
