@@ -236,8 +236,12 @@ def main():
         t4 = time.perf_counter()
         rank = engine.rank_from_basis(basis, delta, b, p)
         del basis
+        # the same rank through rr_rank: set-up, sequence and PM-basis in one
+        # call, the PM-basis running beside the passes on this workload
+        rr_rank, rr_ms = engine.rank(A, p, b, dr, dc, V, L, L - 1)
+        assert int(rr_rank) == int(rank), (rr_rank, rank)
         pm = {"b": b, "order": L - 1, "ms": round((t4 - t3) * 1e3, 1), "rank": int(rank),
-              "rank_pipeline_s": round(L / iters_s + (t4 - t3), 3)}
+              "rank_pipeline_s": round(L / iters_s + (t4 - t3), 3), "rr_rank_s": round(rr_ms / 1e3, 3)}
         log(f"PM-basis of order {L - 1}: {pm['ms']} ms; rank {rank}")
     cpu = None if (args.no_cpu or large) else cpu_baseline(A, p, b, dr, dc, V)
     batched = served = None

@@ -161,8 +161,9 @@ uint64_t rr_pm_order(uint64_t seq_len, int reference_policy);
  * Replaces the rank driver of src/main.rs:397-606 (after loading and drawing):
  * Krylov sequence of seq_len entries (0: exact policy), PM-basis of the given
  * order (0: seq_len - 1), rank estimate returned in rank_out.  ms_out (may be
- * NULL) receives the wall time of the call.  For orders of at least 1024 the
- * PM-basis starts on the leading coefficients while the sequence is produced.
+ * NULL) receives the wall time of the call.  For orders of at least 1024 on
+ * matrices of at least 2^21 nonzeros the PM-basis starts on the leading
+ * coefficients while the sequence is produced.
  */
 rr_status rr_rank(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *row_precond, const uint32_t *col_precond,
                   const uint32_t *V, uint64_t seq_len, uint64_t order, int64_t *rank_out, double *ms_out);

@@ -43,6 +43,9 @@ constexpr uint32_t SERVER_TICKS = 4;
 // this order while its sequence is still being produced: the leftmost chain
 // of the recursion reads only leading coefficients of G.
 constexpr uint64_t PM_STREAM_D = 1024;
+// rr_rank streams only matrices of at least this many nonzeros: on smaller
+// ones the chunked sequence costs more than the overlap saves.
+constexpr uint64_t RANK_STREAM_NNZ = 1ull << 21;
 
 struct Request {
     uint32_t p = 0, b = 0, B = 0, P = 0, su = 4, nwg = 0;

@@ -54,7 +54,8 @@ rr_status rr_rank(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *row_p
     rr_krylov *k = nullptr;
     rr_status s = rr_krylov_create(A, p, b, row_precond, col_precond, V, 0, &k);
     if (s != RR_OK) return s;
-    if (order >= rr::PM_STREAM_D) {  // the PM-basis runs on the leading coefficients while the sequence grows
+    if (order >= rr::PM_STREAM_D && A->row_ptr[A->m] >= rr::RANK_STREAM_NNZ) {
+        // the PM-basis runs on the leading coefficients while the sequence grows
         s = rr::rank_streamed(k, b, seq_len, order, p, seq.data(), rank_out);
     } else {
         s = rr_krylov_run(k, seq_len, seq.data());

@@ -95,3 +95,27 @@ def test_rank_plain_small_end_to_end():
     basis, delta = eng.pm_basis(seq, b, L - 1, P)
     r = eng.rank_from_basis(basis, delta, b, P)
     assert r == O.dense_rank(A.to_dense(P), P)
+
+
+def test_rank_streamed_equals_sequence_then_basis():
+    """rr_rank on a matrix of at least 2^21 nonzeros and an order of at least
+    1024 runs the PM-basis beside the Krylov passes (chunked sequence, G's
+    leading coefficients handed over as they appear); its rank equals the
+    sequence computed in one run followed by the device PM-basis."""
+    from rust_rank_amd import engine as eng
+    assert eng.device_count() >= 1, "no GPU visible"
+    A = synth.graph_complex(seed=7, m=180000, n=8000, k=12)
+    assert A.nnz >= 1 << 21
+    b = 8
+    L = 2 * min(A.m, A.n) // b + 8
+    d = L - 1
+    assert d >= 1024
+    dr, dc, V = synth.preconditioners(11, A.m, A.n, b, P)
+    K = eng.Krylov(A, P, b, dr, dc, V)
+    seq = K.run(L)
+    K.close()
+    basis, delta = eng.pm_basis(seq, b, d, P)
+    want = eng.rank_from_basis(basis, delta, b, P)
+    got, ms = eng.rank(A, P, b, dr, dc, V)
+    print(f"m={A.m} n={A.n} nnz={A.nnz} b={b} d={d}: streamed rank {got} in {ms:.0f} ms, two-step {want}", flush=True)
+    assert got == want
