@@ -1,27 +1,34 @@
 """Benchmark: Krylov iterations per second of the block-Wiedemann sequence on
-one MI355X, on the graph-complex-like workload (200000 x 180000, 12 distinct
-Zipf(1.1) columns per row, entries +-1/+-2 mod p = 2013265921, b = 32).
+one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-A step is one Krylov iteration: one sparse pass (alternately by A and by A^T)
-plus the b x b projection, producing one sequence matrix S_t.  The default K
-is the workload's full sequence length (2 * 180000 / 32 + 8 = 11258 entries),
-so the timed region is the whole Krylov phase of one rank computation, with
-the matrix and vectors already resident in HBM.
+The headline is the largest single-GPU workload: a 5,000,000 x 4,500,000
+matrix with 20 uniform columns per row and values uniform in [1, p)
+(p = 2013265921, 5000 planted dependent columns, about 1.0e8 nonzeros) at
+block width b = 64.  A step is one Krylov iteration: one sparse pass
+(alternately by A and by A^T) plus the b x b projection, producing one
+sequence matrix S_t.  The timed region is K iterations after W untimed ones,
+with A, A^T and the block vectors resident in HBM and the device sequence
+buffer already sized, ended by the copy of the K new entries to the host.
 
-Prints one JSON line.  `roofline` prices the dominant kernel (the fused
-SpMM + projection pass) against HBM peak using algorithmic bytes per pass
-(int32 columns, u32 values, each array touched once) and its average
-duration from HIP events on the engine's stream; `cpu_baseline` times the
-CPU oracle (a restatement of the reference's Rust path) on a bounded prefix
-of the same workload on this host.
+Prints one JSON line.  `roofline` prices the dominant kernel (the sparse
+pass k_spmm) against HBM peak with the algorithmic bytes of one pass (int32
+columns, u32 values, each array touched once) over its average duration from
+events the dispatches stamp on the engine's stream; `cpu_baseline` times the
+CPU oracle (a restatement of the reference's Rust path) on a prefix of the
+same workload on this host.  Extra objects: the 200000 x 180000 graph-complex
+workload (b = 32) over its whole sequence with its PM-basis and rank, the
+512-matrix batched form against plain solves, and the served form.
+
+    python bench.py --full-rank     # config-3 rank end to end (about 15 min), then exit
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -31,29 +38,134 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_k_spmm_graph_complex.json")
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(A, p, b, dr, dc, V, iters=1024):
-    """Oracle (restated reference CPU path, OpenMP) on a prefix of the same
-    workload: iterations/s from the difference of two runs (setup cancels)."""
+def host_info():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    return {"nproc": os.cpu_count(), "cpu_model": model, "threads": threads}
+
+
+def traffic_from_profile(name):
+    """Fabric bytes per launch from a committed counter summary (FETCH_SIZE and
+    WRITE_SIZE scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(PROFILES, name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def cpu_baseline(A, p, b, dr, dc, V, warm, timed, what):
+    """Oracle (restated reference CPU path, u128 OpenMP loop over all the
+    threads this host gives the job) on a prefix of the same workload:
+    iterations/s from the difference of two runs (set-up and warm-up cancel)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    threads = max(1, min(threads, os.cpu_count() or 1, 16))
+    h = host_info()
     t0 = time.perf_counter()
-    O.krylov(A, p, b, dr, dc, V, 0, 1, nthreads=threads)
+    O.krylov(A, p, b, dr, dc, V, 0, warm, nthreads=h["threads"])
     t1 = time.perf_counter()
-    O.krylov(A, p, b, dr, dc, V, 0, 1 + iters, nthreads=threads)
+    O.krylov(A, p, b, dr, dc, V, 0, warm + timed, nthreads=h["threads"])
     t2 = time.perf_counter()
-    per_iter = ((t2 - t1) - (t1 - t0)) / iters
-    return {"value": round(1.0 / per_iter, 3), "unit": "iters/s", "cores": threads, "kind": "port",
-            "sample": f"S_1..S_{iters} of the same 200000x180000 b=32 workload (oracle/krylov_oracle.c, "
-                      f"{threads} OpenMP threads; setup cancelled by differencing two runs)"}
+    per_iter = ((t2 - t1) - (t1 - t0)) / timed
+    return {"value": round(1.0 / per_iter, 4), "unit": "iters/s", "cores": h["threads"], "kind": "port",
+            "sample": f"prefix: S_{warm}..S_{warm + timed - 1} of the same {what} after {warm} warm-up entries "
+                      f"(oracle/krylov_oracle.c, {h['threads']} OpenMP threads: u128 row-parallel SpMV and "
+                      f"pair-parallel projection; set-up cancelled by differencing two runs)",
+            "nproc": h["nproc"], "cpu_model": h["cpu_model"]}
+
+
+def krylov_leg(engine, A, p, b, dr, dc, V, steps, warmup, prof_steps):
+    """Timed Krylov iterations on one matrix, then a profiled stretch with
+    per-kernel events.  Returns (rate dict, sequence produced before timing)."""
+    K = engine.Krylov(A, p, b, dr, dc, V)
+    warm = K.run(max(warmup, 1))
+    t1 = time.perf_counter()
+    K.run(steps)
+    wall = time.perf_counter() - t1
+    K.set_profiling(True)
+    K.run(prof_steps)
+    st = K.stats()
+    K.close()
+    pa, pt = max(st["passes_a"], 1), max(st["passes_at"], 1)
+    passes = st["passes_a"] + st["passes_at"]
+    spmm_ms = (st["ms_a"] + st["ms_at"]) / max(passes, 1)
+    proj_ms = st["ms_proj"] / max(passes, 1)
+    bytes_pass = (st["bytes_a"] * st["passes_a"] + st["bytes_at"] * st["passes_at"]) / max(passes, 1)
+    log(f"  pass by A {st['ms_a'] / pa * 1e3:.1f} us, by A^T {st['ms_at'] / pt * 1e3:.1f} us, projection "
+        f"{proj_ms * 1e3:.1f} us/pass; timed {steps} iterations in {wall * 1e3:.1f} ms")
+    return {"wall_s": wall, "iters_s": steps / wall, "spmm_ms": spmm_ms, "proj_ms": proj_ms,
+            "a_us": st["ms_a"] / pa * 1e3, "at_us": st["ms_at"] / pt * 1e3, "bytes_pass": bytes_pass,
+            "stream_ms": st["ms_total"], "prof_steps": prof_steps}, warm
+
+
+def roofline(r, traffic, kernel, copy_gbs):
+    achieved = r["bytes_pass"] / (r["spmm_ms"] * 1e-3) / 1e9
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
+           "bytes_per_launch": int(r["bytes_pass"]), "avg_launch_us": round(r["spmm_ms"] * 1e3, 2),
+           "pass_a_us": round(r["a_us"], 2), "pass_at_us": round(r["at_us"], 2)}
+    if traffic:
+        out["traffic_gbs"] = round(traffic / (r["spmm_ms"] * 1e-3) / 1e9, 1)
+        out["traffic_frac"] = round(out["traffic_gbs"] / HBM_PEAK_GBS, 4)
+    if r["proj_ms"] > 0:
+        out["projection_kernel"] = {"name": "k_proj64", "avg_launch_us": round(r["proj_ms"] * 1e3, 2)}
+    if copy_gbs:
+        out["copy_ceiling_gbs"] = round(copy_gbs, 1)
+    return out
+
+
+def graph_complex_leg(engine, synth, p, warmup):
+    """The 200000 x 180000 graph-complex workload at b = 32 over its whole
+    sequence (11258 entries), its device PM-basis of order L - 1 and rank,
+    and the same rank through one rr_rank call."""
+    b = 32
+    A = synth.graph_complex()
+    dr, dc, V = synth.preconditioners(2, A.m, A.n, b, p)
+    L = synth.seq_len_plain(min(A.m, A.n), b)
+    log(f"graph_complex {A.m}x{A.n} nnz={A.nnz} b={b} L={L}")
+    K = engine.Krylov(A, p, b, dr, dc, V)
+    seq = K.run(L)  # the sequence for the PM-basis; also sizes the device buffer
+    t1 = time.perf_counter()
+    K.run(L)  # timed: a whole sequence's worth of iterations in one call
+    wall = time.perf_counter() - t1
+    K.set_profiling(True)
+    K.run(2000)
+    st = K.stats()
+    K.close()
+    passes = st["passes_a"] + st["passes_at"]
+    r = {"spmm_ms": (st["ms_a"] + st["ms_at"]) / passes, "proj_ms": 0.0,
+         "a_us": st["ms_a"] / st["passes_a"] * 1e3, "at_us": st["ms_at"] / st["passes_at"] * 1e3,
+         "bytes_pass": (st["bytes_a"] * st["passes_a"] + st["bytes_at"] * st["passes_at"]) / passes}
+    t3 = time.perf_counter()
+    basis, delta = engine.pm_basis(seq, b, L - 1, p)
+    t4 = time.perf_counter()
+    rank = engine.rank_from_basis(basis, delta, b, p)
+    del basis
+    rr_rank, rr_ms = engine.rank(A, p, b, dr, dc, V, L, L - 1)
+    assert int(rr_rank) == int(rank), (rr_rank, rank)
+    out = {"workload": f"graph_complex {A.m}x{A.n} 12 nnz/row Zipf(1.1) +-1/+-2 p=2013265921 b={b}",
+           "seq_len": L, "iters_s": round(L / wall, 1), "ms_per_step": round(wall * 1e3 / L, 5),
+           "roofline": roofline(r, traffic_from_profile("traffic_k_spmm_graph_complex.json"),
+                                "k_spmm<32, packed>", None),
+           "pm_basis_ms": round((t4 - t3) * 1e3, 1), "rank": int(rank), "rr_rank_s": round(rr_ms / 1e3, 3)}
+    log(f"  {out['iters_s']} iters/s; PM-basis {out['pm_basis_ms']} ms; rank {rank}; rr_rank {out['rr_rank_s']} s")
+    return out
 
 
 def batched_and_served(eng, synth, p, nmat, rate_frac, seed=4, plain=True):
@@ -146,118 +258,90 @@ def batched_and_served(eng, synth, p, nmat, rate_frac, seed=4, plain=True):
     return batched, served
 
 
+def full_rank(engine, synth, p):
+    """The large single matrix end to end through rr_rank: 140633 Krylov
+    entries at b = 64 and the PM-basis of order 140632 beside them; the rank
+    must be the planted n - 5000 = 4495000."""
+    b = 64
+    t0 = time.perf_counter()
+    A = synth.large_single()
+    dr, dc, V = synth.preconditioners(2, A.m, A.n, b, p)
+    L = synth.seq_len_plain(min(A.m, A.n), b)
+    log(f"large_single {A.m}x{A.n} nnz={A.nnz} b={b} L={L}, generated in {time.perf_counter() - t0:.1f}s")
+    rank, ms = engine.rank(A, p, b, dr, dc, V, L, L - 1)
+    expect = A.n - 5000
+    out = {"workload": f"large_single {A.m}x{A.n} 20 nnz/row uniform, values uniform in [1,p), 5000 planted "
+                       f"dependent columns, p=2013265921 b={b}", "seq_len": L, "order": L - 1, "rank": int(rank),
+           "planted_rank": expect, "rank_ok": int(rank) == expect, "rr_rank_s": round(ms / 1e3, 2),
+           "nnz": A.nnz}
+    print(json.dumps(out), flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=0, help="timed iterations (default: full sequence)")
-    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=0, help="timed iterations (default 200)")
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-pm", action="store_true", help="skip the PM-basis and rank of the sequence")
-    ap.add_argument("--m", type=int, default=200000)
-    ap.add_argument("--n", type=int, default=180000)
-    ap.add_argument("--b", type=int, default=0, help="block width (default: 32, or 64 for --workload large)")
-    ap.add_argument("--workload", choices=["graph_complex", "large"], default="graph_complex",
-                    help="large: the 5000000 x 4500000, 20 nnz/row single matrix (Krylov only, bounded steps)")
+    ap.add_argument("--cpu-timed", type=int, default=6, help="timed oracle iterations for the CPU baseline")
+    ap.add_argument("--no-graph-complex", action="store_true", help="skip the 200000 x 180000 workload")
     ap.add_argument("--batched", type=int, default=512, help="matrices in the batched/served comparison (0: skip)")
     ap.add_argument("--rate-frac", type=float, default=0.7, help="served offered load / batched capacity")
     ap.add_argument("--no-plain", action="store_true",
                     help="batched call only: skip the plain solves and the served form (for kernel traces)")
+    ap.add_argument("--workload", choices=["large", "graph_complex"], default="large",
+                    help="graph_complex: only the 200000 x 180000 leg (for kernel traces and counters)")
+    ap.add_argument("--full-rank", action="store_true", help="only the large matrix's rank end to end")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
                     help="only run the plain copy kernel on MB megabytes (counter calibration) and exit")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the engine does not shard a matrix "
+              f"across GPUs, and this build runs on one", file=sys.stderr)
+        sys.exit(2)
+    from rust_rank_amd import engine, synth
+    p = synth.P_DEFAULT
     if args.copy_calib:
-        from rust_rank_amd import engine
         ms, gbs = engine.copy_probe(args.copy_calib << 20, 20)
         print(json.dumps({"copy_bytes": args.copy_calib << 20, "avg_ms": ms, "read_write_gbs": gbs}), flush=True)
         return
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the engine does not shard "
-              f"a single matrix across GPUs", file=sys.stderr)
-        sys.exit(2)
+    if args.full_rank:
+        full_rank(engine, synth, p)
+        return
+    if args.workload == "graph_complex":
+        print(json.dumps(graph_complex_leg(engine, synth, p, args.warmup)), flush=True)
+        return
 
-    from rust_rank_amd import engine, synth
-    large = args.workload == "large"
-    with_pm = not (large or args.no_pm)
-    p, b = synth.P_DEFAULT, args.b or (64 if large else 32)
+    b = 64
+    steps = args.steps or 200
     t0 = time.perf_counter()
-    if large:  # Krylov rate on a bounded stretch: the whole sequence is about 140k iterations
-        A = synth.large_single()
-        desc = f"large_single {A.m}x{A.n} 20 nnz/row uniform, values uniform in [1,p), deficit 5000, p=2013265921 b={b}"
-    else:
-        A = synth.graph_complex(m=args.m, n=args.n)
-        desc = f"graph_complex {A.m}x{A.n} 12 nnz/row Zipf(1.1) +-1/+-2 p=2013265921 b={b}"
+    A = synth.large_single()
     dr, dc, V = synth.preconditioners(2, A.m, A.n, b, p)
     L = synth.seq_len_plain(min(A.m, A.n), b)
-    steps = args.steps or (256 if large else L)
-    log(f"workload {A.m}x{A.n} nnz={A.nnz} b={b} L={L}; generated in {time.perf_counter() - t0:.1f}s")
-
-    K = engine.Krylov(A, p, b, dr, dc, V)
-    seq = [K.run(max(args.warmup, 1))]
-    t1 = time.perf_counter()
-    seq.append(K.run(steps))  # timed: no per-pass events
-    t2 = time.perf_counter()
-    have = sum(len(x) for x in seq)
-    if have < L and with_pm:  # the rest of the sequence for the PM-basis, untimed
-        seq.append(K.run(L - have))
-    seq = np.concatenate(seq)[:L]
-    # kernel breakdown on a further stretch of the same sequence, with HIP
-    # events around every launch on the engine's stream
-    K.set_profiling(True)
-    K.run(min(steps, 2000))
-    st = K.stats()
-    K.close()
-    wall = t2 - t1
-    iters_s = steps / wall
-    passes = st["passes_a"] + st["passes_at"]
-    pass_ms = (st["ms_a"] + st["ms_at"]) / max(passes, 1)
-    bytes_pass = (st["bytes_a"] * st["passes_a"] + st["bytes_at"] * st["passes_at"]) / max(passes, 1)
-    achieved = bytes_pass / (pass_ms * 1e-3) / 1e9
+    desc = (f"large_single {A.m}x{A.n} 20 nnz/row uniform, values uniform in [1,p), 5000 planted dependent "
+            f"columns, p=2013265921 b={b}")
+    log(f"workload {desc}: nnz={A.nnz} L={L}; generated in {time.perf_counter() - t0:.1f}s")
+    r, _ = krylov_leg(engine, A, p, b, dr, dc, V, steps, args.warmup, 40)
+    _, copy_gbs = engine.copy_probe(1 << 30, 20)
     b_iter = 8 * A.nnz + (4 * b + 2) * (A.m + A.n)
-    log(f"pass by A avg {st['ms_a'] / max(st['passes_a'], 1) * 1e3:.2f} us, by A^T avg "
-        f"{st['ms_at'] / max(st['passes_at'], 1) * 1e3:.2f} us")
-    log(f"timed {steps} iterations in {wall:.3f}s -> {iters_s:.1f} iters/s; profiled stretch: k_spmm avg "
-        f"{pass_ms * 1e3:.2f} us, "
-        f"reduce {st['ms_reduce']:.2f} ms, stream total {st['ms_total']:.1f} ms")
-
-    traffic = None
-    if os.path.exists(PROFILE_TRAFFIC) and not large:
-        with open(PROFILE_TRAFFIC) as f:
-            traffic = json.load(f).get("bytes_per_launch")
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": f"k_spmm<{1 << (b - 1).bit_length()}, {'general' if large else 'packed'}>",
-                "bytes_per_launch": int(bytes_pass), "avg_launch_us": round(pass_ms * 1e3, 3)}
-    # the rest of the rank pipeline on this sequence: device PM-basis of order
-    # L - 1, then the rank formula
-    pm = None
-    if with_pm:
-        t3 = time.perf_counter()
-        basis, delta = engine.pm_basis(seq, b, L - 1, p)
-        t4 = time.perf_counter()
-        rank = engine.rank_from_basis(basis, delta, b, p)
-        del basis
-        pm = {"b": b, "order": L - 1, "ms": round((t4 - t3) * 1e3, 1), "rank": int(rank),
-              "rank_pipeline_s": round(L / iters_s + (t4 - t3), 3)}
-        log(f"PM-basis of order {L - 1}: {pm['ms']} ms; rank {rank}")
-    cpu = None if (args.no_cpu or large) else cpu_baseline(A, p, b, dr, dc, V)
+    roof = roofline(r, traffic_from_profile("traffic_k_spmm_large_single.json"), "k_spmm<64, general>", copy_gbs)
+    cpu = None if args.no_cpu else cpu_baseline(A, p, b, dr, dc, V, 2, args.cpu_timed,
+                                                "5000000x4500000 b=64 workload")
+    nnz = A.nnz
+    del A
+    gc = None if args.no_graph_complex else graph_complex_leg(engine, synth, p, args.warmup)
     batched = served = None
-    if args.batched and not large:
+    if args.batched:
         batched, served = batched_and_served(engine, synth, p, args.batched, args.rate_frac, plain=not args.no_plain)
-    if pm is not None:
-        # the same rank through rr_rank: set-up, sequence and PM-basis in one
-        # call, the PM-basis running beside the passes on this workload (run
-        # last: the batched figures above were measured without it before them)
-        rr_rank, rr_ms = engine.rank(A, p, b, dr, dc, V, L, L - 1)
-        assert int(rr_rank) == pm["rank"], (rr_rank, pm["rank"])
-        pm["rr_rank_s"] = round(rr_ms / 1e3, 3)
     out = {
         "metric": "Krylov iters/s + HBM GB/s; batched ranks/s & p50/p99 vs plain, same GPU",
-        "value": round(iters_s, 2), "unit": "iters/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
-        "ms_per_step": round(wall * 1e3 / steps, 5), "higher_is_better": True, "scaling": "weak",
+        "value": round(r["iters_s"], 3), "unit": "iters/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": round(r["wall_s"] * 1e3 / steps, 5), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32", "data": "synthetic (seeded SplitMix64)",
-        "config": {"workload": desc,
-                   "seq_len": L, "nnz": A.nnz, "hbm_gbs_algorithmic": round(b_iter * iters_s / 1e9, 1)},
-        "roofline": roofline, "cpu_baseline": cpu, "pm_basis": pm, "batched": batched, "served": served,
+        "config": {"workload": desc, "seq_len": L, "nnz": nnz, "b": b,
+                   "hbm_gbs_algorithmic": round(b_iter * r["iters_s"] / 1e9, 1)},
+        "roofline": roof, "cpu_baseline": cpu, "graph_complex": gc, "batched": batched, "served": served,
     }
     print(json.dumps(out), flush=True)
 

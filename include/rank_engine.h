@@ -102,6 +102,7 @@ typedef struct {
     double ms_reduce;             /* sequence reduction kernels */
     double ms_total;              /* wall time of the run on the stream */
     uint64_t bytes_a, bytes_at;   /* algorithmic bytes of one pass by A / by A^T */
+    double ms_proj;               /* b > 32: projection kernels run after the passes (else 0: fused) */
 } rr_krylov_stats;
 rr_status rr_krylov_set_profiling(rr_krylov *ctx, int on);
 rr_status rr_krylov_stats_get(rr_krylov *ctx, rr_krylov_stats *out);

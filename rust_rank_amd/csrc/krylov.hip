@@ -42,6 +42,7 @@
 namespace rr {
 
 constexpr int CHUNK = 64;             // passes per sequence reduction
+constexpr uint64_t SEQ_RESERVE = 4096;  // sequence entries the device buffer holds from the start
 
 // Lane layout of a produced row: VW consecutive words per lane (one 4-, 8- or
 // 16-byte access), LPR lanes per row, RPW rows per wavefront.
@@ -107,12 +108,25 @@ __device__ __forceinline__ void vstore(uint32_t *dst, const uint32_t (&x)[VW]) {
     else *g = x[0];
 }
 
-// One sparse pass: every slice of this wavefront's stream, then the
+// Stand-in projection for passes whose projection runs in a kernel of its own
+// (k_proj64): it keeps nothing and writes nothing.
+struct GramNone {
+    static constexpr int LDS_WORDS = 1;
+    __device__ void init() {}
+    __device__ void add(const uint32_t *, uint32_t, uint32_t, double, double) {}
+    __device__ void add_slice(const uint32_t *, const uint32_t *, double, double) {}
+    __device__ void add_row(const uint32_t *, const uint32_t *, double, double) {}
+    __device__ void finish(uint32_t *, uint32_t, uint32_t, double, double, uint32_t *) {}
+};
+
+// One sparse pass: every slice of this wavefront's stream, then (PROJ) the
 // workgroup's projection partial.  Column blocks are loaded two blocks ahead
 // and gathered one block ahead of the block being accumulated, across slice
 // boundaries, so a wavefront keeps SU row gathers per row group plus a column
-// load in flight while it computes.
-template <int B, bool PACKED, int SU>
+// load in flight while it computes.  Without PROJ the pass only produces the
+// rows; the projection of lane width 64 (whose per-lane accumulators would
+// leave one wavefront per SIMD) is then taken by k_proj64.
+template <int B, bool PACKED, int SU, bool PROJ>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_spmm(
     const SpmmArgs *__restrict__ jobs, const uint32_t *__restrict__ wg_begin, uint32_t njobs, uint32_t *gram_base) {
     // this workgroup's job: the last j with wg_begin[j] <= blockIdx.x
@@ -126,11 +140,11 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     const uint32_t lwg = blockIdx.x - wg_begin[jlo];  // workgroup within the job
     using L = Lay<B>;
     constexpr int VW = L::VW, LPR = L::LPR, RPW = L::RPW, BW = RPW * SU;
-    __shared__ uint32_t o2s[WG / 64][RPW * 2 * B];  // produced rows, each twice
-    __shared__ uint32_t ss[WG / 64][RPW * B];       // their scaled rows
+    __shared__ uint32_t o2s[WG / 64][PROJ ? RPW * 2 * B : 1];  // produced rows, each twice
+    __shared__ uint32_t ss[WG / 64][RPW * B];                   // their scaled rows (segments: partials)
     // the matrix-core projection where the registers allow it (no spills)
     constexpr bool MMA = B == 32 && PACKED && SU == 3;
-    using GramT = std::conditional_t<MMA, GramMma32, GramRing<B>>;
+    using GramT = std::conditional_t<!PROJ, GramNone, std::conditional_t<MMA, GramMma32, GramRing<B>>>;
     __shared__ uint32_t fin[GramT::LDS_WORDS];
     __shared__ uint32_t pend[WG / 64][MAX_PEND];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -218,13 +232,15 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
                 s[c] = mulmod(o[c], un.y, a.p, a.inv);
             }
             if (ok) vstore<VW>(a.out + (size_t)un.x * B + q * VW, s);
+            if constexpr (PROJ) {
 #pragma unroll
-            for (int c = 0; c < VW; c++) {
-                o2[g * 2 * B + q * VW + c] = o[c];
-                o2[g * 2 * B + B + q * VW + c] = o[c];
-                sv[g * B + q * VW + c] = s[c];
+                for (int c = 0; c < VW; c++) {
+                    o2[g * 2 * B + q * VW + c] = o[c];
+                    o2[g * 2 * B + B + q * VW + c] = o[c];
+                    sv[g * B + q * VW + c] = s[c];
+                }
+                rows_ready = true;
             }
-            rows_ready = true;
         }
         zero_acc();
         sl++;
@@ -348,18 +364,24 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
         wave_sync();
         if (hh == 0) {
             as_global(a.out)[(size_t)li.x * B + ci] = sj;
-            o2[ci] = oj;
-            o2[B + ci] = oj;
-            sv[ci] = sj;
+            if constexpr (PROJ) {
+                o2[ci] = oj;
+                o2[B + ci] = oj;
+                sv[ci] = sj;
+            }
         }
         if (lane == 0) atomicExch(a.lcnt + l, 0u);
         wave_sync();
-        if constexpr (MMA) gr.add_row(o2, sv, a.pd, a.inv);
-        else gr.add(o2, sv[ci], a.p, a.pd, a.inv);
+        if constexpr (PROJ) {
+            if constexpr (MMA) gr.add_row(o2, sv, a.pd, a.inv);
+            else gr.add(o2, sv[ci], a.p, a.pd, a.inv);
+        }
         wave_sync();
     }
-    __syncthreads();
-    gr.finish(gram_base + a.gram_off + (size_t)lwg * (a.b * (a.b + 1) / 2), a.b, a.p, a.pd, a.inv, fin);
+    if constexpr (PROJ) {
+        __syncthreads();
+        gr.finish(gram_base + a.gram_off + (size_t)lwg * (a.b * (a.b + 1) / 2), a.b, a.p, a.pd, a.inv, fin);
+    }
 }
 
 // S = x^T x of a block vector (the t = 0 entry), partials per workgroup.
@@ -381,6 +403,115 @@ __global__ __launch_bounds__(WG) void k_gram(const uint32_t *x, uint32_t nrows, 
         g.add(o, o, lane, wl[wave], pd, inv);
     }
     g.finish(gram + (size_t)blockIdx.x * (b * (b + 1) / 2), b, p, pd, inv, lds);
+}
+
+// The projection of one pass at lane width 64: the workgroup's partial of
+// S = sum_r o_r s_r^T over the rows the pass produced (s, stride 64 words),
+// with o_r = s_r * sinv_r (sinv: the inverse of the row's scale).  Rows are
+// staged 64 at a time in LDS (o and s, row pitch 80 words so the four rows an
+// operand read spans fall in different banks); the 10 tiles of 16 x 16 in the
+// 64 x 64 upper triangle are v_mfma_f64_16x16x4f64 products of o with the
+// 16-bit halves of s, every product an integer below 2^47 and 64 rows of them
+// below 2^53, so the fp64 sums are exact and are folded mod p once per chunk.
+// Wavefront w owns tiles 2w and 2w + 1 for all rows and tile 8 + w/2 for the
+// even (w even) or odd half of each chunk's row groups: 2.5 tiles each.
+constexpr int PJ_ROWS = 64, PJ_PITCH = 80;
+__global__ __launch_bounds__(WG) void k_proj64(const uint32_t *__restrict__ s, const uint32_t *__restrict__ sinv,
+                                               uint32_t nrows, uint32_t *gram, uint32_t b, uint32_t p, double pd,
+                                               double inv) {
+    __shared__ uint32_t lds[2 * PJ_ROWS * PJ_PITCH];  // o rows, then s rows; at the end the wavefronts' triangles
+    uint32_t *lo = lds, *ls = lds + PJ_ROWS * PJ_PITCH;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int k = lane >> 4, i = lane & 15;
+    // tiles (I, J), I <= J, in the order 2w, 2w + 1 (all row groups), 8 + w / 2 (half of them)
+    constexpr uint8_t TI[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3}, TJ[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+    const int t0 = 2 * wave, t1 = 2 * wave + 1, t2 = 8 + (wave >> 1), half = wave & 1;
+    gram_v4d acc[6];
+#pragma unroll
+    for (int t = 0; t < 6; t++) acc[t] = gram_v4d{0.0, 0.0, 0.0, 0.0};
+    const uint32_t nchunks = (nrows + PJ_ROWS - 1) / PJ_ROWS;
+    // staging: thread -> row tid / 4, words 16 (tid % 4) .. + 15
+    const uint32_t srow = tid >> 2, sw = (tid & 3) * 16;
+    gv4u rv[4];
+    uint32_t rinv = 0;
+    auto load = [&](uint32_t c) {
+        const uint32_t r = c * PJ_ROWS + srow;
+        const bool ok = r < nrows;
+        const GLOBAL_AS gv4u *src = (const GLOBAL_AS gv4u *)(as_global(s) + (size_t)(ok ? r : 0) * 64 + sw);
+#pragma unroll
+        for (int u = 0; u < 4; u++) rv[u] = ok ? src[u] : gv4u{0, 0, 0, 0};
+        rinv = ok ? as_global(sinv)[r] : 0u;
+    };
+    uint32_t c = blockIdx.x;
+    if (c < nchunks) load(c);
+    for (; c < nchunks; c += gridDim.x) {
+        __syncthreads();  // the previous chunk's products have read the LDS rows
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const uint32_t sv = rv[u][e];
+                ls[srow * PJ_PITCH + sw + 4 * u + e] = sv;
+                lo[srow * PJ_PITCH + sw + 4 * u + e] = mulmod(sv, rinv, p, inv);
+            }
+        if (c + gridDim.x < nchunks) load(c + gridDim.x);  // in flight during the products
+        __syncthreads();
+#pragma unroll 2
+        for (int g = 0; g < PJ_ROWS / 4; g++) {
+            const int r = 4 * g + k;
+            const uint32_t *orow = lo + r * PJ_PITCH, *srw = ls + r * PJ_PITCH;
+            auto tile = [&](int t, int slot) {
+                const uint32_t sv = srw[16 * TJ[t] + i];
+                const double a = (double)orow[16 * TI[t] + i];
+                acc[2 * slot] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, (double)(sv >> 16), acc[2 * slot], 0, 0, 0);
+                acc[2 * slot + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, (double)(sv & 0xFFFFu), acc[2 * slot + 1], 0, 0, 0);
+            };
+            tile(t0, 0);
+            tile(t1, 1);
+            if ((g & 1) == half) tile(t2, 2);
+        }
+#pragma unroll
+        for (int t = 0; t < 6; t++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[t][v] = fold(acc[t][v], pd, inv);
+    }
+    // the wavefronts' triangles (zero where a wavefront owns no entry), then their sum
+    const uint32_t Pb = b * (b + 1) / 2;
+    __syncthreads();
+    for (uint32_t t = lane; t < Pb; t += 64) lds[wave * Pb + t] = 0;
+    wave_sync();
+    const int tl[3] = {t0, t1, t2};
+#pragma unroll
+    for (int sl = 0; sl < 3; sl++) {
+        const int t = tl[sl];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const uint32_t ii = 16 * TI[t] + 4 * v + k, jj = 16 * TJ[t] + i;
+            if (ii <= jj && jj < b) {
+                const double h = fold(acc[2 * sl][v], pd, inv), l = fold(acc[2 * sl + 1][v], pd, inv);
+                lds[wave * Pb + ii * b - ii * (ii - 1) / 2 + (jj - ii)] = (uint32_t)fold(fma(h, 65536.0, l), pd, inv);
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = tid; t < Pb; t += WG) {
+        uint32_t sum = 0;
+        for (int w = 0; w < WG / 64; w++) sum = addmod(sum, lds[w * Pb + t], p);
+        as_global(gram)[(size_t)blockIdx.x * Pb + t] = sum;
+    }
+}
+
+// out[i] = in[i]^-1 mod p (entries in [1, p), p prime), by Fermat: in^(p-2).
+__global__ void k_inv_mod(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t p, double inv) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    uint32_t x = in[idx], r = 1, e = p - 2;
+    while (e) {
+        if (e & 1) r = mulmod(r, x, p, inv);
+        x = mulmod(x, x, p, inv);
+        e >>= 1;
+    }
+    out[idx] = r;
 }
 
 // seq[t][ii] = sum_w part[t][w][ii] mod p.  Workgroup (ii block of 64, t):
@@ -433,6 +564,10 @@ struct rr_krylov {
     rr::ModP mp;
     rr::SliceMat A, At;
     rr::DevBuf dr, dc, dc2, dcinv, z, y, xtmp, part, seq;
+    // lane width 64: the projection runs in k_proj64 over the produced rows,
+    // which needs the inverses of their scales (D_r^-1, D_c^-2)
+    bool sep_proj = false;
+    rr::DevBuf drinv, dc2inv;
     rr::DevBuf jobs;  // [2] pass descriptors (by A, by A^T) and the one-job workgroup table {0, grid}
     uint64_t seq_cap = 0;  // entries the device sequence buffer holds
     uint32_t grid = 0, P = 0;
@@ -713,55 +848,73 @@ SpmmArgs spmm_args(const SliceMat &M, const uint32_t *in, uint64_t in_rows, uint
     return a;
 }
 
-template <int B>
-static void launch_b(bool packed, uint32_t su, const SpmmArgs *jobs, const uint32_t *wg_begin, uint32_t njobs,
-                     uint32_t total_wg, uint32_t *gram_base, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+template <int B, bool PROJ>
+static void launch_bp(bool packed, uint32_t su, const SpmmArgs *jobs, const uint32_t *wg_begin, uint32_t njobs,
+                      uint32_t total_wg, uint32_t *gram_base, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     dim3 g(total_wg), t(WG);
     // with events, the dispatch itself stamps its start and end (no marker between launches)
     if (packed && su == 3)
-        hipExtLaunchKernelGGL((k_spmm<B, true, 3>), g, t, 0, st, ev0, ev1, 0, jobs, wg_begin, njobs, gram_base);
+        hipExtLaunchKernelGGL((k_spmm<B, true, 3, PROJ>), g, t, 0, st, ev0, ev1, 0, jobs, wg_begin, njobs, gram_base);
     else if (packed)
-        hipExtLaunchKernelGGL((k_spmm<B, true, 4>), g, t, 0, st, ev0, ev1, 0, jobs, wg_begin, njobs, gram_base);
+        hipExtLaunchKernelGGL((k_spmm<B, true, 4, PROJ>), g, t, 0, st, ev0, ev1, 0, jobs, wg_begin, njobs, gram_base);
     else if (su == 3)
-        hipExtLaunchKernelGGL((k_spmm<B, false, 3>), g, t, 0, st, ev0, ev1, 0, jobs, wg_begin, njobs, gram_base);
+        hipExtLaunchKernelGGL((k_spmm<B, false, 3, PROJ>), g, t, 0, st, ev0, ev1, 0, jobs, wg_begin, njobs, gram_base);
     else
-        hipExtLaunchKernelGGL((k_spmm<B, false, 4>), g, t, 0, st, ev0, ev1, 0, jobs, wg_begin, njobs, gram_base);
+        hipExtLaunchKernelGGL((k_spmm<B, false, 4, PROJ>), g, t, 0, st, ev0, ev1, 0, jobs, wg_begin, njobs, gram_base);
+}
+
+template <int B>
+static void launch_b(bool packed, uint32_t su, bool proj, const SpmmArgs *jobs, const uint32_t *wg_begin,
+                     uint32_t njobs, uint32_t total_wg, uint32_t *gram_base, hipStream_t st, hipEvent_t ev0,
+                     hipEvent_t ev1) {
+    if constexpr (B == 64) {
+        if (!proj) return launch_bp<B, false>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1);
+    }
+    launch_bp<B, true>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1);
 }
 
 void launch_spmm(uint32_t B, bool packed, uint32_t su, const SpmmArgs *jobs, const uint32_t *wg_begin,
                  uint32_t njobs, uint32_t total_wg, uint32_t *gram_base, hipStream_t st, hipEvent_t ev0,
-                 hipEvent_t ev1) {
+                 hipEvent_t ev1, bool proj) {
     switch (B) {
-        case 1: launch_b<1>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
-        case 2: launch_b<2>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
-        case 4: launch_b<4>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
-        case 8: launch_b<8>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
-        case 16: launch_b<16>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
-        case 32: launch_b<32>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
-        default: launch_b<64>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
+        case 1: launch_b<1>(packed, su, proj, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
+        case 2: launch_b<2>(packed, su, proj, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
+        case 4: launch_b<4>(packed, su, proj, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
+        case 8: launch_b<8>(packed, su, proj, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
+        case 16: launch_b<16>(packed, su, proj, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
+        case 32: launch_b<32>(packed, su, proj, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
+        default: launch_b<64>(packed, su, proj, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
     }
 }
 
-template <int B>
-static int blocks_b(bool packed, uint32_t su) {
+template <int B, bool PROJ>
+static int blocks_bp(bool packed, uint32_t su) {
     int n = 0;
     hipError_t e;
-    if (packed && su == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_spmm<B, true, 3>, WG, 0);
-    else if (packed) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_spmm<B, true, 4>, WG, 0);
-    else if (su == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_spmm<B, false, 3>, WG, 0);
-    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_spmm<B, false, 4>, WG, 0);
+    if (packed && su == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_spmm<B, true, 3, PROJ>, WG, 0);
+    else if (packed) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_spmm<B, true, 4, PROJ>, WG, 0);
+    else if (su == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_spmm<B, false, 3, PROJ>, WG, 0);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_spmm<B, false, 4, PROJ>, WG, 0);
     return (e == hipSuccess && n > 0) ? n : 2;
 }
 
-int spmm_blocks_per_cu(uint32_t B, bool packed, uint32_t su) {
+template <int B>
+static int blocks_b(bool packed, uint32_t su, bool proj) {
+    if constexpr (B == 64) {
+        if (!proj) return blocks_bp<B, false>(packed, su);
+    }
+    return blocks_bp<B, true>(packed, su);
+}
+
+int spmm_blocks_per_cu(uint32_t B, bool packed, uint32_t su, bool proj) {
     switch (B) {
-        case 1: return blocks_b<1>(packed, su);
-        case 2: return blocks_b<2>(packed, su);
-        case 4: return blocks_b<4>(packed, su);
-        case 8: return blocks_b<8>(packed, su);
-        case 16: return blocks_b<16>(packed, su);
-        case 32: return blocks_b<32>(packed, su);
-        default: return blocks_b<64>(packed, su);
+        case 1: return blocks_b<1>(packed, su, proj);
+        case 2: return blocks_b<2>(packed, su, proj);
+        case 4: return blocks_b<4>(packed, su, proj);
+        case 8: return blocks_b<8>(packed, su, proj);
+        case 16: return blocks_b<16>(packed, su, proj);
+        case 32: return blocks_b<32>(packed, su, proj);
+        default: return blocks_b<64>(packed, su, proj);
     }
 }
 
@@ -913,8 +1066,9 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
         dcinv[i] = h_invmod(col_precond[i], p);
     }
     c->su = choose_su(rp, rpt);
+    c->sep_proj = c->B == 64;
     // one resident wave of workgroups; every wavefront owns a fixed stream of slices
-    c->grid = (uint32_t)std::max(1, cus * spmm_blocks_per_cu(c->B, small, c->su));
+    c->grid = (uint32_t)std::max(1, cus * spmm_blocks_per_cu(c->B, small, c->su, !c->sep_proj));
     rr_status s;
     if ((s = build_slices(c->A, m, rp, colw, vals, small, c->B, c->su, c->grid * (WG / 64), row_precond, n)) !=
         RR_OK)
@@ -926,6 +1080,15 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
     if ((s = upload(c->dc, col_precond, n)) != RR_OK) return s;
     if ((s = upload(c->dc2, dc2.data(), n)) != RR_OK) return s;
     if ((s = upload(c->dcinv, dcinv.data(), n)) != RR_OK) return s;
+    if (c->sep_proj) {
+        if ((s = c->drinv.alloc((size_t)m * 4)) != RR_OK) return s;
+        if ((s = c->dc2inv.alloc((size_t)n * 4)) != RR_OK) return s;
+        hipLaunchKernelGGL(k_inv_mod, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->stream,
+                           c->dr.as<uint32_t>(), c->drinv.as<uint32_t>(), m, p, c->mp.inv);
+        hipLaunchKernelGGL(k_inv_mod, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                           c->dc2.as<uint32_t>(), c->dc2inv.as<uint32_t>(), n, p, c->mp.inv);
+        RR_HIP(hipGetLastError());
+    }
 
     // block vectors, padded to stride B
     const uint32_t B = c->B;
@@ -952,6 +1115,9 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
     }
     uint32_t nparts = c->grid;
     if ((s = c->part.alloc((size_t)CHUNK * nparts * c->P * 4)) != RR_OK) return s;
+    // room for SEQ_RESERVE entries, so short runs never reallocate
+    if ((s = c->seq.alloc((size_t)SEQ_RESERVE * c->P * 4)) != RR_OK) return s;
+    c->seq_cap = SEQ_RESERVE;
     RR_HIP(hipMemsetAsync(c->part.ptr, 0, (size_t)CHUNK * nparts * c->P * 4, c->stream));
     RR_HIP(hipStreamSynchronize(c->stream));
     // algorithmic bytes per pass (int32 columns, u32 values, each array once)
@@ -989,16 +1155,19 @@ rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
     hipEvent_t e_begin = nullptr, e_end = nullptr;
     RR_HIP(hipEventCreate(&e_begin));
     RR_HIP(hipEventCreate(&e_end));
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> red_ev;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> red_ev, proj_ev;
     RR_HIP(hipEventRecord(e_begin, c->stream));
 
-    if (count > c->seq_cap) {
+    if (count > c->seq_cap) {  // grows geometrically; the old buffer goes back to the cache
+        uint64_t cap = c->seq_cap;
+        while (cap < count) cap *= 2;
         RR_HIP(hipStreamSynchronize(c->stream));
-        c->seq.~DevBuf();
-        new (&c->seq) DevBuf();
-        rr_status sa = c->seq.alloc((size_t)count * P * 4);
-        if (sa != RR_OK) return sa;
-        c->seq_cap = count;
+        rr_status sa = c->seq.alloc((size_t)cap * P * 4);
+        if (sa != RR_OK) {
+            c->seq_cap = 0;
+            return sa;
+        }
+        c->seq_cap = cap;
     }
     uint64_t done = 0;
     while (done < count) {
@@ -1023,7 +1192,20 @@ rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
                 bool byA = (t & 1) == 1;
                 const SpmmArgs *job = c->jobs.as<SpmmArgs>() + (byA ? 0 : 1);
                 const uint32_t *wgs = (const uint32_t *)(c->jobs.as<char>() + 2 * sizeof(SpmmArgs));
-                launch_spmm(c->B, c->packed, c->su, job, wgs, 1, c->grid, gram, c->stream, e0, e1);
+                launch_spmm(c->B, c->packed, c->su, job, wgs, 1, c->grid, gram, c->stream, e0, e1, !c->sep_proj);
+                if (c->sep_proj) {  // the projection of the rows this pass produced
+                    const uint32_t *rows = byA ? c->y.as<uint32_t>() : c->z.as<uint32_t>();
+                    const uint32_t *sinv = byA ? c->drinv.as<uint32_t>() : c->dc2inv.as<uint32_t>();
+                    const uint32_t nr = (uint32_t)(byA ? c->m : c->n);
+                    hipEvent_t g0 = nullptr, g1 = nullptr;
+                    if (c->profiling) {
+                        RR_HIP(hipEventCreate(&g0));
+                        RR_HIP(hipEventCreate(&g1));
+                    }
+                    hipExtLaunchKernelGGL(k_proj64, dim3(c->grid), dim3(WG), 0, c->stream, g0, g1, 0, rows, sinv, nr,
+                                          gram, c->b, c->p, c->mp.pd, c->mp.inv);
+                    if (c->profiling) proj_ev.push_back({g0, g1});
+                }
                 k = byA ? 1 : 2;
             }
             RR_HIP(hipGetLastError());
@@ -1070,11 +1252,17 @@ rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
             RR_HIP(hipEventElapsedTime(&r, pr.first, pr.second));
             c->stats.ms_reduce += r;
         }
-        for (auto e : ev) (void)hipEventDestroy(e);
-        for (auto &pr : red_ev) {
-            (void)hipEventDestroy(pr.first);
-            (void)hipEventDestroy(pr.second);
+        for (auto &pr : proj_ev) {
+            float r = 0;
+            RR_HIP(hipEventElapsedTime(&r, pr.first, pr.second));
+            c->stats.ms_proj += r;
         }
+        for (auto e : ev) (void)hipEventDestroy(e);
+        for (auto *v : {&red_ev, &proj_ev})
+            for (auto &pr : *v) {
+                (void)hipEventDestroy(pr.first);
+                (void)hipEventDestroy(pr.second);
+            }
     }
     (void)hipEventDestroy(e_begin);
     (void)hipEventDestroy(e_end);

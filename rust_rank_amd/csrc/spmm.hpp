@@ -73,13 +73,15 @@ rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> 
 SpmmArgs spmm_args(const SliceMat &M, const uint32_t *in, uint64_t in_rows, uint32_t B, uint32_t *out,
                    const uint32_t *scale, uint32_t p, uint32_t b);
 
-// Resident workgroups per CU of the pass kernel for (B, packed, SU).
-int spmm_blocks_per_cu(uint32_t B, bool packed, uint32_t su);
+// Resident workgroups per CU of the pass kernel for (B, packed, SU, proj).
+int spmm_blocks_per_cu(uint32_t B, bool packed, uint32_t su, bool proj = true);
 
 // One launch over njobs jobs (device arrays); total_wg = wg_begin[njobs].
 // ev0 / ev1 (may be null): stamped with the kernel's own start and end.
+// proj = false (lane width 64 only): the pass produces its rows and writes no
+// projection partials; k_proj64 then takes the projection.
 void launch_spmm(uint32_t B, bool packed, uint32_t su, const SpmmArgs *jobs, const uint32_t *wg_begin,
                  uint32_t njobs, uint32_t total_wg, uint32_t *gram_base, hipStream_t st, hipEvent_t ev0 = nullptr,
-                 hipEvent_t ev1 = nullptr);
+                 hipEvent_t ev1 = nullptr, bool proj = true);
 
 }  // namespace rr

@@ -36,6 +36,25 @@ def _mix(z: np.ndarray) -> np.ndarray:
     return z ^ (z >> np.uint64(31))
 
 
+_PAR_MIN = 1 << 22  # work items below which the helpers below stay on one thread
+
+
+def _blocks(n: int, align: int = 1):
+    """[lo, hi) ranges splitting n items (multiples of align) over up to 16 threads."""
+    t = max(1, min(16, os.cpu_count() or 1))
+    step = max(align, (n + t - 1) // t // align * align or align)
+    return [(lo, min(n, lo + step)) for lo in range(0, n, step)]
+
+
+def _pmap(fn, ranges):
+    """fn(lo, hi) over the ranges on a thread pool (numpy releases the GIL)."""
+    if len(ranges) <= 1:
+        return [fn(lo, hi) for lo, hi in ranges]
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=len(ranges)) as ex:
+        return list(ex.map(lambda r: fn(*r), ranges))
+
+
 class Rng:
     """SplitMix64 stream: output i = mix(key + (i + 1) * golden)."""
 
@@ -47,9 +66,15 @@ class Rng:
         self.ctr = 0
 
     def u64(self, n: int) -> np.ndarray:
-        with np.errstate(over="ignore"):
-            idx = np.arange(self.ctr + 1, self.ctr + 1 + n, dtype=np.uint64)
-            out = _mix(self.key + idx * _GOLD)
+        out = np.empty(n, dtype=np.uint64)
+        base = self.ctr + 1
+
+        def fill(lo, hi):
+            with np.errstate(over="ignore"):
+                idx = np.arange(base + lo, base + hi, dtype=np.uint64)
+                out[lo:hi] = _mix(self.key + idx * _GOLD)
+
+        _pmap(fill, _blocks(n) if n >= _PAR_MIN else [(0, n)])
         self.ctr += n
         return out
 
@@ -86,6 +111,15 @@ class Csr:
 def _distinct_rows(cands: np.ndarray, k: np.ndarray | int):
     """For each row of cands (R x K), the first k distinct entries in draw order.
     Returns (flat selected values in row order, per-row count found)."""
+    R, K = cands.shape
+    if R * K >= _PAR_MIN:  # rows are independent: blocks of rows on threads
+        kk = np.broadcast_to(np.asarray(k), (R,)) if np.ndim(k) else None
+        parts = _pmap(lambda lo, hi: _distinct_rows_1(cands[lo:hi], k if kk is None else kk[lo:hi]), _blocks(R))
+        return np.concatenate([v for v, _ in parts]), np.concatenate([f for _, f in parts])
+    return _distinct_rows_1(cands, k)
+
+
+def _distinct_rows_1(cands: np.ndarray, k: np.ndarray | int):
     R, K = cands.shape
     order = np.argsort(cands, axis=1, kind="stable")
     srt = np.take_along_axis(cands, order, axis=1)
@@ -133,7 +167,20 @@ def _rows_to_csr(m: int, n: int, cols_per_row: list, vals_per_row: list) -> Csr:
 def _fixed_k_csr(m: int, n: int, k: np.ndarray, cols_flat: np.ndarray, vals_flat: np.ndarray) -> Csr:
     rp = np.zeros(m + 1, dtype=np.uint64)
     np.cumsum(k.astype(np.uint64), out=rp[1:])
-    # sort columns inside each row
+    # sort columns inside each row (distinct within a row, so the order is unique)
+    if m and np.all(k == k[0]):
+        kk = int(k[0])
+        c2, v2 = cols_flat.reshape(m, kk), vals_flat.reshape(m, kk)
+        col = np.empty((m, kk), dtype=np.uint32)
+        val = np.empty((m, kk), dtype=np.uint32)
+
+        def srt(lo, hi):
+            o2 = np.argsort(c2[lo:hi], axis=1, kind="stable")
+            col[lo:hi] = np.take_along_axis(c2[lo:hi], o2, axis=1)
+            val[lo:hi] = np.take_along_axis(v2[lo:hi], o2, axis=1)
+
+        _pmap(srt, _blocks(m) if m * kk >= _PAR_MIN else [(0, m)])
+        return Csr(m, n, rp, col.reshape(-1), val.reshape(-1))
     rows = np.repeat(np.arange(m, dtype=np.int64), k.astype(np.int64))
     order = np.argsort(rows * n + cols_flat.astype(np.int64), kind="stable")  # by row, then column
     return Csr(m, n, rp, cols_flat[order].astype(np.uint32), vals_flat[order].astype(np.uint32))
@@ -181,38 +228,53 @@ def plant_dependent_columns(rng: Rng, A: Csr, k: int, p: int, small: bool) -> Cs
         coef = np.where(rng.u64(k * terms) & np.uint64(1), p - 1, 1).astype(np.uint64).reshape(k, terms)
     else:
         coef = rng.integers(1, p, k * terms).astype(np.uint64).reshape(k, terms)
+    # Only the entries of the source columns move: collect them (each with
+    # every (t, s) pair that reads its column), merge the new entries of each
+    # (row, dependent column) mod p, and splice them into the kept rows.
     rows = np.repeat(np.arange(m, dtype=np.int64), np.diff(A.row_ptr.astype(np.int64)))
     col = A.col.astype(np.int64)
-    val = A.val.astype(np.uint64)
     is_dep = np.zeros(n, dtype=bool)
     is_dep[dep] = True
-    keep = ~is_dep[col]
-    rows_k, col_k, val_k = rows[keep], col[keep], val[keep]
-    # for every (dependent column t, term s) collect the entries of column src[t, s]
-    # (t, s) in row-major order, each term's entries in the column's row order
-    order = np.argsort(col_k, kind="stable")
-    starts = np.searchsorted(col_k[order], np.arange(n + 1))
     cflat = src.reshape(-1)
+    is_src = np.zeros(n, dtype=bool)
+    is_src[cflat] = True
+    sel = np.nonzero(is_src[col] & ~is_dep[col])[0]  # entries of the source columns (row order)
+    srow, scol, sval = rows[sel], col[sel], A.val[sel].astype(np.uint64)
+    so = np.argsort(scol, kind="stable")  # by column, each column in row order
+    srow, scol, sval = srow[so], scol[so], sval[so]
+    starts = np.searchsorted(scol, np.arange(n + 1))
     cnt = starts[cflat + 1] - starts[cflat]
     first = np.repeat(starts[cflat] - np.concatenate([[0], np.cumsum(cnt)[:-1]]), cnt)
-    idx = order[first + np.arange(int(cnt.sum()))]
+    idx = first + np.arange(int(cnt.sum()))
     pair = np.repeat(np.arange(cflat.size), cnt)
-    r = np.concatenate([rows_k, rows_k[idx]])
-    c = np.concatenate([col_k, dep[pair // terms].astype(np.int64)])
-    v = np.concatenate([val_k, val_k[idx] * coef.reshape(-1)[pair] % np.uint64(p)])
-    # merge duplicates (same row, same dependent column) mod p, drop exact zeros
-    key = r * n + c
+    nr = srow[idx]
+    nc = dep[pair // terms].astype(np.int64)
+    nv = sval[idx] * coef.reshape(-1)[pair] % np.uint64(p)
+    key = nr * n + nc
     o = np.argsort(key, kind="stable")
-    key, v = key[o], v[o]
-    uk, first = np.unique(key, return_index=True)
-    sums = np.add.reduceat(v, first) % np.uint64(p) if v.size else v
+    key, nv = key[o], nv[o]
+    uk, fst = np.unique(key, return_index=True)
+    sums = np.add.reduceat(nv, fst) % np.uint64(p) if nv.size else nv
     nz = sums != 0
     uk, sums = uk[nz], sums[nz]
-    rr = uk // n
-    cc = uk % n
+    keep = ~is_dep[col]
+    krow, kcol, kval = rows[keep], col[keep], A.val[keep]
+    kkey = krow * n + kcol  # ascending: rows in order, columns sorted within rows
+    assert kkey.size < 2 or bool(np.all(kkey[1:] > kkey[:-1])), "rows must hold sorted, distinct columns"
+    pos = np.searchsorted(kkey, uk) + np.arange(uk.size)  # final places of the new entries
+    tot = kkey.size + uk.size
+    isnew = np.zeros(tot, dtype=bool)
+    isnew[pos] = True
+    out_c = np.empty(tot, dtype=np.uint32)
+    out_v = np.empty(tot, dtype=np.uint32)
+    out_c[pos] = (uk % n).astype(np.uint32)
+    out_v[pos] = sums.astype(np.uint32)
+    out_c[~isnew] = kcol.astype(np.uint32)
+    out_v[~isnew] = kval
+    cnt_row = np.bincount(krow, minlength=m) + np.bincount((uk // n).astype(np.int64), minlength=m)
     rp = np.zeros(m + 1, dtype=np.uint64)
-    np.cumsum(np.bincount(rr.astype(np.int64), minlength=m), out=rp[1:])
-    return Csr(m, n, rp, cc.astype(np.uint32), sums.astype(np.uint32))
+    np.cumsum(cnt_row, out=rp[1:])
+    return Csr(m, n, rp, out_c, out_v)
 
 
 def plain_small(seed: int = 1, p: int = P_DEFAULT, m_base: int = 1800, n: int = 1800,

@@ -31,13 +31,13 @@ def _case(A, p, b, seed, L, t_have=0):
     assert np.array_equal(curv, x_want)
 
 
-@pytest.mark.parametrize("b", [1, 2, 3, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("b", [1, 2, 3, 4, 8, 16, 32, 48, 64])
 def test_general_values(b):
     A = synth.uniform_sparse(b, 1, 300, 260, 1, 9, P, small=False)
     _case(A, P, b, b, 2 * 260 // b + 8)
 
 
-@pytest.mark.parametrize("b", [4, 8, 32])
+@pytest.mark.parametrize("b", [4, 8, 32, 64])
 def test_small_values_packed(b):
     A = synth.batch_matrix(3, b, rows_lo=400, rows_hi=600)
     _case(A, P, b, b + 1, 40)
@@ -53,6 +53,21 @@ def test_long_rows_split():
     A = synth.graph_complex(seed=5, m=40000, n=2000, k=12)
     assert int(np.bincount(A.col).max()) > 4096
     _case(A, P, 8, 5, 12)
+
+
+def test_long_rows_split_b64():
+    """Lane width 64 (projection in its own kernel) with split A^T rows finished in-pass."""
+    A = synth.graph_complex(seed=5, m=40000, n=2000, k=12)
+    _case(A, P, 64, 5, 12)
+
+
+@pytest.mark.timeout(900)
+def test_large_single_full_size_prefix():
+    """The 5000000 x 4500000, 20 nonzeros per row, b = 64 workload itself
+    (general values, 5000 planted dependent columns): S_0..S_5, three passes
+    by A and two by A^T, and the state x_2, against the oracle."""
+    A = synth.large_single()
+    _case(A, P, 64, 2, 6)
 
 
 def test_graph_complex_full_size_prefix():

@@ -82,6 +82,75 @@ def test_pm_basis_matches_oracle_big_p(b, d):
     assert np.array_equal(basis, want_b)
 
 
+def _seq_to_g(seq, b, D):
+    """G = [S(x); I_b] as [2b][b][D] (src/sigma_basis.rs:9-54)."""
+    G = np.zeros((2 * b, b, D), np.uint32)
+    iu = np.triu_indices(b)
+    for k in range(D):
+        S = np.zeros((b, b), np.uint32)
+        S[iu] = seq[k]
+        G[:b, :, k] = S + np.triu(S, 1).T
+    G[b:, :, 0] = np.eye(b, dtype=np.uint32)
+    return G
+
+
+def _left_combine(u, M, p):
+    """sum_i u_i M[i] mod p, exact in uint64 (u split in 16-bit halves)."""
+    hi = np.zeros(M.shape[1:], np.uint64)
+    lo = np.zeros(M.shape[1:], np.uint64)
+    for i in range(M.shape[0]):
+        Mi = M[i].astype(np.uint64)
+        hi += np.uint64(int(u[i]) >> 16) * Mi
+        lo += np.uint64(int(u[i]) & 0xFFFF) * Mi
+        if i % 64 == 63:
+            hi %= np.uint64(p)
+            lo %= np.uint64(p)
+    return ((hi % np.uint64(p)) * np.uint64(65536) + lo) % np.uint64(p)
+
+
+@pytest.mark.timeout(900)
+def test_pm_basis_b64_four_step_invariants():
+    """b = 64, d = 2^15 (2b x 2b = 128 x 128 bases; the top products are
+    four-step NTTs at 2^16 points).  The CPU oracle is too slow at this size,
+    so the basis is checked against what fixes it: u^T M(x) G(x) v = 0 mod x^d
+    for random u, v (products by the oracle's NTT), det M(z) = +-z^(-sum delta)
+    at a random z, and every row degree bounded by -delta."""
+    eng = _eng()
+    b, d = 64, 1 << 15
+    print(f"pm_basis b={b} d={d}", flush=True)
+    rng = np.random.default_rng(64015)
+    seq = rng.integers(0, P, (d + 2, b * (b + 1) // 2)).astype(np.uint32)
+    M, delta = eng.pm_basis(seq, b, d, P)
+    D = M.shape[2]
+    assert M.shape == (2 * b, 2 * b, d + 1)
+    # order: u^T M G v vanishes below x^d
+    u = rng.integers(0, P, 2 * b)
+    v = rng.integers(0, P, b)
+    a = _left_combine(u, M, P)                                  # [2b][D]
+    bv = _left_combine(v, np.transpose(_seq_to_g(seq, b, D), (1, 0, 2)), P)  # [2b][D]
+    N = 1 << (2 * D - 1).bit_length()
+    acc = np.zeros(N, np.uint64)
+    for j in range(2 * b):
+        fa = np.zeros(N, np.uint32)
+        fb = np.zeros(N, np.uint32)
+        fa[:D] = a[j]
+        fb[:D] = bv[j]
+        acc = (acc + O.ntt(fa).astype(np.uint64) * O.ntt(fb).astype(np.uint64)) % np.uint64(P)
+    c = O.ntt(acc.astype(np.uint32), invert=True)
+    assert not c[:d].any(), np.nonzero(c[:d])[0][:5]
+    # det M(z) = +-z^(-sum delta)
+    z = int(rng.integers(2, P))
+    Mz = np.zeros((2 * b, 2 * b), np.uint64)
+    for k in range(D - 1, -1, -1):
+        Mz = (Mz * np.uint64(z) + M[:, :, k]) % np.uint64(P)
+    from test_oracle import _det_mod
+    det = _det_mod(Mz, P)
+    e = int(-delta.sum())
+    assert det in (pow(z, e, P), (-pow(z, e, P)) % P)
+    nzmax = np.where(M.any(axis=1), np.arange(D)[None, :], -1).max(axis=1)
+    assert np.all(nzmax <= -delta)
+
+
 def test_rank_plain_small_end_to_end():
     """2000 x 1800 plain workload: device Krylov (b = 8, L = 458) -> device PM-basis
     (d = L - 1) -> rank; equals dense Gaussian elimination."""
