@@ -141,6 +141,21 @@ rr_status rr_rank_from_basis(const uint32_t *basis, const int64_t *delta, uint32
 rr_status rr_ntt(uint32_t *data, uint32_t logn, uint32_t count, int inverse);
 
 /*
+ * No counterpart in the reference: rr_ntt for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  The same transforms (p = 2013265921, natural order,
+ * inverse scaled by 1/n) in place on `data`, a DEVICE buffer of
+ * count x 2^logn words with entries < p (not checked), queued on `stream` (a
+ * hipStream_t; NULL: the null stream) and asynchronous.  All work memory is
+ * one fixed 24 MiB buffer per device (twiddle tables and 2^22 words of work
+ * space), so count x 2^logn <= 2^22 (RR_EINVAL beyond) and the call
+ * allocates nothing.  The first call on a device sets that buffer up and waits
+ * for it (RR_ESTATE if that call is made under stream capture).  Calls that
+ * overlap on one device race on the buffer.
+ */
+rr_status rr_ntt_noalloc(uint32_t *data, uint32_t logn, uint32_t count, int inverse, void *stream);
+
+/*
  * Replaces poly_mat_mul_red_adaptive / poly_mat_mul_fft_red
  * (src/poly_mat_mul.rs:89-101, :240-256): C [rows][cols][hi-lo] = coefficients
  * [lo, hi) of A [rows][inner][la] * B [inner][cols][lb], exact mod p

@@ -18,6 +18,7 @@
 #include "common.hpp"
 #include "ntt.hpp"
 #include <algorithm>
+#include <mutex>
 
 namespace rr {
 
@@ -360,6 +361,10 @@ rr_status NttTables::init(uint32_t logn_max, hipStream_t stream) {
     RR_HIP(hipMemcpyAsync(inv.ptr, iv.data(), half * 4, hipMemcpyHostToDevice, stream));
     RR_HIP(hipMemcpyAsync(invq.ptr, ivq.data(), half * 4, hipMemcpyHostToDevice, stream));
     RR_HIP(hipStreamSynchronize(stream));
+    tf = fwd.as<uint32_t>();
+    tfq = fwdq.as<uint32_t>();
+    ti = inv.as<uint32_t>();
+    tiq = invq.as<uint32_t>();
     return RR_OK;
 }
 
@@ -383,7 +388,7 @@ static rr_status ntt_forward_big(const NttTables &T, const NttFwdPack &pk, uint3
     const ModP mp = ModP::make(NTT_PRIME);
     const size_t lds = (size_t)(N2 + 1) * TC * 4;
     hipLaunchKernelGGL(k_ntt_cols_fwd, dim3(qmax * (N1 / TC), ndesc), dim3(NTT_THREADS), lds, st, pk, logn,
-                       T.fwd.as<uint32_t>(), T.fwdq.as<uint32_t>(), T.logn, NTT_PRIME, mp.inv);
+                       T.tf, T.tfq, T.logn, NTT_PRIME, mp.inv);
     RR_HIP(hipGetLastError());
     return ntt_forward_pack(T, rows, ndesc, l1, st);
 }
@@ -404,7 +409,7 @@ static rr_status ntt_inverse_big(const NttTables &T, const NttInvPack &pk, uint3
     const ModP mp = ModP::make(NTT_PRIME);
     const size_t lds = (size_t)(N2 + 1) * TC * 4;
     hipLaunchKernelGGL(k_ntt_cols_inv, dim3(qmax * (N1 / TC), ndesc), dim3(NTT_THREADS), lds, st, pk, logn,
-                       T.inv.as<uint32_t>(), T.invq.as<uint32_t>(), T.logn, NTT_PRIME, mp.inv);
+                       T.ti, T.tiq, T.logn, NTT_PRIME, mp.inv);
     RR_HIP(hipGetLastError());
     return RR_OK;
 }
@@ -417,7 +422,7 @@ rr_status ntt_forward_pack(const NttTables &T, const NttFwdPack &pk, uint32_t nd
     uint32_t bx = 0;
     for (uint32_t i = 0; i < ndesc; i++) bx = std::max(bx, blocks_for(pk.d[i].count, logn));
     hipLaunchKernelGGL(k_ntt_fwd, dim3(std::max(bx, 1u), ndesc), dim3(NTT_THREADS), lds_for(logn), st, pk, logn,
-                       T.fwd.as<uint32_t>(), T.fwdq.as<uint32_t>(), T.logn, NTT_PRIME);
+                       T.tf, T.tfq, T.logn, NTT_PRIME);
     RR_HIP(hipGetLastError());
     return RR_OK;
 }
@@ -430,7 +435,7 @@ rr_status ntt_inverse_pack(const NttTables &T, const NttInvPack &pk, uint32_t nd
     uint32_t bx = 0;
     for (uint32_t i = 0; i < ndesc; i++) bx = std::max(bx, blocks_for(pk.d[i].count, logn));
     hipLaunchKernelGGL(k_ntt_inv, dim3(std::max(bx, 1u), ndesc), dim3(NTT_THREADS), lds_for(logn), st, pk, logn,
-                       T.inv.as<uint32_t>(), T.invq.as<uint32_t>(), T.logn, NTT_PRIME);
+                       T.ti, T.tiq, T.logn, NTT_PRIME);
     RR_HIP(hipGetLastError());
     return RR_OK;
 }
@@ -478,3 +483,118 @@ rr_status pointwise(const uint32_t *A, const uint32_t *B, uint32_t *C, uint32_t 
 }
 
 }  // namespace rr
+
+// ---------------------------------------------------------------------------
+// rr_ntt_noalloc: rr_ntt's transforms on a caller-owned device buffer, with
+// every work array in one fixed buffer per device, so a call allocates
+// nothing (for callers under HIP graph capture or in loops that must not
+// call hipMalloc).
+
+namespace rr {
+
+constexpr uint32_t NA_TW_WORDS = 4u << (NTT_MAX_LOGN - 1);  // the four twiddle tables of 2^20-point transforms
+constexpr uint32_t NA_MAX_WORDS = 1u << 22;                // largest count x 2^logn (16 MiB)
+// This device's work buffer: twiddle tables (8 MiB), then the reordered
+// input or the spectrum (16 MiB).  Host code uses it only through the
+// address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) uint32_t scratch[NA_TW_WORDS + NA_MAX_WORDS];
+
+namespace {
+struct NoAllocDev {
+    uint32_t *ptr = nullptr;  // this device's copy of scratch
+    bool ready = false;       // twiddle tables written
+};
+constexpr int NA_MAX_DEV = 64;
+NoAllocDev g_na[NA_MAX_DEV];
+std::mutex g_na_mu;
+}  // namespace
+
+// tf[k] = w^k, tfq[k] = floor(tf[k] 2^32 / p), and the same for w^-1, k < half
+// (the tables NttTables::init builds on the host).
+__global__ void k_ntt_twiddles(uint32_t *tf, uint32_t *tfq, uint32_t *ti, uint32_t *tiq, uint32_t half, uint32_t w,
+                               uint32_t wi, uint32_t p, double inv) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= half) return;
+    uint32_t a = 1, b = 1, x = w, y = wi;
+    for (uint32_t e = k; e; e >>= 1) {
+        if (e & 1) {
+            a = mulmod(a, x, p, inv);
+            b = mulmod(b, y, p, inv);
+        }
+        x = mulmod(x, x, p, inv);
+        y = mulmod(y, y, p, inv);
+    }
+    tf[k] = a;
+    tfq[k] = (uint32_t)(((uint64_t)a << 32) / p);
+    ti[k] = b;
+    tiq[k] = (uint32_t)(((uint64_t)b << 32) / p);
+}
+
+// dst[q][rev(i)] = src[q][i] * scale mod p, i < 2^logn, q < total / 2^logn.
+__global__ void k_bitrev_scale(const uint32_t *src, uint32_t *dst, uint32_t logn, uint32_t total, uint32_t scale,
+                               uint32_t p, double inv) {
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const uint32_t mask = (1u << logn) - 1, i = idx & mask;
+    const uint32_t r = __brev(i) >> (32 - logn);
+    const uint32_t v = src[idx];
+    dst[(idx & ~mask) | r] = scale == 1 ? v : mulmod(v, scale, p, inv);
+}
+
+}  // namespace rr
+
+extern "C" rr_status rr_ntt_noalloc(uint32_t *data, uint32_t logn, uint32_t count, int inverse, void *stream) {
+    using namespace rr;
+    clear_error();
+    RR_CHECK(data && logn >= 1 && logn <= NTT_MAX_LOGN && count >= 1, RR_EINVAL, "bad NTT arguments");
+    RR_CHECK(((uint64_t)count << logn) <= NA_MAX_WORDS, RR_EINVAL,
+             "count x 2^logn = %llu words exceeds the %u-word work buffer",
+             (unsigned long long)((uint64_t)count << logn), NA_MAX_WORDS);
+    hipStream_t st = (hipStream_t)stream;
+    int dev = 0;
+    RR_HIP(hipGetDevice(&dev));
+    RR_CHECK(dev >= 0 && dev < NA_MAX_DEV, RR_EINVAL, "device %d out of range", dev);
+    const uint32_t p = NTT_PRIME, half = 1u << (NTT_MAX_LOGN - 1);
+    const ModP mp = ModP::make(p);
+    NoAllocDev &D = g_na[dev];
+    {
+        std::lock_guard<std::mutex> g(g_na_mu);
+        if (!D.ready) {  // set-up, once per device: the buffer's address and the twiddle tables
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            RR_HIP(hipStreamIsCapturing(st, &cs));
+            RR_CHECK(cs == hipStreamCaptureStatusNone, RR_ESTATE,
+                     "the first rr_ntt_noalloc call on a device sets up its tables and must not be captured");
+            if (!D.ptr) {
+                void *addr = nullptr;
+                RR_HIP(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)));
+                D.ptr = (uint32_t *)addr;
+            }
+            const uint32_t w = h_powmod(NTT_ROOT, (p - 1) >> NTT_MAX_LOGN, p), wi = h_invmod(w, p);
+            hipLaunchKernelGGL(k_ntt_twiddles, dim3((half + 255) / 256), dim3(256), 0, st, D.ptr, D.ptr + half,
+                               D.ptr + 2 * half, D.ptr + 3 * half, half, w, wi, p, mp.inv);
+            RR_HIP(hipGetLastError());
+            RR_HIP(hipStreamSynchronize(st));
+            D.ready = true;
+        }
+    }
+    NttTables T;  // a view of the tables in scratch (its own buffers stay empty)
+    T.logn = NTT_MAX_LOGN;
+    T.tf = D.ptr;
+    T.tfq = D.ptr + half;
+    T.ti = D.ptr + 2 * half;
+    T.tiq = D.ptr + 3 * half;
+    uint32_t *work = D.ptr + NA_TW_WORDS;
+    const uint32_t N = 1u << logn, total = count << logn;
+    const dim3 pg((total + 255) / 256), pt(256);
+    rr_status s;
+    if (!inverse) {  // spectrum (bit-reversed) into work, then into data in natural order
+        if ((s = ntt_forward(T, data, N, 0, N, work, count, logn, st)) != RR_OK) return s;
+        hipLaunchKernelGGL(k_bitrev_scale, pg, pt, 0, st, work, data, logn, total, 1u, p, mp.inv);
+    } else {  // bit-reversed and scaled by 1/N into work, then the inverse transform into data
+        hipLaunchKernelGGL(k_bitrev_scale, pg, pt, 0, st, data, work, logn, total, h_invmod(N, p), p, mp.inv);
+        RR_HIP(hipGetLastError());
+        if ((s = ntt_inverse(T, work, count, logn, 0, N, data, N, st)) != RR_OK) return s;
+    }
+    RR_HIP(hipGetLastError());
+    return RR_OK;
+}

@@ -14,7 +14,9 @@ constexpr uint32_t NTT_MAX_LOGN = 20;        // larger transforms: four steps th
 // with their Shoup companions.
 struct NttTables {
     uint32_t logn = 0;
-    DevBuf fwd, fwdq, inv, invq;
+    DevBuf fwd, fwdq, inv, invq;  // owned tables (init)
+    // the tables the transforms read: the owned ones, or tables laid out by the caller
+    const uint32_t *tf = nullptr, *tfq = nullptr, *ti = nullptr, *tiq = nullptr;
     rr_status init(uint32_t logn_max, hipStream_t stream);
 };
 
