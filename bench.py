@@ -130,28 +130,38 @@ def roofline(r, traffic, kernel, copy_gbs):
     return out
 
 
-def graph_complex_leg(engine, synth, p, warmup):
+def graph_complex_leg(engine, synth, p, steps=0, with_pm=True):
     """The 200000 x 180000 graph-complex workload at b = 32 over its whole
-    sequence (11258 entries), its device PM-basis of order L - 1 and rank,
-    and the same rank through one rr_rank call."""
+    sequence (11258 entries; `steps` timed entries instead if given), its
+    device PM-basis of order L - 1 and rank, and the same rank through one
+    rr_rank call."""
     b = 32
     A = synth.graph_complex()
     dr, dc, V = synth.preconditioners(2, A.m, A.n, b, p)
     L = synth.seq_len_plain(min(A.m, A.n), b)
+    steps = steps or L
     log(f"graph_complex {A.m}x{A.n} nnz={A.nnz} b={b} L={L}")
     K = engine.Krylov(A, p, b, dr, dc, V)
-    seq = K.run(L)  # the sequence for the PM-basis; also sizes the device buffer
+    seq = K.run(L if with_pm else min(L, 64))  # the sequence for the PM-basis; also sizes the device buffer
     t1 = time.perf_counter()
-    K.run(L)  # timed: a whole sequence's worth of iterations in one call
+    K.run(steps)  # timed: a whole sequence's worth of iterations in one call
     wall = time.perf_counter() - t1
     K.set_profiling(True)
-    K.run(2000)
+    K.run(min(steps, 2000))
     st = K.stats()
     K.close()
     passes = st["passes_a"] + st["passes_at"]
     r = {"spmm_ms": (st["ms_a"] + st["ms_at"]) / passes, "proj_ms": 0.0,
          "a_us": st["ms_a"] / st["passes_a"] * 1e3, "at_us": st["ms_at"] / st["passes_at"] * 1e3,
          "bytes_pass": (st["bytes_a"] * st["passes_a"] + st["bytes_at"] * st["passes_at"]) / passes}
+    out = {"workload": f"graph_complex {A.m}x{A.n} 12 nnz/row Zipf(1.1) +-1/+-2 p=2013265921 b={b}",
+           "seq_len": L, "steps": steps, "iters_s": round(steps / wall, 1),
+           "ms_per_step": round(wall * 1e3 / steps, 5),
+           "roofline": roofline(r, traffic_from_profile("traffic_k_spmm_graph_complex.json"),
+                                "k_spmm<32, packed>", None)}
+    log(f"  {out['iters_s']} iters/s, pass by A {r['a_us']:.1f} us, by A^T {r['at_us']:.1f} us")
+    if not with_pm:
+        return out
     t3 = time.perf_counter()
     basis, delta = engine.pm_basis(seq, b, L - 1, p)
     t4 = time.perf_counter()
@@ -159,12 +169,8 @@ def graph_complex_leg(engine, synth, p, warmup):
     del basis
     rr_rank, rr_ms = engine.rank(A, p, b, dr, dc, V, L, L - 1)
     assert int(rr_rank) == int(rank), (rr_rank, rank)
-    out = {"workload": f"graph_complex {A.m}x{A.n} 12 nnz/row Zipf(1.1) +-1/+-2 p=2013265921 b={b}",
-           "seq_len": L, "iters_s": round(L / wall, 1), "ms_per_step": round(wall * 1e3 / L, 5),
-           "roofline": roofline(r, traffic_from_profile("traffic_k_spmm_graph_complex.json"),
-                                "k_spmm<32, packed>", None),
-           "pm_basis_ms": round((t4 - t3) * 1e3, 1), "rank": int(rank), "rr_rank_s": round(rr_ms / 1e3, 3)}
-    log(f"  {out['iters_s']} iters/s; PM-basis {out['pm_basis_ms']} ms; rank {rank}; rr_rank {out['rr_rank_s']} s")
+    out.update({"pm_basis_ms": round((t4 - t3) * 1e3, 1), "rank": int(rank), "rr_rank_s": round(rr_ms / 1e3, 3)})
+    log(f"  PM-basis {out['pm_basis_ms']} ms; rank {rank}; rr_rank {out['rr_rank_s']} s")
     return out
 
 
@@ -268,6 +274,7 @@ def full_rank(engine, synth, p):
     dr, dc, V = synth.preconditioners(2, A.m, A.n, b, p)
     L = synth.seq_len_plain(min(A.m, A.n), b)
     log(f"large_single {A.m}x{A.n} nnz={A.nnz} b={b} L={L}, generated in {time.perf_counter() - t0:.1f}s")
+    os.environ.setdefault("RR_PROGRESS", "20")  # the library's own progress lines on stderr
     rank, ms = engine.rank(A, p, b, dr, dc, V, L, L - 1)
     expect = A.n - 5000
     out = {"workload": f"large_single {A.m}x{A.n} 20 nnz/row uniform, values uniform in [1,p), 5000 planted "
@@ -286,6 +293,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-timed", type=int, default=6, help="timed oracle iterations for the CPU baseline")
     ap.add_argument("--no-graph-complex", action="store_true", help="skip the 200000 x 180000 workload")
+    ap.add_argument("--no-pm", action="store_true", help="graph_complex: skip its PM-basis and rank")
     ap.add_argument("--batched", type=int, default=512, help="matrices in the batched/served comparison (0: skip)")
     ap.add_argument("--rate-frac", type=float, default=0.7, help="served offered load / batched capacity")
     ap.add_argument("--no-plain", action="store_true",
@@ -310,7 +318,7 @@ def main():
         full_rank(engine, synth, p)
         return
     if args.workload == "graph_complex":
-        print(json.dumps(graph_complex_leg(engine, synth, p, args.warmup)), flush=True)
+        print(json.dumps(graph_complex_leg(engine, synth, p, args.steps, not args.no_pm)), flush=True)
         return
 
     b = 64
@@ -330,9 +338,10 @@ def main():
                                                 "5000000x4500000 b=64 workload")
     nnz = A.nnz
     del A
-    gc = None if args.no_graph_complex else graph_complex_leg(engine, synth, p, args.warmup)
+    gc = None if args.no_graph_complex else graph_complex_leg(engine, synth, p)
     batched = served = None
     if args.batched:
+        engine.release_device_memory()  # the large legs' cached blocks go back before the many small matrices
         batched, served = batched_and_served(engine, synth, p, args.batched, args.rate_frac, plain=not args.no_plain)
     out = {
         "metric": "Krylov iters/s + HBM GB/s; batched ranks/s & p50/p99 vs plain, same GPU",

@@ -179,7 +179,9 @@ uint64_t rr_pm_order(uint64_t seq_len, int reference_policy);
  * order (0: seq_len - 1), rank estimate returned in rank_out.  ms_out (may be
  * NULL) receives the wall time of the call.  For orders of at least 1024 on
  * matrices of at least 2^21 nonzeros the PM-basis starts on the leading
- * coefficients while the sequence is produced.
+ * coefficients while the sequence is produced; then, with the environment
+ * variable RR_PROGRESS=<seconds> set, the call prints a progress line (entries
+ * produced, PM-basis rounds queued) to stderr at most that often.
  */
 rr_status rr_rank(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *row_precond, const uint32_t *col_precond,
                   const uint32_t *V, uint64_t seq_len, uint64_t order, int64_t *rank_out, double *ms_out);

@@ -597,17 +597,32 @@ rr_status rank_streamed(rr_krylov *k, uint32_t b, uint64_t L, uint64_t d, uint32
     std::vector<PmJob *> jobs{&J};
     EventRing ring;
     if ((s = ring.init(STREAM_ROUNDS_AHEAD)) != RR_OK) return s;
+    uint64_t nrounds = 0, t = 0;
+    // RR_PROGRESS=<seconds>: a line on stderr at most that often (long calls)
+    const char *pe = getenv("RR_PROGRESS");
+    const double every = pe ? atof(pe) : 0.0;
+    auto t_last = std::chrono::steady_clock::now();
+    auto progress = [&](bool force) {
+        if (every <= 0) return;
+        const auto now = std::chrono::steady_clock::now();
+        if (!force && std::chrono::duration<double>(now - t_last).count() < every) return;
+        t_last = now;
+        fprintf(stderr, "rr_rank: %llu/%llu sequence entries, %llu PM-basis rounds%s\n", (unsigned long long)t,
+                (unsigned long long)L, (unsigned long long)nrounds, J.done ? ", basis done" : "");
+        fflush(stderr);
+    };
     auto rounds = [&]() -> rr_status {  // the rounds the coefficients allow, within the ring
         while (!J.done && ring.room()) {
             bool launched = false;
             rr_status e = S.round(jobs, &launched);
             if (e != RR_OK) return e;
             if (!launched) break;
+            nrounds++;
             if ((e = ring.push(sp)) != RR_OK) return e;
         }
         return RR_OK;
     };
-    for (uint64_t t = 0; t < L;) {
+    for (t = 0; t < L;) {
         const uint64_t n = std::min<uint64_t>(STREAM_CHUNK, L - t);
         if ((s = rr_krylov_run(k, n, seq_out + t * P)) != RR_OK) return s;
         const uint64_t k1 = std::min<uint64_t>(t + n, d + 1);
@@ -619,6 +634,7 @@ rr_status rank_streamed(rr_krylov *k, uint32_t b, uint64_t L, uint64_t d, uint32
         }
         t += n;
         if ((s = rounds()) != RR_OK) return s;
+        progress(false);
     }
     while (!J.done) {
         if ((s = rounds()) != RR_OK) return s;
@@ -627,7 +643,9 @@ rr_status rank_streamed(rr_krylov *k, uint32_t b, uint64_t L, uint64_t d, uint32
             set_error("PM-basis waits for coefficients past the sequence");
             return RR_EINVAL;
         }
+        progress(false);
     }
+    progress(true);
     return rank_from_device_basis(M.as<uint32_t>(), dl.as<int64_t>(), b, d + 1, p, sc.as<uint32_t>(), sp, rank);
 }
 
