@@ -174,94 +174,128 @@ def graph_complex_leg(engine, synth, p, steps=0, with_pm=True):
     return out
 
 
-def batched_and_served(eng, synth, p, nmat, rate_frac, seed=4, plain=True):
+def batched(eng, synth, p, nmat, seed=4, plain=True):
     """Batched form: nmat matrices of the batched workload (rows log-uniform in
     [1000, 50000], cols 0.9 rows, 6-16 nnz/row, +-1/+-2, 1-5 % dependent
     columns), b per matrix from the autotuner, solved in one rr_batch_rank, vs
-    the same matrices solved one at a time by the plain GPU path (rr_rank).
-    Served form: the same matrices as a Poisson stream at rate_frac x the
-    measured batched capacity through rr_server; FIFO plain latencies are
-    computed from the measured plain service times on the same arrival times."""
-    import threading
+    the same matrices solved one at a time by the plain GPU path (rr_rank)."""
     t0 = time.perf_counter()
     mats = synth.batch_matrices(seed, nmat)
     bs, pred, probe = eng.autotune_b(mats, p)
     pre = [synth.preconditioners(5000 + i, A.m, A.n, int(b), p) for i, (A, b) in enumerate(zip(mats, bs))]
     log(f"batched: {nmat} matrices, rows {min(A.m for A in mats)}..{max(A.m for A in mats)}, "
         f"b histogram {dict(zip(*np.unique(bs, return_counts=True)))}, setup {time.perf_counter() - t0:.1f}s")
-    if not plain:  # the batched call alone (for kernel traces)
-        t2 = time.perf_counter()
-        ranks, st = eng.batch_rank(mats, p, bs, [x[0] for x in pre], [x[1] for x in pre], [x[2] for x in pre])
-        t_batch = time.perf_counter() - t2
-        log(f"batched {t_batch:.2f}s (plain solves skipped)")
-        return {"matrices": nmat, "ranks_per_s": round(nmat / t_batch, 3), "wall_s": round(t_batch, 3),
-                "ms_krylov": round(st["ms_krylov"], 1), "ms_pm": round(st["ms_pm"], 1)}, None
-    # batched: one call (first, while the device is as fresh as for the Krylov leg)
     t2 = time.perf_counter()
     ranks, st = eng.batch_rank(mats, p, bs, [x[0] for x in pre], [x[1] for x in pre], [x[2] for x in pre])
     t_batch = time.perf_counter() - t2
-    # plain: one matrix at a time
-    plain_ms, plain_rank = [], []
+    out = {"workload": "batched: rows log-uniform in [1000, 50000], cols 0.9 rows, 6-16 nnz/row, +-1/+-2, "
+                       "1-5 % planted dependent columns", "matrices": nmat,
+           "ranks_per_s": round(nmat / t_batch, 3), "wall_s": round(t_batch, 3), "ticks": st["ticks"],
+           "pm_rounds": st["pm_rounds"], "ms_krylov": round(st["ms_krylov"], 1), "ms_pm": round(st["ms_pm"], 1),
+           "b_hist": {int(k): int(v) for k, v in zip(*np.unique(bs, return_counts=True))}, "autotune_probe": probe}
+    if not plain:  # the batched call alone (for kernel traces)
+        log(f"batched {t_batch:.2f}s (plain solves skipped)")
+        return out
+    plain_rank = []
     t1 = time.perf_counter()
     for (A, b, (dr, dc, V)) in zip(mats, bs, pre):
-        r, ms = eng.rank(A, p, int(b), dr, dc, V)
-        plain_rank.append(r)
-        plain_ms.append(ms)
+        plain_rank.append(eng.rank(A, p, int(b), dr, dc, V)[0])
     t_plain = time.perf_counter() - t1
     assert list(ranks) == plain_rank, "batched ranks differ from plain ranks"
-    batched = {"matrices": nmat, "ranks_per_s": round(nmat / t_batch, 3), "plain_ranks_per_s": round(nmat / t_plain, 3),
-               "speedup_vs_plain": round(t_plain / t_batch, 2), "wall_s": round(t_batch, 3),
-               "plain_wall_s": round(t_plain, 3), "ticks": st["ticks"], "pm_rounds": st["pm_rounds"],
-               "ms_krylov": round(st["ms_krylov"], 1), "ms_pm": round(st["ms_pm"], 1),
-               "b_hist": {int(k): int(v) for k, v in zip(*np.unique(bs, return_counts=True))},
-               "autotune_probe": probe, "ranks_equal_plain": True}
+    out.update({"plain_ranks_per_s": round(nmat / t_plain, 3), "speedup_vs_plain": round(t_plain / t_batch, 2),
+                "plain_wall_s": round(t_plain, 3), "ranks_equal_plain": True})
     log(f"batched {t_batch:.2f}s vs plain {t_plain:.2f}s -> {t_plain / t_batch:.1f}x")
-    # served: Poisson arrivals at rate_frac x batched capacity
-    lam = rate_frac * nmat / t_batch
-    rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / lam, nmat))
+    return out
+
+
+def _serve(eng, p, reqs, arrivals):
+    """Submits reqs (A, b, dr, dc, V) to a fresh rr_server at the given arrival
+    times (s from the start); returns (ranks, latencies ms, wall s, stats)."""
+    import threading
+    n = len(reqs)
     results = {}
     srv = eng.Server(p, 200 << 30)
-    stop = threading.Event()
+    done = threading.Event()
 
     def poller():
-        while not stop.is_set() or len(results) < nmat:
+        while len(results) < n:
             for r in srv.poll(wait_ms=50):
                 results[r["id"]] = r
-            if len(results) >= nmat:
-                break
+        done.set()
 
     th = threading.Thread(target=poller)
     th.start()
-    t3 = time.perf_counter()
+    t0 = time.perf_counter()
     ids = []
-    for i, (A, b, (dr, dc, V)) in enumerate(zip(mats, bs, pre)):
-        delay = t3 + arrivals[i] - time.perf_counter()
+    for (A, b, dr, dc, V), a in zip(reqs, arrivals):
+        delay = t0 + a - time.perf_counter()
         if delay > 0:
             time.sleep(delay)
         ids.append(srv.submit(A, int(b), dr, dc, V))
-    stop.set()
-    th.join(timeout=600)
-    t_served = time.perf_counter() - t3
-    sst = srv.stats()
+    done.wait(timeout=900)
+    wall = time.perf_counter() - t0
+    th.join(timeout=5)
+    st = srv.stats()
     srv.close()
+    ranks = [results[i]["rank"] for i in ids]
     lat = np.array([results[i]["latency_ms"] for i in ids])
-    served_ok = all(results[i]["rank"] == plain_rank[k] for k, i in enumerate(ids))
-    # FIFO plain on the same arrivals: one solve at a time in arrival order
-    fin, fifo = 0.0, []
-    for a, ms in zip(arrivals * 1e3, plain_ms):
-        fin = max(fin, a) + ms
-        fifo.append(fin - a)
+    assert all(results[i]["status"] == 0 for i in ids), "a served request failed"
+    return ranks, lat, wall, st
+
+
+def served(eng, synth, p, nreq, rate_frac, rows_hi, seed=5):
+    """Served form: nreq requests of the served workload (rows log-uniform in
+    [10^3, rows_hi], 8-20 nnz/row, +-1/+-2, planted deficits), b per request
+    from the autotuner, through rr_server with a 200 GB HBM budget.  Two
+    points: all requests at once (saturated: the server's capacity), and
+    Poisson arrivals at rate_frac x that capacity, the latter replayed as FIFO
+    plain solves (rr_rank one at a time in arrival order, run for real)."""
+    t0 = time.perf_counter()
+    mats = synth.served_matrices(seed, nreq, rows_hi)
+    bs, _, _ = eng.autotune_b(mats, p)
+    reqs = [(A, int(b)) + tuple(synth.preconditioners(9000 + i, A.m, A.n, int(b), p))
+            for i, (A, b) in enumerate(zip(mats, bs))]
+    log(f"served: {nreq} requests, rows {min(A.m for A in mats)}..{max(A.m for A in mats)}, "
+        f"b histogram {dict(zip(*np.unique(bs, return_counts=True)))}, setup {time.perf_counter() - t0:.1f}s")
+    # saturated: every request at t = 0
+    r_sat, lat_sat, wall_sat, st_sat = _serve(eng, p, reqs, np.zeros(nreq))
+    cap = nreq / wall_sat
+    # offered load rate_frac x capacity, Poisson
+    lam = rate_frac * cap
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, nreq))
+    r_off, lat_off, wall_off, st_off = _serve(eng, p, reqs, arrivals)
+    # FIFO plain on the same arrivals, run: one solve at a time in arrival order
+    t1 = time.perf_counter()
+    fifo, plain_rank, service = [], [], 0.0
+    for (A, b, dr, dc, V), a in zip(reqs, arrivals):
+        delay = t1 + a - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        s0 = time.perf_counter()
+        plain_rank.append(eng.rank(A, p, b, dr, dc, V)[0])
+        fin = time.perf_counter()
+        service += fin - s0
+        fifo.append((fin - t1 - a) * 1e3)
     fifo = np.array(fifo)
-    served = {"requests": nmat, "offered_rate_per_s": round(lam, 3), "ranks_per_s": round(nmat / t_served, 3),
-              "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
-              "fifo_plain_p50_ms": round(float(np.percentile(fifo, 50)), 2),
-              "fifo_plain_p99_ms": round(float(np.percentile(fifo, 99)), 2),
-              "fifo_plain_ranks_per_s": round(nmat / (fin / 1e3), 3),
-              "max_active": sst["max_active"], "hbm_budget_gb": 200, "ranks_equal_plain": bool(served_ok)}
-    log(f"served: p50 {served['p50_ms']} ms p99 {served['p99_ms']} ms vs FIFO plain p50 {served['fifo_plain_p50_ms']} "
-        f"p99 {served['fifo_plain_p99_ms']}")
-    return batched, served
+    wall_fifo = time.perf_counter() - t1
+    assert r_sat == plain_rank and r_off == plain_rank, "served ranks differ from plain ranks"
+    pct = lambda x, q: round(float(np.percentile(x, q)), 2)
+    out = {"workload": f"served (rehearsal): rows log-uniform in [1000, {rows_hi}], cols 0.9 rows, 8-20 nnz/row, "
+                       f"+-1/+-2, 1-5 % planted dependent columns; HBM budget 200 GB", "requests": nreq,
+           "saturated": {"ranks_per_s": round(cap, 3), "p50_ms": pct(lat_sat, 50), "p99_ms": pct(lat_sat, 99),
+                         "max_active": st_sat["max_active"]},
+           "offered": {"rate_frac": rate_frac, "offered_rate_per_s": round(lam, 3),
+                       "ranks_per_s": round(nreq / wall_off, 3), "p50_ms": pct(lat_off, 50),
+                       "p99_ms": pct(lat_off, 99), "max_active": st_off["max_active"]},
+           "fifo_plain": {"ranks_per_s": round(nreq / wall_fifo, 3), "p50_ms": pct(fifo, 50),
+                          "p99_ms": pct(fifo, 99), "service_ranks_per_s": round(nreq / service, 3)},
+           "saturated_vs_plain": round(cap * service / nreq, 2), "ranks_equal_plain": True}
+    log(f"served: saturated {cap:.1f} ranks/s (p50 {out['saturated']['p50_ms']} ms); at {rate_frac} x: p50 "
+        f"{out['offered']['p50_ms']} p99 {out['offered']['p99_ms']} ms; FIFO plain p50 {out['fifo_plain']['p50_ms']} "
+        f"p99 {out['fifo_plain']['p99_ms']} ms; plain service {nreq / service:.1f} ranks/s "
+        f"-> {out['saturated_vs_plain']}x")
+    return out
 
 
 def full_rank(engine, synth, p):
@@ -294,8 +328,11 @@ def main():
     ap.add_argument("--cpu-timed", type=int, default=6, help="timed oracle iterations for the CPU baseline")
     ap.add_argument("--no-graph-complex", action="store_true", help="skip the 200000 x 180000 workload")
     ap.add_argument("--no-pm", action="store_true", help="graph_complex: skip its PM-basis and rank")
-    ap.add_argument("--batched", type=int, default=512, help="matrices in the batched/served comparison (0: skip)")
-    ap.add_argument("--rate-frac", type=float, default=0.7, help="served offered load / batched capacity")
+    ap.add_argument("--batched", type=int, default=512, help="matrices in the batched comparison (0: skip)")
+    ap.add_argument("--served", type=int, default=256, help="requests in the served comparison (0: skip)")
+    ap.add_argument("--served-rows-hi", type=int, default=100000,
+                    help="largest rows of a served request (the full workload: 1000000)")
+    ap.add_argument("--rate-frac", type=float, default=0.7, help="served offered load / saturated capacity")
     ap.add_argument("--no-plain", action="store_true",
                     help="batched call only: skip the plain solves and the served form (for kernel traces)")
     ap.add_argument("--workload", choices=["large", "graph_complex"], default="large",
@@ -339,10 +376,12 @@ def main():
     nnz = A.nnz
     del A
     gc = None if args.no_graph_complex else graph_complex_leg(engine, synth, p)
-    batched = served = None
+    bat = srv = None
+    engine.release_device_memory()  # the large legs' cached blocks go back before the many small matrices
     if args.batched:
-        engine.release_device_memory()  # the large legs' cached blocks go back before the many small matrices
-        batched, served = batched_and_served(engine, synth, p, args.batched, args.rate_frac, plain=not args.no_plain)
+        bat = batched(engine, synth, p, args.batched, plain=not args.no_plain)
+    if args.served and not args.no_plain:
+        srv = served(engine, synth, p, args.served, args.rate_frac, args.served_rows_hi)
     out = {
         "metric": "Krylov iters/s + HBM GB/s; batched ranks/s & p50/p99 vs plain, same GPU",
         "value": round(r["iters_s"], 3), "unit": "iters/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
@@ -350,7 +389,7 @@ def main():
         "vs_baseline": None, "dtype": "u32", "data": "synthetic (seeded SplitMix64)",
         "config": {"workload": desc, "seq_len": L, "nnz": nnz, "b": b,
                    "hbm_gbs_algorithmic": round(b_iter * r["iters_s"] / 1e9, 1)},
-        "roofline": roof, "cpu_baseline": cpu, "graph_complex": gc, "batched": batched, "served": served,
+        "roofline": roof, "cpu_baseline": cpu, "graph_complex": gc, "batched": bat, "served": srv,
     }
     print(json.dumps(out), flush=True)
 

@@ -165,6 +165,56 @@ rr_status rr_poly_mat_mul(const uint32_t *A, uint64_t la, const uint32_t *B, uin
                           uint32_t inner, uint32_t cols, uint64_t lo, uint64_t hi, uint32_t p, uint32_t *C);
 
 /*
+ * Replaces CsrMatrix::load_csr_matrix_from_sms (src/matrices.rs:662-732):
+ * reads an SMS text file ("m n x" header, then "row col value" lines, 1-based,
+ * '%' comments; lines of another token count are skipped, as are entries
+ * with a zero index such as the closing "0 0 0").  Values are signed 32-bit
+ * and are stored as their residue in [0, p).  Entries must come row by row
+ * (an entry of an earlier row joins the row being read, as in the
+ * reference); an entry outside the m x n header is RR_EINVAL (the reference
+ * builds an inconsistent matrix there).  rr_sms_open parses the file and
+ * returns its sizes; rr_sms_copy fills row_ptr (m + 1), col and val (nnz);
+ * rr_sms_close frees the parsed copy.
+ */
+typedef struct rr_sms rr_sms;
+rr_status rr_sms_open(const char *path, uint32_t p, rr_sms **out, uint64_t *m, uint64_t *n, uint64_t *nnz);
+rr_status rr_sms_copy(const rr_sms *h, uint64_t *row_ptr, uint32_t *col, uint32_t *val);
+void rr_sms_close(rr_sms *h);
+
+/*
+ * Replaces load_wdm_file_sym (src/wdm_files.rs:20-128), in two steps:
+ * rr_wdm_info reads the header line "m n p Nlen num_v"; rr_wdm_load reads the
+ * rest into caller buffers of those sizes: row_precond (m), col_precond (n),
+ * V and curv (n x b row-interleaved; the file holds b lines of n values each)
+ * and seq ([len][P], t-major; the file holds P lines, one per pair i <= j in
+ * row-major order, each the pair's len values over t).  A vector line of the
+ * wrong length is RR_EINVAL with the reference's message; sequence lines are
+ * read to their first len values.
+ */
+rr_status rr_wdm_info(const char *path, uint64_t *m, uint64_t *n, uint32_t *p, uint64_t *len, uint32_t *b);
+rr_status rr_wdm_load(const char *path, uint32_t *row_precond, uint32_t *col_precond, uint32_t *V, uint32_t *curv,
+                      uint32_t *seq);
+
+/*
+ * Replaces save_wdm_file_sym (src/wdm_files.rs:132-210): writes the .wdm text
+ * file of the same layout (V and curv written as canonical residues, as
+ * prettify_vect does).  With curv and t from rr_krylov_state and the first t
+ * sequence entries this is the reference's checkpoint, and rr_krylov_create
+ * with t_have = t resumes from it.
+ */
+rr_status rr_wdm_save(const char *path, uint64_t m, uint64_t n, uint32_t p, uint32_t b, const uint32_t *row_precond,
+                      const uint32_t *col_precond, const uint32_t *V, const uint32_t *curv, const uint32_t *seq,
+                      uint64_t len);
+
+/*
+ * The number of entries the reference's sequence loop (src/main.rs:105-131)
+ * appends to a sequence holding `have` entries for the target max_nlen: a
+ * fresh one (have = 0) gets S_0 and then two entries per step,
+ * 1 + 2 floor(max_nlen / 2); a resumed one 2 floor((max_nlen - have) / 2).
+ */
+uint64_t rr_seq_resume_count(uint64_t have, uint64_t max_nlen);
+
+/*
  * Sequence length and PM-basis order.  reference_policy = 1 reproduces
  * src/main.rs:497-504 (next power of two of floor(2 min(m,n)/b + 4), plus the
  * t = 0 entry) and src/main.rs:599-603 (largest power of two <= length);

@@ -89,6 +89,8 @@ def ref():
                                    C.POINTER(C.c_int64), C.c_void_p, C.c_void_p, C.c_void_p]
         R.ref_transpose.argtypes = [C.c_int64, C.c_int64, _i64p, _i32p, _u32p, _i64p, _i32p, _u32p]
         R.ref_rescale.argtypes = [C.c_int64, C.c_int64, _i64p, _i32p, _u32p, C.c_void_p, C.c_void_p, C.c_uint32]
+        R.ref_save_wdm.argtypes = [C.c_char_p, C.c_int64, C.c_int64, C.c_uint32, C.c_uint32, _u32p, _u32p, _u32p,
+                                   _u32p, _u32p, C.c_int64]
         _ref = R
     return _ref
 
@@ -306,3 +308,12 @@ def ref_rescale(m, n, rp, col, val, dr, dc, p):
                       drc.ctypes.data if drc is not None else None,
                       dcc.ctypes.data if dcc is not None else None, p)
     return v
+
+
+def ref_save_wdm(path, m, n, p, dr, dc, V, curv, seq):
+    """The reference's save_wdm_file_sym (cuda_test/wdmfiles.h:92-160); V, curv n x b, seq [len][P]."""
+    V = _c(V, np.uint32)
+    b = V.shape[1]
+    seq = _c(seq, np.uint32).reshape(-1, b * (b + 1) // 2)
+    ref().ref_save_wdm(os.fsencode(path), m, n, p, b, _c(dr, np.uint32), _c(dc, np.uint32), V, _c(curv, np.uint32),
+                       seq, seq.shape[0])

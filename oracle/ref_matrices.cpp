@@ -1,5 +1,6 @@
 // ref_matrices.cpp -- C entry points over the reference's cuda_test/matrices.h
-// (SMS loader, CSR transpose, row/column rescaling), compiled from the
+// (SMS loader, CSR transpose, row/column rescaling) and cuda_test/wdmfiles.h
+// (the .wdm checkpoint writer), compiled from the
 // reference checkout by oracle/Makefile into oracle/_ref/libref.so.  This is a
 // separate translation unit because matrices.h and sigma_basis.h both define
 // display_vector (matrices.h:392, sigma_basis.h:31).
@@ -11,6 +12,7 @@
 #include <cstdint>
 #include <string>
 #include "matrices.h"
+#include "wdmfiles.h"
 
 extern "C" {
 
@@ -56,6 +58,25 @@ int ref_rescale(int64_t m, int64_t n, const int64_t *rp, const int32_t *col, uin
         A.csr_columnrescale(s, (int64_t)p);
     }
     for (size_t e = 0; e < A.values.size(); e++) val[e] = (uint32_t)A.values[e];
+    return 0;
+}
+
+// wdmfiles.h:92-160: V and curv as b column lists of n values, seq as P pair
+// lists of len values (pairs i <= j row-major).
+int ref_save_wdm(const char *path, int64_t m, int64_t n, uint32_t p, uint32_t b, const uint32_t *dr,
+                 const uint32_t *dc, const uint32_t *V, const uint32_t *curv, const uint32_t *seq, int64_t len) {
+    std::vector<uint32_t> r(dr, dr + m), c(dc, dc + n);
+    std::vector<std::vector<uint32_t>> v(b, std::vector<uint32_t>(n)), cv(b, std::vector<uint32_t>(n));
+    for (uint32_t i = 0; i < b; i++)
+        for (int64_t k = 0; k < n; k++) {
+            v[i][k] = V[k * b + i];
+            cv[i][k] = curv[k * b + i];
+        }
+    const uint32_t P = b * (b + 1) / 2;
+    std::vector<std::vector<uint32_t>> s(P, std::vector<uint32_t>(len));
+    for (uint32_t ii = 0; ii < P; ii++)
+        for (int64_t t = 0; t < len; t++) s[ii][t] = seq[t * P + ii];
+    save_wdm_file_sym<uint32_t>(std::string(path), (size_t)m, (size_t)n, p, r, c, v, cv, s);
     return 0;
 }
 

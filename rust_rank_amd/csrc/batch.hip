@@ -383,6 +383,42 @@ static void launch_gram_tiles(const GramTile *tiles, uint32_t n, uint32_t grid, 
 // k_reduce_jobs launch folds every request's partials of every tick (a
 // request's partials are always reduced alone).
 rr_status TickRunner::tick(std::vector<Request *> &reqs, uint32_t p, uint32_t nticks) {
+    launched = false;
+    if (reqs.empty() || nticks == 0) return RR_OK;
+    for (auto *R : reqs)
+        RR_CHECK(!fail_id || R->id != fail_id, RR_ENOMEM, "injected tick failure (request %llu)",
+                 (unsigned long long)R->id);
+    // Requests whose descriptors do not fit one staging slot together run in
+    // several staged launches (a request's results do not depend on which
+    // requests share its launches).
+    auto bytes_of = [&](const Request *R) {
+        return R->t == 0 ? R->tiles_gram.size() * sizeof(GramTile) + sizeof(RedJob) + 64
+                         : (size_t)nticks * (sizeof(SpmmArgs) + sizeof(uint32_t) + sizeof(RedJob) + 64);
+    };
+    const size_t cap = STAGE_BYTES - 7 * 16 - 1024;
+    size_t tot = 0;
+    for (auto *R : reqs) tot += bytes_of(R);
+    if (tot <= cap) return tick_staged(reqs, p, nticks);
+    std::vector<Request *> part_reqs;
+    size_t acc = 0;
+    for (size_t i = 0; i <= reqs.size(); i++) {
+        if (i == reqs.size() || (acc + bytes_of(reqs[i]) > cap && !part_reqs.empty())) {
+            const bool before = launched;
+            rr_status s = tick_staged(part_reqs, p, nticks);
+            launched |= before;
+            if (s != RR_OK) return s;
+            part_reqs.clear();
+            acc = 0;
+            if (i == reqs.size()) break;
+        }
+        RR_CHECK(bytes_of(reqs[i]) <= cap, RR_ENOMEM, "one request's tick descriptors exceed staging");
+        part_reqs.push_back(reqs[i]);
+        acc += bytes_of(reqs[i]);
+    }
+    return RR_OK;
+}
+
+rr_status TickRunner::tick_staged(std::vector<Request *> &reqs, uint32_t p, uint32_t nticks) {
     if (reqs.empty() || nticks == 0) return RR_OK;
     ModP mp = ModP::make(p);
     for (auto *R : reqs)
@@ -482,6 +518,7 @@ rr_status TickRunner::tick(std::vector<Request *> &reqs, uint32_t p, uint32_t nt
             std::memcpy(h + wg_off[c][gi], G.wg.data(), G.wg.size() * sizeof(uint32_t));
         }
     std::memcpy(h + red_off, red.data(), red.size() * sizeof(RedJob));
+    launched = true;  // from here on, work of this tick is on the stream
     RR_HIP(hipMemcpyAsync(desc_dev.ptr, h, bytes, hipMemcpyHostToDevice, st));
     RR_HIP(hipEventRecord(stage_ev[si], st));
     const char *dbase = desc_dev.as<char>();
@@ -575,12 +612,12 @@ rr_status rank_streamed(rr_krylov *k, uint32_t b, uint64_t L, uint64_t d, uint32
         ~Guard() {
             if (sp) {
                 (void)hipStreamSynchronize(sp);
-                (void)hipStreamDestroy(sp);
+                rr::stream_destroy(sp);
             }
             if (ev) (void)hipEventDestroy(ev);
         }
     } guard{sp, ev};
-    RR_HIP(hipStreamCreateWithFlags(&sp, hipStreamNonBlocking));
+    RR_HIP(rr::stream_create(&sp));
     RR_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     uint32_t logn = 2;
     while ((1ull << logn) < d + 1) logn++;
@@ -646,6 +683,7 @@ rr_status rank_streamed(rr_krylov *k, uint32_t b, uint64_t L, uint64_t d, uint32
         progress(false);
     }
     progress(true);
+    if (J.err != RR_OK) return J.err;
     return rank_from_device_basis(M.as<uint32_t>(), dl.as<int64_t>(), b, d + 1, p, sc.as<uint32_t>(), sp, rank);
 }
 
@@ -665,10 +703,10 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
     RR_HIP(hipGetDevice(&dev));
     RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     hipStream_t st;
-    RR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    RR_HIP(rr::stream_create(&st));
     struct SG {
         hipStream_t s;
-        ~SG() { (void)hipStreamDestroy(s); }
+        ~SG() { rr::stream_destroy(s); }
     } guard{st};
     std::vector<std::unique_ptr<Request>> reqs(count);
     rr_status s;
@@ -695,7 +733,7 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
             for (auto &t : pool) t.join();
             for (auto w : ws) {
                 (void)hipStreamSynchronize(w);
-                (void)hipStreamDestroy(w);
+                rr::stream_destroy(w);
             }
             for (auto e : ev)
                 if (e) (void)hipEventDestroy(e);
@@ -719,7 +757,7 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
         const uint32_t nth = std::max(1u, std::min<uint32_t>(count, std::min<uint32_t>(hw, 16) - 1));
         for (uint32_t w = 0; w < nth; w++) {
             hipStream_t ws;
-            RR_HIP(hipStreamCreateWithFlags(&ws, hipStreamNonBlocking));
+            RR_HIP(rr::stream_create(&ws));
             su.ws.push_back(ws);
         }
         for (uint32_t w = 0; w < nth; w++)
@@ -760,7 +798,7 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
     // Each stream is kept a few items ahead of the device (events), so
     // neither waits for the host.
     hipStream_t st_pm;
-    RR_HIP(hipStreamCreateWithFlags(&st_pm, hipStreamNonBlocking));
+    RR_HIP(rr::stream_create(&st_pm));
     SG guard_pm{st_pm};
     TickRunner T;
     if ((s = T.init(st, cus)) != RR_OK) return s;
@@ -816,8 +854,9 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
         uint64_t left = ~0ull;
         size_t words = 0;  // partial words of one tick
         for (uint32_t i = 0; i < count; i++) {
+            if (!joined[i]) continue;  // reqs[i] may still be written by its set-up thread
             Request *r = reqs[i].get();
-            if (!joined[i] || r->stage != STAGE_KRYLOV || r->t >= r->L) continue;
+            if (r->stage != STAGE_KRYLOV || r->t >= r->L) continue;
             if (r->t == 0) {
                 fresh.push_back(r);
                 continue;
@@ -868,6 +907,8 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
         if (!pm_idle && pq.room()) {
             bool launched = false;
             if ((s = S.round(jobs, &launched)) != RR_OK) return s;
+            for (auto *J : jobs)
+                if (J->err != RR_OK) return J->err;
             if (launched) {
                 if ((s = pq.push(st_pm)) != RR_OK) return s;
                 issued = true;
@@ -1055,9 +1096,9 @@ rr_status rr_autotune_b(uint32_t count, const rr_csr *mats, uint32_t p, uint32_t
         RR_HIP(hipGetDevice(&dev));
         RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         hipStream_t st;
-        RR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        RR_HIP(rr::stream_create(&st));
         rr_status s = run_probe(p, cus, st);
-        (void)hipStreamDestroy(st);
+        rr::stream_destroy(st);
         if (s != RR_OK) return s;
     }
     // b = 16 unless another width is predicted TUNE_MARGIN faster: the probes

@@ -52,6 +52,7 @@ struct Request {
     uint64_t m = 0, n = 0, nnz = 0, L = 0, d = 0, t = 0;
     bool packed = false;
     int stage = STAGE_KRYLOV;
+    rr_status fail = RR_OK;  // why the request failed (stage STAGE_FAILED)
     int64_t rank = -1;
     uint64_t id = 0, budget_bytes = 0;
     double t_arrive = 0, t_admit = 0, t_done = 0;
@@ -99,10 +100,21 @@ struct TickRunner {
     DevBuf desc_dev, part;
     size_t part_cap = 0;
     uint64_t ticks = 0;
+    // after tick(): whether any of its work reached the stream (a tick that
+    // fails with this false changed nothing, and may be retried)
+    bool launched = false;
+    // tests of failure handling: a tick holding this request id fails with
+    // RR_ENOMEM before any of its work is queued (0: never)
+    uint64_t fail_id = 0;
 
     rr_status init(hipStream_t st, int cus);
+    // nticks entries of every request; descriptors beyond one staging slot
+    // are split over several staged launches
     rr_status tick(std::vector<Request *> &reqs, uint32_t p, uint32_t nticks = 1);
     ~TickRunner();
+
+  private:
+    rr_status tick_staged(std::vector<Request *> &reqs, uint32_t p, uint32_t nticks);
 };
 
 }  // namespace rr

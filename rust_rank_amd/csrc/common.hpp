@@ -85,14 +85,21 @@ static inline bool h_is_prime(uint32_t p) {
     return true;
 }
 
+// Streams of the library.  Every stream the library creates goes through
+// these, so the device memory cache below knows every stream that can still
+// be using a freed block; stream_destroy waits for the stream first.
+hipError_t stream_create(hipStream_t *st);
+void stream_destroy(hipStream_t st);
+
 // Device memory through a per-device cache of freed blocks (size classes
 // within 12.5 % of the request): hipMalloc / hipFree cost tens of
 // microseconds each and hipFree waits for the device, which a batch of
 // hundreds of matrices with dozens of buffers each pays thousands of times.
-// A freed block may still be in use by kernels in flight, so the cache hands
-// it out again only after a device synchronisation (one for all the blocks
-// freed before it), the wait hipFree would have done.  When hipMalloc fails
-// the cached blocks are released and the allocation is retried.
+// A freed block may still be in use by kernels in flight.  The cache hands it
+// out again once marker events recorded on every library stream after the
+// free have completed; until then a request of its size gets a fresh block,
+// so no allocation waits for the device.  When hipMalloc fails the cache
+// waits for the device, releases its blocks and retries.
 void *dev_alloc(size_t bytes, size_t *cls, int *device, hipError_t *err);
 void dev_free(void *ptr, size_t cls, int device);
 void dev_cache_release(int device);  // waits for the device, frees its cached blocks

@@ -577,7 +577,7 @@ struct rr_krylov {
     bool profiling = false;
     rr_krylov_stats stats{};
     ~rr_krylov() {
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream) rr::stream_destroy(stream);
     }
 };
 
@@ -1052,7 +1052,7 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
     c->P = b * (b + 1) / 2;
     c->t = t_have;
     RR_HIP(hipGetDevice(&c->device));
-    RR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    RR_HIP(rr::stream_create(&c->stream));
     int cus = 0;
     RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
     c->grid = 0;  // set below from the occupancy of the chosen pass kernel

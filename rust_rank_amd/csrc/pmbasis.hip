@@ -813,6 +813,7 @@ rr_status PmScheduler::add(PmJob *J) {
     J->stack.push_back(Frame{J->G, J->d + 1, J->d, J->M, 0, nullptr, nullptr, nullptr, 0});
     J->done = false;
     J->has_op = false;
+    J->err = RR_OK;
     return RR_OK;
 }
 
@@ -825,6 +826,7 @@ rr_status PmScheduler::round(std::vector<PmJob *> &jobs, bool *launched) {
     std::vector<MulDesc> school;
     std::vector<NttProduct> ntt;
     *launched = false;
+    last.clear();
     // Products go first and leaves wait for a round in which no job has a
     // product pending, so that the leaves of many jobs share one launch (a
     // job reaches a leaf after at most its recursion depth of products).
@@ -836,7 +838,10 @@ rr_status PmScheduler::round(std::vector<PmJob *> &jobs, bool *launched) {
             Op op;
             rr_status e;
             if (!next_op(*J, op, e)) {
-                if (e != RR_OK) return e;
+                if (e != RR_OK) {  // this job's own failure: it ends, the others go on
+                    J->err = e;
+                    J->done = true;
+                }
                 continue;
             }
             J->has_op = true;
@@ -854,11 +859,26 @@ rr_status PmScheduler::round(std::vector<PmJob *> &jobs, bool *launched) {
         any_mul |= J->op_kind == 1;
     }
     auto ready = [](const PmJob *J) { return !J->done && J->has_op && J->op_need <= J->avail; };
+    for (auto *J : jobs)
+        RR_CHECK(!fail_tag || J->tag != fail_tag || !ready(J), RR_ENOMEM, "injected PM-basis round failure (job %llu)",
+                 (unsigned long long)J->tag);
     size_t nmul = 0;
     if (any_mul)
         for (auto *J : jobs) nmul += ready(J) && J->op_kind == 1;
+    if (any_mul) {  // the round's transform work space, before any job's operation is taken
+        size_t words = 0;
+        for (auto *J : jobs)
+            if (ready(J) && J->op_kind == 1 && !use_school(J->op_mul, p, nmul)) {
+                const MulDesc &D = J->op_mul;
+                words += (size_t)(D.rows * D.inner + D.inner * D.cols + D.rows * D.cols) *
+                             ((size_t)1 << ntt_logn_for(D)) + 192;
+            }
+        rr_status e;
+        if (words && (e = ensure_scratch(words)) != RR_OK) return e;
+    }
     for (auto *J : jobs) {
         if (!ready(J) || J->op_kind != (any_mul ? 1 : 0)) continue;
+        last.push_back(J);
         J->has_op = false;
         if (J->op_kind == 0) {
             leaves.push_back(J->op_leaf);
@@ -922,6 +942,8 @@ rr_status PmScheduler::run(std::vector<PmJob *> &jobs) {
         if ((s = round(jobs, &launched)) != RR_OK) return s;
         if (!launched) break;
     }
+    for (auto *J : jobs)
+        if (J->err != RR_OK) return J->err;
     return RR_OK;
 }
 
@@ -1128,10 +1150,10 @@ rr_status rr_pm_basis(const uint32_t *seq, uint32_t b, uint64_t L, uint64_t d, u
     const uint32_t P = b * (b + 1) / 2, m = 2 * b;
     const uint64_t len = d + 1;
     hipStream_t st;
-    RR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    RR_HIP(rr::stream_create(&st));
     struct S {
         hipStream_t s;
-        ~S() { (void)hipStreamDestroy(s); }
+        ~S() { rr::stream_destroy(s); }
     } guard{st};
     DevBuf dseq, G, M;
     rr_status s;
@@ -1178,10 +1200,10 @@ rr_status rr_ntt(uint32_t *data, uint32_t logn, uint32_t count, int inverse) {
         rev[i] = r;
     }
     hipStream_t st;
-    RR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    RR_HIP(rr::stream_create(&st));
     struct S {
         hipStream_t s;
-        ~S() { (void)hipStreamDestroy(s); }
+        ~S() { rr::stream_destroy(s); }
     } guard{st};
     NttTables T;
     rr_status s;
@@ -1220,10 +1242,10 @@ rr_status rr_poly_mat_mul(const uint32_t *A, uint64_t la, const uint32_t *B, uin
              "bad product arguments");
     RR_CHECK(inner <= 128 && hi <= la + lb - 1 + 0ull, RR_EINVAL, "inner <= 128 and hi <= la + lb - 1");
     hipStream_t st;
-    RR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    RR_HIP(rr::stream_create(&st));
     struct S {
         hipStream_t s;
-        ~S() { (void)hipStreamDestroy(s); }
+        ~S() { rr::stream_destroy(s); }
     } guard{st};
     DevBuf dA, dB, dC;
     rr_status s;

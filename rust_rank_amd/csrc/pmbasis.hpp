@@ -91,6 +91,8 @@ struct PmJob {
     Frame *op_keep = nullptr;   // product keeps its A spectrum in this frame's SP
     Frame *op_given = nullptr;  // product takes its B spectrum from this frame's SP
     uint64_t op_need = 0;       // leading coefficients of G the next operation reads
+    rr_status err = RR_OK;      // set (with done) when this job's recursion failed on its own
+    uint64_t tag = 0;           // the caller's name for the job (the server's request id)
 };
 
 struct PmScheduler {
@@ -101,6 +103,14 @@ struct PmScheduler {
     size_t scratch_words = 0, lds = 0;
     uint64_t rounds = 0, launches = 0;
     uint32_t waiting = 0;  // after a round: jobs whose next operation waits for coefficients of G
+    // The jobs whose operations the last round took.  A round that fails
+    // before taking any (its work space could not be allocated) leaves every
+    // job as it was and this list empty; one that fails after taking them
+    // (a launch failed) leaves those jobs unusable.
+    std::vector<PmJob *> last;
+    // tests of failure handling: a round in which the job tagged so has an
+    // operation ready fails with RR_ENOMEM before taking any (0: never)
+    uint64_t fail_tag = 0;
 
     rr_status ensure_scratch(size_t words);
     rr_status init(uint32_t p, hipStream_t st, uint32_t max_logn, uint32_t max_m, uint32_t max_n);

@@ -12,8 +12,13 @@
 //     complete (stream 2, so it overlaps the Krylov launches),
 //   - completion: the rank inputs are extracted on the device, the result is
 //     published and the request's memory returns to the budget.
-// A request that fails (invalid input, allocation) is reported with its own
-// status and does not affect the others.
+// Host set-up of an admitted request (validation, transpose, slicing,
+// uploads) runs on set-up threads with streams of their own, so admissions do
+// not stall the ticks of the requests in flight.  A request that fails
+// (invalid input, allocation, a failing operation of its own) is reported
+// with its own status; a tick or PM-basis round that fails before any of its
+// work reached the device is retried request by request, so only the
+// requests that fail alone are dropped.
 #include "common.hpp"
 #include "batch.hpp"
 #include <thread>
@@ -23,6 +28,7 @@
 #include <memory>
 #include <chrono>
 #include <cstring>
+#include <string>
 
 namespace rr {
 
@@ -42,14 +48,30 @@ static double now_ms() {
 
 }  // namespace rr
 
+// A request between admission and the tick loop: set up on a set-up thread.
+struct Staged {
+    std::unique_ptr<rr::Request> R;
+    std::unique_ptr<rr::Pending> q;
+    rr_status status = RR_OK;
+    std::string msg;
+    hipEvent_t ev = nullptr;  // its uploads are done
+};
+
+constexpr int SERVER_SETUP_THREADS = 4;
+
 struct rr_server {
     uint32_t p = 0;
     uint64_t budget = 0, used = 0;
     int device = 0, cus = 0;
     std::thread worker;
+    std::vector<std::thread> setup;
+    std::vector<hipStream_t> setup_st;
     std::mutex mu;
-    std::condition_variable cv;
-    std::deque<std::unique_ptr<rr::Pending>> queue;
+    std::condition_variable cv, cv_setup;
+    std::deque<std::unique_ptr<rr::Pending>> queue;     // submitted, not admitted
+    std::deque<std::unique_ptr<Staged>> to_setup;       // admitted, set-up pending
+    std::deque<std::unique_ptr<Staged>> set_up;         // set up, joining the tick loop
+    uint64_t in_setup = 0;                              // admitted and not yet joined (any of the three steps)
     std::vector<rr_result> results;
     bool stop = false;
     uint64_t next_id = 1;
@@ -61,8 +83,49 @@ struct rr_server {
     uint64_t completed = 0, failed = 0, pm_rounds = 0, max_active = 0;
 
     void run();
+    void setup_loop(int w);
     void publish(rr::Request *R, int32_t status, uint64_t bytes);
 };
+
+void rr_server::setup_loop(int w) {
+    using namespace rr;
+    const bool dev_ok = hipSetDevice(device) == hipSuccess;
+    for (;;) {
+        std::unique_ptr<Staged> S;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_setup.wait(lk, [this] { return stop || !to_setup.empty(); });
+            if (stop) return;
+            S = std::move(to_setup.front());
+            to_setup.pop_front();
+        }
+        Pending &q = *S->q;
+        Request &R = *S->R;
+        rr_status e = RR_EDEVICE;
+        try {
+            if (!dev_ok) {
+                set_error("set-up thread could not select the device");
+            } else {
+                rr_csr A{q.m, q.n, q.rp.data(), q.col.data(), q.val.data()};
+                e = R.init(&A, p, q.b, q.dr.data(), q.dc.data(), q.V.data(), q.L, q.d, 2048, setup_st[w]);
+                if (e == RR_OK && (hipEventCreateWithFlags(&S->ev, hipEventDisableTiming) != hipSuccess ||
+                                   hipEventRecord(S->ev, setup_st[w]) != hipSuccess)) {
+                    e = RR_EDEVICE;
+                    set_error("set-up event failed");
+                }
+            }
+        } catch (const std::bad_alloc &) {
+            e = RR_ENOMEM;
+            set_error("host allocation failed");
+        }
+        S->status = e;
+        if (e != RR_OK) S->msg = rr_last_error();
+        S->q.reset();  // the host copy of the inputs is no longer needed
+        std::lock_guard<std::mutex> g(mu);
+        set_up.push_back(std::move(S));
+        cv.notify_all();
+    }
+}
 
 void rr_server::publish(rr::Request *R, int32_t status, uint64_t bytes) {
     rr_result res;
@@ -84,13 +147,19 @@ void rr_server::run() {
     (void)hipSetDevice(device);
     std::vector<Request *> kry;
     std::vector<PmJob *> jobs;
+    auto fail = [](Request *R, rr_status e) {
+        if (R->stage == STAGE_FAILED) return;
+        R->stage = STAGE_FAILED;
+        R->fail = e;
+    };
     for (;;) {
-        // admission, arrival order, while the footprint fits the budget
-        std::vector<std::unique_ptr<Pending>> admit;
+        // admission, arrival order, while the footprint fits the budget; the
+        // admitted requests go to the set-up threads
+        std::vector<std::unique_ptr<Staged>> joining;
         {
             std::unique_lock<std::mutex> lk(mu);
             if (stop) return;
-            if (active.empty() && queue.empty()) {
+            if (active.empty() && queue.empty() && in_setup == 0) {
                 cv.wait_for(lk, std::chrono::milliseconds(20));
                 continue;
             }
@@ -98,22 +167,49 @@ void rr_server::run() {
                 Pending &q = *queue.front();
                 uint64_t need = request_bytes(q.m, q.n, q.rp[q.m], q.b, q.L, q.d, false);
                 // a request larger than the whole budget runs alone
-                if (used + need > budget && !(active.empty() && admit.empty())) break;
+                const bool alone = active.empty() && in_setup == 0;
+                if (used + need > budget && !alone) break;
                 used += need;
-                admit.push_back(std::move(queue.front()));
+                in_setup++;
+                std::unique_ptr<Staged> S(new Staged());
+                S->R.reset(new Request());
+                S->R->id = q.id;
+                S->R->t_arrive = q.t_arrive;
+                S->R->t_admit = now_ms();
+                S->R->budget_bytes = need;
+                S->q = std::move(queue.front());
                 queue.pop_front();
+                to_setup.push_back(std::move(S));
+                cv_setup.notify_one();
+            }
+            // set-up done: join once the uploads are complete
+            for (size_t i = 0; i < set_up.size();) {
+                Staged &S = *set_up[i];
+                if (S.status == RR_OK && hipEventQuery(S.ev) == hipErrorNotReady) {
+                    i++;
+                    continue;
+                }
+                joining.push_back(std::move(set_up[i]));
+                set_up.erase(set_up.begin() + i);
+                in_setup--;
+            }
+            if (active.empty() && joining.empty()) {  // only set-ups in flight: wait for one to end
+                cv.wait_for(lk, std::chrono::milliseconds(1));
+                continue;
             }
         }
-        for (auto &q : admit) {
-            std::unique_ptr<Request> R(new Request());
-            rr_csr A{q->m, q->n, q->rp.data(), q->col.data(), q->val.data()};
-            R->id = q->id;
-            R->t_arrive = q->t_arrive;
-            R->t_admit = now_ms();
-            R->budget_bytes = request_bytes(q->m, q->n, q->rp[q->m], q->b, q->L, q->d, false);
-            rr_status s = R->init(&A, p, q->b, q->dr.data(), q->dc.data(), q->V.data(), q->L, q->d, 2048, st_k);
+        for (auto &S : joining) {
+            std::unique_ptr<Request> R = std::move(S->R);
+            if (S->ev) (void)hipEventDestroy(S->ev);
+            rr_status s = S->status;
+            if (s != RR_OK) {
+                set_error("%s", S->msg.c_str());
+                publish(R.get(), s, R->budget_bytes);
+                continue;
+            }
+            R->pm.tag = R->id;
             // a long sequence's PM-basis starts on its leading coefficients
-            if (s == RR_OK && R->d >= PM_STREAM_D && (s = R->alloc_pm()) == RR_OK) s = pm.add(&R->pm);
+            if (R->d >= PM_STREAM_D && (s = R->alloc_pm()) == RR_OK) s = pm.add(&R->pm);
             if (s != RR_OK) {
                 publish(R.get(), s, R->budget_bytes);
                 continue;
@@ -134,8 +230,19 @@ void rr_server::run() {
                 left = std::min<uint64_t>(left, R->L - R->t);
             }
         const uint32_t nt = first ? 1u : (uint32_t)std::min<uint64_t>(SERVER_TICKS, left);
-        if (!kry.empty() && ticks.tick(kry, p, nt) != RR_OK)
-            for (auto *R : kry) R->stage = STAGE_FAILED;
+        if (!kry.empty()) {
+            const rr_status ts = ticks.tick(kry, p, nt);
+            if (ts != RR_OK && !ticks.launched) {
+                // nothing reached the device: each request alone, so only those failing alone drop out
+                for (auto *R : kry) {
+                    std::vector<Request *> one{R};
+                    const rr_status e = ticks.tick(one, p, nt);
+                    if (e != RR_OK) fail(R, e);
+                }
+            } else if (ts != RR_OK) {
+                for (auto *R : kry) fail(R, ts);  // the stream holds part of this tick's work
+            }
+        }
         // sequences completed: build G on the Krylov stream, hand over to the
         // PM stream; PM-basis jobs already running get the coefficients their
         // next operation waits for as soon as the sequence has them
@@ -143,15 +250,16 @@ void rr_server::run() {
         for (auto &R : active) {
             if (R->stage != STAGE_KRYLOV) continue;
             const bool running = R->G.ptr != nullptr;
+            rr_status e = RR_OK;
             if (R->t >= R->L) {
-                if (R->start_pm(st_k) != RR_OK || (!running && pm.add(&R->pm) != RR_OK)) {
-                    R->stage = STAGE_FAILED;
+                if ((e = R->start_pm(st_k)) != RR_OK || (!running && (e = pm.add(&R->pm)) != RR_OK)) {
+                    fail(R.get(), e);
                     continue;
                 }
                 handed = true;
             } else if (running && R->pm.has_op && R->pm.op_need > R->pm.avail && R->t >= R->pm.op_need) {
-                if (R->grow_G(st_k) != RR_OK) {
-                    R->stage = STAGE_FAILED;
+                if ((e = R->grow_G(st_k)) != RR_OK) {
+                    fail(R.get(), e);
                     continue;
                 }
                 handed = true;
@@ -170,15 +278,32 @@ void rr_server::run() {
         auto in_pm = [](const Request *R) {
             return !R->pm.done && (R->stage == STAGE_PM || (R->stage == STAGE_KRYLOV && R->G.ptr));
         };
+        auto owner = [&](const PmJob *J) -> Request * {
+            for (auto &R : active)
+                if (&R->pm == J) return R.get();
+            return nullptr;
+        };
         jobs.clear();
         for (auto &R : active)
             if (in_pm(R.get())) jobs.push_back(&R->pm);
         if (!jobs.empty()) {
             bool launched = false;
-            if (pm.round(jobs, &launched) != RR_OK)
-                for (auto &R : active)
-                    if (in_pm(R.get())) R->stage = STAGE_FAILED;
+            const rr_status ps = pm.round(jobs, &launched);
             pm_rounds += launched;
+            if (ps != RR_OK && pm.last.empty()) {
+                // no job's operation was taken: each job alone
+                for (auto *J : jobs) {
+                    std::vector<PmJob *> one{J};
+                    bool l1 = false;
+                    const rr_status e = pm.round(one, &l1);
+                    pm_rounds += l1;
+                    if (e != RR_OK) fail(owner(J), e);
+                }
+            } else if (ps != RR_OK) {
+                for (auto *J : pm.last) fail(owner(J), ps);  // their operations were taken and not completed
+            }
+            for (auto *J : jobs)  // a job's own failure (its recursion could not go on)
+                if (J->err != RR_OK) fail(owner(J), J->err);
         }
         // completion
         for (size_t i = 0; i < active.size();) {
@@ -187,7 +312,7 @@ void rr_server::run() {
             int32_t status = RR_OK;
             if (R->stage == STAGE_FAILED) {
                 finished = true;
-                status = RR_EDEVICE;
+                status = R->fail != RR_OK ? R->fail : RR_EDEVICE;
             } else if (R->stage == STAGE_PM && R->pm.done) {
                 status = R->finish_rank(st_pm);
                 finished = true;
@@ -216,12 +341,25 @@ rr_status rr_server_create(uint32_t p, uint64_t hbm_budget_bytes, rr_server **ou
     S->budget = hbm_budget_bytes;
     RR_HIP(hipGetDevice(&S->device));
     RR_HIP(hipDeviceGetAttribute(&S->cus, hipDeviceAttributeMultiprocessorCount, S->device));
-    RR_HIP(hipStreamCreateWithFlags(&S->st_k, hipStreamNonBlocking));
-    RR_HIP(hipStreamCreateWithFlags(&S->st_pm, hipStreamNonBlocking));
+    RR_HIP(rr::stream_create(&S->st_k));
+    RR_HIP(rr::stream_create(&S->st_pm));
     rr_status s;
     if ((s = S->ticks.init(S->st_k, S->cus)) != RR_OK) return s;
     if ((s = S->pm.init(p, S->st_pm, NTT_MAX_LOGN, 128, 64)) != RR_OK) return s;
+    // RR_SERVER_FAIL_AT=tick:<id> or pm:<id> (tests of failure handling): the
+    // ticks or PM-basis rounds holding that request fail before queueing work
+    if (const char *f = getenv("RR_SERVER_FAIL_AT")) {
+        const uint64_t id = strtoull(strchr(f, ':') ? strchr(f, ':') + 1 : f, nullptr, 10);
+        if (!strncmp(f, "tick:", 5)) S->ticks.fail_id = id;
+        if (!strncmp(f, "pm:", 3)) S->pm.fail_tag = id;
+    }
+    for (int w = 0; w < SERVER_SETUP_THREADS; w++) {
+        hipStream_t ws;
+        RR_HIP(rr::stream_create(&ws));
+        S->setup_st.push_back(ws);
+    }
     rr_server *raw = S.release();
+    for (int w = 0; w < SERVER_SETUP_THREADS; w++) raw->setup.emplace_back([raw, w] { raw->setup_loop(w); });
     raw->worker = std::thread([raw] { raw->run(); });
     *out = raw;
     return RR_OK;
@@ -289,12 +427,27 @@ void rr_server_destroy(rr_server *S) {
         S->stop = true;
         S->queue.clear();
         S->cv.notify_all();
+        S->cv_setup.notify_all();
     }
     if (S->worker.joinable()) S->worker.join();
+    for (auto &t : S->setup)
+        if (t.joinable()) t.join();
+    for (auto ws : S->setup_st) {
+        (void)hipStreamSynchronize(ws);
+        rr::stream_destroy(ws);
+    }
+    for (auto *dq : {&S->to_setup, &S->set_up})
+        for (auto &st : *dq)
+            if (st->ev) (void)hipEventDestroy(st->ev);
+    S->to_setup.clear();
+    S->set_up.clear();
     if (S->st_k) (void)hipStreamSynchronize(S->st_k);
     if (S->st_pm) (void)hipStreamSynchronize(S->st_pm);
     S->active.clear();
-    delete S;
+    hipStream_t sk = S->st_k, sp = S->st_pm;
+    delete S;  // its tick runner and scheduler still use the streams while they go
+    rr::stream_destroy(sk);
+    rr::stream_destroy(sp);
 }
 
 }  // extern "C"

@@ -13,14 +13,21 @@ namespace rr {
 namespace {
 struct FreeBlock {
     void *ptr;
-    uint64_t gen;  // free generation: safe to reuse once the device was synchronised after it
+    uint64_t gen;  // free generation: safe to reuse once a marker recorded after it has completed
 };
 struct DevCache {
     std::map<size_t, std::deque<FreeBlock>> free;  // per size class, oldest first
     uint64_t gen_freed = 0, gen_synced = 0;
+    // markers in flight: one event per library stream of this device, recorded
+    // after every free up to gen_marker
+    std::vector<hipEvent_t> marker;
+    size_t nmark = 0;  // events of the live marker (one per stream it was recorded on)
+    uint64_t gen_marker = 0;
+    bool marker_live = false;
 };
 std::mutex g_cache_mu;
 std::map<int, DevCache> g_cache;
+std::map<hipStream_t, int> g_streams;  // library streams and their devices (under g_cache_mu)
 
 size_t size_class(size_t n) {
     if (n <= 256) return 256;
@@ -29,7 +36,63 @@ size_t size_class(size_t n) {
     const size_t step = (size_t)1 << (k - 3);
     return (n + step - 1) / step * step;
 }
+
+// Advances gen_synced when the markers in flight have completed, and puts
+// new markers on the device's streams when frees are not covered yet
+// (caller holds g_cache_mu; the device is current).  Never waits.
+void advance_markers_(DevCache &D, int dev) {
+    if (D.marker_live) {
+        for (size_t i = 0; i < D.nmark; i++)
+            if (hipEventQuery(D.marker[i]) != hipSuccess) return;  // still running (or an error: try again later)
+        D.gen_synced = std::max(D.gen_synced, D.gen_marker);
+        D.marker_live = false;
+    }
+    if (D.gen_freed <= D.gen_synced) return;
+    // one event per library stream of the device (streams destroyed since the
+    // frees waited for their work in stream_destroy)
+    size_t k = 0;
+    for (auto &kv : g_streams) {
+        if (kv.second != dev) continue;
+        if (k == D.marker.size()) {
+            hipEvent_t e;
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return;
+            D.marker.push_back(e);
+        }
+        if (hipEventRecord(D.marker[k], kv.first) != hipSuccess) return;
+        k++;
+    }
+    D.nmark = k;
+    D.gen_marker = D.gen_freed;
+    D.marker_live = true;
+}
+
+void advance_markers(DevCache &D, int dev) {
+    // leaves the thread's last-error state as it found it when it was clean
+    const bool clean = hipPeekAtLastError() == hipSuccess;
+    advance_markers_(D, dev);
+    if (clean) (void)hipGetLastError();
+}
 }  // namespace
+
+hipError_t stream_create(hipStream_t *st) {
+    hipError_t e = hipStreamCreateWithFlags(st, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> g(g_cache_mu);
+    g_streams[*st] = dev;
+    return hipSuccess;
+}
+
+void stream_destroy(hipStream_t st) {
+    if (!st) return;
+    (void)hipStreamSynchronize(st);  // nothing on it can use a freed block any more
+    {
+        std::lock_guard<std::mutex> g(g_cache_mu);
+        g_streams.erase(st);
+    }
+    (void)hipStreamDestroy(st);
+}
 
 void *dev_alloc(size_t bytes, size_t *cls, int *device, hipError_t *err) {
     int dev = 0;
@@ -38,23 +101,16 @@ void *dev_alloc(size_t bytes, size_t *cls, int *device, hipError_t *err) {
     *cls = c;
     *device = dev;
     {
-        std::unique_lock<std::mutex> lk(g_cache_mu);
+        std::lock_guard<std::mutex> lk(g_cache_mu);
         DevCache &D = g_cache[dev];
-        for (;;) {
-            auto it = D.free.find(c);
-            if (it == D.free.end() || it->second.empty()) break;
-            FreeBlock fb = it->second.front();
-            if (fb.gen > D.gen_synced) {  // possibly still in use: wait for the device once
-                const uint64_t g = D.gen_freed;
-                lk.unlock();
-                *err = hipDeviceSynchronize();
-                lk.lock();
-                if (*err != hipSuccess) return nullptr;
-                D.gen_synced = std::max(D.gen_synced, g);
-                continue;
+        auto it = D.free.find(c);
+        if (it != D.free.end() && !it->second.empty()) {
+            if (it->second.front().gen > D.gen_synced) advance_markers(D, dev);
+            if (it->second.front().gen <= D.gen_synced) {
+                void *p = it->second.front().ptr;
+                it->second.pop_front();
+                return p;
             }
-            it->second.pop_front();
-            return fb.ptr;
         }
     }
     void *p = nullptr;
@@ -85,6 +141,7 @@ void dev_cache_release(int device) {
         for (auto &kv : D.free)
             for (const FreeBlock &fb : kv.second) blocks.push_back(fb.ptr);
         D.free.clear();
+        D.gen_synced = D.gen_freed;  // after the synchronisation below nothing freed is in use
     }
     int cur = 0;
     (void)hipGetDevice(&cur);
@@ -123,7 +180,7 @@ rr_status rr_copy_probe(uint64_t bytes, uint32_t iters, double *ms_out, double *
     if ((s = a.alloc(bytes)) != RR_OK) return s;
     if ((s = b.alloc(bytes)) != RR_OK) return s;
     hipStream_t st;
-    RR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    RR_HIP(stream_create(&st));
     RR_HIP(hipMemsetAsync(a.ptr, 1, bytes, st));
     int dev = 0, cus = 0;
     RR_HIP(hipGetDevice(&dev));
@@ -144,7 +201,7 @@ rr_status rr_copy_probe(uint64_t bytes, uint32_t iters, double *ms_out, double *
     *gbs_out = 2.0 * (double)bytes / (*ms_out * 1e-3) / 1e9;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(st);
+    stream_destroy(st);
     return RR_OK;
 }
 

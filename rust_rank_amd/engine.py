@@ -112,6 +112,19 @@ def lib():
         getattr(L, name).restype = C.c_int
     L.rr_copy_probe.argtypes = [C.c_uint64, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.rr_copy_probe.restype = C.c_int
+    _u64ptr = C.POINTER(C.c_uint64)
+    L.rr_sms_open.argtypes = [C.c_char_p, C.c_uint32, C.POINTER(C.c_void_p), _u64ptr, _u64ptr, _u64ptr]
+    L.rr_sms_copy.argtypes = [C.c_void_p, _u64p, _u32p, _u32p]
+    L.rr_sms_close.argtypes = [C.c_void_p]
+    L.rr_sms_close.restype = None
+    L.rr_wdm_info.argtypes = [C.c_char_p, _u64ptr, _u64ptr, C.POINTER(C.c_uint32), _u64ptr, C.POINTER(C.c_uint32)]
+    L.rr_wdm_load.argtypes = [C.c_char_p, _u32p, _u32p, _u32p, _u32p, _u32p]
+    L.rr_wdm_save.argtypes = [C.c_char_p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, _u32p, _u32p, _u32p,
+                              _u32p, _u32p, C.c_uint64]
+    L.rr_seq_resume_count.argtypes = [C.c_uint64, C.c_uint64]
+    L.rr_seq_resume_count.restype = C.c_uint64
+    for name in ("rr_sms_open", "rr_sms_copy", "rr_wdm_info", "rr_wdm_load", "rr_wdm_save"):
+        getattr(L, name).restype = C.c_int
     for name in ("rr_krylov_create", "rr_krylov_run", "rr_krylov_state", "rr_krylov_set_profiling",
                  "rr_krylov_stats_get", "rr_pm_basis", "rr_rank_from_basis", "rr_ntt", "rr_poly_mat_mul"):
         getattr(L, name).restype = C.c_int
@@ -228,6 +241,56 @@ def poly_mat_mul(A, B, lo: int, hi: int, p: int):
     out = np.zeros((rows, cols, hi - lo), np.uint32)
     _check(lib().rr_poly_mat_mul(A, la, B, lb, rows, inner, cols, lo, hi, p, out))
     return out
+
+
+def load_sms(path: str, p: int):
+    """SMS text file -> Csr (rr_sms_open / rr_sms_copy), values as residues mod p."""
+    from .synth import Csr
+    h = C.c_void_p()
+    m, n, nnz = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    _check(lib().rr_sms_open(os.fsencode(path), p, C.byref(h), C.byref(m), C.byref(n), C.byref(nnz)))
+    try:
+        rp = np.zeros(m.value + 1, np.uint64)
+        col = np.zeros(max(nnz.value, 1), np.uint32)
+        val = np.zeros(max(nnz.value, 1), np.uint32)
+        _check(lib().rr_sms_copy(h, rp, col, val))
+    finally:
+        lib().rr_sms_close(h)
+    return Csr(m.value, n.value, rp, col[: nnz.value], val[: nnz.value])
+
+
+def wdm_info(path: str) -> dict:
+    m, n, ln = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    p, b = C.c_uint32(), C.c_uint32()
+    _check(lib().rr_wdm_info(os.fsencode(path), C.byref(m), C.byref(n), C.byref(p), C.byref(ln), C.byref(b)))
+    return {"m": m.value, "n": n.value, "p": p.value, "len": ln.value, "b": b.value}
+
+
+def wdm_load(path: str) -> dict:
+    """.wdm checkpoint -> dict(m, n, p, b, row_precond, col_precond, V (n x b), curv (n x b), seq [len][P])."""
+    h = wdm_info(path)
+    m, n, b, ln = h["m"], h["n"], h["b"], h["len"]
+    P = b * (b + 1) // 2
+    h.update(row_precond=np.zeros(m, np.uint32), col_precond=np.zeros(n, np.uint32),
+             V=np.zeros((n, b), np.uint32), curv=np.zeros((n, b), np.uint32), seq=np.zeros((max(ln, 1), P), np.uint32))
+    _check(lib().rr_wdm_load(os.fsencode(path), h["row_precond"], h["col_precond"], h["V"], h["curv"], h["seq"]))
+    h["seq"] = h["seq"][:ln]
+    return h
+
+
+def wdm_save(path: str, m: int, n: int, p: int, row_precond, col_precond, V, curv, seq):
+    """Writes a .wdm checkpoint (rr_wdm_save); V, curv n x b, seq [len][b(b+1)/2]."""
+    V = np.ascontiguousarray(V, dtype=np.uint32)
+    b = V.shape[1]
+    seq = np.ascontiguousarray(seq, dtype=np.uint32).reshape(-1, b * (b + 1) // 2)
+    _check(lib().rr_wdm_save(os.fsencode(path), m, n, p, b, np.ascontiguousarray(row_precond, dtype=np.uint32),
+                             np.ascontiguousarray(col_precond, dtype=np.uint32), V,
+                             np.ascontiguousarray(curv, dtype=np.uint32),
+                             seq if seq.size else np.zeros((1, 1), np.uint32), seq.shape[0]))
+
+
+def seq_resume_count(have: int, max_nlen: int) -> int:
+    return int(lib().rr_seq_resume_count(have, max_nlen))
 
 
 def seq_len(m: int, n: int, b: int, reference_policy: bool = False) -> int:

@@ -10,8 +10,10 @@ Workloads (sizes as named):
   plain_small      2000 x 1800, 8 nnz per base row, 200 dependent rows, b = 8
   graph_complex    200000 x 180000, 12 distinct Zipf(1.1) columns per row, +-1/+-2, b = 32
   large_single     5,000,000 x 4,500,000, 20 uniform columns per row, 5000 dependent columns, b = 64
-  batched / served rows log-uniform, cols = 0.9 rows, 6-16 (8-20) nnz per row, +-1/+-2,
-                   1-5 % planted dependent columns
+  batched          rows log-uniform in [1000, 50000], cols = 0.9 rows, 6-16 nnz per row,
+                   +-1/+-2, 1-5 % planted dependent columns
+  served           the same with rows log-uniform in [10^3, 10^6] (rehearsal: [10^3, 10^5])
+                   and 8-20 nnz per row
 
 Matrices are CSR with uint64 row pointers, uint32 column indices and uint32
 values in [0, p) (negative entries stored as p - |v|).
@@ -363,8 +365,23 @@ def batch_matrix(seed: int, index: int, p: int = P_DEFAULT, rows_lo: int = 1000,
     return plant_dependent_columns(Rng(seed, 30_000 + index), A, int(round(frac * rows)), p, small=True)
 
 
+SERVED_ROWS_HI = 1_000_000     # the served workload's row range is [10^3, 10^6]
+SERVED_REHEARSAL_ROWS_HI = 100_000  # the scaled rehearsal caps rows at 10^5
+
+
+def served_matrix(seed: int, index: int, rows_hi: int = SERVED_ROWS_HI, p: int = P_DEFAULT) -> Csr:
+    """One request of the served workload: rows log-uniform in [10^3, rows_hi],
+    cols = floor(0.9 rows), 8-20 nnz per row, +-1/+-2, 1-5 % of rows planted
+    as dependent columns (the batched generator with the served ranges)."""
+    return batch_matrix(seed, index, p, rows_lo=1000, rows_hi=rows_hi, k_lo=8, k_hi=20)
+
+
 def _batch_matrix_args(args):
     return batch_matrix(*args)
+
+
+def _served_matrix_args(args):
+    return served_matrix(*args)
 
 
 def batch_matrices(seed: int, count: int, workers: int = 0) -> list:
@@ -377,6 +394,17 @@ def batch_matrices(seed: int, count: int, workers: int = 0) -> list:
     from concurrent.futures import ProcessPoolExecutor
     with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
         return list(ex.map(_batch_matrix_args, [(seed, i) for i in range(count)], chunksize=8))
+
+
+def served_matrices(seed: int, count: int, rows_hi: int = SERVED_ROWS_HI, workers: int = 0) -> list:
+    """served_matrix(seed, i, rows_hi) for i < count, on up to `workers` processes."""
+    workers = workers or min(16, os.cpu_count() or 1)
+    if workers <= 1 or count < 16:
+        return [served_matrix(seed, i, rows_hi) for i in range(count)]
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_served_matrix_args, [(seed, i, rows_hi) for i in range(count)], chunksize=4))
 
 
 def preconditioners(seed: int, m: int, n: int, b: int, p: int = P_DEFAULT):

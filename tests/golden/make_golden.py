@@ -14,6 +14,8 @@ Every fixture is data: inputs and the reference's outputs for them.
                      on Krylov sequences of small matrices at p in {7, 97, 3323, 65521}
   matrix_txt.npz     cuda_test/matrix.txt through the reference SMS loader, transpose and
                      rescaling (cuda_test/matrices.h), and its dense rank (modular_rank)
+  wdm_ref.npz        .wdm files written by the reference's save_wdm_file_sym
+                     (cuda_test/wdmfiles.h:92-160): the inputs and the file's bytes
 """
 from __future__ import annotations
 
@@ -144,5 +146,29 @@ def main():
              val_t=valt, dr=dr, dc=dc, a_scaled=a_scaled, at_scaled=at_scaled, rank=np.array([rk]))
 
 
+def wdm_fixture():
+    """Inputs and the bytes of the .wdm file the reference's writer makes of them."""
+    import tempfile
+    rng = np.random.default_rng(2024)
+    out = {}
+    for i, (p, m, n, b, ln) in enumerate([(3323, 7, 5, 3, 4), (P, 6, 4, 2, 5), (97, 3, 9, 1, 1)]):
+        dr = rng.integers(1, p, m).astype(np.uint32)
+        dc = rng.integers(1, p, n).astype(np.uint32)
+        V = rng.integers(0, p, (n, b)).astype(np.uint32)
+        curv = rng.integers(0, p, (n, b)).astype(np.uint32)
+        seq = rng.integers(0, p, (ln, b * (b + 1) // 2)).astype(np.uint32)
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "ref.wdm")
+            O.ref_save_wdm(path, m, n, p, dr, dc, V, curv, seq)
+            data = np.frombuffer(open(path, "rb").read(), dtype=np.uint8)
+        out.update({f"meta{i}": np.array([p, m, n, b, ln], np.int64), f"dr{i}": dr, f"dc{i}": dc, f"V{i}": V,
+                    f"curv{i}": curv, f"seq{i}": seq, f"file{i}": data})
+    save("wdm_ref.npz", **out)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["wdm"]:
+        wdm_fixture()
+    else:
+        main()
+        wdm_fixture()

@@ -135,3 +135,25 @@ def test_cached_device_memory_reuse_and_release():
     assert list(first) == list(again) == list(fresh)
     singles = [eng.rank(A, P, 16, *pre[i])[0] for i, A in enumerate(mats)]
     assert list(first) == singles
+
+
+def test_batch_beyond_one_staging_slot():
+    """2000 tiny matrices in one rr_batch_rank: a chunk's tick descriptors
+    (16 ticks x 2000 requests) exceed one staging slot and go down in several
+    staged launches; every rank equals dense Gaussian elimination, and a
+    sample equals the single-matrix path."""
+    eng = _eng()
+    n = 2000
+    mats, bs, drs, dcs, Vs = [], [], [], [], []
+    for i in range(n):
+        A = synth.batch_matrix(41, i, rows_lo=40, rows_hi=120)
+        b = (4, 8)[i % 2]
+        dr, dc, V = synth.preconditioners(7000 + i, A.m, A.n, b, P)
+        mats.append(A); bs.append(b); drs.append(dr); dcs.append(dc); Vs.append(V)
+    print(f"batch of {n} matrices, rows {min(A.m for A in mats)}..{max(A.m for A in mats)}", flush=True)
+    ranks, stats = eng.batch_rank(mats, P, bs, drs, dcs, Vs)
+    print(stats, flush=True)
+    for i, A in enumerate(mats):
+        assert ranks[i] == O.dense_rank(A.to_dense(P), P), i
+    for i in range(0, n, 250):
+        assert ranks[i] == eng.rank(mats[i], P, bs[i], drs[i], dcs[i], Vs[i])[0], i

@@ -122,3 +122,27 @@ def test_bad_inputs_raise():
         eng.Krylov(B, P, 4, dr, dc, V)
     with pytest.raises(eng.RankEngineError):
         eng.Krylov(A, P, 4, dr, dc, V, t_have=2)
+
+
+def test_resume_from_wdm_checkpoint(tmp_path):
+    """A .wdm checkpoint written mid-sequence (src/main.rs:186-202) and read
+    back resumes to the same entries as the continuous run, with the entry
+    count the reference's loop would append (src/main.rs:105-131)."""
+    eng = _engine()
+    A = synth.uniform_sparse(4, 1, 600, 500, 3, 9, P, small=False)
+    b, max_nlen = 8, 41
+    dr, dc, V = synth.preconditioners(4, A.m, A.n, b, P)
+    with eng.Krylov(A, P, b, dr, dc, V) as K:
+        first = K.run(21)
+        curv, t = K.state()
+        rest = K.run(20)
+    path = str(tmp_path / "ck.wdm")
+    eng.wdm_save(path, A.m, A.n, P, dr, dc, V, curv, first)
+    w = eng.wdm_load(path)
+    assert w["len"] == t == 21 and np.array_equal(w["V"], V) and np.array_equal(w["seq"], first)
+    todo = eng.seq_resume_count(w["len"], max_nlen)
+    assert todo == 20
+    with eng.Krylov(A, P, b, w["row_precond"], w["col_precond"], w["curv"], t_have=w["len"]) as K2:
+        again = K2.run(todo)
+    assert np.array_equal(again, rest)
+    assert np.array_equal(np.vstack([first, again]), O.krylov(A, P, b, dr, dc, V, 0, 41))

@@ -103,3 +103,54 @@ def test_server_streamed_pm_basis_equals_single():
     print(f"m={A.m} n={A.n} b=4: rank {got[ids[0]]['rank']} alone {alone[0]}", flush=True)
     for i, rid in enumerate(ids):
         assert got[rid]["status"] == 0 and got[rid]["rank"] == alone[i], (i, got[rid], alone[i])
+
+
+@pytest.mark.parametrize("stage", ["tick", "pm"])
+def test_server_request_failing_after_admission(stage, monkeypatch):
+    """A request admitted and set up fails in a Krylov tick or in a PM-basis
+    round (failure forced from the host before its work is queued): it is
+    reported with its own status (RR_ENOMEM) and every other request, ticked
+    and reduced in the same launches, still gets the single-path rank."""
+    eng = _eng()
+    reqs = _requests(6, 34)
+    alone = [eng.rank(A, P, b, dr, dc, V)[0] for (A, b, dr, dc, V) in reqs]
+    monkeypatch.setenv("RR_SERVER_FAIL_AT", f"{stage}:3")  # the third request submitted
+    with eng.Server(P, 64 << 30) as srv:
+        ids = [srv.submit(A, b, dr, dc, V) for (A, b, dr, dc, V) in reqs]
+        got = _collect(srv, ids)
+    assert ids[2] == 3
+    assert got[3]["status"] == -2 and got[3]["rank"] == -1, got[3]
+    for i, rid in enumerate(ids):
+        if rid != 3:
+            assert got[rid]["status"] == 0 and got[rid]["rank"] == alone[i], (i, got[rid], alone[i])
+
+
+@pytest.mark.timeout(600)
+def test_server_served_workload_ranks_match_oracle():
+    """Requests of the served workload (8-20 nonzeros per row, +-1/+-2, planted
+    deficits, block width from the autotuner) arriving while others are in
+    flight: the 16 from the low end of its row range match dense Gaussian
+    elimination; one of 17185 rows matches the oracle's whole pipeline
+    (Krylov sequence, PM-basis, rank formula) on the same inputs."""
+    eng = _eng()
+    small = [synth.served_matrix(51, i, 2500) for i in range(16)]
+    big = synth.served_matrix(5, 9, 100000)
+    assert big.m == 17185
+    mats = small + [big]
+    bs, _, _ = eng.autotune_b(mats, P)
+    reqs = [(A, int(b)) + tuple(synth.preconditioners(600 + i, A.m, A.n, int(b), P))
+            for i, (A, b) in enumerate(zip(mats, bs))]
+    with eng.Server(P, 64 << 30) as srv:
+        ids = []
+        for k, (A, b, dr, dc, V) in enumerate(reqs):
+            ids.append(srv.submit(A, b, dr, dc, V))
+            time.sleep(0.005 * (k % 3))
+        got = _collect(srv, ids, timeout=300)
+    for i, (A, b, dr, dc, V) in enumerate(reqs[:-1]):
+        assert got[ids[i]]["status"] == 0 and got[ids[i]]["rank"] == O.dense_rank(A.to_dense(P), P), i
+    A, b, dr, dc, V = reqs[-1]
+    L = 2 * min(A.m, A.n) // b + 8
+    print(f"oracle pipeline m={A.m} n={A.n} b={b} L={L}", flush=True)
+    seq = O.krylov(A, P, b, dr, dc, V, 0, L)
+    M, delta = O.pm_basis(seq, b, L - 1, P)
+    assert got[ids[-1]]["status"] == 0 and got[ids[-1]]["rank"] == O.rank_from_basis(M, delta, b, P)
