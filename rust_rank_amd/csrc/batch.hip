@@ -295,8 +295,8 @@ rr_status Request::start_pm(hipStream_t st, bool release) {
     if ((s = grow_G(st)) != RR_OK) return s;
     stage = STAGE_PM;
     if (!release) return RR_OK;
-    // the operands are no longer needed (the stream orders this after their last pass)
-    RR_HIP(hipStreamSynchronize(st));
+    // the operands are no longer needed; the device memory cache hands their
+    // blocks out again only once the passes queued before this have run
     sa.~SliceMat();
     new (&sa) SliceMat();
     sat.~SliceMat();
@@ -306,17 +306,28 @@ rr_status Request::start_pm(hipStream_t st, bool release) {
 
 // Rank from a device-resident basis [2b][2b][len] and delta: the two coefficient
 // matrices the formula needs are extracted on the device (2b^2 + 1 words come back).
-rr_status rank_from_device_basis(const uint32_t *M, const int64_t *delta, uint32_t b, uint64_t len, uint32_t p,
-                                 uint32_t *scratch, hipStream_t st, int64_t *rank) {
+rr_status rank_extract_async(const uint32_t *M, const int64_t *delta, uint32_t b, uint64_t len, uint32_t *scratch,
+                             uint32_t *host, hipStream_t st) {
     hipLaunchKernelGGL(k_rank_extract, dim3(1), dim3(256), 0, st, M, delta, b, len, scratch);
     RR_HIP(hipGetLastError());
+    RR_HIP(hipMemcpyAsync(host, scratch, ((size_t)2 * b * b + 1) * 4, hipMemcpyDeviceToHost, st));
+    return RR_OK;
+}
+
+int64_t rank_from_extracted(const uint32_t *h, uint32_t b, uint32_t p) {
+    const uint64_t K = h[0];
+    std::vector<uint32_t> F0(h + 1, h + 1 + b * b), FK(h + 1 + b * b, h + 1 + 2 * b * b);
+    const uint64_t r = (uint64_t)b * (K - 3) + host_rank_bxb(FK, b, p) + host_rank_bxb(F0, b, p);
+    return (int64_t)r;  // the reference's usize arithmetic wraps the same way when K < 3
+}
+
+rr_status rank_from_device_basis(const uint32_t *M, const int64_t *delta, uint32_t b, uint64_t len, uint32_t p,
+                                 uint32_t *scratch, hipStream_t st, int64_t *rank) {
     std::vector<uint32_t> h(2 * b * b + 1);
-    RR_HIP(hipMemcpyAsync(h.data(), scratch, h.size() * 4, hipMemcpyDeviceToHost, st));
+    rr_status s = rank_extract_async(M, delta, b, len, scratch, h.data(), st);
+    if (s != RR_OK) return s;
     RR_HIP(hipStreamSynchronize(st));
-    uint64_t K = h[0];
-    std::vector<uint32_t> F0(h.begin() + 1, h.begin() + 1 + b * b), FK(h.begin() + 1 + b * b, h.end());
-    uint64_t r = (uint64_t)b * (K - 3) + host_rank_bxb(FK, b, p) + host_rank_bxb(F0, b, p);
-    *rank = (int64_t)r;  // the reference's usize arithmetic wraps the same way when K < 3
+    *rank = rank_from_extracted(h.data(), b, p);
     return RR_OK;
 }
 
@@ -555,39 +566,6 @@ rr_status TickRunner::tick_staged(std::vector<Request *> &reqs, uint32_t p, uint
     ticks += nticks;
     return RR_OK;
 }
-
-// Events marking the last `cap` items issued to one stream, oldest first.
-struct EventRing {
-    std::vector<hipEvent_t> ev;
-    size_t head = 0, used = 0;
-    rr_status init(size_t cap) {
-        ev.assign(cap, nullptr);
-        for (auto &e : ev) RR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        return RR_OK;
-    }
-    ~EventRing() {
-        for (auto e : ev)
-            if (e) (void)hipEventDestroy(e);
-    }
-    bool room() {
-        retire();
-        return used < ev.size();
-    }
-    rr_status push(hipStream_t st) {
-        RR_HIP(hipEventRecord(ev[(head + used) % ev.size()], st));
-        used++;
-        return RR_OK;
-    }
-    bool retire() {  // drops the items the device has finished; true if any
-        bool any = false;
-        while (used && hipEventQuery(ev[head]) == hipSuccess) {
-            head = (head + 1) % ev.size();
-            used--;
-            any = true;
-        }
-        return any;
-    }
-};
 
 // Entries per rr_krylov_run call and PM-basis rounds kept queued by rank_streamed.
 constexpr uint64_t STREAM_CHUNK = 128;

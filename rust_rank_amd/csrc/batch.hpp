@@ -23,7 +23,8 @@ struct RedJob {  // S_t of one request = sum of its tiles' partials
     uint32_t *out;
 };
 
-enum : int { STAGE_KRYLOV = 0, STAGE_PM = 1, STAGE_DONE = 2, STAGE_FAILED = 3 };
+// STAGE_EXTRACT: the basis is complete and the rank inputs are on their way to the host
+enum : int { STAGE_KRYLOV = 0, STAGE_PM = 1, STAGE_DONE = 2, STAGE_FAILED = 3, STAGE_EXTRACT = 4 };
 
 // resident device footprint of one request (admission control), bytes
 uint64_t request_bytes(uint64_t m, uint64_t n, uint64_t nnz, uint32_t b, uint64_t L, uint64_t d, bool packed);
@@ -37,8 +38,13 @@ constexpr uint32_t BATCH_STEPS_PER_WAVE = 32;
 constexpr uint32_t BATCH_TICKS = 16;
 // PM-basis rounds rr_batch_rank keeps queued ahead of the device.
 constexpr uint32_t BATCH_PM_AHEAD = 8;
-// Ticks per loop of the server (between two PM-basis rounds).
+// Ticks per chunk of the server: few while requests wait to join (so they
+// join soon), more while none does (fewer launches per entry).
 constexpr uint32_t SERVER_TICKS = 4;
+constexpr uint32_t SERVER_TICKS_IDLE = 16;
+// Chunks and PM-basis rounds the server keeps queued ahead of the device.
+constexpr uint32_t SERVER_K_AHEAD = 2;
+constexpr uint32_t SERVER_PM_AHEAD = 8;
 // The batched and served modes start the PM-basis of a request of at least
 // this order while its sequence is still being produced: the leftmost chain
 // of the recursion reads only leading coefficients of G.
@@ -77,6 +83,12 @@ struct Request {
 
 rr_status rank_from_device_basis(const uint32_t *M, const int64_t *delta, uint32_t b, uint64_t len, uint32_t p,
                                  uint32_t *scratch, hipStream_t st, int64_t *rank);
+// The same in two steps: the 2b^2 + 1 words the formula needs, extracted on
+// the device and queued for copying into host (pinned) memory on st; then the
+// rank from those words once the copy is done.
+rr_status rank_extract_async(const uint32_t *M, const int64_t *delta, uint32_t b, uint64_t len, uint32_t *scratch,
+                             uint32_t *host, hipStream_t st);
+int64_t rank_from_extracted(const uint32_t *host, uint32_t b, uint32_t p);
 rr_status rank_from_device_seq(const uint32_t *seq_dev, uint32_t b, uint64_t d, uint32_t p, hipStream_t st,
                                int64_t *rank);
 
@@ -88,6 +100,39 @@ hipStream_t krylov_stream(rr_krylov *c);
 // runs on a second stream on the coefficients of G produced so far.
 rr_status rank_streamed(rr_krylov *k, uint32_t b, uint64_t L, uint64_t d, uint32_t p, uint32_t *seq_out,
                         int64_t *rank);
+
+// Events marking the last `cap` items issued to one stream, oldest first.
+struct EventRing {
+    std::vector<hipEvent_t> ev;
+    size_t head = 0, used = 0;
+    rr_status init(size_t cap) {
+        ev.assign(cap, nullptr);
+        for (auto &e : ev) RR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return RR_OK;
+    }
+    ~EventRing() {
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    bool room() {
+        retire();
+        return used < ev.size();
+    }
+    rr_status push(hipStream_t st) {
+        RR_HIP(hipEventRecord(ev[(head + used) % ev.size()], st));
+        used++;
+        return RR_OK;
+    }
+    bool retire() {  // drops the items the device has finished; true if any
+        bool any = false;
+        while (used && hipEventQuery(ev[head]) == hipSuccess) {
+            head = (head + 1) % ev.size();
+            used--;
+            any = true;
+        }
+        return any;
+    }
+};
 
 struct TickRunner {
     static constexpr size_t STAGE_BYTES = 4u << 20;

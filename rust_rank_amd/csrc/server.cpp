@@ -29,6 +29,7 @@
 #include <chrono>
 #include <cstring>
 #include <string>
+#include <map>
 
 namespace rr {
 
@@ -58,6 +59,9 @@ struct Staged {
 };
 
 constexpr int SERVER_SETUP_THREADS = 4;
+// Pinned host slots the rank inputs of completed bases are copied into (2b^2 + 1 words each, b <= 64).
+constexpr int SERVER_EXTRACT_SLOTS = 64;
+constexpr size_t EXTRACT_WORDS = 2 * 64 * 64 + 4;
 
 struct rr_server {
     uint32_t p = 0;
@@ -73,7 +77,7 @@ struct rr_server {
     std::deque<std::unique_ptr<Staged>> set_up;         // set up, joining the tick loop
     uint64_t in_setup = 0;                              // admitted and not yet joined (any of the three steps)
     std::vector<rr_result> results;
-    bool stop = false;
+    bool stop = false, broken = false;
     uint64_t next_id = 1;
     // owned by the worker thread
     hipStream_t st_k = nullptr, st_pm = nullptr;
@@ -81,6 +85,13 @@ struct rr_server {
     rr::PmScheduler pm;
     std::vector<std::unique_ptr<rr::Request>> active;
     uint64_t completed = 0, failed = 0, pm_rounds = 0, max_active = 0;
+    uint32_t *pinned = nullptr;     // SERVER_EXTRACT_SLOTS x EXTRACT_WORDS
+    std::vector<int> free_slots;
+    struct Extract {                // a request whose rank inputs are being copied back
+        int slot;
+        hipEvent_t ev;
+    };
+    std::map<const rr::Request *, Extract> extracting;
 
     void run();
     void setup_loop(int w);
@@ -145,10 +156,24 @@ void rr_server::publish(rr::Request *R, int32_t status, uint64_t bytes) {
 void rr_server::run() {
     using namespace rr;
     (void)hipSetDevice(device);
-    std::vector<Request *> kry;
+    std::vector<Request *> kry, fresh;
     std::vector<PmJob *> jobs;
+    EventRing kq, pq;
+    hipEvent_t handoff = nullptr;
+    if (kq.init(SERVER_K_AHEAD) != RR_OK || pq.init(SERVER_PM_AHEAD) != RR_OK ||
+        hipEventCreateWithFlags(&handoff, hipEventDisableTiming) != hipSuccess) {
+        std::lock_guard<std::mutex> g(mu);  // cannot run: every request fails
+        broken = true;
+        cv.notify_all();
+        return;
+    }
+    struct EvGuard {
+        hipEvent_t e;
+        ~EvGuard() { (void)hipEventDestroy(e); }
+    } handoff_guard{handoff};
+    bool pm_idle = true;
     auto fail = [](Request *R, rr_status e) {
-        if (R->stage == STAGE_FAILED) return;
+        if (!R || R->stage == STAGE_FAILED) return;
         R->stage = STAGE_FAILED;
         R->fail = e;
     };
@@ -214,33 +239,57 @@ void rr_server::run() {
                 publish(R.get(), s, R->budget_bytes);
                 continue;
             }
+            if (R->d >= PM_STREAM_D) pm_idle = false;
             active.push_back(std::move(R));
         }
         max_active = std::max<uint64_t>(max_active, active.size());
-        // the next Krylov entries (up to SERVER_TICKS, none past a sequence's
-        // end; one while a request is at its first) of every request still
-        // producing its sequence
-        kry.clear();
-        bool first = false;
-        uint64_t left = ~0ull;
-        for (auto &R : active)
-            if (R->stage == STAGE_KRYLOV && R->t < R->L) {
-                kry.push_back(R.get());
-                first |= R->t == 0;
-                left = std::min<uint64_t>(left, R->L - R->t);
-            }
-        const uint32_t nt = first ? 1u : (uint32_t)std::min<uint64_t>(SERVER_TICKS, left);
-        if (!kry.empty()) {
-            const rr_status ts = ticks.tick(kry, p, nt);
-            if (ts != RR_OK && !ticks.launched) {
-                // nothing reached the device: each request alone, so only those failing alone drop out
-                for (auto *R : kry) {
-                    std::vector<Request *> one{R};
-                    const rr_status e = ticks.tick(one, p, nt);
-                    if (e != RR_OK) fail(R, e);
+        bool issued = false;
+        // The next Krylov chunk, while fewer than SERVER_K_AHEAD are queued:
+        // requests at their first entry take a tick of their own; the others
+        // up to SERVER_TICKS_IDLE entries (SERVER_TICKS while requests wait
+        // to join), none past a sequence's end, partials within 64 M words.
+        if (kq.room()) {
+            kry.clear();
+            fresh.clear();
+            uint64_t left = ~0ull;
+            size_t words = 0;
+            for (auto &R : active)
+                if (R->stage == STAGE_KRYLOV && R->t < R->L) {
+                    if (R->t == 0) {
+                        fresh.push_back(R.get());
+                        continue;
+                    }
+                    kry.push_back(R.get());
+                    left = std::min<uint64_t>(left, R->L - R->t);
+                    words += (size_t)R->nwg * R->P;
                 }
-            } else if (ts != RR_OK) {
-                for (auto *R : kry) fail(R, ts);  // the stream holds part of this tick's work
+            bool waiting_join;
+            {
+                std::lock_guard<std::mutex> g(mu);
+                waiting_join = !queue.empty() || in_setup > 0;
+            }
+            auto run_tick = [&](std::vector<Request *> &rs, uint32_t nt) {
+                const rr_status ts = ticks.tick(rs, p, nt);
+                if (ts != RR_OK && !ticks.launched) {
+                    // nothing reached the device: each request alone, so only those failing alone drop out
+                    for (auto *R : rs) {
+                        std::vector<Request *> one{R};
+                        const rr_status e = ticks.tick(one, p, nt);
+                        if (e != RR_OK) fail(R, e);
+                    }
+                } else if (ts != RR_OK) {
+                    for (auto *R : rs) fail(R, ts);  // the stream holds part of this tick's work
+                }
+            };
+            if (!fresh.empty()) run_tick(fresh, 1);
+            if (!kry.empty()) {
+                uint64_t nt = std::min<uint64_t>(waiting_join ? SERVER_TICKS : SERVER_TICKS_IDLE, left);
+                nt = std::max<uint64_t>(1, std::min<uint64_t>(nt, (64ull << 20) / std::max<size_t>(words, 1)));
+                run_tick(kry, (uint32_t)nt);
+            }
+            if (!fresh.empty() || !kry.empty()) {
+                (void)kq.push(st_k);
+                issued = true;
             }
         }
         // sequences completed: build G on the Krylov stream, hand over to the
@@ -266,15 +315,14 @@ void rr_server::run() {
             }
         }
         if (handed) {
-            hipEvent_t ev;
-            if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
-                (void)hipEventRecord(ev, st_k);
-                (void)hipStreamWaitEvent(st_pm, ev, 0);
-                (void)hipEventDestroy(ev);
-            }
+            (void)hipEventRecord(handoff, st_k);
+            (void)hipStreamWaitEvent(st_pm, handoff, 0);
+            pm_idle = false;
+            issued = true;
         }
-        // one PM-basis round for every request whose sequence is complete or
-        // whose PM-basis runs on its leading coefficients
+        // PM-basis rounds for every request whose sequence is complete or whose
+        // PM-basis runs on its leading coefficients, while fewer than
+        // SERVER_PM_AHEAD are queued
         auto in_pm = [](const Request *R) {
             return !R->pm.done && (R->stage == STAGE_PM || (R->stage == STAGE_KRYLOV && R->G.ptr));
         };
@@ -283,20 +331,23 @@ void rr_server::run() {
                 if (&R->pm == J) return R.get();
             return nullptr;
         };
-        jobs.clear();
-        for (auto &R : active)
-            if (in_pm(R.get())) jobs.push_back(&R->pm);
-        if (!jobs.empty()) {
+        while (!pm_idle && pq.room()) {
+            jobs.clear();
+            for (auto &R : active)
+                if (in_pm(R.get())) jobs.push_back(&R->pm);
+            if (jobs.empty()) {
+                pm_idle = true;
+                break;
+            }
             bool launched = false;
             const rr_status ps = pm.round(jobs, &launched);
-            pm_rounds += launched;
             if (ps != RR_OK && pm.last.empty()) {
                 // no job's operation was taken: each job alone
                 for (auto *J : jobs) {
                     std::vector<PmJob *> one{J};
                     bool l1 = false;
                     const rr_status e = pm.round(one, &l1);
-                    pm_rounds += l1;
+                    launched |= l1;
                     if (e != RR_OK) fail(owner(J), e);
                 }
             } else if (ps != RR_OK) {
@@ -304,6 +355,36 @@ void rr_server::run() {
             }
             for (auto *J : jobs)  // a job's own failure (its recursion could not go on)
                 if (J->err != RR_OK) fail(owner(J), J->err);
+            if (!launched) {
+                pm_idle = true;  // every job is done, failed or waits for its sequence
+                break;
+            }
+            pm_rounds++;
+            (void)pq.push(st_pm);
+            issued = true;
+        }
+        // bases complete: their rank inputs go to pinned slots on the PM stream
+        for (auto &R : active) {
+            if (R->stage != STAGE_PM || !R->pm.done || free_slots.empty()) continue;
+            const int slot = free_slots.back();
+            uint32_t *h = pinned + (size_t)slot * EXTRACT_WORDS;
+            hipEvent_t ev = nullptr;
+            rr_status e = rank_extract_async(R->M.as<uint32_t>(), R->delta.as<int64_t>(), R->b, R->d + 1,
+                                             R->rank_in.as<uint32_t>(), h, st_pm);
+            if (e == RR_OK && (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+                               hipEventRecord(ev, st_pm) != hipSuccess)) {
+                set_error("completion event failed");
+                e = RR_EDEVICE;
+            }
+            if (e != RR_OK) {
+                if (ev) (void)hipEventDestroy(ev);
+                fail(R.get(), e);
+                continue;
+            }
+            free_slots.pop_back();
+            extracting[R.get()] = Extract{slot, ev};
+            R->stage = STAGE_EXTRACT;
+            issued = true;
         }
         // completion
         for (size_t i = 0; i < active.size();) {
@@ -313,9 +394,22 @@ void rr_server::run() {
             if (R->stage == STAGE_FAILED) {
                 finished = true;
                 status = R->fail != RR_OK ? R->fail : RR_EDEVICE;
-            } else if (R->stage == STAGE_PM && R->pm.done) {
-                status = R->finish_rank(st_pm);
-                finished = true;
+            } else if (R->stage == STAGE_EXTRACT) {
+                auto it = extracting.find(R);
+                const hipError_t q = hipEventQuery(it->second.ev);
+                if (q != hipErrorNotReady) {
+                    if (q == hipSuccess) {
+                        R->rank = rank_from_extracted(pinned + (size_t)it->second.slot * EXTRACT_WORDS, R->b, p);
+                        R->stage = STAGE_DONE;
+                    } else {
+                        set_error("completion event: %s", hipGetErrorString(q));
+                        status = RR_EDEVICE;
+                    }
+                    (void)hipEventDestroy(it->second.ev);
+                    free_slots.push_back(it->second.slot);
+                    extracting.erase(it);
+                    finished = true;
+                }
             }
             if (finished) {
                 R->t_done = now_ms();
@@ -325,6 +419,8 @@ void rr_server::run() {
                 i++;
             }
         }
+        // the device holds enough queued work (or is idle): wait a little
+        if (!issued && !kq.retire() && !pq.retire()) std::this_thread::yield();
     }
 }
 
@@ -353,6 +449,8 @@ rr_status rr_server_create(uint32_t p, uint64_t hbm_budget_bytes, rr_server **ou
         if (!strncmp(f, "tick:", 5)) S->ticks.fail_id = id;
         if (!strncmp(f, "pm:", 3)) S->pm.fail_tag = id;
     }
+    RR_HIP(hipHostMalloc((void **)&S->pinned, SERVER_EXTRACT_SLOTS * EXTRACT_WORDS * 4, hipHostMallocDefault));
+    for (int i = 0; i < SERVER_EXTRACT_SLOTS; i++) S->free_slots.push_back(i);
     for (int w = 0; w < SERVER_SETUP_THREADS; w++) {
         hipStream_t ws;
         RR_HIP(rr::stream_create(&ws));
@@ -388,6 +486,7 @@ rr_status rr_server_submit(rr_server *S, const rr_csr *A, uint32_t b, const uint
     q->V.assign(V, V + A->n * b);
     q->t_arrive = now_ms();
     std::lock_guard<std::mutex> g(S->mu);
+    RR_CHECK(!S->broken, RR_EDEVICE, "the server's worker could not start");
     q->id = S->next_id++;
     *id_out = q->id;
     S->queue.push_back(std::move(q));
@@ -443,7 +542,10 @@ void rr_server_destroy(rr_server *S) {
     S->set_up.clear();
     if (S->st_k) (void)hipStreamSynchronize(S->st_k);
     if (S->st_pm) (void)hipStreamSynchronize(S->st_pm);
+    for (auto &kv : S->extracting) (void)hipEventDestroy(kv.second.ev);
+    S->extracting.clear();
     S->active.clear();
+    if (S->pinned) (void)hipHostFree(S->pinned);
     hipStream_t sk = S->st_k, sp = S->st_pm;
     delete S;  // its tick runner and scheduler still use the streams while they go
     rr::stream_destroy(sk);
