@@ -407,78 +407,192 @@ __global__ __launch_bounds__(WG) void k_gram(const uint32_t *x, uint32_t nrows, 
 
 // The projection of one pass at lane width 64: the workgroup's partial of
 // S = sum_r o_r s_r^T over the rows the pass produced (s, stride 64 words),
-// with o_r = s_r * sinv_r (sinv: the inverse of the row's scale).  Rows are
-// staged 64 at a time in LDS (o and s, row pitch 80 words so the four rows an
-// operand read spans fall in different banks); the 10 tiles of 16 x 16 in the
-// 64 x 64 upper triangle are v_mfma_f64_16x16x4f64 products of o with the
-// 16-bit halves of s, every product an integer below 2^47 and 64 rows of them
-// below 2^53, so the fp64 sums are exact and are folded mod p once per chunk.
-// Wavefront w owns tiles 2w and 2w + 1 for all rows and tile 8 + w/2 for the
-// even (w even) or odd half of each chunk's row groups: 2.5 tiles each.
-constexpr int PJ_ROWS = 64, PJ_PITCH = 80;
-__global__ __launch_bounds__(WG) void k_proj64(const uint32_t *__restrict__ s, const uint32_t *__restrict__ sinv,
-                                               uint32_t nrows, uint32_t *gram, uint32_t b, uint32_t p, double pd,
-                                               double inv) {
-    __shared__ uint32_t lds[2 * PJ_ROWS * PJ_PITCH];  // o rows, then s rows; at the end the wavefronts' triangles
-    uint32_t *lo = lds, *ls = lds + PJ_ROWS * PJ_PITCH;
+// with o_r = s_r * sinv_r (sinv: the inverse of the row's scale), on the int8
+// matrix cores.  Each residue v is taken as its centred representative
+// c = v or v - p (|c| < 2^30) and split into four signed 8-bit digits,
+// c = sum_a d_a 256^a; then S = sum_w 256^w C_w with
+// C_w = sum_{a + b = w} O_a^T S_b, each a v_mfma_i32_16x16x64_i8 product over
+// 64 rows.  A row adds at most 4 * 128^2 = 2^16 to an accumulator, so 2^15
+// rows fit int32: the launch gives no workgroup more (checked by the host),
+// and the accumulators are folded mod p at the end.  The grid is one resident
+// wave of workgroups; the pass's nparts partial slots beyond it are zeroed.  Rows are staged 64 at a time: thread (column c, row group g)
+// loads rows 16g .. 16g + 15 of its column, and writes each digit plane as 16
+// consecutive bytes (rows) per column, which is exactly one lane's operand
+// (lane l: column l % 16 of its tile, rows 16 (l / 16) .. + 15).  The 10 tiles
+// of 16 x 16 in the 64 x 64 upper triangle: wavefront w owns tiles 2w and
+// 2w + 1 (all 16 digit pairs) and, of tile 8 + w / 2, the pairs of weights
+// {0, 3, 4} (w even) or {1, 2, 5, 6} (w odd): 40 products each per 64 rows.
+typedef int pj_v4i __attribute__((ext_vector_type(4)));
+constexpr int PJ_ROWS = 64, PJ_FOLD_ROWS = 1 << 15, PJ_CS = 5;
+// The four signed digits of v's centred representative c (v - p when v > p/2),
+// byte a = digit a: c + 0x80808080 has the bytes d_a + 128 (all in [0, 255],
+// so no carries, as |c| < 2^30), and flipping their top bits gives d_a.
+__device__ __forceinline__ uint32_t pj_digits(uint32_t v, uint32_t p, int32_t half) {
+    const uint32_t c = (int32_t)v > half ? v - p : v;
+    return (c + 0x80808080u) ^ 0x80808080u;
+}
+// 4 x 4 byte transpose: in, one dword per row (its digits); out, one dword per
+// digit (its rows).
+__device__ __forceinline__ void pj_transpose(uint32_t (&d)[4]) {
+    const uint32_t t01l = __builtin_amdgcn_perm(d[1], d[0], 0x05010400u);
+    const uint32_t t01h = __builtin_amdgcn_perm(d[1], d[0], 0x07030602u);
+    const uint32_t t23l = __builtin_amdgcn_perm(d[3], d[2], 0x05010400u);
+    const uint32_t t23h = __builtin_amdgcn_perm(d[3], d[2], 0x07030602u);
+    d[0] = __builtin_amdgcn_perm(t23l, t01l, 0x05040100u);
+    d[1] = __builtin_amdgcn_perm(t23l, t01l, 0x07060302u);
+    d[2] = __builtin_amdgcn_perm(t23h, t01h, 0x05040100u);
+    d[3] = __builtin_amdgcn_perm(t23h, t01h, 0x07060302u);
+}
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_proj64(const uint32_t *__restrict__ s, const uint32_t *__restrict__ sinv,
+                                               uint32_t nrows, uint32_t *gram, uint32_t nparts, uint32_t b,
+                                               uint32_t p, double pd, double inv) {
+    // digit planes [digit][column][4 x 16 rows + 1 pad], o then s (the pad puts
+    // the 16 lanes of a 16-byte access on distinct banks); at the end the
+    // wavefronts' triangles
+    __shared__ pj_v4i lds[2 * 4 * 64 * PJ_CS];
+    pj_v4i *OL = lds, *SL = lds + 4 * 64 * PJ_CS;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int k = lane >> 4, i = lane & 15;
-    // tiles (I, J), I <= J, in the order 2w, 2w + 1 (all row groups), 8 + w / 2 (half of them)
+    const int ti = lane & 15, tg = lane >> 4;
     constexpr uint8_t TI[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3}, TJ[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
-    const int t0 = 2 * wave, t1 = 2 * wave + 1, t2 = 8 + (wave >> 1), half = wave & 1;
-    gram_v4d acc[6];
+    const int t0 = 2 * wave, t1 = 2 * wave + 1, t2 = 8 + (wave >> 1), odd = wave & 1;
+    pj_v4i acc[2][7], acc2[4];
 #pragma unroll
-    for (int t = 0; t < 6; t++) acc[t] = gram_v4d{0.0, 0.0, 0.0, 0.0};
-    const uint32_t nchunks = (nrows + PJ_ROWS - 1) / PJ_ROWS;
-    // staging: thread -> row tid / 4, words 16 (tid % 4) .. + 15
-    const uint32_t srow = tid >> 2, sw = (tid & 3) * 16;
-    gv4u rv[4];
-    uint32_t rinv = 0;
-    auto load = [&](uint32_t c) {
-        const uint32_t r = c * PJ_ROWS + srow;
-        const bool ok = r < nrows;
-        const GLOBAL_AS gv4u *src = (const GLOBAL_AS gv4u *)(as_global(s) + (size_t)(ok ? r : 0) * 64 + sw);
+    for (int w = 0; w < 7; w++) acc[0][w] = acc[1][w] = pj_v4i{0, 0, 0, 0};
 #pragma unroll
-        for (int u = 0; u < 4; u++) rv[u] = ok ? src[u] : gv4u{0, 0, 0, 0};
-        rinv = ok ? as_global(sinv)[r] : 0u;
+    for (int w = 0; w < 4; w++) acc2[w] = pj_v4i{0, 0, 0, 0};
+    // per lane: mod-p sums of the folded accumulators, entries (tile slot, register)
+    uint32_t fin[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+    const int64_t pl = (int64_t)p;
+    auto fold_into = [&](uint32_t &f, const int32_t (&cw)[7], int nw_, const int (&ws)[7]) {
+        uint64_t t = f;
+        for (int k = 0; k < nw_; k++) {
+            int64_t c = (int64_t)cw[k] % pl;
+            c += c < 0 ? pl : 0;
+            uint32_t pw = 1;  // 256^w mod p
+            for (int e = 0; e < ws[k]; e++) pw = mulmod(pw, 256u, p, inv);
+            t += (uint64_t)mulmod((uint32_t)c, pw, p, inv);
+        }
+        f = (uint32_t)(t % p);
     };
-    uint32_t c = blockIdx.x;
-    if (c < nchunks) load(c);
-    for (; c < nchunks; c += gridDim.x) {
-        __syncthreads();  // the previous chunk's products have read the LDS rows
+    auto fold = [&]() {
+        const int all[7] = {0, 1, 2, 3, 4, 5, 6};
+        const int part[7] = {odd ? 1 : 0, odd ? 2 : 3, odd ? 5 : 4, 6, 0, 0, 0};
+        const int npart = odd ? 4 : 3;
 #pragma unroll
-        for (int u = 0; u < 4; u++)
+        for (int sl = 0; sl < 3; sl++) {
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const uint32_t sv = rv[u][e];
-                ls[srow * PJ_PITCH + sw + 4 * u + e] = sv;
-                lo[srow * PJ_PITCH + sw + 4 * u + e] = mulmod(sv, rinv, p, inv);
+            for (int v = 0; v < 4; v++) {
+                int32_t cw[7];
+#pragma unroll
+                for (int w = 0; w < 7; w++) cw[w] = sl < 2 ? acc[sl < 2 ? sl : 0][w][v] : (w < 4 ? acc2[w][v] : 0);
+                fold_into(fin[sl][v], cw, sl < 2 ? 7 : npart, sl < 2 ? all : part);
+                __builtin_amdgcn_sched_barrier(0);
             }
-        if (c + gridDim.x < nchunks) load(c + gridDim.x);  // in flight during the products
-        __syncthreads();
-#pragma unroll 2
-        for (int g = 0; g < PJ_ROWS / 4; g++) {
-            const int r = 4 * g + k;
-            const uint32_t *orow = lo + r * PJ_PITCH, *srw = ls + r * PJ_PITCH;
-            auto tile = [&](int t, int slot) {
-                const uint32_t sv = srw[16 * TJ[t] + i];
-                const double a = (double)orow[16 * TI[t] + i];
-                acc[2 * slot] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, (double)(sv >> 16), acc[2 * slot], 0, 0, 0);
-                acc[2 * slot + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, (double)(sv & 0xFFFFu), acc[2 * slot + 1], 0, 0, 0);
-            };
-            tile(t0, 0);
-            tile(t1, 1);
-            if ((g & 1) == half) tile(t2, 2);
         }
 #pragma unroll
-        for (int t = 0; t < 6; t++)
+        for (int w = 0; w < 7; w++) acc[0][w] = acc[1][w] = pj_v4i{0, 0, 0, 0};
 #pragma unroll
-            for (int v = 0; v < 4; v++) acc[t][v] = fold(acc[t][v], pd, inv);
+        for (int w = 0; w < 4; w++) acc2[w] = pj_v4i{0, 0, 0, 0};
+    };
+    __shared__ uint32_t ivs[2][PJ_ROWS];  // the chunk's row inverses (double buffered)
+    const uint32_t nchunks = (nrows + PJ_ROWS - 1) / PJ_ROWS;
+    const int col = tid & 63, grp = tid >> 6;  // staging: column, group of 16 rows
+    uint32_t sv[16], ivn = 0;
+    auto load = [&](uint32_t c) {
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const uint32_t r = c * PJ_ROWS + grp * 16 + j;
+            sv[j] = r < nrows ? as_global(s)[(size_t)r * 64 + col] : 0u;
+        }
+        const uint32_t r = c * PJ_ROWS + tid;
+        ivn = (tid < PJ_ROWS && r < nrows) ? as_global(sinv)[r] : 0u;
+    };
+    const int32_t half = (int32_t)(p >> 1);
+    uint32_t buf = 0;
+    uint32_t c = blockIdx.x;
+    if (c < nchunks) load(c);
+    for (; c < nchunks; c += gridDim.x, buf ^= 1) {
+        if (tid < PJ_ROWS) ivs[buf][tid] = ivn;
+        __syncthreads();  // the previous chunk's products have read the planes; the inverses are in
+#pragma unroll
+        for (int side = 0; side < 2; side++) {  // o's planes, then s's: a thread's 16 rows of each digit
+            uint32_t d[4][4];                     // [digit][dword of four rows]
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                uint32_t q[4];
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++) {
+                    const int j = 4 * k + jj;
+                    const uint32_t v = side ? sv[j] : mulmod(sv[j], ivs[buf][grp * 16 + j], p, inv);
+                    q[jj] = pj_digits(v, p, half);
+                }
+                pj_transpose(q);
+#pragma unroll
+                for (int a = 0; a < 4; a++) d[a][k] = q[a];
+            }
+            pj_v4i *pl = side ? SL : OL;
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+                pl[(a * 64 + col) * PJ_CS + grp] = pj_v4i{(int)d[a][0], (int)d[a][1], (int)d[a][2], (int)d[a][3]};
+            __builtin_amdgcn_sched_barrier(0);  // one side's temporaries live at a time
+        }
+        if (c + gridDim.x < nchunks) load(c + gridDim.x);  // in flight during the products
+        __syncthreads();
+        auto tile = [&](int t, pj_v4i (&A)[4], pj_v4i (&B)[4]) {
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                A[a] = OL[(a * 64 + 16 * TI[t] + ti) * PJ_CS + tg];
+                B[a] = SL[(a * 64 + 16 * TJ[t] + ti) * PJ_CS + tg];
+            }
+        };
+        {
+            pj_v4i A[4], B[4];
+            tile(t0, A, B);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int bb = 0; bb < 4; bb++)
+                    acc[0][a + bb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[a], B[bb], acc[0][a + bb], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);  // one tile's operands live at a time
+            tile(t1, A, B);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int bb = 0; bb < 4; bb++)
+                    acc[1][a + bb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[a], B[bb], acc[1][a + bb], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            tile(t2, A, B);
+            if (odd) {  // weights 1, 2, 5, 6 -> acc2[0..3]
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+#pragma unroll
+                    for (int bb = 0; bb < 4; bb++) {
+                        const int w = a + bb;
+                        if (w == 1 || w == 2 || w == 5 || w == 6) {
+                            const int k = w == 1 ? 0 : w == 2 ? 1 : w == 5 ? 2 : 3;
+                            acc2[k] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[a], B[bb], acc2[k], 0, 0, 0);
+                        }
+                    }
+            } else {  // weights 0, 3, 4 -> acc2[0..2]
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+#pragma unroll
+                    for (int bb = 0; bb < 4; bb++) {
+                        const int w = a + bb;
+                        if (w == 0 || w == 3 || w == 4) {
+                            const int k = w == 0 ? 0 : w == 3 ? 1 : 2;
+                            acc2[k] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[a], B[bb], acc2[k], 0, 0, 0);
+                        }
+                    }
+            }
+        }
     }
+    fold();
     // the wavefronts' triangles (zero where a wavefront owns no entry), then their sum
     const uint32_t Pb = b * (b + 1) / 2;
+    uint32_t *tri = (uint32_t *)lds;  // 4 x 2080 words fit the 40 KB of planes
     __syncthreads();
-    for (uint32_t t = lane; t < Pb; t += 64) lds[wave * Pb + t] = 0;
+    for (uint32_t t = lane; t < Pb; t += 64) tri[wave * Pb + t] = 0;
     wave_sync();
     const int tl[3] = {t0, t1, t2};
 #pragma unroll
@@ -486,19 +600,19 @@ __global__ __launch_bounds__(WG) void k_proj64(const uint32_t *__restrict__ s, c
         const int t = tl[sl];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const uint32_t ii = 16 * TI[t] + 4 * v + k, jj = 16 * TJ[t] + i;
-            if (ii <= jj && jj < b) {
-                const double h = fold(acc[2 * sl][v], pd, inv), l = fold(acc[2 * sl + 1][v], pd, inv);
-                lds[wave * Pb + ii * b - ii * (ii - 1) / 2 + (jj - ii)] = (uint32_t)fold(fma(h, 65536.0, l), pd, inv);
-            }
+            const uint32_t ii = 16 * TI[t] + 4 * tg + v, jj = 16 * TJ[t] + ti;
+            if (ii <= jj && jj < b) tri[wave * Pb + ii * b - ii * (ii - 1) / 2 + (jj - ii)] = fin[sl][v];
         }
     }
     __syncthreads();
     for (uint32_t t = tid; t < Pb; t += WG) {
         uint32_t sum = 0;
-        for (int w = 0; w < WG / 64; w++) sum = addmod(sum, lds[w * Pb + t], p);
+        for (int w = 0; w < WG / 64; w++) sum = addmod(sum, tri[w * Pb + t], p);
         as_global(gram)[(size_t)blockIdx.x * Pb + t] = sum;
     }
+    // the pass's partial slots beyond this grid hold nothing
+    for (uint32_t x = blockIdx.x + gridDim.x; x < nparts; x += gridDim.x)
+        for (uint32_t t = tid; t < Pb; t += WG) as_global(gram)[(size_t)x * Pb + t] = 0u;
 }
 
 // out[i] = in[i]^-1 mod p (entries in [1, p), p prime), by Fermat: in^(p-2).
@@ -567,6 +681,7 @@ struct rr_krylov {
     // lane width 64: the projection runs in k_proj64 over the produced rows,
     // which needs the inverses of their scales (D_r^-1, D_c^-2)
     bool sep_proj = false;
+    uint32_t pgrid = 0;  // k_proj64 workgroups: one resident wave (<= grid)
     rr::DevBuf drinv, dc2inv;
     rr::DevBuf jobs;  // [2] pass descriptors (by A, by A^T) and the one-job workgroup table {0, grid}
     uint64_t seq_cap = 0;  // entries the device sequence buffer holds
@@ -1081,6 +1196,12 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
     if ((s = upload(c->dc2, dc2.data(), n)) != RR_OK) return s;
     if ((s = upload(c->dcinv, dcinv.data(), n)) != RR_OK) return s;
     if (c->sep_proj) {
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_proj64, WG, 0) != hipSuccess || occ < 1) occ = 1;
+        c->pgrid = std::min<uint32_t>(c->grid, (uint32_t)(cus * occ));
+        // k_proj64's int32 accumulators hold 2^15 rows per workgroup
+        RR_CHECK((std::max(m, n) + 64ull * c->pgrid - 1) / (64ull * c->pgrid) * 64 <= PJ_FOLD_ROWS, RR_EINVAL,
+                 "%llu rows exceed the projection's %u workgroups", (unsigned long long)std::max(m, n), c->pgrid);
         if ((s = c->drinv.alloc((size_t)m * 4)) != RR_OK) return s;
         if ((s = c->dc2inv.alloc((size_t)n * 4)) != RR_OK) return s;
         hipLaunchKernelGGL(k_inv_mod, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->stream,
@@ -1202,8 +1323,8 @@ rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
                         RR_HIP(hipEventCreate(&g0));
                         RR_HIP(hipEventCreate(&g1));
                     }
-                    hipExtLaunchKernelGGL(k_proj64, dim3(c->grid), dim3(WG), 0, c->stream, g0, g1, 0, rows, sinv, nr,
-                                          gram, c->b, c->p, c->mp.pd, c->mp.inv);
+                    hipExtLaunchKernelGGL(k_proj64, dim3(c->pgrid), dim3(WG), 0, c->stream, g0, g1, 0, rows, sinv,
+                                          nr, gram, c->grid, c->b, c->p, c->mp.pd, c->mp.inv);
                     if (c->profiling) proj_ev.push_back({g0, g1});
                 }
                 k = byA ? 1 : 2;
