@@ -146,3 +146,35 @@ def test_resume_from_wdm_checkpoint(tmp_path):
         again = K2.run(todo)
     assert np.array_equal(again, rest)
     assert np.array_equal(np.vstack([first, again]), O.krylov(A, P, b, dr, dc, V, 0, 41))
+
+
+@pytest.mark.timeout(600)
+def test_graph_complex_full_sequence_digest_and_rank():
+    """The whole 11258-entry sequence of the 200000 x 180000, b = 32 workload
+    (every split-row hand-off of every A^T pass included) against the
+    oracle's: SHA-256 of the t-major array and of its prefixes; the device
+    PM-basis of order 11257 against the oracle's degree shifts and rank; and
+    rr_rank against that rank (tests/golden/graph_complex_digest.json, written
+    by tests/golden/make_graph_complex_digest.py)."""
+    import hashlib
+    import json
+    import os
+    eng = _engine()
+    g = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                    "graph_complex_digest.json")))
+    A = synth.graph_complex()
+    b = 32
+    dr, dc, V = synth.preconditioners(2, A.m, A.n, b, P)
+    L = synth.seq_len_plain(min(A.m, A.n), b)
+    assert L == g["seq_len"]
+    with eng.Krylov(A, P, b, dr, dc, V) as K:
+        seq = K.run(L)
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a, dtype="<u4").tobytes()).hexdigest()
+    for t, h in g["sha256_prefix"].items():
+        assert sha(seq[: int(t)]) == h, t
+    assert sha(seq) == g["sha256_seq"]
+    basis, delta = eng.pm_basis(seq, b, L - 1, P)
+    assert int(delta.sum()) == g["delta_sum"]
+    assert eng.rank_from_basis(basis, delta, b, P) == g["rank"]
+    del basis
+    assert eng.rank(A, P, b, dr, dc, V, L, L - 1)[0] == g["rank"]
