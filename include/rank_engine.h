@@ -117,8 +117,10 @@ void rr_krylov_destroy(rr_krylov *ctx);
  * exactly mod p.  Unlike the reference, d is not rounded up to a power of two:
  * any 1 <= d < L is accepted (for a power of two the result is the
  * reference's).  basis_out: [2b][2b][d+1] (coefficients innermost);
- * delta_out: 2b degree shifts.  For p = 2013265921 products use NTTs (d + 1
- * <= 2^14); any other prime p < 2^31 uses direct products.
+ * delta_out: 2b degree shifts.  d + 1 <= 2^20.  Short products are direct;
+ * longer ones use NTTs, mod p itself for p = 2013265921, and for any other
+ * prime p < 2^31 mod the three primes 2013265921, 469762049 and 754974721,
+ * recombined exactly (Garner) and reduced mod p.
  */
 rr_status rr_pm_basis(const uint32_t *seq, uint32_t b, uint64_t L, uint64_t d, uint32_t p, uint32_t *basis_out,
                       int64_t *delta_out);
@@ -158,8 +160,11 @@ rr_status rr_ntt_noalloc(uint32_t *data, uint32_t logn, uint32_t count, int inve
 /*
  * Replaces poly_mat_mul_red_adaptive / poly_mat_mul_fft_red
  * (src/poly_mat_mul.rs:89-101, :240-256): C [rows][cols][hi-lo] = coefficients
- * [lo, hi) of A [rows][inner][la] * B [inner][cols][lb], exact mod p
- * (NTT mod p for p = 2013265921, direct products otherwise).  inner <= 128.
+ * [lo, hi) of A [rows][inner][la] * B [inner][cols][lb], exact mod p for a
+ * prime 2 < p < 2^31 and entries < p (RR_EINVAL otherwise).  Direct products
+ * when short, else NTTs: mod p for p = 2013265921, mod three NTT primes with
+ * Garner recombination for any other p.  inner <= 128, hi <= la + lb - 1,
+ * transforms up to 2^20 points.
  */
 rr_status rr_poly_mat_mul(const uint32_t *A, uint64_t la, const uint32_t *B, uint64_t lb, uint32_t rows,
                           uint32_t inner, uint32_t cols, uint64_t lo, uint64_t hi, uint32_t p, uint32_t *C);

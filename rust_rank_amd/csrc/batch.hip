@@ -194,7 +194,7 @@ rr_status Request::init(const rr_csr *A, uint32_t p_, uint32_t b_, const uint32_
     RR_CHECK(m >= 1 && n >= 1 && m < (1ull << 31) && n < (1ull << 31), RR_EINVAL, "matrix size out of range");
     RR_CHECK(L >= 2 && d >= 1 && d < L, RR_EINVAL, "need 1 <= d < L (L=%llu d=%llu)", (unsigned long long)L,
              (unsigned long long)d);
-    RR_CHECK(p != NTT_PRIME || d + 1 <= (1ull << NTT_MAX_LOGN), RR_EINVAL, "order above the transform limit");
+    RR_CHECK(d + 1 <= (1ull << NTT_MAX_LOGN), RR_EINVAL, "order above the transform limit");
     RR_CHECK(A->row_ptr && A->row_ptr[0] == 0, RR_EINVAL, "row_ptr[0] must be 0");
     for (uint64_t r = 0; r < m; r++) RR_CHECK(A->row_ptr[r + 1] >= A->row_ptr[r], RR_EINVAL, "row_ptr not monotone");
     nnz = A->row_ptr[m];

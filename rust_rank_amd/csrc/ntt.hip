@@ -332,16 +332,82 @@ __global__ __launch_bounds__(256) void k_pointwise(PwPack pk, uint32_t ndesc, ui
             }
 }
 
+// dst[i] = src[(i / len) * sstride + soff + i % len] mod q (residues < 2^32 in, < q out)
+__global__ void k_mod_copy(const uint32_t *src, uint64_t sstride, uint32_t soff, uint32_t len, uint64_t n, uint32_t q,
+                           double qd, double qinv, uint32_t *dst) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t poly = i / len, k = i - poly * len;
+        dst[i] = (uint32_t)fold((double)src[poly * sstride + soff + k], qd, qinv);
+    }
+}
+
+// Garner: x = v0 + q0 v1 + q0 q1 v2 with v0 = r0, v1 = (r1 - v0) q0^-1 mod q1,
+// v2 = (r2 - v0 - q0 v1) (q0 q1)^-1 mod q2; then x mod p from the digits.
+struct Garner3 {
+    uint32_t q0, q1, q2, inv01, inv012, q0m2, q01p, q0p, p;
+    double i1, i2, ip;
+};
+__global__ void k_garner3(const uint32_t *r0, const uint32_t *r1, const uint32_t *r2, uint64_t n, Garner3 g,
+                          uint32_t *C) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t v0 = r0[i];
+        const uint32_t v0m1 = v0 % g.q1, v0m2 = v0 % g.q2;
+        const uint32_t v1 = mulmod(submod(r1[i], v0m1, g.q1), g.inv01, g.q1, g.i1);
+        const uint32_t t = submod(submod(r2[i], v0m2, g.q2), mulmod(g.q0m2, v1 % g.q2, g.q2, g.i2), g.q2);
+        const uint32_t v2 = mulmod(t, g.inv012, g.q2, g.i2);
+        uint32_t x = v0 % g.p;
+        x = addmod(x, mulmod(g.q0p, v1 % g.p, g.p, g.ip), g.p);
+        x = addmod(x, mulmod(g.q01p, v2 % g.p, g.p, g.ip), g.p);
+        C[i] = x;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 
-rr_status NttTables::init(uint32_t logn_max, hipStream_t stream) {
-    const uint32_t p = NTT_PRIME;
+rr_status mod_copy(const uint32_t *src, uint64_t sstride, uint32_t soff, uint32_t len, uint32_t count, uint32_t q,
+                   uint32_t *dst, hipStream_t st) {
+    const uint64_t n = (uint64_t)len * count;
+    if (!n) return RR_OK;
+    const ModP mq = ModP::make(q);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_mod_copy, dim3(grid), dim3(256), 0, st, src, sstride, soff, len, n, q, mq.pd, mq.inv, dst);
+    RR_HIP(hipGetLastError());
+    return RR_OK;
+}
+
+rr_status garner3(const uint32_t *r0, const uint32_t *r1, const uint32_t *r2, uint64_t n, uint32_t p, uint32_t *C,
+                  hipStream_t st) {
+    if (!n) return RR_OK;
+    Garner3 g;
+    g.q0 = CRT_PRIMES[0];
+    g.q1 = CRT_PRIMES[1];
+    g.q2 = CRT_PRIMES[2];
+    g.p = p;
+    g.inv01 = h_invmod(g.q0 % g.q1, g.q1);
+    g.inv012 = h_invmod(h_mulmod(g.q0 % g.q2, g.q1 % g.q2, g.q2), g.q2);
+    g.q0m2 = g.q0 % g.q2;
+    g.q0p = g.q0 % p;
+    g.q01p = h_mulmod(g.q0 % p, g.q1 % p, p);
+    g.i1 = ModP::make(g.q1).inv;
+    g.i2 = ModP::make(g.q2).inv;
+    g.ip = ModP::make(p).inv;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_garner3, dim3(grid), dim3(256), 0, st, r0, r1, r2, n, g, C);
+    RR_HIP(hipGetLastError());
+    return RR_OK;
+}
+
+rr_status NttTables::init(uint32_t logn_max, hipStream_t stream, uint32_t prime_, uint32_t root_) {
+    prime = prime_;
+    root = root_;
+    const uint32_t p = prime;
+    RR_CHECK(((p - 1) >> logn_max) << logn_max == p - 1, RR_EINVAL, "2^%u does not divide %u - 1", logn_max, p);
     logn = logn_max;
     uint32_t half = 1u << (logn - 1);
     if (logn == 0) half = 1;
     std::vector<uint32_t> f(half), fq(half), iv(half), ivq(half);
-    uint32_t w = h_powmod(NTT_ROOT, (p - 1) >> logn, p), wi = h_invmod(w, p);
+    uint32_t w = h_powmod(root, (p - 1) >> logn, p), wi = h_invmod(w, p);
     uint64_t a = 1, b = 1;
     for (uint32_t k = 0; k < half; k++) {
         f[k] = (uint32_t)a;
@@ -385,10 +451,10 @@ static rr_status ntt_forward_big(const NttTables &T, const NttFwdPack &pk, uint3
         RR_CHECK((uint64_t)pk.d[i].count * N2 < (1ull << 32), RR_EINVAL, "too many transforms");
         rows.d[i] = NttFwdDesc{pk.d[i].dst, N1, 0, N1, pk.d[i].dst, pk.d[i].count * N2};
     }
-    const ModP mp = ModP::make(NTT_PRIME);
+    const ModP mp = ModP::make(T.prime);
     const size_t lds = (size_t)(N2 + 1) * TC * 4;
     hipLaunchKernelGGL(k_ntt_cols_fwd, dim3(qmax * (N1 / TC), ndesc), dim3(NTT_THREADS), lds, st, pk, logn,
-                       T.tf, T.tfq, T.logn, NTT_PRIME, mp.inv);
+                       T.tf, T.tfq, T.logn, T.prime, mp.inv);
     RR_HIP(hipGetLastError());
     return ntt_forward_pack(T, rows, ndesc, l1, st);
 }
@@ -406,10 +472,10 @@ static rr_status ntt_inverse_big(const NttTables &T, const NttInvPack &pk, uint3
     }
     rr_status s;
     if ((s = ntt_inverse_pack(T, rows, ndesc, l1, st)) != RR_OK) return s;
-    const ModP mp = ModP::make(NTT_PRIME);
+    const ModP mp = ModP::make(T.prime);
     const size_t lds = (size_t)(N2 + 1) * TC * 4;
     hipLaunchKernelGGL(k_ntt_cols_inv, dim3(qmax * (N1 / TC), ndesc), dim3(NTT_THREADS), lds, st, pk, logn,
-                       T.ti, T.tiq, T.logn, NTT_PRIME, mp.inv);
+                       T.ti, T.tiq, T.logn, T.prime, mp.inv);
     RR_HIP(hipGetLastError());
     return RR_OK;
 }
@@ -422,7 +488,7 @@ rr_status ntt_forward_pack(const NttTables &T, const NttFwdPack &pk, uint32_t nd
     uint32_t bx = 0;
     for (uint32_t i = 0; i < ndesc; i++) bx = std::max(bx, blocks_for(pk.d[i].count, logn));
     hipLaunchKernelGGL(k_ntt_fwd, dim3(std::max(bx, 1u), ndesc), dim3(NTT_THREADS), lds_for(logn), st, pk, logn,
-                       T.tf, T.tfq, T.logn, NTT_PRIME);
+                       T.tf, T.tfq, T.logn, T.prime);
     RR_HIP(hipGetLastError());
     return RR_OK;
 }
@@ -435,15 +501,14 @@ rr_status ntt_inverse_pack(const NttTables &T, const NttInvPack &pk, uint32_t nd
     uint32_t bx = 0;
     for (uint32_t i = 0; i < ndesc; i++) bx = std::max(bx, blocks_for(pk.d[i].count, logn));
     hipLaunchKernelGGL(k_ntt_inv, dim3(std::max(bx, 1u), ndesc), dim3(NTT_THREADS), lds_for(logn), st, pk, logn,
-                       T.ti, T.tiq, T.logn, NTT_PRIME);
+                       T.ti, T.tiq, T.logn, T.prime);
     RR_HIP(hipGetLastError());
     return RR_OK;
 }
 
-rr_status pointwise_pack(const PwPack &pk, uint32_t ndesc, uint32_t N, hipStream_t st) {
+rr_status pointwise_pack(const PwPack &pk, uint32_t ndesc, uint32_t N, hipStream_t st, uint32_t p) {
     RR_CHECK(ndesc >= 1 && ndesc <= NTT_PACK, RR_EINVAL, "bad pack size");
     RR_CHECK(N >= 1 && (N & (N - 1)) == 0, RR_EINVAL, "transform length %u is not a power of two", N);
-    const uint32_t p = NTT_PRIME;
     ModP mp = ModP::make(p);
     constexpr int TI = 4, TJ = 4;
     uint32_t rows = 0, cols = 0, logn = 0;

@@ -14,11 +14,26 @@ constexpr uint32_t NTT_MAX_LOGN = 20;        // larger transforms: four steps th
 // with their Shoup companions.
 struct NttTables {
     uint32_t logn = 0;
+    uint32_t prime = NTT_PRIME, root = NTT_ROOT;  // the transforms run mod prime
     DevBuf fwd, fwdq, inv, invq;  // owned tables (init)
     // the tables the transforms read: the owned ones, or tables laid out by the caller
     const uint32_t *tf = nullptr, *tfq = nullptr, *ti = nullptr, *tiq = nullptr;
-    rr_status init(uint32_t logn_max, hipStream_t stream);
+    rr_status init(uint32_t logn_max, hipStream_t stream, uint32_t prime = NTT_PRIME, uint32_t root = NTT_ROOT);
 };
+
+// Products mod a prime p that is not an NTT prime: three NTT primes with
+// 2-adicity >= 24 whose product (about 2^89.2) exceeds every exact
+// coefficient of a product of residues < 2^31 with at most 128 x 2^20 terms
+// (below 2^89); each coefficient is recovered by Garner's method and reduced
+// mod p.  CRT_PRIMES[0] is NTT_PRIME.
+constexpr uint32_t CRT_PRIMES[3] = {2013265921u, 469762049u, 754974721u};  // 15 2^27 + 1, 7 2^26 + 1, 45 2^24 + 1
+constexpr uint32_t CRT_ROOTS[3] = {31u, 3u, 11u};                        // their generators
+// dst[i] = src[(i / len) * sstride + soff + i % len] mod q, i < count * len
+rr_status mod_copy(const uint32_t *src, uint64_t sstride, uint32_t soff, uint32_t len, uint32_t count, uint32_t q,
+                   uint32_t *dst, hipStream_t st);
+// C[i] = x mod p for the x < q0 q1 q2 with x = r_k[i] mod q_k (Garner), i < n
+rr_status garner3(const uint32_t *r0, const uint32_t *r1, const uint32_t *r2, uint64_t n, uint32_t p, uint32_t *C,
+                  hipStream_t st);
 
 // Descriptor packs: one launch runs up to NTT_PACK independent transforms or
 // pointwise products of the same size (blockIdx selects the descriptor).
@@ -52,7 +67,7 @@ struct PwPack {
 };
 rr_status ntt_forward_pack(const NttTables &T, const NttFwdPack &pk, uint32_t ndesc, uint32_t logn, hipStream_t st);
 rr_status ntt_inverse_pack(const NttTables &T, const NttInvPack &pk, uint32_t ndesc, uint32_t logn, hipStream_t st);
-rr_status pointwise_pack(const PwPack &pk, uint32_t ndesc, uint32_t N, hipStream_t st);
+rr_status pointwise_pack(const PwPack &pk, uint32_t ndesc, uint32_t N, hipStream_t st, uint32_t p = NTT_PRIME);
 
 // dst[q][0..2^logn) = forward spectrum (bit-reversed order) of src[q * sstride + soff + k], k < len.
 rr_status ntt_forward(const NttTables &T, const uint32_t *src, uint64_t sstride, uint32_t soff, uint32_t len,

@@ -760,11 +760,29 @@ static uint32_t ntt_logn_for(const MulDesc &D) {
 // products (a PM-basis running alone, latency-bound) takes the direct form
 // up to la lb = 256, a round with many (batched, throughput-bound) only up
 // to 16.
+// Mod any other prime a transform product runs three times (one per CRT
+// prime) plus a reduction and a Garner pass, so the direct form goes four
+// times as far.
 constexpr size_t SCHOOL_MANY = 8;
 static bool use_school(const MulDesc &D, uint32_t p, size_t round_products = 1) {
-    if (p != NTT_PRIME) return true;
     uint64_t cost = (uint64_t)std::min(D.la, D.hi) * std::min(D.lb, D.hi);
-    return cost <= (round_products >= SCHOOL_MANY ? 16u : 256u);
+    uint64_t lim = round_products >= SCHOOL_MANY ? 16u : 256u;
+    if (p != NTT_PRIME) lim *= 4;
+    return cost <= lim;
+}
+
+// Work space of one transform product: operand and product spectra; mod
+// another prime also the operands reduced mod a smaller CRT prime and the
+// three residue windows.
+static size_t product_words(const MulDesc &D, uint32_t logn, uint32_t p) {
+    const size_t N = (size_t)1 << logn, a64 = 64;
+    size_t w = ((size_t)D.rows * D.inner * N + a64) + ((size_t)D.inner * D.cols * N + a64) +
+               ((size_t)D.rows * D.cols * N + a64);
+    if (p == NTT_PRIME) return w;
+    const size_t la = std::min<size_t>(D.la, N), lb = std::min<size_t>(D.lb, N);
+    w += ((size_t)D.rows * D.inner * la + a64) + ((size_t)D.inner * D.cols * lb + a64);
+    w += 3 * ((size_t)D.rows * D.cols * (D.hi - D.lo) + a64);
+    return w;
 }
 
 rr_status PmScheduler::ensure_scratch(size_t words) {
@@ -783,8 +801,7 @@ rr_status PmScheduler::init(uint32_t p_, hipStream_t st_, uint32_t max_logn, uin
     p = p_;
     st = st_;
     rr_status s;
-    if (p == NTT_PRIME && (s = tables.init(std::max<uint32_t>(2, std::min(max_logn, NTT_MAX_LOGN)), st)) != RR_OK)
-        return s;
+    if ((s = init_tables(std::max<uint32_t>(2, std::min(max_logn, NTT_MAX_LOGN)))) != RR_OK) return s;
     // any mix of block widths: the largest leaf (b = 32: m = 64, K = 4) needs 150 KB
     lds = 0;
     for (uint32_t mm = 2; mm <= 128; mm += 2) {
@@ -799,6 +816,14 @@ rr_status PmScheduler::init(uint32_t p_, hipStream_t st_, uint32_t max_logn, uin
     RR_CHECK(lds <= 160 * 1024, RR_EINVAL, "leaf needs %zu bytes of LDS", lds);
     RR_HIP(hipFuncSetAttribute((const void *)k_mbasis_iter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     return RR_OK;
+}
+
+rr_status PmScheduler::init_tables(uint32_t logn) {
+    rr_status s;
+    if (p == NTT_PRIME) return tables.init(logn, st);
+    if ((s = tables.init(logn, st, CRT_PRIMES[0], CRT_ROOTS[0])) != RR_OK) return s;
+    if ((s = crt_tabs[0].init(logn, st, CRT_PRIMES[1], CRT_ROOTS[1])) != RR_OK) return s;
+    return crt_tabs[1].init(logn, st, CRT_PRIMES[2], CRT_ROOTS[2]);
 }
 
 rr_status PmScheduler::add(PmJob *J) {
@@ -868,11 +893,8 @@ rr_status PmScheduler::round(std::vector<PmJob *> &jobs, bool *launched) {
     if (any_mul) {  // the round's transform work space, before any job's operation is taken
         size_t words = 0;
         for (auto *J : jobs)
-            if (ready(J) && J->op_kind == 1 && !use_school(J->op_mul, p, nmul)) {
-                const MulDesc &D = J->op_mul;
-                words += (size_t)(D.rows * D.inner + D.inner * D.cols + D.rows * D.cols) *
-                             ((size_t)1 << ntt_logn_for(D)) + 192;
-            }
+            if (ready(J) && J->op_kind == 1 && !use_school(J->op_mul, p, nmul))
+                words += product_words(J->op_mul, ntt_logn_for(J->op_mul), p);
         rr_status e;
         if (words && (e = ensure_scratch(words)) != RR_OK) return e;
     }
@@ -887,13 +909,15 @@ rr_status PmScheduler::round(std::vector<PmJob *> &jobs, bool *launched) {
             if (J->op_keep) J->op_keep->sp_logn = -1;  // nothing kept: the later product transforms M1 itself
         } else {
             NttProduct o{ntt_logn_for(J->op_mul), J->op_mul, nullptr, nullptr};
-            if (J->op_keep && o.logn == node_logn(J->op_keep->d)) {
+            if (p != NTT_PRIME) {  // spectra mod three primes: none kept
+                if (J->op_keep) J->op_keep->sp_logn = -1;
+            } else if (J->op_keep && o.logn == node_logn(J->op_keep->d)) {
                 o.keep = J->op_keep->SP;
                 J->op_keep->sp_logn = (int)o.logn;
             } else if (J->op_keep) {
                 J->op_keep->sp_logn = -1;
             }
-            if (J->op_given && J->op_given->sp_logn == (int)o.logn) o.given = J->op_given->SP;
+            if (p == NTT_PRIME && J->op_given && J->op_given->sp_logn == (int)o.logn) o.given = J->op_given->SP;
             ntt.push_back(o);
         }
         J->op_keep = J->op_given = nullptr;
@@ -950,13 +974,9 @@ rr_status PmScheduler::run(std::vector<PmJob *> &jobs) {
 // Each NTT product: forward transforms of both operands into round scratch,
 // pointwise products (with 1/N), inverse transforms into the output window.
 rr_status PmScheduler::run_ntt_products(std::vector<NttProduct> &ops) {
+    if (p != NTT_PRIME) return run_crt_products(ops);
     size_t words = 0;
-    for (auto &o : ops) {
-        size_t N = (size_t)1 << o.logn;
-        const MulDesc &D = o.D;
-        words += (size_t)((o.keep ? 0 : D.rows * D.inner) + (o.given ? 0 : D.inner * D.cols) + D.rows * D.cols) * N +
-                 192;
-    }
+    for (auto &o : ops) words += product_words(o.D, o.logn, p);
     rr_status s;
     if ((s = ensure_scratch(words)) != RR_OK) return s;
     // same-size products share their launches (forward, pointwise, inverse)
@@ -998,6 +1018,100 @@ rr_status PmScheduler::run_ntt_products(std::vector<NttProduct> &ops) {
         if ((s = pointwise_pack(pp, np, N, st)) != RR_OK) return s;
         if ((s = ntt_inverse_pack(tables, ip, np, logn, st)) != RR_OK) return s;
         launches += 3;
+        g0 = g1;
+    }
+    return RR_OK;
+}
+
+// Products mod a prime p other than the NTT prime: each product runs mod the
+// three CRT primes (operands reduced first where p exceeds the prime: the
+// butterflies need residues), and Garner's method takes the three residue
+// windows to the exact coefficients, which are reduced mod p.  The exact
+// coefficient in the window [lo, hi) sums at most inner min(la, lb, N)
+// products below p^2 (wrapped terms fall below lo, operand coefficients at
+// or beyond N only reach outputs at or beyond N >= hi): below the product of
+// the primes for inner <= 128 and N <= 2^20.
+rr_status PmScheduler::run_crt_products(std::vector<NttProduct> &ops) {
+    const unsigned __int128 Q = (unsigned __int128)CRT_PRIMES[0] * CRT_PRIMES[1] * CRT_PRIMES[2];
+    size_t words = 0;
+    for (auto &o : ops) {
+        const MulDesc &D = o.D;
+        const uint64_t terms = std::min<uint64_t>(std::min(D.la, D.lb), 1ull << o.logn) * D.inner;
+        RR_CHECK((unsigned __int128)terms * (p - 1) * (p - 1) < Q, RR_EINVAL,
+                 "product of %u x %u entries exceeds the three-prime range", D.inner, std::min(D.la, D.lb));
+        words += product_words(D, o.logn, p);
+    }
+    rr_status s;
+    if ((s = ensure_scratch(words)) != RR_OK) return s;
+    std::stable_sort(ops.begin(), ops.end(),
+                     [](const NttProduct &x, const NttProduct &y) { return x.logn < y.logn; });
+    uint32_t *base = scratch.as<uint32_t>();
+    size_t off = 0;
+    auto take = [&](size_t w) {
+        uint32_t *r = base + off;
+        off += (w + 63) & ~size_t(63);
+        return r;
+    };
+    struct Win {
+        uint32_t *fa, *fb, *fc, *ra, *rb, *res[3];
+        uint32_t la, lb;
+    };
+    std::vector<Win> win(ops.size());
+    for (size_t g0 = 0; g0 < ops.size();) {
+        const uint32_t logn = ops[g0].logn, N = 1u << logn;
+        RR_CHECK(logn <= tables.logn && logn <= crt_tabs[0].logn && logn <= crt_tabs[1].logn, RR_EINVAL,
+                 "transform 2^%u above table 2^%u", logn, tables.logn);
+        size_t g1 = g0;
+        while (g1 < ops.size() && ops[g1].logn == logn && g1 - g0 < NTT_PACK / 2) g1++;
+        for (size_t k = g0; k < g1; k++) {
+            const MulDesc &D = ops[k].D;
+            Win &W = win[k];
+            W.la = std::min(D.la, N);
+            W.lb = std::min(D.lb, N);
+            W.fa = take((size_t)D.rows * D.inner * N);
+            W.fb = take((size_t)D.inner * D.cols * N);
+            W.fc = take((size_t)D.rows * D.cols * N);
+            W.ra = take((size_t)D.rows * D.inner * W.la);
+            W.rb = take((size_t)D.inner * D.cols * W.lb);
+            for (auto &r : W.res) r = take((size_t)D.rows * D.cols * (D.hi - D.lo));
+        }
+        for (int t = 0; t < 3; t++) {
+            const NttTables &T = t == 0 ? tables : crt_tabs[t - 1];
+            const uint32_t q = T.prime;
+            NttFwdPack fp;
+            PwPack pp;
+            NttInvPack ip;
+            uint32_t nf = 0, np = 0;
+            for (size_t k = g0; k < g1; k++) {
+                const MulDesc &D = ops[k].D;
+                const Win &W = win[k];
+                if (p > q) {  // operands reduced mod q, compact
+                    if ((s = mod_copy(D.A, D.la, 0, W.la, D.rows * D.inner, q, W.ra, st)) != RR_OK) return s;
+                    if ((s = mod_copy(D.B, D.sb, D.ob, W.lb, D.inner * D.cols, q, W.rb, st)) != RR_OK) return s;
+                    fp.d[nf++] = NttFwdDesc{W.ra, W.la, 0, W.la, W.fa, D.rows * D.inner};
+                    fp.d[nf++] = NttFwdDesc{W.rb, W.lb, 0, W.lb, W.fb, D.inner * D.cols};
+                    launches += 2;
+                } else {
+                    fp.d[nf++] = NttFwdDesc{D.A, D.la, 0, W.la, W.fa, D.rows * D.inner};
+                    fp.d[nf++] = NttFwdDesc{D.B, D.sb, D.ob, W.lb, W.fb, D.inner * D.cols};
+                }
+                pp.d[np] = PwDesc{W.fa, W.fb, W.fc, D.rows, D.inner, D.cols, h_invmod(N % q, q)};
+                ip.d[np] = NttInvDesc{W.fc, D.rows * D.cols, D.lo, D.hi - D.lo, W.res[t], D.hi - D.lo};
+                np++;
+            }
+            if ((s = ntt_forward_pack(T, fp, nf, logn, st)) != RR_OK) return s;
+            if ((s = pointwise_pack(pp, np, N, st, q)) != RR_OK) return s;
+            if ((s = ntt_inverse_pack(T, ip, np, logn, st)) != RR_OK) return s;
+            launches += 3;
+        }
+        for (size_t k = g0; k < g1; k++) {
+            const MulDesc &D = ops[k].D;
+            const Win &W = win[k];
+            if ((s = garner3(W.res[0], W.res[1], W.res[2], (uint64_t)D.rows * D.cols * (D.hi - D.lo), p, D.C, st)) !=
+                RR_OK)
+                return s;
+            launches++;
+        }
         g0 = g1;
     }
     return RR_OK;
@@ -1145,8 +1259,8 @@ rr_status rr_pm_basis(const uint32_t *seq, uint32_t b, uint64_t L, uint64_t d, u
     RR_CHECK(p > 2 && p < (1u << 31) && h_is_prime(p), RR_EINVAL, "p=%u must be a prime in (2, 2^31)", p);
     RR_CHECK(d >= 1 && d < L, RR_EINVAL, "order d=%llu must satisfy 1 <= d < L=%llu", (unsigned long long)d,
              (unsigned long long)L);
-    RR_CHECK(p != NTT_PRIME || d + 1 <= (1ull << NTT_MAX_LOGN), RR_EINVAL,
-             "order d=%llu above the 2^%u-point transform limit", (unsigned long long)d, NTT_MAX_LOGN);
+    RR_CHECK(d + 1 <= (1ull << NTT_MAX_LOGN), RR_EINVAL, "order d=%llu above the 2^%u-point transform limit",
+             (unsigned long long)d, NTT_MAX_LOGN);
     const uint32_t P = b * (b + 1) / 2, m = 2 * b;
     const uint64_t len = d + 1;
     hipStream_t st;
@@ -1241,6 +1355,7 @@ rr_status rr_poly_mat_mul(const uint32_t *A, uint64_t la, const uint32_t *B, uin
     RR_CHECK(A && B && C && la >= 1 && lb >= 1 && hi > lo && rows && inner && cols, RR_EINVAL,
              "bad product arguments");
     RR_CHECK(inner <= 128 && hi <= la + lb - 1 + 0ull, RR_EINVAL, "inner <= 128 and hi <= la + lb - 1");
+    RR_CHECK(p > 2 && p < (1u << 31) && h_is_prime(p), RR_EINVAL, "p=%u must be a prime in (2, 2^31)", p);
     hipStream_t st;
     RR_HIP(rr::stream_create(&st));
     struct S {
@@ -1253,6 +1368,8 @@ rr_status rr_poly_mat_mul(const uint32_t *A, uint64_t la, const uint32_t *B, uin
     if ((s = dA.alloc(sa * 4)) != RR_OK) return s;
     if ((s = dB.alloc(sbn * 4)) != RR_OK) return s;
     if ((s = dC.alloc(sc * 4)) != RR_OK) return s;
+    for (size_t i = 0; i < sa; i++) RR_CHECK(A[i] < p, RR_EINVAL, "A entry %zu >= p", i);
+    for (size_t i = 0; i < sbn; i++) RR_CHECK(B[i] < p, RR_EINVAL, "B entry %zu >= p", i);
     RR_HIP(hipMemcpyAsync(dA.ptr, A, sa * 4, hipMemcpyHostToDevice, st));
     RR_HIP(hipMemcpyAsync(dB.ptr, B, sbn * 4, hipMemcpyHostToDevice, st));
     MulDesc D{dA.as<uint32_t>(), dB.as<uint32_t>(), dC.as<uint32_t>(), lb, (uint32_t)la, (uint32_t)lb, 0,
@@ -1265,7 +1382,7 @@ rr_status rr_poly_mat_mul(const uint32_t *A, uint64_t la, const uint32_t *B, uin
     } else {
         uint32_t logn = ntt_logn_for(D);
         RR_CHECK(logn <= NTT_MAX_LOGN, RR_EINVAL, "product too long");
-        if ((s = S.tables.init(logn, st)) != RR_OK) return s;
+        if ((s = S.init_tables(logn)) != RR_OK) return s;
         std::vector<NttProduct> ops{{logn, D, nullptr, nullptr}};
         if ((s = S.run_ntt_products(ops)) != RR_OK) return s;
     }

@@ -98,7 +98,8 @@ struct PmJob {
 struct PmScheduler {
     uint32_t p = 0;
     hipStream_t st = nullptr;
-    NttTables tables;
+    NttTables tables;      // mod p when p is the NTT prime, else mod the first CRT prime
+    NttTables crt_tabs[2];  // mod the second and third CRT primes (p not the NTT prime)
     DevBuf scratch;
     size_t scratch_words = 0, lds = 0;
     uint64_t rounds = 0, launches = 0;
@@ -117,7 +118,9 @@ struct PmScheduler {
     rr_status add(PmJob *J);                                          // job joins; its delta is zeroed
     rr_status round(std::vector<PmJob *> &jobs, bool *launched);     // next products, else next leaves
     rr_status run(std::vector<PmJob *> &jobs);                        // init + add + rounds to completion
+    rr_status init_tables(uint32_t logn);  // the transform tables p's products need, up to 2^logn points
     rr_status run_ntt_products(std::vector<NttProduct> &ops);
+    rr_status run_crt_products(std::vector<NttProduct> &ops);  // p not the NTT prime: three primes + Garner
 };
 
 rr_status pm_basis_batch(uint32_t p, std::vector<PmJob *> &jobs, hipStream_t st, uint64_t *rounds,

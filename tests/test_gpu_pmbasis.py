@@ -108,6 +108,75 @@ def _left_combine(u, M, p):
     return ((hi % np.uint64(p)) * np.uint64(65536) + lo) % np.uint64(p)
 
 
+# Primes other than the NTT prime: transform products run mod three NTT primes
+# (2013265921, 469762049, 754974721) and are recombined by Garner's method;
+# above a transform prime the operands are reduced first (2^31 - 1 is above
+# all three, 10^9 + 7 above the two smaller ones, 3323 below all).
+OTHER_P = [2147483647, 1000000007, 3323]
+
+
+@pytest.mark.parametrize("p", OTHER_P)
+@pytest.mark.parametrize("shape,lo,hi", [
+    ((8, 8, 4, 40, 70), 20, 70),          # middle window
+    ((5, 7, 21, 40, 70), 0, 109),         # column tiles with a ragged last tile
+    ((2, 2, 3, 4000, 4000), 0, 7999),     # one-workgroup transforms (2^13 points)
+    ((2, 2, 2, 9000, 9000), 0, 17999),    # four-step transforms (2^15 points)
+    ((2, 3, 2, 3000, 9000), 3000, 9000),  # middle window, operands of unequal length
+])
+def test_poly_mat_mul_other_primes_match_oracle(p, shape, lo, hi):
+    eng = _eng()
+    rows, inner, cols, la, lb = shape
+    print(f"pmm p={p} {shape} [{lo},{hi})", flush=True)
+    rng = np.random.default_rng(sum(shape) + p % 1000)
+    A = rng.integers(0, p, (rows, inner, la)).astype(np.uint32)
+    B = rng.integers(0, p, (inner, cols, lb)).astype(np.uint32)
+    got = eng.poly_mat_mul(A, B, lo, hi, p)
+    want = O.poly_mat_mul(A, B, lo, hi, p)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.timeout(300)
+def test_poly_mat_mul_other_prime_largest_coefficients():
+    """The largest exact coefficients the three primes have to hold: inner 128,
+    2^19 coefficients per operand, every entry p - 1 with p = 2^31 - 1, so
+    coefficient k is 128 c_k (p - 1)^2 (about 2^88) with c_k the number of
+    index pairs summing to k; mod p that is 128 c_k, since (p - 1)^2 = 1."""
+    eng = _eng()
+    p, inner, la = 2147483647, 128, 1 << 19
+    print(f"pmm p={p} inner={inner} la=lb={la} all entries p-1", flush=True)
+    A = np.full((1, inner, la), p - 1, np.uint32)
+    got = eng.poly_mat_mul(A, A.reshape(inner, 1, la), 0, 2 * la - 1, p)
+    k = np.arange(2 * la - 1, dtype=np.int64)
+    want = (inner * (np.minimum(k, 2 * la - 2 - k) + 1)) % p
+    assert np.array_equal(got[0, 0].astype(np.int64), want)
+
+
+@pytest.mark.parametrize("p,b,d", [(2147483647, 2, 700), (2147483647, 8, 160), (1000000007, 1, 6000),
+                                   (3323, 4, 500), (65537, 16, 90)])
+def test_pm_basis_other_primes_match_oracle(p, b, d):
+    eng = _eng()
+    print(f"pm_basis p={p} b={b} d={d}", flush=True)
+    rng = np.random.default_rng(b * 1000 + d)
+    seq = rng.integers(0, p, (d + 2, b * (b + 1) // 2)).astype(np.uint32)
+    basis, delta = eng.pm_basis(seq, b, d, p)
+    want_b, want_d = O.pm_basis(seq, b, d, p)
+    assert np.array_equal(delta, want_d)
+    assert np.array_equal(basis, want_b)
+
+
+def test_rank_other_prime_end_to_end():
+    """rr_rank mod 2^31 - 1 at an order whose PM-basis products run through the
+    three primes equals dense elimination."""
+    eng = _eng()
+    p, b = 2147483647, 4
+    A = synth.batch_matrix(31, 0, p=p, rows_lo=1500, rows_hi=1600)
+    dr, dc, V = synth.preconditioners(12, A.m, A.n, b, p)
+    got, ms = eng.rank(A, p, b, dr, dc, V)
+    want = O.dense_rank(A.to_dense(p), p)
+    print(f"m={A.m} n={A.n} p={p} b={b}: rank {got} in {ms:.0f} ms, elimination {want}", flush=True)
+    assert got == want
+
+
 @pytest.mark.timeout(900)
 def test_pm_basis_b64_four_step_invariants():
     """b = 64, d = 2^15 (2b x 2b = 128 x 128 bases; the top products are
