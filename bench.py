@@ -114,7 +114,7 @@ def krylov_leg(engine, A, p, b, dr, dc, V, steps, warmup, prof_steps):
             "stream_ms": st["ms_total"], "prof_steps": prof_steps}, warm
 
 
-def roofline(r, traffic, kernel, copy_gbs):
+def roofline(r, traffic, kernel, copy_gbs, gather=None):
     achieved = r["bytes_pass"] / (r["spmm_ms"] * 1e-3) / 1e9
     out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
@@ -127,6 +127,10 @@ def roofline(r, traffic, kernel, copy_gbs):
         out["projection_kernel"] = {"name": "k_proj64", "avg_launch_us": round(r["proj_ms"] * 1e3, 2)}
     if copy_gbs:
         out["copy_ceiling_gbs"] = round(copy_gbs, 1)
+    if gather:  # the same number of random 256-byte row gathers alone, from vectors of the passes' sizes
+        g_us = sum(ms for ms, _ in gather) / len(gather) * 1e3
+        out["gather_probe"] = {"avg_us": round(g_us, 2), "gbs": round(sum(g for _, g in gather) / len(gather), 1),
+                               "pass_time_over_gather_time": round(r["spmm_ms"] * 1e3 / g_us, 3)}
     return out
 
 
@@ -375,8 +379,10 @@ def main():
     log(f"workload {desc}: nnz={A.nnz} L={L}; generated in {time.perf_counter() - t0:.1f}s")
     r, _ = krylov_leg(engine, A, p, b, dr, dc, V, steps, args.warmup, 40)
     _, copy_gbs = engine.copy_probe(1 << 30, 20)
+    gather = [engine.gather_probe(rows, A.nnz, 5) for rows in (A.n, A.m)]  # by A reads x (n rows), by A^T y (m)
     b_iter = 8 * A.nnz + (4 * b + 2) * (A.m + A.n)
-    roof = roofline(r, traffic_from_profile("traffic_k_spmm_large_single.json"), "k_spmm<64, general>", copy_gbs)
+    roof = roofline(r, traffic_from_profile("traffic_k_spmm_large_single.json"), "k_spmm<64, general>", copy_gbs,
+                    gather)
     cpu = None if args.no_cpu else cpu_baseline(A, p, b, dr, dc, V, 2, args.cpu_timed,
                                                 "5000000x4500000 b=64 workload")
     nnz = A.nnz

@@ -321,6 +321,15 @@ void rr_server_destroy(rr_server *s);
  */
 rr_status rr_copy_probe(uint64_t bytes, uint32_t iters, double *ms_out, double *gbs_out);
 
+/*
+ * Measurement helper (no counterpart in the reference): `gathers` random
+ * 256-byte rows (uniform row indices, read from a 4-byte index array) out
+ * of an array of `rows` rows, `iters` times; average kernel time and GB/s of
+ * gathered rows plus indices.  The achievable rate of the Krylov pass's
+ * vector gathers when its columns are uniform.
+ */
+rr_status rr_gather_probe(uint64_t rows, uint64_t gathers, uint32_t iters, double *ms_out, double *gbs_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -156,11 +156,87 @@ __global__ __launch_bounds__(256) void k_copy(const uint4 *__restrict__ src, uin
         dst[i] = src[i];
 }
 
+// Gather probe: row indices uniform in [0, rows) from a SplitMix64 hash of the position.
+__global__ __launch_bounds__(256) void k_gather_index(uint32_t *idx, uint64_t n, uint64_t rows, uint64_t seed) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        idx[i] = (uint32_t)(z % rows);
+    }
+}
+
+// Each group of 16 lanes reads one 256-byte row (16 x 16 bytes) per index,
+// four indices in flight per group; the XOR of what it read is stored so the
+// loads stay live.
+__global__ __launch_bounds__(256) void k_gather_rows(const uint4 *__restrict__ X, const uint32_t *__restrict__ idx,
+                                                     uint64_t n, uint32_t *out) {
+    const uint32_t lane = threadIdx.x & 63, g = lane >> 4, q = lane & 15;
+    const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6), nwave = (uint64_t)gridDim.x * 4;
+    uint32_t acc = 0;
+    for (uint64_t base = wave * 16; base < n; base += nwave * 16) {
+        uint32_t r[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint64_t i = base + u * 4 + g;
+            r[u] = i < n ? idx[i] : 0u;
+        }
+        uint4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = X[(uint64_t)r[u] * 16 + q];
+#pragma unroll
+        for (int u = 0; u < 4; u++) acc ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+    }
+    out[(uint64_t)blockIdx.x * 256 + threadIdx.x] = acc;
+}
+
 }  // namespace rr
 
 using namespace rr;
 
 extern "C" {
+
+rr_status rr_gather_probe(uint64_t rows, uint64_t gathers, uint32_t iters, double *ms_out, double *gbs_out) {
+    clear_error();
+    RR_CHECK(rows >= 1 && rows <= (1ull << 32) && gathers >= 1 && iters >= 1 && ms_out && gbs_out, RR_EINVAL,
+             "bad gather probe args");
+    int dev = 0, cus = 0;
+    RR_HIP(hipGetDevice(&dev));
+    RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint32_t grid = (uint32_t)cus * 16;
+    DevBuf X, idx, out;
+    rr_status s;
+    if ((s = X.alloc(rows * 256)) != RR_OK) return s;
+    if ((s = idx.alloc(gathers * 4)) != RR_OK) return s;
+    if ((s = out.alloc((size_t)grid * 256 * 4)) != RR_OK) return s;
+    hipStream_t st;
+    RR_HIP(stream_create(&st));
+    struct S {
+        hipStream_t s;
+        ~S() { stream_destroy(s); }
+    } guard{st};
+    RR_HIP(hipMemsetAsync(X.ptr, 1, rows * 256, st));
+    hipLaunchKernelGGL(k_gather_index, dim3(grid), dim3(256), 0, st, idx.as<uint32_t>(), gathers, rows, 12345ull);
+    hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, st, X.as<uint4>(), idx.as<uint32_t>(), gathers,
+                       out.as<uint32_t>());
+    hipEvent_t e0, e1;
+    RR_HIP(hipEventCreate(&e0));
+    RR_HIP(hipEventCreate(&e1));
+    RR_HIP(hipEventRecord(e0, st));
+    for (uint32_t i = 0; i < iters; i++)
+        hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, st, X.as<uint4>(), idx.as<uint32_t>(), gathers,
+                           out.as<uint32_t>());
+    RR_HIP(hipEventRecord(e1, st));
+    RR_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    RR_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *ms_out = ms / iters;
+    *gbs_out = (double)gathers * 260.0 / (*ms_out * 1e-3) / 1e9;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return RR_OK;
+}
 
 rr_status rr_release_device_memory(void) {
     clear_error();
