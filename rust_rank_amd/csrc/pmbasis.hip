@@ -74,7 +74,7 @@ __host__ __device__ constexpr uint32_t leaf_dl_off(uint32_t m, uint32_t n) {
     return (2 * m * n + 5 * m + 2 * n + 2 * LEAF_PBUF + 1) & ~1u;
 }
 
-__device__ LeafLds leaf_lds(uint32_t m, uint32_t n, uint32_t K) {
+__device__ __forceinline__ LeafLds leaf_lds(uint32_t m, uint32_t n, uint32_t K) {
     extern __shared__ __align__(16) uint32_t sm[];
     LeafLds L;
     L.R = sm;
@@ -125,7 +125,7 @@ __device__ __forceinline__ void leaf_advance(const LeafLds &L, uint32_t *X, uint
                                              uint32_t khi, uint32_t olo, uint32_t ohi, uint32_t rank, uint32_t m,
                                              uint32_t n, uint32_t p, double pd, double inv) {
     const uint32_t tid = threadIdx.x, lane = tid & 63;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = blockDim.x >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = LEAF_THREADS / 64;
     const uint32_t nsl = khi - klo + 1, ncc = cols * nsl, ccb = (ncc + 63) / 64;
     const uint32_t items = ccb * ((m + ADV_ROWS - 1) / ADV_ROWS), rstride = cols * stride;
     // lane q holds pivot row q's offset in X (rank <= n <= 64)
@@ -224,7 +224,7 @@ __device__ __forceinline__ uint32_t neg_inv_2_32(uint32_t p) {  // p odd
 template <int NREG, int S>
 __device__ __forceinline__ uint32_t leaf_lsp(const LeafLds &L, uint32_t m, uint32_t n, uint32_t p) {
     const uint32_t tid = threadIdx.x, lane = tid & 63;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = blockDim.x >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = LEAF_THREADS / 64;
     const uint32_t pneg = neg_inv_2_32(p);
     uint32_t x[S][NREG], sg[S];
 #pragma unroll
@@ -324,7 +324,7 @@ __device__ __forceinline__ uint32_t leaf_lsp(const LeafLds &L, uint32_t m, uint3
 
 // Rows per wavefront and registers per row for m rows of n columns (m <= 128, n <= 64).
 __device__ __forceinline__ uint32_t leaf_lsp_any(const LeafLds &L, uint32_t m, uint32_t n, uint32_t p) {
-    const uint32_t nw = blockDim.x >> 6;
+    const uint32_t nw = LEAF_THREADS / 64;
     if (n > 32) return leaf_lsp<2, 128 / 16>(L, m, n, p);
     if (m <= nw) return leaf_lsp<1, 1>(L, m, n, p);
     if (m <= 2 * nw) return leaf_lsp<1, 2>(L, m, n, p);
@@ -375,7 +375,7 @@ __device__ __forceinline__ uint32_t leaf_eliminate(const LeafLds &L, uint32_t m,
 // ranks one row at a time: lane j compares row j with it, one ballot counts.
 __device__ __forceinline__ void leaf_sort(const LeafLds &L, uint32_t m) {
     const uint32_t tid = threadIdx.x, lane = tid & 63;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = blockDim.x >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = LEAF_THREADS / 64;
     const int64_t d0 = L.dl[min(lane, m - 1)], d1 = L.dl[min(64 + lane, m - 1)];
     for (uint32_t i = wave; i < m; i += nw) {
         const int64_t di = L.dl[i];
