@@ -189,18 +189,20 @@ def batched(eng, synth, p, nmat, seed=4, plain=True):
     the same matrices solved one at a time by the plain GPU path (rr_rank)."""
     t0 = time.perf_counter()
     mats = synth.batch_matrices(seed, nmat)
-    bs, pred, probe = eng.autotune_b(mats, p)
+    bs, sus, pred = eng.autotune(mats, p)
+    _, _, probe = eng.autotune_b(mats[:1], p)
     pre = [synth.preconditioners(5000 + i, A.m, A.n, int(b), p) for i, (A, b) in enumerate(zip(mats, bs))]
+    hist = lambda a: {int(k): int(v) for k, v in zip(*np.unique(a, return_counts=True))}
     log(f"batched: {nmat} matrices, rows {min(A.m for A in mats)}..{max(A.m for A in mats)}, "
-        f"b histogram {dict(zip(*np.unique(bs, return_counts=True)))}, setup {time.perf_counter() - t0:.1f}s")
+        f"b histogram {hist(bs)}, su histogram {hist(sus)}, setup {time.perf_counter() - t0:.1f}s")
     t2 = time.perf_counter()
-    ranks, st = eng.batch_rank(mats, p, bs, [x[0] for x in pre], [x[1] for x in pre], [x[2] for x in pre])
+    ranks, st = eng.batch_rank(mats, p, bs, [x[0] for x in pre], [x[1] for x in pre], [x[2] for x in pre], sus=sus)
     t_batch = time.perf_counter() - t2
     out = {"workload": "batched: rows log-uniform in [1000, 50000], cols 0.9 rows, 6-16 nnz/row, +-1/+-2, "
                        "1-5 % planted dependent columns", "matrices": nmat,
            "ranks_per_s": round(nmat / t_batch, 3), "wall_s": round(t_batch, 3), "ticks": st["ticks"],
            "pm_rounds": st["pm_rounds"], "ms_krylov": round(st["ms_krylov"], 1), "ms_pm": round(st["ms_pm"], 1),
-           "b_hist": {int(k): int(v) for k, v in zip(*np.unique(bs, return_counts=True))}, "autotune_probe": probe}
+           "b_hist": hist(bs), "su_hist": hist(sus), "autotune_probe": probe}
     if not plain:  # the batched call alone (for kernel traces)
         log(f"batched {t_batch:.2f}s (plain solves skipped)")
         return out

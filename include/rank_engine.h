@@ -281,6 +281,27 @@ rr_status rr_autotune_b(uint32_t count, const rr_csr *mats, uint32_t p, uint32_t
 rr_status rr_autotune_probe(double *c_ms_per_nnz, double *pm_ms_d255);
 
 /*
+ * The autotuner's full choice per matrix (no counterpart in the reference):
+ * b as rr_autotune_b picks it, and the column block size su (3 or 4 column
+ * indices per row group and block, which sets the rows' padding to an even
+ * number of blocks) from the measured time per padded slot at each size (the
+ * same probe matrix run at su = 3 and su = 4) times this matrix's padded
+ * slots in both operands.  pred_ms: predicted device time for the choice.
+ * Rows per wavefront follow from b (64 / (b / 4) rows for b >= 4); rows
+ * longer than the wavefront's share of the work are split by the slicer.
+ */
+typedef struct {
+    uint32_t b, su;
+    double pred_ms;
+} rr_tuning;
+rr_status rr_autotune(uint32_t count, const rr_csr *mats, uint32_t p, rr_tuning *out);
+/* rr_batch_rank with each matrix's b and su taken from tune[i]. */
+rr_status rr_batch_rank_tuned(uint32_t count, const rr_csr *mats, uint32_t p, const rr_tuning *tune,
+                              const uint32_t *const *row_precond, const uint32_t *const *col_precond,
+                              const uint32_t *const *V, const uint64_t *seq_len, const uint64_t *order,
+                              int64_t *rank_out, rr_batch_stats *stats);
+
+/*
  * Continuous-batching rank server (no counterpart in the reference).  A
  * worker thread admits submitted requests in arrival order while their
  * resident footprint fits hbm_budget_bytes, keeps each request's matrix,

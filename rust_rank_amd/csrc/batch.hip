@@ -179,7 +179,8 @@ uint64_t request_bytes(uint64_t m, uint64_t n, uint64_t nnz, uint32_t b, uint64_
 uint64_t Request::device_bytes() const { return request_bytes(m, n, nnz, b, L, d, packed); }
 
 rr_status Request::init(const rr_csr *A, uint32_t p_, uint32_t b_, const uint32_t *dr, const uint32_t *dcv,
-                        const uint32_t *V, uint64_t L_, uint64_t d_, uint32_t tile_rows, hipStream_t st) {
+                        const uint32_t *V, uint64_t L_, uint64_t d_, uint32_t tile_rows, hipStream_t st,
+                        uint32_t su_req) {
     p = p_;
     b = b_;
     B = 1;
@@ -191,6 +192,7 @@ rr_status Request::init(const rr_csr *A, uint32_t p_, uint32_t b_, const uint32_
     d = d_;
     RR_CHECK(p > 2 && p < (1u << 31) && h_is_prime(p), RR_EINVAL, "p=%u must be a prime in (2, 2^31)", p);
     RR_CHECK(b >= 1 && b <= 64, RR_EINVAL, "b=%u must be in 1..64", b);
+    RR_CHECK(su_req == 0 || su_req == 3 || su_req == 4, RR_EINVAL, "su=%u must be 3 or 4 (0: chosen)", su_req);
     RR_CHECK(m >= 1 && n >= 1 && m < (1ull << 31) && n < (1ull << 31), RR_EINVAL, "matrix size out of range");
     RR_CHECK(L >= 2 && d >= 1 && d < L, RR_EINVAL, "need 1 <= d < L (L=%llu d=%llu)", (unsigned long long)L,
              (unsigned long long)d);
@@ -210,7 +212,7 @@ rr_status Request::init(const rr_csr *A, uint32_t p_, uint32_t b_, const uint32_
     std::vector<uint32_t> rp, cw, vw, rpt, ct, vt;
     packed = csr_packs(A, p);
     build_operands(A, p, packed, rp, cw, vw, rpt, ct, vt);
-    su = choose_su(rp, rpt);
+    su = su_req ? su_req : choose_su(rp, rpt);
     std::vector<uint32_t> dc2(n);
     for (uint64_t i = 0; i < n; i++) dc2[i] = h_mulmod(dcv[i], dcv[i], p);
     std::vector<uint32_t> xpad((size_t)n * B, 0);
@@ -671,10 +673,34 @@ using namespace rr;
 
 extern "C" {
 
+static rr_status batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const uint32_t *b, const uint32_t *su_of,
+                            const uint32_t *const *dr, const uint32_t *const *dc, const uint32_t *const *V,
+                            const uint64_t *seq_len, const uint64_t *order, int64_t *rank_out, rr_batch_stats *stats);
+
 rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const uint32_t *b, const uint32_t *const *dr,
                         const uint32_t *const *dc, const uint32_t *const *V, const uint64_t *seq_len,
                         const uint64_t *order, int64_t *rank_out, rr_batch_stats *stats) {
     clear_error();
+    return batch_rank(count, mats, p, b, nullptr, dr, dc, V, seq_len, order, rank_out, stats);
+}
+
+rr_status rr_batch_rank_tuned(uint32_t count, const rr_csr *mats, uint32_t p, const rr_tuning *tune,
+                              const uint32_t *const *dr, const uint32_t *const *dc, const uint32_t *const *V,
+                              const uint64_t *seq_len, const uint64_t *order, int64_t *rank_out,
+                              rr_batch_stats *stats) {
+    clear_error();
+    RR_CHECK(count >= 1 && tune, RR_EINVAL, "null argument");
+    std::vector<uint32_t> b(count), su(count);
+    for (uint32_t i = 0; i < count; i++) {
+        b[i] = tune[i].b;
+        su[i] = tune[i].su;
+    }
+    return batch_rank(count, mats, p, b.data(), su.data(), dr, dc, V, seq_len, order, rank_out, stats);
+}
+
+static rr_status batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const uint32_t *b, const uint32_t *su_of,
+                            const uint32_t *const *dr, const uint32_t *const *dc, const uint32_t *const *V,
+                            const uint64_t *seq_len, const uint64_t *order, int64_t *rank_out, rr_batch_stats *stats) {
     RR_CHECK(count >= 1 && mats && b && dr && dc && V && rank_out, RR_EINVAL, "null argument");
     auto t0 = std::chrono::steady_clock::now();
     int dev = 0, cus = 0;
@@ -749,7 +775,8 @@ rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const ui
                             reqs[i].reset(new Request());
                             const uint64_t L = seq_len_of(i);
                             const uint64_t d = order && order[i] ? order[i] : L - 1;
-                            e = reqs[i]->init(&mats[i], p, b[i], dr[i], dc[i], V[i], L, d, 2048, su.ws[w]);
+                            e = reqs[i]->init(&mats[i], p, b[i], dr[i], dc[i], V[i], L, d, 2048, su.ws[w],
+                                              su_of ? su_of[i] : 0);
                             if (e == RR_OK && hipEventRecord(su.ev[i], su.ws[w]) != hipSuccess) {
                                 e = RR_EDEVICE;
                                 set_error("event record failed");
@@ -948,6 +975,7 @@ struct TuneProbe {
     bool done = false;
     double c[3] = {0, 0, 0}, pm[3] = {0, 0, 0};
     double pm_at[3][2] = {}, alpha[3] = {0, 0, 0}, beta[3] = {0, 0, 0}, f[3] = {0, 0, 0};
+    double slot[3][2] = {};  // ms per tick per padded slot (nonzeros plus padding) at su = 3, 4
 };
 static const uint64_t TUNE_D[2] = {255, 1023};
 static const uint64_t TUNE_M[2] = {24000, 3000};
@@ -968,7 +996,7 @@ static rr_status run_probe(uint32_t p, int cus, hipStream_t st) {
     RR_HIP(hipEventCreate(&e1));
     // ms per tick and copy: TUNE_KCOPIES copies of an m x 0.9m matrix with 11
     // +-1/+-2 entries per row ticking together
-    auto krylov_probe = [&](uint64_t m, uint32_t b, double *ms_tick) -> rr_status {
+    auto krylov_probe = [&](uint64_t m, uint32_t b, uint32_t su, double *ms_tick, double *slots) -> rr_status {
         const uint64_t n = m * 9 / 10, k = 11;
         std::vector<uint64_t> rp(m + 1);
         std::vector<uint32_t> col(m * k), val(m * k);
@@ -981,6 +1009,13 @@ static rr_status run_probe(uint32_t p, int cus, hipStream_t st) {
             }
         }
         rp[m] = m * k;
+        {  // padded slots of the two operands at this block size
+            std::vector<uint32_t> r32(m + 1), ct(n + 1, 0);
+            for (uint64_t r = 0; r <= m; r++) r32[r] = (uint32_t)rp[r];
+            for (uint64_t e = 0; e < m * k; e++) ct[col[e] + 1]++;
+            for (uint64_t c = 0; c < n; c++) ct[c + 1] += ct[c];
+            *slots = (double)padded_slots(r32, ct, su);
+        }
         std::vector<uint32_t> dr(m, 3), dc(n, 5), V(n * b);
         for (auto &v : V) v = (uint32_t)(rnd() % p);
         rr_csr A{m, n, rp.data(), col.data(), val.data()};
@@ -989,7 +1024,7 @@ static rr_status run_probe(uint32_t p, int cus, hipStream_t st) {
         std::vector<Request *> all;
         for (auto &r : R) {
             r.reset(new Request());
-            if ((s = r->init(&A, p, b, dr.data(), dc.data(), V.data(), 40, 39, 2048, st)) != RR_OK) return s;
+            if ((s = r->init(&A, p, b, dr.data(), dc.data(), V.data(), 40, 39, 2048, st, su)) != RR_OK) return s;
             all.push_back(r.get());
         }
         TickRunner T;
@@ -1009,12 +1044,16 @@ static rr_status run_probe(uint32_t p, int cus, hipStream_t st) {
     for (int i = 0; i < 3; i++) {
         uint32_t b = TUNE_B[i];
         // cost per tick = c nnz + f: c from the difference of two sizes, f the rest
-        double t_big = 0, t_small = 0;
-        if ((s = krylov_probe(TUNE_M[0], b, &t_big)) != RR_OK) return s;
-        if ((s = krylov_probe(TUNE_M[1], b, &t_small)) != RR_OK) return s;
+        double t_big = 0, t_small = 0, t_big4 = 0, sl_big = 0, sl_small = 0, sl_big4 = 0;
+        if ((s = krylov_probe(TUNE_M[0], b, 3, &t_big, &sl_big)) != RR_OK) return s;
+        if ((s = krylov_probe(TUNE_M[1], b, 3, &t_small, &sl_small)) != RR_OK) return s;
+        if ((s = krylov_probe(TUNE_M[0], b, 4, &t_big4, &sl_big4)) != RR_OK) return s;
         const double nz_big = 11.0 * TUNE_M[0], nz_small = 11.0 * TUNE_M[1];
         g_probe.c[i] = std::max(0.0, (t_big - t_small) / (nz_big - nz_small));
         g_probe.f[i] = std::max(0.0, t_small - g_probe.c[i] * nz_small);
+        // the same matrix at both block sizes: the tick time beyond f per padded slot
+        g_probe.slot[i][0] = std::max(0.0, t_big - g_probe.f[i]) / sl_big;
+        g_probe.slot[i][1] = std::max(0.0, t_big4 - g_probe.f[i]) / sl_big4;
         // PM-basis probe: TUNE_COPIES jobs of random sequences at each order of TUNE_D
         const uint64_t P = b * (b + 1) / 2;
         for (int o = 0; o < 2; o++) {
@@ -1098,6 +1137,39 @@ rr_status rr_autotune_b(uint32_t count, const rr_csr *mats, uint32_t p, uint32_t
             if (k != 1 && t[k] < TUNE_MARGIN * t[1] && t[k] < t[kb]) kb = k;
         b_out[i] = TUNE_B[kb];
         if (pred_ms) pred_ms[i] = t[kb];
+    }
+    return RR_OK;
+}
+
+rr_status rr_autotune(uint32_t count, const rr_csr *mats, uint32_t p, rr_tuning *out) {
+    clear_error();
+    RR_CHECK(count == 0 || (mats && out), RR_EINVAL, "null argument");
+    std::vector<uint32_t> b(count);
+    std::vector<double> pred(count);
+    rr_status s = rr_autotune_b(count, mats, p, b.data(), pred.data());
+    if (s != RR_OK) return s;
+    for (uint32_t i = 0; i < count; i++) {
+        const rr_csr &A = mats[i];
+        RR_CHECK(A.row_ptr && A.row_ptr[0] == 0 && A.row_ptr[A.m] < (1ull << 32), RR_EINVAL, "matrix %u: bad row_ptr",
+                 i);
+        const uint64_t nnz = A.row_ptr[A.m];
+        std::vector<uint32_t> rp(A.m + 1), ct(A.n + 1, 0);
+        for (uint64_t r = 0; r <= A.m; r++) rp[r] = (uint32_t)A.row_ptr[r];
+        for (uint64_t e = 0; e < nnz; e++) {
+            RR_CHECK(A.col[e] < A.n, RR_EINVAL, "matrix %u: column index out of range", i);
+            ct[A.col[e] + 1]++;
+        }
+        for (uint64_t c = 0; c < A.n; c++) ct[c + 1] += ct[c];
+        const int k = b[i] == 8 ? 0 : b[i] == 16 ? 1 : 2;
+        const double L = 2.0 * (double)std::min(A.m, A.n) / b[i] + 8;
+        double t[2];
+        for (int u = 0; u < 2; u++) t[u] = L * g_probe.slot[k][u] * (double)padded_slots(rp, ct, 3 + u);
+        const int uk = t[1] < t[0] ? 1 : 0;
+        out[i].b = b[i];
+        out[i].su = 3 + uk;
+        // the b model's Krylov term replaced by the slot model's at the chosen block size
+        const double kr = L * ((double)nnz * g_probe.c[k] + g_probe.f[k]);
+        out[i].pred_ms = pred[i] - kr + L * g_probe.f[k] + t[uk];
     }
     return RR_OK;
 }

@@ -68,8 +68,9 @@ struct Request {
     std::vector<GramTile> tiles_gram;
     PmJob pm;
 
+    // su: column block size 3 or 4 (0: the one that pads least)
     rr_status init(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *dr, const uint32_t *dc,
-                   const uint32_t *V, uint64_t L, uint64_t d, uint32_t tile_rows, hipStream_t st);
+                   const uint32_t *V, uint64_t L, uint64_t d, uint32_t tile_rows, hipStream_t st, uint32_t su = 0);
     // PM-basis buffers; the job then reads the coefficients of G built so far
     rr_status alloc_pm();
     // G's coefficients for the sequence entries produced so far (t), on st

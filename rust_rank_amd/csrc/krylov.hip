@@ -1075,15 +1075,18 @@ void build_operands(const rr_csr *A, uint32_t p, bool packed, std::vector<uint32
         }
 }
 
+uint64_t padded_slots(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &rpt, uint32_t su) {
+    // a row of len nonzeros takes an even number of blocks of su slots (rows
+    // past 256 nonzeros are split, their padding is negligible)
+    auto cost = [&](uint32_t len) { return (uint64_t)((len + 2 * su - 1) / (2 * su)) * 2 * su; };
+    uint64_t pad = 0;
+    for (size_t r = 0; r + 1 < rp.size(); r++) pad += cost(std::min<uint32_t>(rp[r + 1] - rp[r], 256));
+    for (size_t r = 0; r + 1 < rpt.size(); r++) pad += cost(std::min<uint32_t>(rpt[r + 1] - rpt[r], 256));
+    return pad;
+}
+
 uint32_t choose_su(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &rpt) {
-    uint64_t pad[2] = {0, 0};
-    for (int k = 0; k < 2; k++) {
-        const uint32_t su = 3 + k;
-        auto cost = [&](uint32_t len) { return (uint64_t)((len + 2 * su - 1) / (2 * su)) * 2 * su; };
-        for (size_t r = 0; r + 1 < rp.size(); r++) pad[k] += cost(std::min<uint32_t>(rp[r + 1] - rp[r], 256));
-        for (size_t r = 0; r + 1 < rpt.size(); r++) pad[k] += cost(std::min<uint32_t>(rpt[r + 1] - rpt[r], 256));
-    }
-    return pad[0] < pad[1] ? 3 : 4;
+    return padded_slots(rp, rpt, 3) < padded_slots(rp, rpt, 4) ? 3 : 4;
 }
 
 static void launch_gram_any(rr_krylov *c, const uint32_t *x, uint32_t nrows, uint32_t *gram) {

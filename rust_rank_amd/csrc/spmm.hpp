@@ -61,6 +61,8 @@ void build_operands(const rr_csr *A, uint32_t p, bool packed, std::vector<uint32
 // True when every value is a signed residue in [-8, 7] and m, n < 2^28.
 bool csr_packs(const rr_csr *A, uint32_t p);
 
+// Slots (nonzeros plus padding) of both operands' rows in column blocks of su.
+uint64_t padded_slots(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &rpt, uint32_t su);
 // Column block size (3 or 4) that pads the rows of both operands least.
 uint32_t choose_su(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &rpt);
 

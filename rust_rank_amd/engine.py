@@ -106,9 +106,12 @@ def lib():
     L.rr_server_destroy.restype = None
     for name in ("rr_server_create", "rr_server_submit", "rr_server_poll", "rr_server_stats_get"):
         getattr(L, name).restype = C.c_int
+    L.rr_batch_rank_tuned.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, _i64p, C.POINTER(BatchStats)]
     L.rr_autotune_b.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, _u32p, C.c_void_p]
     L.rr_autotune_probe.argtypes = [C.c_void_p, C.c_void_p]
-    for name in ("rr_autotune_b", "rr_autotune_probe"):
+    L.rr_autotune.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]
+    for name in ("rr_autotune_b", "rr_autotune_probe", "rr_autotune", "rr_batch_rank_tuned"):
         getattr(L, name).restype = C.c_int
     L.rr_copy_probe.argtypes = [C.c_uint64, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.rr_copy_probe.restype = C.c_int
@@ -314,8 +317,13 @@ def rank(A, p: int, b: int, row_precond, col_precond, V, seq_len: int = 0, order
     return r.value, ms.value
 
 
-def batch_rank(mats, p: int, bs, drs, dcs, Vs, seq_lens=None, orders=None):
-    """Ranks of many matrices in one batched run (rr_batch_rank).  Returns (ranks, stats)."""
+class Tuning(C.Structure):
+    _fields_ = [("b", C.c_uint32), ("su", C.c_uint32), ("pred_ms", C.c_double)]
+
+
+def batch_rank(mats, p: int, bs, drs, dcs, Vs, seq_lens=None, orders=None, sus=None):
+    """Ranks of many matrices in one batched run (rr_batch_rank; with `sus`,
+    the column block size per matrix, rr_batch_rank_tuned).  Returns (ranks, stats)."""
     n = len(mats)
     keep = []
     csrs = (_Csr * n)()
@@ -334,8 +342,13 @@ def batch_rank(mats, p: int, bs, drs, dcs, Vs, seq_lens=None, orders=None):
     d = np.ascontiguousarray(orders if orders is not None else np.zeros(n), dtype=np.uint64)
     ranks = np.zeros(n, np.int64)
     st = BatchStats()
-    _check(lib().rr_batch_rank(n, C.cast(csrs, C.c_void_p), p, b, ptrs(drs), ptrs(dcs), ptrs(Vs),
-                               L.ctypes.data, d.ctypes.data, ranks, C.byref(st)))
+    if sus is None:
+        _check(lib().rr_batch_rank(n, C.cast(csrs, C.c_void_p), p, b, ptrs(drs), ptrs(dcs), ptrs(Vs),
+                                   L.ctypes.data, d.ctypes.data, ranks, C.byref(st)))
+    else:
+        tune = (Tuning * n)(*[Tuning(int(b[i]), int(sus[i]), 0.0) for i in range(n)])
+        _check(lib().rr_batch_rank_tuned(n, C.cast(csrs, C.c_void_p), p, C.cast(tune, C.c_void_p), ptrs(drs),
+                                         ptrs(dcs), ptrs(Vs), L.ctypes.data, d.ctypes.data, ranks, C.byref(st)))
     return ranks, st.as_dict()
 
 
@@ -401,6 +414,21 @@ def autotune_b(mats, p: int):
     pm = np.zeros(3)
     _check(lib().rr_autotune_probe(c.ctypes.data, pm.ctypes.data))
     return out, pred, {"c_ms_per_nnz": c.tolist(), "pm_ms_d255": pm.tolist()}
+
+
+def autotune(mats, p: int):
+    """Block width and column block size per matrix (rr_autotune): (b, su, pred_ms) arrays."""
+    n = len(mats)
+    keep = []
+    csrs = (_Csr * n)()
+    for i, A in enumerate(mats):
+        s, k = _csr_struct(A)
+        csrs[i] = s
+        keep.append(k)
+    out = (Tuning * n)()
+    _check(lib().rr_autotune(n, C.cast(csrs, C.c_void_p), p, C.cast(out, C.c_void_p)))
+    return (np.array([t.b for t in out], np.uint32), np.array([t.su for t in out], np.uint32),
+            np.array([t.pred_ms for t in out]))
 
 
 def copy_probe(nbytes: int, iters: int = 20):

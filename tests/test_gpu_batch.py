@@ -157,3 +157,29 @@ def test_batch_beyond_one_staging_slot():
         assert ranks[i] == O.dense_rank(A.to_dense(P), P), i
     for i in range(0, n, 250):
         assert ranks[i] == eng.rank(mats[i], P, bs[i], drs[i], dcs[i], Vs[i])[0], i
+
+
+def test_autotuned_widths_and_block_sizes_give_dense_ranks():
+    """rr_autotune picks b and the column block size su per matrix from its
+    probes; a batch run with those choices (rr_batch_rank_tuned) gives every
+    matrix the rank dense elimination gives.  Rows of 8 nonzeros pad least in
+    blocks of 4, rows of 12 in blocks of 3: the mix gives the model both
+    cases, and forcing the other block size gives the same ranks."""
+    eng = _eng()
+    mats = []
+    for i in range(9):
+        k = (8, 12, 10)[i % 3]
+        A = synth.uniform_sparse(40 + i, 7, 300 + 37 * i, 270 + 33 * i, k, k, P, small=True)
+        mats.append(synth.plant_dependent_columns(synth.Rng(40 + i, 9), A, 5 + i, P, small=True))
+    bs, sus, pred = eng.autotune(mats, P)
+    print(f"autotune: b {bs.tolist()} su {sus.tolist()} pred ms {np.round(pred, 3).tolist()}", flush=True)
+    assert set(sus.tolist()) <= {3, 4} and set(bs.tolist()) <= {8, 16, 32}
+    drs, dcs, Vs = [], [], []
+    for i, A in enumerate(mats):
+        dr, dc, V = synth.preconditioners(200 + i, A.m, A.n, int(bs[i]), P)
+        drs.append(dr); dcs.append(dc); Vs.append(V)
+    ranks, _ = eng.batch_rank(mats, P, bs, drs, dcs, Vs, sus=sus)
+    other, _ = eng.batch_rank(mats, P, bs, drs, dcs, Vs, sus=7 - sus)
+    for i, A in enumerate(mats):
+        ge = O.dense_rank(A.to_dense(P), P)
+        assert ranks[i] == other[i] == ge, (i, ranks[i], other[i], ge)
