@@ -127,10 +127,11 @@ def roofline(r, traffic, kernel, copy_gbs, gather=None):
         out["projection_kernel"] = {"name": "k_proj64", "avg_launch_us": round(r["proj_ms"] * 1e3, 2)}
     if copy_gbs:
         out["copy_ceiling_gbs"] = round(copy_gbs, 1)
-    if gather:  # the same number of random 256-byte row gathers alone, from vectors of the passes' sizes
-        g_us = sum(ms for ms, _ in gather) / len(gather) * 1e3
-        out["gather_probe"] = {"avg_us": round(g_us, 2), "gbs": round(sum(g for _, g in gather) / len(gather), 1),
-                               "pass_time_over_gather_time": round(r["spmm_ms"] * 1e3 / g_us, 3)}
+    if gather:  # the same number of random 256-byte row gathers, from vectors of the passes' sizes
+        for key, runs in gather.items():
+            g_us = sum(ms for ms, _ in runs) / len(runs) * 1e3
+            out[key] = {"avg_us": round(g_us, 2), "gbs": round(sum(g for _, g in runs) / len(runs), 1),
+                        "pass_time_over_probe_time": round(r["spmm_ms"] * 1e3 / g_us, 3)}
     return out
 
 
@@ -381,7 +382,10 @@ def main():
     log(f"workload {desc}: nnz={A.nnz} L={L}; generated in {time.perf_counter() - t0:.1f}s")
     r, _ = krylov_leg(engine, A, p, b, dr, dc, V, steps, args.warmup, 40)
     _, copy_gbs = engine.copy_probe(1 << 30, 20)
-    gather = [engine.gather_probe(rows, A.nnz, 5) for rows in (A.n, A.m)]  # by A reads x (n rows), by A^T y (m)
+    # by A gathers rows of x (n rows), by A^T of y (m rows): the gathers alone, then with the pass's
+    # value reads and output rows
+    gather = {"gather_probe": [engine.gather_probe(rows, A.nnz, 5) for rows in (A.n, A.m)],
+              "pass_traffic_probe": [engine.gather_probe(rows, A.nnz, 5, True) for rows in (A.n, A.m)]}
     b_iter = 8 * A.nnz + (4 * b + 2) * (A.m + A.n)
     roof = roofline(r, traffic_from_profile("traffic_k_spmm_large_single.json"), "k_spmm<64, general>", copy_gbs,
                     gather)

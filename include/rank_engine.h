@@ -346,10 +346,13 @@ rr_status rr_copy_probe(uint64_t bytes, uint32_t iters, double *ms_out, double *
  * Measurement helper (no counterpart in the reference): `gathers` random
  * 256-byte rows (uniform row indices, read from a 4-byte index array) out
  * of an array of `rows` rows, `iters` times; average kernel time and GB/s of
- * gathered rows plus indices.  The achievable rate of the Krylov pass's
- * vector gathers when its columns are uniform.
+ * the bytes moved.  pass_mix = 0: the gathers and indices alone; pass_mix =
+ * 1: the traffic of a Krylov pass over rows of 20 uniform entries (a 4-byte
+ * value per gather and one 256-byte output row per 20 gathers besides).
+ * The achievable rate of the pass's memory traffic on this GPU.
  */
-rr_status rr_gather_probe(uint64_t rows, uint64_t gathers, uint32_t iters, double *ms_out, double *gbs_out);
+rr_status rr_gather_probe(uint64_t rows, uint64_t gathers, uint32_t iters, int pass_mix, double *ms_out,
+                          double *gbs_out);
 
 #ifdef __cplusplus
 }

@@ -169,25 +169,40 @@ __global__ __launch_bounds__(256) void k_gather_index(uint32_t *idx, uint64_t n,
 
 // Each group of 16 lanes reads one 256-byte row (16 x 16 bytes) per index,
 // four indices in flight per group; the XOR of what it read is stored so the
-// loads stay live.
+// loads stay live.  With val != nullptr (the Krylov pass's traffic mix on
+// rows of 20 entries) every gather also reads a 4-byte value, and each group
+// writes one 256-byte row of Y (nout rows) per 20 gathers: every fifth
+// iteration of its loop.
 __global__ __launch_bounds__(256) void k_gather_rows(const uint4 *__restrict__ X, const uint32_t *__restrict__ idx,
-                                                     uint64_t n, uint32_t *out) {
+                                                     const uint32_t *__restrict__ val, uint64_t n, uint4 *__restrict__ Y,
+                                                     uint64_t nout, uint32_t *out) {
     const uint32_t lane = threadIdx.x & 63, g = lane >> 4, q = lane & 15;
     const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6), nwave = (uint64_t)gridDim.x * 4;
     uint32_t acc = 0;
-    for (uint64_t base = wave * 16; base < n; base += nwave * 16) {
-        uint32_t r[4];
+    uint4 o = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t k = 0;
+    for (uint64_t base = wave * 16; base < n; base += nwave * 16, k++) {
+        uint32_t r[4], w[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const uint64_t i = base + u * 4 + g;
             r[u] = i < n ? idx[i] : 0u;
+            w[u] = (val && i < n) ? val[i] : 1u;
         }
         uint4 v[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) v[u] = X[(uint64_t)r[u] * 16 + q];
 #pragma unroll
-        for (int u = 0; u < 4; u++) acc ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+        for (int u = 0; u < 4; u++) {
+            o.x += v[u].x * w[u];
+            o.y += v[u].y * w[u];
+            o.z += v[u].z * w[u];
+            o.w += v[u].w * w[u];
+        }
+        const uint64_t row = base / 16 / 5 * 4 + g;  // < n / 20 + 4
+        if (Y && k % 5 == 4 && row < nout) Y[row * 16 + q] = o;
     }
+    acc = o.x ^ o.y ^ o.z ^ o.w;
     out[(uint64_t)blockIdx.x * 256 + threadIdx.x] = acc;
 }
 
@@ -197,7 +212,8 @@ using namespace rr;
 
 extern "C" {
 
-rr_status rr_gather_probe(uint64_t rows, uint64_t gathers, uint32_t iters, double *ms_out, double *gbs_out) {
+rr_status rr_gather_probe(uint64_t rows, uint64_t gathers, uint32_t iters, int pass_mix, double *ms_out,
+                          double *gbs_out) {
     clear_error();
     RR_CHECK(rows >= 1 && rows <= (1ull << 32) && gathers >= 1 && iters >= 1 && ms_out && gbs_out, RR_EINVAL,
              "bad gather probe args");
@@ -205,11 +221,16 @@ rr_status rr_gather_probe(uint64_t rows, uint64_t gathers, uint32_t iters, doubl
     RR_HIP(hipGetDevice(&dev));
     RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const uint32_t grid = (uint32_t)cus * 16;
-    DevBuf X, idx, out;
+    const uint64_t nout = gathers / 20 + 64;  // one row per 20 gathers
+    DevBuf X, idx, out, val, Y;
     rr_status s;
     if ((s = X.alloc(rows * 256)) != RR_OK) return s;
     if ((s = idx.alloc(gathers * 4)) != RR_OK) return s;
     if ((s = out.alloc((size_t)grid * 256 * 4)) != RR_OK) return s;
+    if (pass_mix) {
+        if ((s = val.alloc(gathers * 4)) != RR_OK) return s;
+        if ((s = Y.alloc(nout * 256)) != RR_OK) return s;
+    }
     hipStream_t st;
     RR_HIP(stream_create(&st));
     struct S {
@@ -217,22 +238,26 @@ rr_status rr_gather_probe(uint64_t rows, uint64_t gathers, uint32_t iters, doubl
         ~S() { stream_destroy(s); }
     } guard{st};
     RR_HIP(hipMemsetAsync(X.ptr, 1, rows * 256, st));
+    if (pass_mix) RR_HIP(hipMemsetAsync(val.ptr, 3, gathers * 4, st));
     hipLaunchKernelGGL(k_gather_index, dim3(grid), dim3(256), 0, st, idx.as<uint32_t>(), gathers, rows, 12345ull);
-    hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, st, X.as<uint4>(), idx.as<uint32_t>(), gathers,
-                       out.as<uint32_t>());
+    const uint32_t *vp = pass_mix ? val.as<uint32_t>() : nullptr;
+    uint4 *yp = pass_mix ? Y.as<uint4>() : nullptr;
+    hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, st, X.as<uint4>(), idx.as<uint32_t>(), vp, gathers, yp,
+                       nout, out.as<uint32_t>());
     hipEvent_t e0, e1;
     RR_HIP(hipEventCreate(&e0));
     RR_HIP(hipEventCreate(&e1));
     RR_HIP(hipEventRecord(e0, st));
     for (uint32_t i = 0; i < iters; i++)
-        hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, st, X.as<uint4>(), idx.as<uint32_t>(), gathers,
-                           out.as<uint32_t>());
+        hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, st, X.as<uint4>(), idx.as<uint32_t>(), vp, gathers,
+                           yp, nout, out.as<uint32_t>());
     RR_HIP(hipEventRecord(e1, st));
     RR_HIP(hipEventSynchronize(e1));
     float ms = 0;
     RR_HIP(hipEventElapsedTime(&ms, e0, e1));
     *ms_out = ms / iters;
-    *gbs_out = (double)gathers * 260.0 / (*ms_out * 1e-3) / 1e9;
+    const double bytes = (double)gathers * (pass_mix ? 264.0 + 256.0 / 20.0 : 260.0);
+    *gbs_out = bytes / (*ms_out * 1e-3) / 1e9;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return RR_OK;

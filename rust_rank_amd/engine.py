@@ -115,7 +115,8 @@ def lib():
         getattr(L, name).restype = C.c_int
     L.rr_copy_probe.argtypes = [C.c_uint64, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.rr_copy_probe.restype = C.c_int
-    L.rr_gather_probe.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.rr_gather_probe.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int, C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]
     L.rr_gather_probe.restype = C.c_int
     _u64ptr = C.POINTER(C.c_uint64)
     L.rr_sms_open.argtypes = [C.c_char_p, C.c_uint32, C.POINTER(C.c_void_p), _u64ptr, _u64ptr, _u64ptr]
@@ -438,8 +439,9 @@ def copy_probe(nbytes: int, iters: int = 20):
     return ms.value, gbs.value
 
 
-def gather_probe(rows: int, gathers: int, iters: int = 5):
-    """Random 256-byte row gathers: (avg kernel ms, GB/s of rows + indices)."""
+def gather_probe(rows: int, gathers: int, iters: int = 5, pass_mix: bool = False):
+    """Random 256-byte row gathers (with pass_mix, plus the Krylov pass's value
+    reads and output rows): (avg kernel ms, GB/s of the bytes moved)."""
     ms, gbs = C.c_double(), C.c_double()
-    _check(lib().rr_gather_probe(rows, gathers, iters, C.byref(ms), C.byref(gbs)))
+    _check(lib().rr_gather_probe(rows, gathers, iters, int(pass_mix), C.byref(ms), C.byref(gbs)))
     return ms.value, gbs.value
