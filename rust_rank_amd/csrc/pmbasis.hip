@@ -331,6 +331,28 @@ __device__ __forceinline__ uint32_t leaf_lsp_any(const LeafLds &L, uint32_t m, u
     return leaf_lsp<1, 4>(L, m, n, p);
 }
 
+// M <- T M for M = I (a multi-step leaf's first step): row i of T is
+// sigma_i at column perm(i) plus U_iq at column prow(q), and no two of these
+// meet (prow(q) = perm(i) only for q = row i's own pivot, where U_iq = 0), so
+// the product is T itself, each row at coefficient isx_i, with no arithmetic.
+// L.R holds the elimination's rows, free by now: it maps a row of M to its
+// pivot index + 1 (0: none).
+__device__ __forceinline__ void leaf_advance_identity(const LeafLds &L, uint32_t *M, uint32_t m, uint32_t n,
+                                                      uint32_t MS, uint32_t rank) {
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t j = tid; j < m; j += LEAF_THREADS) L.R[j] = 0;
+    __syncthreads();
+    for (uint32_t q = tid; q < rank; q += LEAF_THREADS) L.R[L.prow[q]] = q + 1;
+    __syncthreads();
+    for (uint32_t idx = tid; idx < m * m; idx += LEAF_THREADS) {
+        const uint32_t i = idx / m, j = idx - i * m, qq = L.R[j], sh = L.isx[i];
+        const uint32_t v = j == L.perm[i] ? L.sig[i] : qq ? L.U[i * n + qq - 1] : 0u;
+        M[(size_t)idx * MS + sh] = v;
+        M[(size_t)idx * MS + 1 - sh] = 0u;
+    }
+    __syncthreads();
+}
+
 // The elimination and what the advance needs from it.  normalize: Linv =
 // U / sigma (one inversion per row; a single-step leaf writes its basis from
 // it).  Otherwise rows stay scaled: the step basis of the scaled rows is row
@@ -444,7 +466,8 @@ __global__ __launch_bounds__(LEAF_THREADS) void k_mbasis_iter(LeafPack pk, uint3
         // P <- T P on the coefficients later steps read (s .. K-1 in, s+1 .. K-1 out)
         if (s + 1 < K) leaf_advance(L, L.P, n, K, s, K - 1, s + 1, K - 1, rank, m, n, p, pd, inv);
         // M <- T M (degree s -> s + 1)
-        leaf_advance(L, L.M, m, MS, 0, s, 0, s + 1, rank, m, n, p, pd, inv);
+        if (s == 0) leaf_advance_identity(L, L.M, m, n, MS, rank);
+        else leaf_advance(L, L.M, m, MS, 0, s, 0, s + 1, rank, m, n, p, pd, inv);
         for (uint32_t i = tid; i < m; i += nt) {
             L.dl[i] = L.dln[i];
             L.cs[i] = L.fcol[i];
