@@ -76,21 +76,92 @@ __device__ __forceinline__ void lds_stage(uint32_t *s, uint32_t total, uint32_t 
     __syncthreads();
 }
 
+// Two radix-2 stages at once (spans 2g and g), one radix-4 butterfly per
+// thread and step: the four words base + j + k g, k < 4, of a block of 4g
+// (j < g).  DIF: the span-2g pairs (0, 2), (1, 3) with w_4g^j, w_4g^(j+g),
+// then the span-g pairs (0, 1), (2, 3) with w_2g^j; DIT the same in reverse
+// (span g first).  Every word sees the operations of the two radix-2 stages
+// in the same order, with half the LDS passes and barriers.
+template <bool DIF>
+__device__ __forceinline__ void lds_stage4(uint32_t *s, uint32_t total4, uint32_t logn, uint32_t lg,
+                                           const uint32_t *tw, const uint32_t *twq, uint32_t tstride4, uint32_t p,
+                                           uint32_t ps) {
+    constexpr uint32_t U4 = 2;  // radix-4 butterflies per thread in flight
+    const uint32_t g = 1u << lg, quarter = 1u << (logn - 2);
+    for (uint32_t t0 = threadIdx.x; t0 < total4; t0 += NTT_THREADS * U4) {
+        uint32_t i0[U4], x[U4][4], w[U4][3], wq[U4][3];
+#pragma unroll
+        for (uint32_t u = 0; u < U4; u++) {
+            uint32_t t = t0 + u * NTT_THREADS;
+            t = t < total4 ? t : t0;
+            const uint32_t poly = t >> (logn - 2), bfl = t & (quarter - 1);
+            const uint32_t j = bfl & (g - 1), blk = bfl >> lg;
+            i0[u] = poly * ps + blk * 4 * g + j;
+#pragma unroll
+            for (uint32_t k = 0; k < 4; k++) x[u][k] = s[i0[u] + k * g];
+            // w_4g^j, w_4g^(j+g), w_2g^j = w_4g^(2j)
+            const uint32_t e[3] = {j * tstride4, (j + g) * tstride4, 2 * j * tstride4};
+#pragma unroll
+            for (uint32_t k = 0; k < 3; k++) {
+                w[u][k] = tw[e[k]];
+                wq[u][k] = twq[e[k]];
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U4; u++) {
+            uint32_t y[4];
+            if constexpr (DIF) {
+                const uint32_t b0 = addmod(x[u][0], x[u][2], p);
+                const uint32_t b2 = shoup_mul(submod(x[u][0], x[u][2], p), w[u][0], wq[u][0], p);
+                const uint32_t b1 = addmod(x[u][1], x[u][3], p);
+                const uint32_t b3 = shoup_mul(submod(x[u][1], x[u][3], p), w[u][1], wq[u][1], p);
+                y[0] = addmod(b0, b1, p);
+                y[1] = shoup_mul(submod(b0, b1, p), w[u][2], wq[u][2], p);
+                y[2] = addmod(b2, b3, p);
+                y[3] = shoup_mul(submod(b2, b3, p), w[u][2], wq[u][2], p);
+            } else {
+                const uint32_t t1 = shoup_mul(x[u][1], w[u][2], wq[u][2], p);
+                const uint32_t b0 = addmod(x[u][0], t1, p), b1 = submod(x[u][0], t1, p);
+                const uint32_t t3 = shoup_mul(x[u][3], w[u][2], wq[u][2], p);
+                const uint32_t b2 = addmod(x[u][2], t3, p), b3 = submod(x[u][2], t3, p);
+                const uint32_t t2 = shoup_mul(b2, w[u][0], wq[u][0], p);
+                y[0] = addmod(b0, t2, p);
+                y[2] = submod(b0, t2, p);
+                const uint32_t t4 = shoup_mul(b3, w[u][1], wq[u][1], p);
+                y[1] = addmod(b1, t4, p);
+                y[3] = submod(b1, t4, p);
+            }
+            if (t0 + u * NTT_THREADS < total4) {
+#pragma unroll
+                for (uint32_t k = 0; k < 4; k++) s[i0[u] + k * g] = y[k];
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // In-LDS DIF over `npoly` polynomials of length N = 2^logn, ps words apart
-// (N, or N + 1 where a transposed fill must avoid bank conflicts).
+// (N, or N + 1 where a transposed fill must avoid bank conflicts): radix-4
+// stages from the largest spans down, one radix-2 stage of span 1 when logn
+// is odd.
 __device__ void lds_dif(uint32_t *s, uint32_t npoly, uint32_t logn, const uint32_t *tw, const uint32_t *twq,
                         uint32_t tw_logn, uint32_t p, uint32_t ps) {
     const uint32_t total = npoly << (logn - 1);
-    for (uint32_t lh = logn; lh-- > 0;)  // butterfly span h = 2^lh; w_{2h}^j = w_Ntab^(j * Ntab / 2h)
-        lds_stage<true>(s, total, logn, lh, tw, twq, 1u << (tw_logn - lh - 1), p, ps);
+    uint32_t lh = logn;  // spans 2^(lh-1) and 2^(lh-2) next; w_{2h}^j = w_Ntab^(j * Ntab / 2h)
+    for (; lh >= 2; lh -= 2)
+        lds_stage4<true>(s, npoly << (logn - 2), logn, lh - 2, tw, twq, 1u << (tw_logn - lh), p, ps);
+    if (lh == 1) lds_stage<true>(s, total, logn, 0, tw, twq, 1u << (tw_logn - 1), p, ps);
 }
 
-// In-LDS DIT (inverse twiddles in tw), bit-reversed in, natural out, unscaled.
+// In-LDS DIT (inverse twiddles in tw), bit-reversed in, natural out, unscaled:
+// radix-4 stages from span 1 up, one radix-2 stage of span N/2 when logn is odd.
 __device__ void lds_dit(uint32_t *s, uint32_t npoly, uint32_t logn, const uint32_t *tw, const uint32_t *twq,
                         uint32_t tw_logn, uint32_t p, uint32_t ps) {
     const uint32_t total = npoly << (logn - 1);
-    for (uint32_t lh = 0; lh < logn; lh++)
-        lds_stage<false>(s, total, logn, lh, tw, twq, 1u << (tw_logn - lh - 1), p, ps);
+    uint32_t lh = 0;  // spans 2^lh and 2^(lh+1) next
+    for (; lh + 2 <= logn; lh += 2)
+        lds_stage4<false>(s, npoly << (logn - 2), logn, lh, tw, twq, 1u << (tw_logn - lh - 2), p, ps);
+    if (lh < logn) lds_stage<false>(s, total, logn, lh, tw, twq, 1u << (tw_logn - lh - 1), p, ps);
 }
 
 constexpr uint32_t NTT_LD = 8;  // global loads per thread in flight when staging
