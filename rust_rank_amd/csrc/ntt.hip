@@ -194,8 +194,15 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_fwd(NttFwdPack pk, uint32_t
             if (i < words) s[i] = (i & (N - 1)) < D.len ? v[e] : 0u;
         }
     }
+    // the N-point twiddles w_N^k, k < N/2, into LDS behind the data
+    uint32_t *tws = s + max(N, NTT_WG_WORDS), *twsq = tws + (N >> 1);
+    const uint32_t tstep = 1u << (tw_logn - logn);
+    for (uint32_t k = threadIdx.x; k < (N >> 1); k += NTT_THREADS) {
+        tws[k] = tw[k * tstep];
+        twsq[k] = twq[k * tstep];
+    }
     __syncthreads();
-    lds_dif(s, np, logn, tw, twq, tw_logn, p, N);
+    lds_dif(s, np, logn, tws, twsq, logn, p, N);
     uint32_t *dst = D.dst + (size_t)q0 * N;
     for (uint32_t i0 = threadIdx.x; i0 < words; i0 += NTT_THREADS * NTT_LD) {
         uint32_t v[NTT_LD];
@@ -226,8 +233,14 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_inv(NttInvPack pk, uint32_t
         for (uint32_t e = 0; e < NTT_LD; e++)
             if (i0 + e * NTT_THREADS < words) s[i0 + e * NTT_THREADS] = v[e];
     }
+    uint32_t *tws = s + max(N, NTT_WG_WORDS), *twsq = tws + (N >> 1);
+    const uint32_t tstep = 1u << (tw_logn - logn);
+    for (uint32_t k = threadIdx.x; k < (N >> 1); k += NTT_THREADS) {
+        tws[k] = tw[k * tstep];
+        twsq[k] = twq[k * tstep];
+    }
     __syncthreads();
-    lds_dit(s, np, logn, tw, twq, tw_logn, p, N);
+    lds_dit(s, np, logn, tws, twsq, logn, p, N);
     const uint32_t outs = np * D.outlen;
     for (uint32_t i0 = threadIdx.x; i0 < outs; i0 += NTT_THREADS * NTT_LD) {
         uint32_t v[NTT_LD];
@@ -510,7 +523,22 @@ static inline uint32_t blocks_for(uint32_t count, uint32_t logn) {
     return (count + per - 1) / per;
 }
 
-static inline size_t lds_for(uint32_t logn) { return (size_t)std::max(1u << logn, NTT_WG_WORDS) * 4; }
+// data (one polynomial, or several sharing NTT_WG_WORDS) and the N/2 twiddle pairs
+static inline size_t lds_for(uint32_t logn) { return ((size_t)std::max(1u << logn, NTT_WG_WORDS) + (1u << logn)) * 4; }
+
+// Dynamic LDS above 64 KB needs the kernels' attribute raised (once per process).
+static rr_status ntt_lds_attrs() {
+    static std::once_flag once;
+    static hipError_t err = hipSuccess;
+    std::call_once(once, [] {
+        const int bytes = (int)lds_for(NTT_LDS_LOGN);
+        err = hipFuncSetAttribute((const void *)k_ntt_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (err == hipSuccess)
+            err = hipFuncSetAttribute((const void *)k_ntt_inv, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    });
+    RR_HIP(err);
+    return RR_OK;
+}
 
 static rr_status ntt_forward_big(const NttTables &T, const NttFwdPack &pk, uint32_t ndesc, uint32_t logn,
                                  hipStream_t st) {
@@ -556,6 +584,8 @@ rr_status ntt_forward_pack(const NttTables &T, const NttFwdPack &pk, uint32_t nd
     RR_CHECK(logn >= 1 && logn <= T.logn && logn <= NTT_MAX_LOGN, RR_EINVAL, "NTT size 2^%u unsupported", logn);
     RR_CHECK(ndesc >= 1 && ndesc <= NTT_PACK, RR_EINVAL, "bad pack size");
     if (logn > NTT_LDS_LOGN) return ntt_forward_big(T, pk, ndesc, logn, st);
+    rr_status e;
+    if ((e = ntt_lds_attrs()) != RR_OK) return e;
     uint32_t bx = 0;
     for (uint32_t i = 0; i < ndesc; i++) bx = std::max(bx, blocks_for(pk.d[i].count, logn));
     hipLaunchKernelGGL(k_ntt_fwd, dim3(std::max(bx, 1u), ndesc), dim3(NTT_THREADS), lds_for(logn), st, pk, logn,
@@ -569,6 +599,8 @@ rr_status ntt_inverse_pack(const NttTables &T, const NttInvPack &pk, uint32_t nd
     RR_CHECK(logn >= 1 && logn <= T.logn && logn <= NTT_MAX_LOGN, RR_EINVAL, "NTT size 2^%u unsupported", logn);
     RR_CHECK(ndesc >= 1 && ndesc <= NTT_PACK, RR_EINVAL, "bad pack size");
     if (logn > NTT_LDS_LOGN) return ntt_inverse_big(T, pk, ndesc, logn, st);
+    rr_status e;
+    if ((e = ntt_lds_attrs()) != RR_OK) return e;
     uint32_t bx = 0;
     for (uint32_t i = 0; i < ndesc; i++) bx = std::max(bx, blocks_for(pk.d[i].count, logn));
     hipLaunchKernelGGL(k_ntt_inv, dim3(std::max(bx, 1u), ndesc), dim3(NTT_THREADS), lds_for(logn), st, pk, logn,
