@@ -406,9 +406,9 @@ rr_status TickRunner::tick(std::vector<Request *> &reqs, uint32_t p, uint32_t nt
     // requests share its launches).
     auto bytes_of = [&](const Request *R) {
         return R->t == 0 ? R->tiles_gram.size() * sizeof(GramTile) + sizeof(RedJob) + 64
-                         : (size_t)nticks * (sizeof(SpmmArgs) + sizeof(uint32_t) + sizeof(RedJob) + 64);
+                         : (size_t)nticks * (sizeof(SpmmArgs) + sizeof(RedJob) + 64) + NXCD * 16 + 64;
     };
-    const size_t cap = STAGE_BYTES - 7 * 16 - 1024;
+    const size_t cap = STAGE_BYTES - 7 * 16 - 1024 - 2 * 16 * (WGMAP_HEAD_WORDS * 4 + 16);
     size_t tot = 0;
     for (auto *R : reqs) tot += bytes_of(R);
     if (tot <= cap) return tick_staged(reqs, p, nticks);
@@ -440,7 +440,7 @@ rr_status TickRunner::tick_staged(std::vector<Request *> &reqs, uint32_t p, uint
         uint32_t B = 0, su = 0;
         bool packed = false;
         std::vector<SpmmArgs> jobs;
-        std::vector<uint32_t> wg;  // prefix of workgroups, jobs + 1 entries
+        std::vector<uint32_t> nwg;  // workgroups of each job
     };
     std::vector<std::vector<Group>> groups(nticks);
     std::vector<GramTile> gram_by_b[7];
@@ -485,34 +485,46 @@ rr_status TickRunner::tick_staged(std::vector<Request *> &reqs, uint32_t p, uint
                     G->B = R->B;
                     G->su = R->su;
                     G->packed = R->packed;
-                    G->wg.push_back(0);
                 }
                 SpmmArgs a = R->args[(t & 1) ? 0 : 1];
                 a.gram_off = off;
                 G->jobs.push_back(a);
-                G->wg.push_back(G->wg.back() + R->nwg);
+                G->nwg.push_back(R->nwg);
                 J.ntiles = R->nwg;
                 off += (size_t)R->nwg * R->P;
             }
             red.push_back(J);
         }
     }
+    // Every tick has the same groups with the same jobs' workgroups (the same
+    // requests in the same order), so one workgroup map per group serves all.
+    std::vector<std::vector<uint32_t>> maps(groups[0].size());
+    std::vector<uint32_t> grids(groups[0].size());
+    for (size_t gi = 0; gi < groups[0].size(); gi++) maps[gi] = build_wg_map(groups[0][gi].nwg, &grids[gi]);
+    for (uint32_t c = 1; c < nticks; c++) {
+        RR_CHECK(groups[c].size() == groups[0].size(), RR_EINVAL, "tick groups differ");
+        for (size_t gi = 0; gi < groups[c].size(); gi++)
+            RR_CHECK(groups[c][gi].nwg == groups[0][gi].nwg, RR_EINVAL, "tick groups differ");
+    }
     // stage descriptors in one pinned block, one copy:
-    // [gram tiles by width][per tick, per group: jobs, workgroup prefix][reduction jobs]
+    // [gram tiles by width][workgroup map per group][per tick, per group: jobs][reduction jobs]
     auto align = [](size_t x) { return (x + 15) & ~(size_t)15; };
     size_t bytes = 0;
     size_t gram_off[7];
-    std::vector<std::vector<size_t>> job_off(nticks), wg_off(nticks);
+    std::vector<std::vector<size_t>> job_off(nticks);
+    std::vector<size_t> map_off;
     for (int lb = 0; lb < 7; lb++) {
         gram_off[lb] = bytes;
         bytes = align(bytes + gram_by_b[lb].size() * sizeof(GramTile));
+    }
+    for (const auto &mw : maps) {
+        map_off.push_back(bytes);
+        bytes = align(bytes + mw.size() * sizeof(uint32_t));
     }
     for (uint32_t c = 0; c < nticks; c++)
         for (size_t gi = 0; gi < groups[c].size(); gi++) {
             job_off[c].push_back(bytes);
             bytes = align(bytes + groups[c][gi].jobs.size() * sizeof(SpmmArgs));
-            wg_off[c].push_back(bytes);
-            bytes = align(bytes + groups[c][gi].wg.size() * sizeof(uint32_t));
         }
     const size_t red_off = bytes;
     bytes += red.size() * sizeof(RedJob);
@@ -524,11 +536,12 @@ rr_status TickRunner::tick_staged(std::vector<Request *> &reqs, uint32_t p, uint
     for (int lb = 0; lb < 7; lb++)
         if (!gram_by_b[lb].empty())
             std::memcpy(h + gram_off[lb], gram_by_b[lb].data(), gram_by_b[lb].size() * sizeof(GramTile));
+    for (size_t gi = 0; gi < maps.size(); gi++)
+        std::memcpy(h + map_off[gi], maps[gi].data(), maps[gi].size() * sizeof(uint32_t));
     for (uint32_t c = 0; c < nticks; c++)
         for (size_t gi = 0; gi < groups[c].size(); gi++) {
             const Group &G = groups[c][gi];
             std::memcpy(h + job_off[c][gi], G.jobs.data(), G.jobs.size() * sizeof(SpmmArgs));
-            std::memcpy(h + wg_off[c][gi], G.wg.data(), G.wg.size() * sizeof(uint32_t));
         }
     std::memcpy(h + red_off, red.data(), red.size() * sizeof(RedJob));
     launched = true;  // from here on, work of this tick is on the stream
@@ -555,7 +568,7 @@ rr_status TickRunner::tick_staged(std::vector<Request *> &reqs, uint32_t p, uint
         for (size_t gi = 0; gi < groups[c].size(); gi++) {
             const Group &G = groups[c][gi];
             launch_spmm(G.B, G.packed, G.su, (const SpmmArgs *)(dbase + job_off[c][gi]),
-                        (const uint32_t *)(dbase + wg_off[c][gi]), (uint32_t)G.jobs.size(), G.wg.back(), pbase, st);
+                        (const uint32_t *)(dbase + map_off[gi]), grids[gi], pbase, st);
             RR_HIP(hipGetLastError());
         }
     uint32_t maxP = 0;

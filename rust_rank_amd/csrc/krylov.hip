@@ -128,16 +128,21 @@ struct GramNone {
 // leave one wavefront per SIMD) is then taken by k_proj64.
 template <int B, bool PACKED, int SU, bool PROJ>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_spmm(
-    const SpmmArgs *__restrict__ jobs, const uint32_t *__restrict__ wg_begin, uint32_t njobs, uint32_t *gram_base) {
-    // this workgroup's job: the last j with wg_begin[j] <= blockIdx.x
-    uint32_t jlo = 0, jhi = njobs;
+    const SpmmArgs *__restrict__ jobs, const uint32_t *__restrict__ wgmap, uint32_t *gram_base) {
+    // this workgroup's job and workgroup within it, from its XCD's list
+    const uint32_t xcd = blockIdx.x % NXCD, k = blockIdx.x / NXCD;
+    const GLOBAL_AS uint32_t *wm = as_global(wgmap);
+    if (k >= wm[NXCD + 1 + xcd]) return;  // past this XCD's workgroups
+    uint32_t jlo = wm[xcd], jhi = wm[xcd + 1];
+    const GLOBAL_AS gv4u *ent = (const GLOBAL_AS gv4u *)(wm + WGMAP_HEAD_WORDS);
     while (jhi - jlo > 1) {
         const uint32_t mid = (jlo + jhi) / 2;
-        if (wg_begin[mid] <= blockIdx.x) jlo = mid;
+        if (ent[mid][0] <= k) jlo = mid;
         else jhi = mid;
     }
-    const SpmmArgs a = jobs[jlo];
-    const uint32_t lwg = blockIdx.x - wg_begin[jlo];  // workgroup within the job
+    const gv4u e = ent[jlo];
+    const SpmmArgs a = jobs[e[1]];
+    const uint32_t lwg = e[2] + (k - e[0]) * e[3];  // workgroup within the job
     using L = Lay<B>;
     constexpr int VW = L::VW, LPR = L::LPR, RPW = L::RPW, BW = RPW * SU;
     __shared__ uint32_t o2s[WG / 64][PROJ ? RPW * 2 * B : 1];  // produced rows, each twice
@@ -683,7 +688,8 @@ struct rr_krylov {
     bool sep_proj = false;
     uint32_t pgrid = 0;  // k_proj64 workgroups: one resident wave (<= grid)
     rr::DevBuf drinv, dc2inv;
-    rr::DevBuf jobs;  // [2] pass descriptors (by A, by A^T) and the one-job workgroup table {0, grid}
+    rr::DevBuf jobs;  // [2] pass descriptors (by A, by A^T) and the one-job workgroup map
+    uint32_t hgrid = 0;  // hardware workgroups of a pass launch (grid rounded up to the XCDs)
     uint64_t seq_cap = 0;  // entries the device sequence buffer holds
     uint32_t grid = 0, P = 0;
     uint64_t t = 0;  // entries produced so far
@@ -964,42 +970,82 @@ SpmmArgs spmm_args(const SliceMat &M, const uint32_t *in, uint64_t in_rows, uint
 }
 
 template <int B, bool PROJ>
-static void launch_bp(bool packed, uint32_t su, const SpmmArgs *jobs, const uint32_t *wg_begin, uint32_t njobs,
-                      uint32_t total_wg, uint32_t *gram_base, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
-    dim3 g(total_wg), t(WG);
+static void launch_bp(bool packed, uint32_t su, const SpmmArgs *jobs, const uint32_t *wgmap, uint32_t grid,
+                      uint32_t *gram_base, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+    dim3 g(grid), t(WG);
     // with events, the dispatch itself stamps its start and end (no marker between launches)
     if (packed && su == 3)
-        hipExtLaunchKernelGGL((k_spmm<B, true, 3, PROJ>), g, t, 0, st, ev0, ev1, 0, jobs, wg_begin, njobs, gram_base);
+        hipExtLaunchKernelGGL((k_spmm<B, true, 3, PROJ>), g, t, 0, st, ev0, ev1, 0, jobs, wgmap, gram_base);
     else if (packed)
-        hipExtLaunchKernelGGL((k_spmm<B, true, 4, PROJ>), g, t, 0, st, ev0, ev1, 0, jobs, wg_begin, njobs, gram_base);
+        hipExtLaunchKernelGGL((k_spmm<B, true, 4, PROJ>), g, t, 0, st, ev0, ev1, 0, jobs, wgmap, gram_base);
     else if (su == 3)
-        hipExtLaunchKernelGGL((k_spmm<B, false, 3, PROJ>), g, t, 0, st, ev0, ev1, 0, jobs, wg_begin, njobs, gram_base);
+        hipExtLaunchKernelGGL((k_spmm<B, false, 3, PROJ>), g, t, 0, st, ev0, ev1, 0, jobs, wgmap, gram_base);
     else
-        hipExtLaunchKernelGGL((k_spmm<B, false, 4, PROJ>), g, t, 0, st, ev0, ev1, 0, jobs, wg_begin, njobs, gram_base);
+        hipExtLaunchKernelGGL((k_spmm<B, false, 4, PROJ>), g, t, 0, st, ev0, ev1, 0, jobs, wgmap, gram_base);
 }
 
 template <int B>
-static void launch_b(bool packed, uint32_t su, bool proj, const SpmmArgs *jobs, const uint32_t *wg_begin,
-                     uint32_t njobs, uint32_t total_wg, uint32_t *gram_base, hipStream_t st, hipEvent_t ev0,
-                     hipEvent_t ev1) {
+static void launch_b(bool packed, uint32_t su, bool proj, const SpmmArgs *jobs, const uint32_t *wgmap, uint32_t grid,
+                     uint32_t *gram_base, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     if constexpr (B == 64) {
-        if (!proj) return launch_bp<B, false>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1);
+        if (!proj) return launch_bp<B, false>(packed, su, jobs, wgmap, grid, gram_base, st, ev0, ev1);
     }
-    launch_bp<B, true>(packed, su, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1);
+    launch_bp<B, true>(packed, su, jobs, wgmap, grid, gram_base, st, ev0, ev1);
 }
 
-void launch_spmm(uint32_t B, bool packed, uint32_t su, const SpmmArgs *jobs, const uint32_t *wg_begin,
-                 uint32_t njobs, uint32_t total_wg, uint32_t *gram_base, hipStream_t st, hipEvent_t ev0,
-                 hipEvent_t ev1, bool proj) {
+void launch_spmm(uint32_t B, bool packed, uint32_t su, const SpmmArgs *jobs, const uint32_t *wgmap, uint32_t grid,
+                 uint32_t *gram_base, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, bool proj) {
     switch (B) {
-        case 1: launch_b<1>(packed, su, proj, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
-        case 2: launch_b<2>(packed, su, proj, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
-        case 4: launch_b<4>(packed, su, proj, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
-        case 8: launch_b<8>(packed, su, proj, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
-        case 16: launch_b<16>(packed, su, proj, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
-        case 32: launch_b<32>(packed, su, proj, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
-        default: launch_b<64>(packed, su, proj, jobs, wg_begin, njobs, total_wg, gram_base, st, ev0, ev1); break;
+        case 1: launch_b<1>(packed, su, proj, jobs, wgmap, grid, gram_base, st, ev0, ev1); break;
+        case 2: launch_b<2>(packed, su, proj, jobs, wgmap, grid, gram_base, st, ev0, ev1); break;
+        case 4: launch_b<4>(packed, su, proj, jobs, wgmap, grid, gram_base, st, ev0, ev1); break;
+        case 8: launch_b<8>(packed, su, proj, jobs, wgmap, grid, gram_base, st, ev0, ev1); break;
+        case 16: launch_b<16>(packed, su, proj, jobs, wgmap, grid, gram_base, st, ev0, ev1); break;
+        case 32: launch_b<32>(packed, su, proj, jobs, wgmap, grid, gram_base, st, ev0, ev1); break;
+        default: launch_b<64>(packed, su, proj, jobs, wgmap, grid, gram_base, st, ev0, ev1); break;
     }
+}
+
+std::vector<uint32_t> build_wg_map(const std::vector<uint32_t> &nwg, uint32_t *grid) {
+    uint64_t total = 0;
+    for (uint32_t w : nwg) total += w;
+    const uint64_t share = (total + NXCD - 1) / NXCD;
+    std::vector<std::vector<uint4>> lists(NXCD);  // per XCD: {k_begin, job, lwg0, stride}
+    std::vector<uint64_t> load(NXCD, 0);
+    std::vector<uint32_t> whole;
+    for (uint32_t j = 0; j < nwg.size(); j++) {
+        if (!nwg[j]) continue;
+        if ((uint64_t)nwg[j] * 2 > share) {  // spread: workgroup w on XCD w mod NXCD
+            for (uint32_t x = 0; x < NXCD && x < nwg[j]; x++) {
+                const uint32_t cnt = (nwg[j] - x + NXCD - 1) / NXCD;
+                lists[x].push_back(make_uint4((uint32_t)load[x], j, x, NXCD));
+                load[x] += cnt;
+            }
+        } else {
+            whole.push_back(j);
+        }
+    }
+    std::stable_sort(whole.begin(), whole.end(), [&](uint32_t a, uint32_t b) { return nwg[a] > nwg[b]; });
+    for (uint32_t j : whole) {
+        const uint32_t x = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        lists[x].push_back(make_uint4((uint32_t)load[x], j, 0, 1));
+        load[x] += nwg[j];
+    }
+    WgMapHead h{};
+    std::vector<uint32_t> ent;
+    uint64_t kmax = 0;
+    for (uint32_t x = 0; x < NXCD; x++) {
+        h.xoff[x] = (uint32_t)(ent.size() / 4);
+        h.xcnt[x] = (uint32_t)load[x];
+        kmax = std::max(kmax, load[x]);
+        for (const uint4 &e : lists[x]) ent.insert(ent.end(), {e.x, e.y, e.z, e.w});
+    }
+    h.xoff[NXCD] = (uint32_t)(ent.size() / 4);
+    std::vector<uint32_t> words(WGMAP_HEAD_WORDS);
+    std::memcpy(words.data(), &h, sizeof h);
+    words.insert(words.end(), ent.begin(), ent.end());
+    *grid = (uint32_t)(kmax * NXCD);
+    return words;
 }
 
 template <int B, bool PROJ>
@@ -1232,10 +1278,10 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
         SpmmArgs args[2] = {spmm_args(c->A, c->z.as<uint32_t>(), n, B, c->y.as<uint32_t>(), c->dr.as<uint32_t>(), p, b),
                             spmm_args(c->At, c->y.as<uint32_t>(), m, B, c->z.as<uint32_t>(), c->dc2.as<uint32_t>(), p,
                                       b)};
-        uint32_t wgs[2] = {0, c->grid};
-        if ((s = c->jobs.alloc(sizeof args + sizeof wgs)) != RR_OK) return s;
+        const std::vector<uint32_t> wgmap = build_wg_map({c->grid}, &c->hgrid);
+        if ((s = c->jobs.alloc(sizeof args + wgmap.size() * 4)) != RR_OK) return s;
         RR_HIP(hipMemcpy(c->jobs.ptr, args, sizeof args, hipMemcpyHostToDevice));
-        RR_HIP(hipMemcpy(c->jobs.as<char>() + sizeof args, wgs, sizeof wgs, hipMemcpyHostToDevice));
+        RR_HIP(hipMemcpy(c->jobs.as<char>() + sizeof args, wgmap.data(), wgmap.size() * 4, hipMemcpyHostToDevice));
     }
     uint32_t nparts = c->grid;
     if ((s = c->part.alloc((size_t)CHUNK * nparts * c->P * 4)) != RR_OK) return s;
@@ -1315,8 +1361,8 @@ rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
             } else {
                 bool byA = (t & 1) == 1;
                 const SpmmArgs *job = c->jobs.as<SpmmArgs>() + (byA ? 0 : 1);
-                const uint32_t *wgs = (const uint32_t *)(c->jobs.as<char>() + 2 * sizeof(SpmmArgs));
-                launch_spmm(c->B, c->packed, c->su, job, wgs, 1, c->grid, gram, c->stream, e0, e1, !c->sep_proj);
+                const uint32_t *wgmap = (const uint32_t *)(c->jobs.as<char>() + 2 * sizeof(SpmmArgs));
+                launch_spmm(c->B, c->packed, c->su, job, wgmap, c->hgrid, gram, c->stream, e0, e1, !c->sep_proj);
                 if (c->sep_proj) {  // the projection of the rows this pass produced
                     const uint32_t *rows = byA ? c->y.as<uint32_t>() : c->z.as<uint32_t>();
                     const uint32_t *sinv = byA ? c->drinv.as<uint32_t>() : c->dc2inv.as<uint32_t>();

@@ -2,9 +2,10 @@
 // Krylov context and the batched / served tick runner.
 //
 // One launch runs any number of jobs of the same lane width, value kind and
-// column block size: job j owns workgroups [wg_begin[j], wg_begin[j + 1]) and
-// describes one operand (A or A^T of one matrix) in sliced form, the vector
-// it gathers, the vector it produces and where its projection partials go.
+// column block size.  Each job describes one operand (A or A^T of one matrix)
+// in sliced form, the vector it gathers, the vector it produces and where its
+// projection partials go; a workgroup map (WgMap) says which workgroup of
+// which job each hardware workgroup runs, and on which XCD.
 #pragma once
 #include "common.hpp"
 #include <vector>
@@ -44,6 +45,28 @@ struct SpmmArgs {
     uint32_t b;
 };
 
+// Workgroup map of one k_spmm launch.  The hardware deals workgroup h to XCD
+// h mod NXCD, so XCD x runs the workgroups k = h / NXCD < xcnt[x] of its own
+// list: entries ent[xoff[x] .. xoff[x + 1]), each {k_begin, job, lwg0,
+// stride}; workgroup k belongs to the last entry with k_begin <= k and is
+// workgroup lwg0 + (k - k_begin) * stride of that job.  Hardware workgroups
+// past xcnt[x] exit at once.  Stored as words: the head, then the entries.
+struct WgMapHead {
+    uint32_t xoff[NXCD + 1];
+    uint32_t xcnt[NXCD];
+    uint32_t pad[3];
+};
+constexpr uint32_t WGMAP_HEAD_WORDS = sizeof(WgMapHead) / 4;
+static_assert(sizeof(WgMapHead) % 16 == 0, "entries must stay 16-byte aligned");
+
+// Places jobs of nwg[j] workgroups on the XCDs.  A job larger than half of
+// an XCD's share of the launch is spread over every XCD (its workgroup w on
+// XCD w mod NXCD, as in a launch of that job alone); the others go whole to
+// one XCD, largest first onto the least loaded, so the vectors a job gathers
+// stay in one XCD's L2.  Returns the map's words; *grid: the hardware
+// workgroups to launch.
+std::vector<uint32_t> build_wg_map(const std::vector<uint32_t> &nwg, uint32_t *grid);
+
 struct SliceMat {  // one operand (A or A^T) on the device, in sliced form
     uint32_t nrows = 0, nslices = 0, nseg = 0, nlong = 0, nw = 0;
     uint64_t blocks = 0;
@@ -78,12 +101,13 @@ SpmmArgs spmm_args(const SliceMat &M, const uint32_t *in, uint64_t in_rows, uint
 // Resident workgroups per CU of the pass kernel for (B, packed, SU, proj).
 int spmm_blocks_per_cu(uint32_t B, bool packed, uint32_t su, bool proj = true);
 
-// One launch over njobs jobs (device arrays); total_wg = wg_begin[njobs].
-// ev0 / ev1 (may be null): stamped with the kernel's own start and end.
+// One launch over the jobs of a workgroup map (device arrays); grid: the
+// hardware workgroups build_wg_map gave.  ev0 / ev1 (may be null): stamped
+// with the kernel's own start and end.
 // proj = false (lane width 64 only): the pass produces its rows and writes no
 // projection partials; k_proj64 then takes the projection.
-void launch_spmm(uint32_t B, bool packed, uint32_t su, const SpmmArgs *jobs, const uint32_t *wg_begin,
-                 uint32_t njobs, uint32_t total_wg, uint32_t *gram_base, hipStream_t st, hipEvent_t ev0 = nullptr,
-                 hipEvent_t ev1 = nullptr, bool proj = true);
+void launch_spmm(uint32_t B, bool packed, uint32_t su, const SpmmArgs *jobs, const uint32_t *wgmap, uint32_t grid,
+                 uint32_t *gram_base, hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
+                 bool proj = true);
 
 }  // namespace rr
