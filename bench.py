@@ -111,7 +111,8 @@ def krylov_leg(engine, A, p, b, dr, dc, V, steps, warmup, prof_steps):
         f"{proj_ms * 1e3:.1f} us/pass; timed {steps} iterations in {wall * 1e3:.1f} ms")
     return {"wall_s": wall, "iters_s": steps / wall, "spmm_ms": spmm_ms, "proj_ms": proj_ms,
             "a_us": st["ms_a"] / pa * 1e3, "at_us": st["ms_at"] / pt * 1e3, "bytes_pass": bytes_pass,
-            "stream_ms": st["ms_total"], "prof_steps": prof_steps}, warm
+            "stream_ms": st["ms_total"], "prof_steps": prof_steps,
+            "setup": {"ms": round(st["ms_setup"], 1), "on_device": bool(st["setup_on_device"])}}, warm
 
 
 def roofline(r, traffic, kernel, copy_gbs, gather=None):
@@ -202,7 +203,8 @@ def batched(eng, synth, p, nmat, seed=4, plain=True):
     out = {"workload": "batched: rows log-uniform in [1000, 50000], cols 0.9 rows, 6-16 nnz/row, +-1/+-2, "
                        "1-5 % planted dependent columns", "matrices": nmat,
            "ranks_per_s": round(nmat / t_batch, 3), "wall_s": round(t_batch, 3), "ticks": st["ticks"],
-           "pm_rounds": st["pm_rounds"], "ms_krylov": round(st["ms_krylov"], 1), "ms_pm": round(st["ms_pm"], 1),
+           "pm_rounds": st["pm_rounds"], "ms_setup": round(st["ms_setup"], 1), "ms_krylov": round(st["ms_krylov"], 1),
+           "ms_pm": round(st["ms_pm"], 1),
            "b_hist": hist(bs), "su_hist": hist(sus), "autotune_probe": probe}
     if not plain:  # the batched call alone (for kernel traces)
         log(f"batched {t_batch:.2f}s (plain solves skipped)")
@@ -348,8 +350,8 @@ def main():
     ap.add_argument("--rate-frac", type=float, default=0.7, help="served offered load / saturated capacity")
     ap.add_argument("--no-plain", action="store_true",
                     help="batched call only: skip the plain solves and the served form (for kernel traces)")
-    ap.add_argument("--workload", choices=["large", "graph_complex"], default="large",
-                    help="graph_complex: only the 200000 x 180000 leg (for kernel traces and counters)")
+    ap.add_argument("--workload", choices=["large", "graph_complex", "batched"], default="large",
+                    help="graph_complex / batched: only that leg (for kernel traces and counters)")
     ap.add_argument("--full-rank", action="store_true", help="only the large matrix's rank end to end")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
                     help="only run the plain copy kernel on MB megabytes (counter calibration) and exit")
@@ -369,6 +371,9 @@ def main():
         return
     if args.workload == "graph_complex":
         print(json.dumps(graph_complex_leg(engine, synth, p, args.steps, not args.no_pm, args.pm_order)), flush=True)
+        return
+    if args.workload == "batched":
+        print(json.dumps(batched(engine, synth, p, args.batched or 512, plain=not args.no_plain)), flush=True)
         return
 
     b = 64
@@ -407,6 +412,7 @@ def main():
         "vs_baseline": None, "dtype": "u32", "data": "synthetic (seeded SplitMix64)",
         "config": {"workload": desc, "seq_len": L, "nnz": nnz, "b": b,
                    "hbm_gbs_algorithmic": round(b_iter * r["iters_s"] / 1e9, 1)},
+        "setup": r["setup"],
         "roofline": roof, "cpu_baseline": cpu, "graph_complex": gc, "batched": bat, "served": srv,
     }
     print(json.dumps(out), flush=True)

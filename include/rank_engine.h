@@ -103,6 +103,9 @@ typedef struct {
     double ms_total;              /* wall time of the run on the stream */
     uint64_t bytes_a, bytes_at;   /* algorithmic bytes of one pass by A / by A^T */
     double ms_proj;               /* b > 32: projection kernels run after the passes (else 0: fused) */
+    double ms_setup;              /* wall time of rr_krylov_create */
+    int32_t setup_on_device;      /* 1: A and A^T were checked, transposed and laid out on the device */
+    int32_t reserved;
 } rr_krylov_stats;
 rr_status rr_krylov_set_profiling(rr_krylov *ctx, int on);
 rr_status rr_krylov_stats_get(rr_krylov *ctx, rr_krylov_stats *out);

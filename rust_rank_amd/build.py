@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "librank_engine.so")
-SOURCES = ["krylov.hip", "ntt.hip", "pmbasis.hip", "batch.hip", "server.cpp", "driver.cpp", "util.hip", "io.cpp"]
+SOURCES = ["krylov.hip", "setup.hip", "ntt.hip", "pmbasis.hip", "batch.hip", "server.cpp", "driver.cpp", "util.hip", "io.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -203,16 +203,10 @@ rr_status Request::init(const rr_csr *A, uint32_t p_, uint32_t b_, const uint32_
     RR_CHECK(nnz < (1ull << 32), RR_EINVAL, "nnz must be < 2^32");
     RR_CHECK(std::max(m, n) * B * 4 < (1ull << 31), RR_EINVAL, "block vectors exceed 2^31 bytes");
     RR_CHECK(nnz == 0 || (A->col && A->val), RR_EINVAL, "null col/val");
-    for (uint64_t e = 0; e < nnz; e++)
-        RR_CHECK(A->col[e] < n && A->val[e] < p, RR_EINVAL, "entry %llu out of range", (unsigned long long)e);
     for (uint64_t i = 0; i < m; i++) RR_CHECK(dr[i] >= 1 && dr[i] < p, RR_EINVAL, "row_precond out of range");
     for (uint64_t i = 0; i < n; i++) RR_CHECK(dcv[i] >= 1 && dcv[i] < p, RR_EINVAL, "col_precond out of range");
     for (uint64_t i = 0; i < n * b; i++) RR_CHECK(V[i] < p, RR_EINVAL, "V out of range");
 
-    std::vector<uint32_t> rp, cw, vw, rpt, ct, vt;
-    packed = csr_packs(A, p);
-    build_operands(A, p, packed, rp, cw, vw, rpt, ct, vt);
-    su = su_req ? su_req : choose_su(rp, rpt);
     std::vector<uint32_t> dc2(n);
     for (uint64_t i = 0; i < n; i++) dc2[i] = h_mulmod(dcv[i], dcv[i], p);
     std::vector<uint32_t> xpad((size_t)n * B, 0);
@@ -232,19 +226,28 @@ rr_status Request::init(const rr_csr *A, uint32_t p_, uint32_t b_, const uint32_
                        dc_d.as<uint32_t>(), z.as<uint32_t>(), (uint32_t)n, B, p, mp.inv);
     RR_HIP(hipGetLastError());
 
-    // Sliced operands for nwg workgroups (a multiple of the XCD count, so the
-    // job keeps its XCD placement wherever it starts in a tick's launch).
+    // Sliced operands for nwg workgroups (a multiple of the XCD count, so a
+    // job spread over the XCDs keeps its XCD placement), checked and laid
+    // out on the device.
     const uint32_t RPW = 64 / (B / (B < 4 ? B : 4));
     const uint64_t steps = std::max(nnz, std::max(m, n)) / RPW + 1;
     const uint64_t waves = (steps + BATCH_STEPS_PER_WAVE - 1) / BATCH_STEPS_PER_WAVE;
     int dev = 0, cus = 0;
     RR_HIP(hipGetDevice(&dev));
     RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const uint64_t max_wg = (uint64_t)cus * spmm_blocks_per_cu(B, packed, su);
-    nwg = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(NXCD, (waves / (WG / 64) + NXCD - 1) / NXCD * NXCD),
-                                       std::max<uint64_t>(NXCD, max_wg / NXCD * NXCD));
-    if ((s = build_slices(sa, m, rp, cw, vw, packed, B, su, nwg * (WG / 64), dr, n)) != RR_OK) return s;
-    if ((s = build_slices(sat, n, rpt, ct, vt, packed, B, su, nwg * (WG / 64), dc2.data(), m)) != RR_OK) return s;
+    OperandInfo oi;
+    s = setup_operands(
+        A, p, B, su_req,
+        [&](bool pk, uint32_t su_) {
+            const uint64_t max_wg = (uint64_t)cus * spmm_blocks_per_cu(B, pk, su_);
+            nwg = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(NXCD, (waves / (WG / 64) + NXCD - 1) / NXCD * NXCD),
+                                               std::max<uint64_t>(NXCD, max_wg / NXCD * NXCD));
+            return nwg * (uint32_t)(WG / 64);
+        },
+        dr, dc2.data(), sa, sat, &oi, st);
+    if (s != RR_OK) return s;
+    packed = oi.packed;
+    su = oi.su;
     args[0] = spmm_args(sa, z.as<uint32_t>(), n, B, y.as<uint32_t>(), dr_d.as<uint32_t>(), p, b);
     args[1] = spmm_args(sat, y.as<uint32_t>(), m, B, z.as<uint32_t>(), dc2_d.as<uint32_t>(), p, b);
 

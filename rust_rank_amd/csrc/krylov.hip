@@ -34,6 +34,7 @@
 #include <hip/hip_ext.h>
 #include <vector>
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <queue>
@@ -690,6 +691,7 @@ struct rr_krylov {
     rr::DevBuf drinv, dc2inv;
     rr::DevBuf jobs;  // [2] pass descriptors (by A, by A^T) and the one-job workgroup map
     uint32_t hgrid = 0;  // hardware workgroups of a pass launch (grid rounded up to the XCDs)
+    bool on_device_setup = false;  // operands laid out by the device path
     uint64_t seq_cap = 0;  // entries the device sequence buffer holds
     uint32_t grid = 0, P = 0;
     uint64_t t = 0;  // entries produced so far
@@ -729,26 +731,45 @@ static rr_status upload(DevBuf &buf, const T *host, size_t count) {
 // blocks (rounded up to an even count); longer rows become segment slices.  Slices are dealt to wavefronts
 // longest first, each to the least loaded one, and every wavefront's slices
 // and column blocks are stored contiguously in that order.
+constexpr uint32_t FIN = 8;  // cost of producing a slice, in steps
+
+uint32_t long_row_threshold(uint64_t nnz, uint64_t nrows, uint32_t B, uint32_t SU, uint32_t nw) {
+    const uint32_t VW = B < 4 ? B : 4, LPR = B / VW, RPW = 64 / LPR;
+    const uint64_t per_wave = (nnz / RPW + (nrows / RPW + 1) * FIN) / std::max<uint32_t>(nw, 1) + 1;
+    const uint64_t t = std::max<uint64_t>(64, std::min<uint64_t>(per_wave, 4096));
+    return (uint32_t)((t + SU - 1) / SU * SU);
+}
+
 rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> &rp,
-                              const std::vector<uint32_t> &colw, const std::vector<uint32_t> &vals, bool packed,
-                              uint32_t B, uint32_t SU, uint32_t nw, const uint32_t *scale, uint64_t in_rows) {
+                       const std::vector<uint32_t> &colw, const std::vector<uint32_t> &vals, bool packed, uint32_t B,
+                       uint32_t SU, uint32_t nw, const uint32_t *scale, uint64_t in_rows, const DevOperand *dev,
+                       hipStream_t st) {
     const uint32_t VW = B < 4 ? B : 4, LPR = B / VW, RPW = 64 / LPR, BW = RPW * SU;
     const uint64_t nnz = rp[nrows];
-    constexpr uint32_t FIN = 8;  // cost of producing a slice, in steps
     auto up = [](uint64_t x, uint64_t m) { return (x + m - 1) / m * m; };
-    const uint64_t per_wave = (nnz / RPW + (nrows / RPW + 1) * FIN) / std::max<uint32_t>(nw, 1) + 1;
-    const uint32_t long_row = (uint32_t)up(std::max<uint64_t>(64, std::min<uint64_t>(per_wave, 4096)), SU);
+    const uint32_t long_row = long_row_threshold(nnz, nrows, B, SU, nw);
     const uint32_t chunk = (uint32_t)up(std::max<uint32_t>(16, long_row / 2), SU);
     M.nrows = (uint32_t)nrows;
 
+    // short rows by length, longest first, in row order among equal lengths
+    // (a counting sort: lengths are at most long_row)
     std::vector<uint32_t> shortr, lrows;
     std::vector<uint4> linfo;
-    for (uint64_t r = 0; r < nrows; r++) {
-        if (rp[r + 1] - rp[r] > long_row) lrows.push_back((uint32_t)r);
-        else shortr.push_back((uint32_t)r);
+    {
+        std::vector<uint64_t> start((size_t)long_row + 2, 0);
+        for (uint64_t r = 0; r < nrows; r++) {
+            const uint32_t len = rp[r + 1] - rp[r];
+            if (len > long_row) lrows.push_back((uint32_t)r);
+            else start[long_row - len + 1]++;
+        }
+        for (size_t i = 1; i < start.size(); i++) start[i] += start[i - 1];
+        shortr.resize(nrows - lrows.size());
+        for (uint64_t r = 0; r < nrows; r++) {
+            const uint32_t len = rp[r + 1] - rp[r];
+            if (len <= long_row) shortr[start[long_row - len]++] = (uint32_t)r;
+        }
     }
-    std::stable_sort(shortr.begin(), shortr.end(),
-                     [&](uint32_t x, uint32_t y) { return rp[x + 1] - rp[x] > rp[y + 1] - rp[y]; });
+    RR_CHECK(!dev || lrows.empty(), RR_EINVAL, "rows to split need the host layout");
     struct Sl {
         uint32_t nblk;
         uint32_t seg;    // accumulator row of a segment, or NO_ROW for a row slice
@@ -894,9 +915,40 @@ rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> 
     wsl[nw] = (uint32_t)meta.size();
     wblk[nw] = (uint32_t)blocks;
 
-    std::vector<uint32_t> cw((size_t)blocks * BW, 0u), vw(packed ? 0 : (size_t)blocks * BW, 0u);
+    M.nw = nw;
+    M.nslices = (uint32_t)sls.size();
+    M.nseg = nseg;
+    M.nlong = (uint32_t)lrows.size();
+    M.blocks = blocks;
+    rr_status s;
+    if (dev) {  // the device copies the nonzeros; the host sends where each slice goes
+        std::vector<uint2> sl;
+        sl.reserve(sls.size());
+        uint64_t bk = 0;
+        for (uint32_t w = 0; w < nw; w++) {
+            for (uint32_t i : mine[w]) {
+                sl.push_back(make_uint2(sls[i].first, (uint32_t)bk));
+                bk += sls[i].nblk;
+            }
+            bk += 2;
+        }
+        DevBuf d_short, d_sl;
+        if ((s = upload(d_short, shortr.data(), shortr.size())) != RR_OK) return s;
+        if ((s = upload(d_sl, sl.data(), sl.size())) != RR_OK) return s;
+        if ((s = M.col.alloc((size_t)blocks * BW * 4)) != RR_OK) return s;
+        if (!packed && (s = M.val.alloc((size_t)blocks * BW * 4)) != RR_OK) return s;
+        RR_HIP(hipMemsetAsync(M.col.ptr, 0, (size_t)blocks * BW * 4, st));
+        if (!packed) RR_HIP(hipMemsetAsync(M.val.ptr, 0, (size_t)blocks * BW * 4, st));
+        if (!sl.empty())
+            launch_fill_slices(*dev, d_short.as<uint32_t>(), (uint32_t)shortr.size(), d_sl.as<uint2>(),
+                               (uint32_t)sl.size(), RPW, SU, M.col.as<uint32_t>(), packed ? nullptr : M.val.as<uint32_t>(),
+                               st);
+        RR_HIP(hipGetLastError());
+        RR_HIP(hipStreamSynchronize(st));  // before the slice lists go back to the cache
+    }
+    std::vector<uint32_t> cw(dev ? 0 : (size_t)blocks * BW, 0u), vw(packed || dev ? 0 : (size_t)blocks * BW, 0u);
     size_t blk = 0;
-    for (uint32_t w = 0; w < nw; w++) {
+    for (uint32_t w = 0; w < nw && !dev; w++) {
         for (uint32_t i : mine[w]) {
             const Sl &S = sls[i];
             for (uint32_t g = 0; g < RPW; g++) {
@@ -923,18 +975,12 @@ rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> 
         }
         blk += 2;
     }
-    M.nw = nw;
-    M.nslices = (uint32_t)sls.size();
-    M.nseg = nseg;
-    M.nlong = (uint32_t)lrows.size();
-    M.blocks = blocks;
-    rr_status s;
     if ((s = upload(M.meta, meta.data(), meta.size())) != RR_OK) return s;
     if ((s = upload(M.unit, unit.data(), unit.size())) != RR_OK) return s;
     if ((s = upload(M.wave_sl, wsl.data(), wsl.size())) != RR_OK) return s;
     if ((s = upload(M.wave_blk, wblk.data(), wblk.size())) != RR_OK) return s;
-    if ((s = upload(M.col, cw.data(), cw.size())) != RR_OK) return s;
-    if (!packed && (s = upload(M.val, vw.data(), vw.size())) != RR_OK) return s;
+    if (!dev && (s = upload(M.col, cw.data(), cw.size())) != RR_OK) return s;
+    if (!dev && !packed && (s = upload(M.val, vw.data(), vw.size())) != RR_OK) return s;
     RR_CHECK((uint64_t)nacc * B * 8 < OOB, RR_EINVAL, "too many split rows");
     if ((s = M.lacc.alloc((size_t)nacc * B * 8)) != RR_OK) return s;
     if (nacc) RR_HIP(hipMemset(M.lacc.ptr, 0, (size_t)nacc * B * 8));
@@ -1167,6 +1213,7 @@ int rr_device_count(void) {
 rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *row_precond,
                            const uint32_t *col_precond, const uint32_t *curv, uint64_t t_have, rr_krylov **out) {
     clear_error();
+    const auto t_create = std::chrono::steady_clock::now();
     RR_CHECK(A && out && row_precond && col_precond && curv, RR_EINVAL, "null argument");
     *out = nullptr;
     RR_CHECK(p > 2 && p < (1u << 31) && h_is_prime(p), RR_EINVAL, "p=%u must be a prime in (2, 2^31)", p);
@@ -1183,17 +1230,7 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
     RR_CHECK(t_have == 0 || (t_have & 1), RR_EINVAL, "t_have=%llu must be 0 or odd", (unsigned long long)t_have);
     const uint64_t m = A->m, n = A->n;
     RR_CHECK(A->row_ptr && A->row_ptr[0] == 0, RR_EINVAL, "row_ptr[0] must be 0");
-    for (uint64_t r = 0; r < m; r++)
-        RR_CHECK(A->row_ptr[r + 1] >= A->row_ptr[r], RR_EINVAL, "row_ptr not monotone at row %llu",
-                 (unsigned long long)r);
     const uint64_t nnz = A->row_ptr[m];
-    RR_CHECK(nnz < (1ull << 32), RR_EINVAL, "nnz=%llu must be < 2^32", (unsigned long long)nnz);
-    RR_CHECK(nnz == 0 || (A->col && A->val), RR_EINVAL, "null col/val");
-    for (uint64_t e = 0; e < nnz; e++) {
-        RR_CHECK(A->col[e] < n, RR_EINVAL, "col[%llu]=%u >= n", (unsigned long long)e, A->col[e]);
-        RR_CHECK(A->val[e] < p, RR_EINVAL, "val[%llu]=%u >= p", (unsigned long long)e, A->val[e]);
-    }
-    const bool small = csr_packs(A, p);
     for (uint64_t i = 0; i < m; i++)
         RR_CHECK(row_precond[i] >= 1 && row_precond[i] < p, RR_EINVAL, "row_precond[%llu] not in [1,p)",
                  (unsigned long long)i);
@@ -1211,7 +1248,6 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
     c->m = m;
     c->n = n;
     c->nnz = nnz;
-    c->packed = small;
     c->mp = ModP::make(p);
     c->P = b * (b + 1) / 2;
     c->t = t_have;
@@ -1221,25 +1257,27 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
     RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
     c->grid = 0;  // set below from the occupancy of the chosen pass kernel
 
-    std::vector<uint32_t> rp, colw, vals, rpt, colt, valt;
-    build_operands(A, p, small, rp, colw, vals, rpt, colt, valt);
     // preconditioners: D_r, D_c, D_c^2, D_c^-1
     std::vector<uint32_t> dc2(n), dcinv(n);
     for (uint64_t i = 0; i < n; i++) {
         dc2[i] = h_mulmod(col_precond[i], col_precond[i], p);
         dcinv[i] = h_invmod(col_precond[i], p);
     }
-    c->su = choose_su(rp, rpt);
     c->sep_proj = c->B == 64;
-    // one resident wave of workgroups; every wavefront owns a fixed stream of slices
-    c->grid = (uint32_t)std::max(1, cus * spmm_blocks_per_cu(c->B, small, c->su, !c->sep_proj));
-    rr_status s;
-    if ((s = build_slices(c->A, m, rp, colw, vals, small, c->B, c->su, c->grid * (WG / 64), row_precond, n)) !=
-        RR_OK)
-        return s;
-    if ((s = build_slices(c->At, n, rpt, colt, valt, small, c->B, c->su, c->grid * (WG / 64), dc2.data(), m)) !=
-        RR_OK)
-        return s;
+    // A and A^T in sliced form for one resident wave of workgroups (every
+    // wavefront owns a fixed stream of slices); checked and laid out on the device
+    OperandInfo oi;
+    rr_status s = setup_operands(
+        A, p, c->B, 0,
+        [&](bool packed, uint32_t su) {
+            c->grid = (uint32_t)std::max(1, cus * spmm_blocks_per_cu(c->B, packed, su, !c->sep_proj));
+            return c->grid * (uint32_t)(WG / 64);
+        },
+        row_precond, dc2.data(), c->A, c->At, &oi, c->stream);
+    if (s != RR_OK) return s;
+    c->packed = oi.packed;
+    c->su = oi.su;
+    c->on_device_setup = oi.on_device;
     if ((s = upload(c->dr, row_precond, m)) != RR_OK) return s;
     if ((s = upload(c->dc, col_precond, n)) != RR_OK) return s;
     if ((s = upload(c->dc2, dc2.data(), n)) != RR_OK) return s;
@@ -1293,6 +1331,8 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
     // algorithmic bytes per pass (int32 columns, u32 values, each array once)
     c->stats.bytes_a = 8 * nnz + 4 * (m + 1) + (uint64_t)4 * b * (n + m);
     c->stats.bytes_at = 8 * nnz + 4 * (n + 1) + (uint64_t)4 * b * (n + m);
+    c->stats.setup_on_device = c->on_device_setup ? 1 : 0;
+    c->stats.ms_setup = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_create).count();
     *out = c.release();
     return RR_OK;
 }
@@ -1315,9 +1355,13 @@ rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
     const uint32_t P = c->P, nparts = c->grid;
     const size_t part_stride = (size_t)nparts * P;
     uint64_t bytes_a = c->stats.bytes_a, bytes_at = c->stats.bytes_at;
+    const double ms_setup = c->stats.ms_setup;
+    const int32_t on_dev = c->stats.setup_on_device;
     c->stats = rr_krylov_stats{};
     c->stats.bytes_a = bytes_a;
     c->stats.bytes_at = bytes_at;
+    c->stats.ms_setup = ms_setup;
+    c->stats.setup_on_device = on_dev;
 
     // event triples per pass when profiling: [start, after pass, after pass]
     std::vector<hipEvent_t> ev;

@@ -8,6 +8,7 @@
 // which job each hardware workgroup runs, and on which XCD.
 #pragma once
 #include "common.hpp"
+#include <functional>
 #include <vector>
 
 namespace rr {
@@ -89,11 +90,48 @@ uint64_t padded_slots(const std::vector<uint32_t> &rp, const std::vector<uint32_
 // Column block size (3 or 4) that pads the rows of both operands least.
 uint32_t choose_su(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &rpt);
 
+// One operand's CSR arrays on the device: row pointers (nrows + 1), column
+// words (packed or plain) and plain values (null when packed).
+struct DevOperand {
+    const uint32_t *rp, *colw, *vals;
+};
+
+// Rows of an operand longer than this are cut into segments (split rows).
+uint32_t long_row_threshold(uint64_t nnz, uint64_t nrows, uint32_t B, uint32_t SU, uint32_t nw);
+
 // Lays out one operand for nw wavefronts (nw a multiple of 4); in_rows: rows
-// of the vector it gathers.
+// of the vector it gathers.  With dev (an operand with no row above the
+// threshold), the host plans the slices from the row lengths in rp alone
+// and the device copies the nonzeros from dev's arrays on stream st (colw
+// and vals are then not read).
 rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> &rp,
                        const std::vector<uint32_t> &colw, const std::vector<uint32_t> &vals, bool packed, uint32_t B,
-                       uint32_t SU, uint32_t nw, const uint32_t *scale, uint64_t in_rows);
+                       uint32_t SU, uint32_t nw, const uint32_t *scale, uint64_t in_rows,
+                       const DevOperand *dev = nullptr, hipStream_t st = nullptr);
+
+// Device part of build_slices: the column blocks of row slices sl[s] =
+// {first row in the length order, first block}.
+void launch_fill_slices(const DevOperand &op, const uint32_t *shortr, uint32_t nshort, const uint2 *sl, uint32_t nsl,
+                        uint32_t RPW, uint32_t SU, uint32_t *cw, uint32_t *vw, hipStream_t st);
+
+// Wavefronts to lay the operands out for, given the value kind and column
+// block size the matrix gets.
+typedef std::function<uint32_t(bool packed, uint32_t su)> NwOf;
+struct OperandInfo {
+    bool packed = false;     // every value a signed residue in [-8, 7]
+    uint32_t su = 4, nw = 0;
+    bool on_device = false;  // laid out by the device path
+};
+
+// Validates A (row_ptr monotone, columns < n, values < p) and lays out A
+// (scaled by row_scale) and A^T (scaled by col_scale2) in sliced form.  The
+// entries are uploaded once; the checks, the transpose and the column-block
+// streams run on the device on stream st, and the host plans the slices from
+// row lengths; a matrix with rows to cut into segments is laid out on the
+// host.  su_req: 3 or 4, or 0 for the size that pads least.
+rr_status setup_operands(const rr_csr *A, uint32_t p, uint32_t B, uint32_t su_req, const NwOf &nw_of,
+                         const uint32_t *row_scale, const uint32_t *col_scale2, SliceMat &sa, SliceMat &sat,
+                         OperandInfo *info, hipStream_t st);
 
 SpmmArgs spmm_args(const SliceMat &M, const uint32_t *in, uint64_t in_rows, uint32_t B, uint32_t *out,
                    const uint32_t *scale, uint32_t p, uint32_t b);

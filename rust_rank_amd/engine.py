@@ -33,7 +33,8 @@ class _Csr(C.Structure):
 class KrylovStats(C.Structure):
     _fields_ = [("passes_a", C.c_uint64), ("passes_at", C.c_uint64), ("ms_a", C.c_double), ("ms_at", C.c_double),
                 ("ms_reduce", C.c_double), ("ms_total", C.c_double),
-                ("bytes_a", C.c_uint64), ("bytes_at", C.c_uint64), ("ms_proj", C.c_double)]
+                ("bytes_a", C.c_uint64), ("bytes_at", C.c_uint64), ("ms_proj", C.c_double),
+                ("ms_setup", C.c_double), ("setup_on_device", C.c_int32), ("reserved", C.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

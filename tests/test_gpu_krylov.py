@@ -26,9 +26,11 @@ def _case(A, p, b, seed, L, t_have=0):
     with eng.Krylov(A, p, b, dr, dc, V) as K:
         got = K.run(L)
         curv, t = K.state()
+        on_device = K.stats()["setup_on_device"]
     assert t == L
     assert np.array_equal(got, want), np.argwhere(got != want)[:5]
     assert np.array_equal(curv, x_want)
+    return on_device
 
 
 @pytest.mark.parametrize("b", [1, 2, 3, 4, 8, 16, 32, 48, 64])
@@ -49,10 +51,33 @@ def test_small_prime():
 
 
 def test_long_rows_split():
-    """A^T rows far above the split threshold (Zipf head): segments finished in the pass."""
+    """A^T rows far above the split threshold (Zipf head): segments finished
+    in the pass; such a matrix is laid out on the host."""
     A = synth.graph_complex(seed=5, m=40000, n=2000, k=12)
     assert int(np.bincount(A.col).max()) > 4096
-    _case(A, P, 8, 5, 12)
+    assert _case(A, P, 8, 5, 12) == 0
+
+
+@pytest.mark.parametrize("b,small", [(4, True), (16, False), (64, False)])
+def test_device_setup_edge_cases(b, small):
+    """Set-up on the device (entry checks, transpose, column-block streams):
+    empty rows and columns, repeated columns within a row, rows of every
+    length up to 40, packed and general values; the sequence and the state
+    equal the oracle's."""
+    rng = np.random.default_rng(b)
+    m, n = 700, 500
+    lens = rng.integers(0, 41, m)
+    lens[rng.choice(m, 60, replace=False)] = 0  # empty rows
+    live = rng.choice(n, n - 50, replace=False)  # 50 empty columns
+    cols = [np.sort(rng.choice(live, k)) for k in lens]  # with repeats
+    col = np.concatenate(cols).astype(np.uint32)
+    if small:
+        val = rng.choice(np.array([1, 2, 7, P - 1, P - 2, P - 8], np.uint32), len(col))
+    else:
+        val = rng.integers(1, P, len(col), dtype=np.uint64).astype(np.uint32)
+    row_ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    A = synth.Csr(m, n, row_ptr, col, val)
+    assert _case(A, P, b, 40 + b, 30) == 1
 
 
 def test_long_rows_split_b64():
@@ -116,10 +141,20 @@ def test_bad_inputs_raise():
     with pytest.raises(eng.RankEngineError):
         eng.Krylov(A, 2013265920, 4, dr, dc, V)  # not prime
     bad = A.val.copy()
-    bad[0] = P
+    bad[7] = P
     B = synth.Csr(A.m, A.n, A.row_ptr, A.col, bad)
-    with pytest.raises(eng.RankEngineError):
+    with pytest.raises(eng.RankEngineError, match=r"val\[7\]"):
         eng.Krylov(B, P, 4, dr, dc, V)
+    badc = A.col.copy()
+    badc[-1] = A.n
+    C = synth.Csr(A.m, A.n, A.row_ptr, badc, A.val)
+    with pytest.raises(eng.RankEngineError, match=r"col\[%d\]" % (A.nnz - 1)):
+        eng.Krylov(C, P, 4, dr, dc, V)
+    rp = A.row_ptr.copy()
+    rp[5], rp[6] = rp[6], rp[5] - 1
+    D = synth.Csr(A.m, A.n, rp, A.col, A.val)
+    with pytest.raises(eng.RankEngineError):
+        eng.Krylov(D, P, 4, dr, dc, V)
     with pytest.raises(eng.RankEngineError):
         eng.Krylov(A, P, 4, dr, dc, V, t_have=2)
 
