@@ -350,7 +350,7 @@ def main():
     ap.add_argument("--rate-frac", type=float, default=0.7, help="served offered load / saturated capacity")
     ap.add_argument("--no-plain", action="store_true",
                     help="batched call only: skip the plain solves and the served form (for kernel traces)")
-    ap.add_argument("--workload", choices=["large", "graph_complex", "batched"], default="large",
+    ap.add_argument("--workload", choices=["large", "graph_complex", "batched", "served"], default="large",
                     help="graph_complex / batched: only that leg (for kernel traces and counters)")
     ap.add_argument("--full-rank", action="store_true", help="only the large matrix's rank end to end")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
@@ -374,6 +374,9 @@ def main():
         return
     if args.workload == "batched":
         print(json.dumps(batched(engine, synth, p, args.batched or 512, plain=not args.no_plain)), flush=True)
+        return
+    if args.workload == "served":
+        print(json.dumps(served(engine, synth, p, args.served or 256, args.rate_frac, args.served_rows_hi)), flush=True)
         return
 
     b = 64
