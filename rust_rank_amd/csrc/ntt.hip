@@ -40,7 +40,7 @@ __host__ __device__ constexpr uint32_t ntt_per_wg(uint32_t N) { return N >= NTT_
 // butterflies past `total` repeat the thread's first one and are not stored.
 template <bool DIF>
 __device__ __forceinline__ void lds_stage(uint32_t *s, uint32_t total, uint32_t logn, uint32_t lh, const uint32_t *tw,
-                                          const uint32_t *twq, uint32_t tstride, uint32_t p, uint32_t ps) {
+                                          const uint32_t *twq, uint32_t tstride, uint32_t p, uint32_t ps, uint32_t tsh) {
     const uint32_t h = 1u << lh, N = 1u << logn, half = N >> 1;
     for (uint32_t t0 = threadIdx.x; t0 < total; t0 += NTT_THREADS * NTT_U) {
         uint32_t i0[NTT_U], x0[NTT_U], x1[NTT_U], w[NTT_U], wq[NTT_U];
@@ -53,8 +53,9 @@ __device__ __forceinline__ void lds_stage(uint32_t *s, uint32_t total, uint32_t 
             i0[u] = poly * ps + blk * 2 * h + j;
             x0[u] = s[i0[u]];
             x1[u] = s[i0[u] + h];
-            w[u] = tw[j * tstride];
-            wq[u] = twq[j * tstride];
+            const uint32_t e = j * tstride;
+            w[u] = tw[e + (e >> tsh)];
+            wq[u] = twq[e + (e >> tsh)];
         }
 #pragma unroll
         for (uint32_t u = 0; u < NTT_U; u++) {
@@ -85,7 +86,7 @@ __device__ __forceinline__ void lds_stage(uint32_t *s, uint32_t total, uint32_t 
 template <bool DIF>
 __device__ __forceinline__ void lds_stage4(uint32_t *s, uint32_t total4, uint32_t logn, uint32_t lg,
                                            const uint32_t *tw, const uint32_t *twq, uint32_t tstride4, uint32_t p,
-                                           uint32_t ps) {
+                                           uint32_t ps, uint32_t tsh) {
     constexpr uint32_t U4 = 2;  // radix-4 butterflies per thread in flight
     const uint32_t g = 1u << lg, quarter = 1u << (logn - 2);
     for (uint32_t t0 = threadIdx.x; t0 < total4; t0 += NTT_THREADS * U4) {
@@ -103,8 +104,8 @@ __device__ __forceinline__ void lds_stage4(uint32_t *s, uint32_t total4, uint32_
             const uint32_t e[3] = {j * tstride4, (j + g) * tstride4, 2 * j * tstride4};
 #pragma unroll
             for (uint32_t k = 0; k < 3; k++) {
-                w[u][k] = tw[e[k]];
-                wq[u][k] = twq[e[k]];
+                w[u][k] = tw[e[k] + (e[k] >> tsh)];
+                wq[u][k] = twq[e[k] + (e[k] >> tsh)];
             }
         }
 #pragma unroll
@@ -143,26 +144,32 @@ __device__ __forceinline__ void lds_stage4(uint32_t *s, uint32_t total4, uint32_
 // In-LDS DIF over `npoly` polynomials of length N = 2^logn, ps words apart
 // (N, or N + 1 where a transposed fill must avoid bank conflicts): radix-4
 // stages from the largest spans down, one radix-2 stage of span 1 when logn
-// is odd.
+// is odd.  Twiddle k of the table is word k + (k >> tsh): tsh = TW_PAD for
+// the tables staged in LDS (one spare word per 64, so the power-of-two
+// strides of the middle stages spread over the banks), 31 for global ones.
 __device__ void lds_dif(uint32_t *s, uint32_t npoly, uint32_t logn, const uint32_t *tw, const uint32_t *twq,
-                        uint32_t tw_logn, uint32_t p, uint32_t ps) {
+                        uint32_t tw_logn, uint32_t p, uint32_t ps, uint32_t tsh) {
     const uint32_t total = npoly << (logn - 1);
     uint32_t lh = logn;  // spans 2^(lh-1) and 2^(lh-2) next; w_{2h}^j = w_Ntab^(j * Ntab / 2h)
     for (; lh >= 2; lh -= 2)
-        lds_stage4<true>(s, npoly << (logn - 2), logn, lh - 2, tw, twq, 1u << (tw_logn - lh), p, ps);
-    if (lh == 1) lds_stage<true>(s, total, logn, 0, tw, twq, 1u << (tw_logn - 1), p, ps);
+        lds_stage4<true>(s, npoly << (logn - 2), logn, lh - 2, tw, twq, 1u << (tw_logn - lh), p, ps, tsh);
+    if (lh == 1) lds_stage<true>(s, total, logn, 0, tw, twq, 1u << (tw_logn - 1), p, ps, tsh);
 }
 
 // In-LDS DIT (inverse twiddles in tw), bit-reversed in, natural out, unscaled:
 // radix-4 stages from span 1 up, one radix-2 stage of span N/2 when logn is odd.
 __device__ void lds_dit(uint32_t *s, uint32_t npoly, uint32_t logn, const uint32_t *tw, const uint32_t *twq,
-                        uint32_t tw_logn, uint32_t p, uint32_t ps) {
+                        uint32_t tw_logn, uint32_t p, uint32_t ps, uint32_t tsh) {
     const uint32_t total = npoly << (logn - 1);
     uint32_t lh = 0;  // spans 2^lh and 2^(lh+1) next
     for (; lh + 2 <= logn; lh += 2)
-        lds_stage4<false>(s, npoly << (logn - 2), logn, lh, tw, twq, 1u << (tw_logn - lh - 2), p, ps);
-    if (lh < logn) lds_stage<false>(s, total, logn, lh, tw, twq, 1u << (tw_logn - lh - 1), p, ps);
+        lds_stage4<false>(s, npoly << (logn - 2), logn, lh, tw, twq, 1u << (tw_logn - lh - 2), p, ps, tsh);
+    if (lh < logn) lds_stage<false>(s, total, logn, lh, tw, twq, 1u << (tw_logn - lh - 1), p, ps, tsh);
 }
+
+constexpr uint32_t TW_PAD = 6;  // one spare word per 64 in LDS twiddle tables
+constexpr uint32_t TW_GLOBAL = 31;
+__host__ __device__ constexpr uint32_t tw_lds_words(uint32_t n) { return n + (n >> TW_PAD); }
 
 constexpr uint32_t NTT_LD = 8;  // global loads per thread in flight when staging
 
@@ -195,14 +202,14 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_fwd(NttFwdPack pk, uint32_t
         }
     }
     // the N-point twiddles w_N^k, k < N/2, into LDS behind the data
-    uint32_t *tws = s + max(N, NTT_WG_WORDS), *twsq = tws + (N >> 1);
+    uint32_t *tws = s + max(N, NTT_WG_WORDS), *twsq = tws + tw_lds_words(N >> 1);
     const uint32_t tstep = 1u << (tw_logn - logn);
     for (uint32_t k = threadIdx.x; k < (N >> 1); k += NTT_THREADS) {
-        tws[k] = tw[k * tstep];
-        twsq[k] = twq[k * tstep];
+        tws[k + (k >> TW_PAD)] = tw[k * tstep];
+        twsq[k + (k >> TW_PAD)] = twq[k * tstep];
     }
     __syncthreads();
-    lds_dif(s, np, logn, tws, twsq, logn, p, N);
+    lds_dif(s, np, logn, tws, twsq, logn, p, N, TW_PAD);
     uint32_t *dst = D.dst + (size_t)q0 * N;
     for (uint32_t i0 = threadIdx.x; i0 < words; i0 += NTT_THREADS * NTT_LD) {
         uint32_t v[NTT_LD];
@@ -233,14 +240,14 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_inv(NttInvPack pk, uint32_t
         for (uint32_t e = 0; e < NTT_LD; e++)
             if (i0 + e * NTT_THREADS < words) s[i0 + e * NTT_THREADS] = v[e];
     }
-    uint32_t *tws = s + max(N, NTT_WG_WORDS), *twsq = tws + (N >> 1);
+    uint32_t *tws = s + max(N, NTT_WG_WORDS), *twsq = tws + tw_lds_words(N >> 1);
     const uint32_t tstep = 1u << (tw_logn - logn);
     for (uint32_t k = threadIdx.x; k < (N >> 1); k += NTT_THREADS) {
-        tws[k] = tw[k * tstep];
-        twsq[k] = twq[k * tstep];
+        tws[k + (k >> TW_PAD)] = tw[k * tstep];
+        twsq[k + (k >> TW_PAD)] = twq[k * tstep];
     }
     __syncthreads();
-    lds_dit(s, np, logn, tws, twsq, logn, p, N);
+    lds_dit(s, np, logn, tws, twsq, logn, p, N, TW_PAD);
     const uint32_t outs = np * D.outlen;
     for (uint32_t i0 = threadIdx.x; i0 < outs; i0 += NTT_THREADS * NTT_LD) {
         uint32_t v[NTT_LD];
@@ -309,7 +316,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_cols_fwd(NttFwdPack pk, uin
         }
     }
     __syncthreads();
-    lds_dif(s, TC, l2, tw, twq, tw_logn, p, ps);
+    lds_dif(s, TC, l2, tw, twq, tw_logn, p, ps, TW_GLOBAL);
     uint32_t *dst = D.dst + ((size_t)q << logn);
     for (uint32_t i = threadIdx.x; i < words; i += NTT_THREADS) {
         const uint32_t c = i & (TC - 1), j = i >> ltc, n1 = c0 + c;
@@ -335,7 +342,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_cols_inv(NttInvPack pk, uin
         s[c * ps + j] = tw_mul(src[n1 + N1 * j], n1 * k2, logn, tw, tw_logn, p, inv);  // tw holds w^-k here
     }
     __syncthreads();
-    lds_dit(s, TC, l2, tw, twq, tw_logn, p, ps);
+    lds_dit(s, TC, l2, tw, twq, tw_logn, p, ps, TW_GLOBAL);
     uint32_t *dst = D.dst + (size_t)q * D.dstride;
     for (uint32_t i = threadIdx.x; i < words; i += NTT_THREADS) {
         const uint32_t c = i & (TC - 1), n2 = i >> ltc, n = c0 + c + N1 * n2;
@@ -524,7 +531,9 @@ static inline uint32_t blocks_for(uint32_t count, uint32_t logn) {
 }
 
 // data (one polynomial, or several sharing NTT_WG_WORDS) and the N/2 twiddle pairs
-static inline size_t lds_for(uint32_t logn) { return ((size_t)std::max(1u << logn, NTT_WG_WORDS) + (1u << logn)) * 4; }
+static inline size_t lds_for(uint32_t logn) {
+    return ((size_t)std::max(1u << logn, NTT_WG_WORDS) + 2 * tw_lds_words(1u << (logn - 1))) * 4;
+}
 
 // Dynamic LDS above 64 KB needs the kernels' attribute raised (once per process).
 static rr_status ntt_lds_attrs() {
