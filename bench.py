@@ -338,7 +338,7 @@ def main():
     ap.add_argument("--steps", type=int, default=0, help="timed iterations (default 200)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-timed", type=int, default=6, help="timed oracle iterations for the CPU baseline")
+    ap.add_argument("--cpu-timed", type=int, default=16, help="timed oracle iterations for the CPU baseline")
     ap.add_argument("--no-graph-complex", action="store_true", help="skip the 200000 x 180000 workload")
     ap.add_argument("--no-pm", action="store_true", help="graph_complex: skip its PM-basis and rank")
     ap.add_argument("--pm-order", type=int, default=0,
