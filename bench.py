@@ -234,11 +234,13 @@ def _serve(eng, p, reqs, arrivals):
         while len(results) < n:
             for r in srv.poll(wait_ms=50):
                 results[r["id"]] = r
+                if len(results) % 16 == 0:
+                    log(f"  served {len(results)}/{n} ({time.perf_counter() - t0:.1f} s)")
         done.set()
 
+    t0 = time.perf_counter()
     th = threading.Thread(target=poller)
     th.start()
-    t0 = time.perf_counter()
     ids = []
     for (A, b, dr, dc, V), a in zip(reqs, arrivals):
         delay = t0 + a - time.perf_counter()
@@ -290,6 +292,8 @@ def served(eng, synth, p, nreq, rate_frac, rows_hi, seed=5):
         fin = time.perf_counter()
         service += fin - s0
         fifo.append((fin - t1 - a) * 1e3)
+        if len(fifo) % 16 == 0:
+            log(f"  FIFO plain {len(fifo)}/{nreq} ({fin - t1:.1f} s)")
     fifo = np.array(fifo)
     wall_fifo = time.perf_counter() - t1
     assert r_sat == plain_rank and r_off == plain_rank, "served ranks differ from plain ranks"
