@@ -120,10 +120,13 @@ void rr_krylov_destroy(rr_krylov *ctx);
  * exactly mod p.  Unlike the reference, d is not rounded up to a power of two:
  * any 1 <= d < L is accepted (for a power of two the result is the
  * reference's).  basis_out: [2b][2b][d+1] (coefficients innermost);
- * delta_out: 2b degree shifts.  d + 1 <= 2^20.  Short products are direct;
- * longer ones use NTTs, mod p itself for p = 2013265921, and for any other
- * prime p < 2^31 mod the three primes 2013265921, 469762049 and 754974721,
- * recombined exactly (Garner) and reduced mod p.
+ * delta_out: 2b degree shifts.  d + 1 <= 2^26 (the reference's long runs,
+ * orders of 1.7e7 at b = 4, fit).  Short products are direct; longer ones use
+ * NTTs, mod p itself for p = 2013265921, and for any other prime p < 2^31 mod
+ * the three primes 2013265921, 469762049 and 1811939329, recombined exactly
+ * (Garner) and reduced mod p (RR_EINVAL when a product's exact coefficients
+ * could exceed their product, about 2^90.5: large p, 2b = 128 and above
+ * about 2^21 terms).
  */
 rr_status rr_pm_basis(const uint32_t *seq, uint32_t b, uint64_t L, uint64_t d, uint32_t p, uint32_t *basis_out,
                       int64_t *delta_out);
@@ -141,7 +144,7 @@ rr_status rr_rank_from_basis(const uint32_t *basis, const int64_t *delta, uint32
 /*
  * Replaces ntt / matrix_ntt_parallel (src/ntt.rs:164-211, :263-269) for
  * p = 2013265921: `count` transforms of 2^logn points in place, natural order,
- * forward root 31^((p-1)/n), inverse scaled by 1/n.  logn <= 20 (above 14: four steps through device memory).
+ * forward root 31^((p-1)/n), inverse scaled by 1/n.  logn <= 26 (above 14: four steps through device memory).
  */
 rr_status rr_ntt(uint32_t *data, uint32_t logn, uint32_t count, int inverse);
 
@@ -167,7 +170,7 @@ rr_status rr_ntt_noalloc(uint32_t *data, uint32_t logn, uint32_t count, int inve
  * prime 2 < p < 2^31 and entries < p (RR_EINVAL otherwise).  Direct products
  * when short, else NTTs: mod p for p = 2013265921, mod three NTT primes with
  * Garner recombination for any other p.  inner <= 128, hi <= la + lb - 1,
- * transforms up to 2^20 points.
+ * transforms up to 2^26 points.
  */
 rr_status rr_poly_mat_mul(const uint32_t *A, uint64_t la, const uint32_t *B, uint64_t lb, uint32_t rows,
                           uint32_t inner, uint32_t cols, uint64_t lo, uint64_t hi, uint32_t p, uint32_t *C);
@@ -327,7 +330,8 @@ typedef struct {
 } rr_server_stats;
 
 rr_status rr_server_create(uint32_t p, uint64_t hbm_budget_bytes, rr_server **out);
-/* Copies the inputs; seq_len / order 0 select the exact policy. */
+/* Copies the inputs; seq_len / order 0 select the exact policy.  The order
+ * must be below 2^20 (RR_EINVAL otherwise; rr_rank takes longer ones). */
 rr_status rr_server_submit(rr_server *s, const rr_csr *A, uint32_t b, const uint32_t *row_precond,
                            const uint32_t *col_precond, const uint32_t *V, uint64_t seq_len, uint64_t order,
                            uint64_t *id_out);

@@ -45,6 +45,10 @@ constexpr uint32_t SERVER_TICKS_IDLE = 16;
 // Chunks and PM-basis rounds the server keeps queued ahead of the device.
 constexpr uint32_t SERVER_K_AHEAD = 2;
 constexpr uint32_t SERVER_PM_AHEAD = 8;
+// The server's PM-basis scheduler keeps twiddle tables for transforms of up to
+// 2^SERVER_MAX_LOGN points (orders below 2^20); longer requests are refused
+// at submission (rr_rank and rr_batch_rank take orders up to 2^26 - 1).
+constexpr uint32_t SERVER_MAX_LOGN = 20;
 // The batched and served modes start the PM-basis of a request of at least
 // this order while its sequence is still being produced: the leftmost chain
 // of the recursion reads only leading coefficients of G.

@@ -265,15 +265,17 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_inv(NttInvPack pk, uint32_t
 }
 
 // ---------------------------------------------------------------------------
-// Transforms above 2^14 points (up to 2^20): four steps over N = N1 N2, with
-// element n = n1 + N1 n2, N2 = 2^ceil(logn/2) and N1 = 2^floor(logn/2).
+// Transforms above 2^14 points (up to 2^26): four steps over N = N1 N2, with
+// element n = n1 + N1 n2; N2 = 2^ceil(logn/2) and N1 = 2^floor(logn/2) up to
+// 2^20 points, above that N1 = 2^14 (the rows stay one-workgroup transforms)
+// and N2 = N / 2^14.
 // Forward: N2-point DIF along n2 for each n1 (column kernel, a tile of TC
 // columns transposed into LDS), the twiddle w_N^(n1 k2), then N1-point DIF
 // along n1 for each k2 (rows, contiguous, the ordinary kernel in place).
 // Spectrum entry N1 j + j1 holds X[brev(j) + N2 brev(j1)]; the inverse takes
 // the same order back (rows, then the conjugate twiddle, then columns), so
 // a product never permutes.
-__host__ __device__ inline uint32_t big_l2(uint32_t logn) { return (logn + 1) / 2; }
+__host__ __device__ inline uint32_t big_l2(uint32_t logn) { return logn <= 20 ? (logn + 1) / 2 : logn - NTT_LDS_LOGN; }
 __host__ __device__ inline uint32_t big_ltc(uint32_t logn) {  // log2 of the columns per workgroup
     const uint32_t l1 = logn - big_l2(logn), byl = 13 - big_l2(logn);  // (N2 + 1) TC words, about 32 KB
     return l1 < byl ? l1 : byl;
@@ -669,7 +671,8 @@ rr_status pointwise(const uint32_t *A, const uint32_t *B, uint32_t *C, uint32_t 
 
 namespace rr {
 
-constexpr uint32_t NA_TW_WORDS = 4u << (NTT_MAX_LOGN - 1);  // the four twiddle tables of 2^20-point transforms
+constexpr uint32_t NA_MAX_LOGN = 20;                      // transforms of up to 2^20 points
+constexpr uint32_t NA_TW_WORDS = 4u << (NA_MAX_LOGN - 1);  // the four twiddle tables of 2^20-point transforms
 constexpr uint32_t NA_MAX_WORDS = 1u << 22;                // largest count x 2^logn (16 MiB)
 // This device's work buffer: twiddle tables (8 MiB), then the reordered
 // input or the spectrum (16 MiB).  Host code uses it only through the
@@ -723,7 +726,7 @@ __global__ void k_bitrev_scale(const uint32_t *src, uint32_t *dst, uint32_t logn
 extern "C" rr_status rr_ntt_noalloc(uint32_t *data, uint32_t logn, uint32_t count, int inverse, void *stream) {
     using namespace rr;
     clear_error();
-    RR_CHECK(data && logn >= 1 && logn <= NTT_MAX_LOGN && count >= 1, RR_EINVAL, "bad NTT arguments");
+    RR_CHECK(data && logn >= 1 && logn <= NA_MAX_LOGN && count >= 1, RR_EINVAL, "bad NTT arguments");
     RR_CHECK(((uint64_t)count << logn) <= NA_MAX_WORDS, RR_EINVAL,
              "count x 2^logn = %llu words exceeds the %u-word work buffer",
              (unsigned long long)((uint64_t)count << logn), NA_MAX_WORDS);
@@ -731,7 +734,7 @@ extern "C" rr_status rr_ntt_noalloc(uint32_t *data, uint32_t logn, uint32_t coun
     int dev = 0;
     RR_HIP(hipGetDevice(&dev));
     RR_CHECK(dev >= 0 && dev < NA_MAX_DEV, RR_EINVAL, "device %d out of range", dev);
-    const uint32_t p = NTT_PRIME, half = 1u << (NTT_MAX_LOGN - 1);
+    const uint32_t p = NTT_PRIME, half = 1u << (NA_MAX_LOGN - 1);
     const ModP mp = ModP::make(p);
     NoAllocDev &D = g_na[dev];
     {
@@ -746,7 +749,7 @@ extern "C" rr_status rr_ntt_noalloc(uint32_t *data, uint32_t logn, uint32_t coun
                 RR_HIP(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)));
                 D.ptr = (uint32_t *)addr;
             }
-            const uint32_t w = h_powmod(NTT_ROOT, (p - 1) >> NTT_MAX_LOGN, p), wi = h_invmod(w, p);
+            const uint32_t w = h_powmod(NTT_ROOT, (p - 1) >> NA_MAX_LOGN, p), wi = h_invmod(w, p);
             hipLaunchKernelGGL(k_ntt_twiddles, dim3((half + 255) / 256), dim3(256), 0, st, D.ptr, D.ptr + half,
                                D.ptr + 2 * half, D.ptr + 3 * half, half, w, wi, p, mp.inv);
             RR_HIP(hipGetLastError());
@@ -755,7 +758,7 @@ extern "C" rr_status rr_ntt_noalloc(uint32_t *data, uint32_t logn, uint32_t coun
         }
     }
     NttTables T;  // a view of the tables in scratch (its own buffers stay empty)
-    T.logn = NTT_MAX_LOGN;
+    T.logn = NA_MAX_LOGN;
     T.tf = D.ptr;
     T.tfq = D.ptr + half;
     T.ti = D.ptr + 2 * half;

@@ -8,7 +8,7 @@ namespace rr {
 constexpr uint32_t NTT_PRIME = 2013265921u;  // 15 * 2^27 + 1 (src/ntt.rs:52)
 constexpr uint32_t NTT_ROOT = 31u;           // generator (src/ntt.rs:53)
 constexpr uint32_t NTT_LDS_LOGN = 14;        // one workgroup, one polynomial in 64 KB of LDS
-constexpr uint32_t NTT_MAX_LOGN = 20;        // larger transforms: four steps through global memory
+constexpr uint32_t NTT_MAX_LOGN = 26;        // larger transforms: four steps through global memory
 
 // Twiddle tables for transforms up to 2^logn points: w^k and w^-k for k < 2^(logn-1)
 // with their Shoup companions.
@@ -22,12 +22,14 @@ struct NttTables {
 };
 
 // Products mod a prime p that is not an NTT prime: three NTT primes with
-// 2-adicity >= 24 whose product (about 2^89.2) exceeds every exact
-// coefficient of a product of residues < 2^31 with at most 128 x 2^20 terms
-// (below 2^89); each coefficient is recovered by Garner's method and reduced
-// mod p.  CRT_PRIMES[0] is NTT_PRIME.
-constexpr uint32_t CRT_PRIMES[3] = {2013265921u, 469762049u, 754974721u};  // 15 2^27 + 1, 7 2^26 + 1, 45 2^24 + 1
-constexpr uint32_t CRT_ROOTS[3] = {31u, 3u, 11u};                        // their generators
+// 2-adicity >= 26 (so every transform size up to 2^NTT_MAX_LOGN exists mod
+// each) whose product, about 2^90.47, must exceed every exact coefficient of
+// the product (checked per product: inner x terms x (p - 1)^2; for instance
+// 128 x 2^21 terms of residues < 2^31, or any product of residues of a prime
+// below 2^16 up to 2^26 points); each coefficient is recovered by Garner's
+// method and reduced mod p.  CRT_PRIMES[0] is NTT_PRIME.
+constexpr uint32_t CRT_PRIMES[3] = {2013265921u, 469762049u, 1811939329u};  // 15 2^27 + 1, 7 2^26 + 1, 27 2^26 + 1
+constexpr uint32_t CRT_ROOTS[3] = {31u, 3u, 13u};                         // their generators
 // dst[i] = src[(i / len) * sstride + soff + i % len] mod q, i < count * len
 rr_status mod_copy(const uint32_t *src, uint64_t sstride, uint32_t soff, uint32_t len, uint32_t count, uint32_t q,
                    uint32_t *dst, hipStream_t st);

@@ -1052,8 +1052,9 @@ rr_status PmScheduler::run_ntt_products(std::vector<NttProduct> &ops) {
 // windows to the exact coefficients, which are reduced mod p.  The exact
 // coefficient in the window [lo, hi) sums at most inner min(la, lb, N)
 // products below p^2 (wrapped terms fall below lo, operand coefficients at
-// or beyond N only reach outputs at or beyond N >= hi): below the product of
-// the primes for inner <= 128 and N <= 2^20.
+// or beyond N only reach outputs at or beyond N >= hi), which must stay below
+// the product of the primes (checked here; for example inner <= 128 with
+// 2^21 terms of any p < 2^31, or any inner <= 128 and N <= 2^26 for p < 2^16).
 rr_status PmScheduler::run_crt_products(std::vector<NttProduct> &ops) {
     const unsigned __int128 Q = (unsigned __int128)CRT_PRIMES[0] * CRT_PRIMES[1] * CRT_PRIMES[2];
     size_t words = 0;

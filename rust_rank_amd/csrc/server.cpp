@@ -441,7 +441,7 @@ rr_status rr_server_create(uint32_t p, uint64_t hbm_budget_bytes, rr_server **ou
     RR_HIP(rr::stream_create(&S->st_pm));
     rr_status s;
     if ((s = S->ticks.init(S->st_k, S->cus)) != RR_OK) return s;
-    if ((s = S->pm.init(p, S->st_pm, NTT_MAX_LOGN, 128, 64)) != RR_OK) return s;
+    if ((s = S->pm.init(p, S->st_pm, SERVER_MAX_LOGN, 128, 64)) != RR_OK) return s;
     // RR_SERVER_FAIL_AT=tick:<id> or pm:<id> (tests of failure handling): the
     // ticks or PM-basis rounds holding that request fail before queueing work
     if (const char *f = getenv("RR_SERVER_FAIL_AT")) {
@@ -478,6 +478,9 @@ rr_status rr_server_submit(rr_server *S, const rr_csr *A, uint32_t b, const uint
     q->n = A->n;
     q->L = seq_len ? seq_len : 2 * std::min(A->m, A->n) / b + 8;
     q->d = order ? order : q->L - 1;
+    RR_CHECK(q->d + 1 <= (1ull << SERVER_MAX_LOGN), RR_EINVAL,
+             "order %llu above the server's 2^%u-point transforms (rr_rank takes it)", (unsigned long long)q->d,
+             SERVER_MAX_LOGN);
     q->rp.assign(A->row_ptr, A->row_ptr + A->m + 1);
     q->col.assign(A->col, A->col + nnz);
     q->val.assign(A->val, A->val + nnz);

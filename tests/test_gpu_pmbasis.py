@@ -21,7 +21,7 @@ def _eng():
 
 
 @pytest.mark.parametrize("logn,count", [(1, 3), (4, 5), (6, 100), (10, 2), (10, 5), (11, 3), (14, 1), (12, 7),
-                                        (15, 3), (17, 1), (18, 2)])
+                                        (15, 3), (17, 1), (18, 2), (21, 1), (22, 2)])
 def test_ntt_matches_oracle(logn, count):
     eng = _eng()
     print(f"ntt logn={logn} count={count}", flush=True)
@@ -257,3 +257,93 @@ def test_rank_streamed_equals_sequence_then_basis():
     got, ms = eng.rank(A, P, b, dr, dc, V)
     print(f"m={A.m} n={A.n} nnz={A.nnz} b={b} d={d}: streamed rank {got} in {ms:.0f} ms, two-step {want}", flush=True)
     assert got == want
+
+
+# Transforms above 2^20 points (rows of 2^14 points, columns of the rest).
+# Exact integer convolutions for the checks: residues mod two NTT primes by
+# the oracle's prime-generic NTT, recombined in uint64 (coefficients below
+# 2^59, enough for residues of a prime below 2^16 over up to 2^26 terms).
+_Q1, _G1, _Q2, _G2 = 2013265921, 31, 469762049, 3
+
+
+def _exact_conv_mod(a, b, p, n_out):
+    """(a * b)[0:n_out] mod p for uint32 residues of a small prime p."""
+    N = 1 << (len(a) + len(b) - 1).bit_length()
+    res = []
+    for q, g in ((_Q1, _G1), (_Q2, _G2)):
+        fa = np.zeros(N, np.uint64)
+        fb = np.zeros(N, np.uint64)
+        fa[: len(a)] = a
+        fb[: len(b)] = b
+        c = O.ntt_generic((O.ntt_generic(fa, q, g) * O.ntt_generic(fb, q, g)) % np.uint64(q), q, g, invert=True)
+        res.append(c[:n_out].astype(np.uint64))
+    inv = pow(_Q1, -1, _Q2)
+    t = ((res[1] + np.uint64(_Q2) - res[0] % np.uint64(_Q2)) % np.uint64(_Q2)) * np.uint64(inv) % np.uint64(_Q2)
+    return ((res[0] + np.uint64(_Q1) * t) % np.uint64(p)).astype(np.uint32)
+
+
+@pytest.mark.parametrize("p,la,lb", [(P, 1100000, 1050000), (3323, 1200000, 1000000)])
+def test_poly_mat_mul_above_2_20_points(p, la, lb):
+    """Products whose transforms have 2^22 points: mod the NTT prime itself
+    (against the oracle's product) and mod 3323, the reference's default
+    prime, through the three CRT primes (against an exact convolution)."""
+    eng = _eng()
+    print(f"pmm p={p} la={la} lb={lb}", flush=True)
+    rng = np.random.default_rng(la + p)
+    A = rng.integers(0, p, (1, 2, la)).astype(np.uint32)
+    B = rng.integers(0, p, (2, 1, lb)).astype(np.uint32)
+    lo, hi = 1000, la + lb - 1
+    got = eng.poly_mat_mul(A, B, lo, hi, p)
+    if p == P:
+        want = O.poly_mat_mul(A, B, lo, hi, p)
+    else:
+        want = np.zeros((1, 1, hi - lo), np.uint32)
+        acc = np.zeros(hi, np.uint64)
+        for r in range(2):
+            acc = (acc + _exact_conv_mod(A[0, r], B[r, 0], p, hi)) % np.uint64(p)
+        want[0, 0] = acc[lo:].astype(np.uint32)
+    assert np.array_equal(got, want), np.argwhere(got != want)[:5]
+
+
+@pytest.mark.parametrize("p", [P, 3323])
+def test_pm_basis_order_above_2_20(p):
+    """b = 1, d = 1,100,000: the root products need 2^21-point transforms
+    (and, mod 3323, the CRT primes at that size).  Checked against what fixes
+    the basis: u^T M(x) G(x) = 0 mod x^d (an exact convolution), det M(z) =
+    +-z^(-sum delta), and every row degree bounded by -delta."""
+    eng = _eng()
+    b, d = 1, 1100000
+    print(f"pm_basis b={b} d={d} p={p}", flush=True)
+    rng = np.random.default_rng(d + p)
+    seq = rng.integers(0, p, (d + 2, 1)).astype(np.uint32)
+    M, delta = eng.pm_basis(seq, b, d, p)
+    D = M.shape[2]
+    assert M.shape == (2, 2, d + 1)
+    u = rng.integers(0, p, 2)
+    a = _left_combine(u, M, p)  # [2][D]: u^T M
+    g0 = seq[:D, 0].astype(np.uint32)  # G = [S(x); 1]
+    if p == P:
+        N = 1 << (2 * D - 1).bit_length()
+        fa = np.zeros(N, np.uint32)
+        fb = np.zeros(N, np.uint32)
+        fa[:D] = a[0]
+        fb[:D] = g0
+        c = O.ntt((O.ntt(fa).astype(np.uint64) * O.ntt(fb).astype(np.uint64) % np.uint64(P)).astype(np.uint32),
+                  invert=True)[:d].astype(np.uint64)
+    else:
+        c = _exact_conv_mod(a[0].astype(np.uint32), g0, p, d).astype(np.uint64)
+    c = (c + a[1][:d]) % np.uint64(p)  # + (u^T M)[1] * 1
+    assert not c.any(), np.nonzero(c)[0][:5]
+    z = int(rng.integers(2, p))
+    Mz = np.zeros((2, 2), np.uint64)
+    zp = np.ones(D, np.uint64)
+    for k in range(1, D):
+        zp[k] = zp[k - 1] * np.uint64(z) % np.uint64(p)
+    for i in range(2):
+        for j in range(2):
+            Mz[i, j] = int((M[i, j].astype(np.uint64) * zp % np.uint64(p)).sum() % np.uint64(p))
+    det = int((Mz[0, 0] * Mz[1, 1] + np.uint64(p) * np.uint64(p) - Mz[0, 1] * Mz[1, 0]) % np.uint64(p))
+    e = int(-delta.sum())
+    assert det in (pow(z, e, p), (-pow(z, e, p)) % p)
+    nzmax = np.where(M.any(axis=1), np.arange(D)[None, :], -1).max(axis=1)
+    assert np.all(nzmax <= -delta)
