@@ -30,6 +30,9 @@
 #include "common.hpp"
 #include "ntt.hpp"
 #include "pmbasis.hpp"
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 #include <algorithm>
 #include <cstring>
@@ -984,10 +987,23 @@ rr_status PmScheduler::run(std::vector<PmJob *> &jobs) {
     if (jobs.size() == 1 && !jobs[0]->K) jobs[0]->K = leaf_order_alone(2 * jobs[0]->b);
     for (auto *J : jobs)
         if ((s = add(J)) != RR_OK) return s;
+    // RR_PROGRESS=<seconds>: a line on stderr at most that often (long calls)
+    const char *pe = getenv("RR_PROGRESS");
+    const double every = pe ? atof(pe) : 0.0;
+    auto t_last = std::chrono::steady_clock::now();
     for (;;) {
         bool launched;
         if ((s = round(jobs, &launched)) != RR_OK) return s;
         if (!launched) break;
+        if (every > 0) {
+            const auto now = std::chrono::steady_clock::now();
+            if (std::chrono::duration<double>(now - t_last).count() >= every) {
+                t_last = now;
+                fprintf(stderr, "rr_pm_basis: %llu rounds, %llu launches queued\n", (unsigned long long)rounds,
+                        (unsigned long long)launches);
+                fflush(stderr);
+            }
+        }
     }
     for (auto *J : jobs)
         if (J->err != RR_OK) return J->err;
