@@ -154,3 +154,19 @@ def test_server_served_workload_ranks_match_oracle():
     seq = O.krylov(A, P, b, dr, dc, V, 0, L)
     M, delta = O.pm_basis(seq, b, L - 1, P)
     assert got[ids[-1]]["status"] == 0 and got[ids[-1]]["rank"] == O.rank_from_basis(M, delta, b, P)
+
+
+def test_server_refuses_orders_beyond_its_transforms():
+    """The server keeps twiddle tables for transforms of up to 2^20 points:
+    an order of 2^20 or more is refused at submission (rr_rank takes it),
+    and the server goes on serving."""
+    eng = _eng()
+    (A, b, dr, dc, V), = _requests(1, 77)
+    with eng.Server(P, 8 << 30) as srv:
+        with pytest.raises(eng.RankEngineError, match="order"):
+            srv.submit(A, b, dr, dc, V, seq_len=(1 << 20) + 8, order=1 << 20)
+        rid = srv.submit(A, b, dr, dc, V)
+        got = _collect(srv, [rid])
+    assert got[rid]["status"] == 0
+    want, _ = eng.rank(A, P, b, dr, dc, V)
+    assert got[rid]["rank"] == want
