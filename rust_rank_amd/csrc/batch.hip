@@ -201,7 +201,7 @@ rr_status Request::init(const rr_csr *A, uint32_t p_, uint32_t b_, const uint32_
     for (uint64_t r = 0; r < m; r++) RR_CHECK(A->row_ptr[r + 1] >= A->row_ptr[r], RR_EINVAL, "row_ptr not monotone");
     nnz = A->row_ptr[m];
     RR_CHECK(nnz < (1ull << 32), RR_EINVAL, "nnz must be < 2^32");
-    RR_CHECK(std::max(m, n) * B * 4 < (1ull << 31), RR_EINVAL, "block vectors exceed 2^31 bytes");
+    RR_CHECK(std::max(m, n) * B * 4 < 0xFFFFFF00ull, RR_EINVAL, "block vectors reach 4 GiB");
     RR_CHECK(nnz == 0 || (A->col && A->val), RR_EINVAL, "null col/val");
     for (uint64_t i = 0; i < m; i++) RR_CHECK(dr[i] >= 1 && dr[i] < p, RR_EINVAL, "row_precond out of range");
     for (uint64_t i = 0; i < n; i++) RR_CHECK(dcv[i] >= 1 && dcv[i] < p, RR_EINVAL, "col_precond out of range");

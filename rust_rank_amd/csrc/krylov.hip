@@ -84,7 +84,7 @@ __device__ __forceinline__ void order_after_vmem() {
 // Gathers go through a buffer resource over the input block vector: a 32-bit
 // byte offset per lane, and an offset past the end (padding entries) returns
 // zeros without touching memory, so the gathers need no branches.
-constexpr uint32_t OOB = 0x80000000u;  // block vectors stay below 2^31 bytes
+constexpr uint32_t OOB = 0xFFFFFF00u;  // block vectors stay below this many bytes (just under 4 GiB)
 
 template <int VW>
 __device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t rs, uint32_t off, uint32_t (&x)[VW]) {
@@ -1223,8 +1223,8 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
     {
         uint64_t Bp = 1;
         while (Bp < b) Bp <<= 1;
-        RR_CHECK(std::max(A->m, A->n) * Bp * 4 < (1ull << 31), RR_EINVAL,
-                 "block vectors of %llu rows x %llu words exceed 2^31 bytes",
+        RR_CHECK(std::max(A->m, A->n) * Bp * 4 < OOB, RR_EINVAL,
+                 "block vectors of %llu rows x %llu words reach 4 GiB",
                  (unsigned long long)std::max(A->m, A->n), (unsigned long long)Bp);
     }
     RR_CHECK(t_have == 0 || (t_have & 1), RR_EINVAL, "t_have=%llu must be 0 or odd", (unsigned long long)t_have);

@@ -40,7 +40,7 @@ struct SpmmArgs {
     const uint4 *linfo;        // per split row: {row, first accumulator row, accumulator rows, segments}
     uint32_t *lcnt;            // per split row: segments added in this pass (zero between passes)
     uint64_t gram_off;         // words from the launch's partial base to this job's first partial
-    uint32_t in_bytes;         // size of in (< 2^31)
+    uint32_t in_bytes;         // size of in (below the gathers' out-of-range offset, just under 4 GiB)
     uint32_t p;
     double pd, inv;
     uint32_t b;

@@ -95,6 +95,16 @@ def test_large_single_full_size_prefix():
     _case(A, P, 64, 2, 6)
 
 
+@pytest.mark.timeout(900)
+def test_block_vectors_above_2_gib():
+    """8,600,000 x 8,500,000 with 2 nonzeros per row at b = 33 (stride 64):
+    each block vector is 2.2 GB, past 2^31 bytes, so the gathers' byte offsets
+    use the whole 32-bit range.  S_0..S_2 and the state against the oracle."""
+    A = synth.uniform_sparse(11, 1, 8_600_000, 8_500_000, 2, 2, P, small=False)
+    assert A.m * 64 * 4 > 1 << 31
+    _case(A, P, 33, 11, 3)
+
+
 def test_graph_complex_full_size_prefix():
     """The benchmark workload itself (200000 x 180000, b = 32): 48 entries,
     the A^T passes with hundreds of split-row segments finished in-pass."""
