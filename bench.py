@@ -19,7 +19,9 @@ events the dispatches stamp on the engine's stream; `cpu_baseline` times the
 CPU oracle (a restatement of the reference's Rust path) on a prefix of the
 same workload on this host.  Extra objects: the 200000 x 180000 graph-complex
 workload (b = 32) over its whole sequence with its PM-basis and rank, the
-512-matrix batched form against plain solves, and the served form.
+2000 x 1800 plain workload end to end on the GPU and through the oracle on
+the host, the 512-matrix batched form against plain solves, and the served
+form.
 
     python bench.py --full-rank     # config-3 rank end to end (about 15 min), then exit
 """
@@ -181,6 +183,40 @@ def graph_complex_leg(engine, synth, p, steps=0, with_pm=True, pm_order=0):
         assert int(rr_rank) == int(rank), (rr_rank, rank)
         out["rr_rank_s"] = round(rr_ms / 1e3, 3)
     log(f"  PM-basis of order {d}: {out['pm_basis_ms']} ms; rank {rank}; rr_rank {out.get('rr_rank_s')} s")
+    return out
+
+
+def plain_small_leg(engine, synth, p, reps=20):
+    """The 2000 x 1800 plain workload (1800 base rows of 8 uniform nonzeros,
+    200 rows combining 3 base rows each; b = 8, L = 458, order 457) end to end:
+    rr_rank on the GPU (best of `reps` calls) against the oracle's whole
+    pipeline on this host (Krylov loop, PM-basis, rank formula), both checked
+    against dense Gaussian elimination."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    b = 8
+    A = synth.plain_small()
+    dr, dc, V = synth.preconditioners(1, A.m, A.n, b, p)
+    L = synth.seq_len_plain(min(A.m, A.n), b)
+    dense = O.dense_rank(A.to_dense(p), p)
+    gpu = []
+    for _ in range(reps):
+        r, ms = engine.rank(A, p, b, dr, dc, V, L, L - 1)
+        assert int(r) == dense, (r, dense)
+        gpu.append(ms)
+    h = host_info()
+    t0 = time.perf_counter()
+    seq = O.krylov(A, p, b, dr, dc, V, 0, L, nthreads=h["threads"])
+    basis, delta = O.pm_basis(seq, b, L - 1, p)
+    r_cpu = O.rank_from_basis(basis, delta, b, p)
+    t_cpu = time.perf_counter() - t0
+    assert int(r_cpu) == dense, (r_cpu, dense)
+    out = {"workload": f"plain_small {A.m}x{A.n} ({A.nnz} nonzeros), p=2013265921 b={b}", "seq_len": L,
+           "order": L - 1, "rank": dense, "dense_ge_rank": dense, "gpu_rr_rank_ms": round(min(gpu), 2),
+           "gpu_rr_rank_ms_median": round(float(np.median(gpu)), 2), "cpu_oracle_ms": round(t_cpu * 1e3, 1),
+           "cpu_threads": h["threads"]}
+    log(f"plain_small: rank {dense} (dense GE); rr_rank {out['gpu_rr_rank_ms']} ms, oracle pipeline "
+        f"{out['cpu_oracle_ms']} ms on {h['threads']} threads")
     return out
 
 
@@ -413,6 +449,7 @@ def main():
     nnz = A.nnz
     del A
     gc = None if args.no_graph_complex else graph_complex_leg(engine, synth, p)
+    small = plain_small_leg(engine, synth, p)
     bat = srv = None
     engine.release_device_memory()  # the large legs' cached blocks go back before the many small matrices
     if args.batched:
@@ -427,7 +464,8 @@ def main():
         "config": {"workload": desc, "seq_len": L, "nnz": nnz, "b": b,
                    "hbm_gbs_algorithmic": round(b_iter * r["iters_s"] / 1e9, 1)},
         "setup": r["setup"],
-        "roofline": roof, "cpu_baseline": cpu, "graph_complex": gc, "batched": bat, "served": srv,
+        "roofline": roof, "cpu_baseline": cpu, "graph_complex": gc, "plain_small": small, "batched": bat,
+        "served": srv,
     }
     print(json.dumps(out), flush=True)
 
