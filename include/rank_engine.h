@@ -207,6 +207,17 @@ rr_status rr_wdm_load(const char *path, uint32_t *row_precond, uint32_t *col_pre
                       uint32_t *seq);
 
 /*
+ * Replaces the reference's analysis of a stored sequence (src/main.rs:397-401,
+ * 423-436, 599-606: given a file name ending in .wdm it loads the checkpoint,
+ * runs PM_Basis on the sequence and prints the rank from analyze_pm_basis):
+ * the rank from the sequence of the .wdm file at path, with the PM-basis of
+ * the given order (0: rr_pm_order(len, reference_policy), so 1 gives the
+ * reference's largest power of two <= len) computed on the device, exactly
+ * mod the file's prime.
+ */
+rr_status rr_rank_from_wdm(const char *path, int reference_policy, uint64_t order, int64_t *rank_out);
+
+/*
  * Replaces save_wdm_file_sym (src/wdm_files.rs:132-210): writes the .wdm text
  * file of the same layout (V and curv written as canonical residues, as
  * prettify_vect does).  With curv and t from rr_krylov_state and the first t

@@ -69,4 +69,38 @@ rr_status rr_rank(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *row_p
     return RR_OK;
 }
 
+// The reference's analysis of a stored sequence (src/main.rs:397-401,
+// 423-436, 599-606: a file name ending in .wdm loads the checkpoint and runs
+// PM_Basis and analyze_pm_basis on its sequence).  The sequence goes to the
+// device once; G, the PM-basis and the rank inputs stay there.
+rr_status rr_rank_from_wdm(const char *path, int reference_policy, uint64_t order, int64_t *rank_out) {
+    rr::clear_error();
+    RR_CHECK(path && rank_out, RR_EINVAL, "null argument");
+    uint64_t m = 0, n = 0, len = 0;
+    uint32_t p = 0, b = 0;
+    rr_status s;
+    if ((s = rr_wdm_info(path, &m, &n, &p, &len, &b)) != RR_OK) return s;
+    RR_CHECK(b >= 1 && b <= 64, RR_EINVAL, "%s: b=%u must be in 1..64", path, b);
+    RR_CHECK(p > 2 && p < (1u << 31) && rr::h_is_prime(p), RR_EINVAL, "%s: p=%u must be a prime in (2, 2^31)",
+             path, p);
+    if (order == 0) order = rr_pm_order(len, reference_policy);
+    RR_CHECK(order >= 1 && order < len, RR_EINVAL, "%s: order %llu needs 1 <= order < %llu entries", path,
+             (unsigned long long)order, (unsigned long long)len);
+    const uint64_t P = (uint64_t)b * (b + 1) / 2;
+    std::vector<uint32_t> rp(m), cp(n), V(n * b), curv(n * b), seq(len * P);
+    if ((s = rr_wdm_load(path, rp.data(), cp.data(), V.data(), curv.data(), seq.data())) != RR_OK) return s;
+    for (size_t i = 0; i < (order + 1) * P; i++)
+        RR_CHECK(seq[i] < p, RR_EINVAL, "%s: sequence entry %zu >= p", path, i);
+    hipStream_t st;
+    RR_HIP(rr::stream_create(&st));
+    struct SG {
+        hipStream_t s;
+        ~SG() { rr::stream_destroy(s); }
+    } guard{st};
+    rr::DevBuf dseq;
+    if ((s = dseq.alloc((order + 1) * P * 4)) != RR_OK) return s;
+    RR_HIP(hipMemcpyAsync(dseq.ptr, seq.data(), (order + 1) * P * 4, hipMemcpyHostToDevice, st));
+    return rr::rank_from_device_seq(dseq.as<uint32_t>(), b, order, p, st, rank_out);
+}
+
 }  // extern "C"

@@ -126,6 +126,7 @@ def lib():
     L.rr_sms_close.restype = None
     L.rr_wdm_info.argtypes = [C.c_char_p, _u64ptr, _u64ptr, C.POINTER(C.c_uint32), _u64ptr, C.POINTER(C.c_uint32)]
     L.rr_wdm_load.argtypes = [C.c_char_p, _u32p, _u32p, _u32p, _u32p, _u32p]
+    L.rr_rank_from_wdm.argtypes = [C.c_char_p, C.c_int, C.c_uint64, C.POINTER(C.c_int64)]
     L.rr_wdm_save.argtypes = [C.c_char_p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, _u32p, _u32p, _u32p,
                               _u32p, _u32p, C.c_uint64]
     L.rr_seq_resume_count.argtypes = [C.c_uint64, C.c_uint64]
@@ -283,6 +284,13 @@ def wdm_load(path: str) -> dict:
     _check(lib().rr_wdm_load(os.fsencode(path), h["row_precond"], h["col_precond"], h["V"], h["curv"], h["seq"]))
     h["seq"] = h["seq"][:ln]
     return h
+
+
+def rank_from_wdm(path: str, reference_policy: bool = False, order: int = 0) -> int:
+    """Rank from the sequence stored in a .wdm file (rr_rank_from_wdm)."""
+    r = C.c_int64()
+    _check(lib().rr_rank_from_wdm(os.fsencode(path), int(reference_policy), order, C.byref(r)))
+    return r.value
 
 
 def wdm_save(path: str, m: int, n: int, p: int, row_precond, col_precond, V, curv, seq):

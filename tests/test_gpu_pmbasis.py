@@ -235,6 +235,34 @@ def test_rank_plain_small_end_to_end():
     assert r == O.dense_rank(A.to_dense(P), P)
 
 
+def test_rank_from_wdm_file(tmp_path):
+    """A .wdm file holding the oracle's sequence (src/main.rs:397-436: a file
+    name ending in .wdm is only analysed): rr_rank_from_wdm at order L - 1
+    equals dense elimination; under the reference's order policy (largest
+    power of two <= L, src/main.rs:599-606) it equals the oracle's rank from
+    its PM-basis of that order; bad orders are refused."""
+    eng = _eng()
+    A = synth.batch_matrix(33, 0, rows_lo=600, rows_hi=700)
+    b = 4
+    dr, dc, V = synth.preconditioners(13, A.m, A.n, b, P)
+    L = synth.seq_len_plain(min(A.m, A.n), b)
+    seq, x = O.krylov(A, P, b, dr, dc, V, 0, L, want_x=True)
+    path = str(tmp_path / "s.wdm")
+    eng.wdm_save(path, A.m, A.n, P, dr, dc, V, x, seq)
+    d2 = 1 << (L.bit_length() - 1)
+    assert d2 < L, L
+    want2 = O.rank_from_basis(*O.pm_basis(seq, b, d2, P), b, P)
+    print(f"m={A.m} n={A.n} b={b} L={L} power-of-two order {d2}", flush=True)
+    assert eng.rank_from_wdm(path) == O.dense_rank(A.to_dense(P), P)
+    assert eng.rank_from_wdm(path, reference_policy=True) == want2
+    assert eng.rank_from_wdm(path, order=d2) == want2
+    for bad in (L, L + 5):
+        with pytest.raises(eng.RankEngineError, match="order"):
+            eng.rank_from_wdm(path, order=bad)
+    with pytest.raises(eng.RankEngineError):
+        eng.rank_from_wdm(str(tmp_path / "missing.wdm"))
+
+
 def test_rank_streamed_equals_sequence_then_basis():
     """rr_rank on a matrix of at least 2^21 nonzeros and an order of at least
     1024 runs the PM-basis beside the Krylov passes (chunked sequence, G's
