@@ -109,9 +109,9 @@ def _left_combine(u, M, p):
 
 
 # Primes other than the NTT prime: transform products run mod three NTT primes
-# (2013265921, 469762049, 754974721) and are recombined by Garner's method;
+# (2013265921, 469762049, 1811939329) and are recombined by Garner's method;
 # above a transform prime the operands are reduced first (2^31 - 1 is above
-# all three, 10^9 + 7 above the two smaller ones, 3323 below all).
+# all three, 10^9 + 7 above the smallest, 3323 below all).
 OTHER_P = [2147483647, 1000000007, 3323]
 
 
@@ -235,25 +235,29 @@ def test_rank_plain_small_end_to_end():
     assert r == O.dense_rank(A.to_dense(P), P)
 
 
-def test_rank_from_wdm_file(tmp_path):
+@pytest.mark.parametrize("p, seed, pseed, rows", [(P, 33, 13, 600), (3323, 35, 14, 500)])
+def test_rank_from_wdm_file(tmp_path, p, seed, pseed, rows):
     """A .wdm file holding the oracle's sequence (src/main.rs:397-436: a file
     name ending in .wdm is only analysed): rr_rank_from_wdm at order L - 1
     equals dense elimination; under the reference's order policy (largest
     power of two <= L, src/main.rs:599-606) it equals the oracle's rank from
-    its PM-basis of that order; bad orders are refused."""
+    its PM-basis of that order; bad orders are refused.  At p = 3323 (the
+    reference's default prime) the top products run through the three CRT
+    primes, and the power-of-two order is too small for this matrix."""
     eng = _eng()
-    A = synth.batch_matrix(33, 0, rows_lo=600, rows_hi=700)
+    A = synth.batch_matrix(seed, 0, p=p, rows_lo=rows, rows_hi=rows + 100)
     b = 4
-    dr, dc, V = synth.preconditioners(13, A.m, A.n, b, P)
+    dr, dc, V = synth.preconditioners(pseed, A.m, A.n, b, p)
     L = synth.seq_len_plain(min(A.m, A.n), b)
-    seq, x = O.krylov(A, P, b, dr, dc, V, 0, L, want_x=True)
+    seq, x = O.krylov(A, p, b, dr, dc, V, 0, L, want_x=True)
     path = str(tmp_path / "s.wdm")
-    eng.wdm_save(path, A.m, A.n, P, dr, dc, V, x, seq)
+    eng.wdm_save(path, A.m, A.n, p, dr, dc, V, x, seq)
     d2 = 1 << (L.bit_length() - 1)
     assert d2 < L, L
-    want2 = O.rank_from_basis(*O.pm_basis(seq, b, d2, P), b, P)
-    print(f"m={A.m} n={A.n} b={b} L={L} power-of-two order {d2}", flush=True)
-    assert eng.rank_from_wdm(path) == O.dense_rank(A.to_dense(P), P)
+    want2 = O.rank_from_basis(*O.pm_basis(seq, b, d2, p), b, p)
+    dense = O.dense_rank(A.to_dense(p), p)
+    print(f"p={p} m={A.m} n={A.n} b={b} L={L} power-of-two order {d2}: oracle {want2}, dense {dense}", flush=True)
+    assert eng.rank_from_wdm(path) == dense
     assert eng.rank_from_wdm(path, reference_policy=True) == want2
     assert eng.rank_from_wdm(path, order=d2) == want2
     for bad in (L, L + 5):
