@@ -138,11 +138,12 @@ def roofline(r, traffic, kernel, copy_gbs, gather=None):
     return out
 
 
-def graph_complex_leg(engine, synth, p, steps=0, with_pm=True, pm_order=0):
+def graph_complex_leg(engine, synth, p, steps=0, with_pm=True, pm_order=0, cpu_timed=0):
     """The 200000 x 180000 graph-complex workload at b = 32 over its whole
     sequence (11258 entries; `steps` timed entries instead if given), its
-    device PM-basis of order L - 1 and rank, and the same rank through one
-    rr_rank call."""
+    device PM-basis of order L - 1 and rank, the same rank through one
+    rr_rank call, and (cpu_timed > 0) the oracle's Krylov loop on a prefix of
+    the same sequence."""
     b = 32
     A = synth.graph_complex()
     dr, dc, V = synth.preconditioners(2, A.m, A.n, b, p)
@@ -183,6 +184,9 @@ def graph_complex_leg(engine, synth, p, steps=0, with_pm=True, pm_order=0):
         assert int(rr_rank) == int(rank), (rr_rank, rank)
         out["rr_rank_s"] = round(rr_ms / 1e3, 3)
     log(f"  PM-basis of order {d}: {out['pm_basis_ms']} ms; rank {rank}; rr_rank {out.get('rr_rank_s')} s")
+    if cpu_timed:
+        out["cpu_baseline"] = cpu_baseline(A, p, b, dr, dc, V, 2, cpu_timed, "200000x180000 b=32 workload")
+        log(f"  CPU baseline {out['cpu_baseline']['value']} iters/s on {out['cpu_baseline']['cores']} threads")
     return out
 
 
@@ -386,6 +390,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-timed", type=int, default=16, help="timed oracle iterations for the CPU baseline")
+    ap.add_argument("--cpu-gc-timed", type=int, default=256,
+                    help="timed oracle iterations for the graph-complex CPU baseline")
     ap.add_argument("--no-graph-complex", action="store_true", help="skip the 200000 x 180000 workload")
     ap.add_argument("--no-pm", action="store_true", help="graph_complex: skip its PM-basis and rank")
     ap.add_argument("--pm-order", type=int, default=0,
@@ -448,7 +454,8 @@ def main():
                                                 "5000000x4500000 b=64 workload")
     nnz = A.nnz
     del A
-    gc = None if args.no_graph_complex else graph_complex_leg(engine, synth, p)
+    gc = None if args.no_graph_complex else graph_complex_leg(engine, synth, p,
+                                                              cpu_timed=0 if args.no_cpu else args.cpu_gc_timed)
     small = plain_small_leg(engine, synth, p)
     bat = srv = None
     engine.release_device_memory()  # the large legs' cached blocks go back before the many small matrices

@@ -12,6 +12,7 @@
 #include "common.hpp"
 #include "batch.hpp"
 #include <vector>
+#include <exception>
 #include <chrono>
 
 
@@ -50,7 +51,13 @@ rr_status rr_rank(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *row_p
     if (order == 0) order = rr_pm_order(seq_len, 0);
     RR_CHECK(order >= 1 && order < seq_len, RR_EINVAL, "order must satisfy 1 <= order < seq_len");
     const uint64_t P = (uint64_t)b * (b + 1) / 2;
-    std::vector<uint32_t> seq(seq_len * P);
+    std::vector<uint32_t> seq;
+    try {
+        seq.resize(seq_len * P);
+    } catch (const std::exception &) {
+        rr::set_error("host allocation of %llu sequence words failed", (unsigned long long)(seq_len * P));
+        return RR_ENOMEM;
+    }
     rr_krylov *k = nullptr;
     rr_status s = rr_krylov_create(A, p, b, row_precond, col_precond, V, 0, &k);
     if (s != RR_OK) return s;
@@ -87,7 +94,18 @@ rr_status rr_rank_from_wdm(const char *path, int reference_policy, uint64_t orde
     RR_CHECK(order >= 1 && order < len, RR_EINVAL, "%s: order %llu needs 1 <= order < %llu entries", path,
              (unsigned long long)order, (unsigned long long)len);
     const uint64_t P = (uint64_t)b * (b + 1) / 2;
-    std::vector<uint32_t> rp(m), cp(n), V(n * b), curv(n * b), seq(len * P);
+    std::vector<uint32_t> rp, cp, V, curv, seq;
+    try {
+        rp.resize(m);
+        cp.resize(n);
+        V.resize(n * b);
+        curv.resize(n * b);
+        seq.resize(len * P);
+    } catch (const std::exception &) {
+        rr::set_error("%s: host allocation for a %llux%llu, b=%u, %llu-entry file failed", path,
+                      (unsigned long long)m, (unsigned long long)n, b, (unsigned long long)len);
+        return RR_ENOMEM;
+    }
     if ((s = rr_wdm_load(path, rp.data(), cp.data(), V.data(), curv.data(), seq.data())) != RR_OK) return s;
     for (size_t i = 0; i < (order + 1) * P; i++)
         RR_CHECK(seq[i] < p, RR_EINVAL, "%s: sequence entry %zu >= p", path, i);

@@ -95,6 +95,29 @@ def test_wdm_reader_errors_and_extra_tokens(tmp_path):
         assert msg in str(e.value), (msg, str(e.value))
 
 
+def test_rank_from_wdm_refusals_before_the_device(tmp_path):
+    """rr_rank_from_wdm checks the file before any device work: a missing
+    file, a b or prime it cannot use, an order the sequence does not reach,
+    and sizes no host can hold are errors (status and message), not crashes."""
+    def write(name, text):
+        f = tmp_path / name
+        f.write_text(text)
+        return str(f)
+    good = "2 3 97 5 1\n1 2\n3 4 5\n6 7 8\n9 10 11\n1 2 3 4 5\n"
+    cases = [
+        (str(tmp_path / "missing.wdm"), "cannot open"),
+        (write("p91.wdm", good.replace(" 97 ", " 91 ")), "prime"),
+        (write("huge.wdm", "4611686018427387904 3 97 5 1\n"), "host allocation"),
+    ]
+    for path, msg in cases:
+        with pytest.raises(engine.RankEngineError, match=msg):
+            engine.rank_from_wdm(path)
+    path = write("good.wdm", good)
+    for order in (5, 9):
+        with pytest.raises(engine.RankEngineError, match="order"):
+            engine.rank_from_wdm(path, order=order)
+
+
 def _reference_loop_length(have, max_nlen):
     """src/main.rs:105-131 replayed: returns the sequence length after the loop."""
     length = have
