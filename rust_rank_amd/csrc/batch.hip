@@ -696,22 +696,26 @@ static rr_status batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, cons
 rr_status rr_batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const uint32_t *b, const uint32_t *const *dr,
                         const uint32_t *const *dc, const uint32_t *const *V, const uint64_t *seq_len,
                         const uint64_t *order, int64_t *rank_out, rr_batch_stats *stats) {
-    clear_error();
-    return batch_rank(count, mats, p, b, nullptr, dr, dc, V, seq_len, order, rank_out, stats);
+    return rr::guarded([&]() -> rr_status {
+        clear_error();
+        return batch_rank(count, mats, p, b, nullptr, dr, dc, V, seq_len, order, rank_out, stats);
+    });
 }
 
 rr_status rr_batch_rank_tuned(uint32_t count, const rr_csr *mats, uint32_t p, const rr_tuning *tune,
                               const uint32_t *const *dr, const uint32_t *const *dc, const uint32_t *const *V,
                               const uint64_t *seq_len, const uint64_t *order, int64_t *rank_out,
                               rr_batch_stats *stats) {
-    clear_error();
-    RR_CHECK(count >= 1 && tune, RR_EINVAL, "null argument");
-    std::vector<uint32_t> b(count), su(count);
-    for (uint32_t i = 0; i < count; i++) {
-        b[i] = tune[i].b;
-        su[i] = tune[i].su;
-    }
-    return batch_rank(count, mats, p, b.data(), su.data(), dr, dc, V, seq_len, order, rank_out, stats);
+    return rr::guarded([&]() -> rr_status {
+        clear_error();
+        RR_CHECK(count >= 1 && tune, RR_EINVAL, "null argument");
+        std::vector<uint32_t> b(count), su(count);
+        for (uint32_t i = 0; i < count; i++) {
+            b[i] = tune[i].b;
+            su[i] = tune[i].su;
+        }
+        return batch_rank(count, mats, p, b.data(), su.data(), dr, dc, V, seq_len, order, rank_out, stats);
+    });
 }
 
 static rr_status batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, const uint32_t *b, const uint32_t *su_of,
@@ -1121,83 +1125,89 @@ static rr_status run_probe(uint32_t p, int cus, hipStream_t st) {
 extern "C" {
 
 rr_status rr_autotune_b(uint32_t count, const rr_csr *mats, uint32_t p, uint32_t *b_out, double *pred_ms) {
-    clear_error();
-    RR_CHECK(count == 0 || (mats && b_out), RR_EINVAL, "null argument");
-    RR_CHECK(p == NTT_PRIME, RR_EINVAL, "the autotuner's PM-basis probe needs p = 2013265921");
-    if (!g_probe.done) {
-        int dev = 0, cus = 0;
-        RR_HIP(hipGetDevice(&dev));
-        RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        hipStream_t st;
-        RR_HIP(rr::stream_create(&st));
-        rr_status s = run_probe(p, cus, st);
-        rr::stream_destroy(st);
-        if (s != RR_OK) return s;
-    }
-    // b = 16 unless another width is predicted TUNE_MARGIN faster: the probes
-    // vary by 10-20 % between runs, and a width chosen on noise costs the
-    // whole batch (b = 8 doubles a sequence's ticks)
-    for (uint32_t i = 0; i < count; i++) {
-        const rr_csr &A = mats[i];
-        uint64_t nnz = A.row_ptr[A.m], mn = std::min(A.m, A.n);
-        double t[3];
-        for (int k = 0; k < 3; k++) {
-            const double L = 2.0 * (double)mn / TUNE_B[k] + 8, d = L - 1;
-            t[k] = d + 1 > (double)(1u << NTT_MAX_LOGN)
-                       ? 1e300
-                       : L * ((double)nnz * g_probe.c[k] + g_probe.f[k]) +
-                             d * (g_probe.alpha[k] + g_probe.beta[k] * std::log2(d));
+    return rr::guarded([&]() -> rr_status {
+        clear_error();
+        RR_CHECK(count == 0 || (mats && b_out), RR_EINVAL, "null argument");
+        RR_CHECK(p == NTT_PRIME, RR_EINVAL, "the autotuner's PM-basis probe needs p = 2013265921");
+        if (!g_probe.done) {
+            int dev = 0, cus = 0;
+            RR_HIP(hipGetDevice(&dev));
+            RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            hipStream_t st;
+            RR_HIP(rr::stream_create(&st));
+            rr_status s = run_probe(p, cus, st);
+            rr::stream_destroy(st);
+            if (s != RR_OK) return s;
         }
-        int kb = 1;
-        for (int k = 0; k < 3; k++)
-            if (k != 1 && t[k] < TUNE_MARGIN * t[1] && t[k] < t[kb]) kb = k;
-        b_out[i] = TUNE_B[kb];
-        if (pred_ms) pred_ms[i] = t[kb];
-    }
-    return RR_OK;
+        // b = 16 unless another width is predicted TUNE_MARGIN faster: the probes
+        // vary by 10-20 % between runs, and a width chosen on noise costs the
+        // whole batch (b = 8 doubles a sequence's ticks)
+        for (uint32_t i = 0; i < count; i++) {
+            const rr_csr &A = mats[i];
+            uint64_t nnz = A.row_ptr[A.m], mn = std::min(A.m, A.n);
+            double t[3];
+            for (int k = 0; k < 3; k++) {
+                const double L = 2.0 * (double)mn / TUNE_B[k] + 8, d = L - 1;
+                t[k] = d + 1 > (double)(1u << NTT_MAX_LOGN)
+                           ? 1e300
+                           : L * ((double)nnz * g_probe.c[k] + g_probe.f[k]) +
+                                 d * (g_probe.alpha[k] + g_probe.beta[k] * std::log2(d));
+            }
+            int kb = 1;
+            for (int k = 0; k < 3; k++)
+                if (k != 1 && t[k] < TUNE_MARGIN * t[1] && t[k] < t[kb]) kb = k;
+            b_out[i] = TUNE_B[kb];
+            if (pred_ms) pred_ms[i] = t[kb];
+        }
+        return RR_OK;
+    });
 }
 
 rr_status rr_autotune(uint32_t count, const rr_csr *mats, uint32_t p, rr_tuning *out) {
-    clear_error();
-    RR_CHECK(count == 0 || (mats && out), RR_EINVAL, "null argument");
-    std::vector<uint32_t> b(count);
-    std::vector<double> pred(count);
-    rr_status s = rr_autotune_b(count, mats, p, b.data(), pred.data());
-    if (s != RR_OK) return s;
-    for (uint32_t i = 0; i < count; i++) {
-        const rr_csr &A = mats[i];
-        RR_CHECK(A.row_ptr && A.row_ptr[0] == 0 && A.row_ptr[A.m] < (1ull << 32), RR_EINVAL, "matrix %u: bad row_ptr",
-                 i);
-        const uint64_t nnz = A.row_ptr[A.m];
-        std::vector<uint32_t> rp(A.m + 1), ct(A.n + 1, 0);
-        for (uint64_t r = 0; r <= A.m; r++) rp[r] = (uint32_t)A.row_ptr[r];
-        for (uint64_t e = 0; e < nnz; e++) {
-            RR_CHECK(A.col[e] < A.n, RR_EINVAL, "matrix %u: column index out of range", i);
-            ct[A.col[e] + 1]++;
+    return rr::guarded([&]() -> rr_status {
+        clear_error();
+        RR_CHECK(count == 0 || (mats && out), RR_EINVAL, "null argument");
+        std::vector<uint32_t> b(count);
+        std::vector<double> pred(count);
+        rr_status s = rr_autotune_b(count, mats, p, b.data(), pred.data());
+        if (s != RR_OK) return s;
+        for (uint32_t i = 0; i < count; i++) {
+            const rr_csr &A = mats[i];
+            RR_CHECK(A.row_ptr && A.row_ptr[0] == 0 && A.row_ptr[A.m] < (1ull << 32), RR_EINVAL,
+                     "matrix %u: bad row_ptr", i);
+            const uint64_t nnz = A.row_ptr[A.m];
+            std::vector<uint32_t> rp(A.m + 1), ct(A.n + 1, 0);
+            for (uint64_t r = 0; r <= A.m; r++) rp[r] = (uint32_t)A.row_ptr[r];
+            for (uint64_t e = 0; e < nnz; e++) {
+                RR_CHECK(A.col[e] < A.n, RR_EINVAL, "matrix %u: column index out of range", i);
+                ct[A.col[e] + 1]++;
+            }
+            for (uint64_t c = 0; c < A.n; c++) ct[c + 1] += ct[c];
+            const int k = b[i] == 8 ? 0 : b[i] == 16 ? 1 : 2;
+            const double L = 2.0 * (double)std::min(A.m, A.n) / b[i] + 8;
+            double t[2];
+            for (int u = 0; u < 2; u++) t[u] = L * g_probe.slot[k][u] * (double)padded_slots(rp, ct, 3 + u);
+            const int uk = t[1] < t[0] ? 1 : 0;
+            out[i].b = b[i];
+            out[i].su = 3 + uk;
+            // the b model's Krylov term replaced by the slot model's at the chosen block size
+            const double kr = L * ((double)nnz * g_probe.c[k] + g_probe.f[k]);
+            out[i].pred_ms = pred[i] - kr + L * g_probe.f[k] + t[uk];
         }
-        for (uint64_t c = 0; c < A.n; c++) ct[c + 1] += ct[c];
-        const int k = b[i] == 8 ? 0 : b[i] == 16 ? 1 : 2;
-        const double L = 2.0 * (double)std::min(A.m, A.n) / b[i] + 8;
-        double t[2];
-        for (int u = 0; u < 2; u++) t[u] = L * g_probe.slot[k][u] * (double)padded_slots(rp, ct, 3 + u);
-        const int uk = t[1] < t[0] ? 1 : 0;
-        out[i].b = b[i];
-        out[i].su = 3 + uk;
-        // the b model's Krylov term replaced by the slot model's at the chosen block size
-        const double kr = L * ((double)nnz * g_probe.c[k] + g_probe.f[k]);
-        out[i].pred_ms = pred[i] - kr + L * g_probe.f[k] + t[uk];
-    }
-    return RR_OK;
+        return RR_OK;
+    });
 }
 
 rr_status rr_autotune_probe(double *c_ms_per_nnz, double *pm_ms_d255) {
-    RR_CHECK(c_ms_per_nnz && pm_ms_d255, RR_EINVAL, "null argument");
-    RR_CHECK(g_probe.done, RR_ESTATE, "autotuner has not probed yet");
-    for (int k = 0; k < 3; k++) {
-        c_ms_per_nnz[k] = g_probe.c[k];
-        pm_ms_d255[k] = g_probe.pm[k];
-    }
-    return RR_OK;
+    return rr::guarded([&]() -> rr_status {
+        RR_CHECK(c_ms_per_nnz && pm_ms_d255, RR_EINVAL, "null argument");
+        RR_CHECK(g_probe.done, RR_ESTATE, "autotuner has not probed yet");
+        for (int k = 0; k < 3; k++) {
+            c_ms_per_nnz[k] = g_probe.c[k];
+            pm_ms_d255[k] = g_probe.pm[k];
+        }
+        return RR_OK;
+    });
 }
 
 }  // extern "C"

@@ -5,12 +5,33 @@
 #include <cstdio>
 #include <cstdarg>
 #include <string>
+#include <new>
+#include <stdexcept>
 #include "../../include/rank_engine.h"
 
 namespace rr {
 
 void set_error(const char *fmt, ...);
 void clear_error();
+
+// The body of a C entry point: a host exception (an allocation that fails, a
+// size beyond what a container can hold) becomes a status and a message for
+// the calling thread instead of unwinding through the C ABI.
+template <class F>
+rr_status guarded(F &&body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return RR_ENOMEM;
+    } catch (const std::length_error &e) {
+        set_error("host buffer too large (%s)", e.what());
+        return RR_ENOMEM;
+    } catch (const std::exception &e) {
+        set_error("host error: %s", e.what());
+        return RR_EINVAL;
+    }
+}
 
 // Loads and stores through a pointer marked global: the compiler emits
 // global_* (in-order VMEM) instead of flat_* instructions, whose completion

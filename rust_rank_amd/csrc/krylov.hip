@@ -1212,294 +1212,305 @@ int rr_device_count(void) {
 
 rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *row_precond,
                            const uint32_t *col_precond, const uint32_t *curv, uint64_t t_have, rr_krylov **out) {
-    clear_error();
-    const auto t_create = std::chrono::steady_clock::now();
-    RR_CHECK(A && out && row_precond && col_precond && curv, RR_EINVAL, "null argument");
-    *out = nullptr;
-    RR_CHECK(p > 2 && p < (1u << 31) && h_is_prime(p), RR_EINVAL, "p=%u must be a prime in (2, 2^31)", p);
-    RR_CHECK(b >= 1 && b <= 64, RR_EINVAL, "b=%u must be in 1..64", b);
-    RR_CHECK(A->m >= 1 && A->n >= 1 && A->m < (1ull << 31) && A->n < (1ull << 31), RR_EINVAL,
-             "matrix %llux%llu out of range (m, n < 2^31)", (unsigned long long)A->m, (unsigned long long)A->n);
-    {
-        uint64_t Bp = 1;
-        while (Bp < b) Bp <<= 1;
-        RR_CHECK(std::max(A->m, A->n) * Bp * 4 < OOB, RR_EINVAL,
-                 "block vectors of %llu rows x %llu words reach 4 GiB",
-                 (unsigned long long)std::max(A->m, A->n), (unsigned long long)Bp);
-    }
-    RR_CHECK(t_have == 0 || (t_have & 1), RR_EINVAL, "t_have=%llu must be 0 or odd", (unsigned long long)t_have);
-    const uint64_t m = A->m, n = A->n;
-    RR_CHECK(A->row_ptr && A->row_ptr[0] == 0, RR_EINVAL, "row_ptr[0] must be 0");
-    const uint64_t nnz = A->row_ptr[m];
-    for (uint64_t i = 0; i < m; i++)
-        RR_CHECK(row_precond[i] >= 1 && row_precond[i] < p, RR_EINVAL, "row_precond[%llu] not in [1,p)",
-                 (unsigned long long)i);
-    for (uint64_t i = 0; i < n; i++)
-        RR_CHECK(col_precond[i] >= 1 && col_precond[i] < p, RR_EINVAL, "col_precond[%llu] not in [1,p)",
-                 (unsigned long long)i);
-    for (uint64_t i = 0; i < n * b; i++)
-        RR_CHECK(curv[i] < p, RR_EINVAL, "curv[%llu] >= p", (unsigned long long)i);
+    return rr::guarded([&]() -> rr_status {
+        clear_error();
+        const auto t_create = std::chrono::steady_clock::now();
+        RR_CHECK(A && out && row_precond && col_precond && curv, RR_EINVAL, "null argument");
+        *out = nullptr;
+        RR_CHECK(p > 2 && p < (1u << 31) && h_is_prime(p), RR_EINVAL, "p=%u must be a prime in (2, 2^31)", p);
+        RR_CHECK(b >= 1 && b <= 64, RR_EINVAL, "b=%u must be in 1..64", b);
+        RR_CHECK(A->m >= 1 && A->n >= 1 && A->m < (1ull << 31) && A->n < (1ull << 31), RR_EINVAL,
+                 "matrix %llux%llu out of range (m, n < 2^31)", (unsigned long long)A->m, (unsigned long long)A->n);
+        {
+            uint64_t Bp = 1;
+            while (Bp < b) Bp <<= 1;
+            RR_CHECK(std::max(A->m, A->n) * Bp * 4 < OOB, RR_EINVAL,
+                     "block vectors of %llu rows x %llu words reach 4 GiB",
+                     (unsigned long long)std::max(A->m, A->n), (unsigned long long)Bp);
+        }
+        RR_CHECK(t_have == 0 || (t_have & 1), RR_EINVAL, "t_have=%llu must be 0 or odd", (unsigned long long)t_have);
+        const uint64_t m = A->m, n = A->n;
+        RR_CHECK(A->row_ptr && A->row_ptr[0] == 0, RR_EINVAL, "row_ptr[0] must be 0");
+        const uint64_t nnz = A->row_ptr[m];
+        for (uint64_t i = 0; i < m; i++)
+            RR_CHECK(row_precond[i] >= 1 && row_precond[i] < p, RR_EINVAL, "row_precond[%llu] not in [1,p)",
+                     (unsigned long long)i);
+        for (uint64_t i = 0; i < n; i++)
+            RR_CHECK(col_precond[i] >= 1 && col_precond[i] < p, RR_EINVAL, "col_precond[%llu] not in [1,p)",
+                     (unsigned long long)i);
+        for (uint64_t i = 0; i < n * b; i++)
+            RR_CHECK(curv[i] < p, RR_EINVAL, "curv[%llu] >= p", (unsigned long long)i);
 
-    std::unique_ptr<rr_krylov> c(new rr_krylov());
-    c->p = p;
-    c->b = b;
-    c->B = 1;
-    while (c->B < b) c->B <<= 1;
-    c->m = m;
-    c->n = n;
-    c->nnz = nnz;
-    c->mp = ModP::make(p);
-    c->P = b * (b + 1) / 2;
-    c->t = t_have;
-    RR_HIP(hipGetDevice(&c->device));
-    RR_HIP(rr::stream_create(&c->stream));
-    int cus = 0;
-    RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-    c->grid = 0;  // set below from the occupancy of the chosen pass kernel
+        std::unique_ptr<rr_krylov> c(new rr_krylov());
+        c->p = p;
+        c->b = b;
+        c->B = 1;
+        while (c->B < b) c->B <<= 1;
+        c->m = m;
+        c->n = n;
+        c->nnz = nnz;
+        c->mp = ModP::make(p);
+        c->P = b * (b + 1) / 2;
+        c->t = t_have;
+        RR_HIP(hipGetDevice(&c->device));
+        RR_HIP(rr::stream_create(&c->stream));
+        int cus = 0;
+        RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+        c->grid = 0;  // set below from the occupancy of the chosen pass kernel
 
-    // preconditioners: D_r, D_c, D_c^2, D_c^-1
-    std::vector<uint32_t> dc2(n), dcinv(n);
-    for (uint64_t i = 0; i < n; i++) {
-        dc2[i] = h_mulmod(col_precond[i], col_precond[i], p);
-        dcinv[i] = h_invmod(col_precond[i], p);
-    }
-    c->sep_proj = c->B == 64;
-    // A and A^T in sliced form for one resident wave of workgroups (every
-    // wavefront owns a fixed stream of slices); checked and laid out on the device
-    OperandInfo oi;
-    rr_status s = setup_operands(
-        A, p, c->B, 0,
-        [&](bool packed, uint32_t su) {
-            c->grid = (uint32_t)std::max(1, cus * spmm_blocks_per_cu(c->B, packed, su, !c->sep_proj));
-            return c->grid * (uint32_t)(WG / 64);
-        },
-        row_precond, dc2.data(), c->A, c->At, &oi, c->stream);
-    if (s != RR_OK) return s;
-    c->packed = oi.packed;
-    c->su = oi.su;
-    c->on_device_setup = oi.on_device;
-    if ((s = upload(c->dr, row_precond, m)) != RR_OK) return s;
-    if ((s = upload(c->dc, col_precond, n)) != RR_OK) return s;
-    if ((s = upload(c->dc2, dc2.data(), n)) != RR_OK) return s;
-    if ((s = upload(c->dcinv, dcinv.data(), n)) != RR_OK) return s;
-    if (c->sep_proj) {
-        int occ = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_proj64, WG, 0) != hipSuccess || occ < 1) occ = 1;
-        c->pgrid = std::min<uint32_t>(c->grid, (uint32_t)(cus * occ));
-        // k_proj64's int32 accumulators hold 2^15 rows per workgroup
-        RR_CHECK((std::max(m, n) + 64ull * c->pgrid - 1) / (64ull * c->pgrid) * 64 <= PJ_FOLD_ROWS, RR_EINVAL,
-                 "%llu rows exceed the projection's %u workgroups", (unsigned long long)std::max(m, n), c->pgrid);
-        if ((s = c->drinv.alloc((size_t)m * 4)) != RR_OK) return s;
-        if ((s = c->dc2inv.alloc((size_t)n * 4)) != RR_OK) return s;
-        hipLaunchKernelGGL(k_inv_mod, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->stream,
-                           c->dr.as<uint32_t>(), c->drinv.as<uint32_t>(), m, p, c->mp.inv);
-        hipLaunchKernelGGL(k_inv_mod, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                           c->dc2.as<uint32_t>(), c->dc2inv.as<uint32_t>(), n, p, c->mp.inv);
+        // preconditioners: D_r, D_c, D_c^2, D_c^-1
+        std::vector<uint32_t> dc2(n), dcinv(n);
+        for (uint64_t i = 0; i < n; i++) {
+            dc2[i] = h_mulmod(col_precond[i], col_precond[i], p);
+            dcinv[i] = h_invmod(col_precond[i], p);
+        }
+        c->sep_proj = c->B == 64;
+        // A and A^T in sliced form for one resident wave of workgroups (every
+        // wavefront owns a fixed stream of slices); checked and laid out on the device
+        OperandInfo oi;
+        rr_status s = setup_operands(
+            A, p, c->B, 0,
+            [&](bool packed, uint32_t su) {
+                c->grid = (uint32_t)std::max(1, cus * spmm_blocks_per_cu(c->B, packed, su, !c->sep_proj));
+                return c->grid * (uint32_t)(WG / 64);
+            },
+            row_precond, dc2.data(), c->A, c->At, &oi, c->stream);
+        if (s != RR_OK) return s;
+        c->packed = oi.packed;
+        c->su = oi.su;
+        c->on_device_setup = oi.on_device;
+        if ((s = upload(c->dr, row_precond, m)) != RR_OK) return s;
+        if ((s = upload(c->dc, col_precond, n)) != RR_OK) return s;
+        if ((s = upload(c->dc2, dc2.data(), n)) != RR_OK) return s;
+        if ((s = upload(c->dcinv, dcinv.data(), n)) != RR_OK) return s;
+        if (c->sep_proj) {
+            int occ = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_proj64, WG, 0) != hipSuccess || occ < 1) occ = 1;
+            c->pgrid = std::min<uint32_t>(c->grid, (uint32_t)(cus * occ));
+            // k_proj64's int32 accumulators hold 2^15 rows per workgroup
+            RR_CHECK((std::max(m, n) + 64ull * c->pgrid - 1) / (64ull * c->pgrid) * 64 <= PJ_FOLD_ROWS, RR_EINVAL,
+                     "%llu rows exceed the projection's %u workgroups", (unsigned long long)std::max(m, n), c->pgrid);
+            if ((s = c->drinv.alloc((size_t)m * 4)) != RR_OK) return s;
+            if ((s = c->dc2inv.alloc((size_t)n * 4)) != RR_OK) return s;
+            hipLaunchKernelGGL(k_inv_mod, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->stream,
+                               c->dr.as<uint32_t>(), c->drinv.as<uint32_t>(), m, p, c->mp.inv);
+            hipLaunchKernelGGL(k_inv_mod, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                               c->dc2.as<uint32_t>(), c->dc2inv.as<uint32_t>(), n, p, c->mp.inv);
+            RR_HIP(hipGetLastError());
+        }
+
+        // block vectors, padded to stride B
+        const uint32_t B = c->B;
+        std::vector<uint32_t> xpad((size_t)n * B, 0);
+        for (uint64_t r = 0; r < n; r++) std::memcpy(&xpad[r * B], &curv[r * b], b * 4);
+        if ((s = upload(c->xtmp, xpad.data(), xpad.size())) != RR_OK) return s;
+        if ((s = c->z.alloc((size_t)n * B * 4)) != RR_OK) return s;
+        if ((s = c->y.alloc((size_t)m * B * 4)) != RR_OK) return s;
+        RR_HIP(hipMemsetAsync(c->y.ptr, 0, (size_t)m * B * 4, c->stream));
+        size_t tot = (size_t)n * B;
+        hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
+                           c->xtmp.as<uint32_t>(), c->dc.as<uint32_t>(), c->z.as<uint32_t>(), (uint32_t)n, B, p,
+                           c->mp.inv);
         RR_HIP(hipGetLastError());
-    }
-
-    // block vectors, padded to stride B
-    const uint32_t B = c->B;
-    std::vector<uint32_t> xpad((size_t)n * B, 0);
-    for (uint64_t r = 0; r < n; r++) std::memcpy(&xpad[r * B], &curv[r * b], b * 4);
-    if ((s = upload(c->xtmp, xpad.data(), xpad.size())) != RR_OK) return s;
-    if ((s = c->z.alloc((size_t)n * B * 4)) != RR_OK) return s;
-    if ((s = c->y.alloc((size_t)m * B * 4)) != RR_OK) return s;
-    RR_HIP(hipMemsetAsync(c->y.ptr, 0, (size_t)m * B * 4, c->stream));
-    size_t tot = (size_t)n * B;
-    hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
-                       c->xtmp.as<uint32_t>(), c->dc.as<uint32_t>(), c->z.as<uint32_t>(), (uint32_t)n, B, p,
-                       c->mp.inv);
-    RR_HIP(hipGetLastError());
-    // pass descriptors and the one-job workgroup table
-    {
-        SpmmArgs args[2] = {spmm_args(c->A, c->z.as<uint32_t>(), n, B, c->y.as<uint32_t>(), c->dr.as<uint32_t>(), p, b),
-                            spmm_args(c->At, c->y.as<uint32_t>(), m, B, c->z.as<uint32_t>(), c->dc2.as<uint32_t>(), p,
-                                      b)};
-        const std::vector<uint32_t> wgmap = build_wg_map({c->grid}, &c->hgrid);
-        if ((s = c->jobs.alloc(sizeof args + wgmap.size() * 4)) != RR_OK) return s;
-        RR_HIP(hipMemcpy(c->jobs.ptr, args, sizeof args, hipMemcpyHostToDevice));
-        RR_HIP(hipMemcpy(c->jobs.as<char>() + sizeof args, wgmap.data(), wgmap.size() * 4, hipMemcpyHostToDevice));
-    }
-    uint32_t nparts = c->grid;
-    if ((s = c->part.alloc((size_t)CHUNK * nparts * c->P * 4)) != RR_OK) return s;
-    // room for SEQ_RESERVE entries, so short runs never reallocate
-    if ((s = c->seq.alloc((size_t)SEQ_RESERVE * c->P * 4)) != RR_OK) return s;
-    c->seq_cap = SEQ_RESERVE;
-    RR_HIP(hipMemsetAsync(c->part.ptr, 0, (size_t)CHUNK * nparts * c->P * 4, c->stream));
-    RR_HIP(hipStreamSynchronize(c->stream));
-    // algorithmic bytes per pass (int32 columns, u32 values, each array once)
-    c->stats.bytes_a = 8 * nnz + 4 * (m + 1) + (uint64_t)4 * b * (n + m);
-    c->stats.bytes_at = 8 * nnz + 4 * (n + 1) + (uint64_t)4 * b * (n + m);
-    c->stats.setup_on_device = c->on_device_setup ? 1 : 0;
-    c->stats.ms_setup = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_create).count();
-    *out = c.release();
-    return RR_OK;
+        // pass descriptors and the one-job workgroup table
+        {
+            SpmmArgs args[2] = {
+                spmm_args(c->A, c->z.as<uint32_t>(), n, B, c->y.as<uint32_t>(), c->dr.as<uint32_t>(), p, b),
+                spmm_args(c->At, c->y.as<uint32_t>(), m, B, c->z.as<uint32_t>(), c->dc2.as<uint32_t>(), p, b)};
+            const std::vector<uint32_t> wgmap = build_wg_map({c->grid}, &c->hgrid);
+            if ((s = c->jobs.alloc(sizeof args + wgmap.size() * 4)) != RR_OK) return s;
+            RR_HIP(hipMemcpy(c->jobs.ptr, args, sizeof args, hipMemcpyHostToDevice));
+            RR_HIP(hipMemcpy(c->jobs.as<char>() + sizeof args, wgmap.data(), wgmap.size() * 4, hipMemcpyHostToDevice));
+        }
+        uint32_t nparts = c->grid;
+        if ((s = c->part.alloc((size_t)CHUNK * nparts * c->P * 4)) != RR_OK) return s;
+        // room for SEQ_RESERVE entries, so short runs never reallocate
+        if ((s = c->seq.alloc((size_t)SEQ_RESERVE * c->P * 4)) != RR_OK) return s;
+        c->seq_cap = SEQ_RESERVE;
+        RR_HIP(hipMemsetAsync(c->part.ptr, 0, (size_t)CHUNK * nparts * c->P * 4, c->stream));
+        RR_HIP(hipStreamSynchronize(c->stream));
+        // algorithmic bytes per pass (int32 columns, u32 values, each array once)
+        c->stats.bytes_a = 8 * nnz + 4 * (m + 1) + (uint64_t)4 * b * (n + m);
+        c->stats.bytes_at = 8 * nnz + 4 * (n + 1) + (uint64_t)4 * b * (n + m);
+        c->stats.setup_on_device = c->on_device_setup ? 1 : 0;
+        c->stats.ms_setup =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_create).count();
+        *out = c.release();
+        return RR_OK;
+    });
 }
 
 rr_status rr_krylov_set_profiling(rr_krylov *c, int on) {
-    RR_CHECK(c, RR_EINVAL, "null context");
-    c->profiling = on != 0;
-    return RR_OK;
+    return rr::guarded([&]() -> rr_status {
+        RR_CHECK(c, RR_EINVAL, "null context");
+        c->profiling = on != 0;
+        return RR_OK;
+    });
 }
 
 rr_status rr_krylov_stats_get(rr_krylov *c, rr_krylov_stats *out) {
-    RR_CHECK(c && out, RR_EINVAL, "null argument");
-    *out = c->stats;
-    return RR_OK;
+    return rr::guarded([&]() -> rr_status {
+        RR_CHECK(c && out, RR_EINVAL, "null argument");
+        *out = c->stats;
+        return RR_OK;
+    });
 }
 
 rr_status rr_krylov_run(rr_krylov *c, uint64_t count, uint32_t *seq_out) {
-    clear_error();
-    RR_CHECK(c && (seq_out || count == 0), RR_EINVAL, "null argument");
-    const uint32_t P = c->P, nparts = c->grid;
-    const size_t part_stride = (size_t)nparts * P;
-    uint64_t bytes_a = c->stats.bytes_a, bytes_at = c->stats.bytes_at;
-    const double ms_setup = c->stats.ms_setup;
-    const int32_t on_dev = c->stats.setup_on_device;
-    c->stats = rr_krylov_stats{};
-    c->stats.bytes_a = bytes_a;
-    c->stats.bytes_at = bytes_at;
-    c->stats.ms_setup = ms_setup;
-    c->stats.setup_on_device = on_dev;
+    return rr::guarded([&]() -> rr_status {
+        clear_error();
+        RR_CHECK(c && (seq_out || count == 0), RR_EINVAL, "null argument");
+        const uint32_t P = c->P, nparts = c->grid;
+        const size_t part_stride = (size_t)nparts * P;
+        uint64_t bytes_a = c->stats.bytes_a, bytes_at = c->stats.bytes_at;
+        const double ms_setup = c->stats.ms_setup;
+        const int32_t on_dev = c->stats.setup_on_device;
+        c->stats = rr_krylov_stats{};
+        c->stats.bytes_a = bytes_a;
+        c->stats.bytes_at = bytes_at;
+        c->stats.ms_setup = ms_setup;
+        c->stats.setup_on_device = on_dev;
 
-    // event triples per pass when profiling: [start, after pass, after pass]
-    std::vector<hipEvent_t> ev;
-    std::vector<int> kind;  // 0 gram, 1 A pass, 2 A^T pass
-    hipEvent_t e_begin = nullptr, e_end = nullptr;
-    RR_HIP(hipEventCreate(&e_begin));
-    RR_HIP(hipEventCreate(&e_end));
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> red_ev, proj_ev;
-    RR_HIP(hipEventRecord(e_begin, c->stream));
+        // event triples per pass when profiling: [start, after pass, after pass]
+        std::vector<hipEvent_t> ev;
+        std::vector<int> kind;  // 0 gram, 1 A pass, 2 A^T pass
+        hipEvent_t e_begin = nullptr, e_end = nullptr;
+        RR_HIP(hipEventCreate(&e_begin));
+        RR_HIP(hipEventCreate(&e_end));
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> red_ev, proj_ev;
+        RR_HIP(hipEventRecord(e_begin, c->stream));
 
-    if (count > c->seq_cap) {  // grows geometrically; the old buffer goes back to the cache
-        uint64_t cap = c->seq_cap;
-        while (cap < count) cap *= 2;
-        RR_HIP(hipStreamSynchronize(c->stream));
-        rr_status sa = c->seq.alloc((size_t)cap * P * 4);
-        if (sa != RR_OK) {
-            c->seq_cap = 0;
-            return sa;
+        if (count > c->seq_cap) {  // grows geometrically; the old buffer goes back to the cache
+            uint64_t cap = c->seq_cap;
+            while (cap < count) cap *= 2;
+            RR_HIP(hipStreamSynchronize(c->stream));
+            rr_status sa = c->seq.alloc((size_t)cap * P * 4);
+            if (sa != RR_OK) {
+                c->seq_cap = 0;
+                return sa;
+            }
+            c->seq_cap = cap;
         }
-        c->seq_cap = cap;
-    }
-    uint64_t done = 0;
-    while (done < count) {
-        uint32_t chunk = (uint32_t)std::min<uint64_t>(CHUNK, count - done);
-        for (uint32_t q = 0; q < chunk; q++) {
-            uint64_t t = c->t;
-            uint32_t *gram = c->part.as<uint32_t>() + q * part_stride;
-            hipEvent_t e0 = nullptr, e1 = nullptr;
-            if (c->profiling) {
-                RR_HIP(hipEventCreate(&e0));
-                RR_HIP(hipEventCreate(&e1));
-            }
-            int k;
-            if (t == 0) {
-                // S_0 = x_0^T x_0, x_0 = curv
-                if (c->profiling) RR_HIP(hipEventRecord(e0, c->stream));
-                RR_HIP(hipMemsetAsync(gram, 0, part_stride * 4, c->stream));
-                launch_gram_any(c, c->xtmp.as<uint32_t>(), (uint32_t)c->n, gram);
-                k = 0;
-                if (c->profiling) RR_HIP(hipEventRecord(e1, c->stream));
-            } else {
-                bool byA = (t & 1) == 1;
-                const SpmmArgs *job = c->jobs.as<SpmmArgs>() + (byA ? 0 : 1);
-                const uint32_t *wgmap = (const uint32_t *)(c->jobs.as<char>() + 2 * sizeof(SpmmArgs));
-                launch_spmm(c->B, c->packed, c->su, job, wgmap, c->hgrid, gram, c->stream, e0, e1, !c->sep_proj);
-                if (c->sep_proj) {  // the projection of the rows this pass produced
-                    const uint32_t *rows = byA ? c->y.as<uint32_t>() : c->z.as<uint32_t>();
-                    const uint32_t *sinv = byA ? c->drinv.as<uint32_t>() : c->dc2inv.as<uint32_t>();
-                    const uint32_t nr = (uint32_t)(byA ? c->m : c->n);
-                    hipEvent_t g0 = nullptr, g1 = nullptr;
-                    if (c->profiling) {
-                        RR_HIP(hipEventCreate(&g0));
-                        RR_HIP(hipEventCreate(&g1));
-                    }
-                    hipExtLaunchKernelGGL(k_proj64, dim3(c->pgrid), dim3(WG), 0, c->stream, g0, g1, 0, rows, sinv,
-                                          nr, gram, c->grid, c->b, c->p, c->mp.pd, c->mp.inv);
-                    if (c->profiling) proj_ev.push_back({g0, g1});
+        uint64_t done = 0;
+        while (done < count) {
+            uint32_t chunk = (uint32_t)std::min<uint64_t>(CHUNK, count - done);
+            for (uint32_t q = 0; q < chunk; q++) {
+                uint64_t t = c->t;
+                uint32_t *gram = c->part.as<uint32_t>() + q * part_stride;
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (c->profiling) {
+                    RR_HIP(hipEventCreate(&e0));
+                    RR_HIP(hipEventCreate(&e1));
                 }
-                k = byA ? 1 : 2;
+                int k;
+                if (t == 0) {
+                    // S_0 = x_0^T x_0, x_0 = curv
+                    if (c->profiling) RR_HIP(hipEventRecord(e0, c->stream));
+                    RR_HIP(hipMemsetAsync(gram, 0, part_stride * 4, c->stream));
+                    launch_gram_any(c, c->xtmp.as<uint32_t>(), (uint32_t)c->n, gram);
+                    k = 0;
+                    if (c->profiling) RR_HIP(hipEventRecord(e1, c->stream));
+                } else {
+                    bool byA = (t & 1) == 1;
+                    const SpmmArgs *job = c->jobs.as<SpmmArgs>() + (byA ? 0 : 1);
+                    const uint32_t *wgmap = (const uint32_t *)(c->jobs.as<char>() + 2 * sizeof(SpmmArgs));
+                    launch_spmm(c->B, c->packed, c->su, job, wgmap, c->hgrid, gram, c->stream, e0, e1, !c->sep_proj);
+                    if (c->sep_proj) {  // the projection of the rows this pass produced
+                        const uint32_t *rows = byA ? c->y.as<uint32_t>() : c->z.as<uint32_t>();
+                        const uint32_t *sinv = byA ? c->drinv.as<uint32_t>() : c->dc2inv.as<uint32_t>();
+                        const uint32_t nr = (uint32_t)(byA ? c->m : c->n);
+                        hipEvent_t g0 = nullptr, g1 = nullptr;
+                        if (c->profiling) {
+                            RR_HIP(hipEventCreate(&g0));
+                            RR_HIP(hipEventCreate(&g1));
+                        }
+                        hipExtLaunchKernelGGL(k_proj64, dim3(c->pgrid), dim3(WG), 0, c->stream, g0, g1, 0, rows, sinv,
+                                              nr, gram, c->grid, c->b, c->p, c->mp.pd, c->mp.inv);
+                        if (c->profiling) proj_ev.push_back({g0, g1});
+                    }
+                    k = byA ? 1 : 2;
+                }
+                RR_HIP(hipGetLastError());
+                if (c->profiling) {
+                    ev.push_back(e0);
+                    ev.push_back(e1);
+                    kind.push_back(k);
+                }
+                if (k == 1) c->stats.passes_a++;
+                if (k == 2) c->stats.passes_at++;
+                c->t++;
             }
+            hipEvent_t r0 = nullptr, r1 = nullptr;
+            if (c->profiling) {
+                RR_HIP(hipEventCreate(&r0));
+                RR_HIP(hipEventCreate(&r1));
+                RR_HIP(hipEventRecord(r0, c->stream));
+            }
+            hipLaunchKernelGGL(k_reduce, dim3((P + 63) / 64, chunk), dim3(RED_WG), 0, c->stream, c->part.as<uint32_t>(),
+                               nparts, P, c->seq.as<uint32_t>() + done * P, c->p);
             RR_HIP(hipGetLastError());
             if (c->profiling) {
-                ev.push_back(e0);
-                ev.push_back(e1);
-                kind.push_back(k);
+                RR_HIP(hipEventRecord(r1, c->stream));
+                red_ev.push_back({r0, r1});
             }
-            if (k == 1) c->stats.passes_a++;
-            if (k == 2) c->stats.passes_at++;
-            c->t++;
+            // the next chunk's passes reuse the partial buffer after this reduction (stream order)
+            done += chunk;
         }
-        hipEvent_t r0 = nullptr, r1 = nullptr;
+        RR_HIP(hipEventRecord(e_end, c->stream));
+        RR_HIP(hipMemcpyAsync(seq_out, c->seq.ptr, (size_t)count * P * 4, hipMemcpyDeviceToHost, c->stream));
+        RR_HIP(hipStreamSynchronize(c->stream));
+        float ms = 0;
+        RR_HIP(hipEventElapsedTime(&ms, e_begin, e_end));
+        c->stats.ms_total = ms;
         if (c->profiling) {
-            RR_HIP(hipEventCreate(&r0));
-            RR_HIP(hipEventCreate(&r1));
-            RR_HIP(hipEventRecord(r0, c->stream));
-        }
-        hipLaunchKernelGGL(k_reduce, dim3((P + 63) / 64, chunk), dim3(RED_WG), 0, c->stream, c->part.as<uint32_t>(),
-                           nparts, P, c->seq.as<uint32_t>() + done * P, c->p);
-        RR_HIP(hipGetLastError());
-        if (c->profiling) {
-            RR_HIP(hipEventRecord(r1, c->stream));
-            red_ev.push_back({r0, r1});
-        }
-        // the next chunk's passes reuse the partial buffer after this reduction (stream order)
-        done += chunk;
-    }
-    RR_HIP(hipEventRecord(e_end, c->stream));
-    RR_HIP(hipMemcpyAsync(seq_out, c->seq.ptr, (size_t)count * P * 4, hipMemcpyDeviceToHost, c->stream));
-    RR_HIP(hipStreamSynchronize(c->stream));
-    float ms = 0;
-    RR_HIP(hipEventElapsedTime(&ms, e_begin, e_end));
-    c->stats.ms_total = ms;
-    if (c->profiling) {
-        for (size_t i = 0; i < kind.size(); i++) {
-            float a = 0;
-            RR_HIP(hipEventElapsedTime(&a, ev[2 * i], ev[2 * i + 1]));
-            if (kind[i] == 1) c->stats.ms_a += a;
-            if (kind[i] == 2) c->stats.ms_at += a;
-        }
-        for (auto &pr : red_ev) {
-            float r = 0;
-            RR_HIP(hipEventElapsedTime(&r, pr.first, pr.second));
-            c->stats.ms_reduce += r;
-        }
-        for (auto &pr : proj_ev) {
-            float r = 0;
-            RR_HIP(hipEventElapsedTime(&r, pr.first, pr.second));
-            c->stats.ms_proj += r;
-        }
-        for (auto e : ev) (void)hipEventDestroy(e);
-        for (auto *v : {&red_ev, &proj_ev})
-            for (auto &pr : *v) {
-                (void)hipEventDestroy(pr.first);
-                (void)hipEventDestroy(pr.second);
+            for (size_t i = 0; i < kind.size(); i++) {
+                float a = 0;
+                RR_HIP(hipEventElapsedTime(&a, ev[2 * i], ev[2 * i + 1]));
+                if (kind[i] == 1) c->stats.ms_a += a;
+                if (kind[i] == 2) c->stats.ms_at += a;
             }
-    }
-    (void)hipEventDestroy(e_begin);
-    (void)hipEventDestroy(e_end);
-    return RR_OK;
+            for (auto &pr : red_ev) {
+                float r = 0;
+                RR_HIP(hipEventElapsedTime(&r, pr.first, pr.second));
+                c->stats.ms_reduce += r;
+            }
+            for (auto &pr : proj_ev) {
+                float r = 0;
+                RR_HIP(hipEventElapsedTime(&r, pr.first, pr.second));
+                c->stats.ms_proj += r;
+            }
+            for (auto e : ev) (void)hipEventDestroy(e);
+            for (auto *v : {&red_ev, &proj_ev})
+                for (auto &pr : *v) {
+                    (void)hipEventDestroy(pr.first);
+                    (void)hipEventDestroy(pr.second);
+                }
+        }
+        (void)hipEventDestroy(e_begin);
+        (void)hipEventDestroy(e_end);
+        return RR_OK;
+    });
 }
 
 rr_status rr_krylov_state(rr_krylov *c, uint32_t *curv_out, uint64_t *t_out) {
-    clear_error();
-    RR_CHECK(c && curv_out && t_out, RR_EINVAL, "null argument");
-    const uint32_t B = c->B;
-    const uint64_t n = c->n;
-    *t_out = c->t;
-    // z holds D_c x_K for the last even index reached (x_0 before any A^T pass)
-    size_t tot = (size_t)n * B;
-    hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
-                       c->z.as<uint32_t>(), c->dcinv.as<uint32_t>(), c->xtmp.as<uint32_t>(), (uint32_t)n, B, c->p,
-                       c->mp.inv);
-    RR_HIP(hipGetLastError());
-    std::vector<uint32_t> h(tot);
-    RR_HIP(hipMemcpyAsync(h.data(), c->xtmp.ptr, tot * 4, hipMemcpyDeviceToHost, c->stream));
-    RR_HIP(hipStreamSynchronize(c->stream));
-    for (uint64_t r = 0; r < n; r++) std::memcpy(&curv_out[r * c->b], &h[r * B], c->b * 4);
-    return RR_OK;
+    return rr::guarded([&]() -> rr_status {
+        clear_error();
+        RR_CHECK(c && curv_out && t_out, RR_EINVAL, "null argument");
+        const uint32_t B = c->B;
+        const uint64_t n = c->n;
+        *t_out = c->t;
+        // z holds D_c x_K for the last even index reached (x_0 before any A^T pass)
+        size_t tot = (size_t)n * B;
+        hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
+                           c->z.as<uint32_t>(), c->dcinv.as<uint32_t>(), c->xtmp.as<uint32_t>(), (uint32_t)n, B, c->p,
+                           c->mp.inv);
+        RR_HIP(hipGetLastError());
+        std::vector<uint32_t> h(tot);
+        RR_HIP(hipMemcpyAsync(h.data(), c->xtmp.ptr, tot * 4, hipMemcpyDeviceToHost, c->stream));
+        RR_HIP(hipStreamSynchronize(c->stream));
+        for (uint64_t r = 0; r < n; r++) std::memcpy(&curv_out[r * c->b], &h[r * B], c->b * 4);
+        return RR_OK;
+    });
 }
 
 void rr_krylov_destroy(rr_krylov *c) { delete c; }

@@ -1293,45 +1293,49 @@ extern "C" {
 
 rr_status rr_pm_basis(const uint32_t *seq, uint32_t b, uint64_t L, uint64_t d, uint32_t p, uint32_t *basis_out,
                       int64_t *delta_out) {
-    clear_error();
-    RR_CHECK(seq && basis_out && delta_out, RR_EINVAL, "null argument");
-    RR_CHECK(b >= 1 && b <= 64, RR_EINVAL, "b=%u must be in 1..64", b);
-    RR_CHECK(p > 2 && p < (1u << 31) && h_is_prime(p), RR_EINVAL, "p=%u must be a prime in (2, 2^31)", p);
-    RR_CHECK(d >= 1 && d < L, RR_EINVAL, "order d=%llu must satisfy 1 <= d < L=%llu", (unsigned long long)d,
-             (unsigned long long)L);
-    RR_CHECK(d + 1 <= (1ull << NTT_MAX_LOGN), RR_EINVAL, "order d=%llu above the 2^%u-point transform limit",
-             (unsigned long long)d, NTT_MAX_LOGN);
-    const uint32_t P = b * (b + 1) / 2, m = 2 * b;
-    const uint64_t len = d + 1;
-    hipStream_t st;
-    RR_HIP(rr::stream_create(&st));
-    struct S {
-        hipStream_t s;
-        ~S() { rr::stream_destroy(s); }
-    } guard{st};
-    DevBuf dseq, G, M;
-    rr_status s;
-    if ((s = dseq.alloc(len * P * 4)) != RR_OK) return s;
-    if ((s = G.alloc((size_t)m * b * len * 4)) != RR_OK) return s;
-    if ((s = M.alloc((size_t)m * m * len * 4)) != RR_OK) return s;
-    for (uint64_t i = 0; i < len * P; i++)
-        RR_CHECK(seq[i] < p, RR_EINVAL, "seq[%llu] >= p", (unsigned long long)i);
-    RR_HIP(hipMemcpyAsync(dseq.ptr, seq, len * P * 4, hipMemcpyHostToDevice, st));
-    if ((s = build_G_device(dseq.as<uint32_t>(), b, len, G.as<uint32_t>(), st)) != RR_OK) return s;
-    if ((s = pm_basis_device(p, b, d, G.as<uint32_t>(), M.as<uint32_t>(), delta_out, st)) != RR_OK) return s;
-    RR_HIP(hipMemcpyAsync(basis_out, M.ptr, (size_t)m * m * len * 4, hipMemcpyDeviceToHost, st));
-    RR_HIP(hipStreamSynchronize(st));
-    return RR_OK;
+    return rr::guarded([&]() -> rr_status {
+        clear_error();
+        RR_CHECK(seq && basis_out && delta_out, RR_EINVAL, "null argument");
+        RR_CHECK(b >= 1 && b <= 64, RR_EINVAL, "b=%u must be in 1..64", b);
+        RR_CHECK(p > 2 && p < (1u << 31) && h_is_prime(p), RR_EINVAL, "p=%u must be a prime in (2, 2^31)", p);
+        RR_CHECK(d >= 1 && d < L, RR_EINVAL, "order d=%llu must satisfy 1 <= d < L=%llu", (unsigned long long)d,
+                 (unsigned long long)L);
+        RR_CHECK(d + 1 <= (1ull << NTT_MAX_LOGN), RR_EINVAL, "order d=%llu above the 2^%u-point transform limit",
+                 (unsigned long long)d, NTT_MAX_LOGN);
+        const uint32_t P = b * (b + 1) / 2, m = 2 * b;
+        const uint64_t len = d + 1;
+        hipStream_t st;
+        RR_HIP(rr::stream_create(&st));
+        struct S {
+            hipStream_t s;
+            ~S() { rr::stream_destroy(s); }
+        } guard{st};
+        DevBuf dseq, G, M;
+        rr_status s;
+        if ((s = dseq.alloc(len * P * 4)) != RR_OK) return s;
+        if ((s = G.alloc((size_t)m * b * len * 4)) != RR_OK) return s;
+        if ((s = M.alloc((size_t)m * m * len * 4)) != RR_OK) return s;
+        for (uint64_t i = 0; i < len * P; i++)
+            RR_CHECK(seq[i] < p, RR_EINVAL, "seq[%llu] >= p", (unsigned long long)i);
+        RR_HIP(hipMemcpyAsync(dseq.ptr, seq, len * P * 4, hipMemcpyHostToDevice, st));
+        if ((s = build_G_device(dseq.as<uint32_t>(), b, len, G.as<uint32_t>(), st)) != RR_OK) return s;
+        if ((s = pm_basis_device(p, b, d, G.as<uint32_t>(), M.as<uint32_t>(), delta_out, st)) != RR_OK) return s;
+        RR_HIP(hipMemcpyAsync(basis_out, M.ptr, (size_t)m * m * len * 4, hipMemcpyDeviceToHost, st));
+        RR_HIP(hipStreamSynchronize(st));
+        return RR_OK;
+    });
 }
 
 rr_status rr_rank_from_basis(const uint32_t *basis, const int64_t *delta, uint32_t b, uint64_t len, uint32_t p,
                              int64_t *rank_out) {
-    clear_error();
-    RR_CHECK(basis && delta && rank_out, RR_EINVAL, "null argument");
-    RR_CHECK(b >= 1 && b <= 64 && len >= 1, RR_EINVAL, "bad sizes");
-    rr_status s;
-    *rank_out = rank_from_basis_host(basis, delta, b, len, p, &s);
-    return s;
+    return rr::guarded([&]() -> rr_status {
+        clear_error();
+        RR_CHECK(basis && delta && rank_out, RR_EINVAL, "null argument");
+        RR_CHECK(b >= 1 && b <= 64 && len >= 1, RR_EINVAL, "bad sizes");
+        rr_status s;
+        *rank_out = rank_from_basis_host(basis, delta, b, len, p, &s);
+        return s;
+    });
 }
 
 }  // extern "C"
@@ -1343,92 +1347,96 @@ extern "C" {
 // The device transform is DIF (bit-reversed spectrum); the permutation to
 // natural order is done on the host here, products never need it.
 rr_status rr_ntt(uint32_t *data, uint32_t logn, uint32_t count, int inverse) {
-    clear_error();
-    RR_CHECK(data && logn >= 1 && logn <= NTT_MAX_LOGN && count >= 1, RR_EINVAL, "bad NTT arguments");
-    const uint32_t N = 1u << logn, p = NTT_PRIME;
-    for (uint64_t i = 0; i < (uint64_t)N * count; i++) RR_CHECK(data[i] < p, RR_EINVAL, "input >= p");
-    std::vector<uint32_t> rev(N);
-    for (uint32_t i = 0; i < N; i++) {
-        uint32_t r = 0, x = i;
-        for (uint32_t k = 0; k < logn; k++) { r = (r << 1) | (x & 1); x >>= 1; }
-        rev[i] = r;
-    }
-    hipStream_t st;
-    RR_HIP(rr::stream_create(&st));
-    struct S {
-        hipStream_t s;
-        ~S() { rr::stream_destroy(s); }
-    } guard{st};
-    NttTables T;
-    rr_status s;
-    if ((s = T.init(logn, st)) != RR_OK) return s;
-    DevBuf a, o;
-    size_t bytes = (size_t)N * count * 4;
-    if ((s = a.alloc(bytes)) != RR_OK) return s;
-    if ((s = o.alloc(bytes)) != RR_OK) return s;
-    std::vector<uint32_t> h((size_t)N * count);
-    if (!inverse) {
-        RR_HIP(hipMemcpyAsync(a.ptr, data, bytes, hipMemcpyHostToDevice, st));
-        if ((s = ntt_forward(T, a.as<uint32_t>(), N, 0, N, o.as<uint32_t>(), count, logn, st)) != RR_OK) return s;
-        RR_HIP(hipMemcpyAsync(h.data(), o.ptr, bytes, hipMemcpyDeviceToHost, st));
-        RR_HIP(hipStreamSynchronize(st));
-        for (uint32_t q = 0; q < count; q++)
-            for (uint32_t i = 0; i < N; i++) data[(size_t)q * N + rev[i]] = h[(size_t)q * N + i];
-    } else {
-        for (uint32_t q = 0; q < count; q++)
-            for (uint32_t i = 0; i < N; i++) h[(size_t)q * N + i] = data[(size_t)q * N + rev[i]];
-        RR_HIP(hipMemcpyAsync(a.ptr, h.data(), bytes, hipMemcpyHostToDevice, st));
-        if ((s = ntt_inverse(T, a.as<uint32_t>(), count, logn, 0, N, o.as<uint32_t>(), N, st)) != RR_OK) return s;
-        RR_HIP(hipMemcpyAsync(h.data(), o.ptr, bytes, hipMemcpyDeviceToHost, st));
-        RR_HIP(hipStreamSynchronize(st));
-        uint32_t ninv = h_invmod(N, p);
-        for (size_t i = 0; i < (size_t)N * count; i++) data[i] = h_mulmod(h[i], ninv, p);
-    }
-    return RR_OK;
+    return rr::guarded([&]() -> rr_status {
+        clear_error();
+        RR_CHECK(data && logn >= 1 && logn <= NTT_MAX_LOGN && count >= 1, RR_EINVAL, "bad NTT arguments");
+        const uint32_t N = 1u << logn, p = NTT_PRIME;
+        for (uint64_t i = 0; i < (uint64_t)N * count; i++) RR_CHECK(data[i] < p, RR_EINVAL, "input >= p");
+        std::vector<uint32_t> rev(N);
+        for (uint32_t i = 0; i < N; i++) {
+            uint32_t r = 0, x = i;
+            for (uint32_t k = 0; k < logn; k++) { r = (r << 1) | (x & 1); x >>= 1; }
+            rev[i] = r;
+        }
+        hipStream_t st;
+        RR_HIP(rr::stream_create(&st));
+        struct S {
+            hipStream_t s;
+            ~S() { rr::stream_destroy(s); }
+        } guard{st};
+        NttTables T;
+        rr_status s;
+        if ((s = T.init(logn, st)) != RR_OK) return s;
+        DevBuf a, o;
+        size_t bytes = (size_t)N * count * 4;
+        if ((s = a.alloc(bytes)) != RR_OK) return s;
+        if ((s = o.alloc(bytes)) != RR_OK) return s;
+        std::vector<uint32_t> h((size_t)N * count);
+        if (!inverse) {
+            RR_HIP(hipMemcpyAsync(a.ptr, data, bytes, hipMemcpyHostToDevice, st));
+            if ((s = ntt_forward(T, a.as<uint32_t>(), N, 0, N, o.as<uint32_t>(), count, logn, st)) != RR_OK) return s;
+            RR_HIP(hipMemcpyAsync(h.data(), o.ptr, bytes, hipMemcpyDeviceToHost, st));
+            RR_HIP(hipStreamSynchronize(st));
+            for (uint32_t q = 0; q < count; q++)
+                for (uint32_t i = 0; i < N; i++) data[(size_t)q * N + rev[i]] = h[(size_t)q * N + i];
+        } else {
+            for (uint32_t q = 0; q < count; q++)
+                for (uint32_t i = 0; i < N; i++) h[(size_t)q * N + i] = data[(size_t)q * N + rev[i]];
+            RR_HIP(hipMemcpyAsync(a.ptr, h.data(), bytes, hipMemcpyHostToDevice, st));
+            if ((s = ntt_inverse(T, a.as<uint32_t>(), count, logn, 0, N, o.as<uint32_t>(), N, st)) != RR_OK) return s;
+            RR_HIP(hipMemcpyAsync(h.data(), o.ptr, bytes, hipMemcpyDeviceToHost, st));
+            RR_HIP(hipStreamSynchronize(st));
+            uint32_t ninv = h_invmod(N, p);
+            for (size_t i = 0; i < (size_t)N * count; i++) data[i] = h_mulmod(h[i], ninv, p);
+        }
+        return RR_OK;
+    });
 }
 
 // Truncated polynomial-matrix product (src/poly_mat_mul.rs:89-256):
 // C [rows][cols][hi-lo] = coefficients [lo, hi) of A [rows][inner][la] * B [inner][cols][lb], exact mod p.
 rr_status rr_poly_mat_mul(const uint32_t *A, uint64_t la, const uint32_t *B, uint64_t lb, uint32_t rows,
                           uint32_t inner, uint32_t cols, uint64_t lo, uint64_t hi, uint32_t p, uint32_t *C) {
-    clear_error();
-    RR_CHECK(A && B && C && la >= 1 && lb >= 1 && hi > lo && rows && inner && cols, RR_EINVAL,
-             "bad product arguments");
-    RR_CHECK(inner <= 128 && hi <= la + lb - 1 + 0ull, RR_EINVAL, "inner <= 128 and hi <= la + lb - 1");
-    RR_CHECK(p > 2 && p < (1u << 31) && h_is_prime(p), RR_EINVAL, "p=%u must be a prime in (2, 2^31)", p);
-    hipStream_t st;
-    RR_HIP(rr::stream_create(&st));
-    struct S {
-        hipStream_t s;
-        ~S() { rr::stream_destroy(s); }
-    } guard{st};
-    DevBuf dA, dB, dC;
-    rr_status s;
-    size_t sa = (size_t)rows * inner * la, sbn = (size_t)inner * cols * lb, sc = (size_t)rows * cols * (hi - lo);
-    if ((s = dA.alloc(sa * 4)) != RR_OK) return s;
-    if ((s = dB.alloc(sbn * 4)) != RR_OK) return s;
-    if ((s = dC.alloc(sc * 4)) != RR_OK) return s;
-    for (size_t i = 0; i < sa; i++) RR_CHECK(A[i] < p, RR_EINVAL, "A entry %zu >= p", i);
-    for (size_t i = 0; i < sbn; i++) RR_CHECK(B[i] < p, RR_EINVAL, "B entry %zu >= p", i);
-    RR_HIP(hipMemcpyAsync(dA.ptr, A, sa * 4, hipMemcpyHostToDevice, st));
-    RR_HIP(hipMemcpyAsync(dB.ptr, B, sbn * 4, hipMemcpyHostToDevice, st));
-    MulDesc D{dA.as<uint32_t>(), dB.as<uint32_t>(), dC.as<uint32_t>(), lb, (uint32_t)la, (uint32_t)lb, 0,
-              (uint32_t)lo, (uint32_t)hi, rows, inner, cols};
-    PmScheduler S;
-    S.p = p;
-    S.st = st;
-    if (use_school(D, p)) {
-        if ((s = launch_school(std::vector<MulDesc>{D}, p, st, nullptr)) != RR_OK) return s;
-    } else {
-        uint32_t logn = ntt_logn_for(D);
-        RR_CHECK(logn <= NTT_MAX_LOGN, RR_EINVAL, "product too long");
-        if ((s = S.init_tables(logn)) != RR_OK) return s;
-        std::vector<NttProduct> ops{{logn, D, nullptr, nullptr}};
-        if ((s = S.run_ntt_products(ops)) != RR_OK) return s;
-    }
-    RR_HIP(hipMemcpyAsync(C, dC.ptr, sc * 4, hipMemcpyDeviceToHost, st));
-    RR_HIP(hipStreamSynchronize(st));
-    return RR_OK;
+    return rr::guarded([&]() -> rr_status {
+        clear_error();
+        RR_CHECK(A && B && C && la >= 1 && lb >= 1 && hi > lo && rows && inner && cols, RR_EINVAL,
+                 "bad product arguments");
+        RR_CHECK(inner <= 128 && hi <= la + lb - 1 + 0ull, RR_EINVAL, "inner <= 128 and hi <= la + lb - 1");
+        RR_CHECK(p > 2 && p < (1u << 31) && h_is_prime(p), RR_EINVAL, "p=%u must be a prime in (2, 2^31)", p);
+        hipStream_t st;
+        RR_HIP(rr::stream_create(&st));
+        struct S {
+            hipStream_t s;
+            ~S() { rr::stream_destroy(s); }
+        } guard{st};
+        DevBuf dA, dB, dC;
+        rr_status s;
+        size_t sa = (size_t)rows * inner * la, sbn = (size_t)inner * cols * lb, sc = (size_t)rows * cols * (hi - lo);
+        if ((s = dA.alloc(sa * 4)) != RR_OK) return s;
+        if ((s = dB.alloc(sbn * 4)) != RR_OK) return s;
+        if ((s = dC.alloc(sc * 4)) != RR_OK) return s;
+        for (size_t i = 0; i < sa; i++) RR_CHECK(A[i] < p, RR_EINVAL, "A entry %zu >= p", i);
+        for (size_t i = 0; i < sbn; i++) RR_CHECK(B[i] < p, RR_EINVAL, "B entry %zu >= p", i);
+        RR_HIP(hipMemcpyAsync(dA.ptr, A, sa * 4, hipMemcpyHostToDevice, st));
+        RR_HIP(hipMemcpyAsync(dB.ptr, B, sbn * 4, hipMemcpyHostToDevice, st));
+        MulDesc D{dA.as<uint32_t>(), dB.as<uint32_t>(), dC.as<uint32_t>(), lb, (uint32_t)la, (uint32_t)lb, 0,
+                  (uint32_t)lo, (uint32_t)hi, rows, inner, cols};
+        PmScheduler S;
+        S.p = p;
+        S.st = st;
+        if (use_school(D, p)) {
+            if ((s = launch_school(std::vector<MulDesc>{D}, p, st, nullptr)) != RR_OK) return s;
+        } else {
+            uint32_t logn = ntt_logn_for(D);
+            RR_CHECK(logn <= NTT_MAX_LOGN, RR_EINVAL, "product too long");
+            if ((s = S.init_tables(logn)) != RR_OK) return s;
+            std::vector<NttProduct> ops{{logn, D, nullptr, nullptr}};
+            if ((s = S.run_ntt_products(ops)) != RR_OK) return s;
+        }
+        RR_HIP(hipMemcpyAsync(C, dC.ptr, sc * 4, hipMemcpyDeviceToHost, st));
+        RR_HIP(hipStreamSynchronize(st));
+        return RR_OK;
+    });
 }
 
 }  // extern "C"

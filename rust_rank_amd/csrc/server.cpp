@@ -429,97 +429,105 @@ using namespace rr;
 extern "C" {
 
 rr_status rr_server_create(uint32_t p, uint64_t hbm_budget_bytes, rr_server **out) {
-    clear_error();
-    RR_CHECK(out, RR_EINVAL, "null argument");
-    RR_CHECK(p > 2 && p < (1u << 31) && h_is_prime(p), RR_EINVAL, "p=%u must be a prime in (2, 2^31)", p);
-    std::unique_ptr<rr_server> S(new rr_server());
-    S->p = p;
-    S->budget = hbm_budget_bytes;
-    RR_HIP(hipGetDevice(&S->device));
-    RR_HIP(hipDeviceGetAttribute(&S->cus, hipDeviceAttributeMultiprocessorCount, S->device));
-    RR_HIP(rr::stream_create(&S->st_k));
-    RR_HIP(rr::stream_create(&S->st_pm));
-    rr_status s;
-    if ((s = S->ticks.init(S->st_k, S->cus)) != RR_OK) return s;
-    if ((s = S->pm.init(p, S->st_pm, SERVER_MAX_LOGN, 128, 64)) != RR_OK) return s;
-    // RR_SERVER_FAIL_AT=tick:<id> or pm:<id> (tests of failure handling): the
-    // ticks or PM-basis rounds holding that request fail before queueing work
-    if (const char *f = getenv("RR_SERVER_FAIL_AT")) {
-        const uint64_t id = strtoull(strchr(f, ':') ? strchr(f, ':') + 1 : f, nullptr, 10);
-        if (!strncmp(f, "tick:", 5)) S->ticks.fail_id = id;
-        if (!strncmp(f, "pm:", 3)) S->pm.fail_tag = id;
-    }
-    RR_HIP(hipHostMalloc((void **)&S->pinned, SERVER_EXTRACT_SLOTS * EXTRACT_WORDS * 4, hipHostMallocDefault));
-    for (int i = 0; i < SERVER_EXTRACT_SLOTS; i++) S->free_slots.push_back(i);
-    for (int w = 0; w < SERVER_SETUP_THREADS; w++) {
-        hipStream_t ws;
-        RR_HIP(rr::stream_create(&ws));
-        S->setup_st.push_back(ws);
-    }
-    rr_server *raw = S.release();
-    for (int w = 0; w < SERVER_SETUP_THREADS; w++) raw->setup.emplace_back([raw, w] { raw->setup_loop(w); });
-    raw->worker = std::thread([raw] { raw->run(); });
-    *out = raw;
-    return RR_OK;
+    return rr::guarded([&]() -> rr_status {
+        clear_error();
+        RR_CHECK(out, RR_EINVAL, "null argument");
+        RR_CHECK(p > 2 && p < (1u << 31) && h_is_prime(p), RR_EINVAL, "p=%u must be a prime in (2, 2^31)", p);
+        std::unique_ptr<rr_server> S(new rr_server());
+        S->p = p;
+        S->budget = hbm_budget_bytes;
+        RR_HIP(hipGetDevice(&S->device));
+        RR_HIP(hipDeviceGetAttribute(&S->cus, hipDeviceAttributeMultiprocessorCount, S->device));
+        RR_HIP(rr::stream_create(&S->st_k));
+        RR_HIP(rr::stream_create(&S->st_pm));
+        rr_status s;
+        if ((s = S->ticks.init(S->st_k, S->cus)) != RR_OK) return s;
+        if ((s = S->pm.init(p, S->st_pm, SERVER_MAX_LOGN, 128, 64)) != RR_OK) return s;
+        // RR_SERVER_FAIL_AT=tick:<id> or pm:<id> (tests of failure handling): the
+        // ticks or PM-basis rounds holding that request fail before queueing work
+        if (const char *f = getenv("RR_SERVER_FAIL_AT")) {
+            const uint64_t id = strtoull(strchr(f, ':') ? strchr(f, ':') + 1 : f, nullptr, 10);
+            if (!strncmp(f, "tick:", 5)) S->ticks.fail_id = id;
+            if (!strncmp(f, "pm:", 3)) S->pm.fail_tag = id;
+        }
+        RR_HIP(hipHostMalloc((void **)&S->pinned, SERVER_EXTRACT_SLOTS * EXTRACT_WORDS * 4, hipHostMallocDefault));
+        for (int i = 0; i < SERVER_EXTRACT_SLOTS; i++) S->free_slots.push_back(i);
+        for (int w = 0; w < SERVER_SETUP_THREADS; w++) {
+            hipStream_t ws;
+            RR_HIP(rr::stream_create(&ws));
+            S->setup_st.push_back(ws);
+        }
+        rr_server *raw = S.release();
+        for (int w = 0; w < SERVER_SETUP_THREADS; w++) raw->setup.emplace_back([raw, w] { raw->setup_loop(w); });
+        raw->worker = std::thread([raw] { raw->run(); });
+        *out = raw;
+        return RR_OK;
+    });
 }
 
 rr_status rr_server_submit(rr_server *S, const rr_csr *A, uint32_t b, const uint32_t *row_precond,
                            const uint32_t *col_precond, const uint32_t *V, uint64_t seq_len, uint64_t order,
                            uint64_t *id_out) {
-    clear_error();
-    RR_CHECK(S && A && row_precond && col_precond && V && id_out, RR_EINVAL, "null argument");
-    RR_CHECK(b >= 1 && b <= 64, RR_EINVAL, "b=%u must be in 1..64", b);
-    RR_CHECK(A->m >= 1 && A->n >= 1 && A->row_ptr, RR_EINVAL, "empty matrix");
-    const uint64_t nnz = A->row_ptr[A->m];
-    RR_CHECK(nnz == 0 || (A->col && A->val), RR_EINVAL, "null col/val");
-    std::unique_ptr<Pending> q(new Pending());
-    q->b = b;
-    q->m = A->m;
-    q->n = A->n;
-    q->L = seq_len ? seq_len : 2 * std::min(A->m, A->n) / b + 8;
-    q->d = order ? order : q->L - 1;
-    RR_CHECK(q->d + 1 <= (1ull << SERVER_MAX_LOGN), RR_EINVAL,
-             "order %llu above the server's 2^%u-point transforms (rr_rank takes it)", (unsigned long long)q->d,
-             SERVER_MAX_LOGN);
-    q->rp.assign(A->row_ptr, A->row_ptr + A->m + 1);
-    q->col.assign(A->col, A->col + nnz);
-    q->val.assign(A->val, A->val + nnz);
-    q->dr.assign(row_precond, row_precond + A->m);
-    q->dc.assign(col_precond, col_precond + A->n);
-    q->V.assign(V, V + A->n * b);
-    q->t_arrive = now_ms();
-    std::lock_guard<std::mutex> g(S->mu);
-    RR_CHECK(!S->broken, RR_EDEVICE, "the server's worker could not start");
-    q->id = S->next_id++;
-    *id_out = q->id;
-    S->queue.push_back(std::move(q));
-    S->cv.notify_all();
-    return RR_OK;
+    return rr::guarded([&]() -> rr_status {
+        clear_error();
+        RR_CHECK(S && A && row_precond && col_precond && V && id_out, RR_EINVAL, "null argument");
+        RR_CHECK(b >= 1 && b <= 64, RR_EINVAL, "b=%u must be in 1..64", b);
+        RR_CHECK(A->m >= 1 && A->n >= 1 && A->row_ptr, RR_EINVAL, "empty matrix");
+        const uint64_t nnz = A->row_ptr[A->m];
+        RR_CHECK(nnz == 0 || (A->col && A->val), RR_EINVAL, "null col/val");
+        std::unique_ptr<Pending> q(new Pending());
+        q->b = b;
+        q->m = A->m;
+        q->n = A->n;
+        q->L = seq_len ? seq_len : 2 * std::min(A->m, A->n) / b + 8;
+        q->d = order ? order : q->L - 1;
+        RR_CHECK(q->d + 1 <= (1ull << SERVER_MAX_LOGN), RR_EINVAL,
+                 "order %llu above the server's 2^%u-point transforms (rr_rank takes it)", (unsigned long long)q->d,
+                 SERVER_MAX_LOGN);
+        q->rp.assign(A->row_ptr, A->row_ptr + A->m + 1);
+        q->col.assign(A->col, A->col + nnz);
+        q->val.assign(A->val, A->val + nnz);
+        q->dr.assign(row_precond, row_precond + A->m);
+        q->dc.assign(col_precond, col_precond + A->n);
+        q->V.assign(V, V + A->n * b);
+        q->t_arrive = now_ms();
+        std::lock_guard<std::mutex> g(S->mu);
+        RR_CHECK(!S->broken, RR_EDEVICE, "the server's worker could not start");
+        q->id = S->next_id++;
+        *id_out = q->id;
+        S->queue.push_back(std::move(q));
+        S->cv.notify_all();
+        return RR_OK;
+    });
 }
 
 rr_status rr_server_poll(rr_server *S, rr_result *out, uint32_t cap, uint32_t *n_out, int wait_ms) {
-    RR_CHECK(S && n_out && (out || cap == 0), RR_EINVAL, "null argument");
-    std::unique_lock<std::mutex> lk(S->mu);
-    if (S->results.empty() && wait_ms > 0)
-        S->cv.wait_for(lk, std::chrono::milliseconds(wait_ms), [S] { return !S->results.empty(); });
-    uint32_t n = (uint32_t)std::min<size_t>(cap, S->results.size());
-    for (uint32_t i = 0; i < n; i++) out[i] = S->results[i];
-    S->results.erase(S->results.begin(), S->results.begin() + n);
-    *n_out = n;
-    return RR_OK;
+    return rr::guarded([&]() -> rr_status {
+        RR_CHECK(S && n_out && (out || cap == 0), RR_EINVAL, "null argument");
+        std::unique_lock<std::mutex> lk(S->mu);
+        if (S->results.empty() && wait_ms > 0)
+            S->cv.wait_for(lk, std::chrono::milliseconds(wait_ms), [S] { return !S->results.empty(); });
+        uint32_t n = (uint32_t)std::min<size_t>(cap, S->results.size());
+        for (uint32_t i = 0; i < n; i++) out[i] = S->results[i];
+        S->results.erase(S->results.begin(), S->results.begin() + n);
+        *n_out = n;
+        return RR_OK;
+    });
 }
 
 rr_status rr_server_stats_get(rr_server *S, rr_server_stats *out) {
-    RR_CHECK(S && out, RR_EINVAL, "null argument");
-    std::lock_guard<std::mutex> g(S->mu);
-    out->queued = S->queue.size();
-    out->used_bytes = S->used;
-    out->completed = S->completed;
-    out->failed = S->failed;
-    out->ticks = S->ticks.ticks;
-    out->pm_rounds = S->pm_rounds;
-    out->max_active = S->max_active;
-    return RR_OK;
+    return rr::guarded([&]() -> rr_status {
+        RR_CHECK(S && out, RR_EINVAL, "null argument");
+        std::lock_guard<std::mutex> g(S->mu);
+        out->queued = S->queue.size();
+        out->used_bytes = S->used;
+        out->completed = S->completed;
+        out->failed = S->failed;
+        out->ticks = S->ticks.ticks;
+        out->pm_rounds = S->pm_rounds;
+        out->max_active = S->max_active;
+        return RR_OK;
+    });
 }
 
 void rr_server_destroy(rr_server *S) {

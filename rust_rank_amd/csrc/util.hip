@@ -214,96 +214,103 @@ extern "C" {
 
 rr_status rr_gather_probe(uint64_t rows, uint64_t gathers, uint32_t iters, int pass_mix, double *ms_out,
                           double *gbs_out) {
-    clear_error();
-    RR_CHECK(rows >= 1 && rows <= (1ull << 32) && gathers >= 1 && iters >= 1 && ms_out && gbs_out, RR_EINVAL,
-             "bad gather probe args");
-    int dev = 0, cus = 0;
-    RR_HIP(hipGetDevice(&dev));
-    RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const uint32_t grid = (uint32_t)cus * 16;
-    const uint64_t nout = gathers / 20 + 64;  // one row per 20 gathers
-    DevBuf X, idx, out, val, Y;
-    rr_status s;
-    if ((s = X.alloc(rows * 256)) != RR_OK) return s;
-    if ((s = idx.alloc(gathers * 4)) != RR_OK) return s;
-    if ((s = out.alloc((size_t)grid * 256 * 4)) != RR_OK) return s;
-    if (pass_mix) {
-        if ((s = val.alloc(gathers * 4)) != RR_OK) return s;
-        if ((s = Y.alloc(nout * 256)) != RR_OK) return s;
-    }
-    hipStream_t st;
-    RR_HIP(stream_create(&st));
-    struct S {
-        hipStream_t s;
-        ~S() { stream_destroy(s); }
-    } guard{st};
-    RR_HIP(hipMemsetAsync(X.ptr, 1, rows * 256, st));
-    if (pass_mix) RR_HIP(hipMemsetAsync(val.ptr, 3, gathers * 4, st));
-    hipLaunchKernelGGL(k_gather_index, dim3(grid), dim3(256), 0, st, idx.as<uint32_t>(), gathers, rows, 12345ull);
-    const uint32_t *vp = pass_mix ? val.as<uint32_t>() : nullptr;
-    uint4 *yp = pass_mix ? Y.as<uint4>() : nullptr;
-    hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, st, X.as<uint4>(), idx.as<uint32_t>(), vp, gathers, yp,
-                       nout, out.as<uint32_t>());
-    hipEvent_t e0, e1;
-    RR_HIP(hipEventCreate(&e0));
-    RR_HIP(hipEventCreate(&e1));
-    RR_HIP(hipEventRecord(e0, st));
-    for (uint32_t i = 0; i < iters; i++)
-        hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, st, X.as<uint4>(), idx.as<uint32_t>(), vp, gathers,
-                           yp, nout, out.as<uint32_t>());
-    RR_HIP(hipEventRecord(e1, st));
-    RR_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    RR_HIP(hipEventElapsedTime(&ms, e0, e1));
-    *ms_out = ms / iters;
-    const double bytes = (double)gathers * (pass_mix ? 264.0 + 256.0 / 20.0 : 260.0);
-    *gbs_out = bytes / (*ms_out * 1e-3) / 1e9;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return RR_OK;
+    return rr::guarded([&]() -> rr_status {
+        clear_error();
+        RR_CHECK(rows >= 1 && rows <= (1ull << 32) && gathers >= 1 && iters >= 1 && ms_out && gbs_out, RR_EINVAL,
+                 "bad gather probe args");
+        int dev = 0, cus = 0;
+        RR_HIP(hipGetDevice(&dev));
+        RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const uint32_t grid = (uint32_t)cus * 16;
+        const uint64_t nout = gathers / 20 + 64;  // one row per 20 gathers
+        DevBuf X, idx, out, val, Y;
+        rr_status s;
+        if ((s = X.alloc(rows * 256)) != RR_OK) return s;
+        if ((s = idx.alloc(gathers * 4)) != RR_OK) return s;
+        if ((s = out.alloc((size_t)grid * 256 * 4)) != RR_OK) return s;
+        if (pass_mix) {
+            if ((s = val.alloc(gathers * 4)) != RR_OK) return s;
+            if ((s = Y.alloc(nout * 256)) != RR_OK) return s;
+        }
+        hipStream_t st;
+        RR_HIP(stream_create(&st));
+        struct S {
+            hipStream_t s;
+            ~S() { stream_destroy(s); }
+        } guard{st};
+        RR_HIP(hipMemsetAsync(X.ptr, 1, rows * 256, st));
+        if (pass_mix) RR_HIP(hipMemsetAsync(val.ptr, 3, gathers * 4, st));
+        hipLaunchKernelGGL(k_gather_index, dim3(grid), dim3(256), 0, st, idx.as<uint32_t>(), gathers, rows, 12345ull);
+        const uint32_t *vp = pass_mix ? val.as<uint32_t>() : nullptr;
+        uint4 *yp = pass_mix ? Y.as<uint4>() : nullptr;
+        hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, st, X.as<uint4>(), idx.as<uint32_t>(), vp,
+                           gathers, yp, nout, out.as<uint32_t>());
+        hipEvent_t e0, e1;
+        RR_HIP(hipEventCreate(&e0));
+        RR_HIP(hipEventCreate(&e1));
+        RR_HIP(hipEventRecord(e0, st));
+        for (uint32_t i = 0; i < iters; i++)
+            hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, st, X.as<uint4>(), idx.as<uint32_t>(), vp,
+                               gathers, yp, nout, out.as<uint32_t>());
+        RR_HIP(hipEventRecord(e1, st));
+        RR_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        RR_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *ms_out = ms / iters;
+        const double bytes = (double)gathers * (pass_mix ? 264.0 + 256.0 / 20.0 : 260.0);
+        *gbs_out = bytes / (*ms_out * 1e-3) / 1e9;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return RR_OK;
+    });
 }
 
 rr_status rr_release_device_memory(void) {
-    clear_error();
-    int dev = 0;
-    RR_HIP(hipGetDevice(&dev));
-    dev_cache_release(dev);
-    return RR_OK;
+    return rr::guarded([&]() -> rr_status {
+        clear_error();
+        int dev = 0;
+        RR_HIP(hipGetDevice(&dev));
+        dev_cache_release(dev);
+        return RR_OK;
+    });
 }
 
 // Copies `bytes` (multiple of 16) device-to-device with k_copy, `iters` times;
 // ms_out receives the average kernel time and gbs_out read+write GB/s.
 rr_status rr_copy_probe(uint64_t bytes, uint32_t iters, double *ms_out, double *gbs_out) {
-    clear_error();
-    RR_CHECK(bytes >= 16 && bytes % 16 == 0 && iters >= 1 && ms_out && gbs_out, RR_EINVAL, "bad copy probe args");
-    DevBuf a, b;
-    rr_status s;
-    if ((s = a.alloc(bytes)) != RR_OK) return s;
-    if ((s = b.alloc(bytes)) != RR_OK) return s;
-    hipStream_t st;
-    RR_HIP(stream_create(&st));
-    RR_HIP(hipMemsetAsync(a.ptr, 1, bytes, st));
-    int dev = 0, cus = 0;
-    RR_HIP(hipGetDevice(&dev));
-    RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    size_t n = bytes / 16;
-    dim3 grid((unsigned)std::min<size_t>((n + 255) / 256, (size_t)cus * 16));
-    hipLaunchKernelGGL(k_copy, grid, dim3(256), 0, st, a.as<uint4>(), b.as<uint4>(), n);
-    hipEvent_t e0, e1;
-    RR_HIP(hipEventCreate(&e0));
-    RR_HIP(hipEventCreate(&e1));
-    RR_HIP(hipEventRecord(e0, st));
-    for (uint32_t i = 0; i < iters; i++) hipLaunchKernelGGL(k_copy, grid, dim3(256), 0, st, a.as<uint4>(), b.as<uint4>(), n);
-    RR_HIP(hipEventRecord(e1, st));
-    RR_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    RR_HIP(hipEventElapsedTime(&ms, e0, e1));
-    *ms_out = ms / iters;
-    *gbs_out = 2.0 * (double)bytes / (*ms_out * 1e-3) / 1e9;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    stream_destroy(st);
-    return RR_OK;
+    return rr::guarded([&]() -> rr_status {
+        clear_error();
+        RR_CHECK(bytes >= 16 && bytes % 16 == 0 && iters >= 1 && ms_out && gbs_out, RR_EINVAL, "bad copy probe args");
+        DevBuf a, b;
+        rr_status s;
+        if ((s = a.alloc(bytes)) != RR_OK) return s;
+        if ((s = b.alloc(bytes)) != RR_OK) return s;
+        hipStream_t st;
+        RR_HIP(stream_create(&st));
+        RR_HIP(hipMemsetAsync(a.ptr, 1, bytes, st));
+        int dev = 0, cus = 0;
+        RR_HIP(hipGetDevice(&dev));
+        RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        size_t n = bytes / 16;
+        dim3 grid((unsigned)std::min<size_t>((n + 255) / 256, (size_t)cus * 16));
+        hipLaunchKernelGGL(k_copy, grid, dim3(256), 0, st, a.as<uint4>(), b.as<uint4>(), n);
+        hipEvent_t e0, e1;
+        RR_HIP(hipEventCreate(&e0));
+        RR_HIP(hipEventCreate(&e1));
+        RR_HIP(hipEventRecord(e0, st));
+        for (uint32_t i = 0; i < iters; i++)
+            hipLaunchKernelGGL(k_copy, grid, dim3(256), 0, st, a.as<uint4>(), b.as<uint4>(), n);
+        RR_HIP(hipEventRecord(e1, st));
+        RR_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        RR_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *ms_out = ms / iters;
+        *gbs_out = 2.0 * (double)bytes / (*ms_out * 1e-3) / 1e9;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        stream_destroy(st);
+        return RR_OK;
+    });
 }
 
 }  // extern "C"
