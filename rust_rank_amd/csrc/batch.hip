@@ -205,17 +205,18 @@ rr_status Request::init(const rr_csr *A, uint32_t p_, uint32_t b_, const uint32_
     RR_CHECK(nnz == 0 || (A->col && A->val), RR_EINVAL, "null col/val");
     for (uint64_t i = 0; i < m; i++) RR_CHECK(dr[i] >= 1 && dr[i] < p, RR_EINVAL, "row_precond out of range");
     for (uint64_t i = 0; i < n; i++) RR_CHECK(dcv[i] >= 1 && dcv[i] < p, RR_EINVAL, "col_precond out of range");
-    for (uint64_t i = 0; i < n * b; i++) RR_CHECK(V[i] < p, RR_EINVAL, "V out of range");
 
     std::vector<uint32_t> dc2(n);
-    for (uint64_t i = 0; i < n; i++) dc2[i] = h_mulmod(dcv[i], dcv[i], p);
-    std::vector<uint32_t> xpad((size_t)n * B, 0);
-    for (uint64_t r = 0; r < n; r++) std::memcpy(&xpad[r * B], &V[r * b], b * 4);
+    {
+        const HostMod hm(p);
+        for (uint64_t i = 0; i < n; i++) dc2[i] = hm.mul(dcv[i], dcv[i]);
+    }
     rr_status s;
     if ((s = upload_vec(dr_d, std::vector<uint32_t>(dr, dr + m), st)) != RR_OK) return s;
     if ((s = upload_vec(dc_d, std::vector<uint32_t>(dcv, dcv + n), st)) != RR_OK) return s;
     if ((s = upload_vec(dc2_d, dc2, st)) != RR_OK) return s;
-    if ((s = upload_vec(x0, xpad, st)) != RR_OK) return s;
+    // V < p is checked on the device as it is uploaded (padded to stride B there)
+    if ((s = upload_block_vector(x0, V, n, b, B, p, "V", st)) != RR_OK) return s;
     if ((s = z.alloc((size_t)n * B * 4)) != RR_OK) return s;
     if ((s = y.alloc((size_t)m * B * 4)) != RR_OK) return s;
     if ((s = seq.alloc((size_t)L * P * 4)) != RR_OK) return s;

@@ -89,6 +89,19 @@ __device__ __forceinline__ uint32_t submod(uint32_t a, uint32_t b, uint32_t p) {
 // ---------------------------------------------------------------------------
 // host helpers
 static inline uint32_t h_mulmod(uint32_t a, uint32_t b, uint32_t p) { return (uint32_t)(((uint64_t)a * b) % p); }
+// Products mod a fixed p < 2^31 by Barrett reduction: m = floor((2^64 - 1) / p)
+// gives a quotient estimate at most two below the true one, so no division.
+struct HostMod {
+    uint32_t p;
+    uint64_t m;
+    explicit HostMod(uint32_t p_) : p(p_), m(~0ull / p_) {}
+    uint32_t mul(uint32_t a, uint32_t b) const {
+        const uint64_t x = (uint64_t)a * b;
+        uint64_t r = x - (uint64_t)(((unsigned __int128)x * m) >> 64) * p;
+        while (r >= p) r -= p;
+        return (uint32_t)r;
+    }
+};
 static inline uint32_t h_powmod(uint32_t a, uint64_t e, uint32_t p) {
     uint64_t r = 1, x = a % p;
     while (e) {

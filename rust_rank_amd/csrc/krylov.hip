@@ -840,7 +840,11 @@ rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> 
     // longest processing time first
     std::vector<uint32_t> order(sls.size());
     for (size_t i = 0; i < order.size(); i++) order[i] = (uint32_t)i;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return sls[x].nblk > sls[y].nblk; });
+    {
+        auto longer = [&](uint32_t x, uint32_t y) { return sls[x].nblk > sls[y].nblk; };
+        if (!std::is_sorted(order.begin(), order.end(), longer))  // row slices alone already are
+            std::stable_sort(order.begin(), order.end(), longer);
+    }
     typedef std::pair<uint64_t, uint32_t> Load;
     std::vector<std::vector<uint32_t>> mine(nw);
     std::vector<uint64_t> load(nw, 0);
@@ -866,14 +870,41 @@ rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> 
         load[l.second] = l.first;
         if (++nsegw[l.second] < MAX_PEND) xh[x].push(l);
     }
-    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
-    for (uint32_t w = 0; w < nw; w++) heap.push(Load(load[w], w));
-    for (uint32_t i : order) {
-        if (sls[i].seg != NO_ROW) continue;  // placed above
-        Load l = heap.top();
-        heap.pop();
-        mine[l.second].push_back(i);
-        heap.push(Load(l.first + (uint64_t)sls[i].nblk * SU + FIN, l.second));
+    // Row slices: each to the least loaded wavefront (ties to the lower
+    // index).  Over a run of equal costs c on loads that differ by less than
+    // c, that rule deals the run round-robin in (load, index) order: every
+    // wavefront not yet dealt to has less load than every one that has been.
+    // So such runs go round-robin in O(1) a slice, the rest one at a time.
+    std::vector<uint32_t> rows_order;
+    rows_order.reserve(order.size());
+    for (uint32_t i : order)
+        if (sls[i].seg == NO_ROW) rows_order.push_back(i);  // segments were placed above
+    auto cost = [&](size_t k) { return (uint64_t)sls[rows_order[k]].nblk * SU + FIN; };
+    std::vector<Load> h(nw);
+    for (uint32_t w = 0; w < nw; w++) h[w] = Load(load[w], w);
+    std::make_heap(h.begin(), h.end(), std::greater<Load>());
+    uint64_t maxl = *std::max_element(load.begin(), load.end());
+    for (size_t k = 0; k < rows_order.size();) {
+        const uint64_t c = cost(k);
+        size_t e = k;
+        while (e < rows_order.size() && cost(e) == c) e++;
+        while (k < e && maxl - h.front().first >= c) {
+            std::pop_heap(h.begin(), h.end(), std::greater<Load>());
+            Load &l = h.back();
+            mine[l.second].push_back(rows_order[k++]);
+            l.first += c;
+            maxl = std::max(maxl, l.first);
+            std::push_heap(h.begin(), h.end(), std::greater<Load>());
+        }
+        if (k == e) continue;
+        std::sort(h.begin(), h.end());
+        const uint64_t cnt = e - k, full = cnt / nw, rem = cnt % nw;
+        for (uint64_t t = 0; t < cnt; t++) mine[h[t % nw].second].push_back(rows_order[k + t]);
+        for (uint32_t j = 0; j < nw; j++) h[j].first += c * (full + (j < rem ? 1 : 0));
+        maxl = 0;
+        for (const Load &l : h) maxl = std::max(maxl, l.first);
+        std::make_heap(h.begin(), h.end(), std::greater<Load>());
+        k = e;
     }
 
     // a wavefront's segment slices in order of their gathered rows, so the
@@ -1238,8 +1269,7 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
         for (uint64_t i = 0; i < n; i++)
             RR_CHECK(col_precond[i] >= 1 && col_precond[i] < p, RR_EINVAL, "col_precond[%llu] not in [1,p)",
                      (unsigned long long)i);
-        for (uint64_t i = 0; i < n * b; i++)
-            RR_CHECK(curv[i] < p, RR_EINVAL, "curv[%llu] >= p", (unsigned long long)i);
+        // curv < p is checked on the device as it is uploaded
 
         std::unique_ptr<rr_krylov> c(new rr_krylov());
         c->p = p;
@@ -1258,11 +1288,11 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
         RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
         c->grid = 0;  // set below from the occupancy of the chosen pass kernel
 
-        // preconditioners: D_r, D_c, D_c^2, D_c^-1
-        std::vector<uint32_t> dc2(n), dcinv(n);
-        for (uint64_t i = 0; i < n; i++) {
-            dc2[i] = h_mulmod(col_precond[i], col_precond[i], p);
-            dcinv[i] = h_invmod(col_precond[i], p);
+        // preconditioners: D_r, D_c, D_c^2 (the host lays out A^T's rows with it), D_c^-1 (device)
+        std::vector<uint32_t> dc2(n);
+        {
+            const HostMod hm(p);
+            for (uint64_t i = 0; i < n; i++) dc2[i] = hm.mul(col_precond[i], col_precond[i]);
         }
         c->sep_proj = c->B == 64;
         // A and A^T in sliced form for one resident wave of workgroups (every
@@ -1282,7 +1312,9 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
         if ((s = upload(c->dr, row_precond, m)) != RR_OK) return s;
         if ((s = upload(c->dc, col_precond, n)) != RR_OK) return s;
         if ((s = upload(c->dc2, dc2.data(), n)) != RR_OK) return s;
-        if ((s = upload(c->dcinv, dcinv.data(), n)) != RR_OK) return s;
+        if ((s = c->dcinv.alloc((size_t)n * 4)) != RR_OK) return s;
+        hipLaunchKernelGGL(k_inv_mod, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                           c->dc.as<uint32_t>(), c->dcinv.as<uint32_t>(), n, p, c->mp.inv);
         if (c->sep_proj) {
             int occ = 0;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_proj64, WG, 0) != hipSuccess || occ < 1) occ = 1;
@@ -1299,11 +1331,9 @@ rr_status rr_krylov_create(const rr_csr *A, uint32_t p, uint32_t b, const uint32
             RR_HIP(hipGetLastError());
         }
 
-        // block vectors, padded to stride B
+        // block vectors, padded to stride B on the device
         const uint32_t B = c->B;
-        std::vector<uint32_t> xpad((size_t)n * B, 0);
-        for (uint64_t r = 0; r < n; r++) std::memcpy(&xpad[r * B], &curv[r * b], b * 4);
-        if ((s = upload(c->xtmp, xpad.data(), xpad.size())) != RR_OK) return s;
+        if ((s = upload_block_vector(c->xtmp, curv, n, b, B, p, "curv", c->stream)) != RR_OK) return s;
         if ((s = c->z.alloc((size_t)n * B * 4)) != RR_OK) return s;
         if ((s = c->y.alloc((size_t)m * B * 4)) != RR_OK) return s;
         RR_HIP(hipMemsetAsync(c->y.ptr, 0, (size_t)m * B * 4, c->stream));

@@ -120,6 +120,54 @@ void launch_fill_slices(const DevOperand &op, const uint32_t *shortr, uint32_t n
                        shortr, nshort, sl, nsl, RPW, SU, cw, vw);
 }
 
+// The first index i with v[i] >= p (atomic minimum; a thread stops at its
+// first such index, all later ones of its stride being larger).
+__global__ void k_first_not_below(const uint32_t *__restrict__ v, uint64_t n, uint32_t p,
+                                  unsigned long long *first) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        if (v[i] >= p) {
+            atomicMin(first, (unsigned long long)i);
+            break;
+        }
+}
+
+// Rows of b words to rows of B words, zero past column b.
+__global__ void k_pad_rows(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, uint64_t nrows, uint32_t b,
+                           uint32_t B) {
+    const uint64_t n = nrows * B;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t r = i / B;
+        const uint32_t j = (uint32_t)(i - r * B);
+        out[i] = j < b ? in[r * b + j] : 0u;
+    }
+}
+
+rr_status upload_block_vector(DevBuf &out, const uint32_t *V, uint64_t n, uint32_t b, uint32_t B, uint32_t p,
+                              const char *name, hipStream_t st) {
+    RR_CHECK(b >= 1 && b <= B, RR_EINVAL, "block width b=%u above the row stride %u", b, B);
+    rr_status s;
+    if ((s = out.alloc((size_t)n * B * 4)) != RR_OK) return s;
+    DevBuf raw, flag;
+    uint32_t *src = out.as<uint32_t>();
+    if (b != B) {
+        if ((s = raw.alloc((size_t)n * b * 4)) != RR_OK) return s;
+        src = raw.as<uint32_t>();
+    }
+    if ((s = flag.alloc(sizeof(unsigned long long))) != RR_OK) return s;
+    RR_HIP(hipMemcpyAsync(src, V, (size_t)n * b * 4, hipMemcpyHostToDevice, st));
+    RR_HIP(hipMemsetAsync(flag.ptr, 0xFF, sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(k_first_not_below, dim3(grid_for(n * b)), dim3(256), 0, st, src, n * b, p,
+                       flag.as<unsigned long long>());
+    if (b != B)
+        hipLaunchKernelGGL(k_pad_rows, dim3(grid_for(n * B)), dim3(256), 0, st, src, out.as<uint32_t>(), n, b, B);
+    RR_HIP(hipGetLastError());
+    unsigned long long first = 0;
+    RR_HIP(hipMemcpyAsync(&first, flag.ptr, sizeof first, hipMemcpyDeviceToHost, st));
+    RR_HIP(hipStreamSynchronize(st));  // also before raw goes back to the cache
+    RR_CHECK(first == ~0ull, RR_EINVAL, "%s[%llu] >= p", name, first);
+    return RR_OK;
+}
+
 template <typename T>
 static rr_status upload_async(DevBuf &buf, const T *host, size_t count, hipStream_t st) {
     rr_status s = buf.alloc(count * sizeof(T));

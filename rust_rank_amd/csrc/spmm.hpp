@@ -133,6 +133,13 @@ rr_status setup_operands(const rr_csr *A, uint32_t p, uint32_t B, uint32_t su_re
                          const uint32_t *row_scale, const uint32_t *col_scale2, SliceMat &sa, SliceMat &sat,
                          OperandInfo *info, hipStream_t st);
 
+// The n x b block vector V (row-major) to the device as n rows of B >= b
+// words, zero past column b, after a check on the device that every word is
+// below p: the first one that is not is reported as name[index], as a host
+// loop over the words would report it.  Synchronises st.
+rr_status upload_block_vector(DevBuf &out, const uint32_t *V, uint64_t n, uint32_t b, uint32_t B, uint32_t p,
+                              const char *name, hipStream_t st);
+
 SpmmArgs spmm_args(const SliceMat &M, const uint32_t *in, uint64_t in_rows, uint32_t B, uint32_t *out,
                    const uint32_t *scale, uint32_t p, uint32_t b);
 

@@ -110,6 +110,12 @@ def test_batch_rejects_invalid_matrix_and_recovers():
     with pytest.raises(RuntimeError, match="matrix 7"):
         eng.batch_rank(mats, P, [8] * 20, [x[0] for x in pre], [x[1] for x in pre], [x[2] for x in pre])
     mats[7] = bad
+    # a start-vector word >= p, found on the device during that matrix's set-up
+    V5 = pre[5][2].copy()
+    V5[3, 1] = P
+    with pytest.raises(RuntimeError, match=r"matrix 5: V\[%d\] >= p" % (3 * 8 + 1)):
+        eng.batch_rank(mats, P, [8] * 20, [x[0] for x in pre], [x[1] for x in pre],
+                       [x[2] if i != 5 else V5 for i, x in enumerate(pre)])
     ranks, _ = eng.batch_rank(mats[:6], P, [8] * 6, [x[0] for x in pre[:6]], [x[1] for x in pre[:6]],
                               [x[2] for x in pre[:6]])
     for i in range(6):

@@ -165,6 +165,14 @@ def test_bad_inputs_raise():
     D = synth.Csr(A.m, A.n, rp, A.col, A.val)
     with pytest.raises(eng.RankEngineError):
         eng.Krylov(D, P, 4, dr, dc, V)
+    # start vectors are checked on the device: the first word >= p is named,
+    # at b = B (uploaded as they are) and at b < B (padded there)
+    for bb in (4, 3):
+        Vb = np.ascontiguousarray(V[:, :bb])
+        Vb[11, 2] = P + 5
+        Vb[30, 0] = P
+        with pytest.raises(eng.RankEngineError, match=r"curv\[%d\] >= p" % (11 * bb + 2)):
+            eng.Krylov(A, P, bb, dr, dc, Vb)
     with pytest.raises(eng.RankEngineError):
         eng.Krylov(A, P, 4, dr, dc, V, t_have=2)
 
