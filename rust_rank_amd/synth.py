@@ -207,6 +207,33 @@ def _sample_fixed_k(rng: Rng, m: int, k: np.ndarray, draw) -> np.ndarray:
     return np.concatenate(pieces)
 
 
+def csr_from_dense(D: np.ndarray) -> Csr:
+    """The nonzero entries of a dense matrix (values already in [0, p)) as CSR,
+    row by row in column order."""
+    rp, cols, vals = [0], [], []
+    for i in range(D.shape[0]):
+        nz = np.nonzero(D[i])[0]
+        cols += nz.tolist()
+        vals += D[i, nz].tolist()
+        rp.append(len(cols))
+    return Csr(int(D.shape[0]), int(D.shape[1]), np.array(rp, np.uint64), np.array(cols, np.uint32),
+               np.array(vals, np.uint32))
+
+
+def edge_shapes(p: int = P_DEFAULT, seed: int = 5) -> dict:
+    """Degenerate dense matrices for end-to-end checks: no nonzeros, 1 x 1,
+    one row, one column, a full-rank diagonal, 3 x 2."""
+    rng = np.random.default_rng(seed)
+    return {
+        "zero 50x40": np.zeros((50, 40), np.uint64),
+        "1x1": np.array([[5]], np.uint64),
+        "row 1x30": rng.integers(0, p, (1, 30)).astype(np.uint64),
+        "column 30x1": rng.integers(0, p, (30, 1)).astype(np.uint64),
+        "diagonal 40": np.diag(rng.integers(1, p, 40)).astype(np.uint64),
+        "3x2": rng.integers(0, p, (3, 2)).astype(np.uint64),
+    }
+
+
 def small_values(rng: Rng, nnz: int, p: int) -> np.ndarray:
     """+-1 (70 %) and +-2 (30 %), equal signs, stored mod p."""
     mag = np.where(rng.uniform(nnz) < 0.7, 1, 2)

@@ -16,6 +16,22 @@ def _eng():
     return engine
 
 
+def test_batch_degenerate_shapes_beside_ordinary_matrices():
+    """No nonzeros, 1 x 1, one row, one column and b above n, batched with
+    ordinary matrices: every rank equals the single path and dense
+    elimination."""
+    eng = _eng()
+    mats = [synth.csr_from_dense(D) for D in synth.edge_shapes(P).values()]
+    mats += [synth.batch_matrix(23, i, rows_lo=200, rows_hi=600) for i in range(3)]
+    bs = [(1, 2, 4)[i % 3] for i in range(len(mats))]
+    pre = [synth.preconditioners(700 + i, A.m, A.n, b, P) for i, (A, b) in enumerate(zip(mats, bs))]
+    print(f"batch of {len(mats)}: shapes {[(A.m, A.n) for A in mats]}, b {bs}", flush=True)
+    ranks, _ = eng.batch_rank(mats, P, bs, [x[0] for x in pre], [x[1] for x in pre], [x[2] for x in pre])
+    for i, A in enumerate(mats):
+        single, _ = eng.rank(A, P, bs[i], *pre[i])
+        assert ranks[i] == single == O.dense_rank(A.to_dense(P), P), (i, ranks[i], single)
+
+
 def test_batch_matches_single_and_dense_ge():
     eng = _eng()
     mats, bs, drs, dcs, Vs = [], [], [], [], []

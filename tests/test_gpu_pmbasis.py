@@ -381,26 +381,7 @@ def test_pm_basis_order_above_2_20(p):
     assert np.all(nzmax <= -delta)
 
 
-def _csr_from_dense(D):
-    rp, cols, vals = [0], [], []
-    for i in range(D.shape[0]):
-        nz = np.nonzero(D[i])[0]
-        cols += nz.tolist()
-        vals += D[i, nz].tolist()
-        rp.append(len(cols))
-    return synth.Csr(D.shape[0], D.shape[1], np.array(rp, np.uint64), np.array(cols, np.uint32),
-                     np.array(vals, np.uint32))
-
-
-_RNG = np.random.default_rng(5)
-_EDGE = {
-    "zero 50x40": np.zeros((50, 40), np.uint64),
-    "1x1": np.array([[5]], np.uint64),
-    "row 1x30": _RNG.integers(0, P, (1, 30)).astype(np.uint64),
-    "column 30x1": _RNG.integers(0, P, (30, 1)).astype(np.uint64),
-    "diagonal 40": np.diag(_RNG.integers(1, P, 40)).astype(np.uint64),
-    "3x2": _RNG.integers(0, P, (3, 2)).astype(np.uint64),
-}
+_EDGE = synth.edge_shapes(P)
 
 
 @pytest.mark.parametrize("b", [1, 2, 4])
@@ -411,7 +392,7 @@ def test_rank_degenerate_shapes(name, b):
     (order L - 1) equals the oracle's pipeline, which here equals dense
     elimination."""
     eng = _eng()
-    A = _csr_from_dense(_EDGE[name])
+    A = synth.csr_from_dense(_EDGE[name])
     dr, dc, V = synth.preconditioners(3, A.m, A.n, b, P)
     L = eng.seq_len(A.m, A.n, b)
     print(f"{name} b={b} L={L}", flush=True)

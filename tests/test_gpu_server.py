@@ -59,6 +59,24 @@ def test_server_results_equal_alone_and_in_mix():
     assert alone[0] == O.dense_rank(A0.to_dense(P), P)
 
 
+def test_server_degenerate_shapes():
+    """Degenerate requests (no nonzeros, 1 x 1, one row, one column, b above
+    n) among ordinary ones: each served rank equals dense elimination."""
+    eng = _eng()
+    reqs = [(synth.csr_from_dense(D), 2) for D in synth.edge_shapes(P).values()]
+    reqs += [(A, b) for A, b, *_ in _requests(3, 29)]
+    srv = eng.Server(P, 8 << 30)
+    ids = []
+    for i, (A, b) in enumerate(reqs):
+        dr, dc, V = synth.preconditioners(800 + i, A.m, A.n, b, P)
+        ids.append(srv.submit(A, b, dr, dc, V))
+    got = _collect(srv, ids)
+    srv.close()
+    for i, (A, b) in enumerate(reqs):
+        r = got[ids[i]]
+        assert r["status"] == 0 and r["rank"] == O.dense_rank(A.to_dense(P), P), (i, r)
+
+
 def test_server_isolates_a_failing_request():
     eng = _eng()
     reqs = _requests(4, 32)
