@@ -790,26 +790,23 @@ static rr_status batch_rank(uint32_t count, const rr_csr *mats, uint32_t p, cons
                 const bool dev_ok = hipSetDevice(dev) == hipSuccess;
                 for (uint32_t k; !su.stop && (k = su.next++) < count;) {
                     const uint32_t i = su.order[k];
-                    rr_status e = RR_EDEVICE;
-                    try {
-                        if (dev_ok) {
-                            reqs[i].reset(new Request());
-                            const uint64_t L = seq_len_of(i);
-                            const uint64_t d = order && order[i] ? order[i] : L - 1;
-                            e = reqs[i]->init(&mats[i], p, b[i], dr[i], dc[i], V[i], L, d, 2048, su.ws[w],
-                                              su_of ? su_of[i] : 0);
-                            if (e == RR_OK && hipEventRecord(su.ev[i], su.ws[w]) != hipSuccess) {
-                                e = RR_EDEVICE;
-                                set_error("event record failed");
-                            }
-                            if (e != RR_OK) su.msg[i] = rr_last_error();
-                        } else {
-                            su.msg[i] = "set-up thread could not select the device";
+                    const rr_status e = guarded([&]() -> rr_status {
+                        if (!dev_ok) {
+                            set_error("set-up thread could not select the device");
+                            return RR_EDEVICE;
                         }
-                    } catch (const std::bad_alloc &) {
-                        e = RR_ENOMEM;
-                        su.msg[i] = "host allocation failed";
-                    }
+                        reqs[i].reset(new Request());
+                        const uint64_t L = seq_len_of(i);
+                        const uint64_t d = order && order[i] ? order[i] : L - 1;
+                        rr_status s = reqs[i]->init(&mats[i], p, b[i], dr[i], dc[i], V[i], L, d, 2048, su.ws[w],
+                                                    su_of ? su_of[i] : 0);
+                        if (s == RR_OK && hipEventRecord(su.ev[i], su.ws[w]) != hipSuccess) {
+                            set_error("event record failed");
+                            s = RR_EDEVICE;
+                        }
+                        return s;
+                    });
+                    if (e != RR_OK) su.msg[i] = rr_last_error();
                     su.st[i] = e;
                     su.ready[i].store(e == RR_OK ? 1 : 2, std::memory_order_release);
                 }

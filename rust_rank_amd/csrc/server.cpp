@@ -112,23 +112,20 @@ void rr_server::setup_loop(int w) {
         }
         Pending &q = *S->q;
         Request &R = *S->R;
-        rr_status e = RR_EDEVICE;
-        try {
+        const rr_status e = guarded([&]() -> rr_status {
             if (!dev_ok) {
                 set_error("set-up thread could not select the device");
-            } else {
-                rr_csr A{q.m, q.n, q.rp.data(), q.col.data(), q.val.data()};
-                e = R.init(&A, p, q.b, q.dr.data(), q.dc.data(), q.V.data(), q.L, q.d, 2048, setup_st[w]);
-                if (e == RR_OK && (hipEventCreateWithFlags(&S->ev, hipEventDisableTiming) != hipSuccess ||
-                                   hipEventRecord(S->ev, setup_st[w]) != hipSuccess)) {
-                    e = RR_EDEVICE;
-                    set_error("set-up event failed");
-                }
+                return RR_EDEVICE;
             }
-        } catch (const std::bad_alloc &) {
-            e = RR_ENOMEM;
-            set_error("host allocation failed");
-        }
+            rr_csr A{q.m, q.n, q.rp.data(), q.col.data(), q.val.data()};
+            rr_status s = R.init(&A, p, q.b, q.dr.data(), q.dc.data(), q.V.data(), q.L, q.d, 2048, setup_st[w]);
+            if (s == RR_OK && (hipEventCreateWithFlags(&S->ev, hipEventDisableTiming) != hipSuccess ||
+                               hipEventRecord(S->ev, setup_st[w]) != hipSuccess)) {
+                set_error("set-up event failed");
+                s = RR_EDEVICE;
+            }
+            return s;
+        });
         S->status = e;
         if (e != RR_OK) S->msg = rr_last_error();
         S->q.reset();  // the host copy of the inputs is no longer needed
