@@ -167,6 +167,7 @@ def graph_complex_leg(engine, synth, p, steps=0, with_pm=True, pm_order=0, cpu_t
     out = {"workload": f"graph_complex {A.m}x{A.n} 12 nnz/row Zipf(1.1) +-1/+-2 p=2013265921 b={b}",
            "seq_len": L, "steps": steps, "iters_s": round(steps / wall, 1),
            "ms_per_step": round(wall * 1e3 / steps, 5),
+           "setup": {"ms": round(st["ms_setup"], 1), "on_device": bool(st["setup_on_device"])},
            "roofline": roofline(r, traffic_from_profile("traffic_k_spmm_graph_complex.json"),
                                 "k_spmm<32, packed>", None)}
     log(f"  {out['iters_s']} iters/s, pass by A {r['a_us']:.1f} us, by A^T {r['at_us']:.1f} us")

@@ -801,7 +801,9 @@ rr_status build_slices(SliceMat &M, uint64_t nrows, const std::vector<uint32_t> 
         const uint64_t base = lc.size(), end = base + (rp[r + 1] - rp[r]);
         idx.resize(end - base);
         for (size_t i = 0; i < idx.size(); i++) idx[i] = rp[r] + (uint32_t)i;
-        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) { return (colw[x] & mask) < (colw[y] & mask); });
+        auto by_col = [&](uint32_t x, uint32_t y) { return (colw[x] & mask) < (colw[y] & mask); };
+        if (!std::is_sorted(idx.begin(), idx.end(), by_col))  // A^T's rows come in order from the transpose
+            std::stable_sort(idx.begin(), idx.end(), by_col);
         for (uint32_t e : idx) {
             lc.push_back(colw[e]);
             if (!packed) lv.push_back(vals[e]);
