@@ -623,6 +623,7 @@ rr_status rank_streamed(rr_krylov *k, uint32_t b, uint64_t L, uint64_t d, uint32
     PmJob J;
     J.b = b;
     J.d = d;
+    J.K = leaf_order_streamed(2 * b);
     J.G = G.as<uint32_t>();
     J.M = M.as<uint32_t>();
     J.delta = dl.as<int64_t>();

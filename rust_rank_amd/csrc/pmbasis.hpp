@@ -13,11 +13,19 @@ namespace rr {
 // many jobs sharing rounds, long leaves mean fewer rounds and fewer products
 // (m = 32: 16 steps made 512 batched matrices 10% faster than 8).
 __host__ __device__ constexpr uint32_t leaf_order(uint32_t m) { return m <= 32 ? 16 : m <= 64 ? 4 : 1; }
-// A PM-basis running alone is a chain of leaf launches, and shorter leaves
-// (less basis update per coefficient) finish it sooner; in batches and in
+// A PM-basis running alone is a chain of leaf launches; in batches and in
 // the server the leaves of many jobs share rounds, and longer leaves mean
-// fewer rounds.  The basis is the same either way.
-__host__ __device__ constexpr uint32_t leaf_order_alone(uint32_t m) { return m <= 16 ? 8 : m <= 32 ? 4 : m <= 64 ? 2 : 1; }
+// fewer rounds.  Alone, the fastest orders measured were 16, 8, 4 for
+// m = 8, 16, 32 and single steps from m = 64 on (order 11257 at m = 64:
+// 733 ms against 784 with two steps and 893 with four).  The basis is the
+// same either way.
+__host__ __device__ constexpr uint32_t leaf_order_alone(uint32_t m) { return m <= 8 ? 16 : m <= 16 ? 8 : m <= 32 ? 4 : 1; }
+// A single PM-basis fed beside the Krylov passes (its leftmost chain runs
+// while the sequence grows, the rest after it): measured on the 200000 x
+// 180000 matrix through rr_rank, 8 steps at m = 16 and 32 (b = 16: 1.19 s
+// against 1.30 s with 16), 2 at m = 64 (1.13 s against 1.16 s with 4 and
+// 1.21 s with 1).
+__host__ __device__ constexpr uint32_t leaf_order_streamed(uint32_t m) { return m <= 32 ? 8 : m <= 64 ? 2 : 1; }
 
 struct LeafDesc {
     const uint32_t *G;  // residual G(i, j, k) at G[(i * n + j) * gs + k], k < d
