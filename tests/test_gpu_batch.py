@@ -5,6 +5,7 @@ import pytest
 
 import pyoracle as O
 from rust_rank_amd import synth
+from gpu_cases import p3323_cases
 
 P = 2013265921
 pytestmark = pytest.mark.gpu
@@ -30,6 +31,16 @@ def test_batch_degenerate_shapes_beside_ordinary_matrices():
     for i, A in enumerate(mats):
         single, _ = eng.rank(A, P, bs[i], *pre[i])
         assert ranks[i] == single == O.dense_rank(A.to_dense(P), P), (i, ranks[i], single)
+
+
+def test_batch_other_prime_matches_oracle():
+    eng = _eng()
+    p, cases = p3323_cases()
+    print(f"p={p}: shapes {[(c[0].m, c[0].n, c[1]) for c in cases]}", flush=True)
+    ranks, _ = eng.batch_rank([c[0] for c in cases], p, [c[1] for c in cases], [c[2] for c in cases],
+                              [c[3] for c in cases], [c[4] for c in cases], [c[5] for c in cases])
+    for i, (A, b, dr, dc, V, L, want) in enumerate(cases):
+        assert ranks[i] == want == O.dense_rank(A.to_dense(p), p), (i, ranks[i], want)
 
 
 def test_batch_matches_single_and_dense_ge():

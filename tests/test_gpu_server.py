@@ -77,6 +77,21 @@ def test_server_degenerate_shapes():
         assert r["status"] == 0 and r["rank"] == O.dense_rank(A.to_dense(P), P), (i, r)
 
 
+def test_server_other_prime_matches_oracle():
+    """The served form mod p = 3323: the PM-basis rounds of the requests run
+    their products through the three CRT primes; every rank equals the
+    oracle's pipeline."""
+    from gpu_cases import p3323_cases
+    eng = _eng()
+    p, cases = p3323_cases()
+    with eng.Server(p, 8 << 30) as srv:
+        ids = [srv.submit(A, b, dr, dc, V, L) for (A, b, dr, dc, V, L, _) in cases]
+        got = _collect(srv, ids)
+    for i, c in enumerate(cases):
+        r = got[ids[i]]
+        assert r["status"] == 0 and r["rank"] == c[6], (i, r, c[6])
+
+
 def test_server_isolates_a_failing_request():
     eng = _eng()
     reqs = _requests(4, 32)
