@@ -412,7 +412,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the engine does not shard a matrix "
-              f"across GPUs, and this build runs on one", file=sys.stderr)
+              f"across GPUs, and runs on one", file=sys.stderr)
         sys.exit(2)
     from rust_rank_amd import engine, synth
     p = synth.P_DEFAULT
