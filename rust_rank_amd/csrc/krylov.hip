@@ -237,7 +237,14 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4, 8))) void
                 o[c] = ok ? o[c] : 0u;
                 s[c] = mulmod(o[c], un.y, a.p, a.inv);
             }
-            if (ok) vstore<VW>(a.out + (size_t)un.x * B + q * VW, s);
+            if (ok) {
+                if constexpr (B == 64) {  // rows of 256 B from a vector far beyond the caches: written past them
+                    GLOBAL_AS gv4u *dst = (GLOBAL_AS gv4u *)as_global(a.out + (size_t)un.x * B + q * VW);
+                    __builtin_nontemporal_store(gv4u{s[0], s[1], s[2], s[3]}, dst);
+                } else {
+                    vstore<VW>(a.out + (size_t)un.x * B + q * VW, s);
+                }
+            }
             if constexpr (PROJ) {
 #pragma unroll
                 for (int c = 0; c < VW; c++) {
