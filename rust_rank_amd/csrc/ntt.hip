@@ -29,7 +29,6 @@ __device__ __forceinline__ uint32_t shoup_mul(uint32_t a, uint32_t w, uint32_t w
 }
 
 constexpr int NTT_THREADS = 256;
-constexpr uint32_t NTT_MAX_WORDS = 16384;  // largest transform, one workgroup's LDS
 constexpr uint32_t NTT_WG_WORDS = 2048;    // shorter transforms share a workgroup up to this many words
 constexpr uint32_t NTT_U = 4;              // butterflies per thread in flight
 
