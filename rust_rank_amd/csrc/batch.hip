@@ -180,7 +180,7 @@ uint64_t Request::device_bytes() const { return request_bytes(m, n, nnz, b, L, d
 
 rr_status Request::init(const rr_csr *A, uint32_t p_, uint32_t b_, const uint32_t *dr, const uint32_t *dcv,
                         const uint32_t *V, uint64_t L_, uint64_t d_, uint32_t tile_rows, hipStream_t st,
-                        uint32_t su_req) {
+                        uint32_t su_req, uint32_t steps_per_wave) {
     p = p_;
     b = b_;
     B = 1;
@@ -232,7 +232,8 @@ rr_status Request::init(const rr_csr *A, uint32_t p_, uint32_t b_, const uint32_
     // out on the device.
     const uint32_t RPW = 64 / (B / (B < 4 ? B : 4));
     const uint64_t steps = std::max(nnz, std::max(m, n)) / RPW + 1;
-    const uint64_t waves = (steps + BATCH_STEPS_PER_WAVE - 1) / BATCH_STEPS_PER_WAVE;
+    RR_CHECK(steps_per_wave >= 1, RR_EINVAL, "steps per wavefront must be positive");
+    const uint64_t waves = (steps + steps_per_wave - 1) / steps_per_wave;
     int dev = 0, cus = 0;
     RR_HIP(hipGetDevice(&dev));
     RR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));

@@ -29,11 +29,15 @@ enum : int { STAGE_KRYLOV = 0, STAGE_PM = 1, STAGE_DONE = 2, STAGE_FAILED = 3, S
 // resident device footprint of one request (admission control), bytes
 uint64_t request_bytes(uint64_t m, uint64_t n, uint64_t nnz, uint32_t b, uint64_t L, uint64_t d, bool packed);
 
-// Steps (nonzeros per row group) a wavefront of a batched request is given
-// when its operands are sliced: small requests get few wavefronts, large ones
-// up to a full device.  512 batched matrices: 2.43 s at 48 or 64, 2.46 s at
-// 32, 2.50 s at 128, 2.82 s at 16.
+// Steps (nonzeros per row group) a wavefront of a request is given when its
+// operands are sliced: small requests get few wavefronts, large ones up to a
+// full device.  A batch is throughput-bound (512 matrices: 2.43 s at 48 or
+// 64, 2.46 s at 32, 2.50 s at 128, 2.82 s at 16); the server also answers
+// for latency, and under partial load a request with twice the wavefronts
+// finishes sooner (256 served requests at 0.7x their saturated rate: p50 108
+// ms and p99 1.24 s at 32, 152 ms and 1.58 s at 64, the same throughput).
 constexpr uint32_t BATCH_STEPS_PER_WAVE = 64;
+constexpr uint32_t SERVER_STEPS_PER_WAVE = 32;
 // Ticks per chunk in rr_batch_rank: their passes are launched back to back
 // and their partials reduced by one launch.
 constexpr uint32_t BATCH_TICKS = 16;
@@ -75,7 +79,8 @@ struct Request {
 
     // su: column block size 3 or 4 (0: the one that pads least)
     rr_status init(const rr_csr *A, uint32_t p, uint32_t b, const uint32_t *dr, const uint32_t *dc,
-                   const uint32_t *V, uint64_t L, uint64_t d, uint32_t tile_rows, hipStream_t st, uint32_t su = 0);
+                   const uint32_t *V, uint64_t L, uint64_t d, uint32_t tile_rows, hipStream_t st, uint32_t su = 0,
+                   uint32_t steps_per_wave = BATCH_STEPS_PER_WAVE);
     // PM-basis buffers; the job then reads the coefficients of G built so far
     rr_status alloc_pm();
     // G's coefficients for the sequence entries produced so far (t), on st

@@ -118,7 +118,8 @@ void rr_server::setup_loop(int w) {
                 return RR_EDEVICE;
             }
             rr_csr A{q.m, q.n, q.rp.data(), q.col.data(), q.val.data()};
-            rr_status s = R.init(&A, p, q.b, q.dr.data(), q.dc.data(), q.V.data(), q.L, q.d, 2048, setup_st[w]);
+            rr_status s = R.init(&A, p, q.b, q.dr.data(), q.dc.data(), q.V.data(), q.L, q.d, 2048, setup_st[w], 0,
+                                 SERVER_STEPS_PER_WAVE);
             if (s == RR_OK && (hipEventCreateWithFlags(&S->ev, hipEventDisableTiming) != hipSuccess ||
                                hipEventRecord(S->ev, setup_st[w]) != hipSuccess)) {
                 set_error("set-up event failed");
