@@ -43,6 +43,22 @@ def test_batch_other_prime_matches_oracle():
         assert ranks[i] == want == O.dense_rank(A.to_dense(p), p), (i, ranks[i], want)
 
 
+def test_batch_lane_width_64():
+    """b = 64 and b = 48 (lane width 64) in a batch: the batched passes fuse the
+    projection into the 64-lane pass, where the single path projects in a
+    kernel of its own; the ranks agree with each other and with dense
+    elimination."""
+    eng = _eng()
+    mats = [synth.batch_matrix(71, i, rows_lo=300, rows_hi=900) for i in range(5)]
+    bs = [64, 48, 64, 16, 64]
+    pre = [synth.preconditioners(950 + i, A.m, A.n, b, P) for i, (A, b) in enumerate(zip(mats, bs))]
+    print(f"batch of {len(mats)}: shapes {[(A.m, A.n) for A in mats]}, b {bs}", flush=True)
+    ranks, _ = eng.batch_rank(mats, P, bs, [x[0] for x in pre], [x[1] for x in pre], [x[2] for x in pre])
+    for i, A in enumerate(mats):
+        single, _ = eng.rank(A, P, bs[i], *pre[i])
+        assert ranks[i] == single == O.dense_rank(A.to_dense(P), P), (i, bs[i], ranks[i], single)
+
+
 def test_batch_matches_single_and_dense_ge():
     eng = _eng()
     mats, bs, drs, dcs, Vs = [], [], [], [], []

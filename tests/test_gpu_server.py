@@ -92,6 +92,22 @@ def test_server_other_prime_matches_oracle():
         assert r["status"] == 0 and r["rank"] == c[6], (i, r, c[6])
 
 
+def test_server_lane_width_64():
+    """b = 64 and b = 48 requests beside b = 8 ones: every served rank equals
+    dense elimination."""
+    eng = _eng()
+    reqs = []
+    for i, b in enumerate((64, 8, 48, 64, 8)):
+        A = synth.batch_matrix(73, i, rows_lo=300, rows_hi=900)
+        reqs.append((A, b) + tuple(synth.preconditioners(970 + i, A.m, A.n, b, P)))
+    with eng.Server(P, 8 << 30) as srv:
+        ids = [srv.submit(A, b, dr, dc, V) for (A, b, dr, dc, V) in reqs]
+        got = _collect(srv, ids)
+    for i, (A, b, *_rest) in enumerate(reqs):
+        r = got[ids[i]]
+        assert r["status"] == 0 and r["rank"] == O.dense_rank(A.to_dense(P), P), (i, b, r)
+
+
 def test_server_isolates_a_failing_request():
     eng = _eng()
     reqs = _requests(4, 32)
