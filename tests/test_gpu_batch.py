@@ -59,6 +59,22 @@ def test_batch_lane_width_64():
         assert ranks[i] == single == O.dense_rank(A.to_dense(P), P), (i, bs[i], ranks[i], single)
 
 
+def test_batch_general_values():
+    """Matrices with values uniform in [1, p) (not small signed residues: the
+    passes take the general-value form) at several lane widths, batched: the
+    ranks agree with the single path and dense elimination."""
+    eng = _eng()
+    shapes = [(400, 360), (500, 520), (300, 300), (650, 600), (420, 380)]
+    bs = [4, 16, 32, 64, 3]
+    mats = [synth.uniform_sparse(75, i, m, n, 3, 9, P, small=False) for i, (m, n) in enumerate(shapes)]
+    mats[0] = synth.plant_dependent_columns(synth.Rng(75, 99), mats[0], 12, P, False)
+    pre = [synth.preconditioners(990 + i, A.m, A.n, b, P) for i, (A, b) in enumerate(zip(mats, bs))]
+    ranks, _ = eng.batch_rank(mats, P, bs, [x[0] for x in pre], [x[1] for x in pre], [x[2] for x in pre])
+    for i, A in enumerate(mats):
+        single, _ = eng.rank(A, P, bs[i], *pre[i])
+        assert ranks[i] == single == O.dense_rank(A.to_dense(P), P), (i, bs[i], ranks[i], single)
+
+
 def test_batch_matches_single_and_dense_ge():
     eng = _eng()
     mats, bs, drs, dcs, Vs = [], [], [], [], []

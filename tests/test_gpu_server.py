@@ -108,6 +108,22 @@ def test_server_lane_width_64():
         assert r["status"] == 0 and r["rank"] == O.dense_rank(A.to_dense(P), P), (i, b, r)
 
 
+def test_server_general_values():
+    """Requests with values uniform in [1, p) at lane widths 8, 32 and 64:
+    every served rank equals dense elimination."""
+    eng = _eng()
+    reqs = []
+    for i, (b, m, n) in enumerate(((8, 400, 370), (32, 500, 450), (64, 450, 500), (8, 300, 280))):
+        A = synth.uniform_sparse(77, i, m, n, 3, 9, P, small=False)
+        reqs.append((A, b) + tuple(synth.preconditioners(1010 + i, A.m, A.n, b, P)))
+    with eng.Server(P, 8 << 30) as srv:
+        ids = [srv.submit(A, b, dr, dc, V) for (A, b, dr, dc, V) in reqs]
+        got = _collect(srv, ids)
+    for i, (A, b, *_rest) in enumerate(reqs):
+        r = got[ids[i]]
+        assert r["status"] == 0 and r["rank"] == O.dense_rank(A.to_dense(P), P), (i, b, r)
+
+
 def test_server_isolates_a_failing_request():
     eng = _eng()
     reqs = _requests(4, 32)
