@@ -1,5 +1,7 @@
 """Inputs shared by several GPU test files, with their expected results from
 the oracle (computed on the host when a test asks for them)."""
+import functools
+
 import pyoracle as O
 from rust_rank_amd import synth
 
@@ -19,3 +21,20 @@ def p3323_cases():
         want = O.rank_from_basis(*O.pm_basis(O.krylov(A, p, b, dr, dc, V, 0, L), b, L - 1, p), b, p)
         out.append((A, b, dr, dc, V, L, want))
     return p, out
+
+
+@functools.lru_cache(maxsize=1)
+def split_row_cases():
+    """Two graph-complex-like matrices (12 nonzeros per row, columns drawn
+    Zipf(1.1)): the most frequent columns hold thousands of entries, so A^T
+    has rows long enough to be cut into segments that several wavefronts add
+    into one accumulator row.  Each with its rank from the oracle's pipeline."""
+    P = synth.P_DEFAULT
+    out = []
+    for seed, m, n, b in ((11, 6000, 5000, 16), (12, 4000, 3500, 8)):
+        A = synth.graph_complex(seed=seed, m=m, n=n, k=12)
+        dr, dc, V = synth.preconditioners(1100 + seed, A.m, A.n, b, P)
+        L = synth.seq_len_plain(min(A.m, A.n), b)
+        seq = O.krylov(A, P, b, dr, dc, V, 0, L)
+        out.append((A, b, dr, dc, V, L, O.rank_from_basis(*O.pm_basis(seq, b, L - 1, P), b, P)))
+    return tuple(out)

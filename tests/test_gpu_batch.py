@@ -5,7 +5,7 @@ import pytest
 
 import pyoracle as O
 from rust_rank_amd import synth
-from gpu_cases import p3323_cases
+from gpu_cases import p3323_cases, split_row_cases
 
 P = 2013265921
 pytestmark = pytest.mark.gpu
@@ -73,6 +73,23 @@ def test_batch_general_values():
     for i, A in enumerate(mats):
         single, _ = eng.rank(A, P, bs[i], *pre[i])
         assert ranks[i] == single == O.dense_rank(A.to_dense(P), P), (i, bs[i], ranks[i], single)
+
+
+def test_batch_split_rows_match_oracle():
+    """Matrices whose A^T rows are cut into segments (long Zipf columns), batched
+    beside an ordinary matrix: each job's segment sums meet in its own
+    accumulator rows within shared launches; ranks equal the oracle's
+    pipeline and the single path."""
+    eng = _eng()
+    cases = list(split_row_cases())
+    A3 = synth.batch_matrix(79, 0, rows_lo=800, rows_hi=1200)
+    cases.append((A3, 16) + tuple(synth.preconditioners(1130, A3.m, A3.n, 16, P)) +
+                 (synth.seq_len_plain(min(A3.m, A3.n), 16), O.dense_rank(A3.to_dense(P), P)))
+    ranks, _ = eng.batch_rank([c[0] for c in cases], P, [c[1] for c in cases], [c[2] for c in cases],
+                              [c[3] for c in cases], [c[4] for c in cases], [c[5] for c in cases])
+    for i, (A, b, dr, dc, V, L, want) in enumerate(cases):
+        single, _ = eng.rank(A, P, b, dr, dc, V, L, L - 1)
+        assert ranks[i] == single == want, (i, ranks[i], single, want)
 
 
 def test_batch_matches_single_and_dense_ge():

@@ -124,6 +124,20 @@ def test_server_general_values():
         assert r["status"] == 0 and r["rank"] == O.dense_rank(A.to_dense(P), P), (i, b, r)
 
 
+def test_server_split_rows_match_oracle():
+    """Requests whose A^T rows are cut into segments: served ranks equal the
+    oracle's pipeline."""
+    from gpu_cases import split_row_cases
+    eng = _eng()
+    cases = split_row_cases()
+    with eng.Server(P, 8 << 30) as srv:
+        ids = [srv.submit(A, b, dr, dc, V, L) for (A, b, dr, dc, V, L, _) in cases]
+        got = _collect(srv, ids)
+    for i, c in enumerate(cases):
+        r = got[ids[i]]
+        assert r["status"] == 0 and r["rank"] == c[6], (i, r, c[6])
+
+
 def test_server_isolates_a_failing_request():
     eng = _eng()
     reqs = _requests(4, 32)
