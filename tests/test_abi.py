@@ -59,3 +59,13 @@ def test_rank_formula_matches_reference_golden():
         p, b, dd, L = (int(x) for x in d[f"meta{i}"])
         assert engine.rank_from_basis(d[f"basis{i}"], d[f"delta{i}"], b, p) == int(d[f"rank{i}"][0]), i
         i += 1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and says
+    why, before any device work."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
