@@ -77,6 +77,25 @@ __device__ __forceinline__ uint32_t mulmod(uint32_t a, uint32_t b, uint32_t p, d
     return r >= p ? r - p : r;
 }
 
+// a * w mod p for a < 2^32 and w < p < 2^31, given w's Shoup companion
+// wq = floor(w 2^32 / p): q = floor(a wq / 2^32) is within 1 below
+// floor(a w / p), so a w - q p lies in [0, 2p) and is exact mod 2^32.
+__device__ __forceinline__ uint32_t shoup_mul(uint32_t a, uint32_t w, uint32_t wq, uint32_t p) {
+    uint32_t q = __umulhi(a, wq);
+    uint32_t r = a * w - q * p;
+    return r >= p ? r - p : r;
+}
+
+// The Shoup companion floor(w 2^32 / p) of w < p < 2^31.  The fp64 estimate
+// w 2^32 inv - 1 lies below the true quotient and within 2 of it (relative
+// error 2^-52 on a value below 2^32), so at most two corrections follow.
+__device__ __forceinline__ uint32_t shoup_companion(uint32_t w, uint32_t p, double inv) {
+    uint64_t q = (uint64_t)fmax((double)w * 0x1p32 * inv - 1.0, 0.0);
+    uint64_t r = ((uint64_t)w << 32) - q * p;
+    while (r >= p) r -= p, q++;
+    return (uint32_t)q;
+}
+
 __device__ __forceinline__ uint32_t addmod(uint32_t a, uint32_t b, uint32_t p) {
     uint32_t s = a + b;
     return s >= p ? s - p : s;

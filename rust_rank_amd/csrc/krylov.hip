@@ -507,7 +507,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
 #pragma unroll
         for (int w = 0; w < 4; w++) acc2[w] = pj_v4i{0, 0, 0, 0};
     };
-    __shared__ uint32_t ivs[2][PJ_ROWS];  // the chunk's row inverses (double buffered)
+    // the chunk's row inverses and their Shoup companions (double buffered):
+    // each inverse multiplies 64 words, so its companion is computed once
+    __shared__ uint32_t ivs[2][PJ_ROWS], ivq[2][PJ_ROWS];
     const uint32_t nchunks = (nrows + PJ_ROWS - 1) / PJ_ROWS;
     const int col = tid & 63, grp = tid >> 6;  // staging: column, group of 16 rows
     uint32_t sv[16], ivn = 0;
@@ -525,7 +527,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
     uint32_t c = blockIdx.x;
     if (c < nchunks) load(c);
     for (; c < nchunks; c += gridDim.x, buf ^= 1) {
-        if (tid < PJ_ROWS) ivs[buf][tid] = ivn;
+        if (tid < PJ_ROWS) ivs[buf][tid] = ivn, ivq[buf][tid] = shoup_companion(ivn, p, inv);
         __syncthreads();  // the previous chunk's products have read the planes; the inverses are in
 #pragma unroll
         for (int side = 0; side < 2; side++) {  // o's planes, then s's: a thread's 16 rows of each digit
@@ -536,7 +538,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
 #pragma unroll
                 for (int jj = 0; jj < 4; jj++) {
                     const int j = 4 * k + jj;
-                    const uint32_t v = side ? sv[j] : mulmod(sv[j], ivs[buf][grp * 16 + j], p, inv);
+                    const uint32_t v =
+                        side ? sv[j] : shoup_mul(sv[j], ivs[buf][grp * 16 + j], ivq[buf][grp * 16 + j], p);
                     q[jj] = pj_digits(v, p, half);
                 }
                 pj_transpose(q);

@@ -22,12 +22,6 @@
 
 namespace rr {
 
-__device__ __forceinline__ uint32_t shoup_mul(uint32_t a, uint32_t w, uint32_t wq, uint32_t p) {
-    uint32_t q = __umulhi(a, wq);
-    uint32_t r = a * w - q * p;  // in [0, 2p)
-    return r >= p ? r - p : r;
-}
-
 constexpr int NTT_THREADS = 256;
 constexpr uint32_t NTT_WG_WORDS = 2048;    // shorter transforms share a workgroup up to this many words
 constexpr uint32_t NTT_U = 4;              // butterflies per thread in flight
