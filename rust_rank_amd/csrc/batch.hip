@@ -1187,7 +1187,10 @@ rr_status rr_autotune(uint32_t count, const rr_csr *mats, uint32_t p, rr_tuning 
             const double L = 2.0 * (double)std::min(A.m, A.n) / b[i] + 8;
             double t[2];
             for (int u = 0; u < 2; u++) t[u] = L * g_probe.slot[k][u] * (double)padded_slots(rp, ct, 3 + u);
-            const int uk = t[1] < t[0] ? 1 : 0;
+            // SU = 4 unless SU = 3 is predicted at least 15% faster: the probes'
+            // per-slot costs vary 10-20% between runs, and choices of SU = 3
+            // inside that spread made a batch slower
+            const int uk = t[0] < 0.85 * t[1] ? 0 : 1;
             out[i].b = b[i];
             out[i].su = 3 + uk;
             // the b model's Krylov term replaced by the slot model's at the chosen block size
